@@ -1,0 +1,166 @@
+"""Fast-APSP benchmark on one MI355X.
+
+Workload (default): road-like undirected unweighted graph with exactly n = 114,599
+vertices (percolated grid, p_keep = 0.52, chords at rate 1e-4, largest component,
+BFS-trimmed), K = 128 parts, full n x n int32 APSP via solve() in the batched form,
+result resident in HBM (n x ld int32, about 52.5 GB).  A step is one whole solve():
+boundary SSSP, build, Floyd-Warshall, min-plus merge with fused decode, for all parts.
+
+Prints ONE JSON line: value = n^2 distance entries per second of wall time.
+Also measured: the plain per-part launch form (one step), per-stage device times,
+the roofline of the dominant kernel and the CPU path on a bounded sample.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+import fastapsp_amd as F  # noqa: E402
+from fastapsp_amd import _lib as L  # noqa: E402
+
+METRIC = "APSP distance entries/s at bit-exact int32; batched vs plain speedup, 1 GPU"
+VALU_PEAK_TOPS = 39.3  # 256 CU x 64 lanes x 2.4 GHz, int32 lane-ops
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def stage_times():
+    t = L.FapStageTimes()
+    L.check(L.load().fap_hip_last_stage_times(C.byref(t)), "stage times")
+    return {k: getattr(t, k) for k, _ in L.FapStageTimes._fields_}
+
+
+def make_graph(cfg):
+    if cfg == "road1":
+        return F.FapGraph.road_like(114599, 0.52, 1e-4, 1, 128), \
+            "road-like n=114599 (percolated grid p=0.52, chords 1e-4, LCC, BFS trim), " \
+            "undirected unweighted, K=128, full APSP via solve()"
+    if cfg == "road0":
+        return F.FapGraph.road_grid(64, 64, 0.10, 0.02, 0, 8), \
+            "road grid 64x64 (10% edges dropped, 2% diagonals, LCC), K=8, full APSP"
+    raise SystemExit(f"unknown config {cfg}")
+
+
+def cpu_baseline(g, budget_s):
+    """The oracle's CPU path (reference structure: OpenMP Dijkstra over boundary sources,
+    single-threaded Floyd-Warshall and min-plus) on parts 1..k until budget_s elapse."""
+    import oracle as O
+
+    lay, csr = g.layout(), g.csr()
+    n = g.num_vertexs
+    t0 = time.perf_counter()
+    entries, parts = 0, 0
+    for p in range(1, g.K + 1):
+        O.run_part(p, csr, lay)
+        entries += int(lay["num"][p]) * n
+        parts += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": entries / dt, "unit": "entries/s", "cores": cores, "kind": "port",
+            "sample": f"parts 1..{parts} of the same graph ({entries} entries, {dt:.1f} s); "
+                      f"Dijkstra stage on {cores} OpenMP threads, Floyd-Warshall and min-plus "
+                      f"on 1 thread as in the reference CPU path"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="road1")
+    ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU sharding is "
+              "a design note, not a run", file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.perf_counter()
+    g, workload = make_graph(args.config)
+    log(f"graph n={g.num_vertexs} m={g.num_edges} K={g.K} built+partitioned in "
+        f"{time.perf_counter() - t0:.1f}s (edge cut {g.getEdgeCut():.0f})")
+    g.preCondition()
+    lay = g.layout()
+    n = g.num_vertexs
+    num = lay["num"][1:].astype(np.int64)
+    bdy = lay["bdy"][1:].astype(np.int64)
+    inner = num - bdy
+    log(f"boundary total {bdy.sum()} (per part mean {bdy.mean():.1f}, max {bdy.max()}); "
+        f"part size mean {num.mean():.1f} max {num.max()}")
+
+    g.setMode(F.FAP_MODE_BATCHED)
+    for i in range(args.warmup):
+        g.solve()
+        log(f"warmup {i + 1}/{args.warmup}: {stage_times()['total_ms']:.2f} ms device")
+    L.check(L.load().fap_hip_device_sync(), "sync")
+    merge_ms, stages = [], []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        g.solve()
+        st = stage_times()
+        stages.append(st)
+        merge_ms.append(st["minplus_ms"])
+    L.check(L.load().fap_hip_device_sync(), "sync")
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = elapsed * 1e3 / args.steps
+    entries = float(n) * n
+    value = entries / (ms_per_step * 1e-3)
+    log(f"batched: {ms_per_step:.2f} ms/step, {value:.3e} entries/s")
+
+    avg = {k: float(np.mean([s[k] for s in stages])) for k in stages[0]}
+    merge_avg = float(np.mean(merge_ms))
+    ops = 2.0 * float((inner * bdy).sum()) * n
+    bytes_alg = 4.0 * float((inner * bdy).sum() + bdy.sum() * n + inner.sum() * n)
+    achieved = ops / (merge_avg * 1e-3) / 1e12
+    roofline = {"bound": "valu", "kernel": "merge_kernel (min-plus + fused decode)",
+                "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
+                "frac": achieved / VALU_PEAK_TOPS, "traffic": None,
+                "ops_per_launch": ops, "alg_bytes_per_launch": bytes_alg,
+                "avg_launch_ms": merge_avg,
+                "hbm_achieved_GBs": bytes_alg / (merge_avg * 1e-3) / 1e9}
+
+    plain = None
+    if not args.skip_plain:
+        g.setMode(F.FAP_MODE_PLAIN)
+        tp = time.perf_counter()
+        g.solve()
+        L.check(L.load().fap_hip_device_sync(), "sync")
+        plain_ms = (time.perf_counter() - tp) * 1e3
+        pst = stage_times()
+        plain = {"ms_per_step": plain_ms, "stages_ms": pst,
+                 "batched_vs_plain": plain_ms / ms_per_step}
+        log(f"plain: {plain_ms:.2f} ms/step ({plain_ms / ms_per_step:.2f}x slower)")
+        g.setMode(F.FAP_MODE_BATCHED)
+
+    cpu = None
+    if not args.skip_cpu:
+        cpu = cpu_baseline(g, args.cpu_budget)
+        log(f"cpu baseline: {cpu['value']:.3e} entries/s ({cpu['sample']})")
+
+    out = {"metric": METRIC, "value": value, "unit": "entries/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+           "data": "synthetic",
+           "config": {"workload": workload, "n": n, "m": g.num_edges, "K": g.K,
+                      "boundary_total": int(bdy.sum()), "mode": "batched",
+                      "result_bytes_in_hbm": int(n) * int(n) * 4},
+           "stages_ms": avg, "roofline": roofline, "cpu_baseline": cpu, "plain_form": plain}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

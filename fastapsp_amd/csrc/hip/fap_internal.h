@@ -1,0 +1,72 @@
+// Internal helpers shared by the HIP translation units of libfap_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "fap_hip.h"
+
+namespace fap_hip {
+
+// Thread-local message of the last failing C-ABI call.
+void set_error(const char *fmt, ...);
+void clear_error();
+
+// Library stream of the current device (created by fap_hip_init).
+hipStream_t lib_stream();
+inline hipStream_t resolve(void *stream) {
+  return stream ? static_cast<hipStream_t>(stream) : lib_stream();
+}
+bool initialized();
+
+// Budgeted device allocation used by every internal workspace.
+int dev_alloc(void **p, int64_t bytes);
+int dev_free(void *p);
+
+// Grow-only device workspace (one per purpose), released by the library at exit.
+struct Workspace {
+  void *ptr = nullptr;
+  int64_t bytes = 0;
+  int ensure(int64_t need);
+};
+
+constexpr int kInf = FAP_INF;
+
+}  // namespace fap_hip
+
+#define FAP_CHECK_HIP(call)                                                            \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      fap_hip::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),        \
+                         __FILE__, __LINE__);                                          \
+      return FAP_ERR_HIP;                                                              \
+    }                                                                                  \
+  } while (0)
+
+#define FAP_CHECK_LAUNCH()                                                             \
+  do {                                                                                 \
+    hipError_t e_ = hipGetLastError();                                                 \
+    if (e_ != hipSuccess) {                                                            \
+      fap_hip::set_error("kernel launch failed: %s (%s:%d)", hipGetErrorString(e_),   \
+                         __FILE__, __LINE__);                                          \
+      return FAP_ERR_HIP;                                                              \
+    }                                                                                  \
+  } while (0)
+
+#define FAP_REQUIRE(cond, ...)                                                         \
+  do {                                                                                 \
+    if (!(cond)) {                                                                     \
+      fap_hip::set_error(__VA_ARGS__);                                                 \
+      return FAP_ERR_INVALID;                                                          \
+    }                                                                                  \
+  } while (0)
+
+#define FAP_TRY(call)                                                                  \
+  do {                                                                                 \
+    int s_ = (call);                                                                   \
+    if (s_ != FAP_OK) return s_;                                                       \
+  } while (0)

@@ -1,0 +1,58 @@
+// Device descriptors shared by the kernels of the part pipeline.
+#pragma once
+
+#include <cstdint>
+
+namespace fap_hip {
+
+// One padded S x S Floyd-Warshall matrix (S = ld = nb * 64, FAP_INF padding).
+struct FwDesc {
+  int *mat;
+  int64_t ld;
+  int nb;
+};
+
+// One min-plus product C = A (x) B with overwrite semantics:
+//   C[i][j] = min_k A[i][k] + B[k][j]  (FAP_INF when K == 0), i < M, j < N.
+// When part_id > 0 the epilogue also performs the decode of the part's own
+// columns: for a column j with graph_id[j] == part_id the stored value is
+// fw[(fw_row0 + i) * ldf + ed2st[j] - offset] (the Floyd-Warshall distance).
+struct MergeDesc {
+  const int *A;
+  int64_t lda;
+  const int *B;
+  int64_t ldb;
+  int *C;
+  int64_t ldc;
+  int M, K;
+  int row_blocks;   // ceil(M / 64)
+  int tile_begin;   // first logical tile of this product
+  int part_id;      // 0 = no fused decode
+  int offset;       // first sorted position of the part
+  const int *fw;    // Floyd-Warshall matrix of the part
+  int64_t ldf;
+  int fw_row0;      // row of the FW matrix that holds C's row 0
+};
+
+// Sub-matrix build of one part into its padded FW matrix.
+struct BuildDesc {
+  int *fw;
+  int64_t ldf;
+  int S;            // padded edge (multiple of 64, <= kBuildMaxS)
+  int s, bdy, offset, part_id;
+  int row_begin;    // first global build row of this part
+  const int *dist_rows;  // the part's SSSP rows (row i = boundary vertex i)
+  int64_t ld_dist;
+};
+
+// Decode of the boundary rows' own columns of one part.
+struct DecodeDesc {
+  const int *fw;
+  int64_t ldf;
+  int *out;         // first row of the part's block
+  int64_t ld_out;
+  int rows, s, offset;
+  int64_t item_begin;  // prefix over parts of rows * s
+};
+
+}  // namespace fap_hip

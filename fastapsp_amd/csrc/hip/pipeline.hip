@@ -1,0 +1,284 @@
+// The per-part Fast-APSP pipeline behind fap_hip_solve_parts.
+//
+// Mirrors fapGraph::run (fap/fap.cpp:195-503) for a list of parts:
+//   1 boundary SSSP            handle_boundry_path            (fap/fap.cpp:405-409)
+//   2 sub-matrix build         MysubMatBuild_path             (fap/fap.cpp:417-423)
+//   3 Floyd-Warshall           floyd_path                     (fap/fap.cpp:428)
+//   4 min-plus of inner rows   MyMat1Build + min_plus_path_advanced (:437-485)
+//   5 decode of own columns    MysubMatDecode_path            (fap/fap.cpp:492-496)
+// Step 5 is fused into the min-plus epilogue for the inner rows; for the boundary
+// rows it is a small scatter kernel.  Results stay in device memory.
+//
+// PLAIN   launches the five steps part after part (the reference's structure:
+//         solve() calls run() once per part, fap/fap.cpp:512-516).
+// BATCHED launches every step once for all parts: one SSSP grid over all boundary
+//         sources, one build grid over all rows, three FW launches per round for
+//         all parts, one min-plus grid through a work table, one decode grid.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "fap_internal.h"
+#include "fw_desc.h"
+
+namespace fap_hip {
+
+int launch_sssp(const int *d_row_off, const int *d_col, const int *d_w, int n, const int *d_src,
+                int nsrc, const int64_t *d_rows, int *d_out, int64_t ld, bool vec_fill,
+                hipStream_t stream);
+int launch_fw(const FwDesc *d_descs, int ndesc, int max_nb, hipStream_t stream);
+int launch_merge(const MergeDesc *d_descs, int nd, int total_tiles, int N, bool vec,
+                 const int *d_graph_id, const int *d_ed2st, hipStream_t stream);
+int merge_tiles(int M, int N);
+int launch_build(const BuildDesc *d_descs, int nd, int total_rows, bool vec, const int *row_off,
+                 const int *col, const int *w, const int *graph_id, const int *st2ed,
+                 const int *ed2st, hipStream_t stream);
+int launch_decode(const DecodeDesc *d_descs, int nd, int64_t total, const int *st2ed,
+                  hipStream_t stream);
+int launch_gather_sources(const int *offsets, const int *src_begin, const int64_t *out_rows,
+                          int nparts, int total, const int *st2ed, int64_t ld, int *src,
+                          int64_t *rows, hipStream_t stream);
+
+constexpr int kBuildMaxS = 8192;
+constexpr int kTile = 64;
+
+namespace {
+
+Workspace g_ws_fw;    // Floyd-Warshall matrices
+Workspace g_ws_meta;  // descriptors, sources, row indices
+hipStream_t g_last_stream = nullptr;
+fap_stage_times g_times{};
+
+struct EventPool {
+  std::vector<hipEvent_t> ev;
+  int used = 0;
+  int take(hipEvent_t *e) {
+    if (used == (int)ev.size()) {
+      hipEvent_t x;
+      FAP_CHECK_HIP(hipEventCreate(&x));
+      ev.push_back(x);
+    }
+    *e = ev[used++];
+    return FAP_OK;
+  }
+};
+EventPool g_events;
+
+// Stage marks: 6 events per group (start, after sssp, build, fw, merge, decode).
+std::vector<hipEvent_t> g_marks;
+
+int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace
+
+}  // namespace fap_hip
+
+using namespace fap_hip;
+
+extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *parts,
+                                   int32_t nparts, int32_t *d_out, int64_t ld_out, int32_t mode,
+                                   void *stream) {
+  if (!initialized()) {
+    set_error("fap_hip_init has not been called");
+    return FAP_ERR_STATE;
+  }
+  FAP_REQUIRE(g && nparts >= 0 && (nparts == 0 || parts) && d_out,
+              "fap_hip_solve_parts: null argument");
+  FAP_REQUIRE(mode == FAP_MODE_PLAIN || mode == FAP_MODE_BATCHED,
+              "fap_hip_solve_parts: unknown mode %d", mode);
+  const int n = g->n;
+  FAP_REQUIRE(n > 0 && g->m >= 0 && ld_out >= n && g->d_row_offset && g->d_graph_id &&
+                  g->d_st2ed && g->d_ed2st && (g->m == 0 || g->d_col),
+              "fap_hip_solve_parts: bad graph view");
+  hipStream_t st = resolve(stream);
+  if (g_last_stream && g_last_stream != st) FAP_CHECK_HIP(hipStreamSynchronize(g_last_stream));
+  g_last_stream = st;
+
+  // ---- host plan ---------------------------------------------------------
+  std::vector<fap_part_desc> act;
+  for (int p = 0; p < nparts; ++p) {
+    const fap_part_desc &q = parts[p];
+    FAP_REQUIRE(q.part_id >= 1 && q.num >= 0 && q.bdy >= 0 && q.bdy <= q.num && q.offset >= 0 &&
+                    (int64_t)q.offset + q.num <= n && q.out_row >= 0,
+                "fap_hip_solve_parts: bad part %d (id %d off %d num %d bdy %d)", p, q.part_id,
+                q.offset, q.num, q.bdy);
+    FAP_REQUIRE(q.num <= kBuildMaxS,
+                "fap_hip_solve_parts: part %d has %d vertices (limit %d); use a larger K",
+                q.part_id, q.num, kBuildMaxS);
+    if (q.num > 0) act.push_back(q);
+  }
+  const int P = (int)act.size();
+  g_times = fap_stage_times{};
+  if (P == 0) return FAP_OK;
+  const bool batched = mode == FAP_MODE_BATCHED;
+  const bool vec = (ld_out % 4 == 0) && ((uintptr_t)d_out % 16 == 0);
+
+  std::vector<int> S(P), nb(P);
+  std::vector<int64_t> fw_off(P);
+  int64_t fw_total = 0, fw_max = 0;
+  int max_nb = 0;
+  for (int p = 0; p < P; ++p) {
+    S[p] = (int)align_up(act[p].num, kTile);
+    nb[p] = S[p] / kTile;
+    max_nb = std::max(max_nb, nb[p]);
+    const int64_t sz = (int64_t)S[p] * S[p];
+    fw_off[p] = batched ? fw_total : 0;
+    fw_total += sz;
+    fw_max = std::max(fw_max, sz);
+  }
+  FAP_TRY(g_ws_fw.ensure((batched ? fw_total : fw_max) * 4));
+  int *fw_base = static_cast<int *>(g_ws_fw.ptr);
+
+  std::vector<BuildDesc> bd(P);
+  std::vector<FwDesc> fd(P);
+  std::vector<MergeDesc> md(P);
+  std::vector<DecodeDesc> dd(P);
+  std::vector<int> offsets(P), src_begin(P);
+  std::vector<int64_t> out_rows(P);
+  int rows_total = 0, tiles_total = 0, src_total = 0;
+  int64_t dec_total = 0;
+  for (int p = 0; p < P; ++p) {
+    const fap_part_desc &q = act[p];
+    int *fw = fw_base + fw_off[p];
+    const int M = q.num - q.bdy;
+    bd[p] = BuildDesc{fw, S[p], S[p], q.num, q.bdy, q.offset, q.part_id,
+                      batched ? rows_total : 0, d_out + q.out_row * ld_out, ld_out};
+    fd[p] = FwDesc{fw, S[p], nb[p]};
+    MergeDesc m{};
+    m.A = fw + (int64_t)q.bdy * S[p];
+    m.lda = S[p];
+    m.B = d_out + q.out_row * ld_out;
+    m.ldb = ld_out;
+    m.C = d_out + (q.out_row + q.bdy) * ld_out;
+    m.ldc = ld_out;
+    m.M = M;
+    m.K = q.bdy;
+    m.row_blocks = (M + 63) / 64;
+    m.tile_begin = batched ? tiles_total : 0;
+    m.part_id = q.part_id;
+    m.offset = q.offset;
+    m.fw = fw;
+    m.ldf = S[p];
+    m.fw_row0 = q.bdy;
+    md[p] = m;
+    dd[p] = DecodeDesc{fw, S[p], d_out + q.out_row * ld_out, ld_out, q.bdy, q.num, q.offset,
+                       batched ? dec_total : 0};
+    offsets[p] = q.offset;
+    src_begin[p] = batched ? src_total : 0;
+    out_rows[p] = q.out_row;
+    rows_total += S[p];
+    const int64_t tiles = merge_tiles(M, n);
+    FAP_REQUIRE((int64_t)tiles_total + tiles < (1LL << 30), "fap_hip_solve_parts: too many tiles");
+    tiles_total += (int)tiles;
+    src_total += q.bdy;
+    dec_total += (int64_t)q.bdy * q.num;
+  }
+
+  // ---- upload descriptors (one buffer) -------------------------------------
+  int64_t off = 0;
+  auto carve = [&](int64_t bytes) {
+    const int64_t o = off;
+    off = align_up(off + bytes, 256);
+    return o;
+  };
+  const int64_t o_bd = carve(sizeof(BuildDesc) * P), o_fd = carve(sizeof(FwDesc) * P);
+  const int64_t o_md = carve(sizeof(MergeDesc) * P), o_dd = carve(sizeof(DecodeDesc) * P);
+  const int64_t o_off = carve(4LL * P), o_sb = carve(4LL * P), o_or = carve(8LL * P);
+  const int64_t o_src = carve(4LL * std::max(src_total, 1));
+  const int64_t o_rows = carve(8LL * std::max(src_total, 1));
+  FAP_TRY(g_ws_meta.ensure(off));
+  char *meta = static_cast<char *>(g_ws_meta.ptr);
+  std::vector<char> host(off);
+  std::memcpy(host.data() + o_bd, bd.data(), sizeof(BuildDesc) * P);
+  std::memcpy(host.data() + o_fd, fd.data(), sizeof(FwDesc) * P);
+  std::memcpy(host.data() + o_md, md.data(), sizeof(MergeDesc) * P);
+  std::memcpy(host.data() + o_dd, dd.data(), sizeof(DecodeDesc) * P);
+  std::memcpy(host.data() + o_off, offsets.data(), 4LL * P);
+  std::memcpy(host.data() + o_sb, src_begin.data(), 4LL * P);
+  std::memcpy(host.data() + o_or, out_rows.data(), 8LL * P);
+  FAP_CHECK_HIP(hipMemcpyAsync(meta, host.data(), o_src, hipMemcpyHostToDevice, st));
+  FAP_CHECK_HIP(hipStreamSynchronize(st));
+  const BuildDesc *d_bd = reinterpret_cast<const BuildDesc *>(meta + o_bd);
+  const FwDesc *d_fd = reinterpret_cast<const FwDesc *>(meta + o_fd);
+  const MergeDesc *d_md = reinterpret_cast<const MergeDesc *>(meta + o_md);
+  const DecodeDesc *d_dd = reinterpret_cast<const DecodeDesc *>(meta + o_dd);
+  const int *d_offsets = reinterpret_cast<const int *>(meta + o_off);
+  const int *d_sb = reinterpret_cast<const int *>(meta + o_sb);
+  const int64_t *d_or = reinterpret_cast<const int64_t *>(meta + o_or);
+  int *d_src = reinterpret_cast<int *>(meta + o_src);
+  int64_t *d_rows = reinterpret_cast<int64_t *>(meta + o_rows);
+
+  // ---- launches -------------------------------------------------------------
+  g_events.used = 0;
+  g_marks.clear();
+  auto mark = [&]() -> int {
+    hipEvent_t e;
+    FAP_TRY(g_events.take(&e));
+    FAP_CHECK_HIP(hipEventRecord(e, st));
+    g_marks.push_back(e);
+    return FAP_OK;
+  };
+
+  if (batched) {
+    FAP_TRY(mark());
+    FAP_TRY(launch_gather_sources(d_offsets, d_sb, d_or, P, src_total, g->d_st2ed, ld_out, d_src,
+                                  d_rows, st));
+    FAP_TRY(launch_sssp(g->d_row_offset, g->d_col, g->d_w, n, d_src, src_total, d_rows, d_out, 1,
+                        vec, st));
+    FAP_TRY(mark());
+    FAP_TRY(launch_build(d_bd, P, rows_total, true, g->d_row_offset, g->d_col, g->d_w,
+                         g->d_graph_id, g->d_st2ed, g->d_ed2st, st));
+    FAP_TRY(mark());
+    FAP_TRY(launch_fw(d_fd, P, max_nb, st));
+    FAP_TRY(mark());
+    FAP_TRY(launch_merge(d_md, P, tiles_total, n, vec, g->d_graph_id, g->d_ed2st, st));
+    FAP_TRY(mark());
+    FAP_TRY(launch_decode(d_dd, P, dec_total, g->d_st2ed, st));
+    FAP_TRY(mark());
+  } else {
+    for (int p = 0; p < P; ++p) {
+      const fap_part_desc &q = act[p];
+      FAP_TRY(mark());
+      FAP_TRY(launch_gather_sources(d_offsets + p, d_sb + p, d_or + p, 1, q.bdy, g->d_st2ed, ld_out,
+                                    d_src, d_rows, st));
+      FAP_TRY(launch_sssp(g->d_row_offset, g->d_col, g->d_w, n, d_src, q.bdy, d_rows, d_out, 1,
+                          vec, st));
+      FAP_TRY(mark());
+      FAP_TRY(launch_build(d_bd + p, 1, S[p], true, g->d_row_offset, g->d_col, g->d_w,
+                           g->d_graph_id, g->d_st2ed, g->d_ed2st, st));
+      FAP_TRY(mark());
+      FAP_TRY(launch_fw(d_fd + p, 1, nb[p], st));
+      FAP_TRY(mark());
+      FAP_TRY(launch_merge(d_md + p, 1, merge_tiles(q.num - q.bdy, n), n, vec, g->d_graph_id,
+                           g->d_ed2st, st));
+      FAP_TRY(mark());
+      FAP_TRY(launch_decode(d_dd + p, 1, (int64_t)q.bdy * q.num, g->d_st2ed, st));
+      FAP_TRY(mark());
+    }
+  }
+  return FAP_OK;
+}
+
+extern "C" int fap_hip_last_stage_times(fap_stage_times *t) {
+  FAP_REQUIRE(t, "fap_hip_last_stage_times: null");
+  fap_stage_times acc{};
+  const size_t groups = g_marks.size() / 6;
+  if (groups > 0) FAP_CHECK_HIP(hipEventSynchronize(g_marks.back()));
+  for (size_t gi = 0; gi < groups; ++gi) {
+    float v[5];
+    for (int s = 0; s < 5; ++s)
+      FAP_CHECK_HIP(hipEventElapsedTime(&v[s], g_marks[6 * gi + s], g_marks[6 * gi + s + 1]));
+    acc.sssp_ms += v[0];
+    acc.build_ms += v[1];
+    acc.floyd_ms += v[2];
+    acc.minplus_ms += v[3];
+    acc.decode_ms += v[4];
+  }
+  if (groups > 0) {
+    float tot;
+    FAP_CHECK_HIP(hipEventElapsedTime(&tot, g_marks.front(), g_marks.back()));
+    acc.total_ms = tot;
+  }
+  *t = acc;
+  return FAP_OK;
+}

@@ -1,0 +1,260 @@
+// Runtime part of the C-ABI: device selection, library stream, budgeted device
+// memory, copies, fills and events.
+#include <cstdarg>
+#include <mutex>
+#include <unordered_map>
+
+#include "fap_internal.h"
+
+namespace fap_hip {
+
+namespace {
+thread_local std::string g_last_error;
+
+std::mutex g_mu;
+bool g_init = false;
+int g_device = -1;
+hipStream_t g_stream = nullptr;
+int64_t g_budget = 0;  // 0 = unlimited
+int64_t g_in_use = 0;
+int64_t g_high = 0;
+std::unordered_map<void *, int64_t> g_sizes;
+}  // namespace
+
+void set_error(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+void clear_error() { g_last_error.clear(); }
+
+hipStream_t lib_stream() { return g_stream; }
+bool initialized() { return g_init; }
+
+int dev_alloc(void **p, int64_t bytes) {
+  if (!p || bytes < 0) {
+    set_error("dev_alloc: invalid arguments");
+    return FAP_ERR_INVALID;
+  }
+  if (!g_init) {
+    set_error("fap_hip_init has not been called");
+    return FAP_ERR_STATE;
+  }
+  *p = nullptr;
+  if (bytes == 0) return FAP_OK;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_budget > 0 && g_in_use + bytes > g_budget) {
+      set_error("HBM budget exceeded: %lld in use + %lld requested > %lld budget",
+                (long long)g_in_use, (long long)bytes, (long long)g_budget);
+      return FAP_ERR_BUDGET;
+    }
+  }
+  hipError_t e = hipMalloc(p, (size_t)bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    set_error("hipMalloc(%lld) failed: %s", (long long)bytes, hipGetErrorString(e));
+    return FAP_ERR_NOMEM;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_sizes[*p] = bytes;
+  g_in_use += bytes;
+  if (g_in_use > g_high) g_high = g_in_use;
+  return FAP_OK;
+}
+
+int dev_free(void *p) {
+  if (!p) return FAP_OK;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_sizes.find(p);
+    if (it == g_sizes.end()) {
+      set_error("fap_hip_free: pointer %p was not allocated by fap_hip_malloc", p);
+      return FAP_ERR_INVALID;
+    }
+    g_in_use -= it->second;
+    g_sizes.erase(it);
+  }
+  FAP_CHECK_HIP(hipFree(p));
+  return FAP_OK;
+}
+
+int Workspace::ensure(int64_t need) {
+  if (need <= bytes) return FAP_OK;
+  if (ptr) {
+    // The old buffer may still be read by queued work on the library stream.
+    FAP_CHECK_HIP(hipDeviceSynchronize());
+    FAP_TRY(dev_free(ptr));
+    ptr = nullptr;
+    bytes = 0;
+  }
+  int64_t grown = need + need / 4;
+  int s = dev_alloc(&ptr, grown);
+  if (s != FAP_OK) {
+    s = dev_alloc(&ptr, need);
+    if (s != FAP_OK) return s;
+    grown = need;
+  }
+  bytes = grown;
+  return FAP_OK;
+}
+
+__global__ void fill_i32_kernel(int32_t *__restrict__ p, int32_t v, int64_t count) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < count; i += stride) p[i] = v;
+}
+
+}  // namespace fap_hip
+
+using namespace fap_hip;
+
+extern "C" {
+
+int fap_hip_init(int device) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_init) {
+    if (device == g_device) return FAP_OK;
+    set_error("fap_hip_init: already initialised on device %d", g_device);
+    return FAP_ERR_STATE;
+  }
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count <= device || device < 0) {
+    (void)hipGetLastError();
+    set_error("fap_hip_init: no HIP device %d (count %d, %s)", device, count,
+              hipGetErrorString(e));
+    return FAP_ERR_NODEV;
+  }
+  FAP_CHECK_HIP(hipSetDevice(device));
+  FAP_CHECK_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  g_device = device;
+  g_init = true;
+  return FAP_OK;
+}
+
+const char *fap_hip_last_error(void) { return g_last_error.c_str(); }
+
+int fap_hip_device_info(int *cu_count, int64_t *lds_bytes, int64_t *hbm_bytes, char *name,
+                        int name_len) {
+  if (!g_init) {
+    set_error("fap_hip_init has not been called");
+    return FAP_ERR_STATE;
+  }
+  hipDeviceProp_t prop;
+  FAP_CHECK_HIP(hipGetDeviceProperties(&prop, g_device));
+  if (cu_count) *cu_count = prop.multiProcessorCount;
+  if (lds_bytes) *lds_bytes = (int64_t)prop.sharedMemPerBlock;
+  if (hbm_bytes) *hbm_bytes = (int64_t)prop.totalGlobalMem;
+  if (name && name_len > 0) snprintf(name, name_len, "%s", prop.gcnArchName);
+  return FAP_OK;
+}
+
+int fap_hip_set_budget(int64_t bytes) {
+  if (bytes < 0) {
+    set_error("fap_hip_set_budget: negative budget");
+    return FAP_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_budget = bytes;
+  return FAP_OK;
+}
+
+int fap_hip_mem_stats(int64_t *in_use, int64_t *high_water, int64_t *budget) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (in_use) *in_use = g_in_use;
+  if (high_water) *high_water = g_high;
+  if (budget) *budget = g_budget;
+  return FAP_OK;
+}
+
+int fap_hip_malloc(void **d_ptr, int64_t bytes) { return dev_alloc(d_ptr, bytes); }
+int fap_hip_free(void *d_ptr) { return dev_free(d_ptr); }
+
+int fap_hip_memcpy_h2d(void *d_dst, const void *h_src, int64_t bytes, void *stream) {
+  if (bytes == 0) return FAP_OK;
+  FAP_REQUIRE(d_dst && h_src && bytes > 0, "fap_hip_memcpy_h2d: invalid arguments");
+  hipStream_t s = resolve(stream);
+  FAP_CHECK_HIP(hipMemcpyAsync(d_dst, h_src, (size_t)bytes, hipMemcpyHostToDevice, s));
+  FAP_CHECK_HIP(hipStreamSynchronize(s));
+  return FAP_OK;
+}
+
+int fap_hip_memcpy_d2h(void *h_dst, const void *d_src, int64_t bytes, void *stream) {
+  if (bytes == 0) return FAP_OK;
+  FAP_REQUIRE(h_dst && d_src && bytes > 0, "fap_hip_memcpy_d2h: invalid arguments");
+  hipStream_t s = resolve(stream);
+  FAP_CHECK_HIP(hipMemcpyAsync(h_dst, d_src, (size_t)bytes, hipMemcpyDeviceToHost, s));
+  FAP_CHECK_HIP(hipStreamSynchronize(s));
+  return FAP_OK;
+}
+
+int fap_hip_memcpy_d2d(void *d_dst, const void *d_src, int64_t bytes, void *stream) {
+  if (bytes == 0) return FAP_OK;
+  FAP_REQUIRE(d_dst && d_src && bytes > 0, "fap_hip_memcpy_d2d: invalid arguments");
+  FAP_CHECK_HIP(hipMemcpyAsync(d_dst, d_src, (size_t)bytes, hipMemcpyDeviceToDevice,
+                               resolve(stream)));
+  return FAP_OK;
+}
+
+int fap_hip_memcpy2d_d2h_i32(int32_t *h_dst, int64_t h_ld, const int32_t *d_src, int64_t d_ld,
+                             int64_t rows, int64_t cols, void *stream) {
+  if (rows == 0 || cols == 0) return FAP_OK;
+  FAP_REQUIRE(h_dst && d_src && rows > 0 && cols > 0 && h_ld >= cols && d_ld >= cols,
+              "fap_hip_memcpy2d_d2h_i32: invalid arguments");
+  hipStream_t s = resolve(stream);
+  FAP_CHECK_HIP(hipMemcpy2DAsync(h_dst, (size_t)h_ld * 4, d_src, (size_t)d_ld * 4,
+                                 (size_t)cols * 4, (size_t)rows, hipMemcpyDeviceToHost, s));
+  FAP_CHECK_HIP(hipStreamSynchronize(s));
+  return FAP_OK;
+}
+
+int fap_hip_fill_i32(int32_t *d_ptr, int32_t value, int64_t count, void *stream) {
+  if (count == 0) return FAP_OK;
+  FAP_REQUIRE(d_ptr && count > 0, "fap_hip_fill_i32: invalid arguments");
+  int64_t blocks = (count + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)blocks), dim3(256), 0, resolve(stream),
+                     d_ptr, value, count);
+  FAP_CHECK_LAUNCH();
+  return FAP_OK;
+}
+
+int fap_hip_stream_sync(void *stream) {
+  FAP_CHECK_HIP(hipStreamSynchronize(resolve(stream)));
+  return FAP_OK;
+}
+
+int fap_hip_device_sync(void) {
+  FAP_CHECK_HIP(hipDeviceSynchronize());
+  return FAP_OK;
+}
+
+int fap_hip_event_create(void **ev) {
+  FAP_REQUIRE(ev, "fap_hip_event_create: null");
+  hipEvent_t e;
+  FAP_CHECK_HIP(hipEventCreate(&e));
+  *ev = e;
+  return FAP_OK;
+}
+int fap_hip_event_destroy(void *ev) {
+  if (ev) FAP_CHECK_HIP(hipEventDestroy(static_cast<hipEvent_t>(ev)));
+  return FAP_OK;
+}
+int fap_hip_event_record(void *ev, void *stream) {
+  FAP_REQUIRE(ev, "fap_hip_event_record: null");
+  FAP_CHECK_HIP(hipEventRecord(static_cast<hipEvent_t>(ev), resolve(stream)));
+  return FAP_OK;
+}
+int fap_hip_event_elapsed_ms(void *a, void *b, float *ms) {
+  FAP_REQUIRE(a && b && ms, "fap_hip_event_elapsed_ms: null");
+  FAP_CHECK_HIP(hipEventSynchronize(static_cast<hipEvent_t>(b)));
+  FAP_CHECK_HIP(hipEventElapsedTime(ms, static_cast<hipEvent_t>(a), static_cast<hipEvent_t>(b)));
+  return FAP_OK;
+}
+
+}  // extern "C"

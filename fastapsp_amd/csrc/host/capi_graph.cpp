@@ -1,0 +1,228 @@
+// C wrapper of fap::fapGraph for FFI callers (declared in fap_hip.h).
+#include <cstring>
+#include <new>
+
+#include "fap/fap.h"
+#include "graph.h"
+
+struct fap_graph {
+  fap::fapGraph *g = nullptr;
+};
+
+namespace fap_hip {
+void set_error(const char *fmt, ...);
+}
+
+namespace {
+
+template <typename F>
+int guarded(F &&f) {
+  try {
+    f();
+    return FAP_OK;
+  } catch (const fap::FapError &e) {
+    fap_hip::set_error("%s", e.what());
+    return e.status();
+  } catch (const std::invalid_argument &e) {
+    fap_hip::set_error("%s", e.what());
+    return FAP_ERR_INVALID;
+  } catch (const std::bad_alloc &) {
+    fap_hip::set_error("host allocation failed");
+    return FAP_ERR_NOMEM;
+  } catch (const std::exception &e) {
+    fap_hip::set_error("%s", e.what());
+    return FAP_ERR_INVALID;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fap_graph_create_from_file(fap_graph **out, const char *file, int directed, int weighted,
+                               int32_t K, const char *partitioner) {
+  if (!out || !file) {
+    fap_hip::set_error("fap_graph_create_from_file: null argument");
+    return FAP_ERR_INVALID;
+  }
+  *out = nullptr;
+  return guarded([&] {
+    std::chrono::time_point<std::chrono::high_resolution_clock> t;
+    auto *h = new fap_graph;
+    try {
+      h->g = new fap::fapGraph(file, directed != 0, weighted != 0, K,
+                               partitioner && *partitioner ? partitioner : "metis", true, t);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int fap_graph_create_from_csr(fap_graph **out, int32_t n, int64_t m, const int32_t *row_offset,
+                              const int32_t *col_val, const float *weight, int directed,
+                              int32_t K) {
+  if (!out || !row_offset || (m > 0 && !col_val) || n < 1 || m < 0) {
+    fap_hip::set_error("fap_graph_create_from_csr: bad argument");
+    return FAP_ERR_INVALID;
+  }
+  *out = nullptr;
+  return guarded([&] {
+    std::vector<int32_t> ro(row_offset, row_offset + n + 1), cv(col_val, col_val + m);
+    std::vector<float> w;
+    if (weight) w.assign(weight, weight + m);
+    auto *h = new fap_graph;
+    try {
+      h->g = new fap::fapGraph(n, m, std::move(ro), std::move(cv), std::move(w), K, directed != 0);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int fap_graph_generate(fap_graph **out, int32_t kind, int32_t W, int32_t H, int32_t n_target,
+                       double p_a, double p_b, uint64_t seed, int32_t K) {
+  if (!out) {
+    fap_hip::set_error("fap_graph_generate: null argument");
+    return FAP_ERR_INVALID;
+  }
+  *out = nullptr;
+  return guarded([&] {
+    fap::GenGraph gg;
+    if (kind == 0) gg = fap::gen_road_grid(W, H, p_a, p_b, seed);
+    else if (kind == 1) gg = fap::gen_road_like(n_target, p_a, p_b, seed);
+    else if (kind == 2) gg = fap::gen_knn(n_target, W, H, seed);
+    else throw fap::FapError(FAP_ERR_INVALID, "unknown generator kind");
+    fap::Csr c = fap::csr_from_arcs(gg.n, gg.arcs, gg.directed);
+    const int64_t m = c.m();
+    std::vector<float> w;
+    if (gg.weighted) w = std::move(c.weight);
+    auto *h = new fap_graph;
+    try {
+      h->g = new fap::fapGraph(c.n, m, std::move(c.row_offset), std::move(c.col_val), std::move(w),
+                               K, gg.directed);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int fap_graph_destroy(fap_graph *h) {
+  if (!h) return FAP_OK;
+  return guarded([&] {
+    delete h->g;
+    delete h;
+  });
+}
+
+#define FAP_HANDLE(h)                                          \
+  if (!(h) || !(h)->g) {                                       \
+    fap_hip::set_error("%s: null graph handle", __func__);     \
+    return FAP_ERR_INVALID;                                    \
+  }
+
+int fap_graph_precondition(fap_graph *h) {
+  FAP_HANDLE(h);
+  return guarded([&] { h->g->preCondition(); });
+}
+
+int fap_graph_is_split(fap_graph *h, int *is_split) {
+  FAP_HANDLE(h);
+  return guarded([&] {
+    if (is_split) *is_split = h->g->isSplit() ? 1 : 0;
+  });
+}
+
+int fap_graph_set_mode(fap_graph *h, int32_t mode) {
+  FAP_HANDLE(h);
+  return guarded([&] { h->g->setMode(mode); });
+}
+
+int fap_graph_solve(fap_graph *h) {
+  FAP_HANDLE(h);
+  return guarded([&] { h->g->solve(); });
+}
+
+int fap_graph_solve_subgraph(fap_graph *h, int32_t id) {
+  FAP_HANDLE(h);
+  return guarded([&] { h->g->solveSubGraph(id); });
+}
+
+int fap_graph_sizes(fap_graph *h, int32_t *n, int64_t *m, int32_t *K) {
+  FAP_HANDLE(h);
+  if (n) *n = h->g->num_vertexs;
+  if (m) *m = h->g->num_edges;
+  if (K) *K = h->g->numParts();
+  return FAP_OK;
+}
+
+int fap_graph_get_csr(fap_graph *h, int32_t *row_offset, int32_t *col, int32_t *weight) {
+  FAP_HANDLE(h);
+  const fap::fapGraph &g = *h->g;
+  if (row_offset) memcpy(row_offset, g.row_offset.data(), 4 * g.row_offset.size());
+  if (col) memcpy(col, g.col_val.data(), 4 * g.col_val.size());
+  if (weight)
+    for (size_t i = 0; i < g.weight.size(); ++i) weight[i] = (int32_t)g.weight[i];
+  return FAP_OK;
+}
+
+int fap_graph_get_layout(fap_graph *h, int32_t *graph_id, int32_t *st2ed, int32_t *ed2st,
+                         int32_t *blk_num, int32_t *blk_off, int32_t *bdy_num) {
+  FAP_HANDLE(h);
+  return guarded([&] {
+    fap::fapGraph &g = *h->g;
+    const std::vector<int> gid = g.getGraphId();
+    if (graph_id) memcpy(graph_id, gid.data(), 4 * gid.size());
+    if (g.sortedToVertex().empty() && (st2ed || ed2st || blk_num || blk_off || bdy_num))
+      throw fap::FapError(FAP_ERR_STATE, "layout requested before preCondition()");
+    if (st2ed) memcpy(st2ed, g.sortedToVertex().data(), 4 * g.sortedToVertex().size());
+    if (ed2st) memcpy(ed2st, g.vertexToSorted().data(), 4 * g.vertexToSorted().size());
+    if (blk_num) memcpy(blk_num, g.blockNum().data(), 4 * g.blockNum().size());
+    if (blk_off) memcpy(blk_off, g.blockOffset().data(), 4 * g.blockOffset().size());
+    if (bdy_num) memcpy(bdy_num, g.blockBoundary().data(), 4 * g.blockBoundary().size());
+  });
+}
+
+int fap_graph_get_dist_rows(fap_graph *h, int64_t row0, int64_t rows, int32_t *h_out) {
+  FAP_HANDLE(h);
+  return guarded([&] { h->g->getDistanceRowsI32(row0, rows, h_out); });
+}
+
+int fap_graph_get_subgraph_dist(fap_graph *h, int32_t id, int32_t *h_out) {
+  FAP_HANDLE(h);
+  return guarded([&] {
+    std::vector<int32_t> d = h->g->getSubGraphDistanceI32(id);
+    memcpy(h_out, d.data(), 4 * d.size());
+  });
+}
+
+int fap_graph_device_dist(fap_graph *h, int32_t **d_dist, int64_t *ld) {
+  FAP_HANDLE(h);
+  if (d_dist) *d_dist = const_cast<int32_t *>(h->g->deviceDistance(ld));
+  return FAP_OK;
+}
+
+int fap_graph_times(fap_graph *h, double *t_part, double *t_sssp, double *t_floyd,
+                    double *t_minplus, double *t_data) {
+  FAP_HANDLE(h);
+  fap::fapGraph &g = *h->g;
+  if (t_part) *t_part = g.getTimePartitioning();
+  if (t_sssp) *t_sssp = g.getTimeSSSP();
+  if (t_floyd) *t_floyd = g.getTimeFloyd();
+  if (t_minplus) *t_minplus = g.getTimeMinPlus();
+  if (t_data) *t_data = g.getTimeData();
+  return FAP_OK;
+}
+
+int fap_graph_edge_cut(fap_graph *h, double *edge_cut) {
+  FAP_HANDLE(h);
+  if (edge_cut) *edge_cut = h->g->getEdgeCut();
+  return FAP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,320 @@
+// fap::fapGraph -- the drop-in host API over the C-ABI of fap_hip.h.
+//
+// Follows fap/fap.cpp:51-688 for the API surface and the per-part layout; all
+// per-part work goes through fap_hip_solve_parts.  There is no CPU fallback: a
+// missing or unusable GPU surfaces as FapError(FAP_ERR_NODEV).
+#include "fap/fap.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <sstream>
+
+#include "graph.h"
+
+namespace fap {
+
+const float MAXVALUE = 1e8f;
+
+namespace {
+
+void check(int status, const char *what) {
+  if (status != FAP_OK) {
+    std::ostringstream os;
+    os << what << ": " << fap_hip_last_error() << " (status " << status << ")";
+    throw FapError(status, os.str());
+  }
+}
+
+int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+float to_float(int32_t d) { return d >= FAP_INF ? MAXVALUE : (float)d; }
+
+}  // namespace
+
+void fapGraph::init_from_csr(bool dir, bool wtd, int32_t k, const std::string &part) {
+  directed = dir;
+  weighted = wtd;
+  K = k;
+  partitioner = part.empty() ? "metis" : part;
+  if (K < 1 || K > num_vertexs)
+    throw FapError(FAP_ERR_INVALID, "K must be in [1, n]");
+  if (partitioner != "metis")
+    throw std::invalid_argument("Invalid partitioner type: " + partitioner +
+                                " (this build provides \"metis\" k-way semantics only)");
+  for (float w : weight)
+    if (!(w >= 0.0f) || w != std::floor(w) || w >= 16777216.0f)
+      throw FapError(FAP_ERR_INVALID,
+                     "weights must be non-negative integers below 2^24 for int32 distances");
+  Csr g;
+  g.n = num_vertexs;
+  g.adj_size = adj_size;
+  g.row_offset = row_offset;
+  g.col_val = col_val;
+  g.weight = weight;
+  auto t0 = std::chrono::high_resolution_clock::now();
+  graph_id = partition_kway(g, K, 0x5eedULL, &edge_cut);
+  auto t1 = std::chrono::high_resolution_clock::now();
+  time_partitioning = std::chrono::duration<double>(t1 - t0).count();
+  BlockVer.clear();
+  for (int i = 0; i < num_vertexs; ++i) BlockVer[graph_id[i]].push_back(i);
+}
+
+fapGraph::fapGraph(std::string file, bool dir, bool wtd, int32_t k) : input_graph(file) {
+  Csr g = read_mtx(file, dir, wtd);
+  num_vertexs = g.n;
+  num_edges = (int32_t)g.m();
+  adj_size = std::move(g.adj_size);
+  row_offset = std::move(g.row_offset);
+  col_val = std::move(g.col_val);
+  weight = std::move(g.weight);
+  init_from_csr(dir, wtd, k, "metis");
+}
+
+fapGraph::fapGraph(std::string file, bool dir, bool wtd, int32_t k, std::string part, bool ver,
+                   std::chrono::time_point<std::chrono::high_resolution_clock> &endTimeIo)
+    : input_graph(file) {
+  version = ver;
+  Csr g = read_mtx(file, dir, wtd);
+  endTimeIo = std::chrono::high_resolution_clock::now();
+  num_vertexs = g.n;
+  num_edges = (int32_t)g.m();
+  adj_size = std::move(g.adj_size);
+  row_offset = std::move(g.row_offset);
+  col_val = std::move(g.col_val);
+  weight = std::move(g.weight);
+  init_from_csr(dir, wtd, k, part);
+}
+
+fapGraph::fapGraph(int32_t n, int64_t m, std::vector<int32_t> ro, std::vector<int32_t> cv,
+                   std::vector<float> w, int32_t k, bool dir) {
+  if (n < 1 || (int64_t)ro.size() != n + 1 || (int64_t)cv.size() != m || ro[0] != 0 ||
+      ro[n] != m || (!w.empty() && (int64_t)w.size() != m))
+    throw FapError(FAP_ERR_INVALID, "inconsistent CSR");
+  for (int32_t i = 0; i < n; ++i)
+    if (ro[i + 1] < ro[i]) throw FapError(FAP_ERR_INVALID, "row_offset not monotone");
+  for (int32_t c : cv)
+    if (c < 0 || c >= n) throw FapError(FAP_ERR_INVALID, "column index out of range");
+  num_vertexs = n;
+  num_edges = (int32_t)m;
+  row_offset = std::move(ro);
+  col_val = std::move(cv);
+  adj_size.resize(n);
+  for (int32_t i = 0; i < n; ++i) adj_size[i] = row_offset[i + 1] - row_offset[i];
+  const bool wtd = !w.empty();
+  weight = wtd ? std::move(w) : std::vector<float>(m, 1.0f);
+  init_from_csr(dir, wtd, k, "metis");
+}
+
+fapGraph::~fapGraph() { release(); }
+
+void fapGraph::release() {
+  int32_t *ptrs[] = {d_row_offset, d_col, d_w, d_graph_id, d_st2ed, d_ed2st, d_dist, d_sub_dist};
+  for (int32_t *p : ptrs)
+    if (p) fap_hip_free(p);
+  d_row_offset = d_col = d_w = d_graph_id = d_st2ed = d_ed2st = d_dist = d_sub_dist = nullptr;
+}
+
+void fapGraph::preCondition() {
+  Csr g;
+  g.n = num_vertexs;
+  g.row_offset = row_offset;
+  g.col_val = col_val;
+  std::vector<int> isb;
+  find_boundary(K, g, graph_id, isb, BlockBoundary);
+  isBoundry = isb;
+  sort_and_encode(K, num_vertexs, graph_id, isb, BlockVer, BlockBoundary, C_BlockVer_num,
+                  C_BlockVer_offset, C_BlockBdy_num, C_BlockBdy_offset, st2ed, ed2st);
+  max_subgraph_vertexs = 0;
+  for (int p = 1; p <= K; ++p) max_subgraph_vertexs = std::max(max_subgraph_vertexs, C_BlockVer_num[p]);
+  is_split = num_vertexs > max_vertexs_num_limit;
+  release();  // device copies of an earlier layout are stale
+  preconditioned = true;
+}
+
+void fapGraph::upload() {
+  check(fap_hip_init(0), "fap_hip_init");
+  release();
+  const int64_t n = num_vertexs, m = num_edges;
+  auto put = [&](int32_t **d, const void *h, int64_t count) {
+    check(fap_hip_malloc((void **)d, std::max<int64_t>(count, 1) * 4), "device allocation");
+    if (count > 0) check(fap_hip_memcpy_h2d(*d, h, count * 4, nullptr), "host to device copy");
+  };
+  put(&d_row_offset, row_offset.data(), n + 1);
+  put(&d_col, col_val.data(), m);
+  bool unit = true;
+  for (float w : weight)
+    if (w != 1.0f) unit = false;
+  if (!unit) {
+    std::vector<int32_t> wi(weight.size());
+    for (size_t i = 0; i < weight.size(); ++i) wi[i] = (int32_t)weight[i];
+    put(&d_w, wi.data(), m);
+  }
+  put(&d_graph_id, graph_id.data(), n);
+  put(&d_st2ed, st2ed.data(), n);
+  put(&d_ed2st, ed2st.data(), n);
+  ld_dist = align_up(n, 64);
+}
+
+bool fapGraph::isSplit() { return is_split; }
+std::vector<int> fapGraph::getGraphId() { return graph_id; }
+
+void fapGraph::setMode(int32_t m) {
+  if (m != FAP_MODE_PLAIN && m != FAP_MODE_BATCHED)
+    throw FapError(FAP_ERR_INVALID, "unknown mode");
+  mode = m;
+}
+
+void fapGraph::add_stage_times() {
+  fap_stage_times t;
+  check(fap_hip_last_stage_times(&t), "stage times");
+  time_sssp += t.sssp_ms * 1e-3;
+  time_floyd += t.floyd_ms * 1e-3;
+  time_min_plus += t.minplus_ms * 1e-3;
+  time_data_transformation += (t.build_ms + t.decode_ms) * 1e-3;
+}
+
+int32_t fapGraph::solve(bool path_needed) {
+  if (!preconditioned) throw FapError(FAP_ERR_STATE, "solve() before preCondition()");
+  if (path_needed) throw FapError(FAP_ERR_STATE, "predecessor matrices are not produced");
+  if (!d_row_offset) upload();
+  if (!d_dist)
+    check(fap_hip_malloc((void **)&d_dist, (int64_t)num_vertexs * ld_dist * 4),
+          "n x n distance allocation");
+  std::vector<fap_part_desc> parts;
+  for (int p = 1; p <= K; ++p)
+    parts.push_back(fap_part_desc{p, C_BlockVer_offset[p], C_BlockVer_num[p], C_BlockBdy_num[p],
+                                  (int64_t)C_BlockVer_offset[p]});
+  fap_graph_dev g{num_vertexs, num_edges, d_row_offset, d_col, d_w, d_graph_id, d_st2ed, d_ed2st};
+  check(fap_hip_solve_parts(&g, parts.data(), (int32_t)parts.size(), d_dist, ld_dist, mode, nullptr),
+        "solve");
+  check(fap_hip_stream_sync(nullptr), "solve");
+  add_stage_times();
+  return 0;
+}
+
+int32_t fapGraph::solveSubGraph(int32_t id, bool path_needed) {
+  if (!preconditioned) throw FapError(FAP_ERR_STATE, "solveSubGraph() before preCondition()");
+  if (path_needed) throw FapError(FAP_ERR_STATE, "predecessor matrices are not produced");
+  if (id < 1 || id > K) throw FapError(FAP_ERR_INVALID, "sub graph id out of range");
+  if (!d_row_offset) upload();
+  const int64_t rows = std::max(max_subgraph_vertexs, 1);
+  if (!d_sub_dist)
+    check(fap_hip_malloc((void **)&d_sub_dist, rows * ld_dist * 4), "sub-graph allocation");
+  fap_part_desc part{id, C_BlockVer_offset[id], C_BlockVer_num[id], C_BlockBdy_num[id], 0};
+  fap_graph_dev g{num_vertexs, num_edges, d_row_offset, d_col, d_w, d_graph_id, d_st2ed, d_ed2st};
+  check(fap_hip_solve_parts(&g, &part, 1, d_sub_dist, ld_dist, mode, nullptr), "solveSubGraph");
+  check(fap_hip_stream_sync(nullptr), "solveSubGraph");
+  add_stage_times();
+  current_subgraph_id = id;
+  sub_rows = C_BlockVer_num[id];
+  return 0;
+}
+
+std::vector<int32_t> fapGraph::getMapping() { return st2ed; }
+
+void fapGraph::getDistanceRowsI32(int64_t row0, int64_t rows, int32_t *out) {
+  if (!d_dist) throw FapError(FAP_ERR_STATE, "no solve() result");
+  if (row0 < 0 || rows < 0 || row0 + rows > num_vertexs)
+    throw FapError(FAP_ERR_INVALID, "row range out of bounds");
+  check(fap_hip_memcpy2d_d2h_i32(out, num_vertexs, d_dist + row0 * ld_dist, ld_dist, rows,
+                                 num_vertexs, nullptr),
+        "device to host copy");
+}
+
+std::vector<int32_t> fapGraph::getAllDistanceI32() {
+  std::vector<int32_t> out((size_t)num_vertexs * num_vertexs);
+  getDistanceRowsI32(0, num_vertexs, out.data());
+  return out;
+}
+
+std::vector<float> fapGraph::getAllDistance() {
+  const int64_t n = num_vertexs;
+  std::vector<float> out((size_t)(n * n));
+  std::vector<int32_t> buf;
+  const int64_t chunk = std::max<int64_t>(1, (64LL << 20) / std::max<int64_t>(n, 1));
+  for (int64_t r = 0; r < n; r += chunk) {
+    const int64_t rows = std::min(chunk, n - r);
+    buf.resize((size_t)(rows * n));
+    getDistanceRowsI32(r, rows, buf.data());
+    for (int64_t i = 0; i < rows * n; ++i) out[(size_t)(r * n + i)] = to_float(buf[(size_t)i]);
+  }
+  return out;
+}
+
+std::vector<int32_t> fapGraph::getAllPath() {
+  throw FapError(FAP_ERR_STATE, "predecessor matrices are not produced");
+}
+
+int32_t fapGraph::getCurrentSubGraphId() { return current_subgraph_id; }
+
+std::vector<int32_t> fapGraph::getSubGraphIndex(int32_t id) {
+  if (id < 1 || id > K) throw FapError(FAP_ERR_INVALID, "sub graph id out of range");
+  const int off = C_BlockVer_offset[id], sz = C_BlockVer_num[id];
+  return std::vector<int32_t>(st2ed.begin() + off, st2ed.begin() + off + sz);
+}
+
+std::vector<int32_t> fapGraph::getSubGraphDistanceI32(int32_t id) {
+  if (id != current_subgraph_id || !d_sub_dist)
+    throw FapError(FAP_ERR_STATE, "sub graph was not the last one solved");
+  std::vector<int32_t> out((size_t)(sub_rows * num_vertexs));
+  if (sub_rows > 0)
+    check(fap_hip_memcpy2d_d2h_i32(out.data(), num_vertexs, d_sub_dist, ld_dist, sub_rows,
+                                   num_vertexs, nullptr),
+          "device to host copy");
+  return out;
+}
+
+std::vector<float> fapGraph::getSubGraphDistance(int32_t id) {
+  std::vector<int32_t> d = getSubGraphDistanceI32(id);
+  std::vector<float> out(d.size());
+  for (size_t i = 0; i < d.size(); ++i) out[i] = to_float(d[i]);
+  return out;
+}
+
+std::vector<int32_t> fapGraph::getSubGraphPath(int32_t) {
+  throw FapError(FAP_ERR_STATE, "predecessor matrices are not produced");
+}
+
+std::vector<float> fapGraph::getDistanceFromOnePoint(int32_t v) {
+  if (v < 0 || v >= num_vertexs) throw FapError(FAP_ERR_INVALID, "vertex out of range");
+  if (graph_id[v] != current_subgraph_id)
+    throw FapError(FAP_ERR_STATE, "vertex is not in the last solved sub graph");
+  const int64_t row = ed2st[v] - C_BlockVer_offset[graph_id[v]];
+  std::vector<int32_t> buf(num_vertexs);
+  check(fap_hip_memcpy_d2h(buf.data(), d_sub_dist + row * ld_dist, (int64_t)num_vertexs * 4, nullptr),
+        "device to host copy");
+  std::vector<float> out(num_vertexs);
+  for (int i = 0; i < num_vertexs; ++i) out[i] = to_float(buf[i]);
+  return out;
+}
+
+std::vector<int32_t> fapGraph::getPathFromOnePoint(int32_t) {
+  throw FapError(FAP_ERR_STATE, "predecessor matrices are not produced");
+}
+
+const int32_t *fapGraph::deviceDistance(int64_t *ld) const {
+  if (ld) *ld = ld_dist;
+  return d_dist;
+}
+
+double fapGraph::getTimeSSSP() { return time_sssp; }
+double fapGraph::getTimeFloyd() { return time_floyd; }
+double fapGraph::getTimeMinPlus() { return time_min_plus; }
+double fapGraph::getTimePartitioning() { return time_partitioning; }
+double fapGraph::getTimeData() { return time_data_transformation; }
+double fapGraph::getEdgeCut() { return edge_cut; }
+
+int fapGraph::getBoundaryNum() {
+  int s = 0;
+  for (size_t i = 0; i < C_BlockBdy_num.size(); ++i) s += C_BlockBdy_num[i];
+  return s;
+}
+
+int fapGraph::getInnerNum() {
+  int s = 0;
+  for (size_t i = 0; i < C_BlockVer_num.size(); ++i) s += C_BlockVer_num[i] - C_BlockBdy_num[i];
+  return s;
+}
+
+}  // namespace fap

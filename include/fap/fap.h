@@ -1,0 +1,156 @@
+// fap/fap.h -- drop-in C++ API of the MI355X Fast-APSP (class fap::fapGraph).
+//
+// Mirrors fap/fap.h:25-162 of TomerHaham/FastAPSP: the (file, directed, weighted,
+// K[, partitioner, version, endTimeIo]) constructors, preCondition, isSplit,
+// solve, solveSubGraph and the getters.  Differences a caller can see:
+//  * distances are computed as int32 on the GPU (FAP_INF = INT32_MAX/2 for
+//    unreachable); getAllDistance / getSubGraphDistance still return float with
+//    the reference's MAXVALUE = 1e8 for unreachable, and *I32 variants return
+//    the int32 values without conversion;
+//  * the result of solve() stays resident in HBM (n x ld int32); host vectors are
+//    produced only by the getters;
+//  * errors throw fap::FapError (carrying the C-ABI status) instead of exit();
+//  * solveSubGraph recomputes on every call (no process-wide "already processed"
+//    skip, fap/fap.cpp:522-528);
+//  * getDistanceFromOnePoint uses the sorted position of the vertex (the
+//    reference indexes C_BlockVer_offset by vertex id, fap/fap.cpp:584-585).
+// Predecessor (path) matrices are not produced: getAllPath / getSubGraphPath /
+// getPathFromOnePoint throw FapError(FAP_ERR_STATE).
+#pragma once
+
+#include <chrono>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "fap_hip.h"
+
+namespace fap {
+
+extern const float MAXVALUE;  // 1e8, fap/utils/parameter.h:26
+
+class FapError : public std::runtime_error {
+ public:
+  FapError(int status, const std::string &msg) : std::runtime_error(msg), status_(status) {}
+  int status() const { return status_; }
+
+ private:
+  int status_;
+};
+
+class fapGraph {
+ protected:
+  std::string input_graph;
+  std::string partitioner;
+  const int32_t max_vertexs_num_limit = 20000;  // fap/fap.h:35
+  bool is_split = false;
+  bool is_path_needed = false;
+  bool version = true;
+  bool preconditioned = false;
+  int32_t mode = FAP_MODE_BATCHED;
+  double time_partitioning = 0;
+  double time_sssp = 0;
+  double time_floyd = 0;
+  double time_min_plus = 0;
+  double time_data_transformation = 0;
+  double edge_cut = 0;
+
+  int32_t current_subgraph_id = 0;
+
+  int32_t K = 0;
+  std::vector<int> graph_id;
+  std::unordered_map<int, std::vector<int>> BlockVer;
+  std::unordered_map<int, std::vector<int>> BlockBoundary;
+  std::vector<int> isBoundry;
+  std::vector<int> st2ed;
+  std::vector<int> ed2st;
+  std::vector<int> C_BlockVer_num;
+  std::vector<int> C_BlockVer_offset;
+  std::vector<int> C_BlockBdy_num;
+  std::vector<int> C_BlockBdy_offset;
+  int32_t max_subgraph_vertexs = 0;
+
+  // Device residency (owned).
+  int32_t *d_row_offset = nullptr, *d_col = nullptr, *d_w = nullptr;
+  int32_t *d_graph_id = nullptr, *d_st2ed = nullptr, *d_ed2st = nullptr;
+  int32_t *d_dist = nullptr;      // n x ld_dist, result of solve()
+  int32_t *d_sub_dist = nullptr;  // s_p x ld_dist, result of solveSubGraph()
+  int64_t ld_dist = 0;
+  int64_t sub_rows = 0;
+
+  void init_from_csr(bool directed, bool weighted, int32_t K, const std::string &partitioner);
+  void upload();
+  void release();
+  void add_stage_times();
+
+ public:
+  int32_t num_vertexs = 0;
+  int32_t num_edges = 0;
+  std::vector<int32_t> adj_size;
+  std::vector<int32_t> row_offset;
+  std::vector<int32_t> col_val;
+  std::vector<float> weight;
+  bool directed = false;
+  bool weighted = false;
+
+  // (file, directed, weighted, K) -- README.md:44; partitioner "metis".
+  fapGraph(std::string input_graph, bool directed, bool weighted, int32_t K);
+  // fap/fap.h:100-101.  `version` is accepted and ignored: every solve runs on the GPU.
+  fapGraph(std::string input_graph, bool directed, bool weighted, int32_t K,
+           std::string partitioner, bool version,
+           std::chrono::time_point<std::chrono::high_resolution_clock> &endTimeIo);
+  // fap/fap.h:102-104 (the reference leaves it empty, fap/fap.cpp:94-98).
+  fapGraph(int32_t num_vertexs, int64_t num_edges, std::vector<int32_t> row_offset,
+           std::vector<int32_t> col_val, std::vector<float> weight, int32_t K,
+           bool directed = false);
+  ~fapGraph();
+  fapGraph(const fapGraph &) = delete;
+  fapGraph &operator=(const fapGraph &) = delete;
+
+  void preCondition();
+  bool isSplit();
+  std::vector<int> getGraphId();
+
+  // Execution form of solve(): FAP_MODE_BATCHED (default) or FAP_MODE_PLAIN.
+  void setMode(int32_t mode);
+
+  int32_t solve(bool is_path_needed = false);
+  int32_t solveSubGraph(int32_t sub_graph_id, bool is_path_needed = false);
+
+  std::vector<int32_t> getMapping();
+  std::vector<float> getAllDistance();
+  std::vector<int32_t> getAllDistanceI32();
+  std::vector<int32_t> getAllPath();
+  int32_t getCurrentSubGraphId();
+  std::vector<int32_t> getSubGraphIndex(int32_t sub_graph_id);
+  std::vector<float> getSubGraphDistance(int32_t sub_graph_id);
+  std::vector<int32_t> getSubGraphDistanceI32(int32_t sub_graph_id);
+  std::vector<int32_t> getSubGraphPath(int32_t sub_graph_id);
+  std::vector<float> getDistanceFromOnePoint(int32_t vertex_id);
+  std::vector<int32_t> getPathFromOnePoint(int32_t vertex_id);
+  // Rows [row0, row0 + rows) of the solve() result (int32), without a full copy.
+  void getDistanceRowsI32(int64_t row0, int64_t rows, int32_t *out);
+  // Device pointer / row stride of the resident solve() result.
+  const int32_t *deviceDistance(int64_t *ld) const;
+
+  double getTimeSSSP();
+  double getTimeFloyd();
+  double getTimeMinPlus();
+  double getTimePartitioning();
+  double getTimeData();
+  int getBoundaryNum();
+  int getInnerNum();
+  double getEdgeCut();
+
+  // Layout accessors (sorted order): C_BlockVer_num / offset, C_BlockBdy_num.
+  const std::vector<int> &blockNum() const { return C_BlockVer_num; }
+  const std::vector<int> &blockOffset() const { return C_BlockVer_offset; }
+  const std::vector<int> &blockBoundary() const { return C_BlockBdy_num; }
+  const std::vector<int> &sortedToVertex() const { return st2ed; }
+  const std::vector<int> &vertexToSorted() const { return ed2st; }
+  int32_t numParts() const { return K; }
+};
+
+}  // namespace fap

@@ -1,0 +1,217 @@
+/*
+ * fap_hip.h -- C-ABI of the MI355X (gfx950) Fast-APSP hot path.
+ *
+ * Plain pointers, int64 sizes and strides, an int status return (never exit()),
+ * explicit streams (a hipStream_t passed as void*, NULL = the library stream).
+ * Every d_* pointer is device memory owned by the caller unless stated.
+ *
+ * Distances are int32 with FAP_INF (= INT32_MAX / 2) for "unreachable"; the
+ * reference's float MAXVALUE = 1e8 (fap/utils/parameter.h:26) maps to FAP_INF and
+ * every finite integer distance below 2^24 maps 1:1, so INF + INF never overflows.
+ *
+ * Each entry point names the reference interface it replaces (file:line inside
+ * TomerHaham/FastAPSP).  The seam there is C++ free functions selected by
+ * #ifdef WITH_CUDA / WITH_HIP (fap/kernel.h:15-17); here it is this C-ABI.
+ */
+#ifndef FAP_HIP_H_
+#define FAP_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FAP_INF 1073741823 /* INT32_MAX / 2 */
+
+enum fap_status {
+  FAP_OK = 0,
+  FAP_ERR_INVALID = 1, /* bad argument (sizes, null pointers, alignment) */
+  FAP_ERR_NOMEM = 2,   /* device allocation failed                        */
+  FAP_ERR_HIP = 3,     /* a HIP runtime call failed                       */
+  FAP_ERR_BUDGET = 4,  /* the request exceeds the HBM budget              */
+  FAP_ERR_IO = 5,      /* file missing or malformed                       */
+  FAP_ERR_STATE = 6,   /* call out of order (e.g. solve before preCondition) */
+  FAP_ERR_NODEV = 7    /* no usable GPU                                   */
+};
+
+/* Execution forms of the per-part pipeline.
+ * PLAIN:   one launch sequence per part (SSSP, build, FW rounds, min-plus,
+ *          decode), parts in turn -- the structure of fapGraph::run
+ *          (fap/fap.cpp:195-503) called from solve() (fap/fap.cpp:507-518).
+ * BATCHED: every stage launched once for all parts (boundary SSSP of all parts
+ *          in one grid, all FW matrices per round in one grid, one min-plus
+ *          grid over all parts with a work table). */
+enum fap_mode { FAP_MODE_PLAIN = 0, FAP_MODE_BATCHED = 1 };
+
+/* ------------------------------------------------------------------ runtime */
+
+/* Select the device and create the library stream.  Idempotent.
+ * Replaces the implicit CUDA context use of fap/fap.cpp:222-388. */
+int fap_hip_init(int device);
+/* Message of the last failing call on this host thread ("" if none). */
+const char *fap_hip_last_error(void);
+/* Device properties: CU count, LDS bytes per workgroup, total HBM bytes. */
+int fap_hip_device_info(int *cu_count, int64_t *lds_bytes, int64_t *hbm_bytes, char *name, int name_len);
+
+/* HBM budget for fap_hip_malloc (0 = no limit).  Allocations beyond it fail with
+ * FAP_ERR_BUDGET before touching the allocator.  Replaces the cudaMemGetInfo
+ * accounting of fap/fap.cpp:240-243,296-299,356-358. */
+int fap_hip_set_budget(int64_t bytes);
+int fap_hip_mem_stats(int64_t *in_use, int64_t *high_water, int64_t *budget);
+int fap_hip_malloc(void **d_ptr, int64_t bytes);
+int fap_hip_free(void *d_ptr);
+int fap_hip_memcpy_h2d(void *d_dst, const void *h_src, int64_t bytes, void *stream);
+int fap_hip_memcpy_d2h(void *h_dst, const void *d_src, int64_t bytes, void *stream);
+int fap_hip_memcpy_d2d(void *d_dst, const void *d_src, int64_t bytes, void *stream);
+/* Strided D2H of rows x cols int32 (pitch in elements). */
+int fap_hip_memcpy2d_d2h_i32(int32_t *h_dst, int64_t h_ld, const int32_t *d_src, int64_t d_ld,
+                             int64_t rows, int64_t cols, void *stream);
+int fap_hip_fill_i32(int32_t *d_ptr, int32_t value, int64_t count, void *stream);
+int fap_hip_stream_sync(void *stream);
+int fap_hip_device_sync(void);
+
+/* Events for timing on a given stream (ms between two recorded events). */
+int fap_hip_event_create(void **ev);
+int fap_hip_event_destroy(void *ev);
+int fap_hip_event_record(void *ev, void *stream);
+int fap_hip_event_elapsed_ms(void *ev_start, void *ev_end, float *ms);
+
+/* ------------------------------------------------------- kernel-level seam */
+
+/* Boundary SSSP: one shortest-path row per source over the WHOLE graph.
+ * d_out[i * ld_out + v] = dist(d_src[i], v), FAP_INF if unreachable.
+ * d_w == NULL: unit weights (BFS).  Otherwise int32 weights >= 0.
+ * CSR: d_row_offset[n + 1], d_col[m], d_w[m].  ld_out >= n.
+ * Replaces handle_boundry_path / handle_boundry_path_data_on_gpu
+ * (fap/kernel/batch_sssp.h:254-303) -> batched_sssp_path (:232-252) ->
+ * dijkstra_path (:79-114); the reference seam takes st2ed + offset, here
+ * d_src = d_st2ed + offset. */
+int fap_hip_sssp_batch(const int32_t *d_row_offset, const int32_t *d_col, const int32_t *d_w,
+                       int32_t n, int64_t m, const int32_t *d_src, int32_t nsrc,
+                       int32_t *d_out, int64_t ld_out, void *stream);
+
+/* Floyd-Warshall closure of one s x s int32 matrix in place (row stride ld >= s).
+ * Entries must be >= 0 or FAP_INF with a 0 diagonal.  Result equals the
+ * k-i-j triple loop of floyd_CPU_path (fap/kernel/floyd.h:38-49) for distances.
+ * Replaces floyd_path_gpu / floyd_path (fap/kernel/floyd.h:150-170). */
+int fap_hip_floyd(int32_t *d_mat, int32_t s, int64_t ld, void *stream);
+
+/* Min-plus product with overwrite semantics:
+ *   C[i][j] = min(FAP_INF, min_k A[i][k] + B[k][j]),  0 <= i < M, 0 <= j < N.
+ * K == 0 gives all FAP_INF.  Strides in elements.
+ * Replaces min_plus_path_advanced / min_plus_path_advanced_gpu
+ * (fap/kernel/minplus.h:81-105) -> min_plus_path_CPU_advanced (:51-64). */
+int fap_hip_minplus(const int32_t *d_A, int64_t lda, const int32_t *d_B, int64_t ldb,
+                    int32_t *d_C, int64_t ldc, int32_t M, int32_t N, int32_t K, void *stream);
+
+/* Dense s x s sub-matrix of one part (row stride ld_sub >= s):
+ *   rows i <  bdy : d_dist_rows[i * ld_dist + st2ed[start + j]]  (the SSSP rows)
+ *   rows i >= bdy : FAP_INF, then each intra-part arc of st2ed[start+i] in CSR
+ *                   order writes its weight (a later arc wins)
+ *   diagonal      : 0.
+ * d_w == NULL: unit weights.  Replaces MysubMatBuild_path
+ * (fap/utils/util_Myalgorithm.h:44-84) / LaunchMysubMatBuildKernel (:143-191). */
+int fap_hip_submat_build(int32_t *d_sub, int64_t ld_sub, const int32_t *d_dist_rows, int64_t ld_dist,
+                         const int32_t *d_row_offset, const int32_t *d_col, const int32_t *d_w,
+                         const int32_t *d_graph_id, const int32_t *d_st2ed, const int32_t *d_ed2st,
+                         int32_t n, int32_t start, int32_t s, int32_t bdy, void *stream);
+
+/* Scatter rows x s of the sub-matrix into final columns:
+ *   d_out[i * ld_out + st2ed[start + j]] = d_sub[i * ld_sub + j].
+ * Replaces MysubMatDecode_path (fap/utils/util_Myalgorithm.h:236-247) /
+ * LaunchMysubMatDecodePathKernel (:215-231). */
+int fap_hip_submat_decode(const int32_t *d_sub, int64_t ld_sub, int32_t *d_out, int64_t ld_out,
+                          const int32_t *d_st2ed, int32_t start, int32_t rows, int32_t s, void *stream);
+
+/* ----------------------------------------------------- fused part pipeline */
+
+/* Device view of one graph after preCondition (all arrays device-resident). */
+typedef struct fap_graph_dev {
+  int32_t n;
+  int64_t m;
+  const int32_t *d_row_offset; /* [n + 1] */
+  const int32_t *d_col;        /* [m]     */
+  const int32_t *d_w;          /* [m] or NULL for unit weights */
+  const int32_t *d_graph_id;   /* [n], part ids 1..K */
+  const int32_t *d_st2ed;      /* [n], sorted position -> vertex */
+  const int32_t *d_ed2st;      /* [n], vertex -> sorted position */
+} fap_graph_dev;
+
+/* One part: C_BlockVer_offset / C_BlockVer_num / C_BlockBdy_num of
+ * fap/utils/util_preprocess.h:98-105 and the first output row of its block. */
+typedef struct fap_part_desc {
+  int32_t part_id; /* 1-based, as graph_id */
+  int32_t offset;  /* first sorted position of the part */
+  int32_t num;     /* s_p: vertices in the part */
+  int32_t bdy;     /* b_p: boundary vertices (first b_p sorted positions) */
+  int64_t out_row; /* first row of the part's s_p x n block in d_out */
+} fap_part_desc;
+
+/* Per-stage device time of the last fap_hip_solve_parts call (ms). */
+typedef struct fap_stage_times {
+  float sssp_ms, build_ms, floyd_ms, minplus_ms, decode_ms, total_ms;
+} fap_stage_times;
+
+/* The whole per-part pipeline of fapGraph::run (fap/fap.cpp:195-503) for a list
+ * of parts: boundary SSSP -> sub-matrix build -> Floyd-Warshall -> min-plus of
+ * inner rows against the boundary rows, with the decode of in-part columns fused
+ * into the min-plus epilogue.  Part p writes rows out_row .. out_row + s_p - 1 of
+ * d_out (row stride ld_out >= n; ld_out % 4 == 0 and 16-byte aligned d_out give
+ * vector stores).  Host-side descriptors; the call is asynchronous on `stream`.
+ * mode: FAP_MODE_PLAIN or FAP_MODE_BATCHED (identical results). */
+int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *parts, int32_t nparts,
+                        int32_t *d_out, int64_t ld_out, int32_t mode, void *stream);
+int fap_hip_last_stage_times(fap_stage_times *t);
+
+/* --------------------------------------------------------------- graph API */
+/* C wrapper of the C++ fap::fapGraph (include/fap/fap.h) for FFI callers
+ * (ctypes, cgo).  Mirrors fap/fap.h:100-150.  Opaque handle. */
+typedef struct fap_graph fap_graph;
+
+/* (file, directed, weighted, K, partitioner) ctor, fap/fap.h:100-101.
+ * file: "<name>" opened as graph/<(un)weight>-<(un)directed>/<name>.mtx relative
+ * to the working directory (fap/utils/readMatFile.h:43-51), or a path ending in
+ * ".mtx" opened as given. partitioner: "metis" (default when NULL or ""). */
+int fap_graph_create_from_file(fap_graph **g, const char *file, int directed, int weighted,
+                               int32_t K, const char *partitioner);
+/* From a CSR in the reference's layout (row_offset[n+1], col_val[m], weight[m]
+ * floats, weight may be NULL = unit).  fap/fap.h:102-104. */
+int fap_graph_create_from_csr(fap_graph **g, int32_t n, int64_t m, const int32_t *row_offset,
+                              const int32_t *col_val, const float *weight, int directed, int32_t K);
+/* Synthetic graphs: kind 0 = road grid (W x H grid, drop p_a of edges, add
+ * down-right diagonals with p_b), 1 = road-like percolated grid trimmed to
+ * exactly n_target vertices (keep p_a, chords at rate p_b), 2 = directed k-NN
+ * geometric graph (n_target points, k = W nearest, int weights in [1, H]).
+ * Seeded std::mt19937_64; identical on every machine. */
+int fap_graph_generate(fap_graph **g, int32_t kind, int32_t W, int32_t H, int32_t n_target,
+                       double p_a, double p_b, uint64_t seed, int32_t K);
+int fap_graph_destroy(fap_graph *g);
+int fap_graph_precondition(fap_graph *g);
+int fap_graph_is_split(fap_graph *g, int *is_split);
+int fap_graph_set_mode(fap_graph *g, int32_t mode);
+int fap_graph_solve(fap_graph *g);
+int fap_graph_solve_subgraph(fap_graph *g, int32_t id);
+int fap_graph_sizes(fap_graph *g, int32_t *n, int64_t *m, int32_t *K);
+/* Host copies: CSR (row_offset[n+1], col[m], weight[m] as int32), layout arrays
+ * (graph_id[n], st2ed[n], ed2st[n]) and per-part counts (num[K+1], offset[K+1],
+ * bdy[K+1], index 0 unused as in the reference).  NULL outputs are skipped. */
+int fap_graph_get_csr(fap_graph *g, int32_t *row_offset, int32_t *col, int32_t *weight);
+int fap_graph_get_layout(fap_graph *g, int32_t *graph_id, int32_t *st2ed, int32_t *ed2st,
+                         int32_t *blk_num, int32_t *blk_off, int32_t *bdy_num);
+/* Rows [row0, row0 + rows) of the full n x n result of solve(), int32 (D2H). */
+int fap_graph_get_dist_rows(fap_graph *g, int64_t row0, int64_t rows, int32_t *h_out);
+/* The s_p x n block of the last solveSubGraph(id). */
+int fap_graph_get_subgraph_dist(fap_graph *g, int32_t id, int32_t *h_out);
+/* Device pointer and row stride of the resident solve() result. */
+int fap_graph_device_dist(fap_graph *g, int32_t **d_dist, int64_t *ld);
+/* Stage times (s): partitioning, sssp, floyd, min-plus, data transformation. */
+int fap_graph_times(fap_graph *g, double *t_part, double *t_sssp, double *t_floyd,
+                    double *t_minplus, double *t_data);
+int fap_graph_edge_cut(fap_graph *g, double *edge_cut);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FAP_HIP_H_ */
