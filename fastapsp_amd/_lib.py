@@ -81,11 +81,15 @@ SIGNATURES = {
                                       _vp, _i64, _i32, _vp]),
     "fap_hip_last_stage_times": (C.c_int, [C.POINTER(FapStageTimes)]),
     "fap_graph_create_from_file": (C.c_int, [C.POINTER(_vp), _cp, C.c_int, C.c_int, _i32, _cp]),
+    "fap_mtx_read": (C.c_int, [_cp, C.c_int, C.c_int, _pi32, _pi64, _pi32, _pi32, _pf32]),
     "fap_graph_create_from_csr": (C.c_int, [C.POINTER(_vp), _i32, _i64, _pi32, _pi32, _pf32,
                                             C.c_int, _i32]),
     "fap_graph_generate": (C.c_int, [C.POINTER(_vp), _i32, _i32, _i32, _i32, _f64, _f64, _u64,
                                      _i32]),
     "fap_graph_destroy": (C.c_int, [_vp]),
+    "fap_graph_set_partition": (C.c_int, [_vp, _pi32]),
+    "fap_layout_for_partition": (C.c_int, [_i32, _pi32, _pi32, _i32, _pi32, _pi32, _pi32, _pi32,
+                                           _pi32, _pi32]),
     "fap_graph_precondition": (C.c_int, [_vp]),
     "fap_graph_is_split": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "fap_graph_set_mode": (C.c_int, [_vp, _i32]),

@@ -22,6 +22,7 @@
 //  * output rows are written once with non-temporal 16-byte stores.
 #include "fap_internal.h"
 #include "fw_desc.h"
+#include "gmem.h"
 #include "mp_tile.h"
 
 namespace fap_hip {
@@ -168,6 +169,176 @@ __global__ __launch_bounds__(256) void merge_kernel(const MergeDesc *__restrict_
       }
     }
   }
+}
+
+// Streaming form for small K (the boundary count of a road-graph part is ~10):
+// the product is a write stream of M x N int32, so one wave keeps its B panel
+// (KP x 256 columns, KP = K rounded up to a multiple of 4, FAP_INF padding) in
+// registers, reads two A rows at a time wave-uniformly (scalar loads) and writes
+// one contiguous 1 KB row segment per row.  Work unit: one 256-column block x
+// kStRows rows of one product; one launch per KP group.  Host requirements:
+// K <= KP and every A row holds KP readable entries (rows of the padded
+// Floyd-Warshall matrices, stride >= 64).  Entries A[i][k] with K <= k < KP are
+// distances in [0, FAP_INF] and meet B = FAP_INF, so they never win a minimum.
+constexpr int kStRows = 128;
+constexpr int kStCols = 256;
+
+constexpr int kStUnroll = 8;  // rows in flight per wave (distinct store registers)
+
+template <int KP>
+__device__ __forceinline__ void stream_row(int (&acc)[4], const FAP_CONST int *arow,
+                                           const int (&b)[KP][4]) {
+  // Scalar loads: the A values land in SGPRs, so they never contend with the
+  // VGPRs of stores still in flight.
+  int a[KP];
+#pragma unroll
+  for (int k = 0; k < KP; ++k) a[k] = arow[k];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) acc[e] = kInf;
+#pragma unroll
+  for (int k = 0; k < KP; k += 2)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[e] = min(acc[e], min(a[k] + b[k][e], a[k + 1] + b[k + 1][e]));
+}
+
+template <bool kVec>
+__device__ __forceinline__ void stream_store(FAP_GLOBAL int *crow, int j, int N,
+                                             const int (&acc)[4]) {
+  if (kVec && j + 3 < N) {
+    v4i v = {acc[0], acc[1], acc[2], acc[3]};
+    __builtin_nontemporal_store(v, reinterpret_cast<FAP_GLOBAL v4i *>(crow + j));
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      if (j + e < N) crow[j + e] = acc[e];
+  }
+}
+
+template <int KP, bool kOwn>
+__device__ __forceinline__ void stream_own(int (&acc)[4], const MergeDesc &d, int i,
+                                           const int (&own)[4]) {
+  if (kOwn) {
+    const FAP_GLOBAL int *f = gptr(d.fw + (int64_t)(d.fw_row0 + i) * d.ldf);
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      if (own[e] >= 0) acc[e] = f[own[e]];
+  }
+}
+
+template <int KP, bool kVec, bool kOwn>
+__device__ __forceinline__ void stream_rows(const MergeDesc &d, int row0, int row1, int j, int N,
+                                            const int (&b)[KP][4], const int (&own)[4]) {
+  FAP_GLOBAL int *crow = gptr(d.C + (int64_t)row0 * d.ldc);
+  const FAP_CONST int *arow = uniform_cptr(d.A + (int64_t)row0 * d.lda);
+  const int64_t ldc = d.ldc, lda = d.lda;
+  int i = row0;
+  for (; i + kStUnroll <= row1; i += kStUnroll) {
+    int acc[kStUnroll][4];
+#pragma unroll
+    for (int u = 0; u < kStUnroll; ++u) {
+      stream_row<KP>(acc[u], arow + u * lda, b);
+      stream_own<KP, kOwn>(acc[u], d, i + u, own);
+      stream_store<kVec>(crow + u * ldc, j, N, acc[u]);
+    }
+    crow += kStUnroll * ldc;
+    arow += kStUnroll * lda;
+  }
+  for (; i < row1; ++i, crow += ldc, arow += lda) {
+    int acc[4];
+    stream_row<KP>(acc, arow, b);
+    stream_own<KP, kOwn>(acc, d, i, own);
+    stream_store<kVec>(crow, j, N, acc);
+  }
+}
+
+template <int KP, bool kVec>
+__global__ __launch_bounds__(64) void merge_stream_kernel(const MergeDesc *__restrict__ descs,
+                                                          int nd, int total_units, int N,
+                                                          const int *__restrict__ graph_id,
+                                                          const int *__restrict__ ed2st) {
+  const int t = xcd_logical_tile();
+  if (t >= total_units) return;
+  const MergeDesc d = descs[find_desc(descs, nd, t)];
+  const int local = t - d.tile_begin;
+  const int cb = local / d.row_blocks;
+  const int rb = local - cb * d.row_blocks;
+  const int row0 = rb * kStRows;
+  const int row1 = min(row0 + kStRows, d.M);
+  const int j = cb * kStCols + threadIdx.x * 4;
+
+  int b[KP][4];
+#pragma unroll
+  for (int k = 0; k < KP; ++k) {
+    if (k < d.K) {
+      const FAP_GLOBAL int *brow = gptr(d.B + (int64_t)k * d.ldb);
+      if (kVec && j + 3 < N) {
+        const v4i v = *reinterpret_cast<const FAP_GLOBAL v4i *>(brow + j);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) b[k][e] = v[e];
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) b[k][e] = (j + e < N) ? brow[j + e] : kInf;
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) b[k][e] = kInf;
+    }
+  }
+  int own[4];
+  bool any_own = false;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    own[e] = -1;
+    if (d.part_id > 0 && j + e < N && graph_id[j + e] == d.part_id) {
+      own[e] = ed2st[j + e] - d.offset;
+      any_own = true;
+    }
+  }
+  // Retire the B-panel and column loads here, so the row loop never waits on them.
+  __builtin_amdgcn_s_waitcnt(0);
+  if (__builtin_amdgcn_ballot_w64(any_own) == 0)
+    stream_rows<KP, kVec, false>(d, row0, row1, j, N, b, own);
+  else
+    stream_rows<KP, kVec, true>(d, row0, row1, j, N, b, own);
+}
+
+int merge_stream_units(int M, int N) {
+  if (M <= 0 || N <= 0) return 0;
+  return ((M + kStRows - 1) / kStRows) * ((N + kStCols - 1) / kStCols);
+}
+int merge_stream_row_blocks(int M) { return (M + kStRows - 1) / kStRows; }
+int merge_stream_kpad(int K) { return K <= 0 ? 4 : (K + 3) / 4 * 4; }
+
+// kp: padded K, a multiple of 4 in [4, 32].
+int launch_merge_stream(const MergeDesc *d_descs, int nd, int total_units, int N, int kp,
+                        bool vec, const int *d_graph_id, const int *d_ed2st, hipStream_t stream) {
+  if (nd <= 0 || total_units <= 0 || N <= 0) return FAP_OK;
+  const int grid = (total_units + 7) / 8 * 8;
+#define FAP_STREAM_CASE(KP)                                                                   \
+  case KP:                                                                                    \
+    if (vec)                                                                                  \
+      hipLaunchKernelGGL((merge_stream_kernel<KP, true>), dim3(grid), dim3(64), 0, stream,    \
+                         d_descs, nd, total_units, N, d_graph_id, d_ed2st);                   \
+    else                                                                                      \
+      hipLaunchKernelGGL((merge_stream_kernel<KP, false>), dim3(grid), dim3(64), 0, stream,   \
+                         d_descs, nd, total_units, N, d_graph_id, d_ed2st);                   \
+    break;
+  switch (kp) {
+    FAP_STREAM_CASE(4)
+    FAP_STREAM_CASE(8)
+    FAP_STREAM_CASE(12)
+    FAP_STREAM_CASE(16)
+    FAP_STREAM_CASE(20)
+    FAP_STREAM_CASE(24)
+    FAP_STREAM_CASE(28)
+    FAP_STREAM_CASE(32)
+    default:
+      set_error("launch_merge_stream: unsupported padded K %d", kp);
+      return FAP_ERR_INVALID;
+  }
+#undef FAP_STREAM_CASE
+  FAP_CHECK_LAUNCH();
+  return FAP_OK;
 }
 
 int launch_merge(const MergeDesc *d_descs, int nd, int total_tiles, int N, bool vec,

@@ -27,9 +27,19 @@ int launch_sssp(const int *d_row_off, const int *d_col, const int *d_w, int n, c
                 int nsrc, const int64_t *d_rows, int *d_out, int64_t ld, bool vec_fill,
                 hipStream_t stream);
 int launch_fw(const FwDesc *d_descs, int ndesc, int max_nb, hipStream_t stream);
+int launch_ell_build(const int *d_row_off, const int *d_col, int n, int4 *d_ell,
+                     hipStream_t stream);
+int launch_bfs_lds(const int4 *d_ell, const int *d_row_off, const int *d_col, int n,
+                   const int *d_src, int nsrc, const int64_t *d_rows, int *d_out, int64_t ld,
+                   bool vec_fill, hipStream_t stream);
 int launch_merge(const MergeDesc *d_descs, int nd, int total_tiles, int N, bool vec,
                  const int *d_graph_id, const int *d_ed2st, hipStream_t stream);
 int merge_tiles(int M, int N);
+int merge_stream_units(int M, int N);
+int merge_stream_row_blocks(int M);
+int merge_stream_kpad(int K);
+int launch_merge_stream(const MergeDesc *d_descs, int nd, int total_units, int N, int kp,
+                        bool vec, const int *d_graph_id, const int *d_ed2st, hipStream_t stream);
 int launch_build(const BuildDesc *d_descs, int nd, int total_rows, bool vec, const int *row_off,
                  const int *col, const int *w, const int *graph_id, const int *st2ed,
                  const int *ed2st, hipStream_t stream);
@@ -41,11 +51,24 @@ int launch_gather_sources(const int *offsets, const int *src_begin, const int64_
 
 constexpr int kBuildMaxS = 8192;
 constexpr int kTile = 64;
+constexpr int kStreamMaxK = 32;  // larger boundary counts use the LDS-tiled product
+
+// Merge kernel of a product: 0 = LDS-tiled, else the streaming form with this
+// padded K (A rows of the padded FW matrices always hold >= 64 entries).
+int merge_kind(int K) { return K <= kStreamMaxK ? merge_stream_kpad(K) : 0; }
+
+struct MergeGroup {
+  int kind;   // 0 = tiled, else padded K of the streaming kernel
+  int first;  // first descriptor of the group
+  int count;  // descriptors in the group
+  int units;  // work units (tiles) of the group
+};
 
 namespace {
 
 Workspace g_ws_fw;    // Floyd-Warshall matrices
 Workspace g_ws_meta;  // descriptors, sources, row indices
+Workspace g_ws_ell;   // ELL adjacency of the unit-weight graph
 hipStream_t g_last_stream = nullptr;
 fap_stage_times g_times{};
 
@@ -135,7 +158,7 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
   std::vector<DecodeDesc> dd(P);
   std::vector<int> offsets(P), src_begin(P);
   std::vector<int64_t> out_rows(P);
-  int rows_total = 0, tiles_total = 0, src_total = 0;
+  int rows_total = 0, src_total = 0;
   int64_t dec_total = 0;
   for (int p = 0; p < P; ++p) {
     const fap_part_desc &q = act[p];
@@ -153,8 +176,8 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
     m.ldc = ld_out;
     m.M = M;
     m.K = q.bdy;
-    m.row_blocks = (M + 63) / 64;
-    m.tile_begin = batched ? tiles_total : 0;
+    m.row_blocks = 0;  // set when the merge groups are formed
+    m.tile_begin = 0;
     m.part_id = q.part_id;
     m.offset = q.offset;
     m.fw = fw;
@@ -167,12 +190,38 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
     src_begin[p] = batched ? src_total : 0;
     out_rows[p] = q.out_row;
     rows_total += S[p];
-    const int64_t tiles = merge_tiles(M, n);
-    FAP_REQUIRE((int64_t)tiles_total + tiles < (1LL << 30), "fap_hip_solve_parts: too many tiles");
-    tiles_total += (int)tiles;
     src_total += q.bdy;
     dec_total += (int64_t)q.bdy * q.num;
   }
+
+  // ---- merge groups: products with M > 0, ordered by kernel kind ------------
+  std::vector<MergeDesc> mdg;
+  std::vector<MergeGroup> groups;
+  std::vector<int> part_merge(P, -1);  // plain form: the group of part p
+  {
+    std::vector<int> order;
+    for (int p = 0; p < P; ++p)
+      if (md[p].M > 0) order.push_back(p);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return merge_kind(md[a].K) < merge_kind(md[b].K); });
+    for (int p : order) {
+      MergeDesc m = md[p];
+      const int kind = merge_kind(m.K);
+      const bool stream_kernel = kind > 0;
+      m.row_blocks = stream_kernel ? merge_stream_row_blocks(m.M) : (m.M + 63) / 64;
+      const int64_t units = stream_kernel ? merge_stream_units(m.M, n) : merge_tiles(m.M, n);
+      const bool new_group = !batched || groups.empty() || groups.back().kind != kind;
+      if (new_group) groups.push_back(MergeGroup{kind, (int)mdg.size(), 0, 0});
+      MergeGroup &gr = groups.back();
+      FAP_REQUIRE((int64_t)gr.units + units < (1LL << 30), "fap_hip_solve_parts: too many tiles");
+      m.tile_begin = gr.units;
+      gr.units += (int)units;
+      gr.count += 1;
+      part_merge[p] = (int)groups.size() - 1;
+      mdg.push_back(m);
+    }
+  }
+  const int nmd = (int)mdg.size();
 
   // ---- upload descriptors (one buffer) -------------------------------------
   int64_t off = 0;
@@ -182,7 +231,8 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
     return o;
   };
   const int64_t o_bd = carve(sizeof(BuildDesc) * P), o_fd = carve(sizeof(FwDesc) * P);
-  const int64_t o_md = carve(sizeof(MergeDesc) * P), o_dd = carve(sizeof(DecodeDesc) * P);
+  const int64_t o_md = carve(sizeof(MergeDesc) * std::max(nmd, 1));
+  const int64_t o_dd = carve(sizeof(DecodeDesc) * P);
   const int64_t o_off = carve(4LL * P), o_sb = carve(4LL * P), o_or = carve(8LL * P);
   const int64_t o_src = carve(4LL * std::max(src_total, 1));
   const int64_t o_rows = carve(8LL * std::max(src_total, 1));
@@ -191,7 +241,7 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
   std::vector<char> host(off);
   std::memcpy(host.data() + o_bd, bd.data(), sizeof(BuildDesc) * P);
   std::memcpy(host.data() + o_fd, fd.data(), sizeof(FwDesc) * P);
-  std::memcpy(host.data() + o_md, md.data(), sizeof(MergeDesc) * P);
+  if (nmd) std::memcpy(host.data() + o_md, mdg.data(), sizeof(MergeDesc) * nmd);
   std::memcpy(host.data() + o_dd, dd.data(), sizeof(DecodeDesc) * P);
   std::memcpy(host.data() + o_off, offsets.data(), 4LL * P);
   std::memcpy(host.data() + o_sb, src_begin.data(), 4LL * P);
@@ -219,19 +269,42 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
     return FAP_OK;
   };
 
+  // Boundary SSSP: the LDS-bitmap BFS for unit weights, else label correcting.
+  int4 *ell = nullptr;
+  if (!g->d_w) {
+    FAP_TRY(g_ws_ell.ensure((int64_t)n * sizeof(int4)));
+    ell = static_cast<int4 *>(g_ws_ell.ptr);
+    FAP_TRY(launch_ell_build(g->d_row_offset, g->d_col, n, ell, st));
+  }
+  auto run_sssp = [&](const int *src, int nsrc) -> int {
+    if (ell) {
+      const int s = launch_bfs_lds(ell, g->d_row_offset, g->d_col, n, src, nsrc, d_rows, d_out, 1,
+                                   vec, st);
+      if (s != FAP_ERR_INVALID) return s;
+    }
+    return launch_sssp(g->d_row_offset, g->d_col, g->d_w, n, src, nsrc, d_rows, d_out, 1, vec, st);
+  };
+
+  auto run_group = [&](const MergeGroup &gr) -> int {
+    if (gr.kind == 0)
+      return launch_merge(d_md + gr.first, gr.count, gr.units, n, vec, g->d_graph_id, g->d_ed2st,
+                          st);
+    return launch_merge_stream(d_md + gr.first, gr.count, gr.units, n, gr.kind, vec,
+                               g->d_graph_id, g->d_ed2st, st);
+  };
+
   if (batched) {
     FAP_TRY(mark());
     FAP_TRY(launch_gather_sources(d_offsets, d_sb, d_or, P, src_total, g->d_st2ed, ld_out, d_src,
                                   d_rows, st));
-    FAP_TRY(launch_sssp(g->d_row_offset, g->d_col, g->d_w, n, d_src, src_total, d_rows, d_out, 1,
-                        vec, st));
+    FAP_TRY(run_sssp(d_src, src_total));
     FAP_TRY(mark());
     FAP_TRY(launch_build(d_bd, P, rows_total, true, g->d_row_offset, g->d_col, g->d_w,
                          g->d_graph_id, g->d_st2ed, g->d_ed2st, st));
     FAP_TRY(mark());
     FAP_TRY(launch_fw(d_fd, P, max_nb, st));
     FAP_TRY(mark());
-    FAP_TRY(launch_merge(d_md, P, tiles_total, n, vec, g->d_graph_id, g->d_ed2st, st));
+    for (const MergeGroup &gr : groups) FAP_TRY(run_group(gr));
     FAP_TRY(mark());
     FAP_TRY(launch_decode(d_dd, P, dec_total, g->d_st2ed, st));
     FAP_TRY(mark());
@@ -241,16 +314,14 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
       FAP_TRY(mark());
       FAP_TRY(launch_gather_sources(d_offsets + p, d_sb + p, d_or + p, 1, q.bdy, g->d_st2ed, ld_out,
                                     d_src, d_rows, st));
-      FAP_TRY(launch_sssp(g->d_row_offset, g->d_col, g->d_w, n, d_src, q.bdy, d_rows, d_out, 1,
-                          vec, st));
+      FAP_TRY(run_sssp(d_src, q.bdy));
       FAP_TRY(mark());
       FAP_TRY(launch_build(d_bd + p, 1, S[p], true, g->d_row_offset, g->d_col, g->d_w,
                            g->d_graph_id, g->d_st2ed, g->d_ed2st, st));
       FAP_TRY(mark());
       FAP_TRY(launch_fw(d_fd + p, 1, nb[p], st));
       FAP_TRY(mark());
-      FAP_TRY(launch_merge(d_md + p, 1, merge_tiles(q.num - q.bdy, n), n, vec, g->d_graph_id,
-                           g->d_ed2st, st));
+      if (part_merge[p] >= 0) FAP_TRY(run_group(groups[part_merge[p]]));
       FAP_TRY(mark());
       FAP_TRY(launch_decode(d_dd + p, 1, (int64_t)q.bdy * q.num, g->d_st2ed, st));
       FAP_TRY(mark());

@@ -181,6 +181,174 @@ __global__ __launch_bounds__(64 * kSsspWaves) void sssp_rows_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Unit-weight BFS with the visited set in LDS (n <= kBfsLdsMaxN).
+//
+// Per level the only global round trip is one 16-byte ELL load per frontier
+// vertex (4 neighbour slots; a vertex of degree > 4 stores -2 in slot 3 and its
+// remaining arcs are read from the CSR).  Claims are ds_or on the LDS bitmap; the
+// label is written once with a plain store.  Each lane takes up to 4 frontier
+// vertices per pass and issues all their loads before using any.
+constexpr int kEllW = 4;
+constexpr int kBfsQ = 1024;
+constexpr int kBfsU = 4;
+constexpr int kBfsLdsMaxN = 1 << 20;
+
+__global__ void ell_build_kernel(const int *__restrict__ row_off, const int *__restrict__ col, int n,
+                                 int4 *__restrict__ ell) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int e0 = row_off[v], deg = row_off[v + 1] - e0;
+  int s[kEllW];
+#pragma unroll
+  for (int t = 0; t < kEllW; ++t) s[t] = t < deg ? col[e0 + t] : -1;
+  if (deg > kEllW) s[kEllW - 1] = -2;
+  ell[v] = make_int4(s[0], s[1], s[2], s[3]);
+}
+
+__device__ __forceinline__ void bfs_claim(unsigned *vis, int *row, int v, int d, int *q_next,
+                                          int *q_cnt) {
+  const unsigned bit = 1u << (v & 31);
+  if (vis[v >> 5] & bit) return;
+  if (atomicOr(&vis[v >> 5], bit) & bit) return;
+  row[v] = d;
+  const int pos = atomicAdd(q_cnt, 1);
+  if (pos < kBfsQ) q_next[pos] = v;
+}
+
+template <bool kVecFill>
+__global__ __launch_bounds__(64) void bfs_lds_kernel(const int4 *__restrict__ ell,
+                                                     const int *__restrict__ row_off,
+                                                     const int *__restrict__ col, int n,
+                                                     const int *__restrict__ src, int nsrc,
+                                                     const int64_t *__restrict__ rows,
+                                                     int *__restrict__ out, int64_t ld) {
+  extern __shared__ int smem[];
+  const int nwords = (n + 31) >> 5;
+  unsigned *vis = reinterpret_cast<unsigned *>(smem);
+  int *queue = smem + ((nwords + 3) & ~3);  // [2][kBfsQ]
+  int *qcount = queue + 2 * kBfsQ;          // [2]
+  const int lane = threadIdx.x;
+  const int sidx = blockIdx.x;
+  if (sidx >= nsrc) return;
+  const int s = src[sidx];
+  int *row = out + (rows ? rows[sidx] : (int64_t)sidx * ld);
+
+  fill_row<kVecFill>(row, n, s, lane);
+  // The fill must be complete before any label store to the same row.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  for (int w = lane; w < nwords; w += 64) vis[w] = 0;
+  wave_sync();
+  if (lane == 0) {
+    vis[s >> 5] = 1u << (s & 31);
+    queue[0] = s;
+    qcount[0] = 1;
+    qcount[1] = 0;
+  }
+  wave_sync();
+
+  int cur = 0, qsize = 1, level = 0;
+  bool scan = false;
+  while (true) {
+    const int nxt = cur ^ 1;
+    int *q_next = queue + nxt * kBfsQ;
+    int *q_cnt = &qcount[nxt];
+    const int d = level + 1;
+    if (!scan) {
+      const int *q_cur = queue + cur * kBfsQ;
+      for (int base = 0; base < qsize; base += 64 * kBfsU) {
+        int u[kBfsU];
+        int4 nb[kBfsU];
+#pragma unroll
+        for (int t = 0; t < kBfsU; ++t) {
+          const int idx = base + t * 64 + lane;
+          u[t] = idx < qsize ? q_cur[idx] : -1;
+        }
+#pragma unroll
+        for (int t = 0; t < kBfsU; ++t) nb[t] = u[t] >= 0 ? ell[u[t]] : make_int4(-1, -1, -1, -1);
+#pragma unroll
+        for (int t = 0; t < kBfsU; ++t) {
+          if (nb[t].x >= 0) bfs_claim(vis, row, nb[t].x, d, q_next, q_cnt);
+          if (nb[t].y >= 0) bfs_claim(vis, row, nb[t].y, d, q_next, q_cnt);
+          if (nb[t].z >= 0) bfs_claim(vis, row, nb[t].z, d, q_next, q_cnt);
+          if (nb[t].w >= 0) {
+            bfs_claim(vis, row, nb[t].w, d, q_next, q_cnt);
+          } else if (nb[t].w == -2) {
+            const int e1 = row_off[u[t] + 1];
+            for (int e = row_off[u[t]] + kEllW - 1; e < e1; ++e)
+              bfs_claim(vis, row, col[e], d, q_next, q_cnt);
+          }
+        }
+      }
+    } else {
+      // The frontier did not fit in LDS: find it in the row.  The release fence
+      // waits for this wave's label stores, the agent-scope loads read them at L2.
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      for (int v = lane; v < n; v += 64) {
+        if (load_agent(row + v) != level) continue;
+        const int4 q = ell[v];
+        if (q.x >= 0) bfs_claim(vis, row, q.x, d, q_next, q_cnt);
+        if (q.y >= 0) bfs_claim(vis, row, q.y, d, q_next, q_cnt);
+        if (q.z >= 0) bfs_claim(vis, row, q.z, d, q_next, q_cnt);
+        if (q.w >= 0) {
+          bfs_claim(vis, row, q.w, d, q_next, q_cnt);
+        } else if (q.w == -2) {
+          const int e1 = row_off[v + 1];
+          for (int e = row_off[v] + kEllW - 1; e < e1; ++e)
+            bfs_claim(vis, row, col[e], d, q_next, q_cnt);
+        }
+      }
+    }
+    wave_sync();
+    const int cnt = __hip_atomic_load(q_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    wave_sync();
+    if (cnt == 0) break;
+    scan = cnt > kBfsQ;
+    qsize = scan ? 0 : cnt;
+    if (lane == 0) qcount[cur] = 0;
+    wave_sync();
+    cur = nxt;
+    ++level;
+  }
+}
+
+int bfs_lds_bytes(int n) { return 4 * (((n + 31) / 32 + 3) / 4 * 4 + 2 * kBfsQ + 4); }
+
+int launch_ell_build(const int *d_row_off, const int *d_col, int n, int4 *d_ell,
+                     hipStream_t stream) {
+  if (n <= 0) return FAP_OK;
+  hipLaunchKernelGGL(ell_build_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, d_row_off,
+                     d_col, n, d_ell);
+  FAP_CHECK_LAUNCH();
+  return FAP_OK;
+}
+
+// Unit weights with an ELL adjacency (built by launch_ell_build) and n within
+// the LDS bitmap limit; returns FAP_ERR_INVALID when the LDS path does not apply.
+int launch_bfs_lds(const int4 *d_ell, const int *d_row_off, const int *d_col, int n,
+                   const int *d_src, int nsrc, const int64_t *d_rows, int *d_out, int64_t ld,
+                   bool vec_fill, hipStream_t stream) {
+  if (nsrc <= 0) return FAP_OK;
+  if (n > kBfsLdsMaxN) return FAP_ERR_INVALID;
+  const int lds = bfs_lds_bytes(n);
+  static bool attr_set = false;
+  if (!attr_set) {
+    FAP_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&bfs_lds_kernel<true>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    FAP_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&bfs_lds_kernel<false>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr_set = true;
+  }
+  if (vec_fill)
+    hipLaunchKernelGGL(bfs_lds_kernel<true>, dim3(nsrc), dim3(64), lds, stream, d_ell, d_row_off,
+                       d_col, n, d_src, nsrc, d_rows, d_out, ld);
+  else
+    hipLaunchKernelGGL(bfs_lds_kernel<false>, dim3(nsrc), dim3(64), lds, stream, d_ell, d_row_off,
+                       d_col, n, d_src, nsrc, d_rows, d_out, ld);
+  FAP_CHECK_LAUNCH();
+  return FAP_OK;
+}
+
 int launch_sssp(const int *d_row_off, const int *d_col, const int *d_w, int n, const int *d_src,
                 int nsrc, const int64_t *d_rows, int *d_out, int64_t ld, bool vec_fill,
                 hipStream_t stream) {
@@ -224,6 +392,17 @@ extern "C" int fap_hip_sssp_batch(const int32_t *d_row_offset, const int32_t *d_
   FAP_REQUIRE(d_row_offset && d_src && d_out && (m == 0 || d_col),
               "fap_hip_sssp_batch: null pointer");
   const bool vec = (ld_out % 4 == 0) && ((uintptr_t)d_out % 16 == 0);
-  return launch_sssp(d_row_offset, d_col, d_w, n, d_src, nsrc, nullptr, d_out, ld_out, vec,
-                     resolve(stream));
+  hipStream_t st = resolve(stream);
+  if (!d_w && n <= kBfsLdsMaxN) {
+    static Workspace ws_ell;
+    FAP_TRY(ws_ell.ensure((int64_t)n * sizeof(int4)));
+    int4 *ell = static_cast<int4 *>(ws_ell.ptr);
+    FAP_TRY(launch_ell_build(d_row_offset, d_col, n, ell, st));
+    FAP_TRY(launch_bfs_lds(ell, d_row_offset, d_col, n, d_src, nsrc, nullptr, d_out, ld_out, vec,
+                           st));
+    // The ELL workspace is shared between calls: finish before it can be reused.
+    FAP_CHECK_HIP(hipStreamSynchronize(st));
+    return FAP_OK;
+  }
+  return launch_sssp(d_row_offset, d_col, d_w, n, d_src, nsrc, nullptr, d_out, ld_out, vec, st);
 }

@@ -60,6 +60,22 @@ int fap_graph_create_from_file(fap_graph **out, const char *file, int directed, 
   });
 }
 
+int fap_mtx_read(const char *file, int directed, int weighted, int32_t *n, int64_t *m,
+                 int32_t *row_offset, int32_t *col, float *weight) {
+  if (!file || !n || !m) {
+    fap_hip::set_error("fap_mtx_read: null argument");
+    return FAP_ERR_INVALID;
+  }
+  return guarded([&] {
+    fap::Csr c = fap::read_mtx(file, directed != 0, weighted != 0);
+    *n = c.n;
+    *m = c.m();
+    if (row_offset) memcpy(row_offset, c.row_offset.data(), 4 * c.row_offset.size());
+    if (col) memcpy(col, c.col_val.data(), 4 * c.col_val.size());
+    if (weight) memcpy(weight, c.weight.data(), 4 * c.weight.size());
+  });
+}
+
 int fap_graph_create_from_csr(fap_graph **out, int32_t n, int64_t m, const int32_t *row_offset,
                               const int32_t *col_val, const float *weight, int directed,
                               int32_t K) {
@@ -120,11 +136,51 @@ int fap_graph_destroy(fap_graph *h) {
   });
 }
 
+int fap_layout_for_partition(int32_t n, const int32_t *row_offset, const int32_t *col, int32_t K,
+                             const int32_t *graph_id, int32_t *st2ed, int32_t *ed2st,
+                             int32_t *blk_num, int32_t *blk_off, int32_t *bdy_num) {
+  if (n < 1 || K < 1 || !row_offset || !graph_id || !st2ed || !ed2st || !blk_num || !blk_off ||
+      !bdy_num || (row_offset[n] > 0 && !col)) {
+    fap_hip::set_error("fap_layout_for_partition: bad argument");
+    return FAP_ERR_INVALID;
+  }
+  return guarded([&] {
+    fap::Csr g;
+    g.n = n;
+    g.row_offset.assign(row_offset, row_offset + n + 1);
+    g.col_val.assign(col, col + row_offset[n]);
+    std::vector<int> gid(graph_id, graph_id + n);
+    for (int v : gid)
+      if (v < 1 || v > K) throw fap::FapError(FAP_ERR_INVALID, "part id out of [1, K]");
+    std::unordered_map<int, std::vector<int>> bv, bb;
+    for (int i = 0; i < n; ++i) bv[gid[i]].push_back(i);
+    std::vector<int> isb, num, off, bdy, boff, s2e, e2s;
+    fap::find_boundary(K, g, gid, isb, bb);
+    fap::sort_and_encode(K, n, gid, isb, bv, bb, num, off, bdy, boff, s2e, e2s);
+    memcpy(st2ed, s2e.data(), 4 * (size_t)n);
+    memcpy(ed2st, e2s.data(), 4 * (size_t)n);
+    memcpy(blk_num, num.data(), 4 * (size_t)(K + 1));
+    memcpy(blk_off, off.data(), 4 * (size_t)(K + 1));
+    memcpy(bdy_num, bdy.data(), 4 * (size_t)(K + 1));
+  });
+}
+
 #define FAP_HANDLE(h)                                          \
   if (!(h) || !(h)->g) {                                       \
     fap_hip::set_error("%s: null graph handle", __func__);     \
     return FAP_ERR_INVALID;                                    \
   }
+
+int fap_graph_set_partition(fap_graph *h, const int32_t *graph_id) {
+  FAP_HANDLE(h);
+  if (!graph_id) {
+    fap_hip::set_error("fap_graph_set_partition: null partition");
+    return FAP_ERR_INVALID;
+  }
+  return guarded([&] {
+    h->g->setPartition(std::vector<int>(graph_id, graph_id + h->g->num_vertexs));
+  });
+}
 
 int fap_graph_precondition(fap_graph *h) {
   FAP_HANDLE(h);

@@ -115,6 +115,23 @@ void fapGraph::release() {
   d_row_offset = d_col = d_w = d_graph_id = d_st2ed = d_ed2st = d_dist = d_sub_dist = nullptr;
 }
 
+void fapGraph::setPartition(const std::vector<int> &ids) {
+  if ((int64_t)ids.size() != num_vertexs)
+    throw FapError(FAP_ERR_INVALID, "partition size differs from n");
+  for (int v : ids)
+    if (v < 1 || v > K) throw FapError(FAP_ERR_INVALID, "part id out of [1, K]");
+  graph_id = ids;
+  BlockVer.clear();
+  for (int i = 0; i < num_vertexs; ++i) BlockVer[graph_id[i]].push_back(i);
+  edge_cut = 0;
+  for (int u = 0; u < num_vertexs; ++u)
+    for (int e = row_offset[u]; e < row_offset[u + 1]; ++e)
+      if (graph_id[col_val[e]] != graph_id[u]) edge_cut += 1;
+  if (!directed) edge_cut /= 2;
+  preconditioned = false;
+  release();
+}
+
 void fapGraph::preCondition() {
   Csr g;
   g.n = num_vertexs;

@@ -156,7 +156,7 @@ void fm_refine(const WGraph &g, std::vector<char> &side, const double tw[2], dou
       locked[v] = 0;
       if (ext > 0) heap[(int)side[v]].push({gain[v], v});
     }
-    auto balanced = [&](const int64_t *ww) { return ww[0] <= maxw[0] && ww[1] <= maxw[1]; };
+
     auto excess = [&](const int64_t *ww) {
       return std::max(0.0, ww[0] - maxw[0]) + std::max(0.0, ww[1] - maxw[1]);
     };
@@ -175,6 +175,12 @@ void fm_refine(const WGraph &g, std::vector<char> &side, const double tw[2], dou
                                     heap[s].top().first != gain[heap[s].top().second] ||
                                     side[heap[s].top().second] != s))
           heap[s].pop();
+      // An overweight side without boundary vertices (its part of the graph is
+      // disconnected from the other side) gives up interior vertices instead.
+      for (int s = 0; s < 2; ++s)
+        if (w[s] > maxw[s] && heap[s].empty())
+          for (int v = 0; v < n; ++v)
+            if (side[v] == s && !locked[v]) heap[s].push({gain[v], v});
       if (w[0] > maxw[0] && !heap[0].empty()) from = 0;
       else if (w[1] > maxw[1] && !heap[1].empty()) from = 1;
       else {
@@ -213,7 +219,7 @@ void fm_refine(const WGraph &g, std::vector<char> &side, const double tw[2], dou
       } else {
         ++since_best;
       }
-      (void)balanced;
+
     }
     for (int i = (int)moved.size() - 1; i >= best_len; --i) side[moved[i]] ^= 1;
     if (best_len == 0) break;
@@ -341,6 +347,114 @@ void recurse(const WGraph &g, const std::vector<int> &orig, int k, int first,
   }
 }
 
+// Greedy k-way balancing and refinement on the original graph (unit vertex
+// weights): boundary vertices of parts above max_w move to the adjacent part that
+// loses the fewest cut edges and has room; then positive-gain moves that keep
+// every part within [1, max_w].  Each candidate is re-validated when popped.
+void kway_balance_refine(const WGraph &g, std::vector<int> &part, int K, double eps) {
+  const int n = g.n;
+  std::vector<int> pw(K + 1, 0);
+  for (int v = 0; v < n; ++v) ++pw[part[v]];
+  const int max_w = std::max((int)std::ceil((double)n / K), (int)std::floor((double)n / K * (1 + eps)));
+  std::vector<int> conn(K + 1, 0);
+  std::vector<int> touched;
+  // Best destination of v (or -1) and its gain, given the balance rule.
+  auto best_move = [&](int v, bool balancing, int *gain_out) {
+    const int p = part[v];
+    touched.clear();
+    int own = 0;
+    for (int e = g.xadj[v]; e < g.xadj[v + 1]; ++e) {
+      const int q = part[g.adj[e]];
+      if (q == p) {
+        own += g.ew[e];
+        continue;
+      }
+      if (conn[q] == 0) touched.push_back(q);
+      conn[q] += g.ew[e];
+    }
+    // Balancing prefers a destination with room; otherwise any strictly lighter
+    // neighbour part, so weight diffuses away from overfull parts over the passes.
+    int best = -1, bg = INT32_MIN;
+    bool best_fits = false;
+    for (int q : touched) {
+      const int gain = conn[q] - own;
+      const bool fits = pw[q] + 1 <= max_w;
+      bool ok;
+      if (balancing) ok = fits || pw[q] + 1 < pw[p];
+      else ok = fits && pw[p] > 1 && (gain > 0 || (gain == 0 && pw[q] + 1 < pw[p]));
+      if (!ok) continue;
+      if (balancing && fits != best_fits ? fits : gain > bg) {
+        bg = gain;
+        best = q;
+        best_fits = fits;
+      }
+    }
+    for (int q : touched) conn[q] = 0;
+    *gain_out = bg;
+    return best;
+  };
+  typedef std::pair<int, int> GV;
+  for (int pass = 0; pass < 24; ++pass) {
+    bool over = false;
+    for (int q = 1; q <= K; ++q) over |= pw[q] > max_w;
+    int moves = 0;
+    if (over) {
+      std::priority_queue<GV> heap;
+      for (int v = 0; v < n; ++v) {
+        if (pw[part[v]] <= max_w) continue;
+        int gain;
+        if (best_move(v, true, &gain) >= 0) heap.push({gain, v});
+      }
+      while (!heap.empty()) {
+        const int v = heap.top().second;
+        heap.pop();
+        if (pw[part[v]] <= max_w) continue;
+        int gain;
+        const int q = best_move(v, true, &gain);
+        if (q < 0) continue;
+        if (!heap.empty() && gain < heap.top().first) {  // stale: re-queue with its gain now
+          heap.push({gain, v});
+          continue;
+        }
+        --pw[part[v]];
+        ++pw[q];
+        part[v] = q;
+        ++moves;
+        for (int e = g.xadj[v]; e < g.xadj[v + 1]; ++e) {
+          const int u = g.adj[e];
+          if (pw[part[u]] > max_w && best_move(u, true, &gain) >= 0) heap.push({gain, u});
+        }
+      }
+    }
+    // Refinement: positive-gain moves within the balance bound.
+    std::priority_queue<GV> heap;
+    for (int v = 0; v < n; ++v) {
+      int gain;
+      if (best_move(v, false, &gain) >= 0) heap.push({gain, v});
+    }
+    while (!heap.empty()) {
+      const int v = heap.top().second;
+      heap.pop();
+      int gain;
+      const int q = best_move(v, false, &gain);
+      if (q < 0) continue;
+      if (!heap.empty() && gain < heap.top().first) {
+        heap.push({gain, v});
+        continue;
+      }
+      --pw[part[v]];
+      ++pw[q];
+      part[v] = q;
+      ++moves;
+      for (int e = g.xadj[v]; e < g.xadj[v + 1]; ++e) {
+        const int u = g.adj[e];
+        if (best_move(u, false, &gain) >= 0) heap.push({gain, u});
+      }
+    }
+    if (moves == 0) break;
+  }
+}
+
 }  // namespace
 
 std::vector<int> partition_kway(const Csr &g, int K, uint64_t seed, double *edge_cut) {
@@ -352,8 +466,8 @@ std::vector<int> partition_kway(const Csr &g, int K, uint64_t seed, double *edge
   std::iota(orig.begin(), orig.end(), 0);
   std::vector<int> local(g.n, -1);
   recurse(w, orig, K, 1, graph_id, local, rng);
+  if (K > 1) kway_balance_refine(w, graph_id, K, 0.03);
   if (edge_cut) {
-    std::vector<char> dummy;
     int64_t cut = 0;
     for (int v = 0; v < w.n; ++v)
       for (int e = w.xadj[v]; e < w.xadj[v + 1]; ++e)

@@ -91,6 +91,13 @@ class FapGraph:
             pass
 
     # ---- the reference API ------------------------------------------------
+    def setPartition(self, graph_id) -> None:
+        """Use a caller's 1-based partition (e.g. METIS output) before preCondition()."""
+        gid = np.ascontiguousarray(graph_id, dtype=np.int32)
+        if gid.shape != (self.num_vertexs,):
+            raise ValueError("partition must have one id per vertex")
+        L.check(L.load().fap_graph_set_partition(self._h, L.ptr_i32(gid)), "setPartition")
+
     def preCondition(self) -> None:
         L.check(L.load().fap_graph_precondition(self._h), "preCondition")
 

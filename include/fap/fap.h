@@ -109,6 +109,8 @@ class fapGraph {
   fapGraph(const fapGraph &) = delete;
   fapGraph &operator=(const fapGraph &) = delete;
 
+  // Use a caller's partition (1-based ids, size n) instead of the built-in one.
+  void setPartition(const std::vector<int> &graph_id);
   void preCondition();
   bool isSplit();
   std::vector<int> getGraphId();

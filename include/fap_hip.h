@@ -176,6 +176,10 @@ typedef struct fap_graph fap_graph;
  * ".mtx" opened as given. partitioner: "metis" (default when NULL or ""). */
 int fap_graph_create_from_file(fap_graph **g, const char *file, int directed, int weighted,
                                int32_t K, const char *partitioner);
+/* The .mtx reader alone (readVerEdges + readMatFile, fap/utils/readMatFile.h:36-173):
+ * call with NULL arrays to get n and m, then with row_offset[n+1], col[m], weight[m]. */
+int fap_mtx_read(const char *file, int directed, int weighted, int32_t *n, int64_t *m,
+                 int32_t *row_offset, int32_t *col, float *weight);
 /* From a CSR in the reference's layout (row_offset[n+1], col_val[m], weight[m]
  * floats, weight may be NULL = unit).  fap/fap.h:102-104. */
 int fap_graph_create_from_csr(fap_graph **g, int32_t n, int64_t m, const int32_t *row_offset,
@@ -188,6 +192,15 @@ int fap_graph_create_from_csr(fap_graph **g, int32_t n, int64_t m, const int32_t
 int fap_graph_generate(fap_graph **g, int32_t kind, int32_t W, int32_t H, int32_t n_target,
                        double p_a, double p_b, uint64_t seed, int32_t K);
 int fap_graph_destroy(fap_graph *g);
+/* Replace the partition with a caller's (1-based ids, e.g. METIS output) before
+ * preCondition; with the reference's partition the row layout is the reference's.
+ * Replaces the readIdFile_METIS result of fap/fap.cpp:87-89. */
+int fap_graph_set_partition(fap_graph *g, const int32_t *graph_id);
+/* findBoundry + sort_and_encode (fap/utils/util_preprocess.h:32-106) for a given
+ * partition: st2ed[n], ed2st[n], per-part num / offset / boundary count [K+1]. */
+int fap_layout_for_partition(int32_t n, const int32_t *row_offset, const int32_t *col, int32_t K,
+                             const int32_t *graph_id, int32_t *st2ed, int32_t *ed2st,
+                             int32_t *blk_num, int32_t *blk_off, int32_t *bdy_num);
 int fap_graph_precondition(fap_graph *g);
 int fap_graph_is_split(fap_graph *g, int *is_split);
 int fap_graph_set_mode(fap_graph *g, int32_t mode);
