@@ -27,6 +27,7 @@ FAP_ERR_NODEV = 7
 
 FAP_MODE_PLAIN = 0
 FAP_MODE_BATCHED = 1
+FAP_MODE_STAGED = 2
 
 # Every exported symbol of include/fap_hip.h: name -> (restype, argtypes).
 _i32, _i64, _u64, _f64 = C.c_int32, C.c_int64, C.c_uint64, C.c_double
@@ -65,6 +66,7 @@ SIGNATURES = {
     "fap_hip_memcpy_d2d": (C.c_int, [_vp, _vp, _i64, _vp]),
     "fap_hip_memcpy2d_d2h_i32": (C.c_int, [_vp, _i64, _vp, _i64, _i64, _i64, _vp]),
     "fap_hip_fill_i32": (C.c_int, [_vp, _i32, _i64, _vp]),
+    "fap_hip_copy_i32": (C.c_int, [_vp, _vp, _i64, _vp]),
     "fap_hip_stream_sync": (C.c_int, [_vp]),
     "fap_hip_device_sync": (C.c_int, []),
     "fap_hip_event_create": (C.c_int, [C.POINTER(_vp)]),

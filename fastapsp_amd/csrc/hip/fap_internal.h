@@ -17,6 +17,8 @@ void clear_error();
 
 // Library stream of the current device (created by fap_hip_init).
 hipStream_t lib_stream();
+// Internal streams (i = 0, 1) used to overlap the stages of different parts.
+hipStream_t aux_stream(int i);
 inline hipStream_t resolve(void *stream) {
   return stream ? static_cast<hipStream_t>(stream) : lib_stream();
 }

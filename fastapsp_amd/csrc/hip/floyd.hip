@@ -17,69 +17,37 @@
 // Integer distances make the result identical to the k-i-j loop.
 #include "fap_internal.h"
 #include "fw_desc.h"
+#include "gmem.h"
 #include "mp_tile.h"
 
 namespace fap_hip {
 
 constexpr int kT = 64;  // tile edge
 
-// Thread (tx, ty) of a 256-thread workgroup owns rows ty*4 .. ty*4+3 and
-// columns tx*4 .. tx*4+3 of a 64 x 64 tile.
-__device__ __forceinline__ int *tile_ptr(const FwDesc &d, int ti, int tj) {
-  return d.mat + (int64_t)(ti * kT) * d.ld + tj * kT;
-}
+// Descriptor pointers are generic to the compiler; tiles live in global memory.
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef FAP_GLOBAL int gint;
+typedef FAP_GLOBAL v4i gv4i;
 
-// Global row-major 64x64 tile -> LDS k-major (transposed) panel As[k][i].
-__device__ __forceinline__ void load_tile_transposed(int (*As)[kT], const int *g, int64_t ld) {
-  const int i = threadIdx.x & 63;
-  const int kq = threadIdx.x >> 6;  // 0..3 -> columns kq*16 .. kq*16+15
-  const int4 *src = reinterpret_cast<const int4 *>(g + (int64_t)i * ld + kq * 16);
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int4 t = src[c];
-    const int k = kq * 16 + c * 4;
-    As[k + 0][i] = t.x;
-    As[k + 1][i] = t.y;
-    As[k + 2][i] = t.z;
-    As[k + 3][i] = t.w;
-  }
+__device__ __forceinline__ gint *tile_ptr(const FwDesc &d, int ti, int tj) {
+  return gptr(d.mat + (int64_t)(ti * kT) * d.ld + tj * kT);
 }
 
 // Global row-major 64x64 tile -> LDS row-major panel Bs[k][j].
-__device__ __forceinline__ void load_tile(int (*Bs)[kT], const int *g, int64_t ld) {
+__device__ __forceinline__ void load_tile(int (*Bs)[kT], const gint *g, int64_t ld) {
   const int k = threadIdx.x >> 2;
   const int jq = threadIdx.x & 3;  // columns jq*16 .. jq*16+15
-  const int4 *src = reinterpret_cast<const int4 *>(g + (int64_t)k * ld + jq * 16);
-  int4 *dst = reinterpret_cast<int4 *>(&Bs[k][jq * 16]);
+  const gv4i *src = reinterpret_cast<const gv4i *>(g + (int64_t)k * ld + jq * 16);
+  v4i *dst = reinterpret_cast<v4i *>(&Bs[k][jq * 16]);
 #pragma unroll
   for (int c = 0; c < 4; ++c) dst[c] = src[c];
-}
-
-__device__ __forceinline__ void load_acc(int (&acc)[4][4], const int *g, int64_t ld, int tx,
-                                         int ty) {
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int4 t = *reinterpret_cast<const int4 *>(g + (int64_t)(ty * 4 + r) * ld + tx * 4);
-    acc[r][0] = t.x;
-    acc[r][1] = t.y;
-    acc[r][2] = t.z;
-    acc[r][3] = t.w;
-  }
-}
-
-__device__ __forceinline__ void store_acc(const int (&acc)[4][4], int *g, int64_t ld, int tx,
-                                          int ty) {
-#pragma unroll
-  for (int r = 0; r < 4; ++r)
-    *reinterpret_cast<int4 *>(g + (int64_t)(ty * 4 + r) * ld + tx * 4) =
-        make_int4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
 }
 
 __global__ __launch_bounds__(256) void fw_phase1_kernel(const FwDesc *__restrict__ descs, int r) {
   const FwDesc d = descs[blockIdx.y];
   if (r >= d.nb) return;
   __shared__ int D[kT][kT];
-  int *g = tile_ptr(d, r, r);
+  gint *g = tile_ptr(d, r, r);
   load_tile(D, g, d.ld);
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   for (int k = 0; k < kT; ++k) {
@@ -101,13 +69,83 @@ __global__ __launch_bounds__(256) void fw_phase1_kernel(const FwDesc *__restrict
   }
   __syncthreads();
   const int k = threadIdx.x >> 2, jq = threadIdx.x & 3;
-  int4 *dst = reinterpret_cast<int4 *>(g + (int64_t)k * d.ld + jq * 16);
-  const int4 *src = reinterpret_cast<const int4 *>(&D[k][jq * 16]);
+  gv4i *dst = reinterpret_cast<gv4i *>(g + (int64_t)k * d.ld + jq * 16);
+  const v4i *src = reinterpret_cast<const v4i *>(&D[k][jq * 16]);
 #pragma unroll
   for (int c = 0; c < 4; ++c) dst[c] = src[c];
 }
 
-__global__ __launch_bounds__(256) void fw_phase2_kernel(const FwDesc *__restrict__ descs, int r) {
+// Phases 2 and 3: one wave64 per 64x64 tile product, 8 x 8 accumulators per lane
+// (rows tr*4 + {0..3} and 32 + tr*4 + {0..3}, columns likewise with tc), k in two
+// LDS chunks of 32.  With 16 KB of LDS per workgroup a CU holds ten such waves.
+constexpr int kKc = 32;
+
+// A tile rows 0..63, columns k0 .. k0+31 -> As[k][i] (transposed): lane = row.
+__device__ __forceinline__ void load_a_chunk(int (*As)[kT], const gint *g, int64_t ld, int k0) {
+  const int i = threadIdx.x;
+  const gv4i *src = reinterpret_cast<const gv4i *>(g + (int64_t)i * ld + k0);
+#pragma unroll
+  for (int c = 0; c < kKc / 4; ++c) {
+    const v4i t = src[c];
+    As[4 * c + 0][i] = t[0];
+    As[4 * c + 1][i] = t[1];
+    As[4 * c + 2][i] = t[2];
+    As[4 * c + 3][i] = t[3];
+  }
+}
+
+// B tile rows k0 .. k0+31, columns 0..63 -> Bs[k][j].
+__device__ __forceinline__ void load_b_chunk(int (*Bs)[kT], const gint *g, int64_t ld, int k0) {
+  const int k = threadIdx.x >> 1;
+  const int half = threadIdx.x & 1;
+  const gv4i *src = reinterpret_cast<const gv4i *>(g + (int64_t)(k0 + k) * ld + half * 32);
+  v4i *dst = reinterpret_cast<v4i *>(&Bs[k][half * 32]);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) dst[c] = src[c];
+}
+
+__device__ __forceinline__ int own_row(int tr, int q) { return (q < 4 ? 0 : 32) + tr * 4 + (q & 3); }
+
+__device__ __forceinline__ void load_acc8(int (&acc)[8][8], const gint *g, int64_t ld, int tr,
+                                          int tc) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const gint *r = g + (int64_t)own_row(tr, q) * ld;
+    const v4i a = *reinterpret_cast<const gv4i *>(r + tc * 4);
+    const v4i b = *reinterpret_cast<const gv4i *>(r + 32 + tc * 4);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      acc[q][e] = a[e];
+      acc[q][4 + e] = b[e];
+    }
+  }
+}
+
+__device__ __forceinline__ void store_acc8(const int (&acc)[8][8], gint *g, int64_t ld, int tr,
+                                           int tc) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    gint *r = g + (int64_t)own_row(tr, q) * ld;
+    *reinterpret_cast<gv4i *>(r + tc * 4) = v4i{acc[q][0], acc[q][1], acc[q][2], acc[q][3]};
+    *reinterpret_cast<gv4i *>(r + 32 + tc * 4) = v4i{acc[q][4], acc[q][5], acc[q][6], acc[q][7]};
+  }
+}
+
+// acc = min(acc, A (x) B) for 64 x 64 tiles A (row-major at a) and B (at b).
+__device__ __forceinline__ void tile_product(int (&acc)[8][8], const gint *a, const gint *b,
+                                             int64_t ld, int (*As)[kT], int (*Bs)[kT], int tr,
+                                             int tc) {
+#pragma unroll
+  for (int k0 = 0; k0 < kT; k0 += kKc) {
+    if (k0 > 0) __syncthreads();
+    load_a_chunk(As, a, ld, k0);
+    load_b_chunk(Bs, b, ld, k0);
+    __syncthreads();
+    mp_accumulate<8, 8, kKc>(acc, &As[0][0], kT, &Bs[0][0], kT, tr * 4, 32, tc * 4, 32);
+  }
+}
+
+__global__ __launch_bounds__(64) void fw_phase2_kernel(const FwDesc *__restrict__ descs, int r) {
   const FwDesc d = descs[blockIdx.y];
   if (r >= d.nb) return;
   const int others = d.nb - 1;
@@ -116,22 +154,19 @@ __global__ __launch_bounds__(256) void fw_phase2_kernel(const FwDesc *__restrict
   const bool is_row = t < others;
   int o = is_row ? t : t - others;
   o = o < r ? o : o + 1;
-  __shared__ int As[kT][kT];
-  __shared__ int Bs[kT][kT];
-  const int *pivot = tile_ptr(d, r, r);
-  int *target = is_row ? tile_ptr(d, r, o) : tile_ptr(d, o, r);
+  __shared__ int As[kKc][kT];
+  __shared__ int Bs[kKc][kT];
+  const gint *pivot = tile_ptr(d, r, r);
+  gint *target = is_row ? tile_ptr(d, r, o) : tile_ptr(d, o, r);
+  const int tr = threadIdx.x >> 3, tc = threadIdx.x & 7;
+  int acc[8][8];
+  load_acc8(acc, target, d.ld, tr, tc);
   // row tile: target = P (x) target;  column tile: target = target (x) P.
-  load_tile_transposed(As, is_row ? pivot : target, d.ld);
-  load_tile(Bs, is_row ? target : pivot, d.ld);
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  int acc[4][4];
-  load_acc(acc, target, d.ld, tx, ty);
-  __syncthreads();
-  mp_accumulate<4, 4, kT>(acc, &As[0][0], kT, &Bs[0][0], kT, ty * 4, 4, tx * 4, 4);
-  store_acc(acc, target, d.ld, tx, ty);
+  tile_product(acc, is_row ? pivot : target, is_row ? target : pivot, d.ld, As, Bs, tr, tc);
+  store_acc8(acc, target, d.ld, tr, tc);
 }
 
-__global__ __launch_bounds__(256) void fw_phase3_kernel(const FwDesc *__restrict__ descs, int r) {
+__global__ __launch_bounds__(64) void fw_phase3_kernel(const FwDesc *__restrict__ descs, int r) {
   const FwDesc d = descs[blockIdx.y];
   if (r >= d.nb) return;
   const int others = d.nb - 1;
@@ -140,17 +175,14 @@ __global__ __launch_bounds__(256) void fw_phase3_kernel(const FwDesc *__restrict
   int i = t / others, j = t - (t / others) * others;
   i = i < r ? i : i + 1;
   j = j < r ? j : j + 1;
-  __shared__ int As[kT][kT];
-  __shared__ int Bs[kT][kT];
-  load_tile_transposed(As, tile_ptr(d, i, r), d.ld);
-  load_tile(Bs, tile_ptr(d, r, j), d.ld);
-  int *c = tile_ptr(d, i, j);
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  int acc[4][4];
-  load_acc(acc, c, d.ld, tx, ty);
-  __syncthreads();
-  mp_accumulate<4, 4, kT>(acc, &As[0][0], kT, &Bs[0][0], kT, ty * 4, 4, tx * 4, 4);
-  store_acc(acc, c, d.ld, tx, ty);
+  __shared__ int As[kKc][kT];
+  __shared__ int Bs[kKc][kT];
+  gint *c = tile_ptr(d, i, j);
+  const int tr = threadIdx.x >> 3, tc = threadIdx.x & 7;
+  int acc[8][8];
+  load_acc8(acc, c, d.ld, tr, tc);
+  tile_product(acc, tile_ptr(d, i, r), tile_ptr(d, r, j), d.ld, As, Bs, tr, tc);
+  store_acc8(acc, c, d.ld, tr, tc);
 }
 
 int launch_fw_round(const FwDesc *d_descs, int ndesc, int max_nb, int r, hipStream_t stream) {
@@ -159,10 +191,10 @@ int launch_fw_round(const FwDesc *d_descs, int ndesc, int max_nb, int r, hipStre
   FAP_CHECK_LAUNCH();
   const int others = max_nb - 1;
   if (others > 0) {
-    hipLaunchKernelGGL(fw_phase2_kernel, dim3(2 * others, ndesc), dim3(256), 0, stream, d_descs,
+    hipLaunchKernelGGL(fw_phase2_kernel, dim3(2 * others, ndesc), dim3(64), 0, stream, d_descs,
                        r);
     FAP_CHECK_LAUNCH();
-    hipLaunchKernelGGL(fw_phase3_kernel, dim3(others * others, ndesc), dim3(256), 0, stream,
+    hipLaunchKernelGGL(fw_phase3_kernel, dim3(others * others, ndesc), dim3(64), 0, stream,
                        d_descs, r);
     FAP_CHECK_LAUNCH();
   }

@@ -11,9 +11,12 @@
 //
 // PLAIN   launches the five steps part after part (the reference's structure:
 //         solve() calls run() once per part, fap/fap.cpp:512-516).
-// BATCHED launches every step once for all parts: one SSSP grid over all boundary
+// STAGED  launches every step once for all parts: one SSSP grid over all boundary
 //         sources, one build grid over all rows, three FW launches per round for
-//         all parts, one min-plus grid through a work table, one decode grid.
+//         all parts, one min-plus launch per padded-K group, one decode grid.
+// BATCHED runs SSSP and build as in STAGED, then splits the parts into chunks
+//         on two internal streams: the FW rounds of one chunk (VALU-bound) run
+//         beside the min-plus stream of another (HBM-bound).
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -62,6 +65,7 @@ struct MergeGroup {
   int first;  // first descriptor of the group
   int count;  // descriptors in the group
   int units;  // work units (tiles) of the group
+  int chunk;  // chunk of parts (BATCHED form)
 };
 
 namespace {
@@ -70,7 +74,6 @@ Workspace g_ws_fw;    // Floyd-Warshall matrices
 Workspace g_ws_meta;  // descriptors, sources, row indices
 Workspace g_ws_ell;   // ELL adjacency of the unit-weight graph
 hipStream_t g_last_stream = nullptr;
-fap_stage_times g_times{};
 
 struct EventPool {
   std::vector<hipEvent_t> ev;
@@ -87,8 +90,15 @@ struct EventPool {
 };
 EventPool g_events;
 
-// Stage marks: 6 events per group (start, after sssp, build, fw, merge, decode).
-std::vector<hipEvent_t> g_marks;
+// Timed spans of the last call: (stage, start event, end event); stage 0..4 =
+// sssp, build, fw, merge, decode.  g_total spans the whole call on the caller stream.
+struct Span {
+  int stage;
+  hipEvent_t a, b;
+};
+std::vector<Span> g_spans;
+hipEvent_t g_total[2] = {nullptr, nullptr};
+constexpr int kChunks = 8;
 
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
@@ -107,7 +117,7 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
   }
   FAP_REQUIRE(g && nparts >= 0 && (nparts == 0 || parts) && d_out,
               "fap_hip_solve_parts: null argument");
-  FAP_REQUIRE(mode == FAP_MODE_PLAIN || mode == FAP_MODE_BATCHED,
+  FAP_REQUIRE(mode == FAP_MODE_PLAIN || mode == FAP_MODE_BATCHED || mode == FAP_MODE_STAGED,
               "fap_hip_solve_parts: unknown mode %d", mode);
   const int n = g->n;
   FAP_REQUIRE(n > 0 && g->m >= 0 && ld_out >= n && g->d_row_offset && g->d_graph_id &&
@@ -131,9 +141,14 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
     if (q.num > 0) act.push_back(q);
   }
   const int P = (int)act.size();
-  g_times = fap_stage_times{};
+  g_spans.clear();
+  g_total[0] = g_total[1] = nullptr;
   if (P == 0) return FAP_OK;
-  const bool batched = mode == FAP_MODE_BATCHED;
+  const bool overlap = mode == FAP_MODE_BATCHED;
+  const bool batched = mode != FAP_MODE_PLAIN;  // one descriptor table for all parts
+  const int nchunks = overlap ? std::min(kChunks, P) : 1;
+  std::vector<int> chunk_of(P);
+  for (int p = 0; p < P; ++p) chunk_of[p] = (int)((int64_t)p * nchunks / P);
   const bool vec = (ld_out % 4 == 0) && ((uintptr_t)d_out % 16 == 0);
 
   std::vector<int> S(P), nb(P);
@@ -160,6 +175,7 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
   std::vector<int64_t> out_rows(P);
   int rows_total = 0, src_total = 0;
   int64_t dec_total = 0;
+  std::vector<int64_t> chunk_dec(nchunks, 0);
   for (int p = 0; p < P; ++p) {
     const fap_part_desc &q = act[p];
     int *fw = fw_base + fw_off[p];
@@ -184,6 +200,7 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
     m.ldf = S[p];
     m.fw_row0 = q.bdy;
     md[p] = m;
+    if (p > 0 && chunk_of[p] != chunk_of[p - 1]) dec_total = 0;
     dd[p] = DecodeDesc{fw, S[p], d_out + q.out_row * ld_out, ld_out, q.bdy, q.num, q.offset,
                        batched ? dec_total : 0};
     offsets[p] = q.offset;
@@ -192,6 +209,7 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
     rows_total += S[p];
     src_total += q.bdy;
     dec_total += (int64_t)q.bdy * q.num;
+    chunk_dec[chunk_of[p]] += (int64_t)q.bdy * q.num;
   }
 
   // ---- merge groups: products with M > 0, ordered by kernel kind ------------
@@ -202,16 +220,19 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
     std::vector<int> order;
     for (int p = 0; p < P; ++p)
       if (md[p].M > 0) order.push_back(p);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int a, int b) { return merge_kind(md[a].K) < merge_kind(md[b].K); });
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      if (chunk_of[a] != chunk_of[b]) return chunk_of[a] < chunk_of[b];
+      return merge_kind(md[a].K) < merge_kind(md[b].K);
+    });
     for (int p : order) {
       MergeDesc m = md[p];
       const int kind = merge_kind(m.K);
       const bool stream_kernel = kind > 0;
       m.row_blocks = stream_kernel ? merge_stream_row_blocks(m.M) : (m.M + 63) / 64;
       const int64_t units = stream_kernel ? merge_stream_units(m.M, n) : merge_tiles(m.M, n);
-      const bool new_group = !batched || groups.empty() || groups.back().kind != kind;
-      if (new_group) groups.push_back(MergeGroup{kind, (int)mdg.size(), 0, 0});
+      const bool new_group = !batched || groups.empty() || groups.back().kind != kind ||
+                             groups.back().chunk != chunk_of[p];
+      if (new_group) groups.push_back(MergeGroup{kind, (int)mdg.size(), 0, 0, chunk_of[p]});
       MergeGroup &gr = groups.back();
       FAP_REQUIRE((int64_t)gr.units + units < (1LL << 30), "fap_hip_solve_parts: too many tiles");
       m.tile_begin = gr.units;
@@ -260,14 +281,22 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
 
   // ---- launches -------------------------------------------------------------
   g_events.used = 0;
-  g_marks.clear();
-  auto mark = [&]() -> int {
-    hipEvent_t e;
-    FAP_TRY(g_events.take(&e));
-    FAP_CHECK_HIP(hipEventRecord(e, st));
-    g_marks.push_back(e);
+  g_spans.clear();
+  auto event = [&](hipStream_t on, hipEvent_t *e) -> int {
+    FAP_TRY(g_events.take(e));
+    FAP_CHECK_HIP(hipEventRecord(*e, on));
     return FAP_OK;
   };
+  // Runs fn on stream `on` inside a timed span of `stage`.
+  auto span = [&](int stage, hipStream_t on, auto &&fn) -> int {
+    Span sp{stage, nullptr, nullptr};
+    FAP_TRY(event(on, &sp.a));
+    FAP_TRY(fn(on));
+    FAP_TRY(event(on, &sp.b));
+    g_spans.push_back(sp);
+    return FAP_OK;
+  };
+  FAP_TRY(event(st, &g_total[0]));
 
   // Boundary SSSP: the LDS-bitmap BFS for unit weights, else label correcting.
   int4 *ell = nullptr;
@@ -276,79 +305,100 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
     ell = static_cast<int4 *>(g_ws_ell.ptr);
     FAP_TRY(launch_ell_build(g->d_row_offset, g->d_col, n, ell, st));
   }
-  auto run_sssp = [&](const int *src, int nsrc) -> int {
+  auto run_sssp = [&](const int *src, int nsrc, hipStream_t on) -> int {
     if (ell) {
       const int s = launch_bfs_lds(ell, g->d_row_offset, g->d_col, n, src, nsrc, d_rows, d_out, 1,
-                                   vec, st);
+                                   vec, on);
       if (s != FAP_ERR_INVALID) return s;
     }
-    return launch_sssp(g->d_row_offset, g->d_col, g->d_w, n, src, nsrc, d_rows, d_out, 1, vec, st);
+    return launch_sssp(g->d_row_offset, g->d_col, g->d_w, n, src, nsrc, d_rows, d_out, 1, vec, on);
   };
-
-  auto run_group = [&](const MergeGroup &gr) -> int {
+  auto run_group = [&](const MergeGroup &gr, hipStream_t on) -> int {
     if (gr.kind == 0)
       return launch_merge(d_md + gr.first, gr.count, gr.units, n, vec, g->d_graph_id, g->d_ed2st,
-                          st);
+                          on);
     return launch_merge_stream(d_md + gr.first, gr.count, gr.units, n, gr.kind, vec,
-                               g->d_graph_id, g->d_ed2st, st);
+                               g->d_graph_id, g->d_ed2st, on);
   };
 
   if (batched) {
-    FAP_TRY(mark());
-    FAP_TRY(launch_gather_sources(d_offsets, d_sb, d_or, P, src_total, g->d_st2ed, ld_out, d_src,
-                                  d_rows, st));
-    FAP_TRY(run_sssp(d_src, src_total));
-    FAP_TRY(mark());
-    FAP_TRY(launch_build(d_bd, P, rows_total, true, g->d_row_offset, g->d_col, g->d_w,
-                         g->d_graph_id, g->d_st2ed, g->d_ed2st, st));
-    FAP_TRY(mark());
-    FAP_TRY(launch_fw(d_fd, P, max_nb, st));
-    FAP_TRY(mark());
-    for (const MergeGroup &gr : groups) FAP_TRY(run_group(gr));
-    FAP_TRY(mark());
-    FAP_TRY(launch_decode(d_dd, P, dec_total, g->d_st2ed, st));
-    FAP_TRY(mark());
+    FAP_TRY(span(0, st, [&](hipStream_t on) -> int {
+      FAP_TRY(launch_gather_sources(d_offsets, d_sb, d_or, P, src_total, g->d_st2ed, ld_out,
+                                    d_src, d_rows, on));
+      return run_sssp(d_src, src_total, on);
+    }));
+    FAP_TRY(span(1, st, [&](hipStream_t on) {
+      return launch_build(d_bd, P, rows_total, true, g->d_row_offset, g->d_col, g->d_w,
+                          g->d_graph_id, g->d_st2ed, g->d_ed2st, on);
+    }));
+    // Chunks of parts: [first part, end part) with their FW rounds, merges and decode.
+    std::vector<int> cfirst(nchunks + 1, P);
+    for (int p = P - 1; p >= 0; --p) cfirst[chunk_of[p]] = p;
+    hipEvent_t ready = nullptr;
+    if (overlap) FAP_TRY(event(st, &ready));
+    std::vector<hipEvent_t> done;
+    for (int c = 0; c < nchunks; ++c) {
+      const int p0 = cfirst[c], p1 = cfirst[c + 1];
+      if (p0 >= p1) continue;
+      hipStream_t on = overlap ? aux_stream(c & 1) : st;
+      if (overlap) FAP_CHECK_HIP(hipStreamWaitEvent(on, ready, 0));
+      int cmax_nb = 0;
+      int64_t cdec = 0;
+      for (int p = p0; p < p1; ++p) cmax_nb = std::max(cmax_nb, nb[p]);
+      cdec = overlap ? chunk_dec[c] : dec_total;
+      FAP_TRY(span(2, on, [&](hipStream_t s2) { return launch_fw(d_fd + p0, p1 - p0, cmax_nb, s2); }));
+      FAP_TRY(span(3, on, [&](hipStream_t s2) -> int {
+        for (const MergeGroup &gr : groups)
+          if (gr.chunk == c) FAP_TRY(run_group(gr, s2));
+        return FAP_OK;
+      }));
+      FAP_TRY(span(4, on, [&](hipStream_t s2) {
+        return launch_decode(d_dd + p0, p1 - p0, cdec, g->d_st2ed, s2);
+      }));
+      if (overlap) {
+        hipEvent_t e;
+        FAP_TRY(event(on, &e));
+        done.push_back(e);
+      }
+    }
+    for (hipEvent_t e : done) FAP_CHECK_HIP(hipStreamWaitEvent(st, e, 0));
   } else {
     for (int p = 0; p < P; ++p) {
       const fap_part_desc &q = act[p];
-      FAP_TRY(mark());
-      FAP_TRY(launch_gather_sources(d_offsets + p, d_sb + p, d_or + p, 1, q.bdy, g->d_st2ed, ld_out,
-                                    d_src, d_rows, st));
-      FAP_TRY(run_sssp(d_src, q.bdy));
-      FAP_TRY(mark());
-      FAP_TRY(launch_build(d_bd + p, 1, S[p], true, g->d_row_offset, g->d_col, g->d_w,
-                           g->d_graph_id, g->d_st2ed, g->d_ed2st, st));
-      FAP_TRY(mark());
-      FAP_TRY(launch_fw(d_fd + p, 1, nb[p], st));
-      FAP_TRY(mark());
-      if (part_merge[p] >= 0) FAP_TRY(run_group(groups[part_merge[p]]));
-      FAP_TRY(mark());
-      FAP_TRY(launch_decode(d_dd + p, 1, (int64_t)q.bdy * q.num, g->d_st2ed, st));
-      FAP_TRY(mark());
+      FAP_TRY(span(0, st, [&](hipStream_t on) -> int {
+        FAP_TRY(launch_gather_sources(d_offsets + p, d_sb + p, d_or + p, 1, q.bdy, g->d_st2ed,
+                                      ld_out, d_src, d_rows, on));
+        return run_sssp(d_src, q.bdy, on);
+      }));
+      FAP_TRY(span(1, st, [&](hipStream_t on) {
+        return launch_build(d_bd + p, 1, S[p], true, g->d_row_offset, g->d_col, g->d_w,
+                            g->d_graph_id, g->d_st2ed, g->d_ed2st, on);
+      }));
+      FAP_TRY(span(2, st, [&](hipStream_t on) { return launch_fw(d_fd + p, 1, nb[p], on); }));
+      FAP_TRY(span(3, st, [&](hipStream_t on) {
+        return part_merge[p] >= 0 ? run_group(groups[part_merge[p]], on) : FAP_OK;
+      }));
+      FAP_TRY(span(4, st, [&](hipStream_t on) {
+        return launch_decode(d_dd + p, 1, (int64_t)q.bdy * q.num, g->d_st2ed, on);
+      }));
     }
   }
+  FAP_TRY(event(st, &g_total[1]));
   return FAP_OK;
 }
 
 extern "C" int fap_hip_last_stage_times(fap_stage_times *t) {
   FAP_REQUIRE(t, "fap_hip_last_stage_times: null");
   fap_stage_times acc{};
-  const size_t groups = g_marks.size() / 6;
-  if (groups > 0) FAP_CHECK_HIP(hipEventSynchronize(g_marks.back()));
-  for (size_t gi = 0; gi < groups; ++gi) {
-    float v[5];
-    for (int s = 0; s < 5; ++s)
-      FAP_CHECK_HIP(hipEventElapsedTime(&v[s], g_marks[6 * gi + s], g_marks[6 * gi + s + 1]));
-    acc.sssp_ms += v[0];
-    acc.build_ms += v[1];
-    acc.floyd_ms += v[2];
-    acc.minplus_ms += v[3];
-    acc.decode_ms += v[4];
-  }
-  if (groups > 0) {
-    float tot;
-    FAP_CHECK_HIP(hipEventElapsedTime(&tot, g_marks.front(), g_marks.back()));
-    acc.total_ms = tot;
+  if (g_total[1]) {
+    FAP_CHECK_HIP(hipEventSynchronize(g_total[1]));
+    float *slot[5] = {&acc.sssp_ms, &acc.build_ms, &acc.floyd_ms, &acc.minplus_ms, &acc.decode_ms};
+    for (const Span &sp : g_spans) {
+      float v;
+      FAP_CHECK_HIP(hipEventElapsedTime(&v, sp.a, sp.b));
+      *slot[sp.stage] += v;
+    }
+    FAP_CHECK_HIP(hipEventElapsedTime(&acc.total_ms, g_total[0], g_total[1]));
   }
   *t = acc;
   return FAP_OK;

@@ -15,6 +15,7 @@ std::mutex g_mu;
 bool g_init = false;
 int g_device = -1;
 hipStream_t g_stream = nullptr;
+hipStream_t g_aux[2] = {nullptr, nullptr};
 int64_t g_budget = 0;  // 0 = unlimited
 int64_t g_in_use = 0;
 int64_t g_high = 0;
@@ -32,6 +33,7 @@ void set_error(const char *fmt, ...) {
 void clear_error() { g_last_error.clear(); }
 
 hipStream_t lib_stream() { return g_stream; }
+hipStream_t aux_stream(int i) { return g_aux[i & 1]; }
 bool initialized() { return g_init; }
 
 int dev_alloc(void **p, int64_t bytes) {
@@ -109,6 +111,28 @@ __global__ void fill_i32_kernel(int32_t *__restrict__ p, int32_t v, int64_t coun
   for (; i < count; i += stride) p[i] = v;
 }
 
+// 16-byte stores; count4 = number of int4 elements.
+__global__ void fill_i32x4_kernel(int4 *__restrict__ p, int32_t v, int64_t count4) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int4 val = make_int4(v, v, v, v);
+  for (; i < count4; i += stride) p[i] = val;
+}
+
+__global__ void copy_i32x4_kernel(int4 *__restrict__ dst, const int4 *__restrict__ src,
+                                  int64_t count4) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < count4; i += stride) dst[i] = src[i];
+}
+
+__global__ void copy_i32_kernel(int32_t *__restrict__ dst, const int32_t *__restrict__ src,
+                                int64_t count) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < count; i += stride) dst[i] = src[i];
+}
+
 }  // namespace fap_hip
 
 using namespace fap_hip;
@@ -132,6 +156,9 @@ int fap_hip_init(int device) {
   }
   FAP_CHECK_HIP(hipSetDevice(device));
   FAP_CHECK_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  // Two more streams (default priority) for overlapping the stages of different parts.
+  FAP_CHECK_HIP(hipStreamCreateWithFlags(&g_aux[0], hipStreamNonBlocking));
+  FAP_CHECK_HIP(hipStreamCreateWithFlags(&g_aux[1], hipStreamNonBlocking));
   g_device = device;
   g_init = true;
   return FAP_OK;
@@ -216,10 +243,36 @@ int fap_hip_memcpy2d_d2h_i32(int32_t *h_dst, int64_t h_ld, const int32_t *d_src,
 int fap_hip_fill_i32(int32_t *d_ptr, int32_t value, int64_t count, void *stream) {
   if (count == 0) return FAP_OK;
   FAP_REQUIRE(d_ptr && count > 0, "fap_hip_fill_i32: invalid arguments");
-  int64_t blocks = (count + 255) / 256;
-  if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)blocks), dim3(256), 0, resolve(stream),
-                     d_ptr, value, count);
+  if (count % 4 == 0 && (uintptr_t)d_ptr % 16 == 0) {
+    int64_t blocks = (count / 4 + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_i32x4_kernel, dim3((unsigned)blocks), dim3(256), 0, resolve(stream),
+                       reinterpret_cast<int4 *>(d_ptr), value, count / 4);
+  } else {
+    int64_t blocks = (count + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)blocks), dim3(256), 0, resolve(stream),
+                       d_ptr, value, count);
+  }
+  FAP_CHECK_LAUNCH();
+  return FAP_OK;
+}
+
+int fap_hip_copy_i32(int32_t *d_dst, const int32_t *d_src, int64_t count, void *stream) {
+  if (count == 0) return FAP_OK;
+  FAP_REQUIRE(d_dst && d_src && count > 0, "fap_hip_copy_i32: invalid arguments");
+  if (count % 4 == 0 && (uintptr_t)d_dst % 16 == 0 && (uintptr_t)d_src % 16 == 0) {
+    int64_t blocks = (count / 4 + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(copy_i32x4_kernel, dim3((unsigned)blocks), dim3(256), 0, resolve(stream),
+                       reinterpret_cast<int4 *>(d_dst), reinterpret_cast<const int4 *>(d_src),
+                       count / 4);
+  } else {
+    int64_t blocks = (count + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(copy_i32_kernel, dim3((unsigned)blocks), dim3(256), 0, resolve(stream),
+                       d_dst, d_src, count);
+  }
   FAP_CHECK_LAUNCH();
   return FAP_OK;
 }

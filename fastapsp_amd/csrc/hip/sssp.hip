@@ -191,7 +191,7 @@ __global__ __launch_bounds__(64 * kSsspWaves) void sssp_rows_kernel(
 // vertices per pass and issues all their loads before using any.
 constexpr int kEllW = 4;
 constexpr int kBfsQ = 1024;
-constexpr int kBfsU = 4;
+constexpr int kBfsU = 8;  // frontier vertices per lane per pass (512 per wave)
 constexpr int kBfsLdsMaxN = 1 << 20;
 
 __global__ void ell_build_kernel(const int *__restrict__ row_off, const int *__restrict__ col, int n,
@@ -206,14 +206,81 @@ __global__ void ell_build_kernel(const int *__restrict__ row_off, const int *__r
   ell[v] = make_int4(s[0], s[1], s[2], s[3]);
 }
 
-__device__ __forceinline__ void bfs_claim(unsigned *vis, int *row, int v, int d, int *q_next,
-                                          int *q_cnt) {
-  const unsigned bit = 1u << (v & 31);
-  if (vis[v >> 5] & bit) return;
-  if (atomicOr(&vis[v >> 5], bit) & bit) return;
-  row[v] = d;
-  const int pos = atomicAdd(q_cnt, 1);
-  if (pos < kBfsQ) q_next[pos] = v;
+// Exclusive prefix sum over the wave of x in [0, 63] (six ballots), and the total.
+__device__ __forceinline__ int wave_excl_prefix(int x, int *total) {
+  int pre = 0, tot = 0;
+#pragma unroll
+  for (int b = 0; b < 6; ++b) {
+    const unsigned long long m = __ballot((x >> b) & 1);
+    pre += (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u)) << b;
+    tot += __popcll(m) << b;
+  }
+  *total = tot;
+  return pre;
+}
+
+// The frontier of one level: q[0, main) and q[kBfsQ - tail, kBfsQ).
+struct Frontier {
+  int main, tail;
+  __device__ int size() const { return main + tail; }
+  __device__ int at(const int *q, int idx) const {
+    return idx < main ? q[idx] : q[kBfsQ - tail + (idx - main)];
+  }
+};
+
+// Expands up to kBfsU vertices per lane (u[t] < 0 = none).  All ELL loads are
+// issued first, then all bitmap claims; queue slots of the main region come from a
+// wave prefix sum (the queue belongs to this wave).  Arcs past an ELL row (degree
+// > 4) are claimed one by one into the tail region with an LDS counter.
+__device__ __forceinline__ void bfs_expand_batch(const int4 *__restrict__ ell,
+                                                 const int *__restrict__ row_off,
+                                                 const int *__restrict__ col, unsigned *vis,
+                                                 int *row, const int (&u)[kBfsU], int d,
+                                                 int *q_next, int &next_main, int *tail_cnt) {
+  int4 nb[kBfsU];
+#pragma unroll
+  for (int t = 0; t < kBfsU; ++t) nb[t] = u[t] >= 0 ? ell[u[t]] : make_int4(-1, -1, -1, -1);
+  int cand[4 * kBfsU];
+#pragma unroll
+  for (int t = 0; t < kBfsU; ++t) {
+    cand[4 * t + 0] = nb[t].x;
+    cand[4 * t + 1] = nb[t].y;
+    cand[4 * t + 2] = nb[t].z;
+    cand[4 * t + 3] = nb[t].w;
+  }
+  unsigned old[4 * kBfsU];
+#pragma unroll
+  for (int c = 0; c < 4 * kBfsU; ++c)
+    old[c] = cand[c] >= 0 ? atomicOr(&vis[cand[c] >> 5], 1u << (cand[c] & 31)) : ~0u;
+  unsigned fresh = 0;
+#pragma unroll
+  for (int c = 0; c < 4 * kBfsU; ++c)
+    if (cand[c] >= 0 && !(old[c] & (1u << (cand[c] & 31)))) fresh |= 1u << c;
+  int total;
+  int pos = next_main + wave_excl_prefix(__popc(fresh), &total);
+#pragma unroll
+  for (int c = 0; c < 4 * kBfsU; ++c) {
+    if (fresh & (1u << c)) {
+      row[cand[c]] = d;
+      if (pos < kBfsQ) q_next[pos] = cand[c];
+      ++pos;
+    }
+  }
+  next_main += total;
+#pragma unroll
+  for (int t = 0; t < kBfsU; ++t) {
+    if (nb[t].w != -2) continue;
+    const int e1 = row_off[u[t] + 1];
+    for (int e = row_off[u[t]] + kEllW - 1; e < e1; ++e) {
+      const int v = col[e];
+      const unsigned bit = 1u << (v & 31);
+      if (atomicOr(&vis[v >> 5], bit) & bit) continue;
+      row[v] = d;
+      const int k = atomicAdd(tail_cnt, 1);
+      if (k < kBfsQ) q_next[kBfsQ - 1 - k] = v;
+    }
+  }
 }
 
 template <bool kVecFill>
@@ -227,7 +294,7 @@ __global__ __launch_bounds__(64) void bfs_lds_kernel(const int4 *__restrict__ el
   const int nwords = (n + 31) >> 5;
   unsigned *vis = reinterpret_cast<unsigned *>(smem);
   int *queue = smem + ((nwords + 3) & ~3);  // [2][kBfsQ]
-  int *qcount = queue + 2 * kBfsQ;          // [2]
+  int *tail_cnt = queue + 2 * kBfsQ;        // [2]
   const int lane = threadIdx.x;
   const int sidx = blockIdx.x;
   if (sidx >= nsrc) return;
@@ -242,70 +309,54 @@ __global__ __launch_bounds__(64) void bfs_lds_kernel(const int4 *__restrict__ el
   if (lane == 0) {
     vis[s >> 5] = 1u << (s & 31);
     queue[0] = s;
-    qcount[0] = 1;
-    qcount[1] = 0;
+    tail_cnt[0] = 0;
+    tail_cnt[1] = 0;
   }
   wave_sync();
 
-  int cur = 0, qsize = 1, level = 0;
+  int cur = 0, level = 0;
+  Frontier f{1, 0};
   bool scan = false;
   while (true) {
     const int nxt = cur ^ 1;
     int *q_next = queue + nxt * kBfsQ;
-    int *q_cnt = &qcount[nxt];
+    int *t_cnt = &tail_cnt[nxt];
     const int d = level + 1;
+    int next_main = 0;
     if (!scan) {
       const int *q_cur = queue + cur * kBfsQ;
-      for (int base = 0; base < qsize; base += 64 * kBfsU) {
+      const int fs = f.size();
+      for (int base = 0; base < fs; base += 64 * kBfsU) {
         int u[kBfsU];
-        int4 nb[kBfsU];
 #pragma unroll
         for (int t = 0; t < kBfsU; ++t) {
           const int idx = base + t * 64 + lane;
-          u[t] = idx < qsize ? q_cur[idx] : -1;
+          u[t] = idx < fs ? f.at(q_cur, idx) : -1;
         }
-#pragma unroll
-        for (int t = 0; t < kBfsU; ++t) nb[t] = u[t] >= 0 ? ell[u[t]] : make_int4(-1, -1, -1, -1);
-#pragma unroll
-        for (int t = 0; t < kBfsU; ++t) {
-          if (nb[t].x >= 0) bfs_claim(vis, row, nb[t].x, d, q_next, q_cnt);
-          if (nb[t].y >= 0) bfs_claim(vis, row, nb[t].y, d, q_next, q_cnt);
-          if (nb[t].z >= 0) bfs_claim(vis, row, nb[t].z, d, q_next, q_cnt);
-          if (nb[t].w >= 0) {
-            bfs_claim(vis, row, nb[t].w, d, q_next, q_cnt);
-          } else if (nb[t].w == -2) {
-            const int e1 = row_off[u[t] + 1];
-            for (int e = row_off[u[t]] + kEllW - 1; e < e1; ++e)
-              bfs_claim(vis, row, col[e], d, q_next, q_cnt);
-          }
-        }
+        bfs_expand_batch(ell, row_off, col, vis, row, u, d, q_next, next_main, t_cnt);
       }
     } else {
       // The frontier did not fit in LDS: find it in the row.  The release fence
       // waits for this wave's label stores, the agent-scope loads read them at L2.
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      for (int v = lane; v < n; v += 64) {
-        if (load_agent(row + v) != level) continue;
-        const int4 q = ell[v];
-        if (q.x >= 0) bfs_claim(vis, row, q.x, d, q_next, q_cnt);
-        if (q.y >= 0) bfs_claim(vis, row, q.y, d, q_next, q_cnt);
-        if (q.z >= 0) bfs_claim(vis, row, q.z, d, q_next, q_cnt);
-        if (q.w >= 0) {
-          bfs_claim(vis, row, q.w, d, q_next, q_cnt);
-        } else if (q.w == -2) {
-          const int e1 = row_off[v + 1];
-          for (int e = row_off[v] + kEllW - 1; e < e1; ++e)
-            bfs_claim(vis, row, col[e], d, q_next, q_cnt);
+      for (int base = 0; base < n; base += 64 * kBfsU) {
+        int u[kBfsU];
+#pragma unroll
+        for (int t = 0; t < kBfsU; ++t) {
+          const int v = base + t * 64 + lane;
+          u[t] = (v < n && load_agent(row + v) == level) ? v : -1;
         }
+        bfs_expand_batch(ell, row_off, col, vis, row, u, d, q_next, next_main, t_cnt);
       }
     }
     wave_sync();
-    const int cnt = __hip_atomic_load(q_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    const int tail = __hip_atomic_load(t_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     wave_sync();
+    const int cnt = next_main + tail;
     if (cnt == 0) break;
     scan = cnt > kBfsQ;
-    qsize = scan ? 0 : cnt;
-    if (lane == 0) qcount[cur] = 0;
+    f = Frontier{scan ? 0 : next_main, scan ? 0 : tail};
+    if (lane == 0) tail_cnt[cur] = 0;
     wave_sync();
     cur = nxt;
     ++level;

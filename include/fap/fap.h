@@ -115,7 +115,8 @@ class fapGraph {
   bool isSplit();
   std::vector<int> getGraphId();
 
-  // Execution form of solve(): FAP_MODE_BATCHED (default) or FAP_MODE_PLAIN.
+  // Execution form of solve(): FAP_MODE_BATCHED (default), FAP_MODE_STAGED or
+  // FAP_MODE_PLAIN (identical results).
   void setMode(int32_t mode);
 
   int32_t solve(bool is_path_needed = false);

@@ -36,14 +36,16 @@ enum fap_status {
   FAP_ERR_NODEV = 7    /* no usable GPU                                   */
 };
 
-/* Execution forms of the per-part pipeline.
+/* Execution forms of the per-part pipeline (identical results).
  * PLAIN:   one launch sequence per part (SSSP, build, FW rounds, min-plus,
  *          decode), parts in turn -- the structure of fapGraph::run
  *          (fap/fap.cpp:195-503) called from solve() (fap/fap.cpp:507-518).
- * BATCHED: every stage launched once for all parts (boundary SSSP of all parts
- *          in one grid, all FW matrices per round in one grid, one min-plus
- *          grid over all parts with a work table). */
-enum fap_mode { FAP_MODE_PLAIN = 0, FAP_MODE_BATCHED = 1 };
+ * BATCHED: boundary SSSP and build of all parts in one grid each; then the parts
+ *          in chunks on two internal streams, so the Floyd-Warshall rounds of one
+ *          chunk (VALU-bound) overlap the min-plus stream of another (HBM-bound).
+ * STAGED:  every stage launched once for all parts, stages one after another
+ *          (no overlap) -- per-stage device times are those of each kernel alone. */
+enum fap_mode { FAP_MODE_PLAIN = 0, FAP_MODE_BATCHED = 1, FAP_MODE_STAGED = 2 };
 
 /* ------------------------------------------------------------------ runtime */
 
@@ -69,6 +71,9 @@ int fap_hip_memcpy_d2d(void *d_dst, const void *d_src, int64_t bytes, void *stre
 int fap_hip_memcpy2d_d2h_i32(int32_t *h_dst, int64_t h_ld, const int32_t *d_src, int64_t d_ld,
                              int64_t rows, int64_t cols, void *stream);
 int fap_hip_fill_i32(int32_t *d_ptr, int32_t value, int64_t count, void *stream);
+/* Plain device copy kernel (16-byte accesses when aligned); also the known-size
+ * reference for HBM traffic counters. */
+int fap_hip_copy_i32(int32_t *d_dst, const int32_t *d_src, int64_t count, void *stream);
 int fap_hip_stream_sync(void *stream);
 int fap_hip_device_sync(void);
 
@@ -149,7 +154,10 @@ typedef struct fap_part_desc {
   int64_t out_row; /* first row of the part's s_p x n block in d_out */
 } fap_part_desc;
 
-/* Per-stage device time of the last fap_hip_solve_parts call (ms). */
+/* Device time of the last fap_hip_solve_parts call (ms): total wall time on the
+ * stream, and per stage the summed duration of that stage's launches (in the
+ * BATCHED form stages of different chunks overlap, so the stages can sum to more
+ * than the total). */
 typedef struct fap_stage_times {
   float sssp_ms, build_ms, floyd_ms, minplus_ms, decode_ms, total_ms;
 } fap_stage_times;
