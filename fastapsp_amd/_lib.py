@@ -27,7 +27,7 @@ FAP_ERR_NODEV = 7
 
 FAP_MODE_PLAIN = 0
 FAP_MODE_BATCHED = 1
-FAP_MODE_STAGED = 2
+FAP_MODE_OVERLAP = 2
 
 # Every exported symbol of include/fap_hip.h: name -> (restype, argtypes).
 _i32, _i64, _u64, _f64 = C.c_int32, C.c_int64, C.c_uint64, C.c_double

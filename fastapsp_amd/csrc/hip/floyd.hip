@@ -5,16 +5,25 @@
 // TILE 32, one launch pair per round per part).
 //
 // Every part's s x s matrix is stored padded to S = roundup(s, 64) with FAP_INF
-// outside the real block, so all tiles are full 64 x 64 tiles and no kernel
-// needs bounds checks.  Round r of every part runs in the same three launches:
-//   phase 1  closes the pivot tile (r, r) in LDS: 64 k-steps, one barrier each;
-//   phase 2  updates the pivot row and column tiles as ONE min-plus product with
-//            the closed pivot:  T(r,j) = P* (x) T(r,j),  T(i,r) = T(i,r) (x) P*.
-//            With non-negative weights P* has a 0 diagonal, so the product equals
-//            the in-place sequential k-loop of the textbook blocked algorithm;
-//   phase 3  C(i,j) = min(C(i,j), T(i,r) (x) T(r,j)) for every other tile.
-// Phases 2 and 3 use the register-tiled min-plus micro-kernel (mp_tile.h).
-// Integer distances make the result identical to the k-i-j loop.
+// outside the real block, so every tile is a full 64 x 64 tile and no kernel
+// needs bounds checks.  A round pivots on a panel P of kPanel 64-tiles (up to two
+// 64-tiles; see kPanel), so a matrix with nb tiles per side takes
+// ceil(nb / kPanel) rounds and every other tile is read and written once per round.  Round r of every part runs in the same three launches:
+//   phase 1  closes the diagonal block (P, P) in LDS (one 1024-thread workgroup);
+//   phase 2  updates the pivot row and column panels as ONE min-plus product with
+//            the closed block:  T(P,j) = P* (x) T(P,j),  T(i,P) = T(i,P) (x) P*.
+//            With non-negative weights P* has a 0 diagonal and satisfies the
+//            triangle inequality, so the product equals the sequential k-loop.
+//            A workgroup writes one 64-row (or 64-column) half of a panel tile
+//            while another reads it as an operand: either value is a path length
+//            at least the new minimum and P*(a,k) + P*(k,m) >= P*(a,m), so the
+//            result is the same whichever value is read;
+//   phase 3  C(i,j) = min(C(i,j), T(i,P) (x) T(P,j)) for every tile outside P.
+// Phases 2 and 3 use the register-tiled min-plus micro-kernel (mp_tile.h) with
+// one wave64 per 64 x 64 output tile.  Integer distances make the result
+// identical to the k-i-j loop.
+#include <algorithm>
+
 #include "fap_internal.h"
 #include "fw_desc.h"
 #include "gmem.h"
@@ -33,46 +42,54 @@ __device__ __forceinline__ gint *tile_ptr(const FwDesc &d, int ti, int tj) {
   return gptr(d.mat + (int64_t)(ti * kT) * d.ld + tj * kT);
 }
 
-// Global row-major 64x64 tile -> LDS row-major panel Bs[k][j].
-__device__ __forceinline__ void load_tile(int (*Bs)[kT], const gint *g, int64_t ld) {
-  const int k = threadIdx.x >> 2;
-  const int jq = threadIdx.x & 3;  // columns jq*16 .. jq*16+15
-  const gv4i *src = reinterpret_cast<const gv4i *>(g + (int64_t)k * ld + jq * 16);
-  v4i *dst = reinterpret_cast<v4i *>(&Bs[k][jq * 16]);
-#pragma unroll
-  for (int c = 0; c < 4; ++c) dst[c] = src[c];
+// Pivot panel width in 64-tiles.  Two-tile panels halve the rounds (and the C-tile
+// traffic) but serialise a 128-step closure in phase 1 (~200 us per round on
+// MI355X against ~20 us for 64 steps), which costs more than it saves.
+constexpr int kPanel = 1;
+constexpr int kP1Threads = kPanel * kT * kPanel * kT / 16;  // 16 entries per thread
+
+__device__ __forceinline__ int panel_tiles(const FwDesc &d, int r) {
+  return min(kPanel, d.nb - kPanel * r);
 }
 
-__global__ __launch_bounds__(256) void fw_phase1_kernel(const FwDesc *__restrict__ descs, int r) {
+// Phase 1: the (W x W) diagonal block, W = 64 * panel tiles, closed in LDS.  Each
+// thread owns 16 consecutive entries of one row.
+__global__ __launch_bounds__(kP1Threads) void fw_phase1_kernel(const FwDesc *__restrict__ descs, int r) {
   const FwDesc d = descs[blockIdx.y];
-  if (r >= d.nb) return;
-  __shared__ int D[kT][kT];
-  gint *g = tile_ptr(d, r, r);
-  load_tile(D, g, d.ld);
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  for (int k = 0; k < kT; ++k) {
-    __syncthreads();
-    // Row k and column k do not change during step k (D[k][k] >= 0), so one
-    // barrier per step orders the in-place update.
-    const int4 rk = *reinterpret_cast<const int4 *>(&D[k][tx * 4]);
+  if (kPanel * r >= d.nb) return;
+  const int W = kT * panel_tiles(d, r);
+  __shared__ int D[kPanel * kT][kPanel * kT];
+  gint *g = tile_ptr(d, kPanel * r, kPanel * r);
+  constexpr int kRowThreads = kPanel * kT / 16;
+  const int row = threadIdx.x / kRowThreads, col0 = (threadIdx.x % kRowThreads) * 16;
+  const bool mine = row < W && col0 < W;
+  if (mine) {
+    const gv4i *src = reinterpret_cast<const gv4i *>(g + (int64_t)row * d.ld + col0);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int i = ty * 4 + q;
-      const int dik = D[i][k];
-      int4 v = *reinterpret_cast<int4 *>(&D[i][tx * 4]);
-      v.x = min(v.x, dik + rk.x);
-      v.y = min(v.y, dik + rk.y);
-      v.z = min(v.z, dik + rk.z);
-      v.w = min(v.w, dik + rk.w);
-      *reinterpret_cast<int4 *>(&D[i][tx * 4]) = v;
+    for (int c = 0; c < 4; ++c) *reinterpret_cast<v4i *>(&D[row][col0 + 4 * c]) = src[c];
+  }
+  for (int k = 0; k < W; ++k) {
+    __syncthreads();
+    // Row k and column k do not change during step k (D[k][k] = 0), so one
+    // barrier per step orders the in-place update.
+    if (mine) {
+      const int dik = D[row][k];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const v4i rk = *reinterpret_cast<const v4i *>(&D[k][col0 + 4 * c]);
+        v4i v = *reinterpret_cast<v4i *>(&D[row][col0 + 4 * c]);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[e] = min(v[e], dik + rk[e]);
+        *reinterpret_cast<v4i *>(&D[row][col0 + 4 * c]) = v;
+      }
     }
   }
   __syncthreads();
-  const int k = threadIdx.x >> 2, jq = threadIdx.x & 3;
-  gv4i *dst = reinterpret_cast<gv4i *>(g + (int64_t)k * d.ld + jq * 16);
-  const v4i *src = reinterpret_cast<const v4i *>(&D[k][jq * 16]);
+  if (mine) {
+    gv4i *dst = reinterpret_cast<gv4i *>(g + (int64_t)row * d.ld + col0);
 #pragma unroll
-  for (int c = 0; c < 4; ++c) dst[c] = src[c];
+    for (int c = 0; c < 4; ++c) dst[c] = *reinterpret_cast<const v4i *>(&D[row][col0 + 4 * c]);
+  }
 }
 
 // Phases 2 and 3: one wave64 per 64x64 tile product, 8 x 8 accumulators per lane
@@ -131,12 +148,12 @@ __device__ __forceinline__ void store_acc8(const int (&acc)[8][8], gint *g, int6
   }
 }
 
-// acc = min(acc, A (x) B) for 64 x 64 tiles A (row-major at a) and B (at b).
+// acc = min(acc, A (x) B): A is 64 x kdim (row-major at a), B is kdim x 64 (at b),
+// kdim a multiple of 32.
 __device__ __forceinline__ void tile_product(int (&acc)[8][8], const gint *a, const gint *b,
-                                             int64_t ld, int (*As)[kT], int (*Bs)[kT], int tr,
-                                             int tc) {
-#pragma unroll
-  for (int k0 = 0; k0 < kT; k0 += kKc) {
+                                             int64_t ld, int kdim, int (*As)[kT], int (*Bs)[kT],
+                                             int tr, int tc) {
+  for (int k0 = 0; k0 < kdim; k0 += kKc) {
     if (k0 > 0) __syncthreads();
     load_a_chunk(As, a, ld, k0);
     load_b_chunk(Bs, b, ld, k0);
@@ -145,54 +162,67 @@ __device__ __forceinline__ void tile_product(int (&acc)[8][8], const gint *a, co
   }
 }
 
+// Tile index o in [0, nb - wt) -> tile outside the pivot panel [p, p + wt).
+__device__ __forceinline__ int skip_panel(int o, int p, int wt) { return o < p ? o : o + wt; }
+
+// Phase 2: 2 * wt * (nb - wt) output tiles per part: the row panel tiles (P_a, j)
+// first, then the column panel tiles (i, P_a).
 __global__ __launch_bounds__(64) void fw_phase2_kernel(const FwDesc *__restrict__ descs, int r) {
   const FwDesc d = descs[blockIdx.y];
-  if (r >= d.nb) return;
-  const int others = d.nb - 1;
+  if (kPanel * r >= d.nb) return;
+  const int wt = panel_tiles(d, r), p = kPanel * r, others = d.nb - wt;
   int t = blockIdx.x;
-  if (t >= 2 * others) return;
-  const bool is_row = t < others;
-  int o = is_row ? t : t - others;
-  o = o < r ? o : o + 1;
+  if (t >= 2 * wt * others) return;
+  const bool is_row = t < wt * others;
+  if (!is_row) t -= wt * others;
+  const int a = t / others;                       // which 64-slice of the panel
+  const int o = skip_panel(t - a * others, p, wt);  // the tile outside the panel
   __shared__ int As[kKc][kT];
   __shared__ int Bs[kKc][kT];
-  const gint *pivot = tile_ptr(d, r, r);
-  gint *target = is_row ? tile_ptr(d, r, o) : tile_ptr(d, o, r);
+  gint *target = is_row ? tile_ptr(d, p + a, o) : tile_ptr(d, o, p + a);
   const int tr = threadIdx.x >> 3, tc = threadIdx.x & 7;
   int acc[8][8];
   load_acc8(acc, target, d.ld, tr, tc);
-  // row tile: target = P (x) target;  column tile: target = target (x) P.
-  tile_product(acc, is_row ? pivot : target, is_row ? target : pivot, d.ld, As, Bs, tr, tc);
+  // row tile: P*(P_a, P) (x) T(P, o);  column tile: T(o, P) (x) P*(P, P_a).
+  if (is_row)
+    tile_product(acc, tile_ptr(d, p + a, p), tile_ptr(d, p, o), d.ld, wt * kT, As, Bs, tr, tc);
+  else
+    tile_product(acc, tile_ptr(d, o, p), tile_ptr(d, p, p + a), d.ld, wt * kT, As, Bs, tr, tc);
   store_acc8(acc, target, d.ld, tr, tc);
 }
 
+// Phase 3: (nb - wt)^2 tiles per part outside the pivot row and column.
 __global__ __launch_bounds__(64) void fw_phase3_kernel(const FwDesc *__restrict__ descs, int r) {
   const FwDesc d = descs[blockIdx.y];
-  if (r >= d.nb) return;
-  const int others = d.nb - 1;
+  if (kPanel * r >= d.nb) return;
+  const int wt = panel_tiles(d, r), p = kPanel * r, others = d.nb - wt;
   const int t = blockIdx.x;
   if (t >= others * others) return;
-  int i = t / others, j = t - (t / others) * others;
-  i = i < r ? i : i + 1;
-  j = j < r ? j : j + 1;
+  const int i = skip_panel(t / others, p, wt);
+  const int j = skip_panel(t - (t / others) * others, p, wt);
   __shared__ int As[kKc][kT];
   __shared__ int Bs[kKc][kT];
   gint *c = tile_ptr(d, i, j);
   const int tr = threadIdx.x >> 3, tc = threadIdx.x & 7;
   int acc[8][8];
   load_acc8(acc, c, d.ld, tr, tc);
-  tile_product(acc, tile_ptr(d, i, r), tile_ptr(d, r, j), d.ld, As, Bs, tr, tc);
+  tile_product(acc, tile_ptr(d, i, p), tile_ptr(d, p, j), d.ld, wt * kT, As, Bs, tr, tc);
   store_acc8(acc, c, d.ld, tr, tc);
 }
 
+// max_nb: the largest tile count of the descriptors; rounds cover kPanel tiles.
 int launch_fw_round(const FwDesc *d_descs, int ndesc, int max_nb, int r, hipStream_t stream) {
-  if (ndesc <= 0 || r >= max_nb) return FAP_OK;
-  hipLaunchKernelGGL(fw_phase1_kernel, dim3(1, ndesc), dim3(256), 0, stream, d_descs, r);
+  if (ndesc <= 0 || kPanel * r >= max_nb) return FAP_OK;
+  hipLaunchKernelGGL(fw_phase1_kernel, dim3(1, ndesc), dim3(kP1Threads), 0, stream, d_descs, r);
   FAP_CHECK_LAUNCH();
-  const int others = max_nb - 1;
-  if (others > 0) {
-    hipLaunchKernelGGL(fw_phase2_kernel, dim3(2 * others, ndesc), dim3(64), 0, stream, d_descs,
-                       r);
+  // Grid sized for the largest part; smaller parts leave the extra blocks idle.
+  const int wt = std::min(kPanel, max_nb - kPanel * r);
+  int others = max_nb - wt;
+  // A part with fewer tiles may have a full panel where the largest has a short one.
+  if (wt < kPanel) others = std::max(others, max_nb - 1 - kPanel);
+  const int p2 = 2 * kPanel * others;
+  if (others > 0) {  // a single-panel matrix has no tile outside the pivot panel
+    hipLaunchKernelGGL(fw_phase2_kernel, dim3(p2, ndesc), dim3(64), 0, stream, d_descs, r);
     FAP_CHECK_LAUNCH();
     hipLaunchKernelGGL(fw_phase3_kernel, dim3(others * others, ndesc), dim3(64), 0, stream,
                        d_descs, r);
@@ -202,7 +232,8 @@ int launch_fw_round(const FwDesc *d_descs, int ndesc, int max_nb, int r, hipStre
 }
 
 int launch_fw(const FwDesc *d_descs, int ndesc, int max_nb, hipStream_t stream) {
-  for (int r = 0; r < max_nb; ++r) FAP_TRY(launch_fw_round(d_descs, ndesc, max_nb, r, stream));
+  const int rounds = (max_nb + kPanel - 1) / kPanel;
+  for (int r = 0; r < rounds; ++r) FAP_TRY(launch_fw_round(d_descs, ndesc, max_nb, r, stream));
   return FAP_OK;
 }
 

@@ -198,7 +198,7 @@ __device__ __forceinline__ void stream_row(int (&acc)[4], const FAP_CONST int *a
 #pragma unroll
   for (int k = 0; k < KP; k += 2)
 #pragma unroll
-    for (int e = 0; e < 4; ++e) acc[e] = min(acc[e], min(a[k] + b[k][e], a[k + 1] + b[k + 1][e]));
+    for (int e = 0; e < 4; ++e) acc[e] = min3_i32(acc[e], a[k] + b[k][e], a[k + 1] + b[k + 1][e]);
 }
 
 template <bool kVec>

@@ -17,6 +17,16 @@
 
 namespace fap_hip {
 
+// acc = min(acc, x, y) as exactly one v_min3_i32.  Written as a one-instruction ALU
+// asm statement because the compiler otherwise re-associates the running minimum
+// across unrolled k-steps into v_min + v_min3 pairs (1.75 instead of 1.5 VALU
+// instructions per min-plus step).  Register operands only; no memory access.
+__device__ __forceinline__ int min3_i32(int acc, int x, int y) {
+  int r;
+  asm("v_min3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(acc), "v"(x), "v"(y));
+  return r;
+}
+
 template <int N>
 __device__ __forceinline__ void lds_load_groups(int (&dst)[N], const int *src, int stride) {
 #pragma unroll
@@ -41,7 +51,7 @@ __device__ __forceinline__ void mp_step2(int (&acc)[TM][TN], const int *As, int 
 #pragma unroll
   for (int r = 0; r < TM; ++r)
 #pragma unroll
-    for (int c = 0; c < TN; ++c) acc[r][c] = min(acc[r][c], min(a0[r] + b0[c], a1[r] + b1[c]));
+    for (int c = 0; c < TN; ++c) acc[r][c] = min3_i32(acc[r][c], a0[r] + b0[c], a1[r] + b1[c]);
 }
 
 // acc[r][c] = min(acc[r][c], min_{k < KSTEPS} A[k][row r] + B[k][col c]), KSTEPS even.

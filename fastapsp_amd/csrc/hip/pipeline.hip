@@ -11,12 +11,12 @@
 //
 // PLAIN   launches the five steps part after part (the reference's structure:
 //         solve() calls run() once per part, fap/fap.cpp:512-516).
-// STAGED  launches every step once for all parts: one SSSP grid over all boundary
+// BATCHED launches every step once for all parts: one SSSP grid over all boundary
 //         sources, one build grid over all rows, three FW launches per round for
 //         all parts, one min-plus launch per padded-K group, one decode grid.
-// BATCHED runs SSSP and build as in STAGED, then splits the parts into chunks
-//         on two internal streams: the FW rounds of one chunk (VALU-bound) run
-//         beside the min-plus stream of another (HBM-bound).
+// OVERLAP runs SSSP and build as in BATCHED, then splits the parts into chunks
+//         on two internal streams, so the FW rounds of one chunk run beside the
+//         min-plus of another (measured slower on MI355X: both contend for memory).
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -117,7 +117,7 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
   }
   FAP_REQUIRE(g && nparts >= 0 && (nparts == 0 || parts) && d_out,
               "fap_hip_solve_parts: null argument");
-  FAP_REQUIRE(mode == FAP_MODE_PLAIN || mode == FAP_MODE_BATCHED || mode == FAP_MODE_STAGED,
+  FAP_REQUIRE(mode == FAP_MODE_PLAIN || mode == FAP_MODE_BATCHED || mode == FAP_MODE_OVERLAP,
               "fap_hip_solve_parts: unknown mode %d", mode);
   const int n = g->n;
   FAP_REQUIRE(n > 0 && g->m >= 0 && ld_out >= n && g->d_row_offset && g->d_graph_id &&
@@ -144,7 +144,7 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
   g_spans.clear();
   g_total[0] = g_total[1] = nullptr;
   if (P == 0) return FAP_OK;
-  const bool overlap = mode == FAP_MODE_BATCHED;
+  const bool overlap = mode == FAP_MODE_OVERLAP;
   const bool batched = mode != FAP_MODE_PLAIN;  // one descriptor table for all parts
   const int nchunks = overlap ? std::min(kChunks, P) : 1;
   std::vector<int> chunk_of(P);

@@ -191,7 +191,7 @@ __global__ __launch_bounds__(64 * kSsspWaves) void sssp_rows_kernel(
 // vertices per pass and issues all their loads before using any.
 constexpr int kEllW = 4;
 constexpr int kBfsQ = 1024;
-constexpr int kBfsU = 8;  // frontier vertices per lane per pass (512 per wave)
+constexpr int kBfsU = 4;  // frontier vertices per lane per pass (256 per wave)
 constexpr int kBfsLdsMaxN = 1 << 20;
 
 __global__ void ell_build_kernel(const int *__restrict__ row_off, const int *__restrict__ col, int n,

@@ -177,7 +177,7 @@ bool fapGraph::isSplit() { return is_split; }
 std::vector<int> fapGraph::getGraphId() { return graph_id; }
 
 void fapGraph::setMode(int32_t m) {
-  if (m != FAP_MODE_PLAIN && m != FAP_MODE_BATCHED && m != FAP_MODE_STAGED)
+  if (m != FAP_MODE_PLAIN && m != FAP_MODE_BATCHED && m != FAP_MODE_OVERLAP)
     throw FapError(FAP_ERR_INVALID, "unknown mode");
   mode = m;
 }

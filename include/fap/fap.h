@@ -115,7 +115,7 @@ class fapGraph {
   bool isSplit();
   std::vector<int> getGraphId();
 
-  // Execution form of solve(): FAP_MODE_BATCHED (default), FAP_MODE_STAGED or
+  // Execution form of solve(): FAP_MODE_BATCHED (default), FAP_MODE_OVERLAP or
   // FAP_MODE_PLAIN (identical results).
   void setMode(int32_t mode);
 

@@ -40,12 +40,14 @@ enum fap_status {
  * PLAIN:   one launch sequence per part (SSSP, build, FW rounds, min-plus,
  *          decode), parts in turn -- the structure of fapGraph::run
  *          (fap/fap.cpp:195-503) called from solve() (fap/fap.cpp:507-518).
- * BATCHED: boundary SSSP and build of all parts in one grid each; then the parts
- *          in chunks on two internal streams, so the Floyd-Warshall rounds of one
- *          chunk (VALU-bound) overlap the min-plus stream of another (HBM-bound).
- * STAGED:  every stage launched once for all parts, stages one after another
- *          (no overlap) -- per-stage device times are those of each kernel alone. */
-enum fap_mode { FAP_MODE_PLAIN = 0, FAP_MODE_BATCHED = 1, FAP_MODE_STAGED = 2 };
+ * BATCHED: every stage launched once for all parts (one SSSP grid over all
+ *          boundary sources, one build grid, three FW launches per round for all
+ *          parts, one min-plus launch per padded-K group), stages in turn.
+ * OVERLAP: SSSP and build as in BATCHED, then the parts in chunks on two internal
+ *          streams so one chunk's FW rounds run beside another chunk's min-plus.
+ *          On MI355X both stages are limited by the memory system and contend, so
+ *          this form measures slower than BATCHED; it is kept for comparison. */
+enum fap_mode { FAP_MODE_PLAIN = 0, FAP_MODE_BATCHED = 1, FAP_MODE_OVERLAP = 2 };
 
 /* ------------------------------------------------------------------ runtime */
 
@@ -156,7 +158,7 @@ typedef struct fap_part_desc {
 
 /* Device time of the last fap_hip_solve_parts call (ms): total wall time on the
  * stream, and per stage the summed duration of that stage's launches (in the
- * BATCHED form stages of different chunks overlap, so the stages can sum to more
+ * OVERLAP form stages of different chunks overlap, so the stages can sum to more
  * than the total). */
 typedef struct fap_stage_times {
   float sssp_ms, build_ms, floyd_ms, minplus_ms, decode_ms, total_ms;

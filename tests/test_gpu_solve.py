@@ -30,7 +30,7 @@ def _check_full(g, mode):
     return got
 
 
-@pytest.mark.parametrize("mode", [F.FAP_MODE_BATCHED, F.FAP_MODE_STAGED, F.FAP_MODE_PLAIN])
+@pytest.mark.parametrize("mode", [F.FAP_MODE_BATCHED, F.FAP_MODE_OVERLAP, F.FAP_MODE_PLAIN])
 def test_solve_road_grid_k8(mode):
     """64 x 64 road grid (10% edges dropped, 2% diagonals), K = 8, full n x n."""
     say(f"solve road grid 64x64 K=8 mode={mode}")
@@ -40,11 +40,11 @@ def test_solve_road_grid_k8(mode):
 
 
 def test_solve_forms_agree():
-    say("plain == staged == batched")
+    say("plain == batched == overlap")
     g = F.FapGraph.road_grid(40, 30, 0.15, 0.05, 3, 19)
     g.preCondition()
     a = _check_full(g, F.FAP_MODE_PLAIN)
-    b = _check_full(g, F.FAP_MODE_STAGED)
+    b = _check_full(g, F.FAP_MODE_OVERLAP)
     c = _check_full(g, F.FAP_MODE_BATCHED)
     assert np.array_equal(a, b) and np.array_equal(b, c)
 
@@ -55,7 +55,7 @@ def test_solve_repeated_and_mixed_forms():
     g = F.FapGraph.road_grid(36, 36, 0.1, 0.02, 8, 12)
     g.preCondition()
     first = _check_full(g, F.FAP_MODE_BATCHED)
-    for mode in (F.FAP_MODE_STAGED, F.FAP_MODE_BATCHED, F.FAP_MODE_PLAIN, F.FAP_MODE_BATCHED):
+    for mode in (F.FAP_MODE_OVERLAP, F.FAP_MODE_BATCHED, F.FAP_MODE_PLAIN, F.FAP_MODE_BATCHED):
         g.setMode(mode)
         g.solve()
         assert np.array_equal(g.getAllDistanceI32(), first)
