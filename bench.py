@@ -35,9 +35,7 @@ def log(*a):
 
 
 def stage_times():
-    t = L.FapStageTimes()
-    L.check(L.load().fap_hip_last_stage_times(C.byref(t)), "stage times")
-    return {k: getattr(t, k) for k, _ in L.FapStageTimes._fields_}
+    return L.last_stage_times()
 
 
 def make_graph(cfg):
@@ -158,6 +156,19 @@ def main():
     log(f"{args.mode}: {ms_per_step:.2f} ms/step, {value:.3e} entries/s")
 
     avg = {k: float(np.mean([s[k] for s in stages])) for k in stages[0]}
+    fw16 = stages[-1]["parts_fw16"] > 0
+    log(f"stages (ms): sssp {avg['sssp_ms']:.2f} build {avg['build_ms']:.2f} "
+        f"fw {avg['floyd_ms']:.2f} merge {avg['minplus_ms']:.2f} decode {avg['decode_ms']:.2f}; "
+        f"16-bit FW parts {stages[-1]['parts_fw16']}, redone {stages[-1]['parts_redone']}")
+    # Floyd-Warshall stage: 2 ops (add, min) per step, sum over parts of s_p^3 steps,
+    # against the VALU peak of the arithmetic it ran in (packed 16-bit: two entries
+    # per lane-op).
+    fw_ops = 2.0 * float((num.astype(np.float64) ** 3).sum())
+    fw_peak = 2 * VALU_PEAK_TOPS if fw16 else VALU_PEAK_TOPS
+    fw_tops = fw_ops / (avg["floyd_ms"] * 1e-3) / 1e12
+    fw_roof = {"form": "packed u16 (v_pk_add_u16 + v_pk_minimum3_f16)" if fw16 else "int32",
+               "ops": fw_ops, "ms": avg["floyd_ms"], "TOPs": fw_tops, "peak_TOPs": fw_peak,
+               "frac": fw_tops / fw_peak}
 
     # Roofline of the dominant kernel: the merge stage of the timed batched steps,
     # whose stages run one after another, so the stage time is the kernels' own.
@@ -195,6 +206,15 @@ def main():
                      "valu_frac": valu_ach / VALU_PEAK_TOPS,
                      "floor_ms": {"hbm": t_hbm * 1e3, "valu": t_valu * 1e3}})
 
+    # The same solve with int32 Floyd-Warshall everywhere (one step), for comparison.
+    g.setMode(modes[args.mode] | F.FAP_MODE_FLAG_INT32_FW)
+    g.solve()
+    st32 = stage_times()
+    g.setMode(modes[args.mode])
+    log(f"int32 FW form: FW {st32['floyd_ms']:.2f} ms, total {st32['total_ms']:.2f} ms")
+    fw_roof["int32_form_ms"] = st32["floyd_ms"]
+    fw_roof["int32_form_total_ms"] = st32["total_ms"]
+
     plain = None
     if not args.skip_plain:
         g.setMode(F.FAP_MODE_PLAIN)
@@ -220,7 +240,8 @@ def main():
            "config": {"workload": workload, "n": n, "m": g.num_edges, "K": g.K,
                       "boundary_total": int(bdy.sum()), "mode": args.mode,
                       "result_bytes_in_hbm": int(n) * int(n) * 4},
-           "stages_ms": avg, "overlap_form": {"ms_per_step": overlap_ms},
+           "stages_ms": avg, "floyd_warshall": fw_roof,
+           "overlap_form": {"ms_per_step": overlap_ms},
            "roofline": roofline, "cpu_baseline": cpu, "plain_form": plain}
     print(json.dumps(out), flush=True)
 

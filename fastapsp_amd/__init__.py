@@ -4,9 +4,9 @@ The compute path is libfap_hip.so (hand-written HIP for gfx950 behind the C-ABI 
 include/fap_hip.h); this package holds the ctypes bindings and a Python mirror of
 the reference's fap::fapGraph API.
 """
-from ._lib import (FAP_INF, FAP_MODE_BATCHED, FAP_MODE_PLAIN, FAP_MODE_OVERLAP, MAXVALUE,
+from ._lib import (FAP_INF, FAP_MODE_BATCHED, FAP_MODE_PLAIN, FAP_MODE_OVERLAP, FAP_MODE_FLAG_INT32_FW, MAXVALUE,
                    DeviceBuffer, FapError, init, load)
 from .graph import FapGraph
 
 __all__ = ["FapGraph", "FapError", "DeviceBuffer", "init", "load", "FAP_INF", "MAXVALUE",
-           "FAP_MODE_PLAIN", "FAP_MODE_BATCHED", "FAP_MODE_OVERLAP"]
+           "FAP_MODE_PLAIN", "FAP_MODE_BATCHED", "FAP_MODE_OVERLAP", "FAP_MODE_FLAG_INT32_FW"]

@@ -28,6 +28,7 @@ FAP_ERR_NODEV = 7
 FAP_MODE_PLAIN = 0
 FAP_MODE_BATCHED = 1
 FAP_MODE_OVERLAP = 2
+FAP_MODE_FLAG_INT32_FW = 0x100
 
 # Every exported symbol of include/fap_hip.h: name -> (restype, argtypes).
 _i32, _i64, _u64, _f64 = C.c_int32, C.c_int64, C.c_uint64, C.c_double
@@ -40,7 +41,7 @@ _pf64 = C.POINTER(C.c_double)
 
 class FapGraphDev(C.Structure):
     _fields_ = [("n", _i32), ("m", _i64), ("d_row_offset", _vp), ("d_col", _vp), ("d_w", _vp),
-                ("d_graph_id", _vp), ("d_st2ed", _vp), ("d_ed2st", _vp)]
+                ("d_graph_id", _vp), ("d_st2ed", _vp), ("d_ed2st", _vp), ("d_comp", _vp)]
 
 
 class FapPartDesc(C.Structure):
@@ -50,7 +51,8 @@ class FapPartDesc(C.Structure):
 
 class FapStageTimes(C.Structure):
     _fields_ = [("sssp_ms", C.c_float), ("build_ms", C.c_float), ("floyd_ms", C.c_float),
-                ("minplus_ms", C.c_float), ("decode_ms", C.c_float), ("total_ms", C.c_float)]
+                ("minplus_ms", C.c_float), ("decode_ms", C.c_float), ("total_ms", C.c_float),
+                ("parts_fw16", _i32), ("parts_redone", _i32)]
 
 
 SIGNATURES = {
@@ -152,6 +154,14 @@ def init(device: int = 0) -> None:
     if not _initialised:
         check(load().fap_hip_init(device), "fap_hip_init")
         _initialised = True
+
+
+def last_stage_times() -> dict:
+    """fap_hip_last_stage_times of the last solve: per-stage device ms, the total, and
+    the counts of parts run in the 16-bit Floyd-Warshall form and redone in int32."""
+    t = FapStageTimes()
+    check(load().fap_hip_last_stage_times(C.byref(t)), "stage times")
+    return {k: getattr(t, k) for k, _ in FapStageTimes._fields_}
 
 
 def ptr_i32(a: np.ndarray):

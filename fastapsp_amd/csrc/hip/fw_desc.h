@@ -12,6 +12,30 @@ struct FwDesc {
   int nb;
 };
 
+// Cap of the 16-bit Floyd-Warshall: an entry holds min(distance, kCap16).  Twice the
+// cap is 0x7C00, the bit pattern of +inf in IEEE half precision, so every stored
+// value and every sum of two stays a non-negative half whose bit pattern orders as
+// the integer does (floyd16.hip relies on this for v_pk_minimum3_f16).
+constexpr int kCap16 = 0x3E00;
+
+// The same matrix held as 16-bit distances (ld in elements, rows 128-byte aligned).
+struct Fw16Desc {
+  uint16_t *mat;
+  int64_t ld;
+  int nb;
+};
+
+// Expansion of one part's 16-bit result into its int32 FW matrix (both S x S,
+// ld = S), 8 entries per work item.  Entries equal to the cap become FAP_INF; a
+// capped entry between two vertices of one connected component marks the part in
+// flags[part_index] (its true distance is >= the cap: the part is redone in int32).
+struct UnpackDesc {
+  const uint16_t *src;
+  int *dst;
+  int S, s, offset, part_index;
+  int64_t item_begin;  // prefix over parts of S * S / 8
+};
+
 // One min-plus product C = A (x) B with overwrite semantics:
 //   C[i][j] = min_k A[i][k] + B[k][j]  (FAP_INF when K == 0), i < M, j < N.
 // When part_id > 0 the epilogue also performs the decode of the part's own
@@ -38,6 +62,7 @@ struct MergeDesc {
 struct BuildDesc {
   int *fw;
   int64_t ldf;
+  uint16_t *fw16;   // non-null: write the capped 16-bit matrix here (ld = S) instead
   int S;            // padded edge (multiple of 64, <= kBuildMaxS)
   int s, bdy, offset, part_id;
   int row_begin;    // first global build row of this part

@@ -29,6 +29,7 @@ __device__ __forceinline__ int find_by_begin(const D *__restrict__ d, int nd, in
 //   row i >= bdy : FAP_INF, then the part's own out-arcs of st2ed[offset + i] in
 //                  CSR order (a later arc overwrites an earlier one)
 //   diagonal     : 0;  padding (j >= s or i >= s): FAP_INF.
+// With d.fw16 set, the row is written as 16-bit entries min(value, kCap16).
 template <bool kVec>
 __global__ __launch_bounds__(256) void build_kernel(const BuildDesc *__restrict__ descs, int nd,
                                                     int total_rows,
@@ -66,6 +67,16 @@ __global__ __launch_bounds__(256) void build_kernel(const BuildDesc *__restrict_
     rowbuf[i] = 0;
   }
   __syncthreads();
+  if (d.fw16) {
+    // Capped 16-bit row, two entries per 32-bit word (S is a multiple of 64).
+    unsigned *dst16 = reinterpret_cast<unsigned *>(d.fw16 + (int64_t)i * d.S);
+    for (int w = tid; w < d.S / 2; w += 256) {
+      const unsigned lo = (unsigned)min(rowbuf[2 * w], kCap16);
+      const unsigned hi = (unsigned)min(rowbuf[2 * w + 1], kCap16);
+      dst16[w] = lo | (hi << 16);
+    }
+    return;
+  }
   int *dst = d.fw + (int64_t)i * d.ldf;
   if (kVec) {
     for (int j = tid * 4; j < d.S; j += 1024)

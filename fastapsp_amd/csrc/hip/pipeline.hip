@@ -17,6 +17,12 @@
 // OVERLAP runs SSSP and build as in BATCHED, then splits the parts into chunks
 //         on two internal streams, so the FW rounds of one chunk run beside the
 //         min-plus of another (measured slower on MI355X: both contend for memory).
+//
+// With component labels (symmetric graphs) the build writes capped 16-bit
+// matrices, Floyd-Warshall runs in the packed 16-bit form (floyd16.hip) and an
+// unpack step writes the int32 matrices the merge reads.  After the launches the
+// call reads one flag per part; a part whose 16-bit closure reached the cap between
+// two vertices of one component is solved again with int32 Floyd-Warshall.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -30,6 +36,11 @@ int launch_sssp(const int *d_row_off, const int *d_col, const int *d_w, int n, c
                 int nsrc, const int64_t *d_rows, int *d_out, int64_t ld, bool vec_fill,
                 hipStream_t stream);
 int launch_fw(const FwDesc *d_descs, int ndesc, int max_nb, hipStream_t stream);
+int launch_fw16(const Fw16Desc *d_descs, int ndesc, int max_nb, hipStream_t stream);
+int launch_fw16_unpack(const UnpackDesc *d_descs, int nd, int64_t total_items,
+                       const int *d_comp_sorted, int *d_flags, hipStream_t stream);
+int launch_comp_sorted(const int *d_comp, const int *d_st2ed, int n, int *d_out,
+                       hipStream_t stream);
 int launch_ell_build(const int *d_row_off, const int *d_col, int n, int4 *d_ell,
                      hipStream_t stream);
 int launch_bfs_lds(const int4 *d_ell, const int *d_row_off, const int *d_col, int n,
@@ -71,6 +82,7 @@ struct MergeGroup {
 namespace {
 
 Workspace g_ws_fw;    // Floyd-Warshall matrices
+Workspace g_ws_fw16;  // 16-bit Floyd-Warshall matrices
 Workspace g_ws_meta;  // descriptors, sources, row indices
 Workspace g_ws_ell;   // ELL adjacency of the unit-weight graph
 hipStream_t g_last_stream = nullptr;
@@ -99,6 +111,14 @@ struct Span {
 std::vector<Span> g_spans;
 hipEvent_t g_total[2] = {nullptr, nullptr};
 constexpr int kChunks = 8;
+int g_parts16 = 0, g_redone = 0;
+
+// What the last pass left for the flag check of the 16-bit form.
+struct Pass16 {
+  std::vector<fap_part_desc> parts;  // parts solved in the 16-bit form
+  const int *d_flags = nullptr;      // one flag per entry of parts
+};
+Pass16 g_pass16;
 
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
@@ -108,17 +128,11 @@ int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 using namespace fap_hip;
 
-extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *parts,
-                                   int32_t nparts, int32_t *d_out, int64_t ld_out, int32_t mode,
-                                   void *stream) {
-  if (!initialized()) {
-    set_error("fap_hip_init has not been called");
-    return FAP_ERR_STATE;
-  }
-  FAP_REQUIRE(g && nparts >= 0 && (nparts == 0 || parts) && d_out,
-              "fap_hip_solve_parts: null argument");
-  FAP_REQUIRE(mode == FAP_MODE_PLAIN || mode == FAP_MODE_BATCHED || mode == FAP_MODE_OVERLAP,
-              "fap_hip_solve_parts: unknown mode %d", mode);
+// One pass of the pipeline over `parts`.  append: a second pass of the same call
+// (its spans are added to the first pass's, the total spans both).
+static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_t nparts,
+                      int32_t *d_out, int64_t ld_out, int32_t mode, void *stream, bool append) {
+  g_pass16 = Pass16{};
   const int n = g->n;
   FAP_REQUIRE(n > 0 && g->m >= 0 && ld_out >= n && g->d_row_offset && g->d_graph_id &&
                   g->d_st2ed && g->d_ed2st && (g->m == 0 || g->d_col),
@@ -141,11 +155,15 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
     if (q.num > 0) act.push_back(q);
   }
   const int P = (int)act.size();
-  g_spans.clear();
-  g_total[0] = g_total[1] = nullptr;
+  if (!append) {
+    g_spans.clear();
+    g_total[0] = g_total[1] = nullptr;
+  }
   if (P == 0) return FAP_OK;
-  const bool overlap = mode == FAP_MODE_OVERLAP;
-  const bool batched = mode != FAP_MODE_PLAIN;  // one descriptor table for all parts
+  const int form = mode & 0xFF;
+  const bool overlap = form == FAP_MODE_OVERLAP;
+  const bool batched = form != FAP_MODE_PLAIN;  // one descriptor table for all parts
+  const bool use16 = g->d_comp != nullptr && !(mode & FAP_MODE_FLAG_INT32_FW);
   const int nchunks = overlap ? std::min(kChunks, P) : 1;
   std::vector<int> chunk_of(P);
   for (int p = 0; p < P; ++p) chunk_of[p] = (int)((int64_t)p * nchunks / P);
@@ -153,7 +171,7 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
 
   std::vector<int> S(P), nb(P);
   std::vector<int64_t> fw_off(P);
-  int64_t fw_total = 0, fw_max = 0;
+  int64_t fw_total = 0, fw_max = 0;  // entries
   int max_nb = 0;
   for (int p = 0; p < P; ++p) {
     S[p] = (int)align_up(act[p].num, kTile);
@@ -166,23 +184,32 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
   }
   FAP_TRY(g_ws_fw.ensure((batched ? fw_total : fw_max) * 4));
   int *fw_base = static_cast<int *>(g_ws_fw.ptr);
+  uint16_t *fw16_base = nullptr;
+  if (use16) {
+    FAP_TRY(g_ws_fw16.ensure((batched ? fw_total : fw_max) * 2));
+    fw16_base = static_cast<uint16_t *>(g_ws_fw16.ptr);
+  }
 
   std::vector<BuildDesc> bd(P);
   std::vector<FwDesc> fd(P);
   std::vector<MergeDesc> md(P);
   std::vector<DecodeDesc> dd(P);
+  std::vector<Fw16Desc> fd16(P);
+  std::vector<UnpackDesc> ud(P);
   std::vector<int> offsets(P), src_begin(P);
   std::vector<int64_t> out_rows(P);
   int rows_total = 0, src_total = 0;
-  int64_t dec_total = 0;
-  std::vector<int64_t> chunk_dec(nchunks, 0);
+  int64_t dec_total = 0, unpack_total = 0;
+  std::vector<int64_t> chunk_dec(nchunks, 0), chunk_unpack(nchunks, 0);
   for (int p = 0; p < P; ++p) {
     const fap_part_desc &q = act[p];
     int *fw = fw_base + fw_off[p];
+    uint16_t *fw16 = use16 ? fw16_base + fw_off[p] : nullptr;
     const int M = q.num - q.bdy;
-    bd[p] = BuildDesc{fw, S[p], S[p], q.num, q.bdy, q.offset, q.part_id,
+    bd[p] = BuildDesc{fw, S[p], fw16, S[p], q.num, q.bdy, q.offset, q.part_id,
                       batched ? rows_total : 0, d_out + q.out_row * ld_out, ld_out};
     fd[p] = FwDesc{fw, S[p], nb[p]};
+    fd16[p] = Fw16Desc{fw16, S[p], nb[p]};
     MergeDesc m{};
     m.A = fw + (int64_t)q.bdy * S[p];
     m.lda = S[p];
@@ -200,7 +227,10 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
     m.ldf = S[p];
     m.fw_row0 = q.bdy;
     md[p] = m;
-    if (p > 0 && chunk_of[p] != chunk_of[p - 1]) dec_total = 0;
+    if (p > 0 && chunk_of[p] != chunk_of[p - 1]) dec_total = unpack_total = 0;
+    ud[p] = UnpackDesc{fw16, fw, S[p], q.num, q.offset, p, batched ? unpack_total : 0};
+    unpack_total += (int64_t)S[p] * S[p] / 8;
+    chunk_unpack[chunk_of[p]] += (int64_t)S[p] * S[p] / 8;
     dd[p] = DecodeDesc{fw, S[p], d_out + q.out_row * ld_out, ld_out, q.bdy, q.num, q.offset,
                        batched ? dec_total : 0};
     offsets[p] = q.offset;
@@ -254,9 +284,11 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
   const int64_t o_bd = carve(sizeof(BuildDesc) * P), o_fd = carve(sizeof(FwDesc) * P);
   const int64_t o_md = carve(sizeof(MergeDesc) * std::max(nmd, 1));
   const int64_t o_dd = carve(sizeof(DecodeDesc) * P);
+  const int64_t o_fd16 = carve(sizeof(Fw16Desc) * P), o_ud = carve(sizeof(UnpackDesc) * P);
   const int64_t o_off = carve(4LL * P), o_sb = carve(4LL * P), o_or = carve(8LL * P);
   const int64_t o_src = carve(4LL * std::max(src_total, 1));
   const int64_t o_rows = carve(8LL * std::max(src_total, 1));
+  const int64_t o_flags = carve(4LL * P), o_comp = carve(4LL * n);  // device-written
   FAP_TRY(g_ws_meta.ensure(off));
   char *meta = static_cast<char *>(g_ws_meta.ptr);
   std::vector<char> host(off);
@@ -264,6 +296,8 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
   std::memcpy(host.data() + o_fd, fd.data(), sizeof(FwDesc) * P);
   if (nmd) std::memcpy(host.data() + o_md, mdg.data(), sizeof(MergeDesc) * nmd);
   std::memcpy(host.data() + o_dd, dd.data(), sizeof(DecodeDesc) * P);
+  std::memcpy(host.data() + o_fd16, fd16.data(), sizeof(Fw16Desc) * P);
+  std::memcpy(host.data() + o_ud, ud.data(), sizeof(UnpackDesc) * P);
   std::memcpy(host.data() + o_off, offsets.data(), 4LL * P);
   std::memcpy(host.data() + o_sb, src_begin.data(), 4LL * P);
   std::memcpy(host.data() + o_or, out_rows.data(), 8LL * P);
@@ -278,10 +312,16 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
   const int64_t *d_or = reinterpret_cast<const int64_t *>(meta + o_or);
   int *d_src = reinterpret_cast<int *>(meta + o_src);
   int64_t *d_rows = reinterpret_cast<int64_t *>(meta + o_rows);
+  const Fw16Desc *d_fd16 = reinterpret_cast<const Fw16Desc *>(meta + o_fd16);
+  const UnpackDesc *d_ud = reinterpret_cast<const UnpackDesc *>(meta + o_ud);
+  int *d_flags = reinterpret_cast<int *>(meta + o_flags);
+  int *d_comp_sorted = reinterpret_cast<int *>(meta + o_comp);
 
   // ---- launches -------------------------------------------------------------
-  g_events.used = 0;
-  g_spans.clear();
+  if (!append) {
+    g_events.used = 0;
+    g_spans.clear();
+  }
   auto event = [&](hipStream_t on, hipEvent_t *e) -> int {
     FAP_TRY(g_events.take(e));
     FAP_CHECK_HIP(hipEventRecord(*e, on));
@@ -296,7 +336,18 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
     g_spans.push_back(sp);
     return FAP_OK;
   };
-  FAP_TRY(event(st, &g_total[0]));
+  if (!append) FAP_TRY(event(st, &g_total[0]));
+  if (use16) {
+    FAP_TRY(fap_hip_fill_i32(d_flags, 0, P, st));
+    FAP_TRY(launch_comp_sorted(g->d_comp, g->d_st2ed, n, d_comp_sorted, st));
+  }
+  // Floyd-Warshall of parts [p0, p1) with max tile count mnb; unpack_items: the
+  // prefix end of those parts' unpack items.
+  auto run_fw = [&](int p0, int p1, int mnb, int64_t unpack_items, hipStream_t on) -> int {
+    if (!use16) return launch_fw(d_fd + p0, p1 - p0, mnb, on);
+    FAP_TRY(launch_fw16(d_fd16 + p0, p1 - p0, mnb, on));
+    return launch_fw16_unpack(d_ud + p0, p1 - p0, unpack_items, d_comp_sorted, d_flags, on);
+  };
 
   // Boundary SSSP: the LDS-bitmap BFS for unit weights, else label correcting.
   int4 *ell = nullptr;
@@ -346,7 +397,8 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
       int64_t cdec = 0;
       for (int p = p0; p < p1; ++p) cmax_nb = std::max(cmax_nb, nb[p]);
       cdec = overlap ? chunk_dec[c] : dec_total;
-      FAP_TRY(span(2, on, [&](hipStream_t s2) { return launch_fw(d_fd + p0, p1 - p0, cmax_nb, s2); }));
+      const int64_t cunpack = overlap ? chunk_unpack[c] : unpack_total;
+      FAP_TRY(span(2, on, [&](hipStream_t s2) { return run_fw(p0, p1, cmax_nb, cunpack, s2); }));
       FAP_TRY(span(3, on, [&](hipStream_t s2) -> int {
         for (const MergeGroup &gr : groups)
           if (gr.chunk == c) FAP_TRY(run_group(gr, s2));
@@ -374,7 +426,9 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
         return launch_build(d_bd + p, 1, S[p], true, g->d_row_offset, g->d_col, g->d_w,
                             g->d_graph_id, g->d_st2ed, g->d_ed2st, on);
       }));
-      FAP_TRY(span(2, st, [&](hipStream_t on) { return launch_fw(d_fd + p, 1, nb[p], on); }));
+      FAP_TRY(span(2, st, [&](hipStream_t on) {
+        return run_fw(p, p + 1, nb[p], (int64_t)S[p] * S[p] / 8, on);
+      }));
       FAP_TRY(span(3, st, [&](hipStream_t on) {
         return part_merge[p] >= 0 ? run_group(groups[part_merge[p]], on) : FAP_OK;
       }));
@@ -384,6 +438,47 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
     }
   }
   FAP_TRY(event(st, &g_total[1]));
+  if (use16) {
+    g_pass16.parts = act;
+    g_pass16.d_flags = d_flags;
+    g_parts16 += P;
+  }
+  return FAP_OK;
+}
+
+extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *parts,
+                                   int32_t nparts, int32_t *d_out, int64_t ld_out, int32_t mode,
+                                   void *stream) {
+  if (!initialized()) {
+    set_error("fap_hip_init has not been called");
+    return FAP_ERR_STATE;
+  }
+  FAP_REQUIRE(g && nparts >= 0 && (nparts == 0 || parts) && d_out,
+              "fap_hip_solve_parts: null argument");
+  const int form = mode & 0xFF;
+  FAP_REQUIRE((form == FAP_MODE_PLAIN || form == FAP_MODE_BATCHED || form == FAP_MODE_OVERLAP) &&
+                  (mode & ~(0xFF | FAP_MODE_FLAG_INT32_FW)) == 0,
+              "fap_hip_solve_parts: unknown mode %d", mode);
+  g_parts16 = g_redone = 0;
+  FAP_TRY(solve_pass(g, parts, nparts, d_out, ld_out, mode, stream, false));
+  if (g_pass16.parts.empty()) return FAP_OK;
+  // Flag check of the 16-bit form: parts whose closure reached the cap inside one
+  // component are solved again with int32 Floyd-Warshall.
+  const Pass16 pass = g_pass16;
+  hipStream_t st = resolve(stream);
+  std::vector<int> flags(pass.parts.size());
+  FAP_CHECK_HIP(hipMemcpyAsync(flags.data(), pass.d_flags, 4 * flags.size(),
+                               hipMemcpyDeviceToHost, st));
+  FAP_CHECK_HIP(hipStreamSynchronize(st));
+  std::vector<fap_part_desc> redo;
+  for (size_t i = 0; i < flags.size(); ++i)
+    if (flags[i]) redo.push_back(pass.parts[i]);
+  if (redo.empty()) return FAP_OK;
+  g_redone = (int)redo.size();
+  const int parts16 = g_parts16;
+  FAP_TRY(solve_pass(g, redo.data(), (int32_t)redo.size(), d_out, ld_out,
+                     mode | FAP_MODE_FLAG_INT32_FW, stream, true));
+  g_parts16 = parts16;
   return FAP_OK;
 }
 
@@ -400,6 +495,8 @@ extern "C" int fap_hip_last_stage_times(fap_stage_times *t) {
     }
     FAP_CHECK_HIP(hipEventElapsedTime(&acc.total_ms, g_total[0], g_total[1]));
   }
+  acc.parts_fw16 = g_parts16;
+  acc.parts_redone = g_redone;
   *t = acc;
   return FAP_OK;
 }

@@ -109,10 +109,12 @@ fapGraph::fapGraph(int32_t n, int64_t m, std::vector<int32_t> ro, std::vector<in
 fapGraph::~fapGraph() { release(); }
 
 void fapGraph::release() {
-  int32_t *ptrs[] = {d_row_offset, d_col, d_w, d_graph_id, d_st2ed, d_ed2st, d_dist, d_sub_dist};
+  int32_t *ptrs[] = {d_row_offset, d_col, d_w,  d_graph_id, d_st2ed,
+                     d_ed2st,      d_comp, d_dist, d_sub_dist};
   for (int32_t *p : ptrs)
     if (p) fap_hip_free(p);
-  d_row_offset = d_col = d_w = d_graph_id = d_st2ed = d_ed2st = d_dist = d_sub_dist = nullptr;
+  d_row_offset = d_col = d_w = d_graph_id = d_st2ed = d_ed2st = d_comp = d_dist = d_sub_dist =
+      nullptr;
 }
 
 void fapGraph::setPartition(const std::vector<int> &ids) {
@@ -149,6 +151,34 @@ void fapGraph::preCondition() {
   preconditioned = true;
 }
 
+// Connected-component labels when every arc u->v has a reverse arc v->u (then
+// reachability is symmetric, and two vertices are mutually unreachable exactly when
+// their labels differ); empty for any other graph.
+static std::vector<int32_t> symmetric_components(int32_t n, const std::vector<int32_t> &ro,
+                                                 const std::vector<int32_t> &col) {
+  std::vector<int32_t> sorted(col);
+  for (int32_t u = 0; u < n; ++u) std::sort(sorted.begin() + ro[u], sorted.begin() + ro[u + 1]);
+  for (int32_t u = 0; u < n; ++u)
+    for (int32_t e = ro[u]; e < ro[u + 1]; ++e) {
+      const int32_t v = sorted[e];
+      if (!std::binary_search(sorted.begin() + ro[v], sorted.begin() + ro[v + 1], u)) return {};
+    }
+  std::vector<int32_t> parent(n);
+  for (int32_t v = 0; v < n; ++v) parent[v] = v;
+  auto find = [&](int32_t v) {
+    while (parent[v] != v) v = parent[v] = parent[parent[v]];
+    return v;
+  };
+  for (int32_t u = 0; u < n; ++u)
+    for (int32_t e = ro[u]; e < ro[u + 1]; ++e) {
+      const int32_t a = find(u), b = find(col[e]);
+      if (a != b) parent[std::max(a, b)] = std::min(a, b);
+    }
+  std::vector<int32_t> label(n);
+  for (int32_t v = 0; v < n; ++v) label[v] = find(v);
+  return label;
+}
+
 void fapGraph::upload() {
   check(fap_hip_init(0), "fap_hip_init");
   release();
@@ -170,6 +200,8 @@ void fapGraph::upload() {
   put(&d_graph_id, graph_id.data(), n);
   put(&d_st2ed, st2ed.data(), n);
   put(&d_ed2st, ed2st.data(), n);
+  const std::vector<int32_t> comp = symmetric_components(num_vertexs, row_offset, col_val);
+  if (!comp.empty()) put(&d_comp, comp.data(), n);
   ld_dist = align_up(n, 64);
 }
 
@@ -177,7 +209,9 @@ bool fapGraph::isSplit() { return is_split; }
 std::vector<int> fapGraph::getGraphId() { return graph_id; }
 
 void fapGraph::setMode(int32_t m) {
-  if (m != FAP_MODE_PLAIN && m != FAP_MODE_BATCHED && m != FAP_MODE_OVERLAP)
+  const int32_t form = m & 0xFF;
+  if ((form != FAP_MODE_PLAIN && form != FAP_MODE_BATCHED && form != FAP_MODE_OVERLAP) ||
+      (m & ~(0xFF | FAP_MODE_FLAG_INT32_FW)) != 0)
     throw FapError(FAP_ERR_INVALID, "unknown mode");
   mode = m;
 }
@@ -202,7 +236,8 @@ int32_t fapGraph::solve(bool path_needed) {
   for (int p = 1; p <= K; ++p)
     parts.push_back(fap_part_desc{p, C_BlockVer_offset[p], C_BlockVer_num[p], C_BlockBdy_num[p],
                                   (int64_t)C_BlockVer_offset[p]});
-  fap_graph_dev g{num_vertexs, num_edges, d_row_offset, d_col, d_w, d_graph_id, d_st2ed, d_ed2st};
+  fap_graph_dev g{num_vertexs, num_edges, d_row_offset, d_col, d_w,
+                  d_graph_id,  d_st2ed,   d_ed2st,      d_comp};
   check(fap_hip_solve_parts(&g, parts.data(), (int32_t)parts.size(), d_dist, ld_dist, mode, nullptr),
         "solve");
   check(fap_hip_stream_sync(nullptr), "solve");
@@ -219,7 +254,8 @@ int32_t fapGraph::solveSubGraph(int32_t id, bool path_needed) {
   if (!d_sub_dist)
     check(fap_hip_malloc((void **)&d_sub_dist, rows * ld_dist * 4), "sub-graph allocation");
   fap_part_desc part{id, C_BlockVer_offset[id], C_BlockVer_num[id], C_BlockBdy_num[id], 0};
-  fap_graph_dev g{num_vertexs, num_edges, d_row_offset, d_col, d_w, d_graph_id, d_st2ed, d_ed2st};
+  fap_graph_dev g{num_vertexs, num_edges, d_row_offset, d_col, d_w,
+                  d_graph_id,  d_st2ed,   d_ed2st,      d_comp};
   check(fap_hip_solve_parts(&g, &part, 1, d_sub_dist, ld_dist, mode, nullptr), "solveSubGraph");
   check(fap_hip_stream_sync(nullptr), "solveSubGraph");
   add_stage_times();

@@ -75,6 +75,7 @@ class fapGraph {
   // Device residency (owned).
   int32_t *d_row_offset = nullptr, *d_col = nullptr, *d_w = nullptr;
   int32_t *d_graph_id = nullptr, *d_st2ed = nullptr, *d_ed2st = nullptr;
+  int32_t *d_comp = nullptr;      // component labels (symmetric graphs only)
   int32_t *d_dist = nullptr;      // n x ld_dist, result of solve()
   int32_t *d_sub_dist = nullptr;  // s_p x ld_dist, result of solveSubGraph()
   int64_t ld_dist = 0;

@@ -48,6 +48,9 @@ enum fap_status {
  *          On MI355X both stages are limited by the memory system and contend, so
  *          this form measures slower than BATCHED; it is kept for comparison. */
 enum fap_mode { FAP_MODE_PLAIN = 0, FAP_MODE_BATCHED = 1, FAP_MODE_OVERLAP = 2 };
+/* Flag OR-ed into a mode: run every Floyd-Warshall in int32 even where the 16-bit
+ * form applies (see fap_hip_solve_parts).  Results are identical either way. */
+#define FAP_MODE_FLAG_INT32_FW 0x100
 
 /* ------------------------------------------------------------------ runtime */
 
@@ -144,6 +147,10 @@ typedef struct fap_graph_dev {
   const int32_t *d_graph_id;   /* [n], part ids 1..K */
   const int32_t *d_st2ed;      /* [n], sorted position -> vertex */
   const int32_t *d_ed2st;      /* [n], vertex -> sorted position */
+  /* [n] connected-component label of each vertex, or NULL.  Only for graphs in which
+   * every arc u->v has a reverse arc v->u (so reachability is symmetric); it enables
+   * the 16-bit Floyd-Warshall of fap_hip_solve_parts. */
+  const int32_t *d_comp;
 } fap_graph_dev;
 
 /* One part: C_BlockVer_offset / C_BlockVer_num / C_BlockBdy_num of
@@ -162,6 +169,8 @@ typedef struct fap_part_desc {
  * than the total). */
 typedef struct fap_stage_times {
   float sssp_ms, build_ms, floyd_ms, minplus_ms, decode_ms, total_ms;
+  int32_t parts_fw16;   /* parts whose Floyd-Warshall ran in the 16-bit form */
+  int32_t parts_redone; /* of those, parts redone in int32 (a distance reached the cap) */
 } fap_stage_times;
 
 /* The whole per-part pipeline of fapGraph::run (fap/fap.cpp:195-503) for a list
@@ -170,7 +179,15 @@ typedef struct fap_stage_times {
  * into the min-plus epilogue.  Part p writes rows out_row .. out_row + s_p - 1 of
  * d_out (row stride ld_out >= n; ld_out % 4 == 0 and 16-byte aligned d_out give
  * vector stores).  Host-side descriptors; the call is asynchronous on `stream`.
- * mode: FAP_MODE_PLAIN or FAP_MODE_BATCHED (identical results). */
+ * mode: FAP_MODE_PLAIN, FAP_MODE_BATCHED or FAP_MODE_OVERLAP, optionally with
+ * FAP_MODE_FLAG_INT32_FW (identical results).
+ * 16-bit Floyd-Warshall: when g->d_comp is set (and the flag is absent), every part's
+ * matrix is closed on 16-bit entries min(d, 15872) (packed VALU arithmetic, twice
+ * the int32 rate).  Entries below the cap are exact; an entry at the cap is
+ * unreachable when its two vertices lie in different components.  A part in which
+ * a capped entry joins two vertices of one component is redone in int32 before the
+ * call returns, so the results never differ from the int32 form.  In this form the
+ * call waits for the stream before it returns (it reads the per-part flags). */
 int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *parts, int32_t nparts,
                         int32_t *d_out, int64_t ld_out, int32_t mode, void *stream);
 int fap_hip_last_stage_times(fap_stage_times *t);

@@ -3,6 +3,7 @@
 import ctypes as C
 import os
 import re
+import subprocess
 
 import fastapsp_amd as F
 from fastapsp_amd import _lib as L
@@ -32,11 +33,29 @@ def test_bindings_cover_every_declared_symbol():
     assert sorted(L.SIGNATURES) == declared_functions()
 
 
-def test_struct_layouts_match_header():
-    # fap_graph_dev: int32 n, int64 m, 6 pointers; fap_part_desc: 4 x int32 + int64.
-    assert C.sizeof(L.FapGraphDev) == 8 + 8 + 6 * 8
-    assert C.sizeof(L.FapPartDesc) == 24
-    assert C.sizeof(L.FapStageTimes) == 24
+def test_struct_layouts_match_header(tmp_path):
+    """sizeof and every field offset of the ctypes structs equal what a C compiler
+    reads from include/fap_hip.h."""
+    structs = {"fap_graph_dev": L.FapGraphDev, "fap_part_desc": L.FapPartDesc,
+               "fap_stage_times": L.FapStageTimes}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fap_hip.h"',
+             "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'  printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for field, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out.splitlines()}
+    for cname, cls in structs.items():
+        assert got[(cname, "sizeof")] == C.sizeof(cls), cname
+        for field, _ in cls._fields_:
+            assert got[(cname, field)] == getattr(cls, field).offset, (cname, field)
 
 
 def test_status_codes_without_device():
