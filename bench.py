@@ -49,6 +49,18 @@ def make_graph(cfg):
     raise SystemExit(f"unknown config {cfg}")
 
 
+def merge_traffic(cfg):
+    """HBM bytes per solve of the merge kernels from the committed FETCH_SIZE /
+    WRITE_SIZE summary of this workload (calibrated counters), or None."""
+    path = os.path.join(ROOT, "profiles", "merge_road114599_k128_pmc_traffic.md")
+    if cfg != "road1" or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("merge_traffic_bytes_per_solve:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(g, budget_s):
     """The oracle's CPU path (reference structure: OpenMP Dijkstra over boundary sources,
     single-threaded Floyd-Warshall and min-plus) on parts 1..k until budget_s elapse."""
@@ -106,11 +118,13 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="road1")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--skip-forms", action="store_true",
+                    help="skip the overlap and int32-FW comparison solves (for profiling)")
     ap.add_argument("--mode", default="batched", choices=["batched", "overlap", "plain"],
                     help="execution form of the timed solves (default: batched)")
     ap.add_argument("--probe", type=float, default=0.0,
@@ -148,6 +162,8 @@ def main():
     for i in range(args.steps):
         g.solve()
         stages.append(stage_times())
+        log(f"step {i + 1}/{args.steps}: {stages[-1]['total_ms']:.2f} ms device "
+            f"(merge {stages[-1]['minplus_ms']:.2f})")
     L.check(L.load().fap_hip_device_sync(), "sync")
     elapsed = time.perf_counter() - t_start
     ms_per_step = elapsed * 1e3 / args.steps
@@ -176,20 +192,26 @@ def main():
     merge_avg = float(np.mean(merge_ms))
 
     # The overlapped form, for comparison (two steps).
-    g.setMode(F.FAP_MODE_OVERLAP)
-    ov = []
-    ts = time.perf_counter()
-    for i in range(2):
-        g.solve()
-        ov.append(stage_times())
-    overlap_ms = (time.perf_counter() - ts) * 1e3 / 2
-    g.setMode(modes[args.mode])
-    log(f"overlap form: {overlap_ms:.2f} ms/step")
-    ops = 2.0 * float((inner * bdy).sum()) * n
+    overlap_ms = None
+    if not args.skip_forms:
+        g.setMode(F.FAP_MODE_OVERLAP)
+        ts = time.perf_counter()
+        for i in range(2):
+            g.solve()
+        overlap_ms = (time.perf_counter() - ts) * 1e3 / 2
+        g.setMode(modes[args.mode])
+        log(f"overlap form: {overlap_ms:.2f} ms/step")
+    # Dominant kernel: the merge (merge_stream_kernel, one launch per padded-K group,
+    # timed together as the merge stage of the batched solves, whose stages run one
+    # after another on one stream).  Per solve it computes sum_p i_p * b_p * n min-plus
+    # steps and its algorithmic bytes are 4 * (|A| + |B| + |C|): the inner x boundary
+    # FW block, the boundary SSSP rows and the inner rows of the result, each once.
+    steps = float((inner * bdy).sum()) * n
+    ops = 2.0 * steps
     bytes_alg = 4.0 * float((inner * bdy).sum() + bdy.sum() * n + inner.sum() * n)
-    # The merge stage (one launch per padded-K group) has two floors: its VALU work
-    # (2 ops per min-plus step) and its algorithmic HBM bytes; the larger one bounds it.
-    t_valu = ops / (VALU_PEAK_TOPS * 1e12)
+    # Floors: VALU at 1.5 instructions per step (two v_add_u32 + one v_min3_i32 per
+    # two steps) against 39.3 T lane-instructions/s, and HBM at 8 TB/s.
+    t_valu = 1.5 * steps / (VALU_PEAK_TOPS * 1e12)
     t_hbm = bytes_alg / (HBM_PEAK_GBS * 1e9)
     hbm_ach = bytes_alg / (merge_avg * 1e-3) / 1e9
     valu_ach = ops / (merge_avg * 1e-3) / 1e12
@@ -197,23 +219,15 @@ def main():
         roofline = {"bound": "hbm", "achieved": hbm_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": hbm_ach / HBM_PEAK_GBS}
     else:
-        roofline = {"bound": "valu", "achieved": valu_ach, "peak": VALU_PEAK_TOPS,
-                    "unit": "TOP/s", "frac": valu_ach / VALU_PEAK_TOPS}
-    roofline.update({"kernel": "merge_stream_kernel (min-plus + fused decode), all K-groups",
-                     "traffic": None, "ops_per_launch": ops, "alg_bytes_per_launch": bytes_alg,
+        inst = 1.5 * steps / (merge_avg * 1e-3) / 1e12
+        roofline = {"bound": "valu", "achieved": inst, "peak": VALU_PEAK_TOPS,
+                    "unit": "T lane-instr/s", "frac": inst / VALU_PEAK_TOPS}
+    roofline.update({"traffic": merge_traffic(args.config),
+                     "kernel": "merge_stream_kernel (min-plus + fused decode), all K-groups",
+                     "alg_bytes_per_launch": bytes_alg, "ops_per_launch": ops,
                      "avg_launch_ms": merge_avg, "hbm_GBs": hbm_ach,
-                     "hbm_frac": hbm_ach / HBM_PEAK_GBS, "valu_TOPs": valu_ach,
-                     "valu_frac": valu_ach / VALU_PEAK_TOPS,
+                     "hbm_frac": hbm_ach / HBM_PEAK_GBS, "valu_ops_TOPs": valu_ach,
                      "floor_ms": {"hbm": t_hbm * 1e3, "valu": t_valu * 1e3}})
-
-    # The same solve with int32 Floyd-Warshall everywhere (one step), for comparison.
-    g.setMode(modes[args.mode] | F.FAP_MODE_FLAG_INT32_FW)
-    g.solve()
-    st32 = stage_times()
-    g.setMode(modes[args.mode])
-    log(f"int32 FW form: FW {st32['floyd_ms']:.2f} ms, total {st32['total_ms']:.2f} ms")
-    fw_roof["int32_form_ms"] = st32["floyd_ms"]
-    fw_roof["int32_form_total_ms"] = st32["total_ms"]
 
     plain = None
     if not args.skip_plain:

@@ -174,8 +174,8 @@ __global__ __launch_bounds__(256) void merge_kernel(const MergeDesc *__restrict_
 // Streaming form for small K (the boundary count of a road-graph part is ~10):
 // the product is a write stream of M x N int32, so one wave keeps its B panel
 // (KP x 256 columns, KP = K rounded up to a multiple of 4, FAP_INF padding) in
-// registers, reads two A rows at a time wave-uniformly (scalar loads) and writes
-// one contiguous 1 KB row segment per row.  Work unit: one 256-column block x
+// registers, reads each A row wave-uniformly (scalar loads) and writes one
+// contiguous 1 KB row segment per row.  Work unit: one 256-column block x
 // kStRows rows of one product; one launch per KP group.  Host requirements:
 // K <= KP and every A row holds KP readable entries (rows of the padded
 // Floyd-Warshall matrices, stride >= 64).  Entries A[i][k] with K <= k < KP are
@@ -183,7 +183,11 @@ __global__ __launch_bounds__(256) void merge_kernel(const MergeDesc *__restrict_
 constexpr int kStRows = 128;
 constexpr int kStCols = 256;
 
-constexpr int kStUnroll = 8;  // rows in flight per wave (distinct store registers)
+// One row per loop iteration, so a wave keeps few row stores in flight.  HBM write
+// bandwidth falls with the number of distinct DRAM rows written at once (waves per
+// CU x stores in flight per wave): on MI355X the merge stage of a 52.5 GB result
+// takes 11.8 ms with 8 rows in flight per wave and 9.6 ms with one (128-row units
+// in both; 32-, 64- and 256-row units measured slower).
 
 template <int KP>
 __device__ __forceinline__ void stream_row(int (&acc)[4], const FAP_CONST int *arow,
@@ -230,20 +234,7 @@ __device__ __forceinline__ void stream_rows(const MergeDesc &d, int row0, int ro
                                             const int (&b)[KP][4], const int (&own)[4]) {
   FAP_GLOBAL int *crow = gptr(d.C + (int64_t)row0 * d.ldc);
   const FAP_CONST int *arow = uniform_cptr(d.A + (int64_t)row0 * d.lda);
-  const int64_t ldc = d.ldc, lda = d.lda;
-  int i = row0;
-  for (; i + kStUnroll <= row1; i += kStUnroll) {
-    int acc[kStUnroll][4];
-#pragma unroll
-    for (int u = 0; u < kStUnroll; ++u) {
-      stream_row<KP>(acc[u], arow + u * lda, b);
-      stream_own<KP, kOwn>(acc[u], d, i + u, own);
-      stream_store<kVec>(crow + u * ldc, j, N, acc[u]);
-    }
-    crow += kStUnroll * ldc;
-    arow += kStUnroll * lda;
-  }
-  for (; i < row1; ++i, crow += ldc, arow += lda) {
+  for (int i = row0; i < row1; ++i, crow += d.ldc, arow += d.lda) {
     int acc[4];
     stream_row<KP>(acc, arow, b);
     stream_own<KP, kOwn>(acc, d, i, own);
