@@ -114,6 +114,46 @@ def probe(gib):
     print(json.dumps({"probe": out, "gib": gib}), flush=True)
 
 
+def minplus_bench(N, reps=3):
+    """Dense min-plus microbench: C = A (x) B on N x N int32 blocks with values uniform in
+    [0, 2^20] (fap_hip_minplus, LDS-tiled kernel), timed with HIP events on the library
+    stream; 64 sampled entries checked against numpy."""
+    lib = L.load()
+    L.init()
+    rng = np.random.default_rng(3)
+    A = rng.integers(0, 1 << 20, size=(N, N), dtype=np.int32)
+    B = rng.integers(0, 1 << 20, size=(N, N), dtype=np.int32)
+    dA, dB = F.DeviceBuffer.from_array(A), F.DeviceBuffer.from_array(B)
+    dC = F.DeviceBuffer(N * N * 4)
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    L.check(lib.fap_hip_event_create(C.byref(e0)), "event")
+    L.check(lib.fap_hip_event_create(C.byref(e1)), "event")
+    run = lambda: L.check(lib.fap_hip_minplus(dA.ptr, N, dB.ptr, N, dC.ptr, N, N, N, N, None),
+                          "minplus")
+    run()
+    times = []
+    for _ in range(reps):
+        L.check(lib.fap_hip_event_record(e0, None), "record")
+        run()
+        L.check(lib.fap_hip_event_record(e1, None), "record")
+        ms = C.c_float()
+        L.check(lib.fap_hip_event_elapsed_ms(e0, e1, C.byref(ms)), "elapsed")
+        times.append(ms.value)
+    Cres = np.empty((N, N), np.int32)
+    L.check(lib.fap_hip_memcpy_d2h(Cres.ctypes.data, dC.ptr, N * N * 4, None), "d2h")
+    idx = rng.integers(0, N, size=(64, 2))
+    ok = all(int(Cres[i, j]) == int((A[i, :].astype(np.int64) + B[:, j]).min()) for i, j in idx)
+    ms = float(np.median(times))
+    ops = 2.0 * N ** 3
+    inst = 1.5 * N ** 3 / (ms * 1e-3) / 1e12
+    out = {"minplus_bench": {"N": N, "ms": ms, "ops": ops, "TOPs": ops / (ms * 1e-3) / 1e12,
+                             "valu_instr_T_per_s": inst, "valu_frac": inst / VALU_PEAK_TOPS,
+                             "sampled_entries_exact": ok}}
+    print(json.dumps(out), flush=True)
+    if not ok:
+        sys.exit(1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,11 +167,16 @@ def main():
                     help="skip the overlap and int32-FW comparison solves (for profiling)")
     ap.add_argument("--mode", default="batched", choices=["batched", "overlap", "plain"],
                     help="execution form of the timed solves (default: batched)")
+    ap.add_argument("--minplus", type=int, default=0,
+                    help="run only the dense N x N x N int32 min-plus microbench")
     ap.add_argument("--probe", type=float, default=0.0,
                     help="run only the HBM fill/copy probe on buffers of this many GiB")
     args = ap.parse_args()
     if args.probe > 0:
         probe(args.probe)
+        return
+    if args.minplus > 0:
+        minplus_bench(args.minplus)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU sharding is "

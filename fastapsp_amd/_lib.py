@@ -104,6 +104,7 @@ SIGNATURES = {
     "fap_graph_get_layout": (C.c_int, [_vp, _pi32, _pi32, _pi32, _pi32, _pi32, _pi32]),
     "fap_graph_get_dist_rows": (C.c_int, [_vp, _i64, _i64, _pi32]),
     "fap_graph_get_subgraph_dist": (C.c_int, [_vp, _i32, _pi32]),
+    "fap_graph_get_subgraph_rows": (C.c_int, [_vp, _i32, _i64, _i64, _pi32]),
     "fap_graph_device_dist": (C.c_int, [_vp, C.POINTER(_vp), _pi64]),
     "fap_graph_times": (C.c_int, [_vp, _pf64, _pf64, _pf64, _pf64, _pf64]),
     "fap_graph_edge_cut": (C.c_int, [_vp, _pf64]),

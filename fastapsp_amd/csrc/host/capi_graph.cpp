@@ -257,6 +257,12 @@ int fap_graph_get_subgraph_dist(fap_graph *h, int32_t id, int32_t *h_out) {
   });
 }
 
+int fap_graph_get_subgraph_rows(fap_graph *h, int32_t id, int64_t row0, int64_t rows,
+                                int32_t *h_out) {
+  FAP_HANDLE(h);
+  return guarded([&] { h->g->getSubGraphDistanceRowsI32(id, row0, rows, h_out); });
+}
+
 int fap_graph_device_dist(fap_graph *h, int32_t **d_dist, int64_t *ld) {
   FAP_HANDLE(h);
   if (d_dist) *d_dist = const_cast<int32_t *>(h->g->deviceDistance(ld));

@@ -318,6 +318,17 @@ std::vector<int32_t> fapGraph::getSubGraphDistanceI32(int32_t id) {
   return out;
 }
 
+void fapGraph::getSubGraphDistanceRowsI32(int32_t id, int64_t row0, int64_t rows, int32_t *out) {
+  if (id != current_subgraph_id || !d_sub_dist)
+    throw FapError(FAP_ERR_STATE, "sub graph was not the last one solved");
+  if (row0 < 0 || rows < 0 || row0 + rows > sub_rows)
+    throw FapError(FAP_ERR_INVALID, "row range out of bounds");
+  if (rows > 0)
+    check(fap_hip_memcpy2d_d2h_i32(out, num_vertexs, d_sub_dist + row0 * ld_dist, ld_dist, rows,
+                                   num_vertexs, nullptr),
+          "device to host copy");
+}
+
 std::vector<float> fapGraph::getSubGraphDistance(int32_t id) {
   std::vector<int32_t> d = getSubGraphDistanceI32(id);
   std::vector<float> out(d.size());

@@ -179,6 +179,14 @@ class FapGraph:
                 "getSubGraphDistance")
         return out
 
+    def getSubGraphDistanceRowsI32(self, sub_graph_id: int, row0: int, rows: int) -> np.ndarray:
+        """Rows [row0, row0 + rows) of the last solveSubGraph block."""
+        out = np.empty((rows, self.num_vertexs), dtype=np.int32)
+        L.check(L.load().fap_graph_get_subgraph_rows(self._h, int(sub_graph_id), int(row0),
+                                                     int(rows), L.ptr_i32(out)),
+                "getSubGraphDistanceRows")
+        return out
+
     def getSubGraphDistance(self, sub_graph_id: int) -> np.ndarray:
         d = self.getSubGraphDistanceI32(sub_graph_id)
         out = d.astype(np.float32)

@@ -131,6 +131,9 @@ class fapGraph {
   std::vector<int32_t> getSubGraphIndex(int32_t sub_graph_id);
   std::vector<float> getSubGraphDistance(int32_t sub_graph_id);
   std::vector<int32_t> getSubGraphDistanceI32(int32_t sub_graph_id);
+  // Rows [row0, row0 + rows) of the last solveSubGraph block (int32) into out.
+  void getSubGraphDistanceRowsI32(int32_t sub_graph_id, int64_t row0, int64_t rows,
+                                  int32_t *out);
   std::vector<int32_t> getSubGraphPath(int32_t sub_graph_id);
   std::vector<float> getDistanceFromOnePoint(int32_t vertex_id);
   std::vector<int32_t> getPathFromOnePoint(int32_t vertex_id);

@@ -244,6 +244,9 @@ int fap_graph_get_layout(fap_graph *g, int32_t *graph_id, int32_t *st2ed, int32_
 int fap_graph_get_dist_rows(fap_graph *g, int64_t row0, int64_t rows, int32_t *h_out);
 /* The s_p x n block of the last solveSubGraph(id). */
 int fap_graph_get_subgraph_dist(fap_graph *g, int32_t id, int32_t *h_out);
+/* Rows [row0, row0 + rows) of that block (D2H of a slice instead of the whole block). */
+int fap_graph_get_subgraph_rows(fap_graph *g, int32_t id, int64_t row0, int64_t rows,
+                                int32_t *h_out);
 /* Device pointer and row stride of the resident solve() result. */
 int fap_graph_device_dist(fap_graph *g, int32_t **d_dist, int64_t *ld);
 /* Stage times (s): partitioning, sssp, floyd, min-plus, data transformation. */
