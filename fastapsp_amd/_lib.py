@@ -41,7 +41,8 @@ _pf64 = C.POINTER(C.c_double)
 
 class FapGraphDev(C.Structure):
     _fields_ = [("n", _i32), ("m", _i64), ("d_row_offset", _vp), ("d_col", _vp), ("d_w", _vp),
-                ("d_graph_id", _vp), ("d_st2ed", _vp), ("d_ed2st", _vp), ("d_comp", _vp)]
+                ("d_graph_id", _vp), ("d_st2ed", _vp), ("d_ed2st", _vp), ("d_comp", _vp),
+                ("d_wf", _vp)]
 
 
 class FapPartDesc(C.Structure):
@@ -84,6 +85,8 @@ SIGNATURES = {
     "fap_hip_solve_parts": (C.c_int, [C.POINTER(FapGraphDev), C.POINTER(FapPartDesc), _i32,
                                       _vp, _i64, _i32, _vp]),
     "fap_hip_last_stage_times": (C.c_int, [C.POINTER(FapStageTimes)]),
+    "fap_hip_solve_parts_f32": (C.c_int, [C.POINTER(FapGraphDev), C.POINTER(FapPartDesc), _i32,
+                                          _vp, _i64, _i32, _vp]),
     "fap_graph_create_from_file": (C.c_int, [C.POINTER(_vp), _cp, C.c_int, C.c_int, _i32, _cp]),
     "fap_mtx_read": (C.c_int, [_cp, C.c_int, C.c_int, _pi32, _pi64, _pi32, _pi32, _pf32]),
     "fap_graph_create_from_csr": (C.c_int, [C.POINTER(_vp), _i32, _i64, _pi32, _pi32, _pf32,
@@ -105,6 +108,10 @@ SIGNATURES = {
     "fap_graph_get_dist_rows": (C.c_int, [_vp, _i64, _i64, _pi32]),
     "fap_graph_get_subgraph_dist": (C.c_int, [_vp, _i32, _pi32]),
     "fap_graph_get_subgraph_rows": (C.c_int, [_vp, _i32, _i64, _i64, _pi32]),
+    "fap_graph_is_float": (C.c_int, [_vp, _pi32]),
+    "fap_graph_get_weights_f32": (C.c_int, [_vp, _pf32]),
+    "fap_graph_get_dist_rows_f32": (C.c_int, [_vp, _i64, _i64, _pf32]),
+    "fap_graph_get_subgraph_rows_f32": (C.c_int, [_vp, _i32, _i64, _i64, _pf32]),
     "fap_graph_device_dist": (C.c_int, [_vp, C.POINTER(_vp), _pi64]),
     "fap_graph_times": (C.c_int, [_vp, _pf64, _pf64, _pf64, _pf64, _pf64]),
     "fap_graph_edge_cut": (C.c_int, [_vp, _pf64]),

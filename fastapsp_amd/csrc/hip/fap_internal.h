@@ -37,6 +37,13 @@ struct Workspace {
 
 constexpr int kInf = FAP_INF;
 
+// Stage timing of the solve pipelines, read by fap_hip_last_stage_times.  Stages:
+// 0 sssp, 1 build, 2 Floyd-Warshall, 3 merge, 4 decode.
+int timing_start(hipStream_t st);                   // new call: reset, record start
+int timing_record(hipStream_t on, hipEvent_t *e);   // pooled event recorded on `on`
+void timing_add_span(int stage, hipEvent_t a, hipEvent_t b);
+int timing_finish(hipStream_t st);                  // record the end of the call
+
 }  // namespace fap_hip
 
 #define FAP_CHECK_HIP(call)                                                            \

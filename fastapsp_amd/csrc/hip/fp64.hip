@@ -1,0 +1,659 @@
+// Float-weight path of the part pipeline, computed in FP64, results in float32.
+//
+// The reference computes everything in float (dijkstra_path, floyd_CPU_path and
+// min_plus_path_CPU_advanced on float matrices, fap/kernel/batch_sssp.h:79-114,
+// floyd.h:38-49, minplus.h:51-64).  Here every distance is carried in double: a
+// float32 weight is an integer multiple of its ulp, so as long as a path sum spans
+// fewer than 53 significant bits (weights in [0.5, 10) and sums below 2^29, for
+// instance) every sum is exact and the results equal an FP64 Dijkstra bit for bit;
+// only the final conversion to float32 rounds (relative error <= 2^-24).  In every
+// other case the FP64 rounding is far below float's.  Unreachable entries, and
+// distances of 1e8 or more, are written as MAXVALUE = 1e8 (fap/parameter.h:26),
+// which is what the reference's float path reports for them.
+//
+// Kernels (each the FP64 twin of an int32 kernel of this library):
+//   sssp64_kernel   one wave per boundary source, label-correcting relaxation with
+//                   atomicMin on the bit patterns (non-negative doubles and +inf
+//                   order as unsigned integers), frontier queues in LDS;
+//   build64_kernel  one workgroup per padded row of a part's matrix;
+//   fw64_phase*     blocked three-phase Floyd-Warshall on 64 x 64 tiles;
+//   merge64_kernel  LDS-tiled min-plus of the inner rows against the boundary rows
+//                   with the decode of the part's own columns fused, float output;
+//   bdy64_kernel    the boundary rows: SSSP row, own columns from the FW matrix.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "fap_internal.h"
+#include "gmem.h"
+
+namespace fap_hip {
+
+int launch_gather_sources(const int *offsets, const int *src_begin, const int64_t *out_rows,
+                          int nparts, int total, const int *st2ed, int64_t ld, int *src,
+                          int64_t *rows, hipStream_t stream);
+
+namespace {
+
+typedef unsigned long long u64;
+constexpr double kInfD = __builtin_inf();
+constexpr float kMaxValue = 1e8f;
+constexpr int kT = 64;
+constexpr int kMaxS64 = 8192;
+
+__device__ __forceinline__ u64 dbits(double x) { return (u64)__double_as_longlong(x); }
+__device__ __forceinline__ double bitsd(u64 x) { return __longlong_as_double((long long)x); }
+__device__ __forceinline__ float to_out(double d) { return d < 1e8 ? (float)d : kMaxValue; }
+
+__device__ __forceinline__ double load_agent(const double *p) {
+  return bitsd(__hip_atomic_load(reinterpret_cast<const u64 *>(p), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT));
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------- SSSP
+constexpr int kSWaves = 4;
+constexpr int kSQ = 1024;
+
+// Relaxes every out-arc of u with its current label.
+__device__ __forceinline__ void relax64(const int *__restrict__ row_off, const int *__restrict__ col,
+                                        const float *__restrict__ w, double *row, int u,
+                                        int *q_next, int *q_count) {
+  const double du = load_agent(row + u);
+  if (!(du < kInfD)) return;
+  const int e1 = row_off[u + 1];
+  for (int e = row_off[u]; e < e1; ++e) {
+    const int v = col[e];
+    const double nd = du + (w ? (double)w[e] : 1.0);
+    if (nd < row[v]) {
+      const u64 old = atomicMin(reinterpret_cast<u64 *>(row + v), dbits(nd));
+      if (nd < bitsd(old)) {
+        const int pos = atomicAdd(q_count, 1);
+        if (pos < kSQ) q_next[pos] = v;
+      }
+    }
+  }
+}
+
+// One wave per source; row i of `rows` (stride ld) belongs to source src[i].
+__global__ __launch_bounds__(64 * kSWaves) void sssp64_kernel(
+    const int *__restrict__ row_off, const int *__restrict__ col, const float *__restrict__ w,
+    int n, const int *__restrict__ src, int nsrc, double *__restrict__ rows, int64_t ld) {
+  __shared__ int queue[kSWaves][2][kSQ];
+  __shared__ int qcount[kSWaves][2];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int sidx = blockIdx.x * kSWaves + wave;
+  if (sidx >= nsrc) return;  // whole wave leaves; no workgroup barrier below
+  const int s = src[sidx];
+  double *row = rows + (int64_t)sidx * ld;
+  for (int v = lane; v < n; v += 64) row[v] = (v == s) ? 0.0 : kInfD;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // fill visible before the atomics
+  if (lane == 0) {
+    queue[wave][0][0] = s;
+    qcount[wave][0] = 1;
+    qcount[wave][1] = 0;
+  }
+  wave_sync();
+  int cur = 0, qsize = 1;
+  bool scan = false;
+  while (true) {
+    const int nxt = cur ^ 1;
+    int *q_next = queue[wave][nxt];
+    int *q_cnt = &qcount[wave][nxt];
+    if (!scan) {
+      for (int i = lane; i < qsize; i += 64) relax64(row_off, col, w, row, queue[wave][cur][i], q_next, q_cnt);
+    } else {
+      for (int u = lane; u < n; u += 64) relax64(row_off, col, w, row, u, q_next, q_cnt);
+    }
+    wave_sync();
+    const int cnt = __hip_atomic_load(q_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    wave_sync();
+    if (cnt == 0) break;
+    scan = cnt > kSQ;  // overflowed queue: next round relaxes every vertex
+    qsize = scan ? 0 : cnt;
+    if (lane == 0) qcount[wave][cur] = 0;
+    wave_sync();
+    cur = nxt;
+  }
+}
+
+// ---------------------------------------------------------------- build
+struct Build64Desc {
+  double *fw;
+  int S, s, bdy, offset, part_id, row_begin;
+  const double *rows;  // the part's SSSP rows (row i = boundary vertex i)
+  int64_t ld_rows;
+};
+
+template <typename D>
+__device__ __forceinline__ int find_begin(const D *__restrict__ d, int nd, int64_t key,
+                                          int64_t (*get)(const D &)) {
+  int lo = 0, hi = nd - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (get(d[mid]) <= key) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ int64_t build_begin(const Build64Desc &d) { return d.row_begin; }
+
+// Row i of part p: boundary rows from the SSSP rows gathered at st2ed, inner rows
+// FAP_INF then the part's own out-arcs in CSR order (a later arc overwrites an
+// earlier one), diagonal 0, padding +inf (MysubMatBuild_path,
+// fap/utils/util_Myalgorithm.h:44-84).
+__global__ __launch_bounds__(256) void build64_kernel(const Build64Desc *__restrict__ descs, int nd,
+                                                      int total_rows, const int *__restrict__ row_off,
+                                                      const int *__restrict__ col,
+                                                      const float *__restrict__ w,
+                                                      const int *__restrict__ graph_id,
+                                                      const int *__restrict__ st2ed,
+                                                      const int *__restrict__ ed2st) {
+  const int grow = blockIdx.x;
+  if (grow >= total_rows) return;
+  const Build64Desc d = descs[find_begin(descs, nd, grow, build_begin)];
+  const int i = grow - d.row_begin;
+  __shared__ double rowbuf[kMaxS64];
+  const int tid = threadIdx.x;
+  if (i < d.s && i < d.bdy) {
+    const double *src = d.rows + (int64_t)i * d.ld_rows;
+    for (int j = tid; j < d.S; j += 256) rowbuf[j] = (j < d.s) ? src[st2ed[d.offset + j]] : kInfD;
+  } else {
+    for (int j = tid; j < d.S; j += 256) rowbuf[j] = kInfD;
+  }
+  __syncthreads();
+  if (tid == 0 && i < d.s) {
+    if (i >= d.bdy) {
+      const int ver = st2ed[d.offset + i];
+      for (int e = row_off[ver]; e < row_off[ver + 1]; ++e) {
+        const int nb = col[e];
+        if (graph_id[nb] != d.part_id) continue;
+        const int idx = ed2st[nb] - d.offset;
+        if (idx >= 0 && idx < d.s) rowbuf[idx] = w ? (double)w[e] : 1.0;
+      }
+    }
+    rowbuf[i] = 0.0;
+  }
+  __syncthreads();
+  double *dst = d.fw + (int64_t)i * d.S;
+  for (int j = tid; j < d.S; j += 256) dst[j] = rowbuf[j];
+}
+
+// ---------------------------------------------------------------- Floyd-Warshall
+struct Fw64Desc {
+  double *mat;  // S x S, ld = S = nb * 64
+  int nb;
+};
+
+__device__ __forceinline__ double *tile64(const Fw64Desc &d, int ti, int tj) {
+  const int64_t S = (int64_t)d.nb * kT;
+  return d.mat + (int64_t)ti * kT * S + tj * kT;
+}
+
+// Phase 1: the diagonal tile closed in LDS; 256 threads, 16 entries each.
+__global__ __launch_bounds__(256) void fw64_phase1_kernel(const Fw64Desc *__restrict__ descs, int r) {
+  const Fw64Desc d = descs[blockIdx.x];
+  if (r >= d.nb) return;
+  const int64_t S = (int64_t)d.nb * kT;
+  __shared__ double D[kT][kT];
+  double *g = tile64(d, r, r);
+  const int row = threadIdx.x >> 2, c0 = (threadIdx.x & 3) * 16;
+#pragma unroll
+  for (int c = 0; c < 16; ++c) D[row][c0 + c] = g[row * S + c0 + c];
+  for (int k = 0; k < kT; ++k) {
+    __syncthreads();  // row k and column k do not change during step k
+    const double dik = D[row][k];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) D[row][c0 + c] = fmin(D[row][c0 + c], dik + D[k][c0 + c]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 16; ++c) g[row * S + c0 + c] = D[row][c0 + c];
+}
+
+// Phases 2 and 3: one wave per 64 x 64 tile, 8 x 8 accumulators per lane (rows
+// tr*4 + {0..3}, 32 + tr*4 + {0..3}; columns likewise with tc), k in LDS chunks
+// of 16: As[k][i] (transposed A), Bs[k][j].
+constexpr int kKc64 = 16;
+
+__device__ __forceinline__ int own_row64(int tr, int q) { return (q < 4 ? 0 : 32) + tr * 4 + (q & 3); }
+
+__device__ __forceinline__ void tile_product64(double (&acc)[8][8], const double *a, const double *b,
+                                               int64_t S, double (*As)[kT], double (*Bs)[kT], int tr,
+                                               int tc) {
+  const int lane = threadIdx.x;
+  for (int k0 = 0; k0 < kT; k0 += kKc64) {
+    if (k0 > 0) __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kKc64; ++q) As[q][lane] = a[lane * S + k0 + q];
+#pragma unroll
+    for (int q = 0; q < kKc64; ++q) Bs[q][lane] = b[(k0 + q) * S + lane];
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < kKc64; ++k) {
+      double av[8], bv[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        av[q] = As[k][own_row64(tr, q)];
+        bv[q] = Bs[k][own_row64(tc, q)];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[q][c] = fmin(acc[q][c], av[q] + bv[c]);
+    }
+  }
+}
+
+__device__ __forceinline__ int skip64(int o, int p) { return o < p ? o : o + 1; }
+
+__global__ __launch_bounds__(64) void fw64_phase2_kernel(const Fw64Desc *__restrict__ descs, int r) {
+  const Fw64Desc d = descs[blockIdx.y];
+  const int others = d.nb - 1;
+  int o = blockIdx.x;
+  if (r >= d.nb || o >= 2 * others) return;
+  const bool is_row = o < others;
+  if (!is_row) o -= others;
+  o = skip64(o, r);
+  const int64_t S = (int64_t)d.nb * kT;
+  __shared__ double As[kKc64][kT];
+  __shared__ double Bs[kKc64][kT];
+  double *t = is_row ? tile64(d, r, o) : tile64(d, o, r);
+  const int tr = threadIdx.x >> 3, tc = threadIdx.x & 7;
+  double acc[8][8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[q][c] = t[own_row64(tr, q) * S + own_row64(tc, c)];
+  if (is_row) tile_product64(acc, tile64(d, r, r), t, S, As, Bs, tr, tc);
+  else tile_product64(acc, t, tile64(d, r, r), S, As, Bs, tr, tc);
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) t[own_row64(tr, q) * S + own_row64(tc, c)] = acc[q][c];
+}
+
+__global__ __launch_bounds__(64) void fw64_phase3_kernel(const Fw64Desc *__restrict__ descs, int r) {
+  const Fw64Desc d = descs[blockIdx.y];
+  const int others = d.nb - 1;
+  const int o = blockIdx.x;
+  if (r >= d.nb || o >= others * others) return;
+  const int i = skip64(o / others, r), j = skip64(o % others, r);
+  const int64_t S = (int64_t)d.nb * kT;
+  __shared__ double As[kKc64][kT];
+  __shared__ double Bs[kKc64][kT];
+  double *t = tile64(d, i, j);
+  const int tr = threadIdx.x >> 3, tc = threadIdx.x & 7;
+  double acc[8][8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[q][c] = t[own_row64(tr, q) * S + own_row64(tc, c)];
+  tile_product64(acc, tile64(d, i, r), tile64(d, r, j), S, As, Bs, tr, tc);
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) t[own_row64(tr, q) * S + own_row64(tc, c)] = acc[q][c];
+}
+
+// Phase 2 reads the row tile it writes only through its own workgroup and the
+// pivot tile, which phase 2 does not write; phase 3 reads panel tiles only.
+int launch_fw64(const Fw64Desc *d_descs, int nd, int max_nb, hipStream_t st) {
+  for (int r = 0; r < max_nb; ++r) {
+    hipLaunchKernelGGL(fw64_phase1_kernel, dim3(nd), dim3(256), 0, st, d_descs, r);
+    FAP_CHECK_LAUNCH();
+    const int others = max_nb - 1;
+    if (others > 0) {
+      hipLaunchKernelGGL(fw64_phase2_kernel, dim3(2 * others, nd), dim3(64), 0, st, d_descs, r);
+      FAP_CHECK_LAUNCH();
+      hipLaunchKernelGGL(fw64_phase3_kernel, dim3(others * others, nd), dim3(64), 0, st, d_descs,
+                         r);
+      FAP_CHECK_LAUNCH();
+    }
+  }
+  return FAP_OK;
+}
+
+// ---------------------------------------------------------------- merge
+// C[i][j] = min_k A[i][k] + B[k][j] (overwrite, +inf when K == 0) for the inner rows
+// of one part, with the part's own columns taken from its FW matrix; float output.
+struct Merge64Desc {
+  const double *A;  // inner x boundary block of the FW matrix (stride lda)
+  int64_t lda;
+  const double *B;  // boundary SSSP rows (stride ldb)
+  int64_t ldb;
+  float *C;
+  int64_t ldc;
+  int M, K, row_blocks, tile_begin, part_id, offset;
+  const double *fw;  // FW matrix; row fw_row0 + i holds C's row i
+  int64_t ldf;
+  int fw_row0;
+};
+
+constexpr int kMRows = 64, kMCols = 256, kMKc = 16;
+
+__device__ int64_t merge_begin(const Merge64Desc &d) { return d.tile_begin; }
+
+template <bool kVec>
+__global__ __launch_bounds__(256) void merge64_kernel(const Merge64Desc *__restrict__ descs, int nd,
+                                                      int total_tiles, int N,
+                                                      const int *__restrict__ graph_id,
+                                                      const int *__restrict__ ed2st) {
+  const int t = blockIdx.x;
+  if (t >= total_tiles) return;
+  const Merge64Desc d = descs[find_begin(descs, nd, t, merge_begin)];
+  const int local = t - d.tile_begin;
+  const int cb = local / d.row_blocks, rb = local - cb * d.row_blocks;
+  const int row0 = rb * kMRows, col0 = cb * kMCols;
+  __shared__ double As[kMKc][kMRows];
+  __shared__ double Bs[kMKc][kMCols];
+  const int tid = threadIdx.x;
+  const int tx = tid & 31;  // columns col0 + tx*4 + {0..3}, col0 + 128 + tx*4 + {0..3}
+  const int ty = tid >> 5;  // rows row0 + ty*8 + {0..7}
+  double acc[8][8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[r][c] = kInfD;
+  for (int k0 = 0; k0 < d.K; k0 += kMKc) {
+    if (k0 > 0) __syncthreads();
+    {  // As[kk][i] = A[row0 + i][k0 + kk]: 64 x 16, four per thread
+      const int i = tid & 63, kq = (tid >> 6) * 4;
+      const bool row_ok = row0 + i < d.M;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int kk = kq + q;
+        As[kk][i] = (row_ok && k0 + kk < d.K) ? d.A[(int64_t)(row0 + i) * d.lda + k0 + kk] : kInfD;
+      }
+    }
+    {  // Bs[kk][c] = B[k0 + kk][col0 + c]: 16 x 256, sixteen per thread
+#pragma unroll
+      for (int q = 0; q < kMKc; ++q) {
+        const int c = tid, j = col0 + c;
+        Bs[q][c] = (k0 + q < d.K && j < N) ? d.B[(int64_t)(k0 + q) * d.ldb + j] : kInfD;
+      }
+    }
+    __syncthreads();
+    const int kc = min(kMKc, d.K - k0);
+    for (int k = 0; k < kc; ++k) {
+      double av[8], bv[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) av[r] = As[k][ty * 8 + r];
+#pragma unroll
+      for (int g = 0; g < 2; ++g)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) bv[4 * g + e] = Bs[k][g * 128 + tx * 4 + e];
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[r][c] = fmin(acc[r][c], av[r] + bv[c]);
+    }
+  }
+  int cols[8], local_col[8];
+  bool any_own = false;
+#pragma unroll
+  for (int g = 0; g < 2; ++g)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int c = 4 * g + e;
+      cols[c] = col0 + g * 128 + tx * 4 + e;
+      local_col[c] = -1;
+      if (d.part_id > 0 && cols[c] < N && graph_id[cols[c]] == d.part_id) {
+        local_col[c] = ed2st[cols[c]] - d.offset;
+        any_own = true;
+      }
+    }
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int i = row0 + ty * 8 + r;
+    if (i >= d.M) break;
+    if (any_own) {
+      const double *frow = d.fw + (int64_t)(d.fw_row0 + i) * d.ldf;
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        if (local_col[c] >= 0) acc[r][c] = frow[local_col[c]];
+    }
+    float *crow = d.C + (int64_t)i * d.ldc;
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      const int j = cols[4 * g];
+      if (kVec && j + 3 < N) {
+        *reinterpret_cast<float4 *>(crow + j) =
+            make_float4(to_out(acc[r][4 * g]), to_out(acc[r][4 * g + 1]),
+                        to_out(acc[r][4 * g + 2]), to_out(acc[r][4 * g + 3]));
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (j + e < N) crow[j + e] = to_out(acc[r][4 * g + e]);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- boundary rows
+struct Bdy64Desc {
+  const double *rows;  // the part's SSSP rows (stride ld_rows)
+  int64_t ld_rows;
+  const double *fw;    // FW matrix (stride ldf), row i = boundary vertex i
+  int64_t ldf;
+  float *out;          // first output row of the part (stride ld_out)
+  int64_t ld_out;
+  int rows_n, offset, part_id;
+  int64_t item_begin;  // prefix over parts of rows_n * n
+};
+
+__device__ int64_t bdy_begin(const Bdy64Desc &d) { return d.item_begin; }
+
+__global__ void bdy64_kernel(const Bdy64Desc *__restrict__ descs, int nd, int64_t total, int n,
+                             const int *__restrict__ graph_id, const int *__restrict__ ed2st) {
+  const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= total) return;
+  const Bdy64Desc d = descs[find_begin(descs, nd, item, bdy_begin)];
+  const int64_t local = item - d.item_begin;
+  const int i = (int)(local / n), j = (int)(local - (int64_t)i * n);
+  const double v = graph_id[j] == d.part_id ? d.fw[(int64_t)i * d.ldf + ed2st[j] - d.offset]
+                                            : d.rows[(int64_t)i * d.ld_rows + j];
+  d.out[(int64_t)i * d.ld_out + j] = to_out(v);
+}
+
+Workspace g_ws_rows, g_ws_fw, g_ws_meta;
+int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace
+}  // namespace fap_hip
+
+using namespace fap_hip;
+
+extern "C" int fap_hip_solve_parts_f32(const fap_graph_dev *g, const fap_part_desc *parts,
+                                       int32_t nparts, float *d_out, int64_t ld_out, int32_t mode,
+                                       void *stream) {
+  if (!initialized()) {
+    set_error("fap_hip_init has not been called");
+    return FAP_ERR_STATE;
+  }
+  FAP_REQUIRE(g && nparts >= 0 && (nparts == 0 || parts) && d_out,
+              "fap_hip_solve_parts_f32: null argument");
+  const int form = mode & 0xFF;
+  FAP_REQUIRE((form == FAP_MODE_PLAIN || form == FAP_MODE_BATCHED || form == FAP_MODE_OVERLAP) &&
+                  (mode & ~(0xFF | FAP_MODE_FLAG_INT32_FW)) == 0,
+              "fap_hip_solve_parts_f32: unknown mode %d", mode);
+  const int n = g->n;
+  FAP_REQUIRE(n > 0 && g->m >= 0 && ld_out >= n && g->d_row_offset && g->d_graph_id &&
+                  g->d_st2ed && g->d_ed2st && (g->m == 0 || g->d_col),
+              "fap_hip_solve_parts_f32: bad graph view");
+  hipStream_t st = resolve(stream);
+  const bool batched = form != FAP_MODE_PLAIN;  // OVERLAP runs as BATCHED here
+  const bool vec = (ld_out % 4 == 0) && ((uintptr_t)d_out % 16 == 0);
+
+  std::vector<fap_part_desc> act;
+  for (int p = 0; p < nparts; ++p) {
+    const fap_part_desc &q = parts[p];
+    FAP_REQUIRE(q.part_id >= 1 && q.num >= 0 && q.bdy >= 0 && q.bdy <= q.num && q.offset >= 0 &&
+                    (int64_t)q.offset + q.num <= n && q.out_row >= 0,
+                "fap_hip_solve_parts_f32: bad part %d", p);
+    FAP_REQUIRE(q.num <= kMaxS64, "fap_hip_solve_parts_f32: part %d has %d vertices (limit %d)",
+                q.part_id, q.num, kMaxS64);
+    if (q.num > 0) act.push_back(q);
+  }
+  const int P = (int)act.size();
+  FAP_TRY(timing_start(st));
+  if (P == 0) return timing_finish(st);
+
+  const int64_t ldr = align_up(n, 2);  // SSSP row stride (doubles)
+  std::vector<int> S(P), nb(P), offsets(P), src_begin(P);
+  std::vector<int64_t> fw_off(P), row_off64(P), out_rows(P);
+  int64_t fw_total = 0, fw_max = 0;
+  int src_total = 0, bdy_max = 0, max_nb = 0, rows_total = 0;
+  for (int p = 0; p < P; ++p) {
+    S[p] = (int)align_up(act[p].num, kT);
+    nb[p] = S[p] / kT;
+    max_nb = std::max(max_nb, nb[p]);
+    fw_off[p] = batched ? fw_total : 0;
+    fw_total += (int64_t)S[p] * S[p];
+    fw_max = std::max<int64_t>(fw_max, (int64_t)S[p] * S[p]);
+    offsets[p] = act[p].offset;
+    src_begin[p] = batched ? src_total : 0;
+    row_off64[p] = (int64_t)src_begin[p] * ldr;
+    out_rows[p] = src_begin[p];
+    src_total += act[p].bdy;
+    bdy_max = std::max(bdy_max, act[p].bdy);
+  }
+  FAP_TRY(g_ws_fw.ensure((batched ? fw_total : fw_max) * 8));
+  FAP_TRY(g_ws_rows.ensure(std::max<int64_t>(1, (int64_t)(batched ? src_total : bdy_max) * ldr) * 8));
+  double *fw_base = static_cast<double *>(g_ws_fw.ptr);
+  double *rows_base = static_cast<double *>(g_ws_rows.ptr);
+
+  std::vector<Build64Desc> bd(P);
+  std::vector<Fw64Desc> fd(P);
+  std::vector<Merge64Desc> md(P);
+  std::vector<Bdy64Desc> yd(P);
+  std::vector<int> tiles(P, 0);
+  int tile_total = 0;
+  int64_t bdy_items = 0;
+  for (int p = 0; p < P; ++p) {
+    const fap_part_desc &q = act[p];
+    double *fw = fw_base + fw_off[p];
+    const double *rows = rows_base + row_off64[p];
+    bd[p] = Build64Desc{fw, S[p], q.num, q.bdy, q.offset, q.part_id, batched ? rows_total : 0, rows,
+                        ldr};
+    rows_total += S[p];
+    fd[p] = Fw64Desc{fw, nb[p]};
+    Merge64Desc m{};
+    m.A = fw + (int64_t)q.bdy * S[p];
+    m.lda = S[p];
+    m.B = rows;
+    m.ldb = ldr;
+    m.C = d_out + (q.out_row + q.bdy) * ld_out;
+    m.ldc = ld_out;
+    m.M = q.num - q.bdy;
+    m.K = q.bdy;
+    m.row_blocks = (m.M + kMRows - 1) / kMRows;
+    tiles[p] = m.M > 0 ? m.row_blocks * ((n + kMCols - 1) / kMCols) : 0;
+    m.tile_begin = batched ? tile_total : 0;
+    tile_total += tiles[p];
+    m.part_id = q.part_id;
+    m.offset = q.offset;
+    m.fw = fw;
+    m.ldf = S[p];
+    m.fw_row0 = q.bdy;
+    md[p] = m;
+    yd[p] = Bdy64Desc{rows, ldr, fw, S[p], d_out + q.out_row * ld_out, ld_out, q.bdy, q.offset,
+                      q.part_id, batched ? bdy_items : 0};
+    bdy_items += (int64_t)q.bdy * n;
+  }
+  FAP_REQUIRE(tile_total < (1 << 30), "fap_hip_solve_parts_f32: too many tiles");
+
+  // Descriptor tables in one upload.
+  int64_t off = 0;
+  auto carve = [&](int64_t bytes) {
+    const int64_t o = off;
+    off = align_up(off + bytes, 256);
+    return o;
+  };
+  const int64_t o_bd = carve(sizeof(Build64Desc) * P), o_fd = carve(sizeof(Fw64Desc) * P);
+  const int64_t o_md = carve(sizeof(Merge64Desc) * P), o_yd = carve(sizeof(Bdy64Desc) * P);
+  const int64_t o_off = carve(4LL * P), o_sb = carve(4LL * P), o_or = carve(8LL * P);
+  const int64_t copy_end = off;
+  const int64_t o_src = carve(4LL * std::max(src_total, 1)), o_rw = carve(8LL * std::max(src_total, 1));
+  FAP_TRY(g_ws_meta.ensure(off));
+  char *meta = static_cast<char *>(g_ws_meta.ptr);
+  std::vector<char> host(copy_end);
+  std::memcpy(host.data() + o_bd, bd.data(), sizeof(Build64Desc) * P);
+  std::memcpy(host.data() + o_fd, fd.data(), sizeof(Fw64Desc) * P);
+  std::memcpy(host.data() + o_md, md.data(), sizeof(Merge64Desc) * P);
+  std::memcpy(host.data() + o_yd, yd.data(), sizeof(Bdy64Desc) * P);
+  std::memcpy(host.data() + o_off, offsets.data(), 4LL * P);
+  std::memcpy(host.data() + o_sb, src_begin.data(), 4LL * P);
+  std::memcpy(host.data() + o_or, out_rows.data(), 8LL * P);
+  FAP_CHECK_HIP(hipMemcpyAsync(meta, host.data(), copy_end, hipMemcpyHostToDevice, st));
+  FAP_CHECK_HIP(hipStreamSynchronize(st));
+  const Build64Desc *d_bd = reinterpret_cast<const Build64Desc *>(meta + o_bd);
+  const Fw64Desc *d_fd = reinterpret_cast<const Fw64Desc *>(meta + o_fd);
+  const Merge64Desc *d_md = reinterpret_cast<const Merge64Desc *>(meta + o_md);
+  const Bdy64Desc *d_yd = reinterpret_cast<const Bdy64Desc *>(meta + o_yd);
+  const int *d_offsets = reinterpret_cast<const int *>(meta + o_off);
+  const int *d_sb = reinterpret_cast<const int *>(meta + o_sb);
+  const int64_t *d_or = reinterpret_cast<const int64_t *>(meta + o_or);
+  int *d_src = reinterpret_cast<int *>(meta + o_src);
+  int64_t *d_rw = reinterpret_cast<int64_t *>(meta + o_rw);
+
+  auto span = [&](int stage, auto &&fn) -> int {
+    hipEvent_t a, b;
+    FAP_TRY(timing_record(st, &a));
+    FAP_TRY(fn());
+    FAP_TRY(timing_record(st, &b));
+    timing_add_span(stage, a, b);
+    return FAP_OK;
+  };
+  // Parts [p0, p1) in one launch per stage.
+  auto run = [&](int p0, int p1, int nsrc, int nrows, int mnb, int ntiles, int64_t nbdy) -> int {
+    FAP_TRY(span(0, [&]() -> int {
+      if (nsrc == 0) return FAP_OK;
+      FAP_TRY(launch_gather_sources(d_offsets + p0, d_sb + p0, d_or + p0, p1 - p0, nsrc,
+                                    g->d_st2ed, 1, d_src, d_rw, st));
+      hipLaunchKernelGGL(sssp64_kernel, dim3((nsrc + kSWaves - 1) / kSWaves), dim3(64 * kSWaves), 0,
+                         st, g->d_row_offset, g->d_col, g->d_wf, n, d_src, nsrc,
+                         rows_base + row_off64[p0], ldr);
+      FAP_CHECK_LAUNCH();
+      return FAP_OK;
+    }));
+    FAP_TRY(span(1, [&]() -> int {
+      hipLaunchKernelGGL(build64_kernel, dim3(nrows), dim3(256), 0, st, d_bd + p0, p1 - p0, nrows,
+                         g->d_row_offset, g->d_col, g->d_wf, g->d_graph_id, g->d_st2ed, g->d_ed2st);
+      FAP_CHECK_LAUNCH();
+      return FAP_OK;
+    }));
+    FAP_TRY(span(2, [&]() { return launch_fw64(d_fd + p0, p1 - p0, mnb, st); }));
+    FAP_TRY(span(3, [&]() -> int {
+      if (ntiles == 0) return FAP_OK;
+      if (vec)
+        hipLaunchKernelGGL(merge64_kernel<true>, dim3(ntiles), dim3(256), 0, st, d_md + p0, p1 - p0,
+                           ntiles, n, g->d_graph_id, g->d_ed2st);
+      else
+        hipLaunchKernelGGL(merge64_kernel<false>, dim3(ntiles), dim3(256), 0, st, d_md + p0, p1 - p0,
+                           ntiles, n, g->d_graph_id, g->d_ed2st);
+      FAP_CHECK_LAUNCH();
+      return FAP_OK;
+    }));
+    return span(4, [&]() -> int {
+      if (nbdy == 0) return FAP_OK;
+      hipLaunchKernelGGL(bdy64_kernel, dim3((unsigned)((nbdy + 255) / 256)), dim3(256), 0, st,
+                         d_yd + p0, p1 - p0, nbdy, n, g->d_graph_id, g->d_ed2st);
+      FAP_CHECK_LAUNCH();
+      return FAP_OK;
+    });
+  };
+  if (batched) {
+    FAP_TRY(run(0, P, src_total, rows_total, max_nb, tile_total, bdy_items));
+  } else {
+    for (int p = 0; p < P; ++p)
+      FAP_TRY(run(p, p + 1, act[p].bdy, S[p], nb[p], tiles[p], (int64_t)act[p].bdy * n));
+  }
+  return timing_finish(st);
+}

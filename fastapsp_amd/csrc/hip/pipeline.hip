@@ -124,6 +124,31 @@ int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 }  // namespace
 
+// Stage timing shared with the float-weight pipeline (fp64.hip).
+int timing_start(hipStream_t st) {
+  g_events.used = 0;
+  g_spans.clear();
+  g_total[0] = g_total[1] = nullptr;
+  g_parts16 = g_redone = 0;
+  FAP_TRY(g_events.take(&g_total[0]));
+  FAP_CHECK_HIP(hipEventRecord(g_total[0], st));
+  return FAP_OK;
+}
+
+int timing_record(hipStream_t on, hipEvent_t *e) {
+  FAP_TRY(g_events.take(e));
+  FAP_CHECK_HIP(hipEventRecord(*e, on));
+  return FAP_OK;
+}
+
+void timing_add_span(int stage, hipEvent_t a, hipEvent_t b) { g_spans.push_back(Span{stage, a, b}); }
+
+int timing_finish(hipStream_t st) {
+  FAP_TRY(g_events.take(&g_total[1]));
+  FAP_CHECK_HIP(hipEventRecord(g_total[1], st));
+  return FAP_OK;
+}
+
 }  // namespace fap_hip
 
 using namespace fap_hip;

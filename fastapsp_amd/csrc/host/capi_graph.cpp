@@ -227,6 +227,13 @@ int fap_graph_get_csr(fap_graph *h, int32_t *row_offset, int32_t *col, int32_t *
   return FAP_OK;
 }
 
+int fap_graph_get_weights_f32(fap_graph *h, float *weight) {
+  FAP_HANDLE(h);
+  const fap::fapGraph &g = *h->g;
+  if (weight) memcpy(weight, g.weight.data(), 4 * g.weight.size());
+  return FAP_OK;
+}
+
 int fap_graph_get_layout(fap_graph *h, int32_t *graph_id, int32_t *st2ed, int32_t *ed2st,
                          int32_t *blk_num, int32_t *blk_off, int32_t *bdy_num) {
   FAP_HANDLE(h);
@@ -261,6 +268,22 @@ int fap_graph_get_subgraph_rows(fap_graph *h, int32_t id, int64_t row0, int64_t 
                                 int32_t *h_out) {
   FAP_HANDLE(h);
   return guarded([&] { h->g->getSubGraphDistanceRowsI32(id, row0, rows, h_out); });
+}
+
+int fap_graph_is_float(fap_graph *h, int32_t *is_float) {
+  FAP_HANDLE(h);
+  return guarded([&] { *is_float = h->g->isFloat() ? 1 : 0; });
+}
+
+int fap_graph_get_dist_rows_f32(fap_graph *h, int64_t row0, int64_t rows, float *h_out) {
+  FAP_HANDLE(h);
+  return guarded([&] { h->g->getDistanceRowsF32(row0, rows, h_out); });
+}
+
+int fap_graph_get_subgraph_rows_f32(fap_graph *h, int32_t id, int64_t row0, int64_t rows,
+                                    float *h_out) {
+  FAP_HANDLE(h);
+  return guarded([&] { h->g->getSubGraphDistanceRowsF32(id, row0, rows, h_out); });
 }
 
 int fap_graph_device_dist(fap_graph *h, int32_t **d_dist, int64_t *ld) {

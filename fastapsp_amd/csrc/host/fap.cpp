@@ -42,10 +42,15 @@ void fapGraph::init_from_csr(bool dir, bool wtd, int32_t k, const std::string &p
   if (partitioner != "metis")
     throw std::invalid_argument("Invalid partitioner type: " + partitioner +
                                 " (this build provides \"metis\" k-way semantics only)");
-  for (float w : weight)
-    if (!(w >= 0.0f) || w != std::floor(w) || w >= 16777216.0f)
-      throw FapError(FAP_ERR_INVALID,
-                     "weights must be non-negative integers below 2^24 for int32 distances");
+  // Non-negative integer weights below 2^24 run the int32 path (exact, as the
+  // reference's float path is for them); any other non-negative finite weights run
+  // the float path (FP64 arithmetic, float32 results).
+  float_weights = false;
+  for (float w : weight) {
+    if (!(w >= 0.0f) || !std::isfinite(w))
+      throw FapError(FAP_ERR_INVALID, "weights must be non-negative and finite");
+    if (w != std::floor(w) || w >= 16777216.0f) float_weights = true;
+  }
   Csr g;
   g.n = num_vertexs;
   g.adj_size = adj_size;
@@ -113,6 +118,8 @@ void fapGraph::release() {
                      d_ed2st,      d_comp, d_dist, d_sub_dist};
   for (int32_t *p : ptrs)
     if (p) fap_hip_free(p);
+  if (d_wf) fap_hip_free(d_wf);
+  d_wf = nullptr;
   d_row_offset = d_col = d_w = d_graph_id = d_st2ed = d_ed2st = d_comp = d_dist = d_sub_dist =
       nullptr;
 }
@@ -192,7 +199,10 @@ void fapGraph::upload() {
   bool unit = true;
   for (float w : weight)
     if (w != 1.0f) unit = false;
-  if (!unit) {
+  if (float_weights) {
+    check(fap_hip_malloc((void **)&d_wf, std::max<int64_t>(m, 1) * 4), "device allocation");
+    if (m > 0) check(fap_hip_memcpy_h2d(d_wf, weight.data(), m * 4, nullptr), "host to device copy");
+  } else if (!unit) {
     std::vector<int32_t> wi(weight.size());
     for (size_t i = 0; i < weight.size(); ++i) wi[i] = (int32_t)weight[i];
     put(&d_w, wi.data(), m);
@@ -237,9 +247,15 @@ int32_t fapGraph::solve(bool path_needed) {
     parts.push_back(fap_part_desc{p, C_BlockVer_offset[p], C_BlockVer_num[p], C_BlockBdy_num[p],
                                   (int64_t)C_BlockVer_offset[p]});
   fap_graph_dev g{num_vertexs, num_edges, d_row_offset, d_col, d_w,
-                  d_graph_id,  d_st2ed,   d_ed2st,      d_comp};
-  check(fap_hip_solve_parts(&g, parts.data(), (int32_t)parts.size(), d_dist, ld_dist, mode, nullptr),
-        "solve");
+                  d_graph_id,  d_st2ed,   d_ed2st,      d_comp, d_wf};
+  if (float_weights)
+    check(fap_hip_solve_parts_f32(&g, parts.data(), (int32_t)parts.size(),
+                                  reinterpret_cast<float *>(d_dist), ld_dist, mode, nullptr),
+          "solve");
+  else
+    check(fap_hip_solve_parts(&g, parts.data(), (int32_t)parts.size(), d_dist, ld_dist, mode,
+                              nullptr),
+          "solve");
   check(fap_hip_stream_sync(nullptr), "solve");
   add_stage_times();
   return 0;
@@ -255,8 +271,13 @@ int32_t fapGraph::solveSubGraph(int32_t id, bool path_needed) {
     check(fap_hip_malloc((void **)&d_sub_dist, rows * ld_dist * 4), "sub-graph allocation");
   fap_part_desc part{id, C_BlockVer_offset[id], C_BlockVer_num[id], C_BlockBdy_num[id], 0};
   fap_graph_dev g{num_vertexs, num_edges, d_row_offset, d_col, d_w,
-                  d_graph_id,  d_st2ed,   d_ed2st,      d_comp};
-  check(fap_hip_solve_parts(&g, &part, 1, d_sub_dist, ld_dist, mode, nullptr), "solveSubGraph");
+                  d_graph_id,  d_st2ed,   d_ed2st,      d_comp, d_wf};
+  if (float_weights)
+    check(fap_hip_solve_parts_f32(&g, &part, 1, reinterpret_cast<float *>(d_sub_dist), ld_dist,
+                                  mode, nullptr),
+          "solveSubGraph");
+  else
+    check(fap_hip_solve_parts(&g, &part, 1, d_sub_dist, ld_dist, mode, nullptr), "solveSubGraph");
   check(fap_hip_stream_sync(nullptr), "solveSubGraph");
   add_stage_times();
   current_subgraph_id = id;
@@ -266,13 +287,39 @@ int32_t fapGraph::solveSubGraph(int32_t id, bool path_needed) {
 
 std::vector<int32_t> fapGraph::getMapping() { return st2ed; }
 
-void fapGraph::getDistanceRowsI32(int64_t row0, int64_t rows, int32_t *out) {
-  if (!d_dist) throw FapError(FAP_ERR_STATE, "no solve() result");
-  if (row0 < 0 || rows < 0 || row0 + rows > num_vertexs)
+// Rows [row0, row0 + rows) of a resident result (4-byte entries, int32 or float32
+// bits) into out, n entries per row.
+void fapGraph::copy_rows(const int32_t *base, int64_t nrows, int64_t row0, int64_t rows,
+                         void *out) const {
+  if (!base) throw FapError(FAP_ERR_STATE, "no result to read");
+  if (row0 < 0 || rows < 0 || row0 + rows > nrows)
     throw FapError(FAP_ERR_INVALID, "row range out of bounds");
-  check(fap_hip_memcpy2d_d2h_i32(out, num_vertexs, d_dist + row0 * ld_dist, ld_dist, rows,
-                                 num_vertexs, nullptr),
-        "device to host copy");
+  if (rows > 0)
+    check(fap_hip_memcpy2d_d2h_i32(static_cast<int32_t *>(out), num_vertexs, base + row0 * ld_dist,
+                                   ld_dist, rows, num_vertexs, nullptr),
+          "device to host copy");
+}
+
+void fapGraph::require_int(const char *what) const {
+  if (float_weights)
+    throw FapError(FAP_ERR_STATE, std::string(what) +
+                                      ": float-weight graph, distances are float32 (use the "
+                                      "float getters)");
+}
+
+void fapGraph::getDistanceRowsI32(int64_t row0, int64_t rows, int32_t *out) {
+  require_int("getDistanceRowsI32");
+  if (!d_dist) throw FapError(FAP_ERR_STATE, "no solve() result");
+  copy_rows(d_dist, num_vertexs, row0, rows, out);
+}
+
+void fapGraph::getDistanceRowsF32(int64_t row0, int64_t rows, float *out) {
+  if (!d_dist) throw FapError(FAP_ERR_STATE, "no solve() result");
+  copy_rows(d_dist, num_vertexs, row0, rows, out);
+  if (!float_weights) {
+    const int32_t *raw = reinterpret_cast<const int32_t *>(out);
+    for (int64_t i = 0; i < rows * num_vertexs; ++i) out[i] = to_float(raw[i]);
+  }
 }
 
 std::vector<int32_t> fapGraph::getAllDistanceI32() {
@@ -284,6 +331,10 @@ std::vector<int32_t> fapGraph::getAllDistanceI32() {
 std::vector<float> fapGraph::getAllDistance() {
   const int64_t n = num_vertexs;
   std::vector<float> out((size_t)(n * n));
+  if (float_weights) {
+    getDistanceRowsF32(0, n, out.data());
+    return out;
+  }
   std::vector<int32_t> buf;
   const int64_t chunk = std::max<int64_t>(1, (64LL << 20) / std::max<int64_t>(n, 1));
   for (int64_t r = 0; r < n; r += chunk) {
@@ -308,6 +359,7 @@ std::vector<int32_t> fapGraph::getSubGraphIndex(int32_t id) {
 }
 
 std::vector<int32_t> fapGraph::getSubGraphDistanceI32(int32_t id) {
+  require_int("getSubGraphDistanceI32");
   if (id != current_subgraph_id || !d_sub_dist)
     throw FapError(FAP_ERR_STATE, "sub graph was not the last one solved");
   std::vector<int32_t> out((size_t)(sub_rows * num_vertexs));
@@ -319,17 +371,30 @@ std::vector<int32_t> fapGraph::getSubGraphDistanceI32(int32_t id) {
 }
 
 void fapGraph::getSubGraphDistanceRowsI32(int32_t id, int64_t row0, int64_t rows, int32_t *out) {
+  require_int("getSubGraphDistanceRowsI32");
   if (id != current_subgraph_id || !d_sub_dist)
     throw FapError(FAP_ERR_STATE, "sub graph was not the last one solved");
-  if (row0 < 0 || rows < 0 || row0 + rows > sub_rows)
-    throw FapError(FAP_ERR_INVALID, "row range out of bounds");
-  if (rows > 0)
-    check(fap_hip_memcpy2d_d2h_i32(out, num_vertexs, d_sub_dist + row0 * ld_dist, ld_dist, rows,
-                                   num_vertexs, nullptr),
-          "device to host copy");
+  copy_rows(d_sub_dist, sub_rows, row0, rows, out);
+}
+
+void fapGraph::getSubGraphDistanceRowsF32(int32_t id, int64_t row0, int64_t rows, float *out) {
+  if (id != current_subgraph_id || !d_sub_dist)
+    throw FapError(FAP_ERR_STATE, "sub graph was not the last one solved");
+  copy_rows(d_sub_dist, sub_rows, row0, rows, out);
+  if (!float_weights) {
+    const int32_t *raw = reinterpret_cast<const int32_t *>(out);
+    for (int64_t i = 0; i < rows * num_vertexs; ++i) out[i] = to_float(raw[i]);
+  }
 }
 
 std::vector<float> fapGraph::getSubGraphDistance(int32_t id) {
+  if (float_weights) {
+    if (id != current_subgraph_id || !d_sub_dist)
+      throw FapError(FAP_ERR_STATE, "sub graph was not the last one solved");
+    std::vector<float> out((size_t)(sub_rows * num_vertexs));
+    getSubGraphDistanceRowsF32(id, 0, sub_rows, out.data());
+    return out;
+  }
   std::vector<int32_t> d = getSubGraphDistanceI32(id);
   std::vector<float> out(d.size());
   for (size_t i = 0; i < d.size(); ++i) out[i] = to_float(d[i]);
@@ -345,6 +410,11 @@ std::vector<float> fapGraph::getDistanceFromOnePoint(int32_t v) {
   if (graph_id[v] != current_subgraph_id)
     throw FapError(FAP_ERR_STATE, "vertex is not in the last solved sub graph");
   const int64_t row = ed2st[v] - C_BlockVer_offset[graph_id[v]];
+  if (float_weights) {
+    std::vector<float> out(num_vertexs);
+    getSubGraphDistanceRowsF32(current_subgraph_id, row, 1, out.data());
+    return out;
+  }
   std::vector<int32_t> buf(num_vertexs);
   check(fap_hip_memcpy_d2h(buf.data(), d_sub_dist + row * ld_dist, (int64_t)num_vertexs * 4, nullptr),
         "device to host copy");

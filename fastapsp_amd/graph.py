@@ -146,6 +146,10 @@ class FapGraph:
         w = np.empty(max(m, 1), dtype=np.int32)
         L.check(L.load().fap_graph_get_csr(self._h, L.ptr_i32(ro), L.ptr_i32(col), L.ptr_i32(w)),
                 "csr")
+        if self.isFloat():  # float32 weights as stored
+            w = np.empty(max(m, 1), dtype=np.float32)
+            L.check(L.load().fap_graph_get_weights_f32(
+                self._h, w.ctypes.data_as(C.POINTER(C.c_float))), "weights")
         return {"row_offset": ro, "col": col[:m], "weight": w[:m]}
 
     def getMapping(self) -> np.ndarray:
@@ -160,12 +164,23 @@ class FapGraph:
     def getAllDistanceI32(self) -> np.ndarray:
         return self.getDistanceRowsI32(0, self.num_vertexs)
 
+    def isFloat(self) -> bool:
+        """True when a weight is not an integer below 2^24: solves run the float path
+        (FP64 arithmetic, float32 results) and the int32 getters raise."""
+        v = C.c_int32()
+        L.check(L.load().fap_graph_is_float(self._h, C.byref(v)), "isFloat")
+        return bool(v.value)
+
+    def getDistanceRowsF32(self, row0: int, rows: int) -> np.ndarray:
+        out = np.empty((rows, self.num_vertexs), dtype=np.float32)
+        L.check(L.load().fap_graph_get_dist_rows_f32(
+            self._h, int(row0), int(rows), out.ctypes.data_as(C.POINTER(C.c_float))),
+            "getDistanceRowsF32")
+        return out
+
     def getAllDistance(self) -> np.ndarray:
         """float32 n x n, unreachable = MAXVALUE (1e8) as in the reference."""
-        d = self.getAllDistanceI32()
-        out = d.astype(np.float32)
-        out[d >= L.FAP_INF] = L.MAXVALUE
-        return out
+        return self.getDistanceRowsF32(0, self.num_vertexs)
 
     def getSubGraphIndex(self, sub_graph_id: int) -> np.ndarray:
         lay = self.layout()
@@ -187,11 +202,17 @@ class FapGraph:
                 "getSubGraphDistanceRows")
         return out
 
-    def getSubGraphDistance(self, sub_graph_id: int) -> np.ndarray:
-        d = self.getSubGraphDistanceI32(sub_graph_id)
-        out = d.astype(np.float32)
-        out[d >= L.FAP_INF] = L.MAXVALUE
+    def getSubGraphDistanceRowsF32(self, sub_graph_id: int, row0: int, rows: int) -> np.ndarray:
+        out = np.empty((rows, self.num_vertexs), dtype=np.float32)
+        L.check(L.load().fap_graph_get_subgraph_rows_f32(
+            self._h, int(sub_graph_id), int(row0), int(rows),
+            out.ctypes.data_as(C.POINTER(C.c_float))), "getSubGraphDistanceRowsF32")
         return out
+
+    def getSubGraphDistance(self, sub_graph_id: int) -> np.ndarray:
+        """float32 s_p x n block, unreachable = MAXVALUE (1e8)."""
+        num = int(self.layout()["num"][sub_graph_id])
+        return self.getSubGraphDistanceRowsF32(sub_graph_id, 0, num)
 
     def getCurrentSubGraphId(self) -> int:
         return getattr(self, "_current", 0)

@@ -76,6 +76,8 @@ class fapGraph {
   int32_t *d_row_offset = nullptr, *d_col = nullptr, *d_w = nullptr;
   int32_t *d_graph_id = nullptr, *d_st2ed = nullptr, *d_ed2st = nullptr;
   int32_t *d_comp = nullptr;      // component labels (symmetric graphs only)
+  float *d_wf = nullptr;          // float32 weights (float-weight graphs)
+  bool float_weights = false;     // some weight is not an integer below 2^24
   int32_t *d_dist = nullptr;      // n x ld_dist, result of solve()
   int32_t *d_sub_dist = nullptr;  // s_p x ld_dist, result of solveSubGraph()
   int64_t ld_dist = 0;
@@ -85,6 +87,8 @@ class fapGraph {
   void upload();
   void release();
   void add_stage_times();
+  void copy_rows(const int32_t *base, int64_t nrows, int64_t row0, int64_t rows, void *out) const;
+  void require_int(const char *what) const;
 
  public:
   int32_t num_vertexs = 0;
@@ -137,8 +141,14 @@ class fapGraph {
   std::vector<int32_t> getSubGraphPath(int32_t sub_graph_id);
   std::vector<float> getDistanceFromOnePoint(int32_t vertex_id);
   std::vector<int32_t> getPathFromOnePoint(int32_t vertex_id);
-  // Rows [row0, row0 + rows) of the solve() result (int32), without a full copy.
+  // Rows [row0, row0 + rows) of the solve() result, without a full copy: int32
+  // (integer-weight graphs only) or float32 with MAXVALUE for unreachable (any graph).
   void getDistanceRowsI32(int64_t row0, int64_t rows, int32_t *out);
+  void getDistanceRowsF32(int64_t row0, int64_t rows, float *out);
+  void getSubGraphDistanceRowsF32(int32_t sub_graph_id, int64_t row0, int64_t rows, float *out);
+  // True when the graph has a weight that is not an integer below 2^24: solves then
+  // run the float path (FP64 arithmetic, float32 results) and the int32 getters throw.
+  bool isFloat() const { return float_weights; }
   // Device pointer / row stride of the resident solve() result.
   const int32_t *deviceDistance(int64_t *ld) const;
 

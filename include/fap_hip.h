@@ -151,6 +151,8 @@ typedef struct fap_graph_dev {
    * every arc u->v has a reverse arc v->u (so reachability is symmetric); it enables
    * the 16-bit Floyd-Warshall of fap_hip_solve_parts. */
   const int32_t *d_comp;
+  /* [m] float32 arc weights for fap_hip_solve_parts_f32, or NULL (unit weights). */
+  const float *d_wf;
 } fap_graph_dev;
 
 /* One part: C_BlockVer_offset / C_BlockVer_num / C_BlockBdy_num of
@@ -191,6 +193,15 @@ typedef struct fap_stage_times {
 int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *parts, int32_t nparts,
                         int32_t *d_out, int64_t ld_out, int32_t mode, void *stream);
 int fap_hip_last_stage_times(fap_stage_times *t);
+
+/* The same pipeline for float32 weights (g->d_wf), the reference's float path:
+ * distances are carried in FP64 (exact sums of float32 weights while a path sum
+ * spans fewer than 53 significant bits) and written as float32, with MAXVALUE = 1e8
+ * (fap/parameter.h:26) for unreachable entries and distances >= 1e8.  Output rows as
+ * in fap_hip_solve_parts.  OVERLAP runs as BATCHED.  Workspace: FP64 FW matrices and
+ * FP64 SSSP rows (sum of b_p x n doubles). */
+int fap_hip_solve_parts_f32(const fap_graph_dev *g, const fap_part_desc *parts, int32_t nparts,
+                            float *d_out, int64_t ld_out, int32_t mode, void *stream);
 
 /* --------------------------------------------------------------- graph API */
 /* C wrapper of the C++ fap::fapGraph (include/fap/fap.h) for FFI callers
@@ -238,6 +249,8 @@ int fap_graph_sizes(fap_graph *g, int32_t *n, int64_t *m, int32_t *K);
  * (graph_id[n], st2ed[n], ed2st[n]) and per-part counts (num[K+1], offset[K+1],
  * bdy[K+1], index 0 unused as in the reference).  NULL outputs are skipped. */
 int fap_graph_get_csr(fap_graph *g, int32_t *row_offset, int32_t *col, int32_t *weight);
+/* The arc weights as stored (float32, CSR order); fap_graph_get_csr truncates to int32. */
+int fap_graph_get_weights_f32(fap_graph *g, float *weight);
 int fap_graph_get_layout(fap_graph *g, int32_t *graph_id, int32_t *st2ed, int32_t *ed2st,
                          int32_t *blk_num, int32_t *blk_off, int32_t *bdy_num);
 /* Rows [row0, row0 + rows) of the full n x n result of solve(), int32 (D2H). */
@@ -247,6 +260,14 @@ int fap_graph_get_subgraph_dist(fap_graph *g, int32_t id, int32_t *h_out);
 /* Rows [row0, row0 + rows) of that block (D2H of a slice instead of the whole block). */
 int fap_graph_get_subgraph_rows(fap_graph *g, int32_t id, int64_t row0, int64_t rows,
                                 int32_t *h_out);
+/* *is_float = 1 when some weight is not an integer below 2^24: solves then run
+ * fap_hip_solve_parts_f32 and the int32 getters return FAP_ERR_STATE. */
+int fap_graph_is_float(fap_graph *g, int32_t *is_float);
+/* Float32 rows (MAXVALUE = 1e8 for unreachable) of the solve() result / of the last
+ * solveSubGraph block, for any graph. */
+int fap_graph_get_dist_rows_f32(fap_graph *g, int64_t row0, int64_t rows, float *h_out);
+int fap_graph_get_subgraph_rows_f32(fap_graph *g, int32_t id, int64_t row0, int64_t rows,
+                                    float *h_out);
 /* Device pointer and row stride of the resident solve() result. */
 int fap_graph_device_dist(fap_graph *g, int32_t **d_dist, int64_t *ld);
 /* Stage times (s): partitioning, sssp, floyd, min-plus, data transformation. */

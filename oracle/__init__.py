@@ -132,6 +132,19 @@ def exact_rows(row_offset, col, weight, sources):
     return out
 
 
+def exact_rows_f64(row_offset, col, weight, sources):
+    """FP64 Dijkstra rows (float32 weights widened to double; None = unit weights),
+    +inf for unreachable: the oracle of the float-weight path."""
+    ro, cv, src = i32(row_offset), i32(col), i32(sources)
+    n = len(ro) - 1
+    out = np.empty((len(src), n), np.float64)
+    wa = None if weight is None else np.ascontiguousarray(weight, np.float32)
+    wp = None if wa is None else _p(wa, C.c_float)
+    lib().orc_exact_rows_f64(C.c_int(n), _p(ro, C.c_int32), _p(cv, C.c_int32), wp,
+                             _p(src, C.c_int32), C.c_int(len(src)), _p(out, C.c_double))
+    return out
+
+
 def floyd_i32(mat):
     m = i32(mat).copy()
     s = m.shape[0]

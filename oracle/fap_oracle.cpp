@@ -17,12 +17,15 @@
 //                             through orc_solve == exact BFS / Dijkstra APSP (a
 //                             wrong index in any of them breaks that equality) and
 //                             permutation round trips.
+//   orc_exact_rows_f64    FP64 Dijkstra: equals orc_exact_rows on integer weights and
+//                             a numpy float64 Floyd-Warshall on small float graphs.
 #include <omp.h>
 
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <queue>
 #include <utility>
 #include <vector>
@@ -233,6 +236,35 @@ void orc_exact_rows(int n, const int *row_offset, const int *col, const int *w, 
             d[col[e]] = (int32_t)nd;
             q.push({nd, col[e]});
           }
+        }
+      }
+    }
+  }
+}
+
+// FP64 oracle for float32 weights: Dijkstra in double (w == NULL: unit weights),
+// one row per source, +inf for unreachable.  The target of the float path's
+// relative-1e-6 parity.  OpenMP over sources.
+void orc_exact_rows_f64(int n, const int *row_offset, const int *col, const float *w,
+                        const int *src, int nsrc, double *out) {
+  const double inf = std::numeric_limits<double>::infinity();
+#pragma omp parallel for schedule(dynamic)
+  for (int i = 0; i < nsrc; ++i) {
+    double *d = out + (int64_t)i * n;
+    std::fill_n(d, n, inf);
+    d[src[i]] = 0.0;
+    typedef std::pair<double, int> P;
+    std::priority_queue<P, std::vector<P>, std::greater<P>> q;
+    q.push({0.0, src[i]});
+    while (!q.empty()) {
+      const P x = q.top();
+      q.pop();
+      if (x.first > d[x.second]) continue;
+      for (int e = row_offset[x.second]; e < row_offset[x.second + 1]; ++e) {
+        const double nd = x.first + (w ? (double)w[e] : 1.0);
+        if (nd < d[col[e]]) {
+          d[col[e]] = nd;
+          q.push({nd, col[e]});
         }
       }
     }

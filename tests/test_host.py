@@ -168,12 +168,13 @@ def test_api_errors():
     assert e.value.status == L.FAP_ERR_IO
     ro = np.array([0, 1, 2], np.int32)
     col = np.array([1, 0], np.int32)
-    with pytest.raises(F.FapError) as e:
-        F.FapGraph.from_csr(ro, col, np.array([1.5, 1.5], np.float32), True, 1)
-    assert e.value.status == L.FAP_ERR_INVALID
-    with pytest.raises(F.FapError) as e:
-        F.FapGraph.from_csr(ro, col, np.array([-1.0, 1.0], np.float32), True, 1)
-    assert e.value.status == L.FAP_ERR_INVALID
+    # Non-integer weights select the float path; negative or NaN weights are refused.
+    assert F.FapGraph.from_csr(ro, col, np.array([1.5, 1.5], np.float32), True, 1).isFloat()
+    assert not F.FapGraph.from_csr(ro, col, np.array([2.0, 7.0], np.float32), True, 1).isFloat()
+    for bad in ([-1.0, 1.0], [np.nan, 1.0], [np.inf, 1.0]):
+        with pytest.raises(F.FapError) as e:
+            F.FapGraph.from_csr(ro, col, np.array(bad, np.float32), True, 1)
+        assert e.value.status == L.FAP_ERR_INVALID
     g = F.FapGraph.from_csr(ro, col, None, False, 1)
     with pytest.raises(F.FapError) as e:
         g.solve()  # before preCondition

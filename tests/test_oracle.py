@@ -7,6 +7,7 @@ import os
 import numpy as np
 import pytest
 
+import fastapsp_amd as F
 import oracle as O
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -177,3 +178,32 @@ def test_exact_rows_detects_a_planted_error():
     bad = got.copy()
     bad[3, lay["st2ed"][5]] += 1
     assert np.array_equal(got, want) and not np.array_equal(bad, want)
+
+
+def test_exact_rows_f64_pins():
+    """The FP64 oracle of the float path: equal to the integer Dijkstra on integer
+    weights, and to a numpy float64 Floyd-Warshall on small float-weighted digraphs
+    (float32 weights in [0.5, 10): every path sum is exact in float64)."""
+    g = F.FapGraph.knn(800, 3, 50, 5, 4)
+    csr = g.csr()
+    src = np.arange(0, 800, 37)
+    a = O.exact_rows(csr["row_offset"], csr["col"], csr["weight"], src)
+    b = O.exact_rows_f64(csr["row_offset"], csr["col"], csr["weight"].astype(np.float32), src)
+    assert np.array_equal(np.where(np.isinf(b), F.FAP_INF, b).astype(np.int64), a)
+    rng = np.random.default_rng(11)
+    for n, m in ((40, 120), (60, 150)):
+        u = rng.integers(0, n, m)
+        v = rng.integers(0, n, m)
+        w = (0.5 + 9.5 * rng.random(m)).astype(np.float32)
+        order = np.argsort(u, kind="stable")
+        ro = np.zeros(n + 1, np.int32)
+        ro[1:] = np.cumsum(np.bincount(u, minlength=n))
+        col, wt = v[order].astype(np.int32), w[order]
+        D = np.full((n, n), np.inf)
+        for a_, b_, w_ in zip(u, v, w):
+            D[a_, b_] = min(D[a_, b_], float(w_))
+        np.fill_diagonal(D, 0.0)
+        for k in range(n):
+            D = np.minimum(D, D[:, k:k + 1] + D[k:k + 1, :])
+        rows = O.exact_rows_f64(ro, col, wt, np.arange(n))
+        assert np.array_equal(rows, D)
