@@ -6,7 +6,8 @@ the reference's fap::fapGraph API.
 """
 from ._lib import (FAP_INF, FAP_MODE_BATCHED, FAP_MODE_PLAIN, FAP_MODE_OVERLAP, FAP_MODE_FLAG_INT32_FW, MAXVALUE,
                    DeviceBuffer, FapError, init, load)
+from .batch import Batch, solve_batch
 from .graph import FapGraph
 
-__all__ = ["FapGraph", "FapError", "DeviceBuffer", "init", "load", "FAP_INF", "MAXVALUE",
+__all__ = ["FapGraph", "Batch", "solve_batch", "FapError", "DeviceBuffer", "init", "load", "FAP_INF", "MAXVALUE",
            "FAP_MODE_PLAIN", "FAP_MODE_BATCHED", "FAP_MODE_OVERLAP", "FAP_MODE_FLAG_INT32_FW"]

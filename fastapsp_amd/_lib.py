@@ -29,6 +29,7 @@ FAP_MODE_PLAIN = 0
 FAP_MODE_BATCHED = 1
 FAP_MODE_OVERLAP = 2
 FAP_MODE_FLAG_INT32_FW = 0x100
+FAP_BATCH_GRAPH = 0x1
 
 # Every exported symbol of include/fap_hip.h: name -> (restype, argtypes).
 _i32, _i64, _u64, _f64 = C.c_int32, C.c_int64, C.c_uint64, C.c_double
@@ -48,6 +49,11 @@ class FapGraphDev(C.Structure):
 class FapPartDesc(C.Structure):
     _fields_ = [("part_id", _i32), ("offset", _i32), ("num", _i32), ("bdy", _i32),
                 ("out_row", _i64)]
+
+
+class FapBatchItem(C.Structure):
+    _fields_ = [("g", FapGraphDev), ("parts", C.POINTER(FapPartDesc)), ("nparts", _i32),
+                ("is_float", _i32), ("d_out", _vp), ("ld_out", _i64)]
 
 
 class FapStageTimes(C.Structure):
@@ -87,6 +93,12 @@ SIGNATURES = {
     "fap_hip_last_stage_times": (C.c_int, [C.POINTER(FapStageTimes)]),
     "fap_hip_solve_parts_f32": (C.c_int, [C.POINTER(FapGraphDev), C.POINTER(FapPartDesc), _i32,
                                           _vp, _i64, _i32, _vp]),
+    "fap_hip_batch_create": (C.c_int, [C.POINTER(_vp), C.POINTER(FapBatchItem), _i32, _i32]),
+    "fap_hip_batch_run": (C.c_int, [_vp, _vp]),
+    "fap_hip_batch_destroy": (C.c_int, [_vp]),
+    "fap_hip_solve_batch": (C.c_int, [C.POINTER(FapBatchItem), _i32, _i32, _vp]),
+    "fap_graph_batch_create": (C.c_int, [C.POINTER(_vp), C.POINTER(_vp), _i32, _i32]),
+    "fap_graph_solve_batch": (C.c_int, [C.POINTER(_vp), _i32, _i32]),
     "fap_graph_create_from_file": (C.c_int, [C.POINTER(_vp), _cp, C.c_int, C.c_int, _i32, _cp]),
     "fap_mtx_read": (C.c_int, [_cp, C.c_int, C.c_int, _pi32, _pi64, _pi32, _pi32, _pf32]),
     "fap_graph_create_from_csr": (C.c_int, [C.POINTER(_vp), _i32, _i64, _pi32, _pi32, _pf32,

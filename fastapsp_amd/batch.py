@@ -1,0 +1,63 @@
+"""Many independent graphs solved together (the batched form for many graphs).
+
+`Batch(graphs)` prepares one work table over every part of every graph (after each
+graph's preCondition) and, with `use_graph=True`, captures the launch sequence as a
+hipGraph; `run()` replays it: one persistent launch per distance type in which
+workgroups pull parts from a device counter.  Results land in each graph's solve()
+buffer, so the graphs' own getters return them.  `solve_batch(graphs, mode)` is the
+one-shot form; FAP_MODE_PLAIN there solves graph after graph with per-part launches
+(the reference's structure) for comparison.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Sequence
+
+from . import _lib as L
+
+
+def _handles(graphs) -> C.Array:
+    arr = (C.c_void_p * len(graphs))()
+    for i, g in enumerate(graphs):
+        arr[i] = g._h
+    return arr
+
+
+class Batch:
+    def __init__(self, graphs: Sequence, use_graph: bool = True):
+        self.graphs = list(graphs)
+        for g in self.graphs:
+            if not getattr(g, "_preconditioned", False):
+                g.preCondition()
+        self._h = None
+        h = C.c_void_p()
+        L.check(L.load().fap_graph_batch_create(C.byref(h), _handles(self.graphs),
+                                                len(self.graphs),
+                                                L.FAP_BATCH_GRAPH if use_graph else 0),
+                "batch create")
+        self._h = h.value
+
+    def run(self, sync: bool = True) -> None:
+        """Run (or replay) the batch on the library stream."""
+        L.check(L.load().fap_hip_batch_run(self._h, None), "batch run")
+        if sync:
+            L.check(L.load().fap_hip_stream_sync(None), "batch sync")
+
+    def close(self) -> None:
+        if self._h is not None:
+            L.load().fap_hip_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_batch(graphs: Sequence, mode: int = L.FAP_MODE_BATCHED) -> None:
+    for g in graphs:
+        if not getattr(g, "_preconditioned", False):
+            g.preCondition()
+    L.check(L.load().fap_graph_solve_batch(_handles(graphs), len(graphs), int(mode)),
+            "solve batch")

@@ -311,3 +311,47 @@ int fap_graph_edge_cut(fap_graph *h, double *edge_cut) {
 }
 
 }  // extern "C"
+
+namespace {
+// Batch items of a list of graphs; parts holds the part arrays the items point to.
+void batch_items(fap_graph **graphs, int32_t count, std::vector<fap_batch_item> &items,
+                 std::vector<std::vector<fap_part_desc>> &parts) {
+  if (count < 0 || (count > 0 && !graphs))
+    throw fap::FapError(FAP_ERR_INVALID, "graph list is null");
+  items.resize(count);
+  parts.resize(count);
+  for (int32_t i = 0; i < count; ++i) {
+    if (!graphs[i] || !graphs[i]->g) throw fap::FapError(FAP_ERR_INVALID, "null graph in list");
+    fap::fapGraph &g = *graphs[i]->g;
+    parts[i] = g.partList();
+    fap_batch_item &it = items[i];
+    it.g = g.deviceView();
+    it.parts = parts[i].data();
+    it.nparts = (int32_t)parts[i].size();
+    it.is_float = g.isFloat() ? 1 : 0;
+    it.d_out = g.resultBuffer(&it.ld_out);
+  }
+}
+}  // namespace
+
+int fap_graph_batch_create(fap_batch **b, fap_graph **graphs, int32_t count, int32_t flags) {
+  if (!b) return FAP_ERR_INVALID;
+  return guarded([&] {
+    std::vector<fap_batch_item> items;
+    std::vector<std::vector<fap_part_desc>> parts;
+    batch_items(graphs, count, items, parts);
+    const int s = fap_hip_batch_create(b, items.data(), count, flags);
+    if (s != FAP_OK) throw fap::FapError(s, fap_hip_last_error());
+  });
+}
+
+int fap_graph_solve_batch(fap_graph **graphs, int32_t count, int32_t mode) {
+  return guarded([&] {
+    std::vector<fap_batch_item> items;
+    std::vector<std::vector<fap_part_desc>> parts;
+    batch_items(graphs, count, items, parts);
+    int s = fap_hip_solve_batch(items.data(), count, mode, nullptr);
+    if (s == FAP_OK) s = fap_hip_stream_sync(nullptr);
+    if (s != FAP_OK) throw fap::FapError(s, fap_hip_last_error());
+  });
+}

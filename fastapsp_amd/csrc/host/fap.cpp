@@ -427,6 +427,30 @@ std::vector<int32_t> fapGraph::getPathFromOnePoint(int32_t) {
   throw FapError(FAP_ERR_STATE, "predecessor matrices are not produced");
 }
 
+fap_graph_dev fapGraph::deviceView() {
+  if (!preconditioned) throw FapError(FAP_ERR_STATE, "batch use before preCondition()");
+  if (!d_row_offset) upload();
+  return fap_graph_dev{num_vertexs, num_edges, d_row_offset, d_col, d_w,
+                       d_graph_id,  d_st2ed,   d_ed2st,      d_comp, d_wf};
+}
+
+std::vector<fap_part_desc> fapGraph::partList() const {
+  std::vector<fap_part_desc> parts;
+  for (int p = 1; p <= K; ++p)
+    parts.push_back(fap_part_desc{p, C_BlockVer_offset[p], C_BlockVer_num[p], C_BlockBdy_num[p],
+                                  (int64_t)C_BlockVer_offset[p]});
+  return parts;
+}
+
+void *fapGraph::resultBuffer(int64_t *ld) {
+  if (!d_row_offset) upload();
+  if (!d_dist)
+    check(fap_hip_malloc((void **)&d_dist, (int64_t)num_vertexs * ld_dist * 4),
+          "n x n distance allocation");
+  if (ld) *ld = ld_dist;
+  return d_dist;
+}
+
 const int32_t *fapGraph::deviceDistance(int64_t *ld) const {
   if (ld) *ld = ld_dist;
   return d_dist;

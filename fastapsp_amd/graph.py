@@ -100,6 +100,7 @@ class FapGraph:
 
     def preCondition(self) -> None:
         L.check(L.load().fap_graph_precondition(self._h), "preCondition")
+        self._preconditioned = True
 
     def isSplit(self) -> bool:
         v = C.c_int()

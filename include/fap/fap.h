@@ -152,6 +152,13 @@ class fapGraph {
   // Device pointer / row stride of the resident solve() result.
   const int32_t *deviceDistance(int64_t *ld) const;
 
+  // For many-graph batches (fap_hip_batch_*): the device view (uploaded on first
+  // use), the parts 1..K with out_row = offset, and the n x n result buffer of
+  // solve() (int32, or float32 bits for float-weight graphs; allocated on first use).
+  fap_graph_dev deviceView();
+  std::vector<fap_part_desc> partList() const;
+  void *resultBuffer(int64_t *ld);
+
   double getTimeSSSP();
   double getTimeFloyd();
   double getTimeMinPlus();

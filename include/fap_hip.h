@@ -203,6 +203,38 @@ int fap_hip_last_stage_times(fap_stage_times *t);
 int fap_hip_solve_parts_f32(const fap_graph_dev *g, const fap_part_desc *parts, int32_t nparts,
                             float *d_out, int64_t ld_out, int32_t mode, void *stream);
 
+/* ------------------------------------------------------------ many graphs */
+/* One graph of a batch: its device view, its parts (out_row must equal offset: the
+ * full n x n result of solve()), the result buffer (int32, or float32 when is_float:
+ * weights in g.d_wf) with row stride ld_out. */
+typedef struct fap_batch_item {
+  fap_graph_dev g;
+  const fap_part_desc *parts;
+  int32_t nparts;
+  int32_t is_float;
+  void *d_out;
+  int64_t ld_out;
+} fap_batch_item;
+
+/* A prepared batch: the work table of every part of every graph (largest first),
+ * one scratch slot per resident workgroup, and optionally the captured launch
+ * sequence (FAP_BATCH_GRAPH: fap_hip_batch_run replays a hipGraph).  Running it is
+ * one persistent launch per distance type (int32, FP64) in which workgroups pull
+ * parts from a device counter and run a part's whole pipeline; parts of at most
+ * 4096 vertices.  The item arrays are copied; device buffers must outlive the batch.
+ * No counterpart in the reference (its solve() loops over parts of one graph,
+ * fap/fap.cpp:507-518). */
+typedef struct fap_batch fap_batch;
+#define FAP_BATCH_GRAPH 0x1
+int fap_hip_batch_create(fap_batch **b, const fap_batch_item *items, int32_t nitems,
+                         int32_t flags);
+int fap_hip_batch_run(fap_batch *b, void *stream);
+int fap_hip_batch_destroy(fap_batch *b);
+/* Solve every item: FAP_MODE_PLAIN = graph after graph, each the per-part launch
+ * sequence (int32 FW); FAP_MODE_BATCHED / OVERLAP = one persistent launch (create,
+ * run, wait, destroy). */
+int fap_hip_solve_batch(const fap_batch_item *items, int32_t nitems, int32_t mode, void *stream);
+
 /* --------------------------------------------------------------- graph API */
 /* C wrapper of the C++ fap::fapGraph (include/fap/fap.h) for FFI callers
  * (ctypes, cgo).  Mirrors fap/fap.h:100-150.  Opaque handle. */
@@ -274,6 +306,11 @@ int fap_graph_device_dist(fap_graph *g, int32_t **d_dist, int64_t *ld);
 int fap_graph_times(fap_graph *g, double *t_part, double *t_sssp, double *t_floyd,
                     double *t_minplus, double *t_data);
 int fap_graph_edge_cut(fap_graph *g, double *edge_cut);
+/* Batches of graphs (after preCondition): fap_hip_batch_create / fap_hip_solve_batch
+ * over the graphs' device views and their solve() result buffers, so the graphs'
+ * getters return the batch results. */
+int fap_graph_batch_create(fap_batch **b, fap_graph **graphs, int32_t count, int32_t flags);
+int fap_graph_solve_batch(fap_graph **graphs, int32_t count, int32_t mode);
 
 #ifdef __cplusplus
 }

@@ -37,7 +37,7 @@ def test_struct_layouts_match_header(tmp_path):
     """sizeof and every field offset of the ctypes structs equal what a C compiler
     reads from include/fap_hip.h."""
     structs = {"fap_graph_dev": L.FapGraphDev, "fap_part_desc": L.FapPartDesc,
-               "fap_stage_times": L.FapStageTimes}
+               "fap_stage_times": L.FapStageTimes, "fap_batch_item": L.FapBatchItem}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fap_hip.h"',
              "int main(void) {"]
     for cname, cls in structs.items():
