@@ -154,12 +154,91 @@ def minplus_bench(N, reps=3):
         sys.exit(1)
 
 
+def batch_bench(args):
+    """Many-graph batch: 512 road-like graphs (64x64-grid construction scaled: 10% of edges
+    dropped, 2% diagonals, largest component) with log-uniform sizes in [256, 8192], half
+    unweighted (int32) and half with float32 weights uniform in [0.5, 10) (FP64 arithmetic,
+    float32 results); K per graph and workgroups per CU from the autotuner; a step is one
+    replay of the captured batch (one persistent launch per distance type)."""
+    from fastapsp_amd import autotune as AT
+    t0 = time.perf_counter()
+    table = AT.tune(k_candidates=(2, 4, 8, 16, 32, 64), per_cu_candidates=(1, 2), max_copies=32)
+    log(f"autotune {time.perf_counter() - t0:.1f}s: K per class {table['K']}, "
+        f"per_cu {table['per_cu']}")
+    rng = np.random.default_rng(2026)
+    sizes = np.exp(rng.uniform(np.log(256), np.log(8192), args.graphs)).astype(int)
+    t0 = time.perf_counter()
+    graphs = []
+    for i, nt in enumerate(sizes):
+        K = AT.k_for(int(nt), table)
+        graphs.append(AT.road_graph(int(nt), 5000 + i, K, float_weights=(i % 2 == 1)))
+    for g in graphs:
+        g.preCondition()
+    ns = [g.num_vertexs for g in graphs]
+    entries = float(sum(n * n for n in ns))
+    log(f"{len(graphs)} graphs built in {time.perf_counter() - t0:.1f}s: n in [{min(ns)}, "
+        f"{max(ns)}], sum n^2 = {entries:.3e} ({entries * 4 / 1e9:.1f} GB of results)")
+    per_cu = AT.per_cu_for(ns, table)
+    b = F.Batch(graphs, use_graph=True, per_cu=per_cu)
+    for _ in range(args.warmup):
+        b.run()
+    L.check(L.load().fap_hip_device_sync(), "sync")
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        b.run(sync=False)
+    L.check(L.load().fap_hip_device_sync(), "sync")
+    ms = (time.perf_counter() - ts) * 1e3 / args.steps
+    value = entries / (ms * 1e-3)
+    log(f"batched (persistent, graph replay): {ms:.2f} ms/step, {value:.3e} entries/s")
+    plain = None
+    if not args.skip_plain:
+        tp = time.perf_counter()
+        F.solve_batch(graphs, F.FAP_MODE_PLAIN)
+        plain_ms = (time.perf_counter() - tp) * 1e3
+        plain = {"ms_per_step": plain_ms, "batched_vs_plain": plain_ms / ms}
+        log(f"plain (graph after graph, per-part launches): {plain_ms:.2f} ms "
+            f"({plain_ms / ms:.2f}x slower)")
+    cpu = None
+    if not args.skip_cpu:
+        import oracle as O
+        tc = time.perf_counter()
+        done, ent = 0, 0.0
+        for g in graphs:
+            if g.isFloat():
+                continue
+            O.solve_parts(1, g.K, g.csr(), g.layout())
+            done += 1
+            ent += float(g.num_vertexs) ** 2
+            if time.perf_counter() - tc >= args.cpu_budget:
+                break
+        dt = time.perf_counter() - tc
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = {"value": ent / dt, "unit": "entries/s", "cores": cores, "kind": "port",
+               "sample": f"{done} integer-weight graphs of the batch through the CPU path "
+                         f"({ent:.3e} entries, {dt:.1f} s)"}
+        log(f"cpu baseline: {cpu['value']:.3e} entries/s")
+    out = {"metric": "APSP distance entries/s, many-graph batch (int32 exact / FP32 at 1e-6)",
+           "value": value, "unit": "entries/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "int32+f64", "data": "synthetic",
+           "config": {"workload": f"{len(graphs)} road-like graphs, n log-uniform in [256, 8192], "
+                                  "half float32-weighted, one persistent launch per type",
+                      "graphs": len(graphs), "sum_n2": entries, "n_min": min(ns),
+                      "n_max": max(ns), "per_cu": per_cu},
+           "autotune": {"K": table["K"], "per_cu": table["per_cu"],
+                        "ms_per_graph": table["ms_per_graph"]},
+           "plain_form": plain, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="road1")
+    ap.add_argument("--config", default="road1",
+                    help="road1 (default), road0, or batch512 (many-graph batch)")
+    ap.add_argument("--graphs", type=int, default=512, help="graphs of the batch512 config")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -177,6 +256,12 @@ def main():
         return
     if args.minplus > 0:
         minplus_bench(args.minplus)
+        return
+    if args.config == "batch512":
+        if args.gpus != 1:
+            print("bench.py measures one GPU only", file=sys.stderr)
+            sys.exit(2)
+        batch_bench(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU sharding is "

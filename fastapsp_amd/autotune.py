@@ -1,0 +1,137 @@
+"""Autotuner of the many-graph batch form.
+
+For each size class of graphs (vertex counts [256, 512), [512, 1024), ..., [4096, 8192])
+it times candidate settings on a batch that fills the GPU -- copies of a road-like
+graph of the class's middle size, half of them with float weights -- and keeps the
+fastest:
+  * K, the number of parts per graph: it sets the Floyd-Warshall size s_p (work
+    ~ K * s_p^3), the boundary count (SSSP sources and merge depth) and the number of
+    independent work items the persistent kernel can spread over the CUs;
+  * the persistent workgroups per CU (occupancy against scratch-slot memory).
+The result is a small table (JSON-serialisable) that `k_for(n)` and `per_cu_for(n)`
+read; `load_or_tune(path)` caches it on disk.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from . import _lib as L
+from .batch import Batch
+from .graph import FapGraph
+
+SIZE_CLASSES = [(256, 512), (512, 1024), (1024, 2048), (2048, 4096), (4096, 8193)]
+K_CANDIDATES = (1, 2, 4, 8, 16, 32)
+PER_CU_CANDIDATES = (1, 2, 4)
+MAX_PART = 4096  # largest part of the batch form
+
+
+def symmetric_float_weights(ro, col, seed: int, directed: bool = False) -> np.ndarray:
+    """Deterministic float32 weights uniform in [0.5, 10) from a hash of the arc's end
+    points (of the unordered pair when undirected, so both arcs of an edge agree)."""
+    n = len(ro) - 1
+    src = np.repeat(np.arange(n, dtype=np.uint64), np.diff(ro))
+    dst = np.asarray(col).astype(np.uint64)
+    a, b = (src, dst) if directed else (np.minimum(src, dst), np.maximum(src, dst))
+    key = (a * np.uint64(n) + b) ^ np.uint64((seed * 0x5851F42D4C957F2D) % (1 << 64))
+    h = (key * np.uint64(0x9E3779B97F4A7C15)) >> np.uint64(40)
+    return (0.5 + 9.5 * (h.astype(np.float64) / float(1 << 24))).astype(np.float32)
+
+
+def road_graph(n_target: int, seed: int, K: int, float_weights: bool = False) -> FapGraph:
+    """Road-like graph of about n_target vertices (64x64-grid construction scaled:
+    10% of edges dropped, 2% diagonals, largest component), optionally with float32
+    weights uniform in [0.5, 10) (equal in both directions)."""
+    side = max(4, int(math.ceil(math.sqrt(n_target / 0.97))))
+    g = FapGraph.road_grid(side, side, 0.10, 0.02, seed, K)
+    if not float_weights:
+        return g
+    c = g.csr()
+    w = symmetric_float_weights(c["row_offset"], c["col"], seed)
+    return FapGraph.from_csr(c["row_offset"], c["col"], w, False, K)
+
+
+def size_class(n: int) -> int:
+    for i, (lo, hi) in enumerate(SIZE_CLASSES):
+        if n < hi:
+            return i
+    return len(SIZE_CLASSES) - 1
+
+
+def _time_batch(graphs, per_cu: int, reps: int = 2) -> float:
+    b = Batch(graphs, use_graph=False, per_cu=per_cu)
+    b.run()
+    best = math.inf
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        b.run()
+        best = min(best, time.perf_counter() - t0)
+    b.close()
+    return best
+
+
+def tune(copies_work: int = 512, max_copies: int = 48, seed: int = 4242,
+         k_candidates=K_CANDIDATES, per_cu_candidates=PER_CU_CANDIDATES,
+         verbose: bool = False) -> dict:
+    """Time every (K, workgroups-per-CU) candidate per size class; returns the table."""
+    table = {"classes": SIZE_CLASSES, "K": [], "per_cu": [], "ms_per_graph": [], "trials": []}
+    for ci, (lo, hi) in enumerate(SIZE_CLASSES):
+        n_mid = int(math.sqrt(lo * min(hi, 8192)))
+        best = None
+        trials = []
+        for K in k_candidates:
+            base = road_graph(n_mid, seed + ci, K)
+            if K > base.num_vertexs // 16:
+                continue
+            base.preCondition()
+            lay_num = base.layout()["num"][1:]
+            if lay_num.max() > MAX_PART:
+                continue
+            copies = int(min(max_copies, max(4, copies_work // K)))
+            graphs = [road_graph(n_mid, seed + ci, K, float_weights=(c % 2 == 1))
+                      for c in range(copies)]
+            for per_cu in per_cu_candidates:
+                dt = _time_batch(graphs, per_cu)
+                ms = dt * 1e3 / copies
+                trials.append({"K": K, "per_cu": per_cu, "ms_per_graph": ms})
+                if verbose:
+                    print(f"class {ci} n~{n_mid} K={K} per_cu={per_cu}: {ms:.3f} ms/graph",
+                          flush=True)
+                if best is None or ms < best[2]:
+                    best = (K, per_cu, ms)
+            del graphs
+        table["K"].append(best[0])
+        table["per_cu"].append(best[1])
+        table["ms_per_graph"].append(best[2])
+        table["trials"].append(trials)
+    return table
+
+
+def k_for(n: int, table: dict) -> int:
+    return int(table["K"][size_class(n)])
+
+
+def per_cu_for(ns, table: dict) -> int:
+    """Workgroups per CU for a mixed batch: the choice of the class holding the most
+    work (sum of n^2)."""
+    work = np.zeros(len(SIZE_CLASSES))
+    for n in ns:
+        work[size_class(n)] += float(n) * n
+    return int(table["per_cu"][int(np.argmax(work))])
+
+
+def load_or_tune(path: str | None = None, **kw) -> dict:
+    path = path or os.path.join(os.path.expanduser("~"), ".cache", "fastapsp_amd",
+                                "batch_tuning_gfx950.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    table = tune(**kw)
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(table, f)
+    return table

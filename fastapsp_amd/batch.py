@@ -24,7 +24,7 @@ def _handles(graphs) -> C.Array:
 
 
 class Batch:
-    def __init__(self, graphs: Sequence, use_graph: bool = True):
+    def __init__(self, graphs: Sequence, use_graph: bool = True, per_cu: int = 0):
         self.graphs = list(graphs)
         for g in self.graphs:
             if not getattr(g, "_preconditioned", False):
@@ -33,7 +33,8 @@ class Batch:
         h = C.c_void_p()
         L.check(L.load().fap_graph_batch_create(C.byref(h), _handles(self.graphs),
                                                 len(self.graphs),
-                                                L.FAP_BATCH_GRAPH if use_graph else 0),
+                                                (L.FAP_BATCH_GRAPH if use_graph else 0)
+                                                | ((int(per_cu) & 0xFF) << 8)),
                 "batch create")
         self._h = h.value
 

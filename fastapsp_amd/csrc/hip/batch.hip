@@ -1,17 +1,21 @@
 // Many independent graphs in one persistent launch fed by a device work queue.
 //
 // The work table holds one entry per part of every graph (the unit fapGraph::run
-// processes, fap/fap.cpp:195-503).  A persistent grid of 256-thread workgroups pulls
-// entries from an atomic counter, largest first (the host sorts the table by a cost
-// model), and runs the whole per-part pipeline inside the workgroup:
-//   1 boundary SSSP   one wave per boundary source, label-correcting relaxation
-//                     with atomicMin on the row, frontier queues in LDS
-//                     (batch_sssp.h:232-252);
+// processes, fap/fap.cpp:195-503).  Per distance type two launches:
+//   1 boundary SSSP   one wave per boundary source of every part of every graph,
+//                     label-correcting relaxation with atomicMin on the row,
+//                     frontier queues in LDS (batch_sssp.h:232-252).  SSSP is
+//                     latency-bound (levels x memory round trips), so it runs as
+//                     one wide launch where thousands of waves hide each other's
+//                     latency, not inside the per-part workgroups;
+// then a persistent grid of 512-thread workgroups pulls parts from an atomic
+// counter, largest first (the host sorts the table by a cost model), and runs the
+// rest of the per-part pipeline inside the workgroup (eight waves):
 //   2 build           the s x s matrix in the workgroup's scratch slot
 //                     (util_Myalgorithm.h:44-84, later arc wins, diagonal 0);
 //   3 Floyd-Warshall  blocked, 64 x 64 tiles: the pivot closed in LDS by the whole
 //                     workgroup, then the panel and remaining tiles shared by the
-//                     four waves, each tile a register-tiled min-plus product staged
+//                     waves, each tile a register-tiled min-plus product staged
 //                     through the wave's own LDS slab (floyd.h:38-49);
 //   4 merge           inner rows x n columns = FW(inner, boundary) (x) SSSP rows,
 //                     64 x 64 output tiles per wave, the part's own columns taken
@@ -36,8 +40,10 @@ namespace {
 constexpr int kBT = 64;        // tile edge
 constexpr int kBKc = 16;       // k per LDS chunk of a tile product
 constexpr int kBQ = 1024;      // frontier queue entries per wave
-constexpr int kBWaves = 4;     // waves per workgroup
+constexpr int kBWaves = 8;     // waves per workgroup of the persistent kernel
 constexpr int kBThreads = 64 * kBWaves;
+constexpr int kP1Per = kBT * kBT / kBThreads;  // pivot entries per thread in phase 1
+constexpr int kSWaves = 4;     // waves (sources) per workgroup of the SSSP launch
 constexpr int kBatchMaxS = 4096;  // largest part the batch form takes
 
 struct BatchWork {
@@ -46,7 +52,14 @@ struct BatchWork {
   const int *graph_id, *st2ed, *ed2st;
   void *out;                     // n x ld_out result of the graph (int32 or float32)
   int64_t ld_out;
+  void *rows;                    // the part's SSSP rows (int32: its boundary rows of out)
+  int64_t ldr;
   int n, part_id, offset, num, bdy, pad;
+};
+
+// One boundary source: part `work`, its q-th boundary vertex.
+struct BatchSrc {
+  int work, q;
 };
 
 template <typename T>
@@ -159,11 +172,28 @@ __device__ __forceinline__ void wave_product(T (&acc)[8][8], const T *a, int64_t
                                              const T *b, int64_t ldb, int bcols, int kdim, T *As,
                                              T *Bs, int lane) {
   const int tr = lane >> 3, tc = lane & 7;
+  typedef T vec4 __attribute__((ext_vector_type(4)));
   for (int k0 = 0; k0 < kdim; k0 += kBKc) {
+    if (lane < arows && k0 + kBKc <= kdim) {
+      // A rows start 64-element aligned and k0 is a multiple of 16: vector loads.
+      const vec4 *src = reinterpret_cast<const vec4 *>(a + (int64_t)lane * lda + k0);
+#pragma unroll
+      for (int c = 0; c < kBKc / 4; ++c) {
+        const vec4 v = src[c];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) As[(4 * c + e) * kBT + lane] = v[e];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < kBKc; ++q) {
+        const int k = k0 + q;
+        As[q * kBT + lane] =
+            (lane < arows && k < kdim) ? a[(int64_t)lane * lda + k] : Dist<T>::inf();
+      }
+    }
 #pragma unroll
     for (int q = 0; q < kBKc; ++q) {
       const int k = k0 + q;
-      As[q * kBT + lane] = (lane < arows && k < kdim) ? a[(int64_t)lane * lda + k] : Dist<T>::inf();
       Bs[q * kBT + lane] = (lane < bcols && k < kdim) ? b[(int64_t)k * ldb + lane] : Dist<T>::inf();
     }
     wave_sync();
@@ -204,22 +234,16 @@ __device__ __forceinline__ void store_tile(const T (&acc)[8][8], T *t, int64_t l
 
 // ------------------------------------------------------------------ one part
 template <typename T>
-__device__ void process_part(const BatchWork &wk, T *M, T *rows, int64_t ldr, char *lds) {
+__device__ void process_part(const BatchWork &wk, T *M, char *lds) {
+  const T *rows = static_cast<const T *>(wk.rows);
+  const int64_t ldr = wk.ldr;
   typedef typename Dist<T>::Out Out;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int num = wk.num, bdy = wk.bdy, off = wk.offset, n = wk.n;
   const int S = (num + kBT - 1) / kBT * kBT, nb = S / kBT;
   Out *out = static_cast<Out *>(wk.out);
 
-  // 1. boundary SSSP, one wave per source
-  {
-    int *queue = reinterpret_cast<int *>(lds) + wave * (2 * kBQ + 2);
-    for (int q = wave; q < bdy; q += kBWaves)
-      wave_sssp<T>(wk, wk.st2ed[off + q], rows + (int64_t)q * ldr, queue, queue + 2 * kBQ, lane);
-  }
-  __syncthreads();
-
-  // 2. build
+  // 2. build (the SSSP rows were written by batch_sssp_kernel)
   for (int i = 0; i < S; ++i)
     for (int j = tid; j < S; j += kBThreads)
       M[(int64_t)i * S + j] = (i < bdy && j < num) ? rows[(int64_t)i * ldr + wk.st2ed[off + j]]
@@ -242,22 +266,22 @@ __device__ void process_part(const BatchWork &wk, T *M, T *rows, int64_t ldr, ch
   T *As = reinterpret_cast<T *>(lds) + wave * 2 * kBKc * kBT;
   T *Bs = As + kBKc * kBT;
   for (int r = 0; r < nb; ++r) {
-    {  // phase 1: pivot tile closed in LDS, 16 entries per thread
+    {  // phase 1: pivot tile closed in LDS, kP1Per consecutive entries per thread
       T *D = reinterpret_cast<T *>(lds);
       T *g = M + (int64_t)r * kBT * S + r * kBT;
-      const int row = tid >> 2, c0 = (tid & 3) * 16;
+      const int row = tid / (kBT / kP1Per), c0 = (tid % (kBT / kP1Per)) * kP1Per;
 #pragma unroll
-      for (int c = 0; c < 16; ++c) D[row * kBT + c0 + c] = g[(int64_t)row * S + c0 + c];
+      for (int c = 0; c < kP1Per; ++c) D[row * kBT + c0 + c] = g[(int64_t)row * S + c0 + c];
       for (int k = 0; k < kBT; ++k) {
         __syncthreads();  // row k and column k do not change during step k
         const T dik = D[row * kBT + k];
 #pragma unroll
-        for (int c = 0; c < 16; ++c)
+        for (int c = 0; c < kP1Per; ++c)
           D[row * kBT + c0 + c] = Dist<T>::mn(D[row * kBT + c0 + c], dik + D[k * kBT + c0 + c]);
       }
       __syncthreads();
 #pragma unroll
-      for (int c = 0; c < 16; ++c) g[(int64_t)row * S + c0 + c] = D[row * kBT + c0 + c];
+      for (int c = 0; c < kP1Per; ++c) g[(int64_t)row * S + c0 + c] = D[row * kBT + c0 + c];
     }
     __syncthreads();
     // phase 2: row tiles (r, o) and column tiles (o, r), o != r
@@ -332,33 +356,37 @@ __device__ void process_part(const BatchWork &wk, T *M, T *rows, int64_t ldr, ch
   __syncthreads();
 }
 
-// Persistent grid: each workgroup owns one scratch slot (an S_max x S_max matrix,
-// plus b_max x n_max SSSP rows for the double path) and pulls parts until the
-// counter passes nwork.
+// Boundary SSSP of every part: one wave per source.
+template <typename T>
+__global__ __launch_bounds__(64 * kSWaves) void batch_sssp_kernel(const BatchWork *__restrict__ work,
+                                                                  const BatchSrc *__restrict__ srcs,
+                                                                  int nsrc) {
+  __shared__ int queues[kSWaves][2 * kBQ + 2];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * kSWaves + wave;
+  if (i >= nsrc) return;  // whole wave leaves; no workgroup barrier is used
+  const BatchSrc sr = srcs[i];
+  const BatchWork wk = work[sr.work];
+  T *row = static_cast<T *>(wk.rows) + (int64_t)sr.q * wk.ldr;
+  wave_sssp<T>(wk, wk.st2ed[wk.offset + sr.q], row, queues[wave], queues[wave] + 2 * kBQ, lane);
+}
+
+// Persistent grid: each workgroup owns one scratch slot (an S_max x S_max matrix)
+// and pulls parts until the counter passes nwork.
 template <typename T>
 __global__ __launch_bounds__(kBThreads) void batch_kernel(const BatchWork *__restrict__ work,
                                                           int nwork, int *counter, char *scratch,
-                                                          int64_t slot_bytes, int64_t mat_bytes,
-                                                          int64_t ldr64) {
+                                                          int64_t slot_bytes) {
   extern __shared__ __attribute__((aligned(16))) char lds[];
   __shared__ int item_s;
-  char *slot = scratch + (int64_t)blockIdx.x * slot_bytes;
-  T *M = reinterpret_cast<T *>(slot);
+  T *M = reinterpret_cast<T *>(scratch + (int64_t)blockIdx.x * slot_bytes);
   while (true) {
     if (threadIdx.x == 0) item_s = atomicAdd(counter, 1);
     __syncthreads();
     const int item = item_s;
     __syncthreads();
     if (item >= nwork) break;  // uniform over the workgroup
-    const BatchWork wk = work[item];
-    if (sizeof(T) == 4) {
-      // int32: the SSSP rows are the part's boundary rows of the result
-      process_part<T>(wk, M, reinterpret_cast<T *>(static_cast<int *>(wk.out) +
-                                                   (int64_t)wk.offset * wk.ld_out),
-                      wk.ld_out, lds);
-    } else {
-      process_part<T>(wk, M, reinterpret_cast<T *>(slot + mat_bytes), ldr64, lds);
-    }
+    process_part<T>(work[item], M, lds);
   }
 }
 
@@ -368,10 +396,9 @@ __global__ void reset_counter_kernel(int *c, int n) {
 }
 
 size_t lds_bytes(int elem) {
-  const size_t sssp = (size_t)kBWaves * (2 * kBQ + 2) * 4;
   const size_t pivot = (size_t)kBT * kBT * elem;
   const size_t slabs = (size_t)kBWaves * 2 * kBKc * kBT * elem;
-  return std::max(sssp, std::max(pivot, slabs));
+  return std::max(pivot, slabs);
 }
 
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
@@ -383,10 +410,11 @@ using namespace fap_hip;
 
 // One kind of entries (int32 or double) of a batch.
 struct fap_batch_kind {
-  int nwork = 0, grid = 0;
+  int nwork = 0, grid = 0, nsrc = 0;
   BatchWork *d_work = nullptr;
-  char *d_scratch = nullptr;
-  int64_t slot_bytes = 0, mat_bytes = 0, ldr = 0;
+  BatchSrc *d_src = nullptr;
+  char *d_scratch = nullptr;  // scratch slots, then (double path) the SSSP rows
+  int64_t slot_bytes = 0;
 };
 
 struct fap_batch {
@@ -404,14 +432,20 @@ static int batch_launch(fap_batch *b, hipStream_t st) {
   for (int k = 0; k < 2; ++k) {
     fap_batch_kind &q = b->kind[k];
     if (q.nwork == 0) continue;
-    if (k == 0)
+    const dim3 sgrid((q.nsrc + kSWaves - 1) / kSWaves);
+    if (k == 0) {
+      if (q.nsrc > 0)
+        hipLaunchKernelGGL(batch_sssp_kernel<int>, sgrid, dim3(64 * kSWaves), 0, st, q.d_work,
+                           q.d_src, q.nsrc);
       hipLaunchKernelGGL(batch_kernel<int>, dim3(q.grid), dim3(kBThreads), lds_bytes(4), st,
-                         q.d_work, q.nwork, b->d_counter + k, q.d_scratch, q.slot_bytes,
-                         q.mat_bytes, q.ldr);
-    else
+                         q.d_work, q.nwork, b->d_counter + k, q.d_scratch, q.slot_bytes);
+    } else {
+      if (q.nsrc > 0)
+        hipLaunchKernelGGL(batch_sssp_kernel<double>, sgrid, dim3(64 * kSWaves), 0, st, q.d_work,
+                           q.d_src, q.nsrc);
       hipLaunchKernelGGL(batch_kernel<double>, dim3(q.grid), dim3(kBThreads), lds_bytes(8), st,
-                         q.d_work, q.nwork, b->d_counter + k, q.d_scratch, q.slot_bytes,
-                         q.mat_bytes, q.ldr);
+                         q.d_work, q.nwork, b->d_counter + k, q.d_scratch, q.slot_bytes);
+    }
     FAP_CHECK_LAUNCH();
   }
   return FAP_OK;
@@ -423,6 +457,7 @@ extern "C" int fap_hip_batch_destroy(fap_batch *b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   for (auto &q : b->kind) {
     if (q.d_work) dev_free(q.d_work);
+    if (q.d_src) dev_free(q.d_src);
     if (q.d_scratch) dev_free(q.d_scratch);
   }
   if (b->d_counter) dev_free(b->d_counter);
@@ -437,11 +472,13 @@ extern "C" int fap_hip_batch_create(fap_batch **out, const fap_batch_item *items
     return FAP_ERR_STATE;
   }
   FAP_REQUIRE(out && nitems >= 0 && (nitems == 0 || items), "fap_hip_batch_create: null argument");
-  FAP_REQUIRE((flags & ~FAP_BATCH_GRAPH) == 0, "fap_hip_batch_create: unknown flags %d", flags);
+  FAP_REQUIRE((flags & ~(FAP_BATCH_GRAPH | 0xFF00)) == 0, "fap_hip_batch_create: unknown flags %d",
+              flags);
+  const int per_cu = ((flags >> 8) & 0xFF) ? ((flags >> 8) & 0xFF) : 2;  // workgroups per CU
   *out = nullptr;
   std::vector<BatchWork> work[2];
   std::vector<double> cost[2];
-  int smax[2] = {0, 0}, bmax[2] = {0, 0}, nmax[2] = {0, 0};
+  int smax[2] = {0, 0};
   for (int g = 0; g < nitems; ++g) {
     const fap_batch_item &it = items[g];
     const fap_graph_dev &d = it.g;
@@ -459,13 +496,11 @@ extern "C" int fap_hip_batch_create(fap_batch **out, const fap_batch_item *items
                   q.num, kBatchMaxS);
       if (q.num == 0) continue;
       BatchWork w{d.d_row_offset, d.d_col, d.d_w, d.d_wf, d.d_graph_id, d.d_st2ed, d.d_ed2st,
-                  it.d_out, it.ld_out, d.n, q.part_id, q.offset, q.num, q.bdy, 0};
+                  it.d_out, it.ld_out, nullptr, 0, d.n, q.part_id, q.offset, q.num, q.bdy, 0};
       work[k].push_back(w);
       const double s = q.num, b = q.bdy, inner = q.num - q.bdy;
       cost[k].push_back(s * s * s + inner * b * d.n + b * (d.n + (double)d.m) * 4.0);
       smax[k] = std::max(smax[k], (int)align_up(q.num, kBT));
-      bmax[k] = std::max(bmax[k], q.bdy);
-      nmax[k] = std::max(nmax[k], d.n);
     }
   }
   fap_batch *b = new fap_batch;
@@ -503,16 +538,39 @@ extern "C" int fap_hip_batch_create(fap_batch **out, const fap_batch_item *items
     fap_batch_kind &q = b->kind[k];
     const int elem = k ? 8 : 4;
     q.nwork = nw;
-    q.grid = std::min(nw, 2 * cus);
-    q.mat_bytes = align_up((int64_t)smax[k] * smax[k] * elem, 256);
-    q.ldr = k ? align_up(nmax[k], 2) : 0;
-    q.slot_bytes = q.mat_bytes + (k ? align_up((int64_t)bmax[k] * q.ldr * 8, 256) : 0);
+    q.grid = std::min(nw, per_cu * cus);
+    q.slot_bytes = align_up((int64_t)smax[k] * smax[k] * elem, 256);
     // Scratch slots: at most 16 GiB in all (fewer resident workgroups for big parts).
     q.grid = (int)std::max<int64_t>(1, std::min<int64_t>(q.grid, (16LL << 30) / q.slot_bytes));
-    s = dev_alloc((void **)&q.d_work, sizeof(BatchWork) * nw);
-    if (s == FAP_OK) s = dev_alloc((void **)&q.d_scratch, std::max<int64_t>(q.slot_bytes, 256) * q.grid);
-    if (s == FAP_OK && hipMemcpy(q.d_work, sorted.data(), sizeof(BatchWork) * nw,
-                                 hipMemcpyHostToDevice) != hipSuccess) {
+    // SSSP rows: the int32 path writes a part's boundary rows of the result in place;
+    // the double path gets b_p x roundup(n, 2) doubles per part after the slots.
+    const int64_t slots_bytes = q.slot_bytes * q.grid;
+    int64_t rows_bytes = 0;
+    std::vector<int64_t> rows_at(nw, 0);
+    std::vector<BatchSrc> srcs;
+    for (int i = 0; i < nw; ++i) {
+      BatchWork &w = sorted[i];
+      if (k) {
+        w.ldr = align_up(w.n, 2);
+        rows_at[i] = slots_bytes + rows_bytes;
+        rows_bytes += align_up((int64_t)w.bdy * w.ldr * 8, 256);
+      } else {
+        w.ldr = w.ld_out;
+        w.rows = static_cast<int32_t *>(w.out) + (int64_t)w.offset * w.ld_out;
+      }
+      for (int j = 0; j < w.bdy; ++j) srcs.push_back(BatchSrc{i, j});
+    }
+    q.nsrc = (int)srcs.size();
+    s = dev_alloc((void **)&q.d_scratch, std::max<int64_t>(slots_bytes + rows_bytes, 256));
+    if (s == FAP_OK && k)
+      for (int i = 0; i < nw; ++i) sorted[i].rows = q.d_scratch + rows_at[i];
+    if (s == FAP_OK) s = dev_alloc((void **)&q.d_work, sizeof(BatchWork) * nw);
+    if (s == FAP_OK) s = dev_alloc((void **)&q.d_src, sizeof(BatchSrc) * std::max(q.nsrc, 1));
+    if (s == FAP_OK &&
+        (hipMemcpy(q.d_work, sorted.data(), sizeof(BatchWork) * nw, hipMemcpyHostToDevice) !=
+             hipSuccess ||
+         (q.nsrc > 0 && hipMemcpy(q.d_src, srcs.data(), sizeof(BatchSrc) * q.nsrc,
+                                  hipMemcpyHostToDevice) != hipSuccess))) {
       set_error("fap_hip_batch_create: work table upload failed");
       s = FAP_ERR_HIP;
     }

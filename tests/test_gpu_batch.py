@@ -116,3 +116,16 @@ def test_batch_without_graph_capture():
         for g in graphs:
             check_graph(g)
     b.close()
+
+
+def test_autotune_small_table():
+    from fastapsp_amd import autotune as AT
+    say("autotune (small candidate set)")
+    table = AT.tune(k_candidates=(2, 4), per_cu_candidates=(1, 2), max_copies=4)
+    assert len(table["K"]) == len(AT.SIZE_CLASSES)
+    assert all(k in (2, 4) for k in table["K"]) and all(p in (1, 2) for p in table["per_cu"])
+    for trials, k, p, ms in zip(table["trials"], table["K"], table["per_cu"],
+                                table["ms_per_graph"]):
+        assert ms == min(t["ms_per_graph"] for t in trials)
+        assert any(t["K"] == k and t["per_cu"] == p and t["ms_per_graph"] == ms for t in trials)
+    assert AT.k_for(300, table) == table["K"][0] and AT.k_for(8000, table) == table["K"][-1]
