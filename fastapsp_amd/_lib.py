@@ -56,6 +56,12 @@ class FapBatchItem(C.Structure):
                 ("is_float", _i32), ("d_out", _vp), ("ld_out", _i64)]
 
 
+class FapServerStats(C.Structure):
+    _fields_ = [("submitted", _i64), ("completed", _i64), ("failed", _i64), ("batches", _i64),
+                ("reserved_bytes", _i64), ("peak_reserved_bytes", _i64), ("entries", _f64),
+                ("span_s", _f64), ("p50_ms", _f64), ("p99_ms", _f64), ("mean_ms", _f64)]
+
+
 class FapStageTimes(C.Structure):
     _fields_ = [("sssp_ms", C.c_float), ("build_ms", C.c_float), ("floyd_ms", C.c_float),
                 ("minplus_ms", C.c_float), ("decode_ms", C.c_float), ("total_ms", C.c_float),
@@ -97,6 +103,17 @@ SIGNATURES = {
     "fap_hip_batch_run": (C.c_int, [_vp, _vp]),
     "fap_hip_batch_destroy": (C.c_int, [_vp]),
     "fap_hip_solve_batch": (C.c_int, [C.POINTER(FapBatchItem), _i32, _i32, _vp]),
+    "fap_hip_event_query": (C.c_int, [_vp, _pi32]),
+    "fap_server_create": (C.c_int, [C.POINTER(_vp), _i64, _i32]),
+    "fap_server_destroy": (C.c_int, [_vp]),
+    "fap_server_add_graph": (C.c_int, [_vp, _vp, _pi32]),
+    "fap_server_submit": (C.c_int, [_vp, _i32, _i32, _pi64]),
+    "fap_server_wait": (C.c_int, [_vp, _i64, _f64, _pi32]),
+    "fap_server_result": (C.c_int, [_vp, _i64, C.POINTER(_vp), _pi64, _pi64, _pi64, _pi32]),
+    "fap_server_release": (C.c_int, [_vp, _i64]),
+    "fap_server_get_stats": (C.c_int, [_vp, C.POINTER(FapServerStats)]),
+    "fap_hip_stream_create": (C.c_int, [C.POINTER(_vp)]),
+    "fap_hip_stream_destroy": (C.c_int, [_vp]),
     "fap_graph_batch_create": (C.c_int, [C.POINTER(_vp), C.POINTER(_vp), _i32, _i32]),
     "fap_graph_solve_batch": (C.c_int, [C.POINTER(_vp), _i32, _i32]),
     "fap_graph_create_from_file": (C.c_int, [C.POINTER(_vp), _cp, C.c_int, C.c_int, _i32, _cp]),

@@ -54,6 +54,7 @@ struct BatchWork {
   int64_t ld_out;
   void *rows;                    // the part's SSSP rows (int32: its boundary rows of out)
   int64_t ldr;
+  int64_t out_row;               // the part's first row in out
   int n, part_id, offset, num, bdy, pad;
 };
 
@@ -339,7 +340,7 @@ __device__ void process_part(const BatchWork &wk, T *M, char *lds) {
           const int i = row0 + orow(tr, q);
           if (i >= mrows) continue;
           const T v = own >= 0 ? M[(int64_t)(bdy + i) * S + own] : acc[q][c];
-          out[(int64_t)(off + bdy + i) * wk.ld_out + j] = Dist<T>::out(v);
+          out[(wk.out_row + bdy + i) * wk.ld_out + j] = Dist<T>::out(v);
         }
       }
     }
@@ -351,7 +352,7 @@ __device__ void process_part(const BatchWork &wk, T *M, char *lds) {
     for (int j = tid; j < n; j += kBThreads) {
       const T v = wk.graph_id[j] == wk.part_id ? M[(int64_t)i * S + wk.ed2st[j] - off]
                                                : rows[(int64_t)i * ldr + j];
-      out[(int64_t)(off + i) * wk.ld_out + j] = Dist<T>::out(v);
+      out[(wk.out_row + i) * wk.ld_out + j] = Dist<T>::out(v);
     }
   __syncthreads();
 }
@@ -490,13 +491,14 @@ extern "C" int fap_hip_batch_create(fap_batch **out, const fap_batch_item *items
     for (int p = 0; p < it.nparts; ++p) {
       const fap_part_desc &q = it.parts[p];
       FAP_REQUIRE(q.part_id >= 1 && q.num >= 0 && q.bdy >= 0 && q.bdy <= q.num && q.offset >= 0 &&
-                      (int64_t)q.offset + q.num <= d.n && q.out_row == q.offset,
-                  "fap_hip_batch_create: item %d part %d bad (out_row must equal offset)", g, p);
+                      (int64_t)q.offset + q.num <= d.n && q.out_row >= 0,
+                  "fap_hip_batch_create: item %d part %d bad", g, p);
       FAP_REQUIRE(q.num <= kBatchMaxS, "fap_hip_batch_create: part with %d vertices (limit %d)",
                   q.num, kBatchMaxS);
       if (q.num == 0) continue;
       BatchWork w{d.d_row_offset, d.d_col, d.d_w, d.d_wf, d.d_graph_id, d.d_st2ed, d.d_ed2st,
-                  it.d_out, it.ld_out, nullptr, 0, d.n, q.part_id, q.offset, q.num, q.bdy, 0};
+                  it.d_out, it.ld_out, nullptr, 0, q.out_row, d.n, q.part_id, q.offset, q.num,
+                  q.bdy, 0};
       work[k].push_back(w);
       const double s = q.num, b = q.bdy, inner = q.num - q.bdy;
       cost[k].push_back(s * s * s + inner * b * d.n + b * (d.n + (double)d.m) * 4.0);
@@ -556,7 +558,7 @@ extern "C" int fap_hip_batch_create(fap_batch **out, const fap_batch_item *items
         rows_bytes += align_up((int64_t)w.bdy * w.ldr * 8, 256);
       } else {
         w.ldr = w.ld_out;
-        w.rows = static_cast<int32_t *>(w.out) + (int64_t)w.offset * w.ld_out;
+        w.rows = static_cast<int32_t *>(w.out) + w.out_row * w.ld_out;
       }
       for (int j = 0; j < w.bdy; ++j) srcs.push_back(BatchSrc{i, j});
     }

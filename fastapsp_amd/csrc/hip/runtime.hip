@@ -303,6 +303,32 @@ int fap_hip_event_record(void *ev, void *stream) {
   FAP_CHECK_HIP(hipEventRecord(static_cast<hipEvent_t>(ev), resolve(stream)));
   return FAP_OK;
 }
+int fap_hip_event_query(void *ev, int32_t *done) {
+  FAP_REQUIRE(ev && done, "fap_hip_event_query: null");
+  const hipError_t e = hipEventQuery(static_cast<hipEvent_t>(ev));
+  if (e == hipErrorNotReady) {
+    (void)hipGetLastError();
+    *done = 0;
+    return FAP_OK;
+  }
+  FAP_CHECK_HIP(e);
+  *done = 1;
+  return FAP_OK;
+}
+
+int fap_hip_stream_create(void **stream) {
+  FAP_REQUIRE(stream, "fap_hip_stream_create: null");
+  hipStream_t s;
+  FAP_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));  // default priority
+  *stream = s;
+  return FAP_OK;
+}
+
+int fap_hip_stream_destroy(void *stream) {
+  if (stream) FAP_CHECK_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+  return FAP_OK;
+}
+
 int fap_hip_event_elapsed_ms(void *a, void *b, float *ms) {
   FAP_REQUIRE(a && b && ms, "fap_hip_event_elapsed_ms: null");
   FAP_CHECK_HIP(hipEventSynchronize(static_cast<hipEvent_t>(b)));

@@ -87,6 +87,11 @@ int fap_hip_event_create(void **ev);
 int fap_hip_event_destroy(void *ev);
 int fap_hip_event_record(void *ev, void *stream);
 int fap_hip_event_elapsed_ms(void *ev_start, void *ev_end, float *ms);
+/* *done = 1 when all work recorded before the event has completed (no wait). */
+int fap_hip_event_query(void *ev, int32_t *done);
+/* Additional non-blocking streams (default priority) for work that may overlap. */
+int fap_hip_stream_create(void **stream);
+int fap_hip_stream_destroy(void *stream);
 
 /* ------------------------------------------------------- kernel-level seam */
 
@@ -204,9 +209,9 @@ int fap_hip_solve_parts_f32(const fap_graph_dev *g, const fap_part_desc *parts, 
                             float *d_out, int64_t ld_out, int32_t mode, void *stream);
 
 /* ------------------------------------------------------------ many graphs */
-/* One graph of a batch: its device view, its parts (out_row must equal offset: the
- * full n x n result of solve()), the result buffer (int32, or float32 when is_float:
- * weights in g.d_wf) with row stride ld_out. */
+/* One graph of a batch: its device view, its parts (part p writes rows out_row ..
+ * out_row + s_p - 1; out_row = offset gives the n x n layout of solve()), the result
+ * buffer (int32, or float32 when is_float: weights in g.d_wf) with row stride ld_out. */
 typedef struct fap_batch_item {
   fap_graph_dev g;
   const fap_part_desc *parts;
@@ -226,6 +231,8 @@ typedef struct fap_batch_item {
  * fap/fap.cpp:507-518). */
 typedef struct fap_batch fap_batch;
 #define FAP_BATCH_GRAPH 0x1
+/* flags bits 8..15: persistent workgroups per CU (0 = 2). */
+#define FAP_BATCH_PER_CU(x) (((x)&0xFF) << 8)
 int fap_hip_batch_create(fap_batch **b, const fap_batch_item *items, int32_t nitems,
                          int32_t flags);
 int fap_hip_batch_run(fap_batch *b, void *stream);
@@ -311,6 +318,36 @@ int fap_graph_edge_cut(fap_graph *g, double *edge_cut);
  * getters return the batch results. */
 int fap_graph_batch_create(fap_batch **b, fap_graph **graphs, int32_t count, int32_t flags);
 int fap_graph_solve_batch(fap_graph **graphs, int32_t count, int32_t mode);
+
+/* ---------------------------------------------------------------- serving */
+/* A request server over resident graphs (no counterpart in the reference).  Graphs
+ * are registered after preCondition and stay resident; requests ask for a full APSP
+ * (sub_id 0: the n x n result in the layout of solve()) or one sub-graph block
+ * (sub_id = part id: the s_p x n rows of solveSubGraph).  A scheduler thread admits
+ * waiting requests in arrival order while their result buffers and batch scratch fit
+ * in budget_bytes next to everything admitted and not yet released, puts every
+ * admitted request into one new batch (fap_hip_batch_*) on a free stream of its own,
+ * and marks requests done when their batch's event has passed.  Results stay in HBM
+ * until fap_server_release.  Thread-safe; a request that can never fit fails. */
+typedef struct fap_server fap_server;
+typedef struct fap_server_stats {
+  int64_t submitted, completed, failed, batches;
+  int64_t reserved_bytes, peak_reserved_bytes;
+  double entries;  /* result entries of completed requests */
+  double span_s;   /* first submit -> last completion */
+  double p50_ms, p99_ms, mean_ms;  /* submit -> completion latency */
+} fap_server_stats;
+/* Request states reported by fap_server_wait. */
+enum { FAP_REQ_PENDING = 0, FAP_REQ_RUNNING = 1, FAP_REQ_DONE = 2, FAP_REQ_FAILED = 3 };
+int fap_server_create(fap_server **s, int64_t budget_bytes, int32_t nstreams);
+int fap_server_destroy(fap_server *s); /* drains pending and running requests */
+int fap_server_add_graph(fap_server *s, fap_graph *g, int32_t *index);
+int fap_server_submit(fap_server *s, int32_t graph, int32_t sub_id, int64_t *ticket);
+int fap_server_wait(fap_server *s, int64_t ticket, double timeout_s, int32_t *state);
+int fap_server_result(fap_server *s, int64_t ticket, void **d_out, int64_t *rows, int64_t *cols,
+                      int64_t *ld, int32_t *is_float);
+int fap_server_release(fap_server *s, int64_t ticket);
+int fap_server_get_stats(fap_server *s, fap_server_stats *st);
 
 #ifdef __cplusplus
 }

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_functions():
     text = open(os.path.join(ROOT, "include", "fap_hip.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(fap_(?:hip|graph|mtx|layout)_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(fap_(?:hip|graph|mtx|layout|server)_\w+)\s*\(", text)))
 
 
 def test_header_declares_the_abi():
@@ -37,7 +37,8 @@ def test_struct_layouts_match_header(tmp_path):
     """sizeof and every field offset of the ctypes structs equal what a C compiler
     reads from include/fap_hip.h."""
     structs = {"fap_graph_dev": L.FapGraphDev, "fap_part_desc": L.FapPartDesc,
-               "fap_stage_times": L.FapStageTimes, "fap_batch_item": L.FapBatchItem}
+               "fap_stage_times": L.FapStageTimes, "fap_batch_item": L.FapBatchItem,
+               "fap_server_stats": L.FapServerStats}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fap_hip.h"',
              "int main(void) {"]
     for cname, cls in structs.items():
