@@ -231,14 +231,87 @@ def batch_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def serve_bench(args):
+    """Serving: a pool of resident graphs (road-like and geometric kNN, sizes log-uniform
+    in [1k, 64k] vertices; integer weights in [1, 1000] or float32 weights in [0.5, 10)),
+    Poisson arrivals of full-APSP and sub-graph requests at 10, 50 and 200 req/s against
+    a 200 GB HBM budget; each result is consumed (released) as soon as it is done.
+    Reports p50 / p99 latency (submit -> completion) and result entries/s per rate."""
+    import math
+    from fastapsp_amd import serve as S
+    from fastapsp_amd.autotune import road_graph, symmetric_float_weights
+    rng = np.random.default_rng(77)
+    sizes = np.exp(rng.uniform(np.log(1000), np.log(65536), args.pool)).astype(int)
+    t0 = time.perf_counter()
+    pool = []
+    for i, nt in enumerate(sizes):
+        # Parts of ~400 vertices for the batch lane, ~900 (as the 114,599-vertex
+        # workload) for graphs that take the pipeline lane (>= 8192 vertices).
+        K = int(max(2, math.ceil(nt / (400 if nt < 8192 else 900))))
+        kind = i % 4  # road int, road float, knn int, knn float
+        if kind < 2:
+            g = road_graph(int(nt), 900 + i, K)
+            c = g.csr()
+            w = symmetric_float_weights(c["row_offset"], c["col"], 900 + i)
+            if kind == 0:  # integers 1..1000, equal both ways
+                w = (np.floor((w - 0.5) / 9.5 * 1000) + 1).astype(np.float32)
+            g = F.FapGraph.from_csr(c["row_offset"], c["col"], w, False, K)
+        else:
+            g = F.FapGraph.knn(int(nt), 3, 1000, 900 + i, K)
+            if kind == 3:
+                c = g.csr()
+                w = symmetric_float_weights(c["row_offset"], c["col"], 900 + i, directed=True)
+                g = F.FapGraph.from_csr(c["row_offset"], c["col"], w, True, K)
+        g.preCondition()
+        pool.append(g)
+    log(f"pool of {len(pool)} graphs built in {time.perf_counter() - t0:.1f}s, n in "
+        f"[{min(g.num_vertexs for g in pool)}, {max(g.num_vertexs for g in pool)}]")
+    srv = S.Server(budget_bytes=200e9, streams=4)
+    idx = [srv.add_graph(g) for g in pool]
+    # warm-up: one request per graph
+    for i in idx:
+        t = srv.submit(i, 0)
+        srv.wait(t)
+        srv.release(t)
+    results = []
+    for rate, dur in ((10.0, 5.0), (50.0, 4.0), (200.0, 3.0)):
+        srv.close()
+        srv = S.Server(budget_bytes=200e9, streams=4)
+        idx = [srv.add_graph(g) for g in pool]
+        nreq = int(rate * dur)
+        reqs = []
+        for _ in range(nreq):
+            gi = int(rng.integers(len(pool)))
+            sub = 0 if rng.random() < 0.5 else int(rng.integers(1, pool[gi].K + 1))
+            reqs.append((idx[gi], sub))
+        tickets, states, wall = S.poisson_load(srv, reqs, rate, seed=int(rate))
+        st = srv.stats()
+        res = {"offered_req_s": rate, "requests": nreq, "completed": st["completed"],
+               "failed": st["failed"], "batches": st["batches"], "wall_s": wall,
+               "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+               "entries_per_s": st["entries"] / max(st["span_s"], 1e-9),
+               "peak_reserved_GB": st["peak_reserved_bytes"] / 1e9}
+        results.append(res)
+        log(f"rate {rate:g} req/s: {st['completed']}/{nreq} done in {wall:.2f}s, "
+            f"{st['batches']} batches, p50 {st['p50_ms']:.1f} ms, p99 {st['p99_ms']:.1f} ms, "
+            f"{res['entries_per_s']:.3e} entries/s, peak {res['peak_reserved_GB']:.1f} GB")
+    srv.close()
+    print(json.dumps({"serve": results, "pool": len(pool),
+                      "workload": "road-like and kNN graphs, n log-uniform in [1k, 64k], int "
+                                  "[1,1000] or float32 [0.5,10) weights, 50% full APSP / 50% "
+                                  "sub-graph requests, Poisson arrivals, 200 GB budget"}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="road1",
-                    help="road1 (default), road0, or batch512 (many-graph batch)")
+                    help="road1 (default), road0, batch512 (many-graph batch) or serve")
     ap.add_argument("--graphs", type=int, default=512, help="graphs of the batch512 config")
+    ap.add_argument("--pool", type=int, default=24, help="resident graphs of the serve config")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -256,6 +329,9 @@ def main():
         return
     if args.minplus > 0:
         minplus_bench(args.minplus)
+        return
+    if args.config == "serve":
+        serve_bench(args)
         return
     if args.config == "batch512":
         if args.gpus != 1:

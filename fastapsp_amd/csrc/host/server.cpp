@@ -15,6 +15,12 @@
 //   * completion: an event per batch; when it has passed, its requests are done (their
 //     latency is submit -> observed completion), the batch scratch is freed and the
 //     result stays resident until the client releases it.
+// Requests on graphs of kLargeN vertices or more take a second lane: a worker thread
+// runs them one after another through the stage-batched pipeline
+// (fap_hip_solve_parts / _f32, whose kernels fill the GPU from a single graph) on a
+// stream of its own, beside the batches.  The pipeline's workspaces belong to the
+// library, so while a server runs no other thread may call solve() or
+// fap_hip_solve_parts.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -62,6 +68,8 @@ struct Inflight {
 
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
+constexpr int kLargeN = 8192;  // graphs from this size take the pipeline lane
+
 }  // namespace
 
 struct fap_server {
@@ -71,7 +79,10 @@ struct fap_server {
   std::vector<GraphEntry> graphs;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
-  std::deque<int64_t> pending;
+  std::deque<int64_t> pending;        // batch lane
+  std::deque<int64_t> pending_large;  // pipeline lane
+  void *pipe_stream = nullptr;
+  std::thread pipe_worker;
   std::map<int64_t, Request> reqs;
   std::vector<Inflight> inflight;
   std::vector<double> latency_ms;
@@ -150,7 +161,16 @@ void scheduler(fap_server *s) {
           s->cv_done.notify_all();
           continue;
         }
-        if (s->reserved + need + want > s->budget) break;  // wait for memory (arrival order kept)
+        if (s->reserved + need + want > s->budget) {
+          if (s->stop && take.empty() && s->inflight.empty()) {  // shutting down: nothing frees
+            r.state = kFailed;
+            ++s->failed;
+            s->pending.pop_front();
+            s->cv_done.notify_all();
+            continue;
+          }
+          break;  // wait for memory (arrival order kept)
+        }
         need += want;
         take.push_back(s->pending.front());
         s->pending.pop_front();
@@ -213,6 +233,79 @@ void scheduler(fap_server *s) {
   }
 }
 
+// The pipeline lane: large requests in arrival order, one at a time.
+void pipeline_lane(fap_server *s) {
+  std::unique_lock<std::mutex> lk(s->mu);
+  while (true) {
+    if (s->pending_large.empty()) {
+      if (s->stop) break;
+      s->cv_work.wait_for(lk, std::chrono::milliseconds(1));
+      continue;
+    }
+    Request &r = s->reqs[s->pending_large.front()];
+    if (r.bytes > s->budget) {
+      r.state = kFailed;
+      ++s->failed;
+      s->pending_large.pop_front();
+      s->cv_done.notify_all();
+      continue;
+    }
+    if (s->reserved + r.bytes > s->budget) {
+      if (s->stop) {  // shutting down: nothing will be released
+        r.state = kFailed;
+        ++s->failed;
+        s->pending_large.pop_front();
+        s->cv_done.notify_all();
+        continue;
+      }
+      s->cv_work.wait_for(lk, std::chrono::milliseconds(1));  // wait for releases
+      continue;
+    }
+    const int64_t tk = s->pending_large.front();
+    s->pending_large.pop_front();
+    s->reserved += r.bytes;
+    s->peak_reserved = std::max(s->peak_reserved, s->reserved);
+    r.state = kRunning;
+    const GraphEntry &ge = s->graphs[r.graph];
+    const fap_graph_dev dev = ge.dev;
+    const bool is_float = ge.is_float;
+    std::vector<fap_part_desc> parts = r.parts;
+    const int64_t bytes = r.bytes, ld = r.ld;
+    lk.unlock();
+    void *d_out = nullptr;
+    bool ok = fap_hip_malloc(&d_out, std::max<int64_t>(bytes, 4)) == FAP_OK;
+    // int32 FW keeps the call asynchronous (the 16-bit form reads flags back).
+    const int32_t mode = FAP_MODE_BATCHED | FAP_MODE_FLAG_INT32_FW;
+    if (ok)
+      ok = (is_float ? fap_hip_solve_parts_f32(&dev, parts.data(), (int32_t)parts.size(),
+                                               static_cast<float *>(d_out), ld, mode,
+                                               s->pipe_stream)
+                     : fap_hip_solve_parts(&dev, parts.data(), (int32_t)parts.size(),
+                                           static_cast<int32_t *>(d_out), ld, mode,
+                                           s->pipe_stream)) == FAP_OK;
+    if (ok) ok = fap_hip_stream_sync(s->pipe_stream) == FAP_OK;
+    const double t = now_s();
+    lk.lock();
+    Request &q = s->reqs[tk];
+    q.d_out = d_out;
+    if (ok) {
+      q.state = kDone;
+      q.t_done = t;
+      s->latency_ms.push_back((t - q.t_submit) * 1e3);
+      s->entries += (double)q.rows * q.cols;
+      ++s->completed;
+      s->t_last = t;
+    } else {
+      q.state = kFailed;
+      ++s->failed;
+      if (d_out) fap_hip_free(d_out);
+      q.d_out = nullptr;
+      s->reserved -= q.bytes;
+    }
+    s->cv_done.notify_all();
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -237,8 +330,15 @@ int fap_server_create(fap_server **out, int64_t budget_bytes, int32_t nstreams) 
     delete s;
     return st;
   }
+  if (st == FAP_OK) st = fap_hip_stream_create(&s->pipe_stream);
+  if (st != FAP_OK) {
+    for (void *p : s->streams) fap_hip_stream_destroy(p);
+    delete s;
+    return st;
+  }
   s->busy.assign(s->streams.size(), false);
   s->worker = std::thread(scheduler, s);
+  s->pipe_worker = std::thread(pipeline_lane, s);
   *out = s;
   return FAP_OK;
 }
@@ -250,10 +350,12 @@ int fap_server_destroy(fap_server *s) {
     s->stop = true;
   }
   s->cv_work.notify_all();
-  s->worker.join();  // drains pending and in-flight batches
+  s->worker.join();       // drains pending and in-flight batches
+  s->pipe_worker.join();  // and the pipeline lane
   for (auto &kv : s->reqs)
     if (kv.second.d_out) fap_hip_free(kv.second.d_out);
   for (void *p : s->streams) fap_hip_stream_destroy(p);
+  fap_hip_stream_destroy(s->pipe_stream);
   delete s;
   return FAP_OK;
 }
@@ -312,8 +414,9 @@ int fap_server_submit(fap_server *s, int32_t graph, int32_t sub_id, int64_t *tic
   const int64_t tk = s->next_ticket++;
   if (s->submitted == 0) s->t_first = r.t_submit;
   ++s->submitted;
+  const bool large = ge.dev.n >= kLargeN;
   s->reqs[tk] = std::move(r);
-  s->pending.push_back(tk);
+  (large ? s->pending_large : s->pending).push_back(tk);
   *ticket = tk;
   lk.unlock();
   s->cv_work.notify_all();

@@ -85,23 +85,40 @@ class Server:
 def poisson_load(server: Server, requests, rate: float, seed: int = 0,
                  keep: bool = False, timeout: float = 120.0):
     """Submit `requests` (a list of (graph index, sub_id)) at Poisson arrival times of
-    the given rate (requests per second), wait for all, release them (unless keep).
-    Returns (tickets, states, wall seconds)."""
+    the given rate (requests per second) and wait for all.  Unless keep, each result is
+    released as soon as it is seen finished (a client that consumes results in place),
+    so held results do not fill the budget.  Returns (tickets, states, wall seconds)."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, len(requests))
-    tickets = []
+    tickets, states = [], {}
+    outstanding = []
+
+    def reap(block: bool):
+        still = []
+        for t in outstanding:
+            st = server.wait(t, timeout if block else 0.0)
+            if st in (DONE, FAILED):
+                states[t] = st
+                if not keep:
+                    server.release(t)
+            else:
+                still.append(t)
+        outstanding[:] = still
+
     t0 = time.perf_counter()
     due = t0
     for (g, sub), gap in zip(requests, gaps):
         due += gap
-        delay = due - time.perf_counter()
-        if delay > 0:
-            time.sleep(delay)
-        tickets.append(server.submit(g, sub))
-    states = [server.wait(t, timeout) for t in tickets]
+        while True:
+            delay = due - time.perf_counter()
+            if delay <= 0:
+                break
+            reap(False)
+            time.sleep(min(delay, 0.002))
+        t = server.submit(g, sub)
+        tickets.append(t)
+        outstanding.append(t)
+    while outstanding:
+        reap(True)
     wall = time.perf_counter() - t0
-    if not keep:
-        for t, st in zip(tickets, states):
-            if st in (DONE, FAILED):
-                server.release(t)
-    return tickets, states, wall
+    return tickets, [states.get(t, PENDING) for t in tickets], wall

@@ -33,8 +33,10 @@ def graphs():
     out.append(F.FapGraph.from_csr(c["row_offset"], c["col"],
                                    symmetric_float_weights(c["row_offset"], c["col"], 5, True),
                                    True, 3))
+    out.append(F.FapGraph.road_grid(96, 96, 0.1, 0.02, 6, 20))  # >= 8192: pipeline lane
     for g in out:
         g.preCondition()
+    assert out[-1].num_vertexs >= 8192
     return out
 
 
