@@ -35,6 +35,14 @@ struct Workspace {
   int ensure(int64_t need);
 };
 
+// Ordering of the pipelines' shared workspaces across caller streams.  Every solve
+// call makes its stream wait for the fence (the end of the previous call that used
+// the workspaces, on whatever stream it ran) before touching them, and records the
+// fence at its end.  The fence is an event owned by the library, so it stays valid
+// when the caller destroys the stream it was recorded on.
+int ws_fence_wait(hipStream_t st);
+int ws_fence_record(hipStream_t st);
+
 constexpr int kInf = FAP_INF;
 
 // Stage timing of the solve pipelines, read by fap_hip_last_stage_times.  Stages:

@@ -470,9 +470,8 @@ int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 using namespace fap_hip;
 
-extern "C" int fap_hip_solve_parts_f32(const fap_graph_dev *g, const fap_part_desc *parts,
-                                       int32_t nparts, float *d_out, int64_t ld_out, int32_t mode,
-                                       void *stream) {
+static int solve_parts_f32(const fap_graph_dev *g, const fap_part_desc *parts, int32_t nparts,
+                           float *d_out, int64_t ld_out, int32_t mode, void *stream) {
   if (!initialized()) {
     set_error("fap_hip_init has not been called");
     return FAP_ERR_STATE;
@@ -656,4 +655,18 @@ extern "C" int fap_hip_solve_parts_f32(const fap_graph_dev *g, const fap_part_de
       FAP_TRY(run(p, p + 1, act[p].bdy, S[p], nb[p], tiles[p], (int64_t)act[p].bdy * n));
   }
   return timing_finish(st);
+}
+
+extern "C" int fap_hip_solve_parts_f32(const fap_graph_dev *g, const fap_part_desc *parts,
+                                       int32_t nparts, float *d_out, int64_t ld_out, int32_t mode,
+                                       void *stream) {
+  if (!initialized()) {
+    set_error("fap_hip_init has not been called");
+    return FAP_ERR_STATE;
+  }
+  hipStream_t st = resolve(stream);
+  FAP_TRY(ws_fence_wait(st));
+  const int s = solve_parts_f32(g, parts, nparts, d_out, ld_out, mode, stream);
+  const int f = ws_fence_record(st);  // also after a failed call: its launches used the workspaces
+  return s != FAP_OK ? s : f;
 }

@@ -85,7 +85,6 @@ Workspace g_ws_fw;    // Floyd-Warshall matrices
 Workspace g_ws_fw16;  // 16-bit Floyd-Warshall matrices
 Workspace g_ws_meta;  // descriptors, sources, row indices
 Workspace g_ws_ell;   // ELL adjacency of the unit-weight graph
-hipStream_t g_last_stream = nullptr;
 
 struct EventPool {
   std::vector<hipEvent_t> ev;
@@ -163,8 +162,6 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
                   g->d_st2ed && g->d_ed2st && (g->m == 0 || g->d_col),
               "fap_hip_solve_parts: bad graph view");
   hipStream_t st = resolve(stream);
-  if (g_last_stream && g_last_stream != st) FAP_CHECK_HIP(hipStreamSynchronize(g_last_stream));
-  g_last_stream = st;
 
   // ---- host plan ---------------------------------------------------------
   std::vector<fap_part_desc> act;
@@ -471,9 +468,8 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   return FAP_OK;
 }
 
-extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *parts,
-                                   int32_t nparts, int32_t *d_out, int64_t ld_out, int32_t mode,
-                                   void *stream) {
+static int solve_parts(const fap_graph_dev *g, const fap_part_desc *parts, int32_t nparts,
+                       int32_t *d_out, int64_t ld_out, int32_t mode, void *stream) {
   if (!initialized()) {
     set_error("fap_hip_init has not been called");
     return FAP_ERR_STATE;
@@ -505,6 +501,20 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
                      mode | FAP_MODE_FLAG_INT32_FW, stream, true));
   g_parts16 = parts16;
   return FAP_OK;
+}
+
+extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *parts,
+                                   int32_t nparts, int32_t *d_out, int64_t ld_out, int32_t mode,
+                                   void *stream) {
+  if (!initialized()) {
+    set_error("fap_hip_init has not been called");
+    return FAP_ERR_STATE;
+  }
+  hipStream_t st = resolve(stream);
+  FAP_TRY(ws_fence_wait(st));
+  const int s = solve_parts(g, parts, nparts, d_out, ld_out, mode, stream);
+  const int f = ws_fence_record(st);  // also after a failed call: its launches used the workspaces
+  return s != FAP_OK ? s : f;
 }
 
 extern "C" int fap_hip_last_stage_times(fap_stage_times *t) {

@@ -16,6 +16,8 @@ bool g_init = false;
 int g_device = -1;
 hipStream_t g_stream = nullptr;
 hipStream_t g_aux[2] = {nullptr, nullptr};
+hipEvent_t g_ws_fence = nullptr;  // end of the last call that used the pipeline workspaces
+bool g_ws_fence_set = false;
 int64_t g_budget = 0;  // 0 = unlimited
 int64_t g_in_use = 0;
 int64_t g_high = 0;
@@ -35,6 +37,17 @@ void clear_error() { g_last_error.clear(); }
 hipStream_t lib_stream() { return g_stream; }
 hipStream_t aux_stream(int i) { return g_aux[i & 1]; }
 bool initialized() { return g_init; }
+
+int ws_fence_wait(hipStream_t st) {
+  if (g_ws_fence_set) FAP_CHECK_HIP(hipStreamWaitEvent(st, g_ws_fence, 0));
+  return FAP_OK;
+}
+
+int ws_fence_record(hipStream_t st) {
+  FAP_CHECK_HIP(hipEventRecord(g_ws_fence, st));
+  g_ws_fence_set = true;
+  return FAP_OK;
+}
 
 int dev_alloc(void **p, int64_t bytes) {
   if (!p || bytes < 0) {
@@ -159,6 +172,7 @@ int fap_hip_init(int device) {
   // Two more streams (default priority) for overlapping the stages of different parts.
   FAP_CHECK_HIP(hipStreamCreateWithFlags(&g_aux[0], hipStreamNonBlocking));
   FAP_CHECK_HIP(hipStreamCreateWithFlags(&g_aux[1], hipStreamNonBlocking));
+  FAP_CHECK_HIP(hipEventCreateWithFlags(&g_ws_fence, hipEventDisableTiming));
   g_device = device;
   g_init = true;
   return FAP_OK;

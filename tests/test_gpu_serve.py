@@ -117,3 +117,26 @@ def test_serve_budget_admission():
     assert tiny.wait(t) == S.FAILED
     tiny.release(t)
     tiny.close()
+
+
+def test_solve_after_server_with_pipeline_lane_closed():
+    """The pipeline lane solves on the server's own stream, which fap_server_destroy
+    destroys; a later solve() on the library stream must order itself after that work
+    without touching the destroyed stream."""
+    big = F.FapGraph.road_grid(96, 96, 0.1, 0.02, 7, 16)
+    big.preCondition()
+    assert big.num_vertexs >= 8192
+    srv = S.Server(budget_bytes=40e9, streams=2)
+    i = srv.add_graph(big)
+    t = srv.submit(i, 1)
+    assert srv.wait(t) == S.DONE
+    lane = srv.result(t)
+    srv.release(t)
+    srv.close()
+    assert np.array_equal(lane, expected(big, 1))
+    g = F.FapGraph.road_grid(30, 28, 0.1, 0.02, 9, 5)
+    g.preCondition()
+    g.solve()
+    assert np.array_equal(g.getAllDistanceI32(), expected(g, 0))
+    big.solveSubGraph(2)
+    assert np.array_equal(big.getSubGraphDistanceI32(2), expected(big, 2))
