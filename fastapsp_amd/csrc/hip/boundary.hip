@@ -1,0 +1,468 @@
+// Boundary rows of every part from the boundary graph, in place of one SSSP per
+// boundary vertex over the whole graph.
+//
+// The reference computes the b_p rows of part p (distances from p's boundary
+// vertices to every vertex) with one Dijkstra per boundary vertex
+// (handle_boundry_path, fap/kernel/batch_sssp.h:280-303 -> dijkstra_path :79-114).
+// When the pipeline solves every part of a graph, the same rows follow from the
+// parts' own Floyd-Warshall results:
+//   * any path leaves or enters a part through its boundary vertices (both ends of
+//     every cut arc are boundary vertices, fap/utils/util_preprocess.h:32-58), so a
+//     path between boundary vertices splits into cut arcs and stretches inside one
+//     part between two of its boundary vertices;
+//   * hence the distances between all boundary vertices are the all-pairs distances
+//     of the boundary graph G_B: vertices = every part's boundary, arcs = each part's
+//     in-part distances between its boundary vertices (the leading b_q x b_q block
+//     of its in-part Floyd-Warshall matrix) plus the cut arcs with their weights;
+//   * a path from boundary vertex g to vertex v of part q enters q for the last time
+//     at some boundary vertex c of q and stays inside q afterwards, so
+//        d(g, v) = min_{c in bdy(q)} D_B(g, c) + d_q(c, v)
+//     with d_q the in-part distance (c = v's own part entry included: D_B(g, g) = 0).
+// Every quantity is an exact integer min/plus expression, so the rows equal the
+// SSSP rows bit for bit.  The inner rows then follow as in the SSSP form: for inner
+// vertex i of part p, d(i, j) = min_b d_p(i, b) + d(b, j) over p's boundary, and for
+// j in p additionally d_p(i, j) (min-plus merge, minplus.hip).
+//
+// Kernels:
+//   bg_vert_kernel   one thread per G_B vertex: its clique in-weights (a column of
+//                    its part's boundary block) and its cut arcs;
+//   bg_closure_kernel one workgroup per G_B vertex s: row s of the closed G_B
+//                    matrix by Bellman-Ford sweeps in LDS (clique pulls, cut-arc
+//                    pushes), as many sweeps as shortest paths cross parts;
+//   bg_rows_kernel   d(g, j) for 1024 output columns per workgroup and a share of
+//                    the G_B rows, staged through LDS; each lane keeps its columns'
+//                    in-part distances from their part's boundary in registers and
+//                    the wave's stores are coalesced, so the rows go to HBM once
+//                    (bg_rows_wide_kernel: parts with more than 32 boundary
+//                    vertices, walked in register chunks).
+#include <algorithm>
+
+#include "fap_internal.h"
+#include "fw_desc.h"
+#include "gmem.h"
+#include "mp_tile.h"
+
+namespace fap_hip {
+
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int find_part_by_bs(const BgPart *__restrict__ parts, int np, int g) {
+  int lo = 0, hi = np - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (parts[mid].bs <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// G_B target of cut arc e out of a vertex of part `pid`, or -1 for an in-part arc.
+__device__ __forceinline__ int cut_target(int e, int pid, const BgPart *__restrict__ parts,
+                                          const int *__restrict__ pidx_by_pid,
+                                          const int *__restrict__ col,
+                                          const int *__restrict__ graph_id,
+                                          const int *__restrict__ ed2st) {
+  const int u = col[e];
+  const int pu = graph_id[u];
+  if (pu == pid) return -1;
+  const BgPart q = parts[pidx_by_pid[pu]];
+  const int lk = ed2st[u] - q.offset;
+  return (lk >= 0 && lk < q.bdy) ? q.bs + lk : -1;
+}
+
+// One thread per G_B vertex g: its BgVert record, its clique in-weights
+// (wt[woff + c] = d_q(c, k), column k of the part's boundary block) and the output
+// row that holds its distance row.
+__global__ void bg_vert_kernel(const BgPart *__restrict__ parts, int np, int nb_total,
+                               const int *__restrict__ pidx_by_pid, const int *__restrict__ row_off,
+                               const int *__restrict__ col, const int *__restrict__ w,
+                               const int *__restrict__ graph_id, const int *__restrict__ st2ed,
+                               const int *__restrict__ ed2st, BgVert *__restrict__ verts,
+                               int *__restrict__ wt, int64_t *__restrict__ orow) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nb_total) return;
+  const BgPart p = parts[find_part_by_bs(parts, np, g)];
+  const int k = g - p.bs;
+  orow[g] = p.out_row + k;
+  BgVert bv;
+  bv.bs = p.bs;
+  bv.bdy = p.bdy;
+  bv.woff = p.cb + k * p.bdy;
+  for (int c = 0; c < p.bdy; ++c) wt[bv.woff + c] = p.fw[(int64_t)c * p.S + k];
+  int nc = 0;
+  const int v = st2ed[p.offset + k];
+  for (int e = row_off[v]; e < row_off[v + 1]; ++e) {
+    const int t = cut_target(e, p.part_id, parts, pidx_by_pid, col, graph_id, ed2st);
+    if (t < 0) continue;
+    if (nc < kBgCutW) {
+      bv.cut_t[nc] = t;
+      bv.cut_w[nc] = w ? w[e] : 1;
+    }
+    ++nc;
+  }
+  for (int i = nc; i < kBgCutW; ++i) {
+    bv.cut_t[i] = g;  // unused slots: a zero-weight self arc, which never improves
+    bv.cut_w[i] = 0;
+  }
+  bv.ncut = nc;
+  verts[g] = bv;
+}
+
+constexpr int kClThreads = 256;
+constexpr int kClPad = 8;  // slack entries after the clique weights and the LDS row
+
+// Row s of the closed G_B matrix by Bellman-Ford sweeps, one workgroup per source,
+// distances in LDS.  A sweep (a) pulls every vertex v over its part's clique,
+// dist[v] = min(dist[v], min_c dist[bs + c] + d_q(c, k)) -- the in-part distances
+// between boundary vertices are already closed, so one pull settles a part -- and
+// (b) pushes every cut arc with an LDS atomicMin.  Values only decrease and are
+// always lengths of real paths; a sweep that changes nothing leaves every arc
+// satisfied, so dist is then the exact distance row (at most nb_total sweeps with
+// non-negative weights).  Thread t owns vertices t + 256 i (i < VPT) and keeps their
+// records in registers for all sweeps.
+template <int VPT>
+__global__ __launch_bounds__(kClThreads) void bg_closure_kernel(
+    const BgVert *__restrict__ verts, const int *__restrict__ wt, int nb_total, int Sb,
+    const BgPart *__restrict__ parts, int np, const int *__restrict__ pidx_by_pid,
+    const int *__restrict__ row_off, const int *__restrict__ col, const int *__restrict__ w,
+    const int *__restrict__ graph_id, const int *__restrict__ st2ed,
+    const int *__restrict__ ed2st, int *__restrict__ out) {
+  extern __shared__ int dist[];
+  const int s = blockIdx.x;
+  const int tid = threadIdx.x;
+  BgVert mine[VPT];
+#pragma unroll
+  for (int i = 0; i < VPT; ++i) {
+    const int v = tid + i * kClThreads;
+    if (v < nb_total) {
+      mine[i] = verts[v];
+    } else {
+      mine[i].bs = mine[i].bdy = mine[i].woff = mine[i].ncut = 0;
+#pragma unroll
+      for (int c = 0; c < kBgCutW; ++c) {
+        mine[i].cut_t[c] = 0;
+        mine[i].cut_w[c] = kInf;  // never improves anything
+      }
+    }
+  }
+  for (int j = tid; j < Sb + kClPad; j += kClThreads) dist[j] = j == s ? 0 : kInf;
+  __syncthreads();
+  for (int sweep = 0; sweep <= nb_total; ++sweep) {
+    bool changed = false;
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) {
+      const int v = tid + i * kClThreads;
+      if (v >= nb_total) continue;
+      const int *wv = wt + mine[i].woff;
+      const int *dv = dist + mine[i].bs;
+      int best = dist[v];
+      // Eight steps per iteration, loads first (wt and dist hold kClPad slack entries).
+      for (int c0 = 0; c0 < mine[i].bdy; c0 += kClPad) {
+        int x[kClPad], y[kClPad];
+#pragma unroll
+        for (int u = 0; u < kClPad; ++u) {
+          x[u] = wv[c0 + u];
+          y[u] = dv[c0 + u];
+        }
+#pragma unroll
+        for (int u = 0; u < kClPad; ++u)
+          if (c0 + u < mine[i].bdy) best = min(best, x[u] + y[u]);
+      }
+      if (best < dist[v]) {
+        dist[v] = best;
+        changed = true;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) {
+      const int u = tid + i * kClThreads;
+      if (u >= nb_total) continue;
+      const int du = dist[u];
+      if (du >= kInf) continue;
+#pragma unroll
+      for (int c = 0; c < kBgCutW; ++c) {
+        const int t = mine[i].cut_t[c];
+        const int nd = du + mine[i].cut_w[c];
+        if (nd < dist[t] && nd < atomicMin(&dist[t], nd)) changed = true;
+      }
+      if (mine[i].ncut > kBgCutW) {  // more cut arcs than the record holds: read the CSR
+        const BgPart p = parts[find_part_by_bs(parts, np, u)];
+        const int vtx = st2ed[p.offset + (u - p.bs)];
+        for (int e = row_off[vtx]; e < row_off[vtx + 1]; ++e) {
+          const int t = cut_target(e, p.part_id, parts, pidx_by_pid, col, graph_id, ed2st);
+          if (t < 0) continue;
+          const int nd = du + (w ? w[e] : 1);
+          if (nd < dist[t] && nd < atomicMin(&dist[t], nd)) changed = true;
+        }
+      }
+    }
+    if (!__syncthreads_or(changed)) break;
+  }
+  int *dst = out + (int64_t)s * Sb;
+  for (int j = tid * 4; j < Sb; j += kClThreads * 4)
+    *reinterpret_cast<v4i *>(dst + j) = *reinterpret_cast<const v4i *>(&dist[j]);
+}
+
+constexpr int kRowsCols = 1024;  // output columns per workgroup (256 threads x 4)
+constexpr int kRowsMax = 16;     // G_B rows staged in LDS at a time
+
+// Stages rows g .. g + rows - 1 of the closed G_B matrix into LDS (stride ldl, INF
+// tail past Sb) and their output row offsets (orow[g] * ld_out) into ooff.  Sb and
+// ldl are multiples of 4; 16-byte loads, eight in flight per thread before the LDS
+// writes.
+__device__ __forceinline__ void stage_rows(int *lds, int64_t *ooff, const int *__restrict__ mat,
+                                           const int64_t *__restrict__ orow, int64_t ld_out,
+                                           int Sb, int ldl, int g, int rows) {
+  if ((int)threadIdx.x < rows) ooff[threadIdx.x] = orow[g + threadIdx.x] * ld_out;
+  const int q4 = ldl / 4, total = rows * q4;
+  const v4i inf4 = {kInf, kInf, kInf, kInf};
+  for (int base = threadIdx.x; base < total; base += 8 * 256) {
+    v4i t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int idx = base + u * 256;
+      const int r = idx / q4, c = (idx - r * q4) * 4;
+      t[u] = (idx < total && c < Sb)
+                 ? *reinterpret_cast<const v4i *>(mat + (int64_t)(g + r) * Sb + c)
+                 : inf4;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int idx = base + u * 256;
+      if (idx < total) *reinterpret_cast<v4i *>(lds + idx * 4) = t[u];
+    }
+  }
+}
+
+struct ColInfo {  // one output column: its part's G_B offset and in-part column
+  int bs, bdy, S, l;
+  const int *fw;
+};
+
+// A column past n gets part 0's matrix with no boundary (every fin entry FAP_INF),
+// so its loads stay in bounds.
+__device__ __forceinline__ ColInfo column_info(const BgPart *__restrict__ parts,
+                                               const int *__restrict__ pidx_by_pid,
+                                               const int *__restrict__ graph_id,
+                                               const int *__restrict__ ed2st, int j, int n) {
+  if (j < n) {
+    const BgPart q = parts[pidx_by_pid[graph_id[j]]];
+    return ColInfo{q.bs, q.bdy, q.S, ed2st[j] - q.offset, q.fw};
+  }
+  return ColInfo{0, 0, parts[0].S, 0, parts[0].fw};
+}
+
+// fin[c] = d_q(c, l) for c < bdy, FAP_INF beyond: unconditional loads (rows c < KP <=
+// 64 <= S of the part's padded matrix are always readable), then the mask.
+template <int KP>
+__device__ __forceinline__ void load_fin(int (&fin)[KP], const ColInfo &ci, int c0) {
+#pragma unroll
+  for (int c = 0; c < KP; ++c) fin[c] = ci.fw[(int64_t)(c0 + c) * ci.S + ci.l];
+#pragma unroll
+  for (int c = 0; c < KP; ++c)
+    if (c0 + c >= ci.bdy) fin[c] = kInf;
+}
+
+// Every part has at most KP boundary vertices.  Thread t handles columns
+// blockIdx.x * 1024 + e * 256 + t (e < 4): the lanes of a wave cover 64 consecutive
+// vertices, so they share one or a few parts (LDS broadcast reads) and every store
+// is one coalesced 256-byte run.  Each lane loads d_q(c, l) of its columns once
+// (registers), then walks its workgroup's share of G_B rows (blockIdx.y of gridDim.y
+// splits) through LDS, R rows at a time.  LDS stride ldl = Sb + KP (INF tail).
+template <int KP>
+__global__ __launch_bounds__(256) void bg_rows_kernel(
+    const BgPart *__restrict__ parts, const int64_t *__restrict__ orow, int nb_total,
+    const int *__restrict__ mat, int Sb, int R, const int *__restrict__ pidx_by_pid,
+    const int *__restrict__ graph_id, const int *__restrict__ ed2st, int n,
+    int *__restrict__ out, int64_t ld_out) {
+  extern __shared__ int lds[];
+  __shared__ int64_t ooff[kRowsMax];
+  const int ldl = Sb + KP;
+  const int per = (nb_total + gridDim.y - 1) / gridDim.y;
+  const int gbeg = blockIdx.y * per, gend = min(nb_total, gbeg + per);
+  const int jb = blockIdx.x * kRowsCols + threadIdx.x;
+  int bs[4];
+  int fin[4][KP];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const ColInfo ci = column_info(parts, pidx_by_pid, graph_id, ed2st, jb + e * 256, n);
+    bs[e] = ci.bs;
+    load_fin<KP>(fin[e], ci, 0);
+  }
+  for (int g0 = gbeg; g0 < gend; g0 += R) {
+    const int rows = min(R, gend - g0);
+    __syncthreads();  // the previous chunk's reads are done
+    stage_rows(lds, ooff, mat, orow, ld_out, Sb, ldl, g0, rows);
+    __syncthreads();
+    for (int r = 0; r < rows; ++r) {
+      const int *lr = lds + r * ldl;
+      int *orow_p = out + ooff[r];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int *a = lr + bs[e];
+        int acc = kInf;
+#pragma unroll
+        for (int c = 0; c < KP; c += 2) acc = min3_i32(acc, a[c] + fin[e][c], a[c + 1] + fin[e][c + 1]);
+        const int j = jb + e * 256;
+        if (j < n) __builtin_nontemporal_store(acc, orow_p + j);
+      }
+    }
+  }
+}
+
+// Parts with more than 32 boundary vertices: the boundary is walked in chunks of KC
+// vertices; per chunk each lane loads d_q(c, l) of its four columns (FAP_INF past
+// b_q) and folds the chunk into all staged rows' accumulators.  A chunk past the
+// lane's own boundary reads the INF tail (base clamped to Sb), so every LDS read
+// stays inside its row whatever the boundary sizes.  Columns as in bg_rows_kernel;
+// LDS stride ldl = Sb + KC.
+template <int KC>
+__global__ __launch_bounds__(256) void bg_rows_wide_kernel(
+    const BgPart *__restrict__ parts, const int64_t *__restrict__ orow, int nb_total,
+    const int *__restrict__ mat, int Sb, int R, int nchunks, const int *__restrict__ pidx_by_pid,
+    const int *__restrict__ graph_id, const int *__restrict__ ed2st, int n,
+    int *__restrict__ out, int64_t ld_out) {
+  extern __shared__ int lds[];
+  __shared__ int64_t ooff[kRowsMax];
+  const int ldl = Sb + KC;
+  const int per = (nb_total + gridDim.y - 1) / gridDim.y;
+  const int gbeg = blockIdx.y * per, gend = min(nb_total, gbeg + per);
+  const int jb = blockIdx.x * kRowsCols + threadIdx.x;
+  ColInfo ci[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+    ci[e] = column_info(parts, pidx_by_pid, graph_id, ed2st, jb + e * 256, n);
+  for (int g0 = gbeg; g0 < gend; g0 += R) {
+    const int rows = min(R, gend - g0);
+    __syncthreads();
+    stage_rows(lds, ooff, mat, orow, ld_out, Sb, ldl, g0, rows);
+    __syncthreads();
+    int acc[kRowsMax][4];
+#pragma unroll
+    for (int r = 0; r < kRowsMax; ++r)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc[r][e] = kInf;
+    for (int ch = 0; ch < nchunks; ++ch) {
+      const int c0 = ch * KC;
+      int fin[4][KC];
+      int base[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        base[e] = min(ci[e].bs + c0, Sb);
+        // Chunks past this column's boundary need no loads (rows c0 + c may pass S).
+        if (c0 < ci[e].bdy) {
+          load_fin<KC>(fin[e], ci[e], c0);
+        } else {
+#pragma unroll
+          for (int c = 0; c < KC; ++c) fin[e][c] = kInf;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kRowsMax; ++r) {
+        if (r >= rows) continue;  // (continue, not break: keeps the loop unrolled)
+        const int *lr = lds + r * ldl;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+          for (int c = 0; c < KC; c += 2)
+            acc[r][e] = min3_i32(acc[r][e], lr[base[e] + c] + fin[e][c],
+                                 lr[base[e] + c + 1] + fin[e][c + 1]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kRowsMax; ++r) {
+      if (r >= rows) continue;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int j = jb + e * 256;
+        if (j < n) __builtin_nontemporal_store(acc[r][e], out + ooff[r] + j);
+      }
+    }
+  }
+}
+
+}  // namespace
+
+int launch_bg_vert(const BgPart *d_parts, int np, int nb_total, const int *d_pidx_by_pid,
+                   const int *row_off, const int *col, const int *w, const int *graph_id,
+                   const int *st2ed, const int *ed2st, BgVert *d_verts, int *d_wt,
+                   int64_t *d_orow, hipStream_t stream) {
+  if (nb_total <= 0) return FAP_OK;
+  hipLaunchKernelGGL(bg_vert_kernel, dim3((nb_total + 255) / 256), dim3(256), 0, stream, d_parts,
+                     np, nb_total, d_pidx_by_pid, row_off, col, w, graph_id, st2ed, ed2st,
+                     d_verts, d_wt, d_orow);
+  FAP_CHECK_LAUNCH();
+  return FAP_OK;
+}
+
+int launch_bg_closure(const BgVert *d_verts, const int *d_wt, int nb_total, int Sb,
+                      const BgPart *d_parts, int np, const int *d_pidx_by_pid, const int *row_off,
+                      const int *col, const int *w, const int *graph_id, const int *st2ed,
+                      const int *ed2st, int *d_mat, hipStream_t stream) {
+  if (nb_total <= 0) return FAP_OK;
+  const int vpt = (nb_total + kClThreads - 1) / kClThreads;
+#define FAP_CL_CASE(V)                                                                        \
+  if (vpt <= V) {                                                                             \
+    hipLaunchKernelGGL(bg_closure_kernel<V>, dim3(nb_total), dim3(kClThreads),                 \
+                       (size_t)(Sb + kClPad) * 4,                                              \
+                       stream, d_verts, d_wt, nb_total, Sb, d_parts, np, d_pidx_by_pid,        \
+                       row_off, col, w, graph_id, st2ed, ed2st, d_mat);                        \
+    FAP_CHECK_LAUNCH();                                                                       \
+    return FAP_OK;                                                                            \
+  }
+  FAP_CL_CASE(2)
+  FAP_CL_CASE(4)
+  FAP_CL_CASE(8)
+  FAP_CL_CASE(16)
+#undef FAP_CL_CASE
+  set_error("launch_bg_closure: %d boundary vertices exceed the closure kernel", nb_total);
+  return FAP_ERR_INVALID;
+}
+
+// Rows of G_B staged at a time: they fit in 64 KB of LDS.
+int bg_rows_chunk(int ldl) { return std::max(1, std::min(kRowsMax, 65536 / (ldl * 4))); }
+
+int launch_bg_rows(const BgPart *d_parts, const int64_t *d_orow, int nb_total, const int *d_mat,
+                   int Sb, int max_bdy, const int *d_pidx_by_pid, const int *graph_id,
+                   const int *ed2st, int n, int *d_out, int64_t ld_out, hipStream_t stream) {
+  if (nb_total <= 0 || n <= 0) return FAP_OK;
+  const int cols = (n + kRowsCols - 1) / kRowsCols;
+  // Row splits: about two workgroups per CU in all, each split at least one chunk.
+  const bool narrow = max_bdy <= 32;
+  const int KP = narrow ? std::max(8, (max_bdy + 7) / 8 * 8) : 0;
+  const int KC = narrow ? 0 : 16;
+  const int ldl = Sb + (narrow ? KP : KC);
+  const int R = bg_rows_chunk(ldl);
+  const int splits = std::max(1, std::min((nb_total + R - 1) / R, (512 + cols - 1) / cols));
+  const dim3 grid(cols, splits);
+  const size_t lds = (size_t)R * ldl * 4;
+#define FAP_BG_CASE(KPV)                                                                      \
+  case KPV:                                                                                   \
+    hipLaunchKernelGGL((bg_rows_kernel<KPV>), grid, dim3(256), lds, stream, d_parts, d_orow,   \
+                       nb_total, d_mat, Sb, R, d_pidx_by_pid, graph_id, ed2st, n, d_out, ld_out); \
+    break;
+  if (narrow) {
+    switch (KP) {
+      FAP_BG_CASE(8)
+      FAP_BG_CASE(16)
+      FAP_BG_CASE(24)
+      FAP_BG_CASE(32)
+      default:
+        set_error("launch_bg_rows: unsupported boundary width %d", KP);
+        return FAP_ERR_INVALID;
+    }
+  } else {
+    const int nchunks = (max_bdy + KC - 1) / KC;
+    hipLaunchKernelGGL((bg_rows_wide_kernel<16>), grid, dim3(256), lds, stream, d_parts, d_orow,
+                       nb_total, d_mat, Sb, R, nchunks, d_pidx_by_pid, graph_id, ed2st, n, d_out,
+                       ld_out);
+  }
+#undef FAP_BG_CASE
+  FAP_CHECK_LAUNCH();
+  return FAP_OK;
+}
+
+}  // namespace fap_hip

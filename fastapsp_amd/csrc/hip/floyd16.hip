@@ -275,7 +275,8 @@ __global__ __launch_bounds__(256) void fw16_unpack_kernel(const UnpackDesc *__re
     const int j = j0 + e;
     if (v >= kCap16) {
       out[e] = kInf;
-      if (i < d.s && j < d.s && comp_sorted[d.offset + i] == comp_sorted[d.offset + j])
+      if (comp_sorted && i < d.s && j < d.s &&
+          comp_sorted[d.offset + i] == comp_sorted[d.offset + j])
         over = true;
     } else {
       out[e] = v;

@@ -26,9 +26,11 @@ struct Fw16Desc {
 };
 
 // Expansion of one part's 16-bit result into its int32 FW matrix (both S x S,
-// ld = S), 8 entries per work item.  Entries equal to the cap become FAP_INF; a
-// capped entry between two vertices of one connected component marks the part in
-// flags[part_index] (its true distance is >= the cap: the part is redone in int32).
+// ld = S), 8 entries per work item.  Entries equal to the cap become FAP_INF; with
+// component labels, a capped entry between two vertices of one connected component
+// marks the part in flags[part_index] (its true distance is >= the cap: the part is
+// redone in int32).  Without labels the caller guarantees that every finite entry
+// is below the cap.
 struct UnpackDesc {
   const uint16_t *src;
   int *dst;
@@ -40,7 +42,9 @@ struct UnpackDesc {
 //   C[i][j] = min_k A[i][k] + B[k][j]  (FAP_INF when K == 0), i < M, j < N.
 // When part_id > 0 the epilogue also performs the decode of the part's own
 // columns: for a column j with graph_id[j] == part_id the stored value is
-// fw[(fw_row0 + i) * ldf + ed2st[j] - offset] (the Floyd-Warshall distance).
+// min(C[i][j], fw[(fw_row0 + i) * ldf + ed2st[j] - offset]).  With boundary rows
+// in the FW matrix (SSSP form) the FW value is the distance itself; with an
+// in-part FW matrix the minimum adds the paths that leave the part.
 struct MergeDesc {
   const int *A;
   int64_t lda;
@@ -66,8 +70,32 @@ struct BuildDesc {
   int S;            // padded edge (multiple of 64, <= kBuildMaxS)
   int s, bdy, offset, part_id;
   int row_begin;    // first global build row of this part
-  const int *dist_rows;  // the part's SSSP rows (row i = boundary vertex i)
+  const int *dist_rows;  // the part's SSSP rows (row i = boundary vertex i); NULL: every
+                         // row from the part's own arcs (in-part distances)
   int64_t ld_dist;
+};
+
+// One part of the boundary graph (boundary.hip): its in-part Floyd-Warshall matrix
+// (int32, ld = S; the leading bdy rows and columns are its boundary vertices), its
+// first index bs among the boundary graph's vertices and its first output row.
+struct BgPart {
+  const int *fw;
+  int S, bdy, offset, part_id;
+  int bs;
+  int cb;  // first entry of the part's clique block in the G_B clique weights
+  int64_t out_row;
+};
+
+// One vertex of the boundary graph as the closure kernel reads it: its part's first
+// G_B index and boundary count, its clique in-weights at woff (woff + c holds the
+// in-part distance from boundary vertex c of the part to this one) and its first
+// kBgCutW cut arcs (G_B target, weight); ncut > kBgCutW means the rest are read from
+// the CSR.
+constexpr int kBgCutW = 4;
+struct BgVert {
+  int bs, bdy, woff, ncut;
+  int cut_t[kBgCutW];
+  int cut_w[kBgCutW];
 };
 
 // Decode of the boundary rows' own columns of one part.

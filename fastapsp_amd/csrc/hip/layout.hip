@@ -25,8 +25,8 @@ __device__ __forceinline__ int find_by_begin(const D *__restrict__ d, int nd, in
 }
 
 // One workgroup per padded row of every part's FW matrix.
-//   row i <  bdy : FW[i][j] = dist_rows[i][st2ed[offset + j]]
-//   row i >= bdy : FAP_INF, then the part's own out-arcs of st2ed[offset + i] in
+//   row i <  bdy : FW[i][j] = dist_rows[i][st2ed[offset + j]]   (dist_rows set)
+//   other rows   : FAP_INF, then the part's own out-arcs of st2ed[offset + i] in
 //                  CSR order (a later arc overwrites an earlier one)
 //   diagonal     : 0;  padding (j >= s or i >= s): FAP_INF.
 // With d.fw16 set, the row is written as 16-bit entries min(value, kCap16).
@@ -46,7 +46,7 @@ __global__ __launch_bounds__(256) void build_kernel(const BuildDesc *__restrict_
   __shared__ int rowbuf[kBuildMaxS];
   const int tid = threadIdx.x;
 
-  if (i < d.s && i < d.bdy) {
+  if (i < d.s && i < d.bdy && d.dist_rows) {
     const int *src = d.dist_rows + (int64_t)i * d.ld_dist;
     for (int j = tid; j < d.S; j += 256) rowbuf[j] = (j < d.s) ? src[st2ed[d.offset + j]] : kInf;
   } else {
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void build_kernel(const BuildDesc *__restrict_
   }
   __syncthreads();
   if (tid == 0 && i < d.s) {
-    if (i >= d.bdy) {
+    if (i >= d.bdy || !d.dist_rows) {
       const int ver = st2ed[d.offset + i];
       const int e1 = row_off[ver + 1];
       for (int e = row_off[ver]; e < e1; ++e) {

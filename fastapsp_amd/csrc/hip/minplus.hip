@@ -5,7 +5,9 @@
 // semantics, init MAXVALUE, strict '>') plus MyMat1Build
 // (fap/utils/util_Myalgorithm.h:278-285: the A operand is read in place as a
 // strided view of the Floyd-Warshall matrix) and MysubMatDecode_path (:236-247,
-// fused: columns of the part's own vertices take the Floyd-Warshall distance).
+// fused: columns of the part's own vertices take the minimum of the product and
+// the Floyd-Warshall distance -- the distance itself when the FW matrix holds the
+// boundary rows, which no product entry undercuts).
 // The reference's GPU twin is MatrixMulCUDA6_path<32,32,32,4,4>
 // (fap/kernel/cuda/minplus.cu:40-189) with padded copies in and out (:268-359).
 //
@@ -153,7 +155,7 @@ __global__ __launch_bounds__(256) void merge_kernel(const MergeDesc *__restrict_
       const int *fwrow = d.fw + (int64_t)(d.fw_row0 + i) * d.ldf;
 #pragma unroll
       for (int c = 0; c < 8; ++c)
-        if (local_col[c] >= 0) acc[r][c] = fwrow[local_col[c]];
+        if (local_col[c] >= 0) acc[r][c] = min(acc[r][c], fwrow[local_col[c]]);
     }
     int *crow = d.C + (int64_t)i * d.ldc;
 #pragma unroll
@@ -225,7 +227,7 @@ __device__ __forceinline__ void stream_own(int (&acc)[4], const MergeDesc &d, in
     const FAP_GLOBAL int *f = gptr(d.fw + (int64_t)(d.fw_row0 + i) * d.ldf);
 #pragma unroll
     for (int e = 0; e < 4; ++e)
-      if (own[e] >= 0) acc[e] = f[own[e]];
+      if (own[e] >= 0) acc[e] = min(acc[e], f[own[e]]);
   }
 }
 

@@ -36,6 +36,17 @@ int launch_sssp(const int *d_row_off, const int *d_col, const int *d_w, int n, c
                 int nsrc, const int64_t *d_rows, int *d_out, int64_t ld, bool vec_fill,
                 hipStream_t stream);
 int launch_fw(const FwDesc *d_descs, int ndesc, int max_nb, hipStream_t stream);
+int launch_bg_vert(const BgPart *d_parts, int np, int nb_total, const int *d_pidx_by_pid,
+                   const int *row_off, const int *col, const int *w, const int *graph_id,
+                   const int *st2ed, const int *ed2st, BgVert *d_verts, int *d_wt,
+                   int64_t *d_orow, hipStream_t stream);
+int launch_bg_closure(const BgVert *d_verts, const int *d_wt, int nb_total, int Sb,
+                      const BgPart *d_parts, int np, const int *d_pidx_by_pid, const int *row_off,
+                      const int *col, const int *w, const int *graph_id, const int *st2ed,
+                      const int *ed2st, int *d_mat, hipStream_t stream);
+int launch_bg_rows(const BgPart *d_parts, const int64_t *d_orow, int nb_total, const int *d_mat,
+                   int Sb, int max_bdy, const int *d_pidx_by_pid, const int *graph_id,
+                   const int *ed2st, int n, int *d_out, int64_t ld_out, hipStream_t stream);
 int launch_fw16(const Fw16Desc *d_descs, int ndesc, int max_nb, hipStream_t stream);
 int launch_fw16_unpack(const UnpackDesc *d_descs, int nd, int64_t total_items,
                        const int *d_comp_sorted, int *d_flags, hipStream_t stream);
@@ -66,6 +77,9 @@ int launch_gather_sources(const int *offsets, const int *src_begin, const int64_
 constexpr int kBuildMaxS = 8192;
 constexpr int kTile = 64;
 constexpr int kStreamMaxK = 32;  // larger boundary counts use the LDS-tiled product
+// Boundary-graph form of the boundary rows (boundary.hip): at most this many boundary
+// vertices in all (one workgroup per boundary vertex holds a distance row in LDS).
+constexpr int kBgMaxTotal = 4096;
 
 // Merge kernel of a product: 0 = LDS-tiled, else the streaming form with this
 // padded K (A rows of the padded FW matrices always hold >= 64 entries).
@@ -85,6 +99,8 @@ Workspace g_ws_fw;    // Floyd-Warshall matrices
 Workspace g_ws_fw16;  // 16-bit Floyd-Warshall matrices
 Workspace g_ws_meta;  // descriptors, sources, row indices
 Workspace g_ws_ell;   // ELL adjacency of the unit-weight graph
+Workspace g_ws_bg;    // closed boundary-graph matrix
+Workspace g_ws_bgv;   // boundary-graph vertices and clique weights
 
 struct EventPool {
   std::vector<hipEvent_t> ev;
@@ -110,7 +126,7 @@ struct Span {
 std::vector<Span> g_spans;
 hipEvent_t g_total[2] = {nullptr, nullptr};
 constexpr int kChunks = 8;
-int g_parts16 = 0, g_redone = 0;
+int g_parts16 = 0, g_redone = 0, g_bgraph = 0;
 
 // What the last pass left for the flag check of the 16-bit form.
 struct Pass16 {
@@ -128,7 +144,7 @@ int timing_start(hipStream_t st) {
   g_events.used = 0;
   g_spans.clear();
   g_total[0] = g_total[1] = nullptr;
-  g_parts16 = g_redone = 0;
+  g_parts16 = g_redone = g_bgraph = 0;
   FAP_TRY(g_events.take(&g_total[0]));
   FAP_CHECK_HIP(hipEventRecord(g_total[0], st));
   return FAP_OK;
@@ -185,7 +201,35 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   const int form = mode & 0xFF;
   const bool overlap = form == FAP_MODE_OVERLAP;
   const bool batched = form != FAP_MODE_PLAIN;  // one descriptor table for all parts
-  const bool use16 = g->d_comp != nullptr && !(mode & FAP_MODE_FLAG_INT32_FW);
+  // Boundary rows from the boundary graph (boundary.hip) instead of one SSSP per
+  // boundary vertex: BATCHED form over parts that cover every vertex, with few
+  // boundary vertices.  The Floyd-Warshall matrices then hold in-part distances.
+  bool hier = false;
+  int bg_total = 0, bg_maxb = 0, max_pid = 0;
+  if (form == FAP_MODE_BATCHED && !(mode & FAP_MODE_FLAG_SSSP)) {
+    std::vector<std::pair<int, int>> span_of;  // (offset, num) of every non-empty part
+    for (const fap_part_desc &q : act) {
+      span_of.push_back({q.offset, q.num});
+      bg_total += q.bdy;
+      bg_maxb = std::max(bg_maxb, q.bdy);
+      max_pid = std::max(max_pid, q.part_id);
+    }
+    std::sort(span_of.begin(), span_of.end());
+    bool cover = true;
+    int next = 0;
+    for (const auto &x : span_of) {
+      cover = cover && x.first == next;
+      next = x.first + x.second;
+    }
+    hier = cover && next == n && bg_total <= kBgMaxTotal;
+  }
+  // 16-bit Floyd-Warshall: on in-part matrices of unit-weight graphs every finite
+  // entry is at most s - 1 < 8192 < the cap, so the cap means "unreachable inside
+  // the part" exactly; on SSSP-form matrices of graphs with component labels the
+  // cap is checked against the labels (and the part redone in int32 if needed).
+  const bool use16 = !(mode & FAP_MODE_FLAG_INT32_FW) &&
+                     (hier ? g->d_w == nullptr : g->d_comp != nullptr);
+  const bool flags16 = use16 && !hier;
   const int nchunks = overlap ? std::min(kChunks, P) : 1;
   std::vector<int> chunk_of(P);
   for (int p = 0; p < P; ++p) chunk_of[p] = (int)((int64_t)p * nchunks / P);
@@ -229,7 +273,8 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
     uint16_t *fw16 = use16 ? fw16_base + fw_off[p] : nullptr;
     const int M = q.num - q.bdy;
     bd[p] = BuildDesc{fw, S[p], fw16, S[p], q.num, q.bdy, q.offset, q.part_id,
-                      batched ? rows_total : 0, d_out + q.out_row * ld_out, ld_out};
+                      batched ? rows_total : 0, hier ? nullptr : d_out + q.out_row * ld_out,
+                      ld_out};
     fd[p] = FwDesc{fw, S[p], nb[p]};
     fd16[p] = Fw16Desc{fw16, S[p], nb[p]};
     MergeDesc m{};
@@ -310,6 +355,36 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   const int64_t o_off = carve(4LL * P), o_sb = carve(4LL * P), o_or = carve(8LL * P);
   const int64_t o_src = carve(4LL * std::max(src_total, 1));
   const int64_t o_rows = carve(8LL * std::max(src_total, 1));
+  // Boundary graph: part table, part id -> table index, the matrix.
+  std::vector<BgPart> bgp;
+  std::vector<int> pidx_by_pid;
+  const int Sb = (int)align_up(bg_total, kTile);
+  int *bg_mat = nullptr, *bg_wt = nullptr;
+  BgVert *bg_verts = nullptr;
+  int64_t *bg_orow = nullptr;
+  if (hier) {
+    FAP_TRY(g_ws_bg.ensure(std::max<int64_t>(1, (int64_t)Sb * Sb) * 4));
+    bg_mat = static_cast<int *>(g_ws_bg.ptr);
+    pidx_by_pid.assign(max_pid + 1, 0);
+    int bs = 0, cb = 0;
+    for (int p = 0; p < P; ++p) {
+      const fap_part_desc &q = act[p];
+      bgp.push_back(
+          BgPart{fw_base + fw_off[p], S[p], q.bdy, q.offset, q.part_id, bs, cb, q.out_row});
+      pidx_by_pid[q.part_id] = p;
+      bs += q.bdy;
+      cb += q.bdy * q.bdy;  // <= 4096^2
+    }
+    const int64_t vbytes = align_up((int64_t)sizeof(BgVert) * std::max(bg_total, 1), 256);
+    const int64_t obytes = align_up(8LL * std::max(bg_total, 1), 256);
+    FAP_TRY(g_ws_bgv.ensure(vbytes + obytes + 4LL * (cb + 64)));  // + slack read past the end
+    char *base = static_cast<char *>(g_ws_bgv.ptr);
+    bg_verts = reinterpret_cast<BgVert *>(base);
+    bg_orow = reinterpret_cast<int64_t *>(base + vbytes);
+    bg_wt = reinterpret_cast<int *>(base + vbytes + obytes);
+  }
+  const int64_t o_bgp = carve(sizeof(BgPart) * std::max<size_t>(bgp.size(), 1));
+  const int64_t o_pidx = carve(4LL * std::max<size_t>(pidx_by_pid.size(), 1));
   const int64_t o_flags = carve(4LL * P), o_comp = carve(4LL * n);  // device-written
   FAP_TRY(g_ws_meta.ensure(off));
   char *meta = static_cast<char *>(g_ws_meta.ptr);
@@ -323,7 +398,11 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   std::memcpy(host.data() + o_off, offsets.data(), 4LL * P);
   std::memcpy(host.data() + o_sb, src_begin.data(), 4LL * P);
   std::memcpy(host.data() + o_or, out_rows.data(), 8LL * P);
-  FAP_CHECK_HIP(hipMemcpyAsync(meta, host.data(), o_src, hipMemcpyHostToDevice, st));
+  if (hier) {
+    std::memcpy(host.data() + o_bgp, bgp.data(), sizeof(BgPart) * bgp.size());
+    std::memcpy(host.data() + o_pidx, pidx_by_pid.data(), 4 * pidx_by_pid.size());
+  }
+  FAP_CHECK_HIP(hipMemcpyAsync(meta, host.data(), o_flags, hipMemcpyHostToDevice, st));
   FAP_CHECK_HIP(hipStreamSynchronize(st));
   const BuildDesc *d_bd = reinterpret_cast<const BuildDesc *>(meta + o_bd);
   const FwDesc *d_fd = reinterpret_cast<const FwDesc *>(meta + o_fd);
@@ -336,6 +415,8 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   int64_t *d_rows = reinterpret_cast<int64_t *>(meta + o_rows);
   const Fw16Desc *d_fd16 = reinterpret_cast<const Fw16Desc *>(meta + o_fd16);
   const UnpackDesc *d_ud = reinterpret_cast<const UnpackDesc *>(meta + o_ud);
+  const BgPart *d_bgp = reinterpret_cast<const BgPart *>(meta + o_bgp);
+  const int *d_pidx = reinterpret_cast<const int *>(meta + o_pidx);
   int *d_flags = reinterpret_cast<int *>(meta + o_flags);
   int *d_comp_sorted = reinterpret_cast<int *>(meta + o_comp);
 
@@ -359,7 +440,7 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
     return FAP_OK;
   };
   if (!append) FAP_TRY(event(st, &g_total[0]));
-  if (use16) {
+  if (flags16) {
     FAP_TRY(fap_hip_fill_i32(d_flags, 0, P, st));
     FAP_TRY(launch_comp_sorted(g->d_comp, g->d_st2ed, n, d_comp_sorted, st));
   }
@@ -368,12 +449,13 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   auto run_fw = [&](int p0, int p1, int mnb, int64_t unpack_items, hipStream_t on) -> int {
     if (!use16) return launch_fw(d_fd + p0, p1 - p0, mnb, on);
     FAP_TRY(launch_fw16(d_fd16 + p0, p1 - p0, mnb, on));
-    return launch_fw16_unpack(d_ud + p0, p1 - p0, unpack_items, d_comp_sorted, d_flags, on);
+    return launch_fw16_unpack(d_ud + p0, p1 - p0, unpack_items,
+                              flags16 ? d_comp_sorted : nullptr, d_flags, on);
   };
 
   // Boundary SSSP: the LDS-bitmap BFS for unit weights, else label correcting.
   int4 *ell = nullptr;
-  if (!g->d_w) {
+  if (!g->d_w && !hier) {
     FAP_TRY(g_ws_ell.ensure((int64_t)n * sizeof(int4)));
     ell = static_cast<int4 *>(g_ws_ell.ptr);
     FAP_TRY(launch_ell_build(g->d_row_offset, g->d_col, n, ell, st));
@@ -394,12 +476,26 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
                                g->d_graph_id, g->d_ed2st, on);
   };
 
+  // Boundary graph: vertices, closure (its distance matrix), then every part's
+  // boundary rows into d_out.
+  auto run_boundary_graph = [&](hipStream_t on) -> int {
+    if (bg_total == 0) return FAP_OK;
+    FAP_TRY(launch_bg_vert(d_bgp, P, bg_total, d_pidx, g->d_row_offset, g->d_col, g->d_w,
+                           g->d_graph_id, g->d_st2ed, g->d_ed2st, bg_verts, bg_wt, bg_orow, on));
+    FAP_TRY(launch_bg_closure(bg_verts, bg_wt, bg_total, Sb, d_bgp, P, d_pidx, g->d_row_offset,
+                              g->d_col, g->d_w, g->d_graph_id, g->d_st2ed, g->d_ed2st, bg_mat,
+                              on));
+    return launch_bg_rows(d_bgp, bg_orow, bg_total, bg_mat, Sb, bg_maxb, d_pidx, g->d_graph_id,
+                          g->d_ed2st, n, d_out, ld_out, on);
+  };
+
   if (batched) {
-    FAP_TRY(span(0, st, [&](hipStream_t on) -> int {
-      FAP_TRY(launch_gather_sources(d_offsets, d_sb, d_or, P, src_total, g->d_st2ed, ld_out,
-                                    d_src, d_rows, on));
-      return run_sssp(d_src, src_total, on);
-    }));
+    if (!hier)
+      FAP_TRY(span(0, st, [&](hipStream_t on) -> int {
+        FAP_TRY(launch_gather_sources(d_offsets, d_sb, d_or, P, src_total, g->d_st2ed, ld_out,
+                                      d_src, d_rows, on));
+        return run_sssp(d_src, src_total, on);
+      }));
     FAP_TRY(span(1, st, [&](hipStream_t on) {
       return launch_build(d_bd, P, rows_total, true, g->d_row_offset, g->d_col, g->d_w,
                           g->d_graph_id, g->d_st2ed, g->d_ed2st, on);
@@ -421,14 +517,16 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
       cdec = overlap ? chunk_dec[c] : dec_total;
       const int64_t cunpack = overlap ? chunk_unpack[c] : unpack_total;
       FAP_TRY(span(2, on, [&](hipStream_t s2) { return run_fw(p0, p1, cmax_nb, cunpack, s2); }));
+      if (hier) FAP_TRY(span(0, on, run_boundary_graph));  // hier: one chunk
       FAP_TRY(span(3, on, [&](hipStream_t s2) -> int {
         for (const MergeGroup &gr : groups)
           if (gr.chunk == c) FAP_TRY(run_group(gr, s2));
         return FAP_OK;
       }));
-      FAP_TRY(span(4, on, [&](hipStream_t s2) {
-        return launch_decode(d_dd + p0, p1 - p0, cdec, g->d_st2ed, s2);
-      }));
+      if (!hier)  // hier: the boundary rows already hold their own columns
+        FAP_TRY(span(4, on, [&](hipStream_t s2) {
+          return launch_decode(d_dd + p0, p1 - p0, cdec, g->d_st2ed, s2);
+        }));
       if (overlap) {
         hipEvent_t e;
         FAP_TRY(event(on, &e));
@@ -460,10 +558,11 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
     }
   }
   FAP_TRY(event(st, &g_total[1]));
-  if (use16) {
+  if (use16) g_parts16 += P;
+  if (hier) g_bgraph = 1;
+  if (flags16) {
     g_pass16.parts = act;
     g_pass16.d_flags = d_flags;
-    g_parts16 += P;
   }
   return FAP_OK;
 }
@@ -478,9 +577,9 @@ static int solve_parts(const fap_graph_dev *g, const fap_part_desc *parts, int32
               "fap_hip_solve_parts: null argument");
   const int form = mode & 0xFF;
   FAP_REQUIRE((form == FAP_MODE_PLAIN || form == FAP_MODE_BATCHED || form == FAP_MODE_OVERLAP) &&
-                  (mode & ~(0xFF | FAP_MODE_FLAG_INT32_FW)) == 0,
+                  (mode & ~(0xFF | FAP_MODE_FLAG_INT32_FW | FAP_MODE_FLAG_SSSP)) == 0,
               "fap_hip_solve_parts: unknown mode %d", mode);
-  g_parts16 = g_redone = 0;
+  g_parts16 = g_redone = g_bgraph = 0;
   FAP_TRY(solve_pass(g, parts, nparts, d_out, ld_out, mode, stream, false));
   if (g_pass16.parts.empty()) return FAP_OK;
   // Flag check of the 16-bit form: parts whose closure reached the cap inside one
@@ -532,6 +631,7 @@ extern "C" int fap_hip_last_stage_times(fap_stage_times *t) {
   }
   acc.parts_fw16 = g_parts16;
   acc.parts_redone = g_redone;
+  acc.boundary_graph = g_bgraph;
   *t = acc;
   return FAP_OK;
 }

@@ -51,6 +51,10 @@ enum fap_mode { FAP_MODE_PLAIN = 0, FAP_MODE_BATCHED = 1, FAP_MODE_OVERLAP = 2 }
 /* Flag OR-ed into a mode: run every Floyd-Warshall in int32 even where the 16-bit
  * form applies (see fap_hip_solve_parts).  Results are identical either way. */
 #define FAP_MODE_FLAG_INT32_FW 0x100
+/* Flag OR-ed into a mode: compute the boundary rows with one SSSP per boundary vertex
+ * even where the BATCHED form would take them from the boundary graph (see
+ * fap_hip_solve_parts).  Results are identical either way. */
+#define FAP_MODE_FLAG_SSSP 0x200
 
 /* ------------------------------------------------------------------ runtime */
 
@@ -178,6 +182,7 @@ typedef struct fap_stage_times {
   float sssp_ms, build_ms, floyd_ms, minplus_ms, decode_ms, total_ms;
   int32_t parts_fw16;   /* parts whose Floyd-Warshall ran in the 16-bit form */
   int32_t parts_redone; /* of those, parts redone in int32 (a distance reached the cap) */
+  int32_t boundary_graph; /* 1: boundary rows from the boundary graph, 0: boundary SSSP */
 } fap_stage_times;
 
 /* The whole per-part pipeline of fapGraph::run (fap/fap.cpp:195-503) for a list
@@ -194,7 +199,15 @@ typedef struct fap_stage_times {
  * unreachable when its two vertices lie in different components.  A part in which
  * a capped entry joins two vertices of one component is redone in int32 before the
  * call returns, so the results never differ from the int32 form.  In this form the
- * call waits for the stream before it returns (it reads the per-part flags). */
+ * call waits for the stream before it returns (it reads the per-part flags).
+ * Boundary rows: in the BATCHED form, when the parts cover every vertex and hold at
+ * most 4096 boundary vertices in all (and FAP_MODE_FLAG_SSSP is absent), the boundary
+ * rows come from the boundary graph instead of one SSSP per
+ * boundary vertex: each part's Floyd-Warshall runs on its in-part arcs, the graph of
+ * all boundary vertices (in-part boundary distances plus cut arcs) is closed by
+ * Floyd-Warshall, and one min-plus pass writes every boundary row.  Unit-weight graphs
+ * then always take the 16-bit Floyd-Warshall (no flag check, no wait), others int32.
+ * The stage times report this stage as sssp_ms. */
 int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *parts, int32_t nparts,
                         int32_t *d_out, int64_t ld_out, int32_t mode, void *stream);
 int fap_hip_last_stage_times(fap_stage_times *t);

@@ -1,0 +1,135 @@
+"""Boundary rows from the boundary graph (the BATCHED form of solve() when the parts
+cover the graph): results must equal the SSSP form, the plain per-part form, the
+oracle's restatement of the reference path and an exact all-pairs BFS / Dijkstra, bit
+for bit, on unit-weight, integer-weighted and directed graphs, with parts whose
+boundary exceeds one register chunk and parts that are disconnected inside."""
+import sys
+
+import numpy as np
+import pytest
+
+import fastapsp_amd as F
+import oracle as O
+from fastapsp_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+SSSP = F.FAP_MODE_FLAG_SSSP
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _exact(g):
+    lay, csr = g.layout(), g.csr()
+    w = None if np.all(csr["weight"] == 1) else csr["weight"]
+    return O.exact_rows(csr["row_offset"], csr["col"], w, lay["st2ed"])
+
+
+def _solve(g, mode):
+    g.setMode(mode)
+    g.solve()
+    info = L.last_stage_times()
+    return g.getAllDistanceI32(), info
+
+
+def _forms_agree(g, oracle_path=True):
+    a, info = _solve(g, F.FAP_MODE_BATCHED)
+    assert info["boundary_graph"] == 1
+    exact = _exact(g)
+    assert np.array_equal(a, exact), "boundary-graph form differs from exact APSP"
+    if oracle_path:
+        lay, csr = g.layout(), g.csr()
+        assert np.array_equal(a, O.solve_parts(1, g.K, csr, lay)), "differs from the oracle"
+    b, info = _solve(g, F.FAP_MODE_BATCHED | SSSP)
+    assert info["boundary_graph"] == 0
+    assert np.array_equal(a, b), "boundary-graph form differs from the SSSP form"
+    c, info = _solve(g, F.FAP_MODE_PLAIN)
+    assert info["boundary_graph"] == 0
+    assert np.array_equal(a, c), "boundary-graph form differs from the plain form"
+    d, info = _solve(g, F.FAP_MODE_BATCHED | F.FAP_MODE_FLAG_INT32_FW)
+    assert info["boundary_graph"] == 1 and info["parts_fw16"] == 0
+    assert np.array_equal(a, d), "int32 in-part Floyd-Warshall differs"
+    return a
+
+
+@pytest.mark.parametrize("args", [(64, 64, 0.10, 0.02, 0, 8),    # max b 79: 16-wide chunks
+                                  (40, 30, 0.15, 0.05, 3, 19),   # max b 27: 8-wide chunks
+                                  (33, 31, 0.10, 0.02, 5, 5)])
+def test_boundary_graph_road_grids(args):
+    say(f"boundary graph: road grid {args}")
+    g = F.FapGraph.road_grid(*args)
+    g.preCondition()
+    _forms_agree(g)
+
+
+def test_boundary_graph_road_like():
+    # The headline construction at a size the exact check finishes in seconds.
+    say("boundary graph: road-like n=9000 K=12")
+    g = F.FapGraph.road_like(9000, 0.52, 1e-4, 4, 12)
+    g.preCondition()
+    a = _forms_agree(g, oracle_path=False)
+    assert np.all(np.diag(a[:, g.getMapping()]) == 0)
+
+
+def _directed_weighted(seed, W=40, H=36, K=7, keep=0.8):
+    """Road grid made directed: each arc kept with probability `keep`, weights drawn
+    independently per arc from [1, 1000]."""
+    base = F.FapGraph.road_grid(W, H, 0.1, 0.02, seed, K)
+    c = base.csr()
+    rng = np.random.default_rng(seed)
+    n = len(c["row_offset"]) - 1
+    src = np.repeat(np.arange(n), np.diff(c["row_offset"]))
+    mask = rng.random(len(c["col"])) < keep
+    src, dst = src[mask], c["col"][mask]
+    w = rng.integers(1, 1001, len(dst)).astype(np.float32)
+    order = np.argsort(src, kind="stable")
+    ro = np.zeros(n + 1, np.int32)
+    ro[1:] = np.cumsum(np.bincount(src, minlength=n))
+    return F.FapGraph.from_csr(ro, dst[order], w[order], True, K)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_boundary_graph_directed_weighted(seed):
+    say(f"boundary graph: directed weighted road grid seed={seed}")
+    g = _directed_weighted(seed)
+    g.preCondition()
+    a = _forms_agree(g)
+    assert (a == F.FAP_INF).any()  # directed: some pairs unreachable
+
+
+def test_boundary_graph_part_disconnected_inside():
+    # Path 0-1-...-199 with part 1 = {0..49, 100..149} and part 2 = the rest: both parts
+    # are two pieces inside, joined only through the other part.
+    say("boundary graph: parts disconnected inside")
+    n = 200
+    out = [[] for _ in range(n)]
+    for i in range(n - 1):
+        out[i].append(i + 1)
+        out[i + 1].append(i)
+    ro = np.zeros(n + 1, np.int32)
+    ro[1:] = np.cumsum([len(x) for x in out])
+    col = np.array([v for x in out for v in x], np.int32)
+    g = F.FapGraph.from_csr(ro, col, None, False, 2)
+    gid = np.where((np.arange(n) // 50) % 2 == 0, 1, 2).astype(np.int32)
+    g.setPartition(gid)
+    g.preCondition()
+    a = _forms_agree(g)
+    st2ed = g.getMapping()
+    assert np.array_equal(a, np.abs(st2ed[:, None] - np.arange(n)[None, :]).astype(np.int32))
+
+
+def test_boundary_graph_solve_subgraph_unchanged():
+    # solveSubGraph solves one part: the SSSP form, same rows as the full solve.
+    say("boundary graph: solve vs solveSubGraph")
+    g = F.FapGraph.road_grid(36, 36, 0.1, 0.02, 8, 12)
+    g.preCondition()
+    full, info = _solve(g, F.FAP_MODE_BATCHED)
+    assert info["boundary_graph"] == 1
+    lay = g.layout()
+    for p in (1, 5, g.K):
+        g.solveSubGraph(p)
+        off, num = lay["offset"][p], lay["num"][p]
+        assert np.array_equal(g.getSubGraphDistanceI32(p), full[off:off + num])

@@ -1,9 +1,10 @@
-"""The 16-bit Floyd-Warshall form of the part pipeline (symmetric graphs): entries are
-min(d, 15872) in packed 16-bit arithmetic; the result must equal the int32 form and
-the oracle bit for bit, pairs in different components stay unreachable, and a part
-whose in-part distances reach the cap is redone in int32.  Road-graph distances below
-1024 are IEEE-half subnormal bit patterns, so every test here also checks that the
-half-precision minimum orders them as integers."""
+"""The 16-bit Floyd-Warshall form of the part pipeline: entries are min(d, 15872) in
+packed 16-bit arithmetic; the result must equal the int32 form and the oracle bit for
+bit.  SSSP form (FAP_MODE_FLAG_SSSP, symmetric graphs): pairs in different components
+stay unreachable and a part whose distances reach the cap is redone in int32.
+Boundary-graph form (unit weights): in-part distances never reach the cap.  Road-graph
+distances below 1024 are IEEE-half subnormal bit patterns, so every test here also
+checks that the half-precision minimum orders them as integers."""
 import sys
 
 import numpy as np
@@ -50,13 +51,17 @@ def _solve_check(g, mode):
     return got, info
 
 
-def test_fw16_taken_on_road_graph_and_equals_int32():
-    say("fw16: road grid 48x40 K=6, 16-bit vs int32")
+SSSP = F.FAP_MODE_FLAG_SSSP
+
+
+@pytest.mark.parametrize("flag", [0, SSSP])
+def test_fw16_taken_on_road_graph_and_equals_int32(flag):
+    say(f"fw16: road grid 48x40 K=6, 16-bit vs int32, flag={flag:#x}")
     g = F.FapGraph.road_grid(48, 40, 0.1, 0.03, 5, 6)
     g.preCondition()
-    a, info = _solve_check(g, F.FAP_MODE_BATCHED)
+    a, info = _solve_check(g, F.FAP_MODE_BATCHED | flag)
     assert info["parts_fw16"] == g.K and info["parts_redone"] == 0
-    b, info = _solve_check(g, F.FAP_MODE_BATCHED | F.FAP_MODE_FLAG_INT32_FW)
+    b, info = _solve_check(g, F.FAP_MODE_BATCHED | F.FAP_MODE_FLAG_INT32_FW | flag)
     assert info["parts_fw16"] == 0
     assert np.array_equal(a, b)
 
@@ -73,9 +78,11 @@ def test_fw16_exact_just_below_cap():
     ro, col, wt = _weighted_path(last)
     g = F.FapGraph.from_csr(ro, col, wt, False, 1)
     g.preCondition()
-    got, info = _solve_check(g, F.FAP_MODE_BATCHED)
+    got, info = _solve_check(g, F.FAP_MODE_BATCHED | SSSP)
     assert info["parts_fw16"] == 1 and info["parts_redone"] == 0
     assert got.max() == CAP - 1
+    again, info = _solve_check(g, F.FAP_MODE_BATCHED)  # weighted: int32 in-part FW
+    assert info["parts_fw16"] == 0 and np.array_equal(got, again)
 
 
 def test_fw16_redo_at_cap():
@@ -84,7 +91,7 @@ def test_fw16_redo_at_cap():
     ro, col, wt = _weighted_path(last)
     g = F.FapGraph.from_csr(ro, col, wt, False, 1)
     g.preCondition()
-    got, info = _solve_check(g, F.FAP_MODE_BATCHED)
+    got, info = _solve_check(g, F.FAP_MODE_BATCHED | SSSP)
     assert info["parts_fw16"] == 1 and info["parts_redone"] == 1
     assert got.max() == CAP
 
@@ -100,7 +107,7 @@ def test_fw16_redo_some_parts(mode):
     ro, col, wt = _undirected(n, edges)
     g = F.FapGraph.from_csr(ro, col, wt, False, 4)
     g.preCondition()
-    got, info = _solve_check(g, mode)
+    got, info = _solve_check(g, mode | SSSP)
     lay = g.layout()
     redo = 0
     for p in range(1, g.K + 1):
@@ -112,16 +119,17 @@ def test_fw16_redo_some_parts(mode):
     assert 1 <= redo < g.K
 
 
-def test_fw16_unreachable_pairs_are_not_redone():
+@pytest.mark.parametrize("flag", [0, SSSP])
+def test_fw16_unreachable_pairs_are_not_redone(flag):
     # Two components inside one part: the capped entries between them are
     # unreachable pairs, not overflow.
-    say("fw16: two components, K=1")
+    say(f"fw16: two components, K=1, flag={flag:#x}")
     n = 400
     edges = [(i, i + 1, 1) for i in range(199)] + [(i, i + 1, 1) for i in range(200, 399)]
     ro, col, _ = _undirected(n, edges)
     g = F.FapGraph.from_csr(ro, col, None, False, 1)
     g.preCondition()
-    got, info = _solve_check(g, F.FAP_MODE_BATCHED)
+    got, info = _solve_check(g, F.FAP_MODE_BATCHED | flag)
     assert info["parts_fw16"] == 1 and info["parts_redone"] == 0
     assert (got == F.FAP_INF).sum() == 2 * 200 * 200
 
