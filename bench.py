@@ -4,7 +4,8 @@ Workload (default): road-like undirected unweighted graph with exactly n = 114,5
 vertices (percolated grid, p_keep = 0.52, chords at rate 1e-4, largest component,
 BFS-trimmed), K = 128 parts, full n x n int32 APSP via solve() in the batched form,
 result resident in HBM (n x ld int32, about 52.5 GB).  A step is one whole solve():
-boundary SSSP, build, Floyd-Warshall, min-plus merge with fused decode, for all parts.
+in-part build and Floyd-Warshall, the boundary rows (from the boundary graph), and the
+min-plus merge with fused decode, for all parts.
 
 Prints ONE JSON line: value = n^2 distance entries per second of wall time.
 Also measured: the plain per-part launch form (one step), per-stage device times,
@@ -379,7 +380,8 @@ def main():
 
     avg = {k: float(np.mean([s[k] for s in stages])) for k in stages[0]}
     fw16 = stages[-1]["parts_fw16"] > 0
-    log(f"stages (ms): sssp {avg['sssp_ms']:.2f} build {avg['build_ms']:.2f} "
+    bform = "boundary graph" if stages[-1]["boundary_graph"] else "boundary SSSP"
+    log(f"stages (ms): boundary rows ({bform}) {avg['sssp_ms']:.2f} build {avg['build_ms']:.2f} "
         f"fw {avg['floyd_ms']:.2f} merge {avg['minplus_ms']:.2f} decode {avg['decode_ms']:.2f}; "
         f"16-bit FW parts {stages[-1]['parts_fw16']}, redone {stages[-1]['parts_redone']}")
     # Floyd-Warshall stage: 2 ops (add, min) per step, sum over parts of s_p^3 steps,
@@ -392,8 +394,9 @@ def main():
                "ops": fw_ops, "ms": avg["floyd_ms"], "TOPs": fw_tops, "peak_TOPs": fw_peak,
                "frac": fw_tops / fw_peak}
 
-    # Roofline of the dominant kernel: the merge stage of the timed batched steps,
-    # whose stages run one after another, so the stage time is the kernels' own.
+    # Roofline of the dominant kernel: the merge stage of the timed batched steps.  Its
+    # launches (one per padded-K group, spread over three streams) are bracketed by HIP
+    # events on the library stream: start before the fork, end after the join.
     merge_ms = [st["minplus_ms"] for st in stages]
     merge_avg = float(np.mean(merge_ms))
 
@@ -408,15 +411,16 @@ def main():
         g.setMode(modes[args.mode])
         log(f"overlap form: {overlap_ms:.2f} ms/step")
     # Dominant kernel: the merge (merge_stream_kernel, one launch per padded-K group,
-    # timed together as the merge stage of the batched solves, whose stages run one
-    # after another on one stream).  Per solve it computes sum_p i_p * b_p * n min-plus
-    # steps and its algorithmic bytes are 4 * (|A| + |B| + |C|): the inner x boundary
-    # FW block, the boundary SSSP rows and the inner rows of the result, each once.
+    # timed together as the merge stage of the batched solves).  Per solve it computes
+    # sum_p i_p * b_p * n min-plus steps and its algorithmic bytes are
+    # 4 * (|A| + |B| + |C|): the inner x boundary FW block, the boundary rows and the
+    # inner rows of the result, each once.
     steps = float((inner * bdy).sum()) * n
     ops = 2.0 * steps
     bytes_alg = 4.0 * float((inner * bdy).sum() + bdy.sum() * n + inner.sum() * n)
     # Floors: VALU at 1.5 instructions per step (two v_add_u32 + one v_min3_i32 per
-    # two steps) against 39.3 T lane-instructions/s, and HBM at 8 TB/s.
+    # two steps; the packed 16-bit rows take half that) against 39.3 T
+    # lane-instructions/s, and HBM at 8 TB/s.
     t_valu = 1.5 * steps / (VALU_PEAK_TOPS * 1e12)
     t_hbm = bytes_alg / (HBM_PEAK_GBS * 1e9)
     hbm_ach = bytes_alg / (merge_avg * 1e-3) / 1e9
@@ -459,6 +463,7 @@ def main():
            "data": "synthetic",
            "config": {"workload": workload, "n": n, "m": g.num_edges, "K": g.K,
                       "boundary_total": int(bdy.sum()), "mode": args.mode,
+                      "boundary_rows": bform,
                       "result_bytes_in_hbm": int(n) * int(n) * 4},
            "stages_ms": avg, "floyd_warshall": fw_roof,
            "overlap_form": {"ms_per_step": overlap_ms},

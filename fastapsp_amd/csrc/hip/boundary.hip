@@ -34,7 +34,8 @@
 //                    in-part distances from their part's boundary in registers and
 //                    the wave's stores are coalesced, so the rows go to HBM once
 //                    (bg_rows_wide_kernel: parts with more than 32 boundary
-//                    vertices, walked in register chunks).
+//                    vertices, walked in register chunks).  Both also record the
+//                    largest finite distance written (the merge's 16-bit bound).
 #include <algorithm>
 
 #include "fap_internal.h"
@@ -266,6 +267,13 @@ __device__ __forceinline__ void load_fin(int (&fin)[KP], const ColInfo &ci, int 
     if (c0 + c >= ci.bdy) fin[c] = kInf;
 }
 
+// *max_b = max(*max_b, the wave's largest value of mx): one atomic per wave.
+__device__ __forceinline__ void publish_max(int *max_b, int mx) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(max_b, mx);
+}
+
 // Every part has at most KP boundary vertices.  Thread t handles columns
 // blockIdx.x * 1024 + e * 256 + t (e < 4): the lanes of a wave cover 64 consecutive
 // vertices, so they share one or a few parts (LDS broadcast reads) and every store
@@ -277,7 +285,7 @@ __global__ __launch_bounds__(256) void bg_rows_kernel(
     const BgPart *__restrict__ parts, const int64_t *__restrict__ orow, int nb_total,
     const int *__restrict__ mat, int Sb, int R, const int *__restrict__ pidx_by_pid,
     const int *__restrict__ graph_id, const int *__restrict__ ed2st, int n,
-    int *__restrict__ out, int64_t ld_out) {
+    int *__restrict__ out, int64_t ld_out, int *__restrict__ max_b) {
   extern __shared__ int lds[];
   __shared__ int64_t ooff[kRowsMax];
   const int ldl = Sb + KP;
@@ -292,6 +300,7 @@ __global__ __launch_bounds__(256) void bg_rows_kernel(
     bs[e] = ci.bs;
     load_fin<KP>(fin[e], ci, 0);
   }
+  int mx = 0;  // largest finite distance written by this thread
   for (int g0 = gbeg; g0 < gend; g0 += R) {
     const int rows = min(R, gend - g0);
     __syncthreads();  // the previous chunk's reads are done
@@ -307,10 +316,14 @@ __global__ __launch_bounds__(256) void bg_rows_kernel(
 #pragma unroll
         for (int c = 0; c < KP; c += 2) acc = min3_i32(acc, a[c] + fin[e][c], a[c + 1] + fin[e][c + 1]);
         const int j = jb + e * 256;
-        if (j < n) __builtin_nontemporal_store(acc, orow_p + j);
+        if (j < n) {
+          __builtin_nontemporal_store(acc, orow_p + j);
+          if (acc < kInf) mx = max(mx, acc);
+        }
       }
     }
   }
+  publish_max(max_b, mx);
 }
 
 // Parts with more than 32 boundary vertices: the boundary is walked in chunks of KC
@@ -324,7 +337,7 @@ __global__ __launch_bounds__(256) void bg_rows_wide_kernel(
     const BgPart *__restrict__ parts, const int64_t *__restrict__ orow, int nb_total,
     const int *__restrict__ mat, int Sb, int R, int nchunks, const int *__restrict__ pidx_by_pid,
     const int *__restrict__ graph_id, const int *__restrict__ ed2st, int n,
-    int *__restrict__ out, int64_t ld_out) {
+    int *__restrict__ out, int64_t ld_out, int *__restrict__ max_b) {
   extern __shared__ int lds[];
   __shared__ int64_t ooff[kRowsMax];
   const int ldl = Sb + KC;
@@ -335,6 +348,7 @@ __global__ __launch_bounds__(256) void bg_rows_wide_kernel(
 #pragma unroll
   for (int e = 0; e < 4; ++e)
     ci[e] = column_info(parts, pidx_by_pid, graph_id, ed2st, jb + e * 256, n);
+  int mx = 0;  // largest finite distance written by this thread
   for (int g0 = gbeg; g0 < gend; g0 += R) {
     const int rows = min(R, gend - g0);
     __syncthreads();
@@ -378,10 +392,14 @@ __global__ __launch_bounds__(256) void bg_rows_wide_kernel(
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int j = jb + e * 256;
-        if (j < n) __builtin_nontemporal_store(acc[r][e], out + ooff[r] + j);
+        if (j < n) {
+          __builtin_nontemporal_store(acc[r][e], out + ooff[r] + j);
+          if (acc[r][e] < kInf) mx = max(mx, acc[r][e]);
+        }
       }
     }
   }
+  publish_max(max_b, mx);
 }
 
 }  // namespace
@@ -427,7 +445,8 @@ int bg_rows_chunk(int ldl) { return std::max(1, std::min(kRowsMax, 65536 / (ldl 
 
 int launch_bg_rows(const BgPart *d_parts, const int64_t *d_orow, int nb_total, const int *d_mat,
                    int Sb, int max_bdy, const int *d_pidx_by_pid, const int *graph_id,
-                   const int *ed2st, int n, int *d_out, int64_t ld_out, hipStream_t stream) {
+                   const int *ed2st, int n, int *d_out, int64_t ld_out, int *d_max_b,
+                   hipStream_t stream) {
   if (nb_total <= 0 || n <= 0) return FAP_OK;
   const int cols = (n + kRowsCols - 1) / kRowsCols;
   // Row splits: about two workgroups per CU in all, each split at least one chunk.
@@ -442,7 +461,8 @@ int launch_bg_rows(const BgPart *d_parts, const int64_t *d_orow, int nb_total, c
 #define FAP_BG_CASE(KPV)                                                                      \
   case KPV:                                                                                   \
     hipLaunchKernelGGL((bg_rows_kernel<KPV>), grid, dim3(256), lds, stream, d_parts, d_orow,   \
-                       nb_total, d_mat, Sb, R, d_pidx_by_pid, graph_id, ed2st, n, d_out, ld_out); \
+                       nb_total, d_mat, Sb, R, d_pidx_by_pid, graph_id, ed2st, n, d_out, ld_out,   \
+                       d_max_b);                                                               \
     break;
   if (narrow) {
     switch (KP) {
@@ -458,7 +478,7 @@ int launch_bg_rows(const BgPart *d_parts, const int64_t *d_orow, int nb_total, c
     const int nchunks = (max_bdy + KC - 1) / KC;
     hipLaunchKernelGGL((bg_rows_wide_kernel<16>), grid, dim3(256), lds, stream, d_parts, d_orow,
                        nb_total, d_mat, Sb, R, nchunks, d_pidx_by_pid, graph_id, ed2st, n, d_out,
-                       ld_out);
+                       ld_out, d_max_b);
   }
 #undef FAP_BG_CASE
   FAP_CHECK_LAUNCH();

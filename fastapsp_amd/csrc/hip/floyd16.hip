@@ -24,6 +24,7 @@
 #include "fap_internal.h"
 #include "fw_desc.h"
 #include "gmem.h"
+#include "pk16.h"
 
 namespace fap_hip {
 
@@ -33,33 +34,11 @@ constexpr int kT = 64;         // tile edge (entries)
 constexpr int kTW = kT / 2;    // 32-bit words per tile row
 
 typedef int v4i __attribute__((ext_vector_type(4)));
-typedef unsigned short us2 __attribute__((ext_vector_type(2)));
 typedef FAP_GLOBAL uint16_t g16;
 typedef FAP_GLOBAL v4i gv4i;
 
 __device__ __forceinline__ g16 *tile16(const Fw16Desc &d, int ti, int tj) {
   return gptr(d.mat + (int64_t)(ti * kT) * d.ld + tj * kT);
-}
-
-// Pairwise sums with the low (resp. high) half of a broadcast to both lanes.
-__device__ __forceinline__ unsigned pk_add_lo(unsigned a, unsigned b) {
-  const us2 av = __builtin_bit_cast(us2, a);
-  return __builtin_bit_cast(unsigned, (us2)(av.xx + __builtin_bit_cast(us2, b)));
-}
-__device__ __forceinline__ unsigned pk_add_hi(unsigned a, unsigned b) {
-  const us2 av = __builtin_bit_cast(us2, a);
-  return __builtin_bit_cast(unsigned, (us2)(av.yy + __builtin_bit_cast(us2, b)));
-}
-
-// Per 16-bit half: min(acc, x, y) on values in [0, 0x7C00] as half-precision
-// minimum (IEEE 754-2019 minimum: exact, associative, no rounding); the compiler
-// folds the two into one v_pk_minimum3_f16.
-typedef _Float16 h2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ unsigned pk_min3(unsigned acc, unsigned x, unsigned y) {
-  const h2 m = __builtin_elementwise_minimum(
-      __builtin_elementwise_minimum(__builtin_bit_cast(h2, acc), __builtin_bit_cast(h2, x)),
-      __builtin_bit_cast(h2, y));
-  return __builtin_bit_cast(unsigned, m);
 }
 
 // Phase 1: the 64 x 64 diagonal tile closed by one wave: lane i holds row i as 32

@@ -26,6 +26,7 @@
 #include "fw_desc.h"
 #include "gmem.h"
 #include "mp_tile.h"
+#include "pk16.h"
 
 namespace fap_hip {
 
@@ -182,23 +183,28 @@ __global__ __launch_bounds__(256) void merge_kernel(const MergeDesc *__restrict_
 // K <= KP and every A row holds KP readable entries (rows of the padded
 // Floyd-Warshall matrices, stride >= 64).  Entries A[i][k] with K <= k < KP are
 // distances in [0, FAP_INF] and meet B = FAP_INF, so they never win a minimum.
-constexpr int kStRows = 128;
 constexpr int kStCols = 256;
 
-// One row per loop iteration, so a wave keeps few row stores in flight.  HBM write
-// bandwidth falls with the number of distinct DRAM rows written at once (waves per
-// CU x stores in flight per wave): on MI355X the merge stage of a 52.5 GB result
-// takes 11.8 ms with 8 rows in flight per wave and 9.6 ms with one (128-row units
-// in both; 32-, 64- and 256-row units measured slower).
+// Work-unit shape of the streaming merge: rows per unit and unit order inside a
+// product (0: the row blocks of one column block back to back, 1: the column blocks
+// of one row block back to back).  Measured in one process on MI355X (the 52.5 GB
+// result of the 114,599-vertex graph, K-groups on three streams): 128 rows in
+// column-block order 10.7 ms; 64 rows 10.9, 256 rows 10.8; 128 rows in row-block
+// order 11.1.
+struct StreamShape {
+  int rows, col_major;
+};
+StreamShape stream_shape() { return StreamShape{128, 1}; }
 
+// One row per loop iteration, so a wave keeps few row stores in flight: on MI355X
+// the merge stage of a 52.5 GB result took 11.8 ms with 8 rows in flight per wave
+// and 9.6 ms with one.
+
+// The A values come from scalar loads (SGPRs), so they never contend with the VGPRs
+// of stores still in flight.
 template <int KP>
-__device__ __forceinline__ void stream_row(int (&acc)[4], const FAP_CONST int *arow,
+__device__ __forceinline__ void stream_row(int (&acc)[4], const int (&a)[KP],
                                            const int (&b)[KP][4]) {
-  // Scalar loads: the A values land in SGPRs, so they never contend with the
-  // VGPRs of stores still in flight.
-  int a[KP];
-#pragma unroll
-  for (int k = 0; k < KP; ++k) a[k] = arow[k];
 #pragma unroll
   for (int e = 0; e < 4; ++e) acc[e] = kInf;
 #pragma unroll
@@ -207,11 +213,41 @@ __device__ __forceinline__ void stream_row(int (&acc)[4], const FAP_CONST int *a
     for (int e = 0; e < 4; ++e) acc[e] = min3_i32(acc[e], a[k] + b[k][e], a[k + 1] + b[k + 1][e]);
 }
 
+// The same row on packed 16-bit distances (pk16.h): the B panel holds column pairs
+// min(B, kCap16), each A value is capped and broadcast to both halves, so a step on
+// two entries is one v_pk_add_u16 and half a v_pk_minimum3_f16 (0.75 VALU
+// instructions per step and entry against 1.5).  Used only when every finite sum
+// A + B lies below kCap16 (the caller's bound), so a result at the cap means
+// unreachable and becomes FAP_INF.
+template <int KP>
+__device__ __forceinline__ void stream_row16(int (&acc)[4], const int (&ai)[KP],
+                                             const unsigned (&bp)[KP][2]) {
+  unsigned a[KP];
+#pragma unroll
+  for (int k = 0; k < KP; ++k) a[k] = min((unsigned)ai[k], (unsigned)kCap16);
+  unsigned s0[KP], s1[KP];  // all sums first: a minimum right after its sum waits a cycle
+#pragma unroll
+  for (int k = 0; k < KP; ++k) {
+    s0[k] = pk_add_lo(a[k], bp[k][0]);
+    s1[k] = pk_add_lo(a[k], bp[k][1]);
+  }
+  unsigned m0 = kPk16Top, m1 = kPk16Top;
+#pragma unroll
+  for (int k = 0; k < KP; k += 2) {
+    m0 = pk_min3(m0, s0[k], s0[k + 1]);
+    m1 = pk_min3(m1, s1[k], s1[k + 1]);
+  }
+  const unsigned v[4] = {m0 & 0xFFFFu, m0 >> 16, m1 & 0xFFFFu, m1 >> 16};
+#pragma unroll
+  for (int e = 0; e < 4; ++e) acc[e] = v[e] >= (unsigned)kCap16 ? kInf : (int)v[e];
+}
+
 template <bool kVec>
 __device__ __forceinline__ void stream_store(FAP_GLOBAL int *crow, int j, int N,
                                              const int (&acc)[4]) {
   if (kVec && j + 3 < N) {
     v4i v = {acc[0], acc[1], acc[2], acc[3]};
+    // Non-temporal: on MI355X the merge measured 1 ms slower with plain stores.
     __builtin_nontemporal_store(v, reinterpret_cast<FAP_GLOBAL v4i *>(crow + j));
   } else {
 #pragma unroll
@@ -231,32 +267,79 @@ __device__ __forceinline__ void stream_own(int (&acc)[4], const MergeDesc &d, in
   }
 }
 
+// A rows are prefetched one row ahead (scalar loads of row i + 1 in flight while
+// row i is computed and stored); the prefetch index is clamped to the last row, so it
+// never reads past the product.
+template <int KP>
+__device__ __forceinline__ void load_arow(int (&a)[KP], const FAP_CONST int *arow) {
+#pragma unroll
+  for (int k = 0; k < KP; ++k) a[k] = arow[k];
+}
+
 template <int KP, bool kVec, bool kOwn>
 __device__ __forceinline__ void stream_rows(const MergeDesc &d, int row0, int row1, int j, int N,
                                             const int (&b)[KP][4], const int (&own)[4]) {
   FAP_GLOBAL int *crow = gptr(d.C + (int64_t)row0 * d.ldc);
-  const FAP_CONST int *arow = uniform_cptr(d.A + (int64_t)row0 * d.lda);
-  for (int i = row0; i < row1; ++i, crow += d.ldc, arow += d.lda) {
+  const FAP_CONST int *abase = uniform_cptr(d.A);
+  int an[KP];
+  load_arow<KP>(an, abase + (int64_t)row0 * d.lda);
+  for (int i = row0; i < row1; ++i, crow += d.ldc) {
+    int a[KP];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) a[k] = an[k];
+    load_arow<KP>(an, abase + (int64_t)min(i + 1, row1 - 1) * d.lda);
     int acc[4];
-    stream_row<KP>(acc, arow, b);
+    stream_row<KP>(acc, a, b);
     stream_own<KP, kOwn>(acc, d, i, own);
     stream_store<kVec>(crow, j, N, acc);
   }
 }
 
+template <int KP, bool kVec, bool kOwn>
+__device__ __forceinline__ void stream_rows16(const MergeDesc &d, int row0, int row1, int j,
+                                              int N, const unsigned (&bp)[KP][2],
+                                              const int (&own)[4]) {
+  FAP_GLOBAL int *crow = gptr(d.C + (int64_t)row0 * d.ldc);
+  const FAP_CONST int *abase = uniform_cptr(d.A);
+  int an[KP];
+  load_arow<KP>(an, abase + (int64_t)row0 * d.lda);
+  for (int i = row0; i < row1; ++i, crow += d.ldc) {
+    int a[KP];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) a[k] = an[k];
+    load_arow<KP>(an, abase + (int64_t)min(i + 1, row1 - 1) * d.lda);
+    int acc[4];
+    stream_row16<KP>(acc, a, bp);
+    stream_own<KP, kOwn>(acc, d, i, own);
+    stream_store<kVec>(crow, j, N, acc);
+  }
+}
+
+// max_b / lim16: when max_b is set and *max_b <= lim16 (every finite B below
+// lim16 + 1, every finite A at most kCap16 - 1 - lim16, both guaranteed by the
+// caller), the rows run on packed 16-bit distances.
 template <int KP, bool kVec>
 __global__ __launch_bounds__(64) void merge_stream_kernel(const MergeDesc *__restrict__ descs,
                                                           int nd, int total_units, int N,
                                                           const int *__restrict__ graph_id,
-                                                          const int *__restrict__ ed2st) {
+                                                          const int *__restrict__ ed2st,
+                                                          const int *__restrict__ max_b,
+                                                          int lim16, int rpu, int col_major) {
   const int t = xcd_logical_tile();
   if (t >= total_units) return;
   const MergeDesc d = descs[find_desc(descs, nd, t)];
   const int local = t - d.tile_begin;
-  const int cb = local / d.row_blocks;
-  const int rb = local - cb * d.row_blocks;
-  const int row0 = rb * kStRows;
-  const int row1 = min(row0 + kStRows, d.M);
+  int cb, rb;
+  if (col_major) {
+    const int ncb = (N + kStCols - 1) / kStCols;
+    rb = local / ncb;
+    cb = local - rb * ncb;
+  } else {
+    cb = local / d.row_blocks;
+    rb = local - cb * d.row_blocks;
+  }
+  const int row0 = rb * rpu;
+  const int row1 = min(row0 + rpu, d.M);
   const int j = cb * kStCols + threadIdx.x * 4;
 
   int b[KP][4];
@@ -287,34 +370,57 @@ __global__ __launch_bounds__(64) void merge_stream_kernel(const MergeDesc *__res
       any_own = true;
     }
   }
+  const bool packed = max_b != nullptr && *max_b <= lim16;  // uniform
   // Retire the B-panel and column loads here, so the row loop never waits on them.
   __builtin_amdgcn_s_waitcnt(0);
-  if (__builtin_amdgcn_ballot_w64(any_own) == 0)
+  const bool none_own = __builtin_amdgcn_ballot_w64(any_own) == 0;
+  if (packed) {
+    unsigned bp[KP][2];
+#pragma unroll
+    for (int k = 0; k < KP; ++k)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        bp[k][h] = (unsigned)min(b[k][2 * h], kCap16) |
+                   ((unsigned)min(b[k][2 * h + 1], kCap16) << 16);
+    if (none_own)
+      stream_rows16<KP, kVec, false>(d, row0, row1, j, N, bp, own);
+    else
+      stream_rows16<KP, kVec, true>(d, row0, row1, j, N, bp, own);
+  } else if (none_own) {
     stream_rows<KP, kVec, false>(d, row0, row1, j, N, b, own);
-  else
+  } else {
     stream_rows<KP, kVec, true>(d, row0, row1, j, N, b, own);
+  }
 }
 
 int merge_stream_units(int M, int N) {
   if (M <= 0 || N <= 0) return 0;
-  return ((M + kStRows - 1) / kStRows) * ((N + kStCols - 1) / kStCols);
+  const int r = stream_shape().rows;
+  return ((M + r - 1) / r) * ((N + kStCols - 1) / kStCols);
 }
-int merge_stream_row_blocks(int M) { return (M + kStRows - 1) / kStRows; }
+int merge_stream_row_blocks(int M) {
+  const int r = stream_shape().rows;
+  return (M + r - 1) / r;
+}
 int merge_stream_kpad(int K) { return K <= 0 ? 4 : (K + 3) / 4 * 4; }
 
 // kp: padded K, a multiple of 4 in [4, 32].
 int launch_merge_stream(const MergeDesc *d_descs, int nd, int total_units, int N, int kp,
-                        bool vec, const int *d_graph_id, const int *d_ed2st, hipStream_t stream) {
+                        bool vec, const int *d_graph_id, const int *d_ed2st, const int *d_max_b,
+                        int lim16, hipStream_t stream) {
   if (nd <= 0 || total_units <= 0 || N <= 0) return FAP_OK;
   const int grid = (total_units + 7) / 8 * 8;
+  const StreamShape sh = stream_shape();
 #define FAP_STREAM_CASE(KP)                                                                   \
   case KP:                                                                                    \
     if (vec)                                                                                  \
       hipLaunchKernelGGL((merge_stream_kernel<KP, true>), dim3(grid), dim3(64), 0, stream,    \
-                         d_descs, nd, total_units, N, d_graph_id, d_ed2st);                   \
+                         d_descs, nd, total_units, N, d_graph_id, d_ed2st, d_max_b, lim16,    \
+                         sh.rows, sh.col_major);                                               \
     else                                                                                      \
       hipLaunchKernelGGL((merge_stream_kernel<KP, false>), dim3(grid), dim3(64), 0, stream,   \
-                         d_descs, nd, total_units, N, d_graph_id, d_ed2st);                   \
+                         d_descs, nd, total_units, N, d_graph_id, d_ed2st, d_max_b, lim16,    \
+                         sh.rows, sh.col_major);                                               \
     break;
   switch (kp) {
     FAP_STREAM_CASE(4)

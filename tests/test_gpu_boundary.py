@@ -133,3 +133,32 @@ def test_boundary_graph_solve_subgraph_unchanged():
         g.solveSubGraph(p)
         off, num = lay["offset"][p], lay["num"][p]
         assert np.array_equal(g.getSubGraphDistanceI32(p), full[off:off + num])
+
+
+def _path(n):
+    out = [[] for _ in range(n)]
+    for i in range(n - 1):
+        out[i].append(i + 1)
+        out[i + 1].append(i)
+    ro = np.zeros(n + 1, np.int32)
+    ro[1:] = np.cumsum([len(x) for x in out])
+    return ro, np.array([v for x in out for v in x], np.int32)
+
+
+@pytest.mark.parametrize("n,K", [(9000, 3),     # distances < 15872 - s_max: packed merge
+                                 (16000, 4)])   # boundary rows reach 15999: int32 merge
+def test_boundary_graph_merge_16bit_bound(n, K):
+    # Path graph: d(i, j) = |i - j|.  The packed 16-bit merge applies only while every
+    # finite boundary-row entry stays below 15872 - s_max; both sides of that bound
+    # must give the closed form.
+    say(f"boundary graph: path n={n} K={K}")
+    ro, col = _path(n)
+    g = F.FapGraph.from_csr(ro, col, None, False, K)
+    g.preCondition()
+    a, info = _solve(g, F.FAP_MODE_BATCHED)
+    assert info["boundary_graph"] == 1
+    st2ed = g.getMapping().astype(np.int64)
+    want = np.abs(st2ed[:, None] - np.arange(n)[None, :]).astype(np.int32)
+    assert np.array_equal(a, want)
+    b, _ = _solve(g, F.FAP_MODE_BATCHED | SSSP)
+    assert np.array_equal(a, b)
