@@ -98,6 +98,8 @@ SIGNATURES = {
     "fap_hip_solve_parts": (C.c_int, [C.POINTER(FapGraphDev), C.POINTER(FapPartDesc), _i32,
                                       _vp, _i64, _i32, _vp]),
     "fap_hip_last_stage_times": (C.c_int, [C.POINTER(FapStageTimes)]),
+    "fap_hip_solve_parts_path": (C.c_int, [C.POINTER(FapGraphDev), _vp, _vp, _vp,
+                                           C.POINTER(FapPartDesc), _i32, _vp, _vp, _i64, _vp]),
     "fap_hip_solve_parts_f32": (C.c_int, [C.POINTER(FapGraphDev), C.POINTER(FapPartDesc), _i32,
                                           _vp, _i64, _i32, _vp]),
     "fap_hip_batch_create": (C.c_int, [C.POINTER(_vp), C.POINTER(FapBatchItem), _i32, _i32]),
@@ -132,6 +134,10 @@ SIGNATURES = {
     "fap_graph_set_mode": (C.c_int, [_vp, _i32]),
     "fap_graph_solve": (C.c_int, [_vp]),
     "fap_graph_solve_subgraph": (C.c_int, [_vp, _i32]),
+    "fap_graph_solve_ex": (C.c_int, [_vp, _i32]),
+    "fap_graph_solve_subgraph_ex": (C.c_int, [_vp, _i32, _i32]),
+    "fap_graph_get_path_rows": (C.c_int, [_vp, _i64, _i64, _pi32]),
+    "fap_graph_get_subgraph_path_rows": (C.c_int, [_vp, _i32, _i64, _i64, _pi32]),
     "fap_graph_sizes": (C.c_int, [_vp, _pi32, _pi64, _pi32]),
     "fap_graph_get_csr": (C.c_int, [_vp, _pi32, _pi32, _pi32]),
     "fap_graph_get_layout": (C.c_int, [_vp, _pi32, _pi32, _pi32, _pi32, _pi32, _pi32]),

@@ -179,6 +179,27 @@ int fap_graph_solve_subgraph(fap_graph *h, int32_t id) {
   return guarded([&] { h->g->solveSubGraph(id); });
 }
 
+int fap_graph_solve_ex(fap_graph *h, int32_t is_path_needed) {
+  FAP_HANDLE(h);
+  return guarded([&] { h->g->solve(is_path_needed != 0); });
+}
+
+int fap_graph_solve_subgraph_ex(fap_graph *h, int32_t id, int32_t is_path_needed) {
+  FAP_HANDLE(h);
+  return guarded([&] { h->g->solveSubGraph(id, is_path_needed != 0); });
+}
+
+int fap_graph_get_path_rows(fap_graph *h, int64_t row0, int64_t rows, int32_t *h_out) {
+  FAP_HANDLE(h);
+  return guarded([&] { h->g->getPathRowsI32(row0, rows, h_out); });
+}
+
+int fap_graph_get_subgraph_path_rows(fap_graph *h, int32_t id, int64_t row0, int64_t rows,
+                                     int32_t *h_out) {
+  FAP_HANDLE(h);
+  return guarded([&] { h->g->getSubGraphPathRowsI32(id, row0, rows, h_out); });
+}
+
 int fap_graph_sizes(fap_graph *h, int32_t *n, int64_t *m, int32_t *K) {
   FAP_HANDLE(h);
   if (n) *n = h->g->num_vertexs;

@@ -114,14 +114,16 @@ fapGraph::fapGraph(int32_t n, int64_t m, std::vector<int32_t> ro, std::vector<in
 fapGraph::~fapGraph() { release(); }
 
 void fapGraph::release() {
-  int32_t *ptrs[] = {d_row_offset, d_col, d_w,  d_graph_id, d_st2ed,
-                     d_ed2st,      d_comp, d_dist, d_sub_dist};
+  int32_t *ptrs[] = {d_row_offset, d_col,  d_w,    d_graph_id, d_st2ed, d_ed2st, d_comp,
+                     d_dist,       d_sub_dist, d_path, d_sub_path, d_rro,  d_rcol,  d_rw};
   for (int32_t *p : ptrs)
     if (p) fap_hip_free(p);
   if (d_wf) fap_hip_free(d_wf);
   d_wf = nullptr;
   d_row_offset = d_col = d_w = d_graph_id = d_st2ed = d_ed2st = d_comp = d_dist = d_sub_dist =
       nullptr;
+  d_path = d_sub_path = d_rro = d_rcol = d_rw = nullptr;
+  have_path = have_sub_path = false;
 }
 
 void fapGraph::setPartition(const std::vector<int> &ids) {
@@ -235,9 +237,46 @@ void fapGraph::add_stage_times() {
   time_data_transformation += (t.build_ms + t.decode_ms) * 1e-3;
 }
 
+// Reverse CSR (in-arcs of every vertex, with weights) for the predecessor rule of
+// the boundary rows.
+void fapGraph::upload_reverse() {
+  if (d_rro) return;
+  const int32_t n = num_vertexs;
+  std::vector<int32_t> ro(n + 1, 0), col(std::max(num_edges, 1)), w(std::max(num_edges, 1));
+  for (int32_t e = 0; e < num_edges; ++e) ++ro[col_val[e] + 1];
+  for (int32_t v = 0; v < n; ++v) ro[v + 1] += ro[v];
+  std::vector<int32_t> fill(ro.begin(), ro.end() - 1);
+  for (int32_t u = 0; u < n; ++u)
+    for (int32_t e = row_offset[u]; e < row_offset[u + 1]; ++e) {
+      const int32_t at = fill[col_val[e]]++;
+      col[at] = u;
+      w[at] = (int32_t)weight[e];
+    }
+  auto put = [&](int32_t **d, const void *h, int64_t count) {
+    check(fap_hip_malloc((void **)d, std::max<int64_t>(count, 1) * 4), "device allocation");
+    if (count > 0) check(fap_hip_memcpy_h2d(*d, h, count * 4, nullptr), "host to device copy");
+  };
+  put(&d_rro, ro.data(), n + 1);
+  put(&d_rcol, col.data(), num_edges);
+  if (d_w) put(&d_rw, w.data(), num_edges);
+}
+
+void fapGraph::run_path(const fap_part_desc *parts, int32_t nparts, int32_t *d_out,
+                        int32_t *d_p) {
+  require_int("predecessor matrices");
+  upload_reverse();
+  fap_graph_dev g{num_vertexs, num_edges, d_row_offset, d_col, d_w,
+                  d_graph_id,  d_st2ed,   d_ed2st,      d_comp, d_wf};
+  check(fap_hip_solve_parts_path(&g, d_rro, d_rcol, d_rw, parts, nparts, d_out, d_p, ld_dist,
+                                 nullptr),
+        "solve with paths");
+  check(fap_hip_stream_sync(nullptr), "solve with paths");
+  add_stage_times();
+}
+
 int32_t fapGraph::solve(bool path_needed) {
   if (!preconditioned) throw FapError(FAP_ERR_STATE, "solve() before preCondition()");
-  if (path_needed) throw FapError(FAP_ERR_STATE, "predecessor matrices are not produced");
+  if (path_needed) require_int("predecessor matrices");
   if (!d_row_offset) upload();
   if (!d_dist)
     check(fap_hip_malloc((void **)&d_dist, (int64_t)num_vertexs * ld_dist * 4),
@@ -246,6 +285,15 @@ int32_t fapGraph::solve(bool path_needed) {
   for (int p = 1; p <= K; ++p)
     parts.push_back(fap_part_desc{p, C_BlockVer_offset[p], C_BlockVer_num[p], C_BlockBdy_num[p],
                                   (int64_t)C_BlockVer_offset[p]});
+  have_path = false;
+  if (path_needed) {
+    if (!d_path)
+      check(fap_hip_malloc((void **)&d_path, (int64_t)num_vertexs * ld_dist * 4),
+            "n x n path allocation");
+    run_path(parts.data(), (int32_t)parts.size(), d_dist, d_path);
+    have_path = true;
+    return 0;
+  }
   fap_graph_dev g{num_vertexs, num_edges, d_row_offset, d_col, d_w,
                   d_graph_id,  d_st2ed,   d_ed2st,      d_comp, d_wf};
   if (float_weights)
@@ -263,13 +311,23 @@ int32_t fapGraph::solve(bool path_needed) {
 
 int32_t fapGraph::solveSubGraph(int32_t id, bool path_needed) {
   if (!preconditioned) throw FapError(FAP_ERR_STATE, "solveSubGraph() before preCondition()");
-  if (path_needed) throw FapError(FAP_ERR_STATE, "predecessor matrices are not produced");
+  if (path_needed) require_int("predecessor matrices");
   if (id < 1 || id > K) throw FapError(FAP_ERR_INVALID, "sub graph id out of range");
   if (!d_row_offset) upload();
   const int64_t rows = std::max(max_subgraph_vertexs, 1);
   if (!d_sub_dist)
     check(fap_hip_malloc((void **)&d_sub_dist, rows * ld_dist * 4), "sub-graph allocation");
   fap_part_desc part{id, C_BlockVer_offset[id], C_BlockVer_num[id], C_BlockBdy_num[id], 0};
+  have_sub_path = false;
+  if (path_needed) {
+    if (!d_sub_path)
+      check(fap_hip_malloc((void **)&d_sub_path, rows * ld_dist * 4), "sub-graph path allocation");
+    run_path(&part, 1, d_sub_dist, d_sub_path);
+    have_sub_path = true;
+    current_subgraph_id = id;
+    sub_rows = C_BlockVer_num[id];
+    return 0;
+  }
   fap_graph_dev g{num_vertexs, num_edges, d_row_offset, d_col, d_w,
                   d_graph_id,  d_st2ed,   d_ed2st,      d_comp, d_wf};
   if (float_weights)
@@ -346,8 +404,15 @@ std::vector<float> fapGraph::getAllDistance() {
   return out;
 }
 
+void fapGraph::getPathRowsI32(int64_t row0, int64_t rows, int32_t *out) {
+  if (!have_path) throw FapError(FAP_ERR_STATE, "no solve(true) result");
+  copy_rows(d_path, num_vertexs, row0, rows, out);
+}
+
 std::vector<int32_t> fapGraph::getAllPath() {
-  throw FapError(FAP_ERR_STATE, "predecessor matrices are not produced");
+  std::vector<int32_t> out((size_t)num_vertexs * num_vertexs);
+  getPathRowsI32(0, num_vertexs, out.data());
+  return out;
 }
 
 int32_t fapGraph::getCurrentSubGraphId() { return current_subgraph_id; }
@@ -401,8 +466,16 @@ std::vector<float> fapGraph::getSubGraphDistance(int32_t id) {
   return out;
 }
 
-std::vector<int32_t> fapGraph::getSubGraphPath(int32_t) {
-  throw FapError(FAP_ERR_STATE, "predecessor matrices are not produced");
+void fapGraph::getSubGraphPathRowsI32(int32_t id, int64_t row0, int64_t rows, int32_t *out) {
+  if (id != current_subgraph_id || !have_sub_path)
+    throw FapError(FAP_ERR_STATE, "sub graph was not the last one solved with paths");
+  copy_rows(d_sub_path, sub_rows, row0, rows, out);
+}
+
+std::vector<int32_t> fapGraph::getSubGraphPath(int32_t id) {
+  std::vector<int32_t> out((size_t)(std::max<int64_t>(sub_rows, 0) * num_vertexs));
+  getSubGraphPathRowsI32(id, 0, sub_rows, out.data());
+  return out;
 }
 
 std::vector<float> fapGraph::getDistanceFromOnePoint(int32_t v) {
@@ -423,8 +496,14 @@ std::vector<float> fapGraph::getDistanceFromOnePoint(int32_t v) {
   return out;
 }
 
-std::vector<int32_t> fapGraph::getPathFromOnePoint(int32_t) {
-  throw FapError(FAP_ERR_STATE, "predecessor matrices are not produced");
+std::vector<int32_t> fapGraph::getPathFromOnePoint(int32_t v) {
+  if (v < 0 || v >= num_vertexs) throw FapError(FAP_ERR_INVALID, "vertex out of range");
+  if (graph_id[v] != current_subgraph_id)
+    throw FapError(FAP_ERR_STATE, "vertex is not in the last solved sub graph");
+  std::vector<int32_t> out(num_vertexs);
+  getSubGraphPathRowsI32(current_subgraph_id, ed2st[v] - C_BlockVer_offset[graph_id[v]], 1,
+                         out.data());
+  return out;
 }
 
 fap_graph_dev fapGraph::deviceView() {

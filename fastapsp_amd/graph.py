@@ -111,17 +111,35 @@ class FapGraph:
         L.check(L.load().fap_graph_set_mode(self._h, int(mode)), "setMode")
 
     def solve(self, is_path_needed: bool = False) -> int:
-        if is_path_needed:
-            raise L.FapError(L.FAP_ERR_STATE, "predecessor matrices are not produced")
-        L.check(L.load().fap_graph_solve(self._h), "solve")
+        """fap/fap.h:127.  With is_path_needed the predecessor matrix is produced too
+        (integer-weight graphs; the reference CPU path's tie-breaking)."""
+        L.check(L.load().fap_graph_solve_ex(self._h, int(bool(is_path_needed))), "solve")
         return 0
 
     def solveSubGraph(self, sub_graph_id: int, is_path_needed: bool = False) -> int:
-        if is_path_needed:
-            raise L.FapError(L.FAP_ERR_STATE, "predecessor matrices are not produced")
-        L.check(L.load().fap_graph_solve_subgraph(self._h, int(sub_graph_id)), "solveSubGraph")
+        L.check(L.load().fap_graph_solve_subgraph_ex(self._h, int(sub_graph_id),
+                                                      int(bool(is_path_needed))),
+                "solveSubGraph")
         self._current = int(sub_graph_id)
         return 0
+
+    def getPathRows(self, row0: int, rows: int) -> np.ndarray:
+        """Rows [row0, row0 + rows) of the predecessors of solve(True): path[r][v] is the
+        vertex before v on the shortest path from st2ed[r], -1 if unreachable."""
+        out = np.empty((rows, self.num_vertexs), dtype=np.int32)
+        L.check(L.load().fap_graph_get_path_rows(self._h, int(row0), int(rows), L.ptr_i32(out)),
+                "getPathRows")
+        return out
+
+    def getAllPath(self) -> np.ndarray:
+        return self.getPathRows(0, self.num_vertexs)
+
+    def getSubGraphPath(self, sub_graph_id: int) -> np.ndarray:
+        num = int(self.layout()["num"][sub_graph_id])
+        out = np.empty((num, self.num_vertexs), dtype=np.int32)
+        L.check(L.load().fap_graph_get_subgraph_path_rows(self._h, int(sub_graph_id), 0, num,
+                                                          L.ptr_i32(out)), "getSubGraphPath")
+        return out
 
     def getGraphId(self) -> np.ndarray:
         gid = np.empty(self.num_vertexs, dtype=np.int32)

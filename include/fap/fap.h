@@ -14,8 +14,9 @@
 //    skip, fap/fap.cpp:522-528);
 //  * getDistanceFromOnePoint uses the sorted position of the vertex (the
 //    reference indexes C_BlockVer_offset by vertex id, fap/fap.cpp:584-585).
-// Predecessor (path) matrices are not produced: getAllPath / getSubGraphPath /
-// getPathFromOnePoint throw FapError(FAP_ERR_STATE).
+// Predecessor (path) matrices: solve(true) / solveSubGraph(id, true) produce them
+// for integer-weight graphs, equal to the reference CPU path's bit for bit (same
+// tie-breaking); float-weight graphs throw FapError(FAP_ERR_STATE) for them.
 #pragma once
 
 #include <chrono>
@@ -80,6 +81,10 @@ class fapGraph {
   bool float_weights = false;     // some weight is not an integer below 2^24
   int32_t *d_dist = nullptr;      // n x ld_dist, result of solve()
   int32_t *d_sub_dist = nullptr;  // s_p x ld_dist, result of solveSubGraph()
+  int32_t *d_path = nullptr;      // n x ld_dist predecessors of solve(true)
+  int32_t *d_sub_path = nullptr;  // s_p x ld_dist predecessors of solveSubGraph(id, true)
+  int32_t *d_rro = nullptr, *d_rcol = nullptr, *d_rw = nullptr;  // reverse CSR (paths)
+  bool have_path = false, have_sub_path = false;
   int64_t ld_dist = 0;
   int64_t sub_rows = 0;
 
@@ -89,6 +94,8 @@ class fapGraph {
   void add_stage_times();
   void copy_rows(const int32_t *base, int64_t nrows, int64_t row0, int64_t rows, void *out) const;
   void require_int(const char *what) const;
+  void upload_reverse();
+  void run_path(const fap_part_desc *parts, int32_t nparts, int32_t *d_out, int32_t *d_p);
 
  public:
   int32_t num_vertexs = 0;
@@ -141,6 +148,10 @@ class fapGraph {
   std::vector<int32_t> getSubGraphPath(int32_t sub_graph_id);
   std::vector<float> getDistanceFromOnePoint(int32_t vertex_id);
   std::vector<int32_t> getPathFromOnePoint(int32_t vertex_id);
+  // Rows [row0, row0 + rows) of the predecessors of solve(true) / of the last
+  // solveSubGraph(id, true) block into out (n entries per row).
+  void getPathRowsI32(int64_t row0, int64_t rows, int32_t *out);
+  void getSubGraphPathRowsI32(int32_t sub_graph_id, int64_t row0, int64_t rows, int32_t *out);
   // Rows [row0, row0 + rows) of the solve() result, without a full copy: int32
   // (integer-weight graphs only) or float32 with MAXVALUE for unreachable (any graph).
   void getDistanceRowsI32(int64_t row0, int64_t rows, int32_t *out);

@@ -212,6 +212,20 @@ int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *parts, int3
                         int32_t *d_out, int64_t ld_out, int32_t mode, void *stream);
 int fap_hip_last_stage_times(fap_stage_times *t);
 
+/* The same pipeline with predecessor matrices, as fapGraph::run with is_path_needed
+ * (fap/fap.cpp:195-503) computes them on its CPU path: d_path[row][v] = the vertex
+ * before v on the chosen shortest path from the row's source, -1 if unreachable, the
+ * source itself on the diagonal, with the reference's tie-breaking at every stage
+ * (Dijkstra's heap order for the boundary rows, k-i-j order with strict '>' for
+ * Floyd-Warshall, first k for the min-plus).  Integer weights only (g->d_wf NULL).
+ * d_rrow_offset / d_rcol / d_rw: the reverse CSR (in-arcs of every vertex; d_rw NULL
+ * exactly when g->d_w is NULL).  d_path has the layout of d_out.  One batched form;
+ * asynchronous on `stream`.  Stage times as for fap_hip_solve_parts. */
+int fap_hip_solve_parts_path(const fap_graph_dev *g, const int32_t *d_rrow_offset,
+                             const int32_t *d_rcol, const int32_t *d_rw,
+                             const fap_part_desc *parts, int32_t nparts, int32_t *d_out,
+                             int32_t *d_path, int64_t ld_out, void *stream);
+
 /* The same pipeline for float32 weights (g->d_wf), the reference's float path:
  * distances are carried in FP64 (exact sums of float32 weights while a path sum
  * spans fewer than 53 significant bits) and written as float32, with MAXVALUE = 1e8
@@ -296,6 +310,13 @@ int fap_graph_is_split(fap_graph *g, int *is_split);
 int fap_graph_set_mode(fap_graph *g, int32_t mode);
 int fap_graph_solve(fap_graph *g);
 int fap_graph_solve_subgraph(fap_graph *g, int32_t id);
+/* solve(is_path_needed) / solveSubGraph(id, is_path_needed) (fap/fap.h:127-128). */
+int fap_graph_solve_ex(fap_graph *g, int32_t is_path_needed);
+int fap_graph_solve_subgraph_ex(fap_graph *g, int32_t id, int32_t is_path_needed);
+/* Rows of the predecessors of solve(true) / of the last solveSubGraph(id, true). */
+int fap_graph_get_path_rows(fap_graph *g, int64_t row0, int64_t rows, int32_t *h_out);
+int fap_graph_get_subgraph_path_rows(fap_graph *g, int32_t id, int64_t row0, int64_t rows,
+                                     int32_t *h_out);
 int fap_graph_sizes(fap_graph *g, int32_t *n, int64_t *m, int32_t *K);
 /* Host copies: CSR (row_offset[n+1], col[m], weight[m] as int32), layout arrays
  * (graph_id[n], st2ed[n], ed2st[n]) and per-part counts (num[K+1], offset[K+1],

@@ -118,6 +118,25 @@ def solve_parts(first, last, csr, lay, row_base=0, rows=None):
     return to_i32(dist)
 
 
+def solve_parts_path(first, last, csr, lay, row_base=0, rows=None):
+    """As solve_parts, also returning the predecessor block: (int32 dist, int32 path)."""
+    ro, cv = i32(csr["row_offset"]), i32(csr["col"])
+    w = f32(csr["weight"])
+    n = len(ro) - 1
+    adj = i32(np.diff(ro))
+    if rows is None:
+        rows = int(sum(lay["num"][first:last + 1]))
+    dist = np.empty((rows, n), np.float32)
+    path = np.empty((rows, n), np.int32)
+    args = [i32(lay[k]) for k in ("graph_id", "st2ed", "ed2st", "num", "offset", "bdy")]
+    lib().orc_solve_parts_path_parallel(C.c_int(first), C.c_int(last), C.c_int(n),
+                                        _p(adj, C.c_int32), _p(ro, C.c_int32), _p(cv, C.c_int32),
+                                        _p(w, C.c_float), *[_p(a, C.c_int32) for a in args],
+                                        _p(dist, C.c_float), _p(path, C.c_int32),
+                                        C.c_int64(row_base))
+    return to_i32(dist), path
+
+
 def exact_rows(row_offset, col, weight, sources):
     """Independent exact distances (BFS if weight is None, else integer Dijkstra)."""
     ro, cv, src = i32(row_offset), i32(col), i32(sources)

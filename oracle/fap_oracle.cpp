@@ -196,6 +196,21 @@ void orc_solve_parts_parallel(int first, int last, int n, const int *adj_size,
   }
 }
 
+// The same, keeping the predecessor (path) block of every part beside the distances
+// (path rows laid out like dist: row r = source st2ed[r], column = vertex).
+void orc_solve_parts_path_parallel(int first, int last, int n, const int *adj_size,
+                                   const int *row_offset, const int *col, const float *w,
+                                   const int *graph_id, const int *st2ed, const int *ed2st,
+                                   const int *blk_num, const int *blk_off, const int *bdy_num,
+                                   float *dist, int *path, int64_t row_base) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int p = first; p <= last; ++p) {
+    const int64_t off = ((int64_t)blk_off[p] - row_base) * n;
+    orc_run_part(p, n, adj_size, row_offset, col, w, graph_id, st2ed, ed2st, blk_num, blk_off,
+                 bdy_num, dist + off, path + off);
+  }
+}
+
 // float result -> int32 with MAXVALUE -> INT32_MAX / 2 (exact for integers < 2^24).
 void orc_to_i32(const float *in, int32_t *out, int64_t count) {
   for (int64_t i = 0; i < count; ++i) out[i] = in[i] >= kMaxValue ? kInf : (int32_t)in[i];

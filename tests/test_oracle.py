@@ -207,3 +207,27 @@ def test_exact_rows_f64_pins():
             D = np.minimum(D, D[:, k:k + 1] + D[k:k + 1, :])
         rows = O.exact_rows_f64(ro, col, wt, np.arange(n))
         assert np.array_equal(rows, D)
+
+
+def test_oracle_paths_are_valid_predecessors():
+    """The oracle's path blocks (reference CPU path with is_path_needed) name, for every
+    reachable entry, an in-neighbour on a shortest path; unreachable entries are -1 and
+    the diagonal is the source itself.  Distances equal the exact APSP."""
+    import fastapsp_amd as F
+    g = F.FapGraph.road_grid(24, 20, 0.1, 0.05, 9, 4)
+    g.preCondition()
+    lay, csr = g.layout(), g.csr()
+    dist, path = O.solve_parts_path(1, g.K, csr, lay)
+    src = lay["st2ed"]
+    assert np.array_equal(dist, O.exact_rows(csr["row_offset"], csr["col"], None, src))
+    ro, col = csr["row_offset"], csr["col"]
+    arcs = {(u, int(col[e])) for u in range(len(ro) - 1) for e in range(ro[u], ro[u + 1])}
+    for r, s in enumerate(src):
+        for v in range(dist.shape[1]):
+            if dist[r, v] >= O.INF:
+                assert path[r, v] == -1
+            elif v == s:
+                assert path[r, v] == s
+            else:
+                u = int(path[r, v])
+                assert (u, v) in arcs and dist[r, u] + 1 == dist[r, v]
