@@ -314,6 +314,8 @@ def main():
     ap.add_argument("--graphs", type=int, default=512, help="graphs of the batch512 config")
     ap.add_argument("--pool", type=int, default=24, help="resident graphs of the serve config")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--paths", action="store_true",
+                    help="also time solve(True): distances and predecessor matrices")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--skip-forms", action="store_true",
@@ -452,6 +454,24 @@ def main():
         log(f"plain: {plain_ms:.2f} ms/step ({plain_ms / ms_per_step:.2f}x slower)")
         g.setMode(F.FAP_MODE_BATCHED)
 
+    paths = None
+    if args.paths:
+        # solve(True): the distances plus the n x n predecessor matrix with the reference
+        # CPU path's tie-breaking (SSSP boundary rows + predecessor rule, blocked
+        # Floyd-Warshall with per-step records, first-k min-plus), both resident.
+        g.solve(True)
+        L.check(L.load().fap_hip_device_sync(), "sync")
+        tp = time.perf_counter()
+        for _ in range(2):
+            g.solve(True)
+        L.check(L.load().fap_hip_device_sync(), "sync")
+        pms = (time.perf_counter() - tp) * 1e3 / 2
+        pst = stage_times()
+        paths = {"ms_per_step": pms, "entries_per_s": entries / (pms * 1e-3), "stages_ms": pst}
+        log(f"solve with paths: {pms:.2f} ms/step ({entries / (pms * 1e-3):.3e} entries/s); "
+            f"stages {pst}")
+        g.solve()  # leave the distance-only result resident
+
     cpu = None
     if not args.skip_cpu:
         cpu = cpu_baseline(g, args.cpu_budget)
@@ -468,6 +488,8 @@ def main():
            "stages_ms": avg, "floyd_warshall": fw_roof,
            "overlap_form": {"ms_per_step": overlap_ms},
            "roofline": roofline, "cpu_baseline": cpu, "plain_form": plain}
+    if paths is not None:
+        out["paths"] = paths
     print(json.dumps(out), flush=True)
 
 

@@ -52,3 +52,36 @@ def test_knn262k_k256_subgraph_every_part():
     assert bad.size == 0, f"{bad.size} sampled rows differ (first source {sources[bad[0]]})"
     # Not strongly connected: unreachable pairs exist and stay FAP_INF.
     assert (got == F.FAP_INF).any()
+
+
+def test_road114599_paths_sampled_rows_valid():
+    """solve(True) at the headline size (n = 114,599, K = 128; 52.5 GB of distances and
+    52.5 GB of predecessors resident): on sampled rows every reachable entry's
+    predecessor is an in-neighbour on a shortest path, the diagonal names the source,
+    and the distances equal an exact BFS."""
+    say("road-like 114599 K=128: solve(True), sampled rows")
+    g = F.FapGraph.road_like(114599, 0.52, 1e-4, 1, 128)
+    g.preCondition()
+    g.solve(True)
+    lay, csr = g.layout(), g.csr()
+    n = g.num_vertexs
+    ro, col = csr["row_offset"].astype(np.int64), csr["col"].astype(np.int64)
+    tails = np.repeat(np.arange(n, dtype=np.int64), np.diff(ro))
+    arc_keys = np.sort(tails * n + col)
+    rng = np.random.default_rng(3)
+    rows = np.unique(np.concatenate([[0, n - 1], rng.integers(0, n, 14)]))
+    for r in rows:
+        d = g.getDistanceRowsI32(int(r), 1)[0].astype(np.int64)
+        p = g.getPathRows(int(r), 1)[0].astype(np.int64)
+        s = int(lay["st2ed"][r])
+        assert np.array_equal(d, O.exact_rows(csr["row_offset"], csr["col"], None, [s])[0])
+        reach = d < F.FAP_INF
+        assert np.all(p[~reach] == -1) and p[s] == s and d[s] == 0
+        v = np.nonzero(reach)[0]
+        v = v[v != s]
+        u = p[v]
+        assert np.all(u >= 0)
+        keys = u * n + v
+        pos = np.searchsorted(arc_keys, keys)
+        assert np.all(arc_keys[np.minimum(pos, len(arc_keys) - 1)] == keys), r
+        assert np.all(d[u] + 1 == d[v]), r
