@@ -51,6 +51,7 @@ int timing_start(hipStream_t st);                   // new call: reset, record s
 int timing_record(hipStream_t on, hipEvent_t *e);   // pooled event recorded on `on`
 void timing_add_span(int stage, hipEvent_t a, hipEvent_t b);
 int timing_finish(hipStream_t st);                  // record the end of the call
+void timing_note_keyed(int merge_parts, int fw_parts);  // path solves: keyed forms taken
 
 }  // namespace fap_hip
 

@@ -58,7 +58,11 @@ namespace {
 
 constexpr int kT = 64;          // Floyd-Warshall tile edge
 constexpr int kTT = kT * kT;    // entries per tile
+constexpr int kTP = kT + 1;     // LDS row stride of a tile: a column of a wave's rows
+                                // then spreads over all banks
 constexpr int kMaxS = 8192;
+
+typedef int v4i __attribute__((ext_vector_type(4)));
 
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
@@ -97,18 +101,31 @@ __device__ __forceinline__ int *rec_rowpanelp(const PathPart &p, int t) {
 }
 
 // ---- boundary rows: predecessors from the distance rows ------------------------
+// The largest finite boundary-row entry is kept in kMaxSlots words kSlotStride ints
+// apart (each wave's maximum goes to one of them, so the atomics spread over
+// channels); -1 when there is none.
+constexpr int kMaxSlots = 64;
+constexpr int kSlotStride = 16;
+
 // Boundary row k = k0 + blockIdx.y (source src[k], row offset rows[k] in d_out and
 // d_path).
-__global__ void pred_rows_kernel(const int *__restrict__ rro, const int *__restrict__ rcol,
-                                 const int *__restrict__ rw, int n, const int *__restrict__ src,
-                                 const int64_t *__restrict__ rows, int k0,
-                                 const int *__restrict__ out, int *__restrict__ path) {
+__global__ __launch_bounds__(256) void pred_rows_kernel(
+    const int *__restrict__ rro, const int *__restrict__ rcol, const int *__restrict__ rw, int n,
+    const int *__restrict__ src, const int64_t *__restrict__ rows, int k0,
+    const int *__restrict__ out, int *__restrict__ path, int *__restrict__ bmax_slots) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n) return;
   const int k = k0 + blockIdx.y;
-  const int s = src[k];
   const int *d = out + rows[k];
-  const int dv = d[v];
+  const int dv = v < n ? d[v] : kInf;
+  int mx = dv < kInf ? dv : -1;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off));
+  if ((threadIdx.x & 63) == 0 && mx >= 0) {
+    const int slot = (blockIdx.x * 4 + (threadIdx.x >> 6) + blockIdx.y) % kMaxSlots;
+    atomicMax(bmax_slots + slot * kSlotStride, mx);
+  }
+  if (v >= n) return;
+  const int s = src[k];
   int p = -1;
   if (v == s) {
     p = s;
@@ -131,14 +148,16 @@ __global__ void pred_rows_kernel(const int *__restrict__ rro, const int *__restr
 
 // ---- build ----------------------------------------------------------------------
 // One workgroup per padded row of every part (rows of PathPart::row_begin prefix).
+// pmax[part] = max(pmax[part], largest finite entry of the built rows).
 __global__ __launch_bounds__(256) void build_path_kernel(
     const PathPart *__restrict__ parts, int np, int total_rows, const int *__restrict__ row_off,
     const int *__restrict__ col, const int *__restrict__ w, const int *__restrict__ graph_id,
     const int *__restrict__ st2ed, const int *__restrict__ ed2st, const int *__restrict__ out,
-    const int *__restrict__ pout, int64_t ld_out) {
+    const int *__restrict__ pout, int64_t ld_out, int *__restrict__ pmax) {
   const int grow = blockIdx.x;
   if (grow >= total_rows) return;
-  const PathPart d = parts[find_part(parts, np, grow, &PathPart::row_begin)];
+  const int pi = find_part(parts, np, grow, &PathPart::row_begin);
+  const PathPart d = parts[pi];
   const int i = grow - d.row_begin;
   __shared__ int rd[kMaxS];
   __shared__ int rp[kMaxS];
@@ -176,10 +195,18 @@ __global__ __launch_bounds__(256) void build_path_kernel(
   __syncthreads();
   int *dd = d.fw + (int64_t)i * d.S;
   int *dp = d.fwp + (int64_t)i * d.S;
+  int mx = -1;
   for (int j = tid; j < d.S; j += 256) {
     dd[j] = rd[j];
     dp[j] = rp[j];
+    if (rd[j] < kInf) mx = max(mx, rd[j]);
   }
+  __shared__ int red;
+  if (tid == 0) red = -1;
+  __syncthreads();
+  atomicMax(&red, mx);
+  __syncthreads();
+  if (tid == 0 && red >= 0) atomicMax(pmax + pi, red);
 }
 
 // ---- Floyd-Warshall with k-i-j path semantics -----------------------------------
@@ -188,7 +215,7 @@ __device__ __forceinline__ int *tile(int *m, const PathPart &p, int ti, int tj) 
   return m + (int64_t)(ti * kT) * p.S + tj * kT;
 }
 
-__device__ __forceinline__ void tile_to_lds(int (*D)[kT], int (*P)[kT], const int *gd,
+__device__ __forceinline__ void tile_to_lds(int (*D)[kTP], int (*P)[kTP], const int *gd,
                                             const int *gp, int64_t ld) {
   const int r = threadIdx.x >> 2, c0 = (threadIdx.x & 3) * 16;
 #pragma unroll
@@ -198,7 +225,7 @@ __device__ __forceinline__ void tile_to_lds(int (*D)[kT], int (*P)[kT], const in
   }
 }
 
-__device__ __forceinline__ void lds_to_tile(int (*D)[kT], int (*P)[kT], int *gd, int *gp,
+__device__ __forceinline__ void lds_to_tile(int (*D)[kTP], int (*P)[kTP], int *gd, int *gp,
                                             int64_t ld) {
   const int r = threadIdx.x >> 2, c0 = (threadIdx.x & 3) * 16;
 #pragma unroll
@@ -208,14 +235,34 @@ __device__ __forceinline__ void lds_to_tile(int (*D)[kT], int (*P)[kT], int *gd,
   }
 }
 
+// Keyed form (a part whose finite Floyd-Warshall entries stay below kFkLim, flag
+// fwkey[part]): an entry d with its path source packed as (min(d, kFkLim) << 7) | tag,
+// tag 0 = the path the cell had when the phase started, tag k + 1 = set at pivot k.
+// A candidate through pivot k is (a << 7) + ((b << 7) | (k + 1)) (a, b clamped), so
+// one unsigned minimum keeps the smallest distance, the old value on a tie and the
+// first pivot among equal candidates: the strict '>' of the k-i-j loop.  Sums that
+// reach kFkLim are never a minimum (every finite entry, and every value the loop
+// keeps, is a simple-path length <= (s - 1) * max entry < kFkLim, checked on the host;
+// weights are non-negative).  Distance records of such a part are keys: diag row /
+// column records (min(d, L) << 7) | (k + 1) for phase 2, column panel records
+// min(d, L) << 7 and row panel records (min(d, L) << 7) | (k + 1) for phase 3.
+constexpr uint32_t kFkLim = (1u << 24) - 1;
+
+__device__ __forceinline__ uint32_t fkey(int d) { return min((uint32_t)d, kFkLim) << 7; }
+__device__ __forceinline__ int fkey_dist(uint32_t key) {
+  const uint32_t d = key >> 7;
+  return d >= kFkLim ? kInf : (int)d;
+}
+
 // Phase 1: the diagonal tile of round r, sequentially over its 64 pivots, recording
 // row k (dist, path) and column k as they stand at step k.
 __global__ __launch_bounds__(256) void fwp_phase1_kernel(const PathPart *__restrict__ parts,
-                                                         int r) {
+                                                         const int *__restrict__ fwkey, int r) {
   const PathPart p = parts[blockIdx.x];
   if (r >= p.nb) return;
-  __shared__ int D[kT][kT];
-  __shared__ int P[kT][kT];
+  const bool keyed = fwkey[blockIdx.x] != 0;
+  __shared__ int D[kT][kTP];
+  __shared__ int P[kT][kTP];
   tile_to_lds(D, P, tile(p.fw, p, r, r), tile(p.fwp, p, r, r), p.S);
   int *rrow = rec_diag_row(p), *rrowp = rec_diag_rowp(p), *rcol = rec_diag_col(p);
   const int i = threadIdx.x >> 2, c0 = (threadIdx.x & 3) * 16;
@@ -224,11 +271,15 @@ __global__ __launch_bounds__(256) void fwp_phase1_kernel(const PathPart *__restr
     if (i == k) {
 #pragma unroll
       for (int c = 0; c < 16; ++c) {
-        rrow[k * kT + c0 + c] = D[k][c0 + c];
+        const int v = D[k][c0 + c];
+        rrow[k * kT + c0 + c] = keyed ? (int)(fkey(v) | (k + 1)) : v;
         rrowp[k * kT + c0 + c] = P[k][c0 + c];
       }
     }
-    if (k >= c0 && k < c0 + 16) rcol[k * kT + i] = D[i][k];  // stored k-major
+    if (k >= c0 && k < c0 + 16) {  // stored k-major
+      const int v = D[i][k];
+      rcol[k * kT + i] = keyed ? (int)(fkey(v) | (k + 1)) : v;
+    }
     const int dik = D[i][k];
 #pragma unroll
     for (int c = 0; c < 16; ++c) {
@@ -245,19 +296,19 @@ __global__ __launch_bounds__(256) void fwp_phase1_kernel(const PathPart *__restr
 
 // Phase 2: blockIdx.x < nb - 1: column panel tile (t, r); else row panel tile (r, t).
 __global__ __launch_bounds__(256) void fwp_phase2_kernel(const PathPart *__restrict__ parts,
-                                                         int r) {
+                                                         const int *__restrict__ fwkey, int r) {
   const PathPart p = parts[blockIdx.y];
-  if (r >= p.nb) return;
+  if (r >= p.nb || fwkey[blockIdx.y]) return;
   const int others = p.nb - 1;
   int t = blockIdx.x;
   if (t >= 2 * others) return;
   const bool colp = t < others;
   if (!colp) t -= others;
   const int o = t < r ? t : t + 1;  // tile index outside the pivot
-  __shared__ int D[kT][kT];
-  __shared__ int P[kT][kT];
-  __shared__ int R0[kT][kT];  // column panel: diag row records; row panel: diag column records
-  __shared__ int R1[kT][kT];  // column panel: diag row path records
+  __shared__ int D[kT][kTP];
+  __shared__ int P[kT][kTP];
+  __shared__ int R0[kT][kTP];  // column panel: diag row records; row panel: diag column records
+  __shared__ int R1[kT][kTP];  // column panel: diag row path records
   int *gd = colp ? tile(p.fw, p, o, r) : tile(p.fw, p, r, o);
   int *gp = colp ? tile(p.fwp, p, o, r) : tile(p.fwp, p, r, o);
   tile_to_lds(D, P, gd, gp, p.S);
@@ -313,9 +364,9 @@ __global__ __launch_bounds__(256) void fwp_phase2_kernel(const PathPart *__restr
 // Phase 3: tile (a, b) outside the pivot row and column, from the panel records.
 // Thread owns a 4 x 4 block: rows ty*4.., columns tx*4.. (ty, tx < 16).
 __global__ __launch_bounds__(256) void fwp_phase3_kernel(const PathPart *__restrict__ parts,
-                                                         int r) {
+                                                         const int *__restrict__ fwkey, int r) {
   const PathPart p = parts[blockIdx.y];
-  if (r >= p.nb) return;
+  if (r >= p.nb || fwkey[blockIdx.y]) return;
   const int others = p.nb - 1;
   const int t = blockIdx.x;
   if (t >= others * others) return;
@@ -369,6 +420,224 @@ __global__ __launch_bounds__(256) void fwp_phase3_kernel(const PathPart *__restr
       gd[(int64_t)(ty * 4 + q) * p.S + tx * 4 + c] = dv[q][c];
       gp[(int64_t)(ty * 4 + q) * p.S + tx * 4 + c] = pv[q][c];
     }
+}
+
+// Phase 2, keyed form: one wave per panel tile, each lane holding one line of 64 keys
+// in registers (no barriers: every line's update depends on that line and the diag
+// records only).  The 64 pivots are expanded at compile time (PanelSteps) so that key
+// k is a fixed register.
+//
+// Column panel tile (o, r), blockIdx.x = o' (o' < nb - 1, skipping r): lane i = row i;
+// pivot k uses the lane's own key k and the diag row record k (scalar loads); the
+// path of a cell whose tag is k + 1 is the diag row path record k.
+// Row panel tile (r, o): lane j = column j; pivot k uses the lane's own key k and the
+// diag column record k; a cell whose tag is k + 1 takes the path of cell (k, j) at
+// step k.  The tag of key k at step k is kept (8 bits each in tagw); after the pivots,
+// the row paths are resolved in pivot order: tab[k][j] holds the tile's original path
+// until it is replaced by the resolved one (tab[tag - 1][j], or itself for tag 0).
+template <int K, bool kRowPanel>
+struct PanelSteps {
+  static __device__ __forceinline__ void run(uint32_t (&key)[kT], uint32_t (&tagw)[kT / 4],
+                                             const FAP_CONST uint32_t *rk,
+                                             FAP_GLOBAL uint32_t *recd, int lane) {
+    if constexpr (K < kT) {
+      const uint32_t kk = key[K];
+      const uint32_t dk = kk & ~0x7Fu;
+      if constexpr (kRowPanel) {
+        recd[K * kT + lane] = dk | (uint32_t)(K + 1);
+        tagw[K / 4] |= (kk & 0x7F) << (8 * (K % 4));
+      } else {
+        recd[K * kT + lane] = dk;  // phase 3 A operand
+      }
+#pragma unroll
+      for (int c = 0; c < kT; ++c) key[c] = min(key[c], dk + rk[K * kT + c]);
+      PanelSteps<K + 1, kRowPanel>::run(key, tagw, rk, recd, lane);
+    }
+  }
+};
+
+__global__ __launch_bounds__(64) void fwp_colpanel_key_kernel(const PathPart *__restrict__ parts,
+                                                              const int *__restrict__ fwkey,
+                                                              int r) {
+  const PathPart p = parts[blockIdx.y];
+  if (r >= p.nb || !fwkey[blockIdx.y]) return;
+  const int t = blockIdx.x;
+  if (t >= p.nb - 1) return;
+  const int o = t < r ? t : t + 1;
+  const int lane = threadIdx.x;
+  __shared__ int tab[kT][kTP];  // diag row path records
+  FAP_GLOBAL int *gd = gptr(tile(p.fw, p, o, r) + (int64_t)lane * p.S);
+  FAP_GLOBAL int *gp = gptr(tile(p.fwp, p, o, r) + (int64_t)lane * p.S);
+  uint32_t key[kT];
+#pragma unroll
+  for (int c = 0; c < kT; c += 4) {
+    const v4i v = *reinterpret_cast<const FAP_GLOBAL v4i *>(gd + c);
+    key[c] = fkey(v.x);
+    key[c + 1] = fkey(v.y);
+    key[c + 2] = fkey(v.z);
+    key[c + 3] = fkey(v.w);
+  }
+  const FAP_GLOBAL int *rp = gptr(rec_diag_rowp(p));
+  for (int e = lane; e < kTT; e += 64) tab[e >> 6][e & 63] = rp[e];
+  __syncthreads();
+  uint32_t tagw[kT / 4];  // unused by the column panel
+  PanelSteps<0, false>::run(key, tagw,
+                            uniform_cptr(reinterpret_cast<const uint32_t *>(rec_diag_row(p))),
+                            gptr(reinterpret_cast<uint32_t *>(rec_colpanel(p, o))), lane);
+#pragma unroll
+  for (int c = 0; c < kT; c += 4) {
+    const v4i v = {fkey_dist(key[c]), fkey_dist(key[c + 1]), fkey_dist(key[c + 2]),
+                   fkey_dist(key[c + 3])};
+    *reinterpret_cast<FAP_GLOBAL v4i *>(gd + c) = v;
+  }
+#pragma unroll
+  for (int c = 0; c < kT; ++c) {
+    const uint32_t tg = key[c] & 0x7F;
+    if (tg) gp[c] = tab[tg - 1][c];
+  }
+}
+
+__global__ __launch_bounds__(64) void fwp_rowpanel_key_kernel(const PathPart *__restrict__ parts,
+                                                              const int *__restrict__ fwkey,
+                                                              int r) {
+  const PathPart p = parts[blockIdx.y];
+  if (r >= p.nb || !fwkey[blockIdx.y]) return;
+  const int t = blockIdx.x;
+  if (t >= p.nb - 1) return;
+  const int o = t < r ? t : t + 1;
+  const unsigned lane = threadIdx.x;
+  __shared__ int tab[kT][kTP];  // row paths of this tile, resolved as each step records them
+  // The tile's rows are staged through tab (16 B per lane, 4 rows per load), the lane
+  // then reads its column: first the distances, then the paths.
+  FAP_GLOBAL int *gd = gptr(tile(p.fw, p, r, o));
+  FAP_GLOBAL int *gp = gptr(tile(p.fwp, p, r, o));
+  const int sr = lane >> 4, sc = (lane & 15) * 4;  // staging: row sr + 4 q, columns sc..
+  auto stage = [&](const FAP_GLOBAL int *g) {
+#pragma unroll
+    for (int q = 0; q < kT / 4; ++q) {
+      const v4i v = *reinterpret_cast<const FAP_GLOBAL v4i *>(g + (int64_t)(sr + 4 * q) * p.S + sc);
+      tab[sr + 4 * q][sc] = v.x;
+      tab[sr + 4 * q][sc + 1] = v.y;
+      tab[sr + 4 * q][sc + 2] = v.z;
+      tab[sr + 4 * q][sc + 3] = v.w;
+    }
+  };
+  auto unstage = [&](FAP_GLOBAL int *g) {
+#pragma unroll
+    for (int q = 0; q < kT / 4; ++q) {
+      const v4i v = {tab[sr + 4 * q][sc], tab[sr + 4 * q][sc + 1], tab[sr + 4 * q][sc + 2],
+                     tab[sr + 4 * q][sc + 3]};
+      *reinterpret_cast<FAP_GLOBAL v4i *>(g + (int64_t)(sr + 4 * q) * p.S + sc) = v;
+    }
+  };
+  uint32_t key[kT];
+  stage(gd);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kT; ++i) key[i] = fkey(tab[i][lane]);
+  __syncthreads();
+  stage(gp);
+  __syncthreads();
+  uint32_t tagw[kT / 4];
+#pragma unroll
+  for (int q = 0; q < kT / 4; ++q) tagw[q] = 0;
+  PanelSteps<0, true>::run(key, tagw,
+                           uniform_cptr(reinterpret_cast<const uint32_t *>(rec_diag_col(p))),
+                           gptr(reinterpret_cast<uint32_t *>(rec_rowpanel(p, o))), lane);
+  {
+    FAP_GLOBAL int *recp = gptr(rec_rowpanelp(p, o));
+#pragma unroll
+    for (int k = 0; k < kT; ++k) {  // tag <= k: row tag - 1 is resolved
+      const uint32_t tg = (tagw[k / 4] >> (8 * (k % 4))) & 0x7F;
+      const int pk = tab[tg ? tg - 1 : k][lane];
+      tab[k][lane] = pk;  // this lane's column only: no barrier
+      recp[k * kT + lane] = pk;
+    }
+  }
+  // Final paths: tag t -> row path t - 1; tag 0 -> tab[i][j], which is the path the
+  // cell had at step i, i.e. its original one when no pivot ever replaced it.
+  int pv[kT];
+#pragma unroll
+  for (int i = 0; i < kT; ++i) {
+    const uint32_t tg = key[i] & 0x7F;
+    pv[i] = tab[tg ? tg - 1 : i][lane];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kT; ++i) tab[i][lane] = fkey_dist(key[i]);
+  __syncthreads();
+  unstage(gd);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kT; ++i) tab[i][lane] = pv[i];
+  __syncthreads();
+  unstage(gp);
+}
+
+// Phase 3, keyed form: tile (a, b) from the keyed panel records; thread owns a 4 x 4
+// block (rows ty * 4.., columns tx * 4..), two pivots per v_min3.  Only cells whose
+// tag is set get a new path (the row panel path record of that pivot).
+__global__ __launch_bounds__(256) void fwp_phase3_key_kernel(const PathPart *__restrict__ parts,
+                                                             const int *__restrict__ fwkey,
+                                                             int r) {
+  const PathPart p = parts[blockIdx.y];
+  if (r >= p.nb || !fwkey[blockIdx.y]) return;
+  const int others = p.nb - 1;
+  const int t = blockIdx.x;
+  if (t >= others * others) return;
+  const int a0 = t / others, b0 = t - a0 * others;
+  const int a = a0 < r ? a0 : a0 + 1, b = b0 < r ? b0 : b0 + 1;
+  __shared__ uint4 As[kT][kT / 4];  // As[k][i / 4]: column panel keys of tile (a, r)
+  __shared__ uint4 Bs[kT][kT / 4];  // Bs[k][j / 4]: row panel keys of tile (r, b)
+  __shared__ int Bp[kT][kTP];       // row panel path records
+  {
+    const uint4 *ra = reinterpret_cast<const uint4 *>(rec_colpanel(p, a));
+    const uint4 *rb = reinterpret_cast<const uint4 *>(rec_rowpanel(p, b));
+    for (int e = threadIdx.x; e < kTT / 4; e += 256) {
+      As[e >> 4][e & 15] = ra[e];
+      Bs[e >> 4][e & 15] = rb[e];
+    }
+    const int *rbp = rec_rowpanelp(p, b);
+    for (int e = threadIdx.x; e < kTT; e += 256) Bp[e >> 6][e & 63] = rbp[e];
+  }
+  const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+  int *gd = tile(p.fw, p, a, b) + (int64_t)(ty * 4) * p.S + tx * 4;
+  int *gp = tile(p.fwp, p, a, b) + (int64_t)(ty * 4) * p.S + tx * 4;
+  uint32_t key[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int4 v = *reinterpret_cast<const int4 *>(gd + (int64_t)q * p.S);
+    key[q][0] = fkey(v.x);
+    key[q][1] = fkey(v.y);
+    key[q][2] = fkey(v.z);
+    key[q][3] = fkey(v.w);
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int k = 0; k < kT; k += 2) {
+    const uint4 x0 = As[k][ty], y0 = Bs[k][tx], x1 = As[k + 1][ty], y1 = Bs[k + 1][tx];
+    const uint32_t av0[4] = {x0.x, x0.y, x0.z, x0.w}, bv0[4] = {y0.x, y0.y, y0.z, y0.w};
+    const uint32_t av1[4] = {x1.x, x1.y, x1.z, x1.w}, bv1[4] = {y1.x, y1.y, y1.z, y1.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        key[q][c] = min(key[q][c], min(av0[q] + bv0[c], av1[q] + bv1[c]));
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    int4 v;
+    v.x = fkey_dist(key[q][0]);
+    v.y = fkey_dist(key[q][1]);
+    v.z = fkey_dist(key[q][2]);
+    v.w = fkey_dist(key[q][3]);
+    *reinterpret_cast<int4 *>(gd + (int64_t)q * p.S) = v;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const uint32_t tg = key[q][c] & 0x7F;
+      if (tg) gp[(int64_t)q * p.S + c] = Bp[tg - 1][tx * 4 + c];
+    }
+  }
 }
 
 // ---- min-plus of the inner rows with paths, and the decode of own columns ----------
@@ -474,7 +743,164 @@ __global__ __launch_bounds__(64) void merge_path_kernel(
   }
 }
 
-// Boundary rows' own columns: out[row][st2ed[offset + c]] = FW[row][c] (dist, path).
+// Largest finite entry of a part's inner rows in its boundary columns (the A operand
+// of the min-plus) after Floyd-Warshall, -1 when there is none; one workgroup per part.
+// Parts with akey_kp[part] = kp > 0 (8, 16 or 32, with kb = 3, 4 or 5 tag bits) also
+// get their A keys for the keyed min-plus: akey[akey_off + i * kp + k] =
+// min(A[i][k], L) << kb for k < bdy, 2^31 for bdy <= k < kp (paired with an infinite
+// B key, never the minimum).
+__global__ __launch_bounds__(256) void path_amax_kernel(const PathPart *__restrict__ parts,
+                                                        int *__restrict__ amax,
+                                                        uint32_t *__restrict__ akey,
+                                                        const int64_t *__restrict__ akey_off,
+                                                        const int *__restrict__ akey_kp) {
+  const PathPart p = parts[blockIdx.x];
+  const int inner = p.s - p.bdy;
+  const int64_t ko = akey_off[blockIdx.x];
+  const int kp = akey_kp[blockIdx.x];
+  const int kb = kp == 8 ? 3 : kp == 16 ? 4 : 5;
+  const uint32_t lim = (1u << (31 - kb)) - 1;
+  int mx = -1;
+  for (int64_t e = threadIdx.x; e < (int64_t)inner * p.bdy; e += 256) {
+    const int i = (int)(e / p.bdy), k = (int)(e - (int64_t)i * p.bdy);
+    const int v = p.fw[(int64_t)(p.bdy + i) * p.S + k];
+    if (v < kInf) mx = max(mx, v);
+  }
+  if (kp > 0) {
+    for (int64_t e = threadIdx.x; e < (int64_t)inner * kp; e += 256) {
+      const int i = (int)(e / kp), k = (int)(e - (int64_t)i * kp);
+      uint32_t key = 0x80000000u;
+      if (k < p.bdy) key = min((uint32_t)p.fw[(int64_t)(p.bdy + i) * p.S + k], lim) << kb;
+      akey[ko + e] = key;
+    }
+  }
+  __shared__ int red;
+  if (threadIdx.x == 0) red = -1;
+  __syncthreads();
+  atomicMax(&red, mx);
+  __syncthreads();
+  if (threadIdx.x == 0) amax[blockIdx.x] = red;
+}
+
+// ---- min-plus with paths, keyed form ---------------------------------------------
+// When every finite A entry plus every finite boundary-row entry stays below
+// L = 2^(31 - KB) - 1 and the part has at most 2^KB boundary vertices, a candidate
+// A[i][k] + B[k][j] and its k fit one 32-bit key: (min(a, L) << KB) +
+// ((min(b, L) << KB) | k).  The unsigned minimum over k is then the smallest sum with
+// ties to the smallest k, which is what the strict '<' over ascending k keeps, and
+// the path is Bpath[key & (2^KB - 1)][j]; a key whose sum reaches L is an infinite
+// candidate (finite sums stay below L), so the initial key L << KB means FAP_INF / -1.
+// One add and half a v_min3 per candidate against an add, a compare and two selects.
+//
+// Unit: a workgroup of 4 waves, kMkCols columns (lane l holds cb * 256 + c * 64 + l,
+// c < 4, so every store writes 256 contiguous bytes) x up to kMkRows inner rows of one
+// part, wave w taking rows w, w + 4, ...  The B keys of the unit's columns stay in
+// registers for all its rows, the boundary paths sit in LDS once per workgroup as
+// [k][c][lane] (a gather of any k per lane is bank-conflict free), the A keys
+// (path_amax_kernel, KP per row) come in with scalar loads, the next row's while the
+// current one is reduced.  The part's own columns are written here too and then
+// overwritten by decode_path_kernel (kept out of this loop: a load between the stores
+// would make every store wait).
+constexpr int kMkCols = 256;
+constexpr int kMkRows = 1024;
+constexpr int kMkWaves = 4;
+
+struct PathKeyDesc {
+  int part;              // index into the part table
+  int unit_begin;        // first unit of this part
+  int col_blocks;        // ceil(n / 256)
+  int pad;
+  const uint32_t *akey;  // A keys of the part's inner rows, KP per row
+};
+
+template <int KP, int KB>
+__global__ __launch_bounds__(256) void merge_path_key_kernel(
+    const PathPart *__restrict__ parts, const PathKeyDesc *__restrict__ md, int nmd,
+    int total_units, int n, int *__restrict__ out, int *__restrict__ pout, int64_t ld_out) {
+  static_assert((1 << KB) >= KP, "tag field too narrow");
+  constexpr uint32_t L = (1u << (31 - KB)) - 1;
+  constexpr uint32_t kInit = L << KB;
+  constexpr uint32_t kTag = (1u << KB) - 1;
+  extern __shared__ int bpl[];  // K x 4 x 64 boundary paths
+  const int u = blockIdx.x;
+  if (u >= total_units) return;
+  int lo = 0, hi = nmd - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (md[mid].unit_begin <= u) lo = mid;
+    else hi = mid - 1;
+  }
+  const PathKeyDesc m = md[lo];
+  const PathPart p = parts[m.part];
+  const int local = u - m.unit_begin;
+  const int rc = local / m.col_blocks, cb = local - rc * m.col_blocks;
+  const int inner = p.s - p.bdy, K = p.bdy;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r0 = rc * kMkRows + wave, r1 = min(inner, rc * kMkRows + kMkRows);
+  const int jb = cb * kMkCols + lane;
+  const int64_t brow0 = p.out_row * ld_out;
+  for (int k = wave; k < K; k += kMkWaves)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int j = jb + c * 64;
+      bpl[(k * 4 + c) * 64 + lane] = j < n ? pout[brow0 + (int64_t)k * ld_out + j] : -1;
+    }
+  uint32_t bk[KP][4];
+#pragma unroll
+  for (int k = 0; k < KP; ++k)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int j = jb + c * 64;
+      bk[k][c] = kInit;  // k >= K: paired with an A key of 2^31, never the minimum
+      if (k < K) {
+        const uint32_t b = j < n ? (uint32_t)out[brow0 + (int64_t)k * ld_out + j] : L;
+        bk[k][c] = (min(b, L) << KB) | (uint32_t)k;
+      }
+    }
+  __syncthreads();
+  // The key array holds kMkWaves rows of slack past the last part: reading the row
+  // after a wave's last one is safe.
+  const FAP_CONST uint32_t *arow = uniform_cptr(m.akey + (int64_t)r0 * KP);
+  uint32_t a[KP];
+#pragma unroll
+  for (int k = 0; k < KP; ++k) a[k] = arow[k];
+  for (int i = r0; i < r1; i += kMkWaves) {
+    arow += kMkWaves * KP;
+    uint32_t an[KP];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) an[k] = arow[k];
+    uint32_t best[4] = {kInit, kInit, kInit, kInit};
+#pragma unroll
+    for (int k = 0; k < KP; k += 2) {
+      if (k < K) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          best[c] = min(best[c], min(a[k] + bk[k][c], a[k + 1] + bk[k + 1][c]));
+      }
+    }
+    const int64_t orow = (p.out_row + p.bdy + i) * ld_out;
+    // The minimum's tag is below K (the initial key has tag 0): the reads stay in the
+    // K rows of bpl.
+    int bp[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) bp[c] = bpl[((best[c] & kTag) * 4 + c) * 64 + lane];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int j = jb + c * 64;
+      const uint32_t sum = best[c] >> KB;
+      if (j < n) {
+        __builtin_nontemporal_store(sum < L ? (int)sum : kInf, out + orow + j);
+        __builtin_nontemporal_store(sum < L ? bp[c] : -1, pout + orow + j);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KP; ++k) a[k] = an[k];
+  }
+}
+
+// Own columns: out[row][st2ed[offset + c]] = FW[row][c] (dist, path) for rows
+// 0 .. rows_p - 1 of each part: its boundary rows, and all its rows when the keyed
+// min-plus wrote the inner rows (item_begin: prefix of rows_p * s_p).
 __global__ void decode_path_kernel(const PathPart *__restrict__ parts, int np,
                                    const int64_t *__restrict__ item_begin, int64_t total,
                                    const int *__restrict__ st2ed, int *__restrict__ out,
@@ -499,6 +925,7 @@ Workspace g_ws_pfw;    // FW distance and path matrices
 Workspace g_ws_psnap;  // per-part records
 Workspace g_ws_pmeta;  // descriptors, sources, row offsets
 Workspace g_ws_pell;   // ELL adjacency (unit weights)
+Workspace g_ws_pakey;  // A keys of the keyed min-plus
 
 }  // namespace
 }  // namespace fap_hip
@@ -547,10 +974,8 @@ static int solve_parts_path(const fap_graph_dev *g, const int32_t *d_rro, const 
 
   std::vector<PathPart> pp(P);
   std::vector<int> offsets(P), src_begin(P);
-  std::vector<int64_t> out_rows(P), dec_begin(P);
-  std::vector<PathMergeDesc> md;
-  int units = 0;
-  int64_t dec_total = 0;
+  std::vector<int64_t> out_rows(P);
+  int merge_parts = 0;
   for (int k = 0; k < P; ++k) {
     const fap_part_desc &q = act[k];
     pp[k] = PathPart{fw_base + fw_off[k], fwp_base + fw_off[k], snap_base + snap_off[k], S[k],
@@ -560,15 +985,7 @@ static int solve_parts_path(const fap_graph_dev *g, const int32_t *d_rro, const 
     src_begin[k] = src_total;
     out_rows[k] = q.out_row;
     src_total += q.bdy;
-    const int inner = q.num - q.bdy;
-    if (inner > 0) {
-      const int rbk = (inner + kMpRows - 1) / kMpRows;
-      md.push_back(PathMergeDesc{k, units, rbk});
-      units += rbk * ((n + kMpCols - 1) / kMpCols);
-      FAP_REQUIRE(units < (1 << 30), "fap_hip_solve_parts_path: too many merge units");
-    }
-    dec_begin[k] = dec_total;
-    dec_total += (int64_t)q.bdy * q.num;
+    merge_parts += q.num > q.bdy;
   }
 
   int64_t off = 0;
@@ -578,29 +995,38 @@ static int solve_parts_path(const fap_graph_dev *g, const int32_t *d_rro, const 
     return o;
   };
   const int64_t o_pp = carve(sizeof(PathPart) * P);
-  const int64_t o_md = carve(sizeof(PathMergeDesc) * std::max<size_t>(md.size(), 1));
   const int64_t o_off = carve(4LL * P), o_sb = carve(4LL * P), o_or = carve(8LL * P);
-  const int64_t o_db = carve(8LL * P);
   const int64_t upload = off;
+  const int64_t o_db = carve(8LL * P);
+  const int64_t o_md = carve(sizeof(PathMergeDesc) * P);
+  const int64_t o_kd = carve(sizeof(PathKeyDesc) * P);
+  const int nstat = kMaxSlots * kSlotStride + P;  // boundary-row maximum slots, then A maxima
+  const int64_t o_stat = carve(4LL * nstat);
+  const int64_t o_akoff = carve(12LL * P);  // A key offsets (int64), then classes (int)
+  const int64_t o_pmax = carve(4LL * P);     // largest finite built entry per part
+  const int64_t o_fwkey = carve(4LL * P);    // 1: keyed Floyd-Warshall
   const int64_t o_src = carve(4LL * std::max(src_total, 1));
   const int64_t o_rows = carve(8LL * std::max(src_total, 1));
   FAP_TRY(g_ws_pmeta.ensure(off));
   char *meta = static_cast<char *>(g_ws_pmeta.ptr);
   std::vector<char> host(upload);
   std::memcpy(host.data() + o_pp, pp.data(), sizeof(PathPart) * P);
-  if (!md.empty()) std::memcpy(host.data() + o_md, md.data(), sizeof(PathMergeDesc) * md.size());
   std::memcpy(host.data() + o_off, offsets.data(), 4LL * P);
   std::memcpy(host.data() + o_sb, src_begin.data(), 4LL * P);
   std::memcpy(host.data() + o_or, out_rows.data(), 8LL * P);
-  std::memcpy(host.data() + o_db, dec_begin.data(), 8LL * P);
   FAP_CHECK_HIP(hipMemcpyAsync(meta, host.data(), upload, hipMemcpyHostToDevice, st));
   FAP_CHECK_HIP(hipStreamSynchronize(st));
   const PathPart *d_pp = reinterpret_cast<const PathPart *>(meta + o_pp);
-  const PathMergeDesc *d_md = reinterpret_cast<const PathMergeDesc *>(meta + o_md);
+  PathMergeDesc *d_md = reinterpret_cast<PathMergeDesc *>(meta + o_md);
+  PathKeyDesc *d_kd = reinterpret_cast<PathKeyDesc *>(meta + o_kd);
+  int *d_stat = reinterpret_cast<int *>(meta + o_stat);
+  int64_t *d_akoff = reinterpret_cast<int64_t *>(meta + o_akoff);
+  int *d_pmax = reinterpret_cast<int *>(meta + o_pmax);
+  int *d_fwkey = reinterpret_cast<int *>(meta + o_fwkey);
   const int *d_offsets = reinterpret_cast<const int *>(meta + o_off);
   const int *d_sb = reinterpret_cast<const int *>(meta + o_sb);
   const int64_t *d_or = reinterpret_cast<const int64_t *>(meta + o_or);
-  const int64_t *d_db = reinterpret_cast<const int64_t *>(meta + o_db);
+  int64_t *d_db = reinterpret_cast<int64_t *>(meta + o_db);
   int *d_src = reinterpret_cast<int *>(meta + o_src);
   int64_t *d_rows = reinterpret_cast<int64_t *>(meta + o_rows);
 
@@ -615,6 +1041,7 @@ static int solve_parts_path(const fap_graph_dev *g, const int32_t *d_rro, const 
   const bool vec = (ld_out % 4 == 0) && ((uintptr_t)d_out % 16 == 0);
   // 1. boundary rows: distances by SSSP, then their predecessors.
   FAP_TRY(span(0, [&]() -> int {
+    FAP_TRY(fap_hip_fill_i32(d_stat, -1, kMaxSlots * kSlotStride, st));
     if (src_total == 0) return FAP_OK;
     FAP_TRY(launch_gather_sources(d_offsets, d_sb, d_or, P, src_total, g->d_st2ed, ld_out, d_src,
                                   d_rows, st));
@@ -633,51 +1060,176 @@ static int solve_parts_path(const fap_graph_dev *g, const int32_t *d_rro, const 
     for (int k0 = 0; k0 < src_total; k0 += 65535) {
       const int rows = std::min(65535, src_total - k0);
       hipLaunchKernelGGL(pred_rows_kernel, dim3((n + 255) / 256, rows), dim3(256), 0, st, d_rro,
-                         d_rcol, d_rw, n, d_src, d_rows, k0, d_out, d_path);
+                         d_rcol, d_rw, n, d_src, d_rows, k0, d_out, d_path, d_stat);
       FAP_CHECK_LAUNCH();
     }
     return FAP_OK;
   }));
   // 2. sub-matrix build with paths.
+  // The Floyd-Warshall of a part is keyed when (s - 1) * (largest finite built entry)
+  // stays below kFkLim (read back here): no simple path in its matrix is longer.
+  int fw_keyed = 0;
   FAP_TRY(span(1, [&]() -> int {
+    FAP_TRY(fap_hip_fill_i32(d_pmax, -1, P, st));
     hipLaunchKernelGGL(build_path_kernel, dim3(rows_total), dim3(256), 0, st, d_pp, P, rows_total,
                        g->d_row_offset, g->d_col, g->d_w, g->d_graph_id, g->d_st2ed, g->d_ed2st,
-                       d_out, d_path, ld_out);
+                       d_out, d_path, ld_out, d_pmax);
     FAP_CHECK_LAUNCH();
+    std::vector<int> pm(P), fk(P);
+    FAP_CHECK_HIP(hipMemcpyAsync(pm.data(), d_pmax, 4LL * P, hipMemcpyDeviceToHost, st));
+    FAP_CHECK_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < P; ++k) {
+      fk[k] = (int64_t)std::max(act[k].num - 1, 0) * std::max(pm[k], 0) < (int64_t)kFkLim;
+      fw_keyed += fk[k];
+    }
+    FAP_CHECK_HIP(hipMemcpyAsync(d_fwkey, fk.data(), 4LL * P, hipMemcpyHostToDevice, st));
+    FAP_CHECK_HIP(hipStreamSynchronize(st));
     return FAP_OK;
   }));
   // 3. Floyd-Warshall with k-i-j path semantics, all parts per launch.
   FAP_TRY(span(2, [&]() -> int {
     for (int r = 0; r < max_nb; ++r) {
-      hipLaunchKernelGGL(fwp_phase1_kernel, dim3(P), dim3(256), 0, st, d_pp, r);
+      hipLaunchKernelGGL(fwp_phase1_kernel, dim3(P), dim3(256), 0, st, d_pp, d_fwkey, r);
       FAP_CHECK_LAUNCH();
-      if (max_nb > 1) {
-        hipLaunchKernelGGL(fwp_phase2_kernel, dim3(2 * (max_nb - 1), P), dim3(256), 0, st, d_pp,
-                           r);
+      if (max_nb == 1) continue;
+      const dim3 g2(2 * (max_nb - 1), P), g3((max_nb - 1) * (max_nb - 1), P);
+      if (fw_keyed > 0) {
+        const dim3 gp(max_nb - 1, P);
+        hipLaunchKernelGGL(fwp_colpanel_key_kernel, gp, dim3(64), 0, st, d_pp, d_fwkey, r);
         FAP_CHECK_LAUNCH();
-        hipLaunchKernelGGL(fwp_phase3_kernel, dim3((max_nb - 1) * (max_nb - 1), P), dim3(256), 0,
-                           st, d_pp, r);
+        hipLaunchKernelGGL(fwp_rowpanel_key_kernel, gp, dim3(64), 0, st, d_pp, d_fwkey, r);
+        FAP_CHECK_LAUNCH();
+      }
+      if (fw_keyed < P) {
+        hipLaunchKernelGGL(fwp_phase2_kernel, g2, dim3(256), 0, st, d_pp, d_fwkey, r);
+        FAP_CHECK_LAUNCH();
+      }
+      if (fw_keyed > 0) {
+        hipLaunchKernelGGL(fwp_phase3_key_kernel, g3, dim3(256), 0, st, d_pp, d_fwkey, r);
+        FAP_CHECK_LAUNCH();
+      }
+      if (fw_keyed < P) {
+        hipLaunchKernelGGL(fwp_phase3_kernel, g3, dim3(256), 0, st, d_pp, d_fwkey, r);
         FAP_CHECK_LAUNCH();
       }
     }
     return FAP_OK;
   }));
-  // 4. min-plus of the inner rows with paths, own columns decoded.
+  // 4. min-plus of the inner rows with paths.  A part takes the keyed form when its
+  // boundary is 1..32 vertices and its largest finite A entry plus the largest finite
+  // boundary-row entry stays below the key limit (both read back here), the
+  // compare-and-select form (own columns decoded in its epilogue) otherwise.
+  std::vector<char> keyed(P, 0);
+  int merge_keyed = 0;
   FAP_TRY(span(3, [&]() -> int {
-    if (units == 0) return FAP_OK;
-    hipLaunchKernelGGL(merge_path_kernel, dim3(units), dim3(64), 0, st, d_pp, d_md,
-                       (int)md.size(), units, n, g->d_graph_id, g->d_ed2st, d_out, d_path, ld_out);
+    if (merge_parts == 0) return FAP_OK;
+    // Key classes by boundary size: (KP, KB) = (8, 3), (16, 4), (32, 5).
+    auto kp_of = [&](int k) {
+      const int K = act[k].bdy;
+      if (act[k].num <= K || K < 1 || K > 32) return 0;
+      return K <= 8 ? 8 : K <= 16 ? 16 : 32;
+    };
+    std::vector<int64_t> akey_off(P, -1);
+    std::vector<int> akey_kp(P);
+    int64_t akey_total = 0;
+    for (int k = 0; k < P; ++k) {
+      akey_kp[k] = kp_of(k);
+      if (akey_kp[k] == 0) continue;
+      akey_off[k] = akey_total;
+      akey_total += (int64_t)(act[k].num - act[k].bdy) * akey_kp[k];
+    }
+    FAP_TRY(g_ws_pakey.ensure(4 * (akey_total + kMkWaves * 32)));  // + rows of slack
+    uint32_t *d_akey = static_cast<uint32_t *>(g_ws_pakey.ptr);
+    int *d_amax = d_stat + kMaxSlots * kSlotStride;
+    std::vector<char> up(12LL * P);
+    std::memcpy(up.data(), akey_off.data(), 8LL * P);
+    std::memcpy(up.data() + 8LL * P, akey_kp.data(), 4LL * P);
+    FAP_CHECK_HIP(hipMemcpyAsync(d_akoff, up.data(), 12LL * P, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(path_amax_kernel, dim3(P), dim3(256), 0, st, d_pp, d_amax, d_akey, d_akoff,
+                       reinterpret_cast<const int *>(d_akoff + P));
     FAP_CHECK_LAUNCH();
+    std::vector<int> stat(nstat);
+    FAP_CHECK_HIP(hipMemcpyAsync(stat.data(), d_stat, 4LL * nstat, hipMemcpyDeviceToHost, st));
+    FAP_CHECK_HIP(hipStreamSynchronize(st));
+    int bmax = -1;
+    for (int q = 0; q < kMaxSlots; ++q) bmax = std::max(bmax, stat[q * kSlotStride]);
+    const int *amax = stat.data() + kMaxSlots * kSlotStride;
+    std::vector<PathMergeDesc> md;
+    std::vector<PathKeyDesc> kd[3];
+    int units = 0, kunits[3] = {0, 0, 0}, kmax[3] = {0, 0, 0};
+    const int cblocks = (n + kMkCols - 1) / kMkCols;
+    for (int k = 0; k < P; ++k) {
+      const int inner = act[k].num - act[k].bdy, K = act[k].bdy;
+      if (inner <= 0) continue;
+      const int kp = akey_kp[k], cls = kp == 8 ? 0 : kp == 16 ? 1 : 2;
+      const int64_t lim = (1LL << (31 - (cls + 3))) - 1;
+      keyed[k] = kp > 0 && (int64_t)std::max(amax[k], 0) + std::max(bmax, 0) < lim;
+      if (keyed[k]) {
+        kd[cls].push_back(PathKeyDesc{k, kunits[cls], cblocks, 0, d_akey + akey_off[k]});
+        kunits[cls] += (inner + kMkRows - 1) / kMkRows * cblocks;
+        kmax[cls] = std::max(kmax[cls], K);
+      } else {
+        const int rbk = (inner + kMpRows - 1) / kMpRows;
+        md.push_back(PathMergeDesc{k, units, rbk});
+        units += rbk * ((n + kMpCols - 1) / kMpCols);
+      }
+      FAP_REQUIRE(units < (1 << 30) && kunits[cls] < (1 << 30),
+                  "fap_hip_solve_parts_path: too many merge units");
+    }
+    // Descriptor lists: compare form at d_md, key classes after one another at d_kd.
+    const size_t nk = kd[0].size() + kd[1].size() + kd[2].size();
+    std::vector<char> desc(sizeof(PathMergeDesc) * md.size() + sizeof(PathKeyDesc) * nk);
+    if (!md.empty()) std::memcpy(desc.data(), md.data(), sizeof(PathMergeDesc) * md.size());
+    char *kdst = desc.data() + sizeof(PathMergeDesc) * md.size();
+    for (int c = 0; c < 3; ++c) {
+      if (kd[c].empty()) continue;
+      std::memcpy(kdst, kd[c].data(), sizeof(PathKeyDesc) * kd[c].size());
+      kdst += sizeof(PathKeyDesc) * kd[c].size();
+    }
+    if (!md.empty())
+      FAP_CHECK_HIP(hipMemcpyAsync(d_md, desc.data(), sizeof(PathMergeDesc) * md.size(),
+                                   hipMemcpyHostToDevice, st));
+    if (nk > 0)
+      FAP_CHECK_HIP(hipMemcpyAsync(d_kd, desc.data() + sizeof(PathMergeDesc) * md.size(),
+                                   sizeof(PathKeyDesc) * nk, hipMemcpyHostToDevice, st));
+    FAP_CHECK_HIP(hipStreamSynchronize(st));
+    merge_keyed = (int)nk;
+    const PathKeyDesc *kbase = d_kd;
+    for (int c = 0; c < 3; ++c) {
+      if (kunits[c] == 0) continue;
+      const size_t lds = (size_t)kmax[c] * 4 * 64 * sizeof(int);
+      auto kern = c == 0 ? merge_path_key_kernel<8, 3>
+                  : c == 1 ? merge_path_key_kernel<16, 4> : merge_path_key_kernel<32, 5>;
+      hipLaunchKernelGGL(kern, dim3(kunits[c]), dim3(64 * kMkWaves), lds, st, d_pp, kbase,
+                         (int)kd[c].size(), kunits[c], n, d_out, d_path, ld_out);
+      FAP_CHECK_LAUNCH();
+      kbase += kd[c].size();
+    }
+    if (units > 0) {
+      hipLaunchKernelGGL(merge_path_kernel, dim3(units), dim3(64), 0, st, d_pp, d_md,
+                         (int)md.size(), units, n, g->d_graph_id, g->d_ed2st, d_out, d_path,
+                         ld_out);
+      FAP_CHECK_LAUNCH();
+    }
     return FAP_OK;
   }));
-  // 5. boundary rows' own columns.
+  // 5. own columns of the boundary rows, and of every row of a keyed part.
+  std::vector<int64_t> dec_begin(P);
+  int64_t dec_total = 0;
+  for (int k = 0; k < P; ++k) {
+    dec_begin[k] = dec_total;
+    dec_total += (int64_t)(keyed[k] ? act[k].num : act[k].bdy) * act[k].num;
+  }
   FAP_TRY(span(4, [&]() -> int {
     if (dec_total == 0) return FAP_OK;
+    FAP_CHECK_HIP(hipMemcpyAsync(d_db, dec_begin.data(), 8LL * P, hipMemcpyHostToDevice, st));
+    FAP_CHECK_HIP(hipStreamSynchronize(st));
     hipLaunchKernelGGL(decode_path_kernel, dim3((unsigned)((dec_total + 255) / 256)), dim3(256),
                        0, st, d_pp, P, d_db, dec_total, g->d_st2ed, d_out, d_path, ld_out);
     FAP_CHECK_LAUNCH();
     return FAP_OK;
   }));
+  timing_note_keyed(merge_keyed, fw_keyed);
   return timing_finish(st);
 }
 

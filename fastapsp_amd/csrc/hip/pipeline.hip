@@ -128,7 +128,7 @@ struct Span {
 std::vector<Span> g_spans;
 hipEvent_t g_total[2] = {nullptr, nullptr};
 constexpr int kChunks = 8;
-int g_parts16 = 0, g_redone = 0, g_bgraph = 0;
+int g_parts16 = 0, g_redone = 0, g_bgraph = 0, g_keyed = 0, g_fw_keyed = 0;
 
 // What the last pass left for the flag check of the 16-bit form.
 struct Pass16 {
@@ -141,12 +141,17 @@ int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 }  // namespace
 
+void timing_note_keyed(int merge_parts, int fw_parts) {
+  g_keyed = merge_parts;
+  g_fw_keyed = fw_parts;
+}
+
 // Stage timing shared with the float-weight pipeline (fp64.hip).
 int timing_start(hipStream_t st) {
   g_events.used = 0;
   g_spans.clear();
   g_total[0] = g_total[1] = nullptr;
-  g_parts16 = g_redone = g_bgraph = 0;
+  g_parts16 = g_redone = g_bgraph = g_keyed = g_fw_keyed = 0;
   FAP_TRY(g_events.take(&g_total[0]));
   FAP_CHECK_HIP(hipEventRecord(g_total[0], st));
   return FAP_OK;
@@ -666,6 +671,8 @@ extern "C" int fap_hip_last_stage_times(fap_stage_times *t) {
   acc.parts_fw16 = g_parts16;
   acc.parts_redone = g_redone;
   acc.boundary_graph = g_bgraph;
+  acc.parts_keyed = g_keyed;
+  acc.parts_fw_keyed = g_fw_keyed;
   *t = acc;
   return FAP_OK;
 }

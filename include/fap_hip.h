@@ -183,6 +183,8 @@ typedef struct fap_stage_times {
   int32_t parts_fw16;   /* parts whose Floyd-Warshall ran in the 16-bit form */
   int32_t parts_redone; /* of those, parts redone in int32 (a distance reached the cap) */
   int32_t boundary_graph; /* 1: boundary rows from the boundary graph, 0: boundary SSSP */
+  int32_t parts_keyed;  /* path solves: parts whose min-plus ran in the keyed form */
+  int32_t parts_fw_keyed; /* path solves: parts whose Floyd-Warshall ran in the keyed form */
 } fap_stage_times;
 
 /* The whole per-part pipeline of fapGraph::run (fap/fap.cpp:195-503) for a list

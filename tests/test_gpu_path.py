@@ -1,7 +1,12 @@
 """Predecessor (path) matrices: solve(True) / solveSubGraph(id, True) on the GPU must
 equal the oracle's restatement of the reference CPU path (Dijkstra heap order for the
 boundary rows, k-i-j Floyd-Warshall with strict '>', first-k min-plus, decode) bit for
-bit -- paths and distances -- on graphs with many equal-length paths."""
+bit -- paths and distances -- on graphs with many equal-length paths.  The min-plus
+runs in the keyed form (sum and k in one 32-bit key) when a part's boundary has at
+most 32 vertices and its sums stay below the key limit, in the compare-and-select
+form otherwise; both forms, and calls that mix them, are checked.  The same holds for
+the panel phases of the Floyd-Warshall (keyed while (s - 1) * largest built entry stays
+below 2^24 - 1)."""
 import sys
 
 import numpy as np
@@ -39,8 +44,19 @@ def _valid(dist, path, src, csr):
                 assert (u, v) in arcw and dist[r, u] + arcw[(u, v)] == d, (r, v, u)
 
 
-def _check_solve(g, validate_rows=64):
+def _keyable(lay):
+    """Parts with inner rows and a boundary of 1..32 vertices."""
+    num, bdy = np.asarray(lay["num"][1:]), np.asarray(lay["bdy"][1:])
+    return int(((num > bdy) & (bdy >= 1) & (bdy <= 32)).sum())
+
+
+def _check_solve(g, validate_rows=64, keyed=None, fw_keyed=None):
     g.solve(True)
+    info = L.last_stage_times()
+    if keyed is not None:
+        assert info["parts_keyed"] == keyed, (info["parts_keyed"], keyed)
+    if fw_keyed is not None:
+        assert info["parts_fw_keyed"] == fw_keyed, (info["parts_fw_keyed"], fw_keyed)
     got_d, got_p = g.getAllDistanceI32(), g.getAllPath()
     lay, csr = g.layout(), g.csr()
     want_d, want_p = O.solve_parts_path(1, g.K, csr, lay)
@@ -63,7 +79,9 @@ def test_paths_road_grid(args):
     say(f"paths: road grid {args}")
     g = F.FapGraph.road_grid(*args)
     g.preCondition()
-    _check_solve(g)
+    # Boundaries of 36..79 (compare form), 10..27 (keyed, 5 tag bits), none (K = 1);
+    # unit weights: every part's Floyd-Warshall is keyed.
+    _check_solve(g, keyed=_keyable(g.layout()), fw_keyed=g.K)
 
 
 def test_paths_directed_weighted():
@@ -89,7 +107,9 @@ def test_paths_knn():
     say("paths: knn directed weighted")
     g = F.FapGraph.knn(1800, 3, 20, 7, 5)
     g.preCondition()
-    _check_solve(g)
+    keyed = _keyable(g.layout())  # boundaries of 7..13: keyed, 4 tag bits
+    assert keyed == g.K
+    _check_solve(g, keyed=keyed)
 
 
 def test_paths_parts_disconnected_inside():
@@ -132,3 +152,115 @@ def test_paths_refused_for_float_weights():
     g.preCondition()
     with pytest.raises(L.FapError):
         g.solve(True)
+
+
+def _grid(W, H, w=None):
+    """W x H 4-neighbour grid, vertex y * W + x, unit weights unless `w` (per arc)."""
+    out = [[] for _ in range(W * H)]
+    for y in range(H):
+        for x in range(W):
+            v = y * W + x
+            if x + 1 < W:
+                out[v].append(v + 1)
+                out[v + 1].append(v)
+            if y + 1 < H:
+                out[v].append(v + W)
+                out[v + W].append(v)
+    ro = np.zeros(W * H + 1, np.int32)
+    ro[1:] = np.cumsum([len(x) for x in out])
+    col = np.array([v for x in out for v in x], np.int32)
+    wt = None if w is None else np.full(len(col), w, np.float32)
+    return ro, col, wt
+
+
+def test_paths_keyed_and_compare_forms_in_one_call():
+    # 48 x 34 grid: rows 0-1 form part 1 (boundary = row 1, 48 vertices: compare form),
+    # rows 2..33 are 8 x 8 blocks (boundary <= 28: keyed form).  Grids have many
+    # equal-length paths, so the first-k tie rule is exercised in both forms.
+    say("paths: keyed and compare forms in one call (grid 48x34, 25 parts)")
+    W, H = 48, 34
+    ro, col, _ = _grid(W, H)
+    y, x = np.divmod(np.arange(W * H), W)
+    gid = np.where(y < 2, 1, 2 + ((y - 2) // 8) * (W // 8) + x // 8).astype(np.int32)
+    g = F.FapGraph.from_csr(ro, col, None, False, int(gid.max()))
+    g.setPartition(gid)
+    g.preCondition()
+    lay = g.layout()
+    assert lay["bdy"][1] == W
+    keyed = _keyable(lay)
+    assert keyed == int(gid.max()) - 1
+    _check_solve(g, keyed=keyed)
+
+
+@pytest.mark.parametrize("side", ["below", "above"])
+def test_paths_key_limit(side):
+    # Path graph of 64 vertices in 4 segments, all arcs of weight w: part p's largest
+    # A entry is a_p * w and the largest boundary-row entry b * w (hop counts from the
+    # unit-weight solve: a_p is 15 in the end segments, 14 in the middle ones).  Part p
+    # takes the keyed form while (a_p + b) * w < 2^28 - 1 (at most 8 boundary
+    # vertices: 3 tag bits); w is set at the limit of the end segments.  Such distances lie far beyond the
+    # oracle's float MAXVALUE (1e8), so both sides are checked against the closed
+    # form: d(s, v) = |s - v| * w, predecessor v - 1 (v > s) or v + 1 (v < s).
+    n, K, lim = 64, 4, (1 << 28) - 1
+    ro, col, _ = _grid(n, 1)
+    gid = (np.arange(n) // 16 + 1).astype(np.int32)
+    unit = F.FapGraph.from_csr(ro, col, None, False, K)
+    unit.setPartition(gid)
+    unit.preCondition()
+    unit.solve()
+    lay = unit.layout()
+    d = unit.getAllDistanceI32()  # rows in sorted order, columns by vertex id
+    st2ed = lay["st2ed"]
+    ap, b = [], 0
+    for p in range(1, K + 1):
+        off, num, bd = lay["offset"][p], lay["num"][p], lay["bdy"][p]
+        bverts = st2ed[off:off + bd]
+        b = max(b, int(d[off:off + bd].max()))
+        ap.append(int(d[off + bd:off + num][:, bverts].max()))
+    a = max(ap)
+    w = (lim - 1) // (a + b) + (0 if side == "below" else 1)
+    assert ((a + b) * w < lim) == (side == "below")
+    keyed = sum((x + b) * w < lim for x in ap)
+    assert keyed == (K if side == "below" else 2)
+    say(f"paths: key limit {side}: a={a} b={b} w={w}")
+    ro, col, wt = _grid(n, 1, float(w))
+    assert float(np.float32(w)) == w  # exact in the float32 weight
+    g = F.FapGraph.from_csr(ro, col, wt, False, K)
+    g.setPartition(gid)
+    g.preCondition()
+    g.solve(True)
+    assert L.last_stage_times()["parts_keyed"] == keyed
+    got_d, got_p = g.getAllDistanceI32(), g.getAllPath()
+    src = g.layout()["st2ed"].astype(np.int64)[:, None]
+    v = np.arange(n)[None, :]
+    assert np.array_equal(got_d, (np.abs(src - v) * w).astype(np.int32))
+    want_p = np.where(v > src, v - 1, np.where(v < src, v + 1, src))
+    assert np.array_equal(got_p, want_p.astype(np.int32))
+
+
+def test_paths_fw_compare_form():
+    # knn with weights up to 100000: (s - 1) * largest built entry exceeds 2^24 - 1, so
+    # the Floyd-Warshall panels run in the compare form (distances stay far below 2^24,
+    # where the oracle's float arithmetic is exact).
+    say("paths: knn, weights up to 1e5 (compare-form Floyd-Warshall)")
+    g = F.FapGraph.knn(1800, 3, 100000, 7, 5)
+    g.preCondition()
+    d, _ = _check_solve(g, fw_keyed=0)
+    assert d[d < F.FAP_INF].max() < (1 << 24)
+
+
+def test_paths_fw_forms_in_one_call():
+    # The 48 x 34 grid of the mixed merge test with weight 200000 on the arcs inside
+    # part 1 (rows 0-1): part 1's Floyd-Warshall runs in the compare form, the 24
+    # unit-weight blocks keyed.
+    say("paths: keyed and compare Floyd-Warshall in one call")
+    W, H, heavy = 48, 34, 200000
+    ro, col, _ = _grid(W, H)
+    y, x = np.divmod(np.arange(W * H), W)
+    gid = np.where(y < 2, 1, 2 + ((y - 2) // 8) * (W // 8) + x // 8).astype(np.int32)
+    src = np.repeat(np.arange(W * H), np.diff(ro))
+    wt = np.where((gid[src] == 1) & (gid[col] == 1), heavy, 1).astype(np.float32)
+    g = F.FapGraph.from_csr(ro, col, wt, False, int(gid.max()))
+    g.setPartition(gid)
+    g.preCondition()
+    _check_solve(g, keyed=int(gid.max()) - 1, fw_keyed=int(gid.max()) - 1)
