@@ -178,8 +178,11 @@ __global__ __launch_bounds__(256) void merge_kernel(const MergeDesc *__restrict_
 // the product is a write stream of M x N int32, so one wave keeps its B panel
 // (KP x 256 columns, KP = K rounded up to a multiple of 4, FAP_INF padding) in
 // registers, reads each A row wave-uniformly (scalar loads) and writes one
-// contiguous 1 KB row segment per row.  Work unit: one 256-column block x
-// kStRows rows of one product; one launch per KP group.  Host requirements:
+// contiguous 1 KB row segment per row.  Lane l holds columns cb * 256 + l + 64 e
+// (e < 4), so each of its four dword stores covers 256 contiguous bytes: measured
+// in one process on MI355X, into the same 52.5 GB result buffer, against 4 adjacent
+// columns per lane with one dwordx4 store: 10.69 ms against 10.82.  Work unit: one
+// 256-column block x kStRows rows of one product; one launch per KP group.  Host requirements:
 // K <= KP and every A row holds KP readable entries (rows of the padded
 // Floyd-Warshall matrices, stride >= 64).  Entries A[i][k] with K <= k < KP are
 // distances in [0, FAP_INF] and meet B = FAP_INF, so they never win a minimum.
@@ -242,18 +245,13 @@ __device__ __forceinline__ void stream_row16(int (&acc)[4], const int (&ai)[KP],
   for (int e = 0; e < 4; ++e) acc[e] = v[e] >= (unsigned)kCap16 ? kInf : (int)v[e];
 }
 
-template <bool kVec>
+// Columns j + 64 e.  Non-temporal: on MI355X the merge measured 1 ms slower with
+// plain stores.
 __device__ __forceinline__ void stream_store(FAP_GLOBAL int *crow, int j, int N,
                                              const int (&acc)[4]) {
-  if (kVec && j + 3 < N) {
-    v4i v = {acc[0], acc[1], acc[2], acc[3]};
-    // Non-temporal: on MI355X the merge measured 1 ms slower with plain stores.
-    __builtin_nontemporal_store(v, reinterpret_cast<FAP_GLOBAL v4i *>(crow + j));
-  } else {
 #pragma unroll
-    for (int e = 0; e < 4; ++e)
-      if (j + e < N) crow[j + e] = acc[e];
-  }
+  for (int e = 0; e < 4; ++e)
+    if (j + 64 * e < N) __builtin_nontemporal_store(acc[e], crow + j + 64 * e);
 }
 
 template <int KP, bool kOwn>
@@ -276,7 +274,7 @@ __device__ __forceinline__ void load_arow(int (&a)[KP], const FAP_CONST int *aro
   for (int k = 0; k < KP; ++k) a[k] = arow[k];
 }
 
-template <int KP, bool kVec, bool kOwn>
+template <int KP, bool kOwn>
 __device__ __forceinline__ void stream_rows(const MergeDesc &d, int row0, int row1, int j, int N,
                                             const int (&b)[KP][4], const int (&own)[4]) {
   FAP_GLOBAL int *crow = gptr(d.C + (int64_t)row0 * d.ldc);
@@ -291,11 +289,11 @@ __device__ __forceinline__ void stream_rows(const MergeDesc &d, int row0, int ro
     int acc[4];
     stream_row<KP>(acc, a, b);
     stream_own<KP, kOwn>(acc, d, i, own);
-    stream_store<kVec>(crow, j, N, acc);
+    stream_store(crow, j, N, acc);
   }
 }
 
-template <int KP, bool kVec, bool kOwn>
+template <int KP, bool kOwn>
 __device__ __forceinline__ void stream_rows16(const MergeDesc &d, int row0, int row1, int j,
                                               int N, const unsigned (&bp)[KP][2],
                                               const int (&own)[4]) {
@@ -311,14 +309,14 @@ __device__ __forceinline__ void stream_rows16(const MergeDesc &d, int row0, int 
     int acc[4];
     stream_row16<KP>(acc, a, bp);
     stream_own<KP, kOwn>(acc, d, i, own);
-    stream_store<kVec>(crow, j, N, acc);
+    stream_store(crow, j, N, acc);
   }
 }
 
 // max_b / lim16: when max_b is set and *max_b <= lim16 (every finite B below
 // lim16 + 1, every finite A at most kCap16 - 1 - lim16, both guaranteed by the
 // caller), the rows run on packed 16-bit distances.
-template <int KP, bool kVec>
+template <int KP>
 __global__ __launch_bounds__(64) void merge_stream_kernel(const MergeDesc *__restrict__ descs,
                                                           int nd, int total_units, int N,
                                                           const int *__restrict__ graph_id,
@@ -340,21 +338,15 @@ __global__ __launch_bounds__(64) void merge_stream_kernel(const MergeDesc *__res
   }
   const int row0 = rb * rpu;
   const int row1 = min(row0 + rpu, d.M);
-  const int j = cb * kStCols + threadIdx.x * 4;
+  const int j = cb * kStCols + threadIdx.x;  // columns j + 64 e
 
   int b[KP][4];
 #pragma unroll
   for (int k = 0; k < KP; ++k) {
     if (k < d.K) {
       const FAP_GLOBAL int *brow = gptr(d.B + (int64_t)k * d.ldb);
-      if (kVec && j + 3 < N) {
-        const v4i v = *reinterpret_cast<const FAP_GLOBAL v4i *>(brow + j);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) b[k][e] = v[e];
-      } else {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) b[k][e] = (j + e < N) ? brow[j + e] : kInf;
-      }
+      for (int e = 0; e < 4; ++e) b[k][e] = (j + 64 * e < N) ? brow[j + 64 * e] : kInf;
     } else {
 #pragma unroll
       for (int e = 0; e < 4; ++e) b[k][e] = kInf;
@@ -365,8 +357,8 @@ __global__ __launch_bounds__(64) void merge_stream_kernel(const MergeDesc *__res
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     own[e] = -1;
-    if (d.part_id > 0 && j + e < N && graph_id[j + e] == d.part_id) {
-      own[e] = ed2st[j + e] - d.offset;
+    if (d.part_id > 0 && j + 64 * e < N && graph_id[j + 64 * e] == d.part_id) {
+      own[e] = ed2st[j + 64 * e] - d.offset;
       any_own = true;
     }
   }
@@ -383,13 +375,13 @@ __global__ __launch_bounds__(64) void merge_stream_kernel(const MergeDesc *__res
         bp[k][h] = (unsigned)min(b[k][2 * h], kCap16) |
                    ((unsigned)min(b[k][2 * h + 1], kCap16) << 16);
     if (none_own)
-      stream_rows16<KP, kVec, false>(d, row0, row1, j, N, bp, own);
+      stream_rows16<KP, false>(d, row0, row1, j, N, bp, own);
     else
-      stream_rows16<KP, kVec, true>(d, row0, row1, j, N, bp, own);
+      stream_rows16<KP, true>(d, row0, row1, j, N, bp, own);
   } else if (none_own) {
-    stream_rows<KP, kVec, false>(d, row0, row1, j, N, b, own);
+    stream_rows<KP, false>(d, row0, row1, j, N, b, own);
   } else {
-    stream_rows<KP, kVec, true>(d, row0, row1, j, N, b, own);
+    stream_rows<KP, true>(d, row0, row1, j, N, b, own);
   }
 }
 
@@ -406,21 +398,16 @@ int merge_stream_kpad(int K) { return K <= 0 ? 4 : (K + 3) / 4 * 4; }
 
 // kp: padded K, a multiple of 4 in [4, 32].
 int launch_merge_stream(const MergeDesc *d_descs, int nd, int total_units, int N, int kp,
-                        bool vec, const int *d_graph_id, const int *d_ed2st, const int *d_max_b,
-                        int lim16, hipStream_t stream) {
+                        const int *d_graph_id, const int *d_ed2st, const int *d_max_b, int lim16,
+                        hipStream_t stream) {
   if (nd <= 0 || total_units <= 0 || N <= 0) return FAP_OK;
   const int grid = (total_units + 7) / 8 * 8;
   const StreamShape sh = stream_shape();
 #define FAP_STREAM_CASE(KP)                                                                   \
   case KP:                                                                                    \
-    if (vec)                                                                                  \
-      hipLaunchKernelGGL((merge_stream_kernel<KP, true>), dim3(grid), dim3(64), 0, stream,    \
-                         d_descs, nd, total_units, N, d_graph_id, d_ed2st, d_max_b, lim16,    \
-                         sh.rows, sh.col_major);                                               \
-    else                                                                                      \
-      hipLaunchKernelGGL((merge_stream_kernel<KP, false>), dim3(grid), dim3(64), 0, stream,   \
-                         d_descs, nd, total_units, N, d_graph_id, d_ed2st, d_max_b, lim16,    \
-                         sh.rows, sh.col_major);                                               \
+    hipLaunchKernelGGL((merge_stream_kernel<KP>), dim3(grid), dim3(64), 0, stream, d_descs,    \
+                       nd, total_units, N, d_graph_id, d_ed2st, d_max_b, lim16, sh.rows,      \
+                       sh.col_major);                                                          \
     break;
   switch (kp) {
     FAP_STREAM_CASE(4)

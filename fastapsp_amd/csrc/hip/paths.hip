@@ -148,7 +148,7 @@ __global__ __launch_bounds__(256) void pred_rows_kernel(
 
 // ---- build ----------------------------------------------------------------------
 // One workgroup per padded row of every part (rows of PathPart::row_begin prefix).
-// pmax[part] = max(pmax[part], largest finite entry of the built rows).
+// pmax[part * kSlotStride] = max(itself, largest finite entry of the built rows).
 __global__ __launch_bounds__(256) void build_path_kernel(
     const PathPart *__restrict__ parts, int np, int total_rows, const int *__restrict__ row_off,
     const int *__restrict__ col, const int *__restrict__ w, const int *__restrict__ graph_id,
@@ -201,12 +201,18 @@ __global__ __launch_bounds__(256) void build_path_kernel(
     dp[j] = rp[j];
     if (rd[j] < kInf) mx = max(mx, rd[j]);
   }
-  __shared__ int red;
-  if (tid == 0) red = -1;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off));
+  __shared__ int red[4];
+  if ((tid & 63) == 0) red[tid >> 6] = mx;
   __syncthreads();
-  atomicMax(&red, mx);
-  __syncthreads();
-  if (tid == 0 && red >= 0) atomicMax(pmax + pi, red);
+  if (tid == 0) {
+    mx = max(max(red[0], red[1]), max(red[2], red[3]));
+    // Most rows hold nothing above what their part's word already has: read first,
+    // so that only a new maximum takes the (serialised, same-line) atomic.
+    int *slot = pmax + (int64_t)pi * kSlotStride;
+    if (mx > __atomic_load_n(slot, __ATOMIC_RELAXED)) atomicMax(slot, mx);
+  }
 }
 
 // ---- Floyd-Warshall with k-i-j path semantics -----------------------------------
@@ -900,25 +906,31 @@ __global__ __launch_bounds__(256) void merge_path_key_kernel(
 
 // Own columns: out[row][st2ed[offset + c]] = FW[row][c] (dist, path) for rows
 // 0 .. rows_p - 1 of each part: its boundary rows, and all its rows when the keyed
-// min-plus wrote the inner rows (item_begin: prefix of rows_p * s_p).
-__global__ void decode_path_kernel(const PathPart *__restrict__ parts, int np,
-                                   const int64_t *__restrict__ item_begin, int64_t total,
-                                   const int *__restrict__ st2ed, int *__restrict__ out,
-                                   int *__restrict__ pout, int64_t ld_out) {
-  const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (item >= total) return;
+// min-plus wrote the inner rows.  One workgroup per row (row_begin: prefix of rows_p).
+__global__ __launch_bounds__(256) void decode_path_kernel(const PathPart *__restrict__ parts,
+                                                          int np,
+                                                          const int64_t *__restrict__ row_begin,
+                                                          int64_t total,
+                                                          const int *__restrict__ st2ed,
+                                                          int *__restrict__ out,
+                                                          int *__restrict__ pout, int64_t ld_out) {
+  const int64_t row = blockIdx.x;
+  if (row >= total) return;
   int lo = 0, hi = np - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (item_begin[mid] <= item) lo = mid;
+    if (row_begin[mid] <= row) lo = mid;
     else hi = mid - 1;
   }
   const PathPart p = parts[lo];
-  const int64_t local = item - item_begin[lo];
-  const int i = (int)(local / p.s), c = (int)(local - (int64_t)i * p.s);
-  const int64_t dst = (p.out_row + i) * ld_out + st2ed[p.offset + c];
-  out[dst] = p.fw[(int64_t)i * p.S + c];
-  pout[dst] = p.fwp[(int64_t)i * p.S + c];
+  const int i = (int)(row - row_begin[lo]);
+  const int64_t orow = (p.out_row + i) * ld_out;
+  const int *fr = p.fw + (int64_t)i * p.S, *fpr = p.fwp + (int64_t)i * p.S;
+  for (int c = threadIdx.x; c < p.s; c += 256) {
+    const int v = st2ed[p.offset + c];
+    out[orow + v] = fr[c];
+    pout[orow + v] = fpr[c];
+  }
 }
 
 Workspace g_ws_pfw;    // FW distance and path matrices
@@ -1003,7 +1015,7 @@ static int solve_parts_path(const fap_graph_dev *g, const int32_t *d_rro, const 
   const int nstat = kMaxSlots * kSlotStride + P;  // boundary-row maximum slots, then A maxima
   const int64_t o_stat = carve(4LL * nstat);
   const int64_t o_akoff = carve(12LL * P);  // A key offsets (int64), then classes (int)
-  const int64_t o_pmax = carve(4LL * P);     // largest finite built entry per part
+  const int64_t o_pmax = carve(4LL * P * kSlotStride);  // largest finite built entry per part
   const int64_t o_fwkey = carve(4LL * P);    // 1: keyed Floyd-Warshall
   const int64_t o_src = carve(4LL * std::max(src_total, 1));
   const int64_t o_rows = carve(8LL * std::max(src_total, 1));
@@ -1070,16 +1082,18 @@ static int solve_parts_path(const fap_graph_dev *g, const int32_t *d_rro, const 
   // stays below kFkLim (read back here): no simple path in its matrix is longer.
   int fw_keyed = 0;
   FAP_TRY(span(1, [&]() -> int {
-    FAP_TRY(fap_hip_fill_i32(d_pmax, -1, P, st));
+    FAP_TRY(fap_hip_fill_i32(d_pmax, -1, (int64_t)P * kSlotStride, st));
     hipLaunchKernelGGL(build_path_kernel, dim3(rows_total), dim3(256), 0, st, d_pp, P, rows_total,
                        g->d_row_offset, g->d_col, g->d_w, g->d_graph_id, g->d_st2ed, g->d_ed2st,
                        d_out, d_path, ld_out, d_pmax);
     FAP_CHECK_LAUNCH();
-    std::vector<int> pm(P), fk(P);
-    FAP_CHECK_HIP(hipMemcpyAsync(pm.data(), d_pmax, 4LL * P, hipMemcpyDeviceToHost, st));
+    std::vector<int> pm((size_t)P * kSlotStride), fk(P);
+    FAP_CHECK_HIP(hipMemcpyAsync(pm.data(), d_pmax, 4LL * P * kSlotStride, hipMemcpyDeviceToHost,
+                                 st));
     FAP_CHECK_HIP(hipStreamSynchronize(st));
     for (int k = 0; k < P; ++k) {
-      fk[k] = (int64_t)std::max(act[k].num - 1, 0) * std::max(pm[k], 0) < (int64_t)kFkLim;
+      const int64_t mk = std::max(pm[(size_t)k * kSlotStride], 0);
+      fk[k] = (int64_t)std::max(act[k].num - 1, 0) * mk < (int64_t)kFkLim;
       fw_keyed += fk[k];
     }
     FAP_CHECK_HIP(hipMemcpyAsync(d_fwkey, fk.data(), 4LL * P, hipMemcpyHostToDevice, st));
@@ -1157,6 +1171,7 @@ static int solve_parts_path(const fap_graph_dev *g, const int32_t *d_rro, const 
     std::vector<PathMergeDesc> md;
     std::vector<PathKeyDesc> kd[3];
     int units = 0, kunits[3] = {0, 0, 0}, kmax[3] = {0, 0, 0};
+    int64_t rows[4] = {0, 0, 0, 0};  // inner rows per key class, then the compare form
     const int cblocks = (n + kMkCols - 1) / kMkCols;
     for (int k = 0; k < P; ++k) {
       const int inner = act[k].num - act[k].bdy, K = act[k].bdy;
@@ -1168,7 +1183,9 @@ static int solve_parts_path(const fap_graph_dev *g, const int32_t *d_rro, const 
         kd[cls].push_back(PathKeyDesc{k, kunits[cls], cblocks, 0, d_akey + akey_off[k]});
         kunits[cls] += (inner + kMkRows - 1) / kMkRows * cblocks;
         kmax[cls] = std::max(kmax[cls], K);
+        rows[cls] += inner;
       } else {
+        rows[3] += inner;
         const int rbk = (inner + kMpRows - 1) / kMpRows;
         md.push_back(PathMergeDesc{k, units, rbk});
         units += rbk * ((n + kMpCols - 1) / kMpCols);
@@ -1194,22 +1211,48 @@ static int solve_parts_path(const fap_graph_dev *g, const int32_t *d_rro, const 
                                    sizeof(PathKeyDesc) * nk, hipMemcpyHostToDevice, st));
     FAP_CHECK_HIP(hipStreamSynchronize(st));
     merge_keyed = (int)nk;
+    // The launches (key classes, compare form) write disjoint rows: they run side by
+    // side on up to three streams, largest first, so that one's tail overlaps another.
+    struct Launch {
+      int cls;  // 0..2 key class, 3 compare form
+      const PathKeyDesc *kd;
+      int64_t work;
+    };
+    std::vector<Launch> ls;
     const PathKeyDesc *kbase = d_kd;
     for (int c = 0; c < 3; ++c) {
-      if (kunits[c] == 0) continue;
-      const size_t lds = (size_t)kmax[c] * 4 * 64 * sizeof(int);
-      auto kern = c == 0 ? merge_path_key_kernel<8, 3>
-                  : c == 1 ? merge_path_key_kernel<16, 4> : merge_path_key_kernel<32, 5>;
-      hipLaunchKernelGGL(kern, dim3(kunits[c]), dim3(64 * kMkWaves), lds, st, d_pp, kbase,
-                         (int)kd[c].size(), kunits[c], n, d_out, d_path, ld_out);
-      FAP_CHECK_LAUNCH();
+      if (kunits[c] > 0) ls.push_back(Launch{c, kbase, rows[c]});
       kbase += kd[c].size();
     }
-    if (units > 0) {
-      hipLaunchKernelGGL(merge_path_kernel, dim3(units), dim3(64), 0, st, d_pp, d_md,
-                         (int)md.size(), units, n, g->d_graph_id, g->d_ed2st, d_out, d_path,
-                         ld_out);
+    if (units > 0) ls.push_back(Launch{3, nullptr, rows[3] * 2});  // slower per row
+    std::sort(ls.begin(), ls.end(), [](const Launch &x, const Launch &y) { return x.work > y.work; });
+    const hipStream_t lanes[3] = {st, aux_stream(0), aux_stream(1)};
+    const int nl = std::min<int>(3, (int)ls.size());
+    if (nl > 1) {
+      hipEvent_t fork;
+      FAP_TRY(timing_record(st, &fork));
+      for (int l = 1; l < nl; ++l) FAP_CHECK_HIP(hipStreamWaitEvent(lanes[l], fork, 0));
+    }
+    for (size_t q = 0; q < ls.size(); ++q) {
+      const hipStream_t on = lanes[q % nl];
+      const int c = ls[q].cls;
+      if (c < 3) {
+        const size_t lds = (size_t)kmax[c] * 4 * 64 * sizeof(int);
+        auto kern = c == 0 ? merge_path_key_kernel<8, 3>
+                    : c == 1 ? merge_path_key_kernel<16, 4> : merge_path_key_kernel<32, 5>;
+        hipLaunchKernelGGL(kern, dim3(kunits[c]), dim3(64 * kMkWaves), lds, on, d_pp, ls[q].kd,
+                           (int)kd[c].size(), kunits[c], n, d_out, d_path, ld_out);
+      } else {
+        hipLaunchKernelGGL(merge_path_kernel, dim3(units), dim3(64), 0, on, d_pp, d_md,
+                           (int)md.size(), units, n, g->d_graph_id, g->d_ed2st, d_out, d_path,
+                           ld_out);
+      }
       FAP_CHECK_LAUNCH();
+    }
+    for (int l = 1; l < nl; ++l) {
+      hipEvent_t joined;
+      FAP_TRY(timing_record(lanes[l], &joined));
+      FAP_CHECK_HIP(hipStreamWaitEvent(st, joined, 0));
     }
     return FAP_OK;
   }));
@@ -1218,14 +1261,14 @@ static int solve_parts_path(const fap_graph_dev *g, const int32_t *d_rro, const 
   int64_t dec_total = 0;
   for (int k = 0; k < P; ++k) {
     dec_begin[k] = dec_total;
-    dec_total += (int64_t)(keyed[k] ? act[k].num : act[k].bdy) * act[k].num;
+    dec_total += keyed[k] ? act[k].num : act[k].bdy;
   }
   FAP_TRY(span(4, [&]() -> int {
     if (dec_total == 0) return FAP_OK;
     FAP_CHECK_HIP(hipMemcpyAsync(d_db, dec_begin.data(), 8LL * P, hipMemcpyHostToDevice, st));
     FAP_CHECK_HIP(hipStreamSynchronize(st));
-    hipLaunchKernelGGL(decode_path_kernel, dim3((unsigned)((dec_total + 255) / 256)), dim3(256),
-                       0, st, d_pp, P, d_db, dec_total, g->d_st2ed, d_out, d_path, ld_out);
+    hipLaunchKernelGGL(decode_path_kernel, dim3((unsigned)dec_total), dim3(256), 0, st, d_pp, P,
+                       d_db, dec_total, g->d_st2ed, d_out, d_path, ld_out);
     FAP_CHECK_LAUNCH();
     return FAP_OK;
   }));

@@ -65,8 +65,8 @@ int merge_stream_units(int M, int N);
 int merge_stream_row_blocks(int M);
 int merge_stream_kpad(int K);
 int launch_merge_stream(const MergeDesc *d_descs, int nd, int total_units, int N, int kp,
-                        bool vec, const int *d_graph_id, const int *d_ed2st, const int *d_max_b,
-                        int lim16, hipStream_t stream);
+                        const int *d_graph_id, const int *d_ed2st, const int *d_max_b, int lim16,
+                        hipStream_t stream);
 int launch_build(const BuildDesc *d_descs, int nd, int total_rows, bool vec, const int *row_off,
                  const int *col, const int *w, const int *graph_id, const int *st2ed,
                  const int *ed2st, hipStream_t stream);
@@ -488,8 +488,8 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
     if (gr.kind == 0)
       return launch_merge(d_md + gr.first, gr.count, gr.units, n, vec, g->d_graph_id, g->d_ed2st,
                           on);
-    return launch_merge_stream(d_md + gr.first, gr.count, gr.units, n, gr.kind, vec,
-                               g->d_graph_id, g->d_ed2st,
+    return launch_merge_stream(d_md + gr.first, gr.count, gr.units, n, gr.kind, g->d_graph_id,
+                               g->d_ed2st,
                                merge16 && bg_total > 0 ? d_maxb : nullptr, lim16, on);
   };
 
