@@ -221,8 +221,13 @@ int fap_hip_last_stage_times(fap_stage_times *t);
  * (Dijkstra's heap order for the boundary rows, k-i-j order with strict '>' for
  * Floyd-Warshall, first k for the min-plus).  Integer weights only (g->d_wf NULL).
  * d_rrow_offset / d_rcol / d_rw: the reverse CSR (in-arcs of every vertex; d_rw NULL
- * exactly when g->d_w is NULL).  d_path has the layout of d_out.  One batched form;
- * asynchronous on `stream`.  Stage times as for fap_hip_solve_parts. */
+ * exactly when g->d_w is NULL).  d_path has the layout of d_out.  Weights must be
+ * non-negative.  One batched form.  Per part, the Floyd-Warshall panels and the
+ * min-plus run in a keyed form (distance and path source in one 32-bit key, one
+ * unsigned minimum per candidate) when the part's values stay below the key limits,
+ * in a compare-and-select form otherwise; the results are the same.  The call waits
+ * for the stream twice to read the maxima that choose the forms.  Stage times as for
+ * fap_hip_solve_parts, with parts_keyed / parts_fw_keyed counting the keyed parts. */
 int fap_hip_solve_parts_path(const fap_graph_dev *g, const int32_t *d_rrow_offset,
                              const int32_t *d_rcol, const int32_t *d_rw,
                              const fap_part_desc *parts, int32_t nparts, int32_t *d_out,
