@@ -381,20 +381,31 @@ def main():
     log(f"{args.mode}: {ms_per_step:.2f} ms/step, {value:.3e} entries/s")
 
     avg = {k: float(np.mean([s[k] for s in stages])) for k in stages[0]}
-    fw16 = stages[-1]["parts_fw16"] > 0
+    bfs = stages[-1]["parts_bfs"] > 0
     bform = "boundary graph" if stages[-1]["boundary_graph"] else "boundary SSSP"
+    inform = "bit-parallel BFS" if bfs else "Floyd-Warshall"
     log(f"stages (ms): boundary rows ({bform}) {avg['sssp_ms']:.2f} build {avg['build_ms']:.2f} "
-        f"fw {avg['floyd_ms']:.2f} merge {avg['minplus_ms']:.2f} decode {avg['decode_ms']:.2f}; "
-        f"16-bit FW parts {stages[-1]['parts_fw16']}, redone {stages[-1]['parts_redone']}")
-    # Floyd-Warshall stage: 2 ops (add, min) per step, sum over parts of s_p^3 steps,
-    # against the VALU peak of the arithmetic it ran in (packed 16-bit: two entries
-    # per lane-op).
-    fw_ops = 2.0 * float((num.astype(np.float64) ** 3).sum())
-    fw_peak = 2 * VALU_PEAK_TOPS if fw16 else VALU_PEAK_TOPS
-    fw_tops = fw_ops / (avg["floyd_ms"] * 1e-3) / 1e12
-    fw_roof = {"form": "packed u16 (v_pk_add_u16 + v_pk_minimum3_f16)" if fw16 else "int32",
-               "ops": fw_ops, "ms": avg["floyd_ms"], "TOPs": fw_tops, "peak_TOPs": fw_peak,
-               "frac": fw_tops / fw_peak}
+        f"in-part ({inform}) {avg['floyd_ms']:.2f} merge {avg['minplus_ms']:.2f} "
+        f"decode {avg['decode_ms']:.2f}; 16-bit FW parts {stages[-1]['parts_fw16']}, "
+        f"redone {stages[-1]['parts_redone']}, BFS parts {stages[-1]['parts_bfs']}")
+    # In-part distances: the s_p x s_p matrices, written once (int32).
+    in_entries = float((num.astype(np.float64) ** 2).sum())
+    in_part = {"form": inform, "ms": avg["floyd_ms"], "entries": in_entries,
+               "entries_per_s": in_entries / (avg["floyd_ms"] * 1e-3)}
+
+    def fw_roofline(st_avg):
+        # Floyd-Warshall stage: 2 ops (add, min) per step, sum over parts of s_p^3
+        # steps, against the VALU peak of the arithmetic it ran in (packed 16-bit: two
+        # entries per lane-op).
+        fw16 = st_avg["parts_fw16"] > 0
+        fw_ops = 2.0 * float((num.astype(np.float64) ** 3).sum())
+        fw_peak = 2 * VALU_PEAK_TOPS if fw16 else VALU_PEAK_TOPS
+        fw_tops = fw_ops / (st_avg["floyd_ms"] * 1e-3) / 1e12
+        return {"form": "packed u16 (v_pk_add_u16 + v_pk_minimum3_f16)" if fw16 else "int32",
+                "ops": fw_ops, "ms": st_avg["floyd_ms"], "TOPs": fw_tops, "peak_TOPs": fw_peak,
+                "frac": fw_tops / fw_peak, "build_ms": st_avg["build_ms"]}
+
+    fw_roof = None if bfs else fw_roofline(avg)
 
     # Roofline of the dominant kernel: the merge stage of the timed batched steps.  Its
     # launches (one per padded-K group, spread over three streams) are bracketed by HIP
@@ -404,14 +415,27 @@ def main():
 
     # The overlapped form, for comparison (two steps).
     overlap_ms = None
+    fw_form_ms = None
     if not args.skip_forms:
         g.setMode(F.FAP_MODE_OVERLAP)
         ts = time.perf_counter()
         for i in range(2):
             g.solve()
         overlap_ms = (time.perf_counter() - ts) * 1e3 / 2
-        g.setMode(modes[args.mode])
         log(f"overlap form: {overlap_ms:.2f} ms/step")
+        if bfs:  # the same solve with the in-part distances from Floyd-Warshall
+            g.setMode(modes[args.mode] | F.FAP_MODE_FLAG_FW)
+            g.solve()
+            fst = []
+            ts = time.perf_counter()
+            for i in range(2):
+                g.solve()
+                fst.append(stage_times())
+            fw_form_ms = (time.perf_counter() - ts) * 1e3 / 2
+            fw_roof = fw_roofline({k: float(np.mean([x[k] for x in fst])) for k in fst[0]})
+            log(f"Floyd-Warshall in-part form: {fw_form_ms:.2f} ms/step "
+                f"(FW {fw_roof['ms']:.2f} ms + build {fw_roof['build_ms']:.2f} ms)")
+        g.setMode(modes[args.mode])
     # Dominant kernel: the merge (merge_stream_kernel, one launch per padded-K group,
     # timed together as the merge stage of the batched solves).  Per solve it computes
     # sum_p i_p * b_p * n min-plus steps and its algorithmic bytes are
@@ -485,7 +509,8 @@ def main():
                       "boundary_total": int(bdy.sum()), "mode": args.mode,
                       "boundary_rows": bform,
                       "result_bytes_in_hbm": int(n) * int(n) * 4},
-           "stages_ms": avg, "floyd_warshall": fw_roof,
+           "stages_ms": avg, "in_part": in_part, "floyd_warshall": fw_roof,
+           "fw_in_part_form": {"ms_per_step": fw_form_ms},
            "overlap_form": {"ms_per_step": overlap_ms},
            "roofline": roofline, "cpu_baseline": cpu, "plain_form": plain}
     if paths is not None:

@@ -5,6 +5,7 @@ include/fap_hip.h); this package holds the ctypes bindings and a Python mirror o
 the reference's fap::fapGraph API.
 """
 from ._lib import (FAP_INF, FAP_MODE_BATCHED, FAP_MODE_PLAIN, FAP_MODE_OVERLAP, FAP_MODE_FLAG_INT32_FW, FAP_MODE_FLAG_SSSP,
+                   FAP_MODE_FLAG_FW,
                    MAXVALUE,
                    DeviceBuffer, FapError, init, load)
 from .batch import Batch, solve_batch
@@ -12,4 +13,4 @@ from .graph import FapGraph
 
 __all__ = ["FapGraph", "Batch", "solve_batch", "FapError", "DeviceBuffer", "init", "load", "FAP_INF", "MAXVALUE",
            "FAP_MODE_PLAIN", "FAP_MODE_BATCHED", "FAP_MODE_OVERLAP", "FAP_MODE_FLAG_INT32_FW",
-           "FAP_MODE_FLAG_SSSP"]
+           "FAP_MODE_FLAG_SSSP", "FAP_MODE_FLAG_FW"]

@@ -30,6 +30,7 @@ FAP_MODE_BATCHED = 1
 FAP_MODE_OVERLAP = 2
 FAP_MODE_FLAG_INT32_FW = 0x100
 FAP_MODE_FLAG_SSSP = 0x200
+FAP_MODE_FLAG_FW = 0x400
 FAP_BATCH_GRAPH = 0x1
 
 # Every exported symbol of include/fap_hip.h: name -> (restype, argtypes).
@@ -67,7 +68,7 @@ class FapStageTimes(C.Structure):
     _fields_ = [("sssp_ms", C.c_float), ("build_ms", C.c_float), ("floyd_ms", C.c_float),
                 ("minplus_ms", C.c_float), ("decode_ms", C.c_float), ("total_ms", C.c_float),
                 ("parts_fw16", _i32), ("parts_redone", _i32), ("boundary_graph", _i32),
-                ("parts_keyed", _i32), ("parts_fw_keyed", _i32)]
+                ("parts_keyed", _i32), ("parts_fw_keyed", _i32), ("parts_bfs", _i32)]
 
 
 SIGNATURES = {

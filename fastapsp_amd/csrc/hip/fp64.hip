@@ -480,7 +480,8 @@ static int solve_parts_f32(const fap_graph_dev *g, const fap_part_desc *parts, i
               "fap_hip_solve_parts_f32: null argument");
   const int form = mode & 0xFF;
   FAP_REQUIRE((form == FAP_MODE_PLAIN || form == FAP_MODE_BATCHED || form == FAP_MODE_OVERLAP) &&
-                  (mode & ~(0xFF | FAP_MODE_FLAG_INT32_FW | FAP_MODE_FLAG_SSSP)) == 0,
+                  (mode & ~(0xFF | FAP_MODE_FLAG_INT32_FW | FAP_MODE_FLAG_SSSP |
+                            FAP_MODE_FLAG_FW)) == 0,
               "fap_hip_solve_parts_f32: unknown mode %d", mode);
   const int n = g->n;
   FAP_REQUIRE(n > 0 && g->m >= 0 && ld_out >= n && g->d_row_offset && g->d_graph_id &&

@@ -31,6 +31,15 @@ struct Fw16Desc {
 // marks the part in flags[part_index] (its true distance is >= the cap: the part is
 // redone in int32).  Without labels the caller guarantees that every finite entry
 // is below the cap.
+// One part of the bit-parallel in-part BFS (inpart.hip): its S x S int32 matrix and
+// the first of its ceil(s / 64) groups of 64 sources.
+struct InpartDesc {
+  int *fw;
+  int S, s, offset, part_id;
+  int group_begin;
+};
+constexpr int kInpartBfsMaxS = 2048;  // LDS: 72 bytes per vertex
+
 struct UnpackDesc {
   const uint16_t *src;
   int *dst;

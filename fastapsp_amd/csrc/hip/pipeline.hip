@@ -67,6 +67,9 @@ int merge_stream_kpad(int K);
 int launch_merge_stream(const MergeDesc *d_descs, int nd, int total_units, int N, int kp,
                         const int *d_graph_id, const int *d_ed2st, const int *d_max_b, int lim16,
                         hipStream_t stream);
+int launch_inpart_bfs(const InpartDesc *d_descs, int nd, int total_groups, int s_max,
+                      const int *row_off, const int *col, const int *graph_id, const int *st2ed,
+                      const int *ed2st, hipStream_t stream);
 int launch_build(const BuildDesc *d_descs, int nd, int total_rows, bool vec, const int *row_off,
                  const int *col, const int *w, const int *graph_id, const int *st2ed,
                  const int *ed2st, hipStream_t stream);
@@ -128,7 +131,7 @@ struct Span {
 std::vector<Span> g_spans;
 hipEvent_t g_total[2] = {nullptr, nullptr};
 constexpr int kChunks = 8;
-int g_parts16 = 0, g_redone = 0, g_bgraph = 0, g_keyed = 0, g_fw_keyed = 0;
+int g_parts16 = 0, g_redone = 0, g_bgraph = 0, g_keyed = 0, g_fw_keyed = 0, g_bfs = 0;
 
 // What the last pass left for the flag check of the 16-bit form.
 struct Pass16 {
@@ -151,7 +154,7 @@ int timing_start(hipStream_t st) {
   g_events.used = 0;
   g_spans.clear();
   g_total[0] = g_total[1] = nullptr;
-  g_parts16 = g_redone = g_bgraph = g_keyed = g_fw_keyed = 0;
+  g_parts16 = g_redone = g_bgraph = g_keyed = g_fw_keyed = g_bfs = 0;
   FAP_TRY(g_events.take(&g_total[0]));
   FAP_CHECK_HIP(hipEventRecord(g_total[0], st));
   return FAP_OK;
@@ -234,7 +237,15 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   // entry is at most s - 1 < 8192 < the cap, so the cap means "unreachable inside
   // the part" exactly; on SSSP-form matrices of graphs with component labels the
   // cap is checked against the labels (and the part redone in int32 if needed).
-  const bool use16 = !(mode & FAP_MODE_FLAG_INT32_FW) &&
+  // Bit-parallel in-part BFS (inpart.hip) instead of build + Floyd-Warshall: the
+  // boundary-graph form needs in-part distances only, which on unit weights with
+  // paired arcs are BFS distances.
+  int s_max_all = 0;
+  for (const fap_part_desc &q : act) s_max_all = std::max(s_max_all, q.num);
+  const bool bfs_parts = hier && g->d_w == nullptr && g->d_comp != nullptr &&
+                         !(mode & (FAP_MODE_FLAG_INT32_FW | FAP_MODE_FLAG_FW)) &&
+                         s_max_all <= kInpartBfsMaxS;
+  const bool use16 = !(mode & FAP_MODE_FLAG_INT32_FW) && !bfs_parts &&
                      (hier ? g->d_w == nullptr : g->d_comp != nullptr);
   const bool flags16 = use16 && !hier;
   const int nchunks = overlap ? std::min(kChunks, P) : 1;
@@ -269,6 +280,8 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   std::vector<DecodeDesc> dd(P);
   std::vector<Fw16Desc> fd16(P);
   std::vector<UnpackDesc> ud(P);
+  std::vector<InpartDesc> ipd(P);
+  int ip_groups = 0;
   std::vector<int> offsets(P), src_begin(P);
   std::vector<int64_t> out_rows(P);
   int rows_total = 0, src_total = 0;
@@ -303,6 +316,8 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
     md[p] = m;
     if (p > 0 && chunk_of[p] != chunk_of[p - 1]) dec_total = unpack_total = 0;
     ud[p] = UnpackDesc{fw16, fw, S[p], q.num, q.offset, p, batched ? unpack_total : 0};
+    ipd[p] = InpartDesc{fw, S[p], q.num, q.offset, q.part_id, ip_groups};
+    ip_groups += nb[p];  // ceil(s / 64) groups of 64 sources
     unpack_total += (int64_t)S[p] * S[p] / 8;
     chunk_unpack[chunk_of[p]] += (int64_t)S[p] * S[p] / 8;
     dd[p] = DecodeDesc{fw, S[p], d_out + q.out_row * ld_out, ld_out, q.bdy, q.num, q.offset,
@@ -359,6 +374,7 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   const int64_t o_md = carve(sizeof(MergeDesc) * std::max(nmd, 1));
   const int64_t o_dd = carve(sizeof(DecodeDesc) * P);
   const int64_t o_fd16 = carve(sizeof(Fw16Desc) * P), o_ud = carve(sizeof(UnpackDesc) * P);
+  const int64_t o_ip = carve(sizeof(InpartDesc) * P);
   const int64_t o_off = carve(4LL * P), o_sb = carve(4LL * P), o_or = carve(8LL * P);
   const int64_t o_src = carve(4LL * std::max(src_total, 1));
   const int64_t o_rows = carve(8LL * std::max(src_total, 1));
@@ -403,6 +419,7 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   std::memcpy(host.data() + o_dd, dd.data(), sizeof(DecodeDesc) * P);
   std::memcpy(host.data() + o_fd16, fd16.data(), sizeof(Fw16Desc) * P);
   std::memcpy(host.data() + o_ud, ud.data(), sizeof(UnpackDesc) * P);
+  std::memcpy(host.data() + o_ip, ipd.data(), sizeof(InpartDesc) * P);
   std::memcpy(host.data() + o_off, offsets.data(), 4LL * P);
   std::memcpy(host.data() + o_sb, src_begin.data(), 4LL * P);
   std::memcpy(host.data() + o_or, out_rows.data(), 8LL * P);
@@ -423,6 +440,7 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   int64_t *d_rows = reinterpret_cast<int64_t *>(meta + o_rows);
   const Fw16Desc *d_fd16 = reinterpret_cast<const Fw16Desc *>(meta + o_fd16);
   const UnpackDesc *d_ud = reinterpret_cast<const UnpackDesc *>(meta + o_ud);
+  const InpartDesc *d_ip = reinterpret_cast<const InpartDesc *>(meta + o_ip);
   const BgPart *d_bgp = reinterpret_cast<const BgPart *>(meta + o_bgp);
   const int *d_pidx = reinterpret_cast<const int *>(meta + o_pidx);
   int *d_flags = reinterpret_cast<int *>(meta + o_flags);
@@ -463,6 +481,9 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   // Floyd-Warshall of parts [p0, p1) with max tile count mnb; unpack_items: the
   // prefix end of those parts' unpack items.
   auto run_fw = [&](int p0, int p1, int mnb, int64_t unpack_items, hipStream_t on) -> int {
+    if (bfs_parts)  // boundary-graph form: one chunk, all parts
+      return launch_inpart_bfs(d_ip, P, ip_groups, s_max_all, g->d_row_offset, g->d_col,
+                               g->d_graph_id, g->d_st2ed, g->d_ed2st, on);
     if (!use16) return launch_fw(d_fd + p0, p1 - p0, mnb, on);
     FAP_TRY(launch_fw16(d_fd16 + p0, p1 - p0, mnb, on));
     return launch_fw16_unpack(d_ud + p0, p1 - p0, unpack_items,
@@ -514,10 +535,11 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
                                       d_src, d_rows, on));
         return run_sssp(d_src, src_total, on);
       }));
-    FAP_TRY(span(1, st, [&](hipStream_t on) {
-      return launch_build(d_bd, P, rows_total, true, g->d_row_offset, g->d_col, g->d_w,
-                          g->d_graph_id, g->d_st2ed, g->d_ed2st, on);
-    }));
+    if (!bfs_parts)  // the BFS writes the whole in-part matrices itself
+      FAP_TRY(span(1, st, [&](hipStream_t on) {
+        return launch_build(d_bd, P, rows_total, true, g->d_row_offset, g->d_col, g->d_w,
+                            g->d_graph_id, g->d_st2ed, g->d_ed2st, on);
+      }));
     // Chunks of parts: [first part, end part) with their FW rounds, merges and decode.
     std::vector<int> cfirst(nchunks + 1, P);
     for (int p = P - 1; p >= 0; --p) cfirst[chunk_of[p]] = p;
@@ -598,6 +620,7 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   }
   FAP_TRY(event(st, &g_total[1]));
   if (use16) g_parts16 += P;
+  if (bfs_parts) g_bfs += P;
   if (hier) g_bgraph = 1;
   if (flags16) {
     g_pass16.parts = act;
@@ -616,9 +639,10 @@ static int solve_parts(const fap_graph_dev *g, const fap_part_desc *parts, int32
               "fap_hip_solve_parts: null argument");
   const int form = mode & 0xFF;
   FAP_REQUIRE((form == FAP_MODE_PLAIN || form == FAP_MODE_BATCHED || form == FAP_MODE_OVERLAP) &&
-                  (mode & ~(0xFF | FAP_MODE_FLAG_INT32_FW | FAP_MODE_FLAG_SSSP)) == 0,
+                  (mode & ~(0xFF | FAP_MODE_FLAG_INT32_FW | FAP_MODE_FLAG_SSSP |
+                            FAP_MODE_FLAG_FW)) == 0,
               "fap_hip_solve_parts: unknown mode %d", mode);
-  g_parts16 = g_redone = g_bgraph = 0;
+  g_parts16 = g_redone = g_bgraph = g_keyed = g_fw_keyed = g_bfs = 0;
   FAP_TRY(solve_pass(g, parts, nparts, d_out, ld_out, mode, stream, false));
   if (g_pass16.parts.empty()) return FAP_OK;
   // Flag check of the 16-bit form: parts whose closure reached the cap inside one
@@ -673,6 +697,7 @@ extern "C" int fap_hip_last_stage_times(fap_stage_times *t) {
   acc.boundary_graph = g_bgraph;
   acc.parts_keyed = g_keyed;
   acc.parts_fw_keyed = g_fw_keyed;
+  acc.parts_bfs = g_bfs;
   *t = acc;
   return FAP_OK;
 }

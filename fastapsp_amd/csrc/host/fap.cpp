@@ -223,7 +223,7 @@ std::vector<int> fapGraph::getGraphId() { return graph_id; }
 void fapGraph::setMode(int32_t m) {
   const int32_t form = m & 0xFF;
   if ((form != FAP_MODE_PLAIN && form != FAP_MODE_BATCHED && form != FAP_MODE_OVERLAP) ||
-      (m & ~(0xFF | FAP_MODE_FLAG_INT32_FW | FAP_MODE_FLAG_SSSP)) != 0)
+      (m & ~(0xFF | FAP_MODE_FLAG_INT32_FW | FAP_MODE_FLAG_SSSP | FAP_MODE_FLAG_FW)) != 0)
     throw FapError(FAP_ERR_INVALID, "unknown mode");
   mode = m;
 }

@@ -55,6 +55,10 @@ enum fap_mode { FAP_MODE_PLAIN = 0, FAP_MODE_BATCHED = 1, FAP_MODE_OVERLAP = 2 }
  * even where the BATCHED form would take them from the boundary graph (see
  * fap_hip_solve_parts).  Results are identical either way. */
 #define FAP_MODE_FLAG_SSSP 0x200
+/* Flag OR-ed into a mode: compute the in-part distances of the boundary-graph form by
+ * Floyd-Warshall even where the bit-parallel BFS applies (see fap_hip_solve_parts).
+ * Results are identical either way. */
+#define FAP_MODE_FLAG_FW 0x400
 
 /* ------------------------------------------------------------------ runtime */
 
@@ -185,6 +189,7 @@ typedef struct fap_stage_times {
   int32_t boundary_graph; /* 1: boundary rows from the boundary graph, 0: boundary SSSP */
   int32_t parts_keyed;  /* path solves: parts whose min-plus ran in the keyed form */
   int32_t parts_fw_keyed; /* path solves: parts whose Floyd-Warshall ran in the keyed form */
+  int32_t parts_bfs;    /* parts whose in-part distances came from the bit-parallel BFS */
 } fap_stage_times;
 
 /* The whole per-part pipeline of fapGraph::run (fap/fap.cpp:195-503) for a list
@@ -209,7 +214,11 @@ typedef struct fap_stage_times {
  * all boundary vertices (in-part boundary distances plus cut arcs) is closed by
  * Floyd-Warshall, and one min-plus pass writes every boundary row.  Unit-weight graphs
  * then always take the 16-bit Floyd-Warshall (no flag check, no wait), others int32.
- * The stage times report this stage as sssp_ms. */
+ * The stage times report this stage as sssp_ms.  In this form, on unit-weight graphs
+ * with component labels (every arc paired with its reverse) and parts of at most 2048
+ * vertices, the in-part distances come from a bit-parallel BFS (64 sources per 64-bit
+ * word) instead of Floyd-Warshall, unless FAP_MODE_FLAG_FW or FAP_MODE_FLAG_INT32_FW
+ * is set; the stage times report it as floyd_ms. */
 int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *parts, int32_t nparts,
                         int32_t *d_out, int64_t ld_out, int32_t mode, void *stream);
 int fap_hip_last_stage_times(fap_stage_times *t);

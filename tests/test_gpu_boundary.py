@@ -2,7 +2,9 @@
 cover the graph): results must equal the SSSP form, the plain per-part form, the
 oracle's restatement of the reference path and an exact all-pairs BFS / Dijkstra, bit
 for bit, on unit-weight, integer-weighted and directed graphs, with parts whose
-boundary exceeds one register chunk and parts that are disconnected inside."""
+boundary exceeds one register chunk and parts that are disconnected inside.  On
+unit-weight undirected graphs the in-part distances come from the bit-parallel BFS;
+FAP_MODE_FLAG_FW (16-bit Floyd-Warshall) and FAP_MODE_FLAG_INT32_FW must agree."""
 import sys
 
 import numpy as np
@@ -35,9 +37,21 @@ def _solve(g, mode):
     return g.getAllDistanceI32(), info
 
 
+def _unit_symmetric(g):
+    c = g.csr()
+    if not np.all(c["weight"] == 1):
+        return False
+    n = len(c["row_offset"]) - 1
+    src = np.repeat(np.arange(n, dtype=np.int64), np.diff(c["row_offset"]))
+    fwd = set(zip(src.tolist(), c["col"].tolist()))
+    return all((v, u) in fwd for u, v in fwd)
+
+
 def _forms_agree(g, oracle_path=True):
     a, info = _solve(g, F.FAP_MODE_BATCHED)
     assert info["boundary_graph"] == 1
+    bfs = _unit_symmetric(g)
+    assert info["parts_bfs"] == (g.K if bfs else 0)
     exact = _exact(g)
     assert np.array_equal(a, exact), "boundary-graph form differs from exact APSP"
     if oracle_path:
@@ -50,8 +64,12 @@ def _forms_agree(g, oracle_path=True):
     assert info["boundary_graph"] == 0
     assert np.array_equal(a, c), "boundary-graph form differs from the plain form"
     d, info = _solve(g, F.FAP_MODE_BATCHED | F.FAP_MODE_FLAG_INT32_FW)
-    assert info["boundary_graph"] == 1 and info["parts_fw16"] == 0
+    assert info["boundary_graph"] == 1 and info["parts_fw16"] == 0 and info["parts_bfs"] == 0
     assert np.array_equal(a, d), "int32 in-part Floyd-Warshall differs"
+    e, info = _solve(g, F.FAP_MODE_BATCHED | F.FAP_MODE_FLAG_FW)
+    assert info["boundary_graph"] == 1 and info["parts_bfs"] == 0
+    assert info["parts_fw16"] == (g.K if np.all(g.csr()["weight"] == 1) else 0)
+    assert np.array_equal(a, e), "16-bit in-part Floyd-Warshall differs"
     return a
 
 
@@ -162,3 +180,45 @@ def test_boundary_graph_merge_16bit_bound(n, K):
     assert np.array_equal(a, want)
     b, _ = _solve(g, F.FAP_MODE_BATCHED | SSSP)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n,K", [(300, 1),      # one part: a path longer than 255 levels
+                                 (2048, 1),     # the largest part the BFS form takes
+                                 (2049, 1)])    # one vertex more: Floyd-Warshall
+def test_inpart_bfs_long_paths_and_size_limit(n, K):
+    # Path graph: in-part distances up to n - 1 (levels above 253 take the direct
+    # store); parts of more than 2048 vertices fall back to Floyd-Warshall.
+    say(f"in-part BFS: path n={n} K={K}")
+    ro, col = _path(n)
+    g = F.FapGraph.from_csr(ro, col, None, False, K)
+    g.preCondition()
+    a, info = _solve(g, F.FAP_MODE_BATCHED)
+    assert info["boundary_graph"] == 1
+    assert info["parts_bfs"] == (K if n <= 2048 else 0)
+    st2ed = g.getMapping().astype(np.int64)
+    want = np.abs(st2ed[:, None] - np.arange(n)[None, :]).astype(np.int32)
+    assert np.array_equal(a, want)
+
+
+def test_inpart_bfs_high_degree_and_disconnected_parts():
+    # A star (hub with 40 in-part neighbours: more than the 8 kept in registers) and a
+    # second component inside the same part; K = 2 with a chain linking the parts.
+    say("in-part BFS: hub of degree 40, part disconnected inside")
+    edges = [(0, i) for i in range(1, 41)]                  # star, part 1
+    edges += [(41 + i, 42 + i) for i in range(20)]           # chain 41..61, part 1 (2nd comp.)
+    edges += [(62 + i, 63 + i) for i in range(30)]           # chain 62..92, part 2
+    edges += [(40, 62), (61, 92)]                            # links through part 2
+    n = 93
+    out = [[] for _ in range(n)]
+    for u, v in edges:
+        out[u].append(v)
+        out[v].append(u)
+    ro = np.zeros(n + 1, np.int32)
+    ro[1:] = np.cumsum([len(x) for x in out])
+    col = np.array([v for x in out for v in x], np.int32)
+    g = F.FapGraph.from_csr(ro, col, None, False, 2)
+    gid = np.where(np.arange(n) < 62, 1, 2).astype(np.int32)
+    g.setPartition(gid)
+    g.preCondition()
+    a = _forms_agree(g)
+    assert (a < F.FAP_INF).all()  # connected through part 2

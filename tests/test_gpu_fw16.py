@@ -2,7 +2,8 @@
 packed 16-bit arithmetic; the result must equal the int32 form and the oracle bit for
 bit.  SSSP form (FAP_MODE_FLAG_SSSP, symmetric graphs): pairs in different components
 stay unreachable and a part whose distances reach the cap is redone in int32.
-Boundary-graph form (unit weights): in-part distances never reach the cap.  Road-graph
+Boundary-graph form (unit weights, FAP_MODE_FLAG_FW: otherwise undirected unit-weight
+graphs take the bit-parallel BFS): in-part distances never reach the cap.  Road-graph
 distances below 1024 are IEEE-half subnormal bit patterns, so every test here also
 checks that the half-precision minimum orders them as integers."""
 import sys
@@ -52,9 +53,10 @@ def _solve_check(g, mode):
 
 
 SSSP = F.FAP_MODE_FLAG_SSSP
+FW = F.FAP_MODE_FLAG_FW
 
 
-@pytest.mark.parametrize("flag", [0, SSSP])
+@pytest.mark.parametrize("flag", [FW, SSSP])
 def test_fw16_taken_on_road_graph_and_equals_int32(flag):
     say(f"fw16: road grid 48x40 K=6, 16-bit vs int32, flag={flag:#x}")
     g = F.FapGraph.road_grid(48, 40, 0.1, 0.03, 5, 6)
@@ -119,7 +121,7 @@ def test_fw16_redo_some_parts(mode):
     assert 1 <= redo < g.K
 
 
-@pytest.mark.parametrize("flag", [0, SSSP])
+@pytest.mark.parametrize("flag", [FW, SSSP])
 def test_fw16_unreachable_pairs_are_not_redone(flag):
     # Two components inside one part: the capped entries between them are
     # unreachable pairs, not overflow.
