@@ -38,7 +38,7 @@ struct InpartDesc {
   int S, s, offset, part_id;
   int group_begin;
 };
-constexpr int kInpartBfsMaxS = 2048;  // LDS: 72 bytes per vertex
+constexpr int kInpartBfsMaxS = 2048;  // levels fit 11 bit-planes; 8 LDS bytes per vertex
 
 struct UnpackDesc {
   const uint16_t *src;

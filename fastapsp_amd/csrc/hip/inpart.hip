@@ -10,15 +10,20 @@
 // sources, against s * 64 min-plus steps per vertex for the same 64 matrix columns
 // in Floyd-Warshall.
 //
-// Workgroup (256 threads): one part and one group of 64 consecutive sorted sources
-// [g * 64, g * 64 + 64).  Thread t owns vertices t + 256 q (q < Q): their visited
-// words and in-part neighbour lists (up to kDeg local ids, the rest re-read from the
-// CSR each level) stay in registers; the frontier words (s x 8 B) and the levels
-// (s x 64 bytes, 0xFF = not reached, 0xFE = level above 253 stored directly) sit in
-// LDS.  By symmetry d(src, v) = d(v, src), so the group fills columns g*64 .. g*64+63
-// of every row of the part's S x S int32 matrix: rows are written as 256 contiguous
-// bytes at the end (FAP_INF where unreached, for source columns >= s and for the
-// padding rows >= s).
+// Workgroup (kThreads = 512): one part and one group of 64 consecutive sorted sources
+// [g * 64, g * 64 + 64).  Thread t owns vertices t + 512 q (q < Q): their visited
+// words, in-part neighbour lists (up to kDeg local ids, the rest re-read from the CSR
+// each level) and level bit-planes stay in registers; only the frontier words
+// (s x 8 B, plus a zero word at index s that pads the neighbour lists) sit in LDS.
+// Each level reads the padded lists up to the wave's largest degree with no per-lane
+// branch, so all the frontier reads of a thread are in flight together.  A level L
+// ORs the new bits into the planes j with bit j of L set (L is uniform, so is the
+// branch): bit b of plane j is bit j of d(v, source b), and no lane loops over the
+// bits it found (a branch per neighbour read and a loop over new bits took 3 us per
+// level).  By symmetry d(src, v) = d(v, src), so the group fills columns
+// g*64 .. g*64+63 of every row of the part's S x S int32 matrix: each owned row as
+// 256 contiguous bytes at the end (FAP_INF where unreached and for source columns
+// >= s), the padding rows >= s with FAP_INF.
 #include <cstdint>
 
 #include "fap_internal.h"
@@ -27,19 +32,19 @@
 namespace fap_hip {
 namespace {
 
-constexpr int kDeg = 8;            // in-part neighbours kept in registers per vertex
-constexpr int kMaxByteLevel = 253;  // levels above are written to global directly
-constexpr uint8_t kUnreached = 0xFF, kDirect = 0xFE;
+constexpr int kDeg = 8;       // in-part neighbours kept in registers per vertex
+constexpr int kPlanes = 11;   // levels below 2048 > kInpartBfsMaxS
+constexpr int kThreads = 512;  // 8 waves: Q = 2 registers' worth of vertices up to 1024
 
 template <int Q>
-__global__ __launch_bounds__(256) void inpart_bfs_kernel(const InpartDesc *__restrict__ descs,
+__global__ __launch_bounds__(kThreads) void inpart_bfs_kernel(const InpartDesc *__restrict__ descs,
                                                          int nd, int total_groups,
                                                          const int *__restrict__ row_off,
                                                          const int *__restrict__ col,
                                                          const int *__restrict__ graph_id,
                                                          const int *__restrict__ st2ed,
                                                          const int *__restrict__ ed2st) {
-  extern __shared__ unsigned long long lds_words[];
+  extern __shared__ unsigned long long F[];  // frontier words, s + 1
   const int blk = blockIdx.x;
   if (blk >= total_groups) return;
   int lo = 0, hi = nd - 1;
@@ -51,18 +56,22 @@ __global__ __launch_bounds__(256) void inpart_bfs_kernel(const InpartDesc *__res
   const InpartDesc d = descs[lo];
   const int src0 = (blk - d.group_begin) * 64;
   const int s = d.s, tid = threadIdx.x;
-  unsigned long long *F = lds_words;                        // frontier words, s
-  uint8_t *lev = reinterpret_cast<uint8_t *>(lds_words + s);  // levels, s x 64
 
-  // Neighbour lists, visited words, first frontier.
+  // Neighbour lists (byte offsets into F), visited words, first frontier.
   int nbr[Q][kDeg], deg[Q], ovf[Q];  // ovf: CSR index of the first neighbour not kept
-  unsigned long long vis[Q];
+  int wdeg[Q];                        // largest deg[q] of the wave (uniform)
+  unsigned long long vis[Q], pl[Q][kPlanes];
+  if (tid == 0) F[s] = 0;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    const int v = tid + 256 * q;
+    const int v = tid + kThreads * q;
     deg[q] = 0;
     ovf[q] = -1;
     vis[q] = 0;
+#pragma unroll
+    for (int j = 0; j < kPlanes; ++j) pl[q][j] = 0;
+#pragma unroll
+    for (int k = 0; k < kDeg; ++k) nbr[q][k] = s * 8;  // the zero word
     if (v < s) {
       const int ver = st2ed[d.offset + v];
       const int e1 = row_off[ver + 1];
@@ -76,34 +85,43 @@ __global__ __launch_bounds__(256) void inpart_bfs_kernel(const InpartDesc *__res
         // A fixed register index per unrolled slot: no dynamic register indexing.
 #pragma unroll
         for (int k = 0; k < kDeg; ++k)
-          if (k == deg[q]) nbr[q][k] = ed2st[nb] - d.offset;
+          if (k == deg[q]) nbr[q][k] = (ed2st[nb] - d.offset) * 8;
         ++deg[q];
       }
       const int b = v - src0;
-      vis[q] = (b >= 0 && b < 64) ? 1ull << b : 0ull;
+      vis[q] = (b >= 0 && b < 64) ? 1ull << b : 0ull;  // d(v, v) = 0: no plane bit
       F[v] = vis[q];
-      uint32_t *row = reinterpret_cast<uint32_t *>(lev + (int64_t)v * 64);
-#pragma unroll
-      for (int w = 0; w < 16; ++w) row[w] = 0xFFFFFFFFu;
     }
+    int m = deg[q];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+    wdeg[q] = __builtin_amdgcn_readfirstlane(m);
   }
   __syncthreads();
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const int v = tid + 256 * q, b = v - src0;
-    if (v < s && b >= 0 && b < 64) lev[(int64_t)v * 64 + b] = 0;
-  }
 
+  // fr[q][k] for k >= wdeg[q] stays 0 for the whole search: no per-level reset.
+  unsigned long long fr[Q][kDeg];
+#pragma unroll
+  for (int q = 0; q < Q; ++q)
+#pragma unroll
+    for (int k = 0; k < kDeg; ++k) fr[q][k] = 0;
   for (int level = 1;; ++level) {
     unsigned long long nx[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+      for (int k = 0; k < kDeg; ++k)
+        if (k < wdeg[q])  // uniform bound: no divergence
+          fr[q][k] = *reinterpret_cast<const unsigned long long *>(
+              reinterpret_cast<const char *>(F) + nbr[q][k]);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       unsigned long long acc = 0;
 #pragma unroll
       for (int k = 0; k < kDeg; ++k)
-        if (k < deg[q]) acc |= F[nbr[q][k]];
+        if (k < wdeg[q]) acc |= fr[q][k];
       if (ovf[q] >= 0) {  // more than kDeg in-part neighbours: the rest from the CSR
-        const int v = tid + 256 * q;
+        const int v = tid + kThreads * q;
         const int ver = st2ed[d.offset + v];
         for (int e = ovf[q]; e < row_off[ver + 1]; ++e) {
           const int nb = col[e];
@@ -116,51 +134,58 @@ __global__ __launch_bounds__(256) void inpart_bfs_kernel(const InpartDesc *__res
     int any = 0;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const int v = tid + 256 * q;
-      if (v >= s) continue;
-      F[v] = nx[q];
+      const int v = tid + kThreads * q;
+      if (v < s) F[v] = nx[q];  // rows >= s never get a bit
       vis[q] |= nx[q];
-      unsigned long long m = nx[q];
-      any |= m != 0;
-      while (m) {
-        const int b = __ffsll((long long)m) - 1;
-        m &= m - 1;
-        if (level <= kMaxByteLevel) {
-          lev[(int64_t)v * 64 + b] = (uint8_t)level;
-        } else {
-          lev[(int64_t)v * 64 + b] = kDirect;
-          d.fw[(int64_t)v * d.S + src0 + b] = level;
-        }
-      }
+      any |= nx[q] != 0;
     }
+    // A uniform branch per plane: the empty asm keeps the compiler from turning it
+    // into an AND with a mask and an OR on every plane of every level.
+#pragma unroll
+    for (int j = 0; j < kPlanes; ++j)
+      if (level & (1 << j)) {
+        asm volatile("");
+#pragma unroll
+        for (int q = 0; q < Q; ++q) pl[q][j] |= nx[q];
+      }
     if (!__syncthreads_or(any)) break;
   }
 
-  // Write columns src0 .. src0 + 63 of every row: 16 lanes per row, 4 columns each.
-  for (int r = tid >> 4; r < d.S; r += 16) {
-    const int c = (tid & 15) * 4;
-    int out[4];
-    bool direct = false;
+  // Owned rows: column src0 + b = d(v, source b) from the planes, on 32-bit halves.
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const uint8_t x = r < s ? lev[(int64_t)r * 64 + c + e] : kUnreached;
-      direct |= x == kDirect;
-      out[e] = (x == kUnreached || src0 + c + e >= s) ? kInf : (int)x;
-    }
-    int *dst = d.fw + (int64_t)r * d.S + src0 + c;
-    if (!direct) {
-      *reinterpret_cast<int4 *>(dst) = make_int4(out[0], out[1], out[2], out[3]);
-    } else {
+  for (int q = 0; q < Q; ++q) {
+    const int v = tid + kThreads * q;
+    if (v >= s) continue;
+    int4 *dst = reinterpret_cast<int4 *>(d.fw + (int64_t)v * d.S + src0);
 #pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (lev[(int64_t)r * 64 + c + e] != kDirect) dst[e] = out[e];
+    for (int h = 0; h < 2; ++h) {
+      uint32_t p32[kPlanes];
+#pragma unroll
+      for (int j = 0; j < kPlanes; ++j) p32[j] = (uint32_t)(pl[q][j] >> (32 * h));
+      const uint32_t v32 = (uint32_t)(vis[q] >> (32 * h));
+      for (int c = 0; c < 32; c += 4) {
+        int out[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int b = c + e;
+          uint32_t x = 0;
+#pragma unroll
+          for (int j = 0; j < kPlanes; ++j) x |= ((p32[j] >> b) & 1u) << j;
+          out[e] = ((v32 >> b) & 1u) ? (int)x : kInf;
+        }
+        dst[(32 * h + c) / 4] = make_int4(out[0], out[1], out[2], out[3]);
+      }
     }
   }
+  // Padding rows s .. S - 1: 16 lanes per row, 4 columns each.
+  for (int r = s + (tid >> 4); r < d.S; r += kThreads / 16)
+    *reinterpret_cast<int4 *>(d.fw + (int64_t)r * d.S + src0 + (tid & 15) * 4) =
+        make_int4(kInf, kInf, kInf, kInf);
 }
 
 }  // namespace
 
-int inpart_bfs_lds_bytes(int s_max) { return s_max * 8 + s_max * 64; }
+int inpart_bfs_lds_bytes(int s_max) { return (s_max + 1) * 8; }
 
 int launch_inpart_bfs(const InpartDesc *d_descs, int nd, int total_groups, int s_max,
                       const int *row_off, const int *col, const int *graph_id, const int *st2ed,
@@ -168,12 +193,12 @@ int launch_inpart_bfs(const InpartDesc *d_descs, int nd, int total_groups, int s
   if (nd <= 0 || total_groups <= 0) return FAP_OK;
   FAP_REQUIRE(s_max <= kInpartBfsMaxS, "launch_inpart_bfs: part of %d vertices", s_max);
   const size_t lds = (size_t)inpart_bfs_lds_bytes(s_max);
-  if (s_max <= 1024)
-    hipLaunchKernelGGL(inpart_bfs_kernel<4>, dim3(total_groups), dim3(256), lds, stream, d_descs,
-                       nd, total_groups, row_off, col, graph_id, st2ed, ed2st);
+  if (s_max <= 2 * kThreads)
+    hipLaunchKernelGGL(inpart_bfs_kernel<2>, dim3(total_groups), dim3(kThreads), lds, stream,
+                       d_descs, nd, total_groups, row_off, col, graph_id, st2ed, ed2st);
   else
-    hipLaunchKernelGGL(inpart_bfs_kernel<8>, dim3(total_groups), dim3(256), lds, stream, d_descs,
-                       nd, total_groups, row_off, col, graph_id, st2ed, ed2st);
+    hipLaunchKernelGGL(inpart_bfs_kernel<4>, dim3(total_groups), dim3(kThreads), lds, stream,
+                       d_descs, nd, total_groups, row_off, col, graph_id, st2ed, ed2st);
   FAP_CHECK_LAUNCH();
   return FAP_OK;
 }
