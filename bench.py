@@ -4,8 +4,9 @@ Workload (default): road-like undirected unweighted graph with exactly n = 114,5
 vertices (percolated grid, p_keep = 0.52, chords at rate 1e-4, largest component,
 BFS-trimmed), K = 128 parts, full n x n int32 APSP via solve() in the batched form,
 result resident in HBM (n x ld int32, about 52.5 GB).  A step is one whole solve():
-in-part build and Floyd-Warshall, the boundary rows (from the boundary graph), and the
-min-plus merge with fused decode, for all parts.
+the in-part distances (bit-parallel BFS on this unit-weight undirected graph), the
+boundary rows (from the boundary graph), and the min-plus merge with fused decode, for
+all parts.
 
 Prints ONE JSON line: value = n^2 distance entries per second of wall time.
 Also measured: the plain per-part launch form (one step), per-stage device times,
