@@ -267,6 +267,24 @@ __device__ __forceinline__ void load_fin(int (&fin)[KP], const ColInfo &ci, int 
     if (c0 + c >= ci.bdy) fin[c] = kInf;
 }
 
+// The same from a symmetric in-part matrix: d_q(c, l) = d_q(l, c), so row l gives the
+// KP values contiguously (16-byte loads; KP <= 32 <= S columns are readable).
+template <int KP>
+__device__ __forceinline__ void load_fin_sym(int (&fin)[KP], const ColInfo &ci) {
+  const int4 *row = reinterpret_cast<const int4 *>(ci.fw + (int64_t)ci.l * ci.S);
+#pragma unroll
+  for (int c = 0; c < KP; c += 4) {
+    const int4 v = row[c / 4];
+    fin[c] = v.x;
+    fin[c + 1] = v.y;
+    fin[c + 2] = v.z;
+    fin[c + 3] = v.w;
+  }
+#pragma unroll
+  for (int c = 0; c < KP; ++c)
+    if (c >= ci.bdy) fin[c] = kInf;
+}
+
 // *max_b = max(*max_b, the wave's largest value of mx): one atomic per wave.
 __device__ __forceinline__ void publish_max(int *max_b, int mx) {
 #pragma unroll
@@ -278,9 +296,10 @@ __device__ __forceinline__ void publish_max(int *max_b, int mx) {
 // blockIdx.x * 1024 + e * 256 + t (e < 4): the lanes of a wave cover 64 consecutive
 // vertices, so they share one or a few parts (LDS broadcast reads) and every store
 // is one coalesced 256-byte run.  Each lane loads d_q(c, l) of its columns once
-// (registers), then walks its workgroup's share of G_B rows (blockIdx.y of gridDim.y
-// splits) through LDS, R rows at a time.  LDS stride ldl = Sb + KP (INF tail).
-template <int KP>
+// (registers; from row l when the in-part matrices are symmetric), then walks its
+// workgroup's share of G_B rows (blockIdx.y of gridDim.y splits) through LDS, R rows
+// at a time.  LDS stride ldl = Sb + KP (INF tail).
+template <int KP, bool kSym>
 __global__ __launch_bounds__(256) void bg_rows_kernel(
     const BgPart *__restrict__ parts, const int64_t *__restrict__ orow, int nb_total,
     const int *__restrict__ mat, int Sb, int R, const int *__restrict__ pidx_by_pid,
@@ -298,7 +317,8 @@ __global__ __launch_bounds__(256) void bg_rows_kernel(
   for (int e = 0; e < 4; ++e) {
     const ColInfo ci = column_info(parts, pidx_by_pid, graph_id, ed2st, jb + e * 256, n);
     bs[e] = ci.bs;
-    load_fin<KP>(fin[e], ci, 0);
+    if (kSym) load_fin_sym<KP>(fin[e], ci);
+    else load_fin<KP>(fin[e], ci, 0);
   }
   int mx = 0;  // largest finite distance written by this thread
   for (int g0 = gbeg; g0 < gend; g0 += R) {
@@ -446,7 +466,7 @@ int bg_rows_chunk(int ldl) { return std::max(1, std::min(kRowsMax, 65536 / (ldl 
 int launch_bg_rows(const BgPart *d_parts, const int64_t *d_orow, int nb_total, const int *d_mat,
                    int Sb, int max_bdy, const int *d_pidx_by_pid, const int *graph_id,
                    const int *ed2st, int n, int *d_out, int64_t ld_out, int *d_max_b,
-                   hipStream_t stream) {
+                   bool sym, hipStream_t stream) {
   if (nb_total <= 0 || n <= 0) return FAP_OK;
   const int cols = (n + kRowsCols - 1) / kRowsCols;
   // Row splits: about two workgroups per CU in all, each split at least one chunk.
@@ -460,9 +480,14 @@ int launch_bg_rows(const BgPart *d_parts, const int64_t *d_orow, int nb_total, c
   const size_t lds = (size_t)R * ldl * 4;
 #define FAP_BG_CASE(KPV)                                                                      \
   case KPV:                                                                                   \
-    hipLaunchKernelGGL((bg_rows_kernel<KPV>), grid, dim3(256), lds, stream, d_parts, d_orow,   \
-                       nb_total, d_mat, Sb, R, d_pidx_by_pid, graph_id, ed2st, n, d_out, ld_out,   \
-                       d_max_b);                                                               \
+    if (sym)                                                                                  \
+      hipLaunchKernelGGL((bg_rows_kernel<KPV, true>), grid, dim3(256), lds, stream, d_parts,   \
+                         d_orow, nb_total, d_mat, Sb, R, d_pidx_by_pid, graph_id, ed2st, n,    \
+                         d_out, ld_out, d_max_b);                                              \
+    else                                                                                      \
+      hipLaunchKernelGGL((bg_rows_kernel<KPV, false>), grid, dim3(256), lds, stream, d_parts,  \
+                         d_orow, nb_total, d_mat, Sb, R, d_pidx_by_pid, graph_id, ed2st, n,    \
+                         d_out, ld_out, d_max_b);                                              \
     break;
   if (narrow) {
     switch (KP) {

@@ -31,12 +31,13 @@ struct Fw16Desc {
 // marks the part in flags[part_index] (its true distance is >= the cap: the part is
 // redone in int32).  Without labels the caller guarantees that every finite entry
 // is below the cap.
-// One part of the bit-parallel in-part BFS (inpart.hip): its S x S int32 matrix and
-// the first of its ceil(s / 64) groups of 64 sources.
+// One part of a bit-parallel in-part BFS launch (inpart.hip): its S x S int32 matrix,
+// the launch's first workgroup for it (group_begin) and the source group that
+// workgroup takes (g0: sources g0 * 64 ..).
 struct InpartDesc {
   int *fw;
   int S, s, offset, part_id;
-  int group_begin;
+  int group_begin, g0;
 };
 constexpr int kInpartBfsMaxS = 2048;  // levels fit 11 bit-planes; 8 LDS bytes per vertex
 

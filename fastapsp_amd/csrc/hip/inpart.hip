@@ -14,7 +14,8 @@
 // [g * 64, g * 64 + 64).  Thread t owns vertices t + 512 q (q < Q): their visited
 // words, in-part neighbour lists (up to kDeg local ids, the rest re-read from the CSR
 // each level) and level bit-planes stay in registers; only the frontier words
-// (s x 8 B, plus a zero word at index s that pads the neighbour lists) sit in LDS.
+// (two buffers of s x 8 B, each with a zero word at index s that pads the neighbour
+// lists) sit in LDS.
 // Each level reads the padded lists up to the wave's largest degree with no per-lane
 // branch, so all the frontier reads of a thread are in flight together.  A level L
 // ORs the new bits into the planes j with bit j of L set (L is uniform, so is the
@@ -44,7 +45,9 @@ __global__ __launch_bounds__(kThreads) void inpart_bfs_kernel(const InpartDesc *
                                                          const int *__restrict__ graph_id,
                                                          const int *__restrict__ st2ed,
                                                          const int *__restrict__ ed2st) {
-  extern __shared__ unsigned long long F[];  // frontier words, s + 1
+  // Frontier words, two buffers of s + 1 (level L reads buffer (L - 1) & 1 and writes
+  // the other: one barrier per level).
+  extern __shared__ unsigned long long F[];
   const int blk = blockIdx.x;
   if (blk >= total_groups) return;
   int lo = 0, hi = nd - 1;
@@ -54,14 +57,17 @@ __global__ __launch_bounds__(kThreads) void inpart_bfs_kernel(const InpartDesc *
     else hi = mid - 1;
   }
   const InpartDesc d = descs[lo];
-  const int src0 = (blk - d.group_begin) * 64;
+  const int src0 = (d.g0 + blk - d.group_begin) * 64;
   const int s = d.s, tid = threadIdx.x;
 
   // Neighbour lists (byte offsets into F), visited words, first frontier.
   int nbr[Q][kDeg], deg[Q], ovf[Q];  // ovf: CSR index of the first neighbour not kept
   int wdeg[Q];                        // largest deg[q] of the wave (uniform)
   unsigned long long vis[Q], pl[Q][kPlanes];
-  if (tid == 0) F[s] = 0;
+  if (tid == 0) {
+    F[s] = 0;
+    F[2 * s + 1] = 0;
+  }
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int v = tid + kThreads * q;
@@ -106,6 +112,8 @@ __global__ __launch_bounds__(kThreads) void inpart_bfs_kernel(const InpartDesc *
 #pragma unroll
     for (int k = 0; k < kDeg; ++k) fr[q][k] = 0;
   for (int level = 1;; ++level) {
+    const unsigned long long *cur = F + ((level - 1) & 1) * (s + 1);
+    unsigned long long *nxt = F + (level & 1) * (s + 1);
     unsigned long long nx[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q)
@@ -113,7 +121,7 @@ __global__ __launch_bounds__(kThreads) void inpart_bfs_kernel(const InpartDesc *
       for (int k = 0; k < kDeg; ++k)
         if (k < wdeg[q])  // uniform bound: no divergence
           fr[q][k] = *reinterpret_cast<const unsigned long long *>(
-              reinterpret_cast<const char *>(F) + nbr[q][k]);
+              reinterpret_cast<const char *>(cur) + nbr[q][k]);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       unsigned long long acc = 0;
@@ -125,17 +133,16 @@ __global__ __launch_bounds__(kThreads) void inpart_bfs_kernel(const InpartDesc *
         const int ver = st2ed[d.offset + v];
         for (int e = ovf[q]; e < row_off[ver + 1]; ++e) {
           const int nb = col[e];
-          if (graph_id[nb] == d.part_id) acc |= F[ed2st[nb] - d.offset];
+          if (graph_id[nb] == d.part_id) acc |= cur[ed2st[nb] - d.offset];
         }
       }
       nx[q] = acc & ~vis[q];
     }
-    __syncthreads();  // every read of this level's frontier is done
     int any = 0;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int v = tid + kThreads * q;
-      if (v < s) F[v] = nx[q];  // rows >= s never get a bit
+      if (v < s) nxt[v] = nx[q];  // rows >= s never get a bit
       vis[q] |= nx[q];
       any |= nx[q] != 0;
     }
@@ -185,7 +192,7 @@ __global__ __launch_bounds__(kThreads) void inpart_bfs_kernel(const InpartDesc *
 
 }  // namespace
 
-int inpart_bfs_lds_bytes(int s_max) { return (s_max + 1) * 8; }
+int inpart_bfs_lds_bytes(int s_max) { return 2 * (s_max + 1) * 8; }
 
 int launch_inpart_bfs(const InpartDesc *d_descs, int nd, int total_groups, int s_max,
                       const int *row_off, const int *col, const int *graph_id, const int *st2ed,

@@ -47,7 +47,7 @@ int launch_bg_closure(const BgVert *d_verts, const int *d_wt, int nb_total, int 
 int launch_bg_rows(const BgPart *d_parts, const int64_t *d_orow, int nb_total, const int *d_mat,
                    int Sb, int max_bdy, const int *d_pidx_by_pid, const int *graph_id,
                    const int *ed2st, int n, int *d_out, int64_t ld_out, int *d_max_b,
-                   hipStream_t stream);
+                   bool sym, hipStream_t stream);
 int launch_fw16(const Fw16Desc *d_descs, int ndesc, int max_nb, hipStream_t stream);
 int launch_fw16_unpack(const UnpackDesc *d_descs, int nd, int64_t total_items,
                        const int *d_comp_sorted, int *d_flags, hipStream_t stream);
@@ -280,8 +280,11 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   std::vector<DecodeDesc> dd(P);
   std::vector<Fw16Desc> fd16(P);
   std::vector<UnpackDesc> ud(P);
-  std::vector<InpartDesc> ipd(P);
-  int ip_groups = 0;
+  // In-part BFS in two launches: the source groups holding each part's boundary
+  // vertices (all the boundary stage reads, the in-part matrices being symmetric),
+  // then the rest, which runs beside the boundary stage.
+  std::vector<InpartDesc> ipa, ipb;
+  int ipa_groups = 0, ipb_groups = 0;
   std::vector<int> offsets(P), src_begin(P);
   std::vector<int64_t> out_rows(P);
   int rows_total = 0, src_total = 0;
@@ -316,8 +319,14 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
     md[p] = m;
     if (p > 0 && chunk_of[p] != chunk_of[p - 1]) dec_total = unpack_total = 0;
     ud[p] = UnpackDesc{fw16, fw, S[p], q.num, q.offset, p, batched ? unpack_total : 0};
-    ipd[p] = InpartDesc{fw, S[p], q.num, q.offset, q.part_id, ip_groups};
-    ip_groups += nb[p];  // ceil(s / 64) groups of 64 sources
+    {  // nb[p] = ceil(s / 64) groups of 64 sources
+      const int ga = std::min(nb[p], (q.bdy + 63) / 64);
+      if (ga > 0) ipa.push_back(InpartDesc{fw, S[p], q.num, q.offset, q.part_id, ipa_groups, 0});
+      if (nb[p] > ga)
+        ipb.push_back(InpartDesc{fw, S[p], q.num, q.offset, q.part_id, ipb_groups, ga});
+      ipa_groups += ga;
+      ipb_groups += nb[p] - ga;
+    }
     unpack_total += (int64_t)S[p] * S[p] / 8;
     chunk_unpack[chunk_of[p]] += (int64_t)S[p] * S[p] / 8;
     dd[p] = DecodeDesc{fw, S[p], d_out + q.out_row * ld_out, ld_out, q.bdy, q.num, q.offset,
@@ -374,7 +383,7 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   const int64_t o_md = carve(sizeof(MergeDesc) * std::max(nmd, 1));
   const int64_t o_dd = carve(sizeof(DecodeDesc) * P);
   const int64_t o_fd16 = carve(sizeof(Fw16Desc) * P), o_ud = carve(sizeof(UnpackDesc) * P);
-  const int64_t o_ip = carve(sizeof(InpartDesc) * P);
+  const int64_t o_ip = carve(sizeof(InpartDesc) * 2 * P);
   const int64_t o_off = carve(4LL * P), o_sb = carve(4LL * P), o_or = carve(8LL * P);
   const int64_t o_src = carve(4LL * std::max(src_total, 1));
   const int64_t o_rows = carve(8LL * std::max(src_total, 1));
@@ -419,7 +428,10 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   std::memcpy(host.data() + o_dd, dd.data(), sizeof(DecodeDesc) * P);
   std::memcpy(host.data() + o_fd16, fd16.data(), sizeof(Fw16Desc) * P);
   std::memcpy(host.data() + o_ud, ud.data(), sizeof(UnpackDesc) * P);
-  std::memcpy(host.data() + o_ip, ipd.data(), sizeof(InpartDesc) * P);
+  if (!ipa.empty()) std::memcpy(host.data() + o_ip, ipa.data(), sizeof(InpartDesc) * ipa.size());
+  if (!ipb.empty())
+    std::memcpy(host.data() + o_ip + sizeof(InpartDesc) * P, ipb.data(),
+                sizeof(InpartDesc) * ipb.size());
   std::memcpy(host.data() + o_off, offsets.data(), 4LL * P);
   std::memcpy(host.data() + o_sb, src_begin.data(), 4LL * P);
   std::memcpy(host.data() + o_or, out_rows.data(), 8LL * P);
@@ -440,7 +452,9 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   int64_t *d_rows = reinterpret_cast<int64_t *>(meta + o_rows);
   const Fw16Desc *d_fd16 = reinterpret_cast<const Fw16Desc *>(meta + o_fd16);
   const UnpackDesc *d_ud = reinterpret_cast<const UnpackDesc *>(meta + o_ud);
-  const InpartDesc *d_ip = reinterpret_cast<const InpartDesc *>(meta + o_ip);
+  const InpartDesc *d_ipa = reinterpret_cast<const InpartDesc *>(meta + o_ip);
+  const InpartDesc *d_ipb = d_ipa + P;
+  hipEvent_t ipb_done = nullptr;  // the second in-part BFS launch (internal stream)
   const BgPart *d_bgp = reinterpret_cast<const BgPart *>(meta + o_bgp);
   const int *d_pidx = reinterpret_cast<const int *>(meta + o_pidx);
   int *d_flags = reinterpret_cast<int *>(meta + o_flags);
@@ -481,9 +495,20 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   // Floyd-Warshall of parts [p0, p1) with max tile count mnb; unpack_items: the
   // prefix end of those parts' unpack items.
   auto run_fw = [&](int p0, int p1, int mnb, int64_t unpack_items, hipStream_t on) -> int {
-    if (bfs_parts)  // boundary-graph form: one chunk, all parts
-      return launch_inpart_bfs(d_ip, P, ip_groups, s_max_all, g->d_row_offset, g->d_col,
-                               g->d_graph_id, g->d_st2ed, g->d_ed2st, on);
+    if (bfs_parts) {  // boundary-graph form: one chunk, all parts
+      if (ipb_groups > 0) {
+        const hipStream_t side = aux_stream(1);
+        hipEvent_t fork;
+        FAP_TRY(event(on, &fork));
+        FAP_CHECK_HIP(hipStreamWaitEvent(side, fork, 0));
+        FAP_TRY(launch_inpart_bfs(d_ipb, (int)ipb.size(), ipb_groups, s_max_all,
+                                  g->d_row_offset, g->d_col, g->d_graph_id, g->d_st2ed,
+                                  g->d_ed2st, side));
+        FAP_TRY(event(side, &ipb_done));
+      }
+      return launch_inpart_bfs(d_ipa, (int)ipa.size(), ipa_groups, s_max_all, g->d_row_offset,
+                               g->d_col, g->d_graph_id, g->d_st2ed, g->d_ed2st, on);
+    }
     if (!use16) return launch_fw(d_fd + p0, p1 - p0, mnb, on);
     FAP_TRY(launch_fw16(d_fd16 + p0, p1 - p0, mnb, on));
     return launch_fw16_unpack(d_ud + p0, p1 - p0, unpack_items,
@@ -524,8 +549,10 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
     FAP_TRY(launch_bg_closure(bg_verts, bg_wt, bg_total, Sb, d_bgp, P, d_pidx, g->d_row_offset,
                               g->d_col, g->d_w, g->d_graph_id, g->d_st2ed, g->d_ed2st, bg_mat,
                               on));
+    // Paired arcs (component labels) of unit weight: the in-part matrices are symmetric.
     return launch_bg_rows(d_bgp, bg_orow, bg_total, bg_mat, Sb, bg_maxb, d_pidx, g->d_graph_id,
-                          g->d_ed2st, n, d_out, ld_out, d_maxb, on);
+                          g->d_ed2st, n, d_out, ld_out, d_maxb,
+                          g->d_comp != nullptr && g->d_w == nullptr, on);
   };
 
   if (batched) {
@@ -558,6 +585,7 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
       const int64_t cunpack = overlap ? chunk_unpack[c] : unpack_total;
       FAP_TRY(span(2, on, [&](hipStream_t s2) { return run_fw(p0, p1, cmax_nb, cunpack, s2); }));
       if (hier) FAP_TRY(span(0, on, run_boundary_graph));  // hier: one chunk
+      if (ipb_done) FAP_CHECK_HIP(hipStreamWaitEvent(on, ipb_done, 0));
       FAP_TRY(span(3, on, [&](hipStream_t s2) -> int {
         if (overlap) {
           for (const MergeGroup &gr : groups)
