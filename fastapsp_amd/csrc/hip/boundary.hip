@@ -207,6 +207,159 @@ __global__ __launch_bounds__(kClThreads) void bg_closure_kernel(
     *reinterpret_cast<v4i *>(dst + j) = *reinterpret_cast<const v4i *>(&dist[j]);
 }
 
+// The same closure for unit-weight graphs with every G_B vertex's clique weights in
+// LDS (16 bits, 0xFFFF = unreachable inside the part; in-part distances stay below
+// s_max < 0xFFFF), NS sources per workgroup sharing them, and work lists: a part is
+// pulled only when one of its boundary distances fell since its last pull (a byte
+// flag per part at the part's first G_B index, double-buffered: the push of a sweep
+// sets the next sweep's flags while this sweep's are cleared), and a vertex pushes
+// its cut arcs (weight 1) only when its distance fell since its last push.  The final
+// sweep changes nothing, so every constraint holds: a part not pulled kept its
+// boundary values since a pull that satisfied its clique constraints, and a vertex
+// not pushed kept its value since a push that satisfied its arcs.  On MI355X, for
+// 1,297 G_B vertices (15-19 sweeps per source), re-pulling every part in every sweep
+// took 251 us.  LDS: the weights (wlen u16), NS distance rows of stride Sb + kClPad,
+// then 2 x NS flag rows of the same stride in bytes and the part start of every G_B
+// vertex (u16, Sb entries).
+constexpr int kClwThreads = 512;
+template <int VPT, int NS>
+__global__ __launch_bounds__(kClwThreads) void bg_closure_lds_kernel(
+    const BgVert *__restrict__ verts, const int *__restrict__ wt, int wlen, int nb_total, int Sb,
+    const BgPart *__restrict__ parts, int np, const int *__restrict__ pidx_by_pid,
+    const int *__restrict__ row_off, const int *__restrict__ col,
+    const int *__restrict__ graph_id, const int *__restrict__ st2ed,
+    const int *__restrict__ ed2st, int cb_total, int *__restrict__ out) {
+  extern __shared__ int sh[];
+  unsigned short *w16 = reinterpret_cast<unsigned short *>(sh);
+  const int ld = Sb + kClPad;
+  int *dist0 = sh + wlen / 2;
+  unsigned char *flags = reinterpret_cast<unsigned char *>(dist0 + NS * ld);  // [2][NS][ld]
+  unsigned short *pbs = reinterpret_cast<unsigned short *>(flags + 2 * NS * ld);
+  const int s0 = blockIdx.x * NS;
+  const int tid = threadIdx.x;
+  int bs[VPT], bdy[VPT], woff[VPT], ncut[VPT], cut_t[VPT][kBgCutW];
+  int pushed[VPT][NS];  // distance at the vertex's last push (FAP_INF: never pushed)
+#pragma unroll
+  for (int i = 0; i < VPT; ++i) {
+    const int v = tid + i * kClwThreads;
+    bs[i] = bdy[i] = woff[i] = ncut[i] = 0;
+#pragma unroll
+    for (int c = 0; c < kBgCutW; ++c) cut_t[i][c] = v;  // a self arc never improves
+    if (v < nb_total) {
+      const BgVert bv = verts[v];
+      bs[i] = bv.bs;
+      bdy[i] = bv.bdy;
+      woff[i] = bv.woff;
+      ncut[i] = bv.ncut;
+#pragma unroll
+      for (int c = 0; c < kBgCutW; ++c) cut_t[i][c] = bv.cut_t[c];
+      pbs[v] = (unsigned short)bv.bs;
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) pushed[i][s] = kInf;
+  }
+  for (int j = tid; j < wlen; j += kClwThreads)
+    w16[j] = j < cb_total ? (unsigned short)min(wt[j], 0xFFFF) : (unsigned short)0xFFFF;
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+    for (int j = tid; j < ld; j += kClwThreads) dist0[s * ld + j] = j == s0 + s ? 0 : kInf;
+  for (int j = tid; j < 2 * NS * ld; j += kClwThreads) flags[j] = 0;
+  __syncthreads();
+  // The source's part is pulled in the first sweep.
+  if (tid < NS && s0 + tid < nb_total) flags[tid * ld + pbs[s0 + tid]] = 1;
+  __syncthreads();
+  for (int sweep = 0; sweep <= nb_total; ++sweep) {
+    unsigned char *cur = flags + (sweep & 1) * NS * ld;
+    unsigned char *nxt = flags + ((sweep & 1) ^ 1) * NS * ld;
+    bool changed = false;
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) {
+      const int v = tid + i * kClwThreads;
+      if (v >= nb_total) continue;
+      bool pull[NS], anyp = false;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        pull[s] = cur[s * ld + bs[i]] != 0;
+        anyp |= pull[s];
+      }
+      if (!anyp) continue;
+      const unsigned short *wv = w16 + woff[i];
+      int best[NS];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) best[s] = dist0[s * ld + v];
+      for (int c0 = 0; c0 < bdy[i]; c0 += kClPad) {
+        int x[kClPad];
+#pragma unroll
+        for (int u = 0; u < kClPad; ++u) {
+          const int t = wv[c0 + u];
+          x[u] = (t == 0xFFFF || c0 + u >= bdy[i]) ? kInf : t;
+        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+          if (!pull[s]) continue;
+          const int *dv = dist0 + s * ld + bs[i] + c0;
+          int y[kClPad];
+#pragma unroll
+          for (int u = 0; u < kClPad; ++u) y[u] = dv[u];
+#pragma unroll
+          for (int u = 0; u < kClPad; u += 2)
+            best[s] = min3_i32(best[s], x[u] + y[u], x[u + 1] + y[u + 1]);
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+        if (pull[s] && best[s] < dist0[s * ld + v]) {
+          dist0[s * ld + v] = best[s];
+          changed = true;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) {
+      const int u = tid + i * kClwThreads;
+      if (u >= nb_total) continue;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        cur[s * ld + bs[i]] = 0;  // this sweep's flags; the push sets nxt
+        int *dist = dist0 + s * ld;
+        const int du = dist[u];
+        if (du >= pushed[i][s]) continue;  // unchanged since its last push (or FAP_INF)
+        pushed[i][s] = du;
+        const int nd = du + 1;
+#pragma unroll
+        for (int c = 0; c < kBgCutW; ++c) {
+          const int t = cut_t[i][c];
+          if (nd < dist[t] && nd < atomicMin(&dist[t], nd)) {
+            changed = true;
+            nxt[s * ld + pbs[t]] = 1;
+          }
+        }
+        if (ncut[i] > kBgCutW) {  // more cut arcs than the record holds: read the CSR
+          const BgPart p = parts[find_part_by_bs(parts, np, u)];
+          const int vtx = st2ed[p.offset + (u - p.bs)];
+          for (int e = row_off[vtx]; e < row_off[vtx + 1]; ++e) {
+            const int t = cut_target(e, p.part_id, parts, pidx_by_pid, col, graph_id, ed2st);
+            if (t < 0) continue;
+            if (nd < dist[t] && nd < atomicMin(&dist[t], nd)) {
+              changed = true;
+              nxt[s * ld + pbs[t]] = 1;
+            }
+          }
+        }
+      }
+    }
+    if (!__syncthreads_or(changed)) break;
+  }
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    if (s0 + s >= nb_total) break;
+    int *dst = out + (int64_t)(s0 + s) * Sb;
+    const int *src = dist0 + s * ld;
+    for (int j = tid * 4; j < Sb; j += kClwThreads * 4)
+      *reinterpret_cast<v4i *>(dst + j) = *reinterpret_cast<const v4i *>(src + j);
+  }
+}
+
 constexpr int kRowsCols = 1024;  // output columns per workgroup (256 threads x 4)
 constexpr int kRowsMax = 16;     // G_B rows staged in LDS at a time
 
@@ -292,13 +445,49 @@ __device__ __forceinline__ void publish_max(int *max_b, int mx) {
   if ((threadIdx.x & 63) == 0) atomicMax(max_b, mx);
 }
 
+// Rows r < rows of the staged chunk for one column j of each lane, W boundary steps:
+// W >= the largest boundary of the wave's columns (uniform), so the entries past a
+// lane's own boundary meet fin = FAP_INF and never win.
+template <int W, int KP>
+__device__ __forceinline__ void bg_rows_col(const int *lds, int ldl, const int64_t *ooff,
+                                            int rows, int bs, const int (&fin)[KP],
+                                            int *__restrict__ out, int j, int n, int &mx) {
+  for (int r = 0; r < rows; ++r) {
+    const int *a = lds + r * ldl + bs;
+    int acc = kInf;
+#pragma unroll
+    for (int c = 0; c < W; c += 2) acc = min3_i32(acc, a[c] + fin[c], a[c + 1] + fin[c + 1]);
+    if (j < n) {
+      __builtin_nontemporal_store(acc, out + ooff[r] + j);
+      if (acc < kInf) mx = max(mx, acc);
+    }
+  }
+}
+
+// The smallest W in {4, 8, ..., KP} with W >= wb (wb uniform, so is the branch).
+template <int KP, int W = 4>
+__device__ __forceinline__ void bg_rows_pick(int wb, const int *lds, int ldl,
+                                             const int64_t *ooff, int rows, int bs,
+                                             const int (&fin)[KP], int *__restrict__ out, int j,
+                                             int n, int &mx) {
+  if constexpr (W >= KP) {
+    bg_rows_col<KP, KP>(lds, ldl, ooff, rows, bs, fin, out, j, n, mx);
+  } else {
+    if (wb <= W) bg_rows_col<W, KP>(lds, ldl, ooff, rows, bs, fin, out, j, n, mx);
+    else bg_rows_pick<KP, W + 4>(wb, lds, ldl, ooff, rows, bs, fin, out, j, n, mx);
+  }
+}
+
 // Every part has at most KP boundary vertices.  Thread t handles columns
 // blockIdx.x * 1024 + e * 256 + t (e < 4): the lanes of a wave cover 64 consecutive
-// vertices, so they share one or a few parts (LDS broadcast reads) and every store
-// is one coalesced 256-byte run.  Each lane loads d_q(c, l) of its columns once
-// (registers; from row l when the in-part matrices are symmetric), then walks its
-// workgroup's share of G_B rows (blockIdx.y of gridDim.y splits) through LDS, R rows
-// at a time.  LDS stride ldl = Sb + KP (INF tail).
+// vertices, so they share a few parts (LDS broadcast reads) and every store is one
+// coalesced 256-byte run.  Each lane loads d_q(c, l) of its columns once (registers;
+// from row l when the in-part matrices are symmetric), then walks its workgroup's
+// share of G_B rows (blockIdx.y of gridDim.y splits) through LDS, R rows at a time.
+// A wave's column e takes as many boundary steps as the largest boundary among its
+// 64 columns, rounded up to 4 (on the 114,599-vertex road-like graph, K = 128:
+// 17 steps per column on average against 24 for the largest part).  LDS stride
+// ldl = Sb + KP (INF tail).
 template <int KP, bool kSym>
 __global__ __launch_bounds__(256) void bg_rows_kernel(
     const BgPart *__restrict__ parts, const int64_t *__restrict__ orow, int nb_total,
@@ -311,7 +500,7 @@ __global__ __launch_bounds__(256) void bg_rows_kernel(
   const int per = (nb_total + gridDim.y - 1) / gridDim.y;
   const int gbeg = blockIdx.y * per, gend = min(nb_total, gbeg + per);
   const int jb = blockIdx.x * kRowsCols + threadIdx.x;
-  int bs[4];
+  int bs[4], wb[4];
   int fin[4][KP];
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
@@ -319,6 +508,10 @@ __global__ __launch_bounds__(256) void bg_rows_kernel(
     bs[e] = ci.bs;
     if (kSym) load_fin_sym<KP>(fin[e], ci);
     else load_fin<KP>(fin[e], ci, 0);
+    int m = ci.bdy;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+    wb[e] = __builtin_amdgcn_readfirstlane(m);
   }
   int mx = 0;  // largest finite distance written by this thread
   for (int g0 = gbeg; g0 < gend; g0 += R) {
@@ -326,22 +519,9 @@ __global__ __launch_bounds__(256) void bg_rows_kernel(
     __syncthreads();  // the previous chunk's reads are done
     stage_rows(lds, ooff, mat, orow, ld_out, Sb, ldl, g0, rows);
     __syncthreads();
-    for (int r = 0; r < rows; ++r) {
-      const int *lr = lds + r * ldl;
-      int *orow_p = out + ooff[r];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int *a = lr + bs[e];
-        int acc = kInf;
-#pragma unroll
-        for (int c = 0; c < KP; c += 2) acc = min3_i32(acc, a[c] + fin[e][c], a[c + 1] + fin[e][c + 1]);
-        const int j = jb + e * 256;
-        if (j < n) {
-          __builtin_nontemporal_store(acc, orow_p + j);
-          if (acc < kInf) mx = max(mx, acc);
-        }
-      }
-    }
+    for (int e = 0; e < 4; ++e)
+      bg_rows_pick<KP>(wb[e], lds, ldl, ooff, rows, bs[e], fin[e], out, jb + e * 256, n, mx);
   }
   publish_max(max_b, mx);
 }
@@ -436,12 +616,45 @@ int launch_bg_vert(const BgPart *d_parts, int np, int nb_total, const int *d_pid
   return FAP_OK;
 }
 
-int launch_bg_closure(const BgVert *d_verts, const int *d_wt, int nb_total, int Sb,
-                      const BgPart *d_parts, int np, const int *d_pidx_by_pid, const int *row_off,
-                      const int *col, const int *w, const int *graph_id, const int *st2ed,
-                      const int *ed2st, int *d_mat, hipStream_t stream) {
+int launch_bg_closure(const BgVert *d_verts, const int *d_wt, int cb_total, bool unit,
+                      int nb_total, int Sb, const BgPart *d_parts, int np,
+                      const int *d_pidx_by_pid, const int *row_off, const int *col, const int *w,
+                      const int *graph_id, const int *st2ed, const int *ed2st, int *d_mat,
+                      hipStream_t stream) {
   if (nb_total <= 0) return FAP_OK;
   const int vpt = (nb_total + kClThreads - 1) / kClThreads;
+  // LDS form (unit weights, cut arcs of weight 1): weights as u16 (slack of kClPad, a
+  // multiple of 8 so the rows after them are 16-byte aligned).  NS sources per
+  // workgroup: the fewest that fit every workgroup on the device at once (two per CU:
+  // the registers of the 512-thread kernel with VPT = 3 allow two), within 64 KB.
+  const int wlen = (cb_total + kClPad + 7) / 8 * 8;
+  const int ld = Sb + kClPad;
+  const int vptw = (nb_total + kClwThreads - 1) / kClwThreads;
+  auto lds_of = [&](int ns) {
+    return ((size_t)wlen / 2 + (size_t)ns * ld) * 4 + (size_t)2 * ns * ld + (size_t)Sb * 2;
+  };
+  int ns = 1;
+  const int slots = 2 * std::max(1, device_cus());
+  while (ns < 4 && (nb_total + ns - 1) / ns > slots && lds_of(ns + 1) <= 65536) ++ns;
+  if (unit && w == nullptr && lds_of(ns) <= 65536 && vptw <= 4) {
+    const int grid = (nb_total + ns - 1) / ns;
+#define FAP_CLW_CASE(V, NS)                                                                   \
+    if (vptw <= V && ns == NS) {                                                              \
+      hipLaunchKernelGGL((bg_closure_lds_kernel<V, NS>), dim3(grid), dim3(kClwThreads),        \
+                         lds_of(NS), stream, d_verts, d_wt, wlen, nb_total, Sb, d_parts, np,  \
+                         d_pidx_by_pid, row_off, col, graph_id, st2ed, ed2st, cb_total,       \
+                         d_mat);                                                              \
+      FAP_CHECK_LAUNCH();                                                                     \
+      return FAP_OK;                                                                          \
+    }
+#define FAP_CLW_NS(V) FAP_CLW_CASE(V, 1) FAP_CLW_CASE(V, 2) FAP_CLW_CASE(V, 3) FAP_CLW_CASE(V, 4)
+    FAP_CLW_NS(1)
+    FAP_CLW_NS(2)
+    FAP_CLW_NS(3)
+    FAP_CLW_NS(4)
+#undef FAP_CLW_NS
+#undef FAP_CLW_CASE
+  }
 #define FAP_CL_CASE(V)                                                                        \
   if (vpt <= V) {                                                                             \
     hipLaunchKernelGGL(bg_closure_kernel<V>, dim3(nb_total), dim3(kClThreads),                 \
@@ -460,8 +673,10 @@ int launch_bg_closure(const BgVert *d_verts, const int *d_wt, int nb_total, int 
   return FAP_ERR_INVALID;
 }
 
-// Rows of G_B staged at a time: they fit in 64 KB of LDS.
-int bg_rows_chunk(int ldl) { return std::max(1, std::min(kRowsMax, 65536 / (ldl * 4))); }
+// Rows of G_B staged at a time: they fit in 52 KB of LDS, so three workgroups can
+// share a CU (the registers of bg_rows_kernel with KP <= 24 allow three, the wider
+// forms two).
+int bg_rows_chunk(int ldl) { return std::max(1, std::min(kRowsMax, 53248 / (ldl * 4))); }
 
 int launch_bg_rows(const BgPart *d_parts, const int64_t *d_orow, int nb_total, const int *d_mat,
                    int Sb, int max_bdy, const int *d_pidx_by_pid, const int *graph_id,
@@ -469,13 +684,15 @@ int launch_bg_rows(const BgPart *d_parts, const int64_t *d_orow, int nb_total, c
                    bool sym, hipStream_t stream) {
   if (nb_total <= 0 || n <= 0) return FAP_OK;
   const int cols = (n + kRowsCols - 1) / kRowsCols;
-  // Row splits: about two workgroups per CU in all, each split at least one chunk.
+  // Row splits: as many workgroups as fit on the device at once (a second round of a
+  // few workgroups would double the time), each split at least one chunk.
   const bool narrow = max_bdy <= 32;
   const int KP = narrow ? std::max(8, (max_bdy + 7) / 8 * 8) : 0;
   const int KC = narrow ? 0 : 16;
   const int ldl = Sb + (narrow ? KP : KC);
   const int R = bg_rows_chunk(ldl);
-  const int splits = std::max(1, std::min((nb_total + R - 1) / R, (512 + cols - 1) / cols));
+  const int slots = std::max(1, device_cus()) * (narrow && KP <= 24 ? 3 : 2);
+  const int splits = std::max(1, std::min((nb_total + R - 1) / R, slots / cols));
   const dim3 grid(cols, splits);
   const size_t lds = (size_t)R * ldl * 4;
 #define FAP_BG_CASE(KPV)                                                                      \

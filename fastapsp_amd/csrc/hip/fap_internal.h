@@ -19,6 +19,8 @@ void clear_error();
 hipStream_t lib_stream();
 // Internal streams (i = 0, 1) used to overlap the stages of different parts.
 hipStream_t aux_stream(int i);
+// Compute units of the device (set by fap_hip_init).
+int device_cus();
 inline hipStream_t resolve(void *stream) {
   return stream ? static_cast<hipStream_t>(stream) : lib_stream();
 }

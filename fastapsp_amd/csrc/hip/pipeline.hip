@@ -40,10 +40,11 @@ int launch_bg_vert(const BgPart *d_parts, int np, int nb_total, const int *d_pid
                    const int *row_off, const int *col, const int *w, const int *graph_id,
                    const int *st2ed, const int *ed2st, BgVert *d_verts, int *d_wt,
                    int64_t *d_orow, hipStream_t stream);
-int launch_bg_closure(const BgVert *d_verts, const int *d_wt, int nb_total, int Sb,
-                      const BgPart *d_parts, int np, const int *d_pidx_by_pid, const int *row_off,
-                      const int *col, const int *w, const int *graph_id, const int *st2ed,
-                      const int *ed2st, int *d_mat, hipStream_t stream);
+int launch_bg_closure(const BgVert *d_verts, const int *d_wt, int cb_total, bool unit,
+                      int nb_total, int Sb, const BgPart *d_parts, int np,
+                      const int *d_pidx_by_pid, const int *row_off, const int *col, const int *w,
+                      const int *graph_id, const int *st2ed, const int *ed2st, int *d_mat,
+                      hipStream_t stream);
 int launch_bg_rows(const BgPart *d_parts, const int64_t *d_orow, int nb_total, const int *d_mat,
                    int Sb, int max_bdy, const int *d_pidx_by_pid, const int *graph_id,
                    const int *ed2st, int n, int *d_out, int64_t ld_out, int *d_max_b,
@@ -394,6 +395,7 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   int *bg_mat = nullptr, *bg_wt = nullptr;
   BgVert *bg_verts = nullptr;
   int64_t *bg_orow = nullptr;
+  int bg_cb = 0;  // clique weights of all G_B vertices
   if (hier) {
     FAP_TRY(g_ws_bg.ensure(std::max<int64_t>(1, (int64_t)Sb * Sb) * 4));
     bg_mat = static_cast<int *>(g_ws_bg.ptr);
@@ -407,6 +409,7 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
       bs += q.bdy;
       cb += q.bdy * q.bdy;  // <= 4096^2
     }
+    bg_cb = cb;
     const int64_t vbytes = align_up((int64_t)sizeof(BgVert) * std::max(bg_total, 1), 256);
     const int64_t obytes = align_up(8LL * std::max(bg_total, 1), 256);
     FAP_TRY(g_ws_bgv.ensure(vbytes + obytes + 4LL * (cb + 64)));  // + slack read past the end
@@ -546,9 +549,11 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
     FAP_TRY(fap_hip_fill_i32(d_maxb, 0, 1, on));
     FAP_TRY(launch_bg_vert(d_bgp, P, bg_total, d_pidx, g->d_row_offset, g->d_col, g->d_w,
                            g->d_graph_id, g->d_st2ed, g->d_ed2st, bg_verts, bg_wt, bg_orow, on));
-    FAP_TRY(launch_bg_closure(bg_verts, bg_wt, bg_total, Sb, d_bgp, P, d_pidx, g->d_row_offset,
-                              g->d_col, g->d_w, g->d_graph_id, g->d_st2ed, g->d_ed2st, bg_mat,
-                              on));
+    // Unit weights: in-part distances below s_max < 2^16, so the clique weights fit
+    // 16 bits.
+    FAP_TRY(launch_bg_closure(bg_verts, bg_wt, bg_cb, g->d_w == nullptr && s_max < 0xFFFF,
+                              bg_total, Sb, d_bgp, P, d_pidx, g->d_row_offset, g->d_col, g->d_w,
+                              g->d_graph_id, g->d_st2ed, g->d_ed2st, bg_mat, on));
     // Paired arcs (component labels) of unit weight: the in-part matrices are symmetric.
     return launch_bg_rows(d_bgp, bg_orow, bg_total, bg_mat, Sb, bg_maxb, d_pidx, g->d_graph_id,
                           g->d_ed2st, n, d_out, ld_out, d_maxb,

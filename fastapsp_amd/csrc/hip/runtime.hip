@@ -16,6 +16,7 @@ bool g_init = false;
 int g_device = -1;
 hipStream_t g_stream = nullptr;
 hipStream_t g_aux[2] = {nullptr, nullptr};
+int g_cus = 0;  // compute units of the device
 hipEvent_t g_ws_fence = nullptr;  // end of the last call that used the pipeline workspaces
 bool g_ws_fence_set = false;
 int64_t g_budget = 0;  // 0 = unlimited
@@ -36,6 +37,7 @@ void clear_error() { g_last_error.clear(); }
 
 hipStream_t lib_stream() { return g_stream; }
 hipStream_t aux_stream(int i) { return g_aux[i & 1]; }
+int device_cus() { return g_cus; }
 bool initialized() { return g_init; }
 
 int ws_fence_wait(hipStream_t st) {
@@ -173,6 +175,7 @@ int fap_hip_init(int device) {
   FAP_CHECK_HIP(hipStreamCreateWithFlags(&g_aux[0], hipStreamNonBlocking));
   FAP_CHECK_HIP(hipStreamCreateWithFlags(&g_aux[1], hipStreamNonBlocking));
   FAP_CHECK_HIP(hipEventCreateWithFlags(&g_ws_fence, hipEventDisableTiming));
+  FAP_CHECK_HIP(hipDeviceGetAttribute(&g_cus, hipDeviceAttributeMultiprocessorCount, device));
   g_device = device;
   g_init = true;
   return FAP_OK;
