@@ -222,3 +222,24 @@ def test_inpart_bfs_high_degree_and_disconnected_parts():
     g.preCondition()
     a = _forms_agree(g)
     assert (a < F.FAP_INF).all()  # connected through part 2
+
+
+def test_boundary_graph_parts_without_inner_vertices():
+    # Path of 400 vertices cut into 40 segments alternating 2 and 18 vertices: the
+    # 2-vertex parts are all boundary (no merge, boundary rows only), and the 20
+    # parts of boundary 2 exceed one merge launch of the boundary-graph form.
+    n = 400
+    say(f"boundary graph: path n={n}, 40 segments of 2 and 18 vertices")
+    ro, col = _path(n)
+    lens = [2, 18] * 20
+    gid = np.repeat(np.arange(1, len(lens) + 1), lens).astype(np.int32)
+    g = F.FapGraph.from_csr(ro, col, None, False, len(lens))
+    g.setPartition(gid)
+    g.preCondition()
+    a, info = _solve(g, F.FAP_MODE_BATCHED)
+    assert info["boundary_graph"] == 1
+    st2ed = g.getMapping().astype(np.int64)
+    want = np.abs(st2ed[:, None] - np.arange(n)[None, :]).astype(np.int32)
+    assert np.array_equal(a, want)
+    b, _ = _solve(g, F.FAP_MODE_BATCHED | SSSP)
+    assert np.array_equal(a, b)
