@@ -37,6 +37,9 @@
 //                    vertices, walked in register chunks).  Both also record the
 //                    largest finite distance written (the merge's 16-bit bound).
 #include <algorithm>
+#include <cstring>
+#include <utility>
+#include <vector>
 
 #include "fap_internal.h"
 #include "fw_desc.h"
@@ -438,6 +441,26 @@ __device__ __forceinline__ void load_fin_sym(int (&fin)[KP], const ColInfo &ci) 
     if (c >= ci.bdy) fin[c] = kInf;
 }
 
+// Chunk c0 .. c0 + KC - 1 of the same from a symmetric matrix: row l, columns c0 + c
+// (16-byte loads; c0 is a multiple of KC, a multiple of 4, and the chunk ends within
+// the first ceil(bdy / 64) * 64 <= S columns, which the in-part BFS writes before the
+// boundary stage).
+template <int KC>
+__device__ __forceinline__ void load_fin_sym_chunk(int (&fin)[KC], const ColInfo &ci, int c0) {
+  const int4 *row = reinterpret_cast<const int4 *>(ci.fw + (int64_t)ci.l * ci.S + c0);
+#pragma unroll
+  for (int c = 0; c < KC; c += 4) {
+    const int4 v = row[c / 4];
+    fin[c] = v.x;
+    fin[c + 1] = v.y;
+    fin[c + 2] = v.z;
+    fin[c + 3] = v.w;
+  }
+#pragma unroll
+  for (int c = 0; c < KC; ++c)
+    if (c0 + c >= ci.bdy) fin[c] = kInf;
+}
+
 // *max_b = max(*max_b, the wave's largest value of mx): one atomic per wave.
 __device__ __forceinline__ void publish_max(int *max_b, int mx) {
 #pragma unroll
@@ -530,9 +553,11 @@ __global__ __launch_bounds__(256) void bg_rows_kernel(
 // vertices; per chunk each lane loads d_q(c, l) of its four columns (FAP_INF past
 // b_q) and folds the chunk into all staged rows' accumulators.  A chunk past the
 // lane's own boundary reads the INF tail (base clamped to Sb), so every LDS read
-// stays inside its row whatever the boundary sizes.  Columns as in bg_rows_kernel;
-// LDS stride ldl = Sb + KC.
-template <int KC>
+// stays inside its row whatever the boundary sizes.  Columns as in bg_rows_kernel
+// (kSym: d_q(c, l) from row l, which the boundary source groups of the in-part BFS
+// write; column l of rows c may come from the other groups, still running beside the
+// boundary stage); LDS stride ldl = Sb + KC.
+template <int KC, bool kSym>
 __global__ __launch_bounds__(256) void bg_rows_wide_kernel(
     const BgPart *__restrict__ parts, const int64_t *__restrict__ orow, int nb_total,
     const int *__restrict__ mat, int Sb, int R, int nchunks, const int *__restrict__ pidx_by_pid,
@@ -568,7 +593,8 @@ __global__ __launch_bounds__(256) void bg_rows_wide_kernel(
         base[e] = min(ci[e].bs + c0, Sb);
         // Chunks past this column's boundary need no loads (rows c0 + c may pass S).
         if (c0 < ci[e].bdy) {
-          load_fin<KC>(fin[e], ci[e], c0);
+          if (kSym) load_fin_sym_chunk<KC>(fin[e], ci[e], c0);
+          else load_fin<KC>(fin[e], ci[e], c0);
         } else {
 #pragma unroll
           for (int c = 0; c < KC; ++c) fin[e][c] = kInf;
@@ -718,13 +744,108 @@ int launch_bg_rows(const BgPart *d_parts, const int64_t *d_orow, int nb_total, c
     }
   } else {
     const int nchunks = (max_bdy + KC - 1) / KC;
-    hipLaunchKernelGGL((bg_rows_wide_kernel<16>), grid, dim3(256), lds, stream, d_parts, d_orow,
-                       nb_total, d_mat, Sb, R, nchunks, d_pidx_by_pid, graph_id, ed2st, n, d_out,
-                       ld_out, d_max_b);
+    if (sym)
+      hipLaunchKernelGGL((bg_rows_wide_kernel<16, true>), grid, dim3(256), lds, stream, d_parts,
+                         d_orow, nb_total, d_mat, Sb, R, nchunks, d_pidx_by_pid, graph_id, ed2st,
+                         n, d_out, ld_out, d_max_b);
+    else
+      hipLaunchKernelGGL((bg_rows_wide_kernel<16, false>), grid, dim3(256), lds, stream, d_parts,
+                         d_orow, nb_total, d_mat, Sb, R, nchunks, d_pidx_by_pid, graph_id, ed2st,
+                         n, d_out, ld_out, d_max_b);
   }
 #undef FAP_BG_CASE
   FAP_CHECK_LAUNCH();
   return FAP_OK;
+}
+
+int launch_inpart_bfs(const InpartDesc *d_descs, int nd, int total_groups, int s_max,
+                      const int *row_off, const int *col, const int *graph_id, const int *st2ed,
+                      const int *ed2st, hipStream_t stream);
+
+namespace {
+Workspace g_ub_meta;   // part tables, G_B vertices, output rows, clique weights, maximum
+Workspace g_ub_mat;    // closed G_B matrix
+}  // namespace
+
+// Boundary rows of parts that cover a unit-weight graph with paired arcs, for callers
+// other than the distance pipeline (whose boundary stage interleaves with its other
+// launches): in-part distances from every part's boundary sources by the bit-parallel
+// BFS into fw[p] (S[p] x S[p], the source groups that hold the boundary, so columns
+// 0 .. 64 ceil(b_p / 64) - 1 of every row), the boundary graph, its closure, and the
+// b_p rows of every part into d_out at rows out_row ..  Returns FAP_ERR_INVALID with
+// nothing launched when the parts do not qualify (weights, no component labels, parts
+// not covering the graph, more than 4096 boundary vertices or a part above the BFS
+// size limit).  Every entry is an exact integer min/plus expression, so the rows equal
+// one SSSP per boundary vertex.
+int boundary_rows_unit(const fap_graph_dev *g, const fap_part_desc *act, int P, int *const *fw,
+                       const int *S, int32_t *d_out, int64_t ld_out, hipStream_t st) {
+  if (g->d_w || !g->d_comp || P <= 0) return FAP_ERR_INVALID;
+  std::vector<std::pair<int, int>> span_of;
+  int nb_total = 0, max_bdy = 0, max_pid = 0, s_max = 0;
+  for (int p = 0; p < P; ++p) {
+    span_of.push_back({act[p].offset, act[p].num});
+    nb_total += act[p].bdy;
+    max_bdy = std::max(max_bdy, act[p].bdy);
+    max_pid = std::max(max_pid, act[p].part_id);
+    s_max = std::max(s_max, act[p].num);
+    if (S[p] % 64 != 0 || S[p] < act[p].num) return FAP_ERR_INVALID;
+  }
+  std::sort(span_of.begin(), span_of.end());
+  int next = 0;
+  for (const auto &x : span_of) {
+    if (x.first != next) return FAP_ERR_INVALID;
+    next = x.first + x.second;
+  }
+  if (next != g->n || nb_total == 0 || nb_total > 4096 || s_max > kInpartBfsMaxS)
+    return FAP_ERR_INVALID;
+  std::vector<InpartDesc> ipa;
+  std::vector<BgPart> bgp(P);
+  std::vector<int> pidx(max_pid + 1, 0);
+  int groups = 0, bs = 0, cb = 0;
+  for (int p = 0; p < P; ++p) {
+    const fap_part_desc &q = act[p];
+    const int ga = std::min(S[p] / 64, (q.bdy + 63) / 64);
+    if (ga > 0) ipa.push_back(InpartDesc{fw[p], S[p], q.num, q.offset, q.part_id, groups, 0});
+    groups += ga;
+    bgp[p] = BgPart{fw[p], S[p], q.bdy, q.offset, q.part_id, bs, cb, q.out_row};
+    pidx[q.part_id] = p;
+    bs += q.bdy;
+    cb += q.bdy * q.bdy;
+  }
+  const int Sb = (nb_total + 63) / 64 * 64;
+  auto al = [](int64_t b) { return (b + 255) / 256 * 256; };
+  const int64_t o_ip = 0, o_bgp = o_ip + al(sizeof(InpartDesc) * ipa.size());
+  const int64_t o_pidx = o_bgp + al(sizeof(BgPart) * P);
+  const int64_t upload = o_pidx + al(4LL * (max_pid + 1));
+  const int64_t o_verts = upload, o_orow = o_verts + al(sizeof(BgVert) * nb_total);
+  const int64_t o_wt = o_orow + al(8LL * nb_total), o_max = o_wt + al(4LL * (cb + 64));
+  FAP_TRY(g_ub_meta.ensure(o_max + 256));
+  FAP_TRY(g_ub_mat.ensure((int64_t)Sb * Sb * 4));
+  char *meta = static_cast<char *>(g_ub_meta.ptr);
+  std::vector<char> host(upload);
+  std::memcpy(host.data() + o_ip, ipa.data(), sizeof(InpartDesc) * ipa.size());
+  std::memcpy(host.data() + o_bgp, bgp.data(), sizeof(BgPart) * P);
+  std::memcpy(host.data() + o_pidx, pidx.data(), 4LL * (max_pid + 1));
+  FAP_CHECK_HIP(hipMemcpyAsync(meta, host.data(), upload, hipMemcpyHostToDevice, st));
+  FAP_CHECK_HIP(hipStreamSynchronize(st));
+  const BgPart *d_bgp = reinterpret_cast<const BgPart *>(meta + o_bgp);
+  const int *d_pidx = reinterpret_cast<const int *>(meta + o_pidx);
+  BgVert *d_verts = reinterpret_cast<BgVert *>(meta + o_verts);
+  int64_t *d_orow = reinterpret_cast<int64_t *>(meta + o_orow);
+  int *d_wt = reinterpret_cast<int *>(meta + o_wt);
+  int *d_max = reinterpret_cast<int *>(meta + o_max);
+  int *d_mat = static_cast<int *>(g_ub_mat.ptr);
+  FAP_TRY(launch_inpart_bfs(reinterpret_cast<const InpartDesc *>(meta + o_ip), (int)ipa.size(),
+                            groups, s_max, g->d_row_offset, g->d_col, g->d_graph_id, g->d_st2ed,
+                            g->d_ed2st, st));
+  FAP_TRY(fap_hip_fill_i32(d_max, 0, 1, st));
+  FAP_TRY(launch_bg_vert(d_bgp, P, nb_total, d_pidx, g->d_row_offset, g->d_col, nullptr,
+                         g->d_graph_id, g->d_st2ed, g->d_ed2st, d_verts, d_wt, d_orow, st));
+  FAP_TRY(launch_bg_closure(d_verts, d_wt, cb, s_max < 0xFFFF, nb_total, Sb, d_bgp, P, d_pidx,
+                            g->d_row_offset, g->d_col, nullptr, g->d_graph_id, g->d_st2ed,
+                            g->d_ed2st, d_mat, st));
+  return launch_bg_rows(d_bgp, d_orow, nb_total, d_mat, Sb, max_bdy, d_pidx, g->d_graph_id,
+                        g->d_ed2st, g->n, d_out, ld_out, d_max, true, st);
 }
 
 }  // namespace fap_hip

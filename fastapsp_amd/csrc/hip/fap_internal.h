@@ -54,6 +54,13 @@ int timing_record(hipStream_t on, hipEvent_t *e);   // pooled event recorded on 
 void timing_add_span(int stage, hipEvent_t a, hipEvent_t b);
 int timing_finish(hipStream_t st);                  // record the end of the call
 void timing_note_keyed(int merge_parts, int fw_parts);  // path solves: keyed forms taken
+void timing_note_bgraph();  // boundary rows came from the boundary graph
+
+// Boundary rows of parts covering a unit-weight graph with paired arcs from the
+// boundary graph (boundary.hip); FAP_ERR_INVALID, nothing launched, when the parts do
+// not qualify.
+int boundary_rows_unit(const fap_graph_dev *g, const fap_part_desc *act, int P, int *const *fw,
+                       const int *S, int32_t *d_out, int64_t ld_out, hipStream_t st);
 
 }  // namespace fap_hip
 

@@ -1051,14 +1051,25 @@ static int solve_parts_path(const fap_graph_dev *g, const int32_t *d_rro, const 
     return FAP_OK;
   };
   const bool vec = (ld_out % 4 == 0) && ((uintptr_t)d_out % 16 == 0);
-  // 1. boundary rows: distances by SSSP, then their predecessors.
+  bool bgraph = false;
+  // 1. boundary rows: distances from the boundary graph (parts covering a unit-weight
+  // graph with paired arcs; 0.8 against 3.0 ms of BFS on the 114,599-vertex road-like
+  // graph, K = 128) or by SSSP, then their predecessors, which depend on the
+  // distances only.
   FAP_TRY(span(0, [&]() -> int {
     FAP_TRY(fap_hip_fill_i32(d_stat, -1, kMaxSlots * kSlotStride, st));
     if (src_total == 0) return FAP_OK;
     FAP_TRY(launch_gather_sources(d_offsets, d_sb, d_or, P, src_total, g->d_st2ed, ld_out, d_src,
                                   d_rows, st));
     int s = FAP_ERR_INVALID;
-    if (!g->d_w) {
+    if (!g->d_w && g->d_comp) {
+      std::vector<int *> fwm(P);
+      for (int k = 0; k < P; ++k) fwm[k] = fw_base + fw_off[k];
+      s = boundary_rows_unit(g, act.data(), P, fwm.data(), S.data(), d_out, ld_out, st);
+      if (s != FAP_OK && s != FAP_ERR_INVALID) return s;
+      bgraph = s == FAP_OK;
+    }
+    if (s != FAP_OK && !g->d_w) {
       FAP_TRY(g_ws_pell.ensure((int64_t)n * sizeof(int4)));
       int4 *ell = static_cast<int4 *>(g_ws_pell.ptr);
       FAP_TRY(launch_ell_build(g->d_row_offset, g->d_col, n, ell, st));
@@ -1273,6 +1284,7 @@ static int solve_parts_path(const fap_graph_dev *g, const int32_t *d_rro, const 
     return FAP_OK;
   }));
   timing_note_keyed(merge_keyed, fw_keyed);
+  if (bgraph) timing_note_bgraph();
   return timing_finish(st);
 }
 

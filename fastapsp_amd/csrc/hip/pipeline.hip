@@ -149,6 +149,7 @@ void timing_note_keyed(int merge_parts, int fw_parts) {
   g_keyed = merge_parts;
   g_fw_keyed = fw_parts;
 }
+void timing_note_bgraph() { g_bgraph = 1; }
 
 // Stage timing shared with the float-weight pipeline (fp64.hip).
 int timing_start(hipStream_t st) {

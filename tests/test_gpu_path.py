@@ -50,9 +50,11 @@ def _keyable(lay):
     return int(((num > bdy) & (bdy >= 1) & (bdy <= 32)).sum())
 
 
-def _check_solve(g, validate_rows=64, keyed=None, fw_keyed=None):
+def _check_solve(g, validate_rows=64, keyed=None, fw_keyed=None, bgraph=None):
     g.solve(True)
     info = L.last_stage_times()
+    if bgraph is not None:  # boundary rows from the boundary graph (else by SSSP)
+        assert info["boundary_graph"] == bgraph, info["boundary_graph"]
     if keyed is not None:
         assert info["parts_keyed"] == keyed, (info["parts_keyed"], keyed)
     if fw_keyed is not None:
@@ -80,8 +82,10 @@ def test_paths_road_grid(args):
     g = F.FapGraph.road_grid(*args)
     g.preCondition()
     # Boundaries of 36..79 (compare form), 10..27 (keyed, 5 tag bits), none (K = 1);
-    # unit weights: every part's Floyd-Warshall is keyed.
-    _check_solve(g, keyed=_keyable(g.layout()), fw_keyed=g.K)
+    # unit weights: every part's Floyd-Warshall is keyed.  Undirected unit weights with
+    # parts covering the graph: boundary rows from the boundary graph (wide form for
+    # boundaries above 32); K = 1 has no boundary row.
+    _check_solve(g, keyed=_keyable(g.layout()), fw_keyed=g.K, bgraph=int(g.K > 1))
 
 
 def test_paths_directed_weighted():
