@@ -27,4 +27,13 @@ __device__ __forceinline__ const FAP_CONST T *uniform_cptr(const T *p) {
   return (const FAP_CONST T *)(((uint64_t)hi << 32) | lo);
 }
 
+// Workgroup barrier that orders LDS accesses only: s_waitcnt lgkmcnt(0) + s_barrier.
+// __syncthreads() also waits for the wave's global stores to complete (vmcnt(0)),
+// which a loop storing to global memory between barriers pays on every step.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 }  // namespace fap_hip

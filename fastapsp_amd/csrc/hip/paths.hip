@@ -221,23 +221,40 @@ __device__ __forceinline__ int *tile(int *m, const PathPart &p, int ti, int tj) 
   return m + (int64_t)(ti * kT) * p.S + tj * kT;
 }
 
+// The part tables hold generic pointers: every global access of the Floyd-Warshall
+// kernels goes through gptr (global_load / global_store).  A flat access also counts
+// against the LDS counter, so each LDS write after a flat load, and each barrier after
+// a flat store, waited for the memory round trip.  Row r, 16 columns c0.. as 16-byte
+// loads (tiles start on 64-int boundaries of rows of S = 64 k ints).
 __device__ __forceinline__ void tile_to_lds(int (*D)[kTP], int (*P)[kTP], const int *gd,
                                             const int *gp, int64_t ld) {
   const int r = threadIdx.x >> 2, c0 = (threadIdx.x & 3) * 16;
+  const FAP_GLOBAL v4i *d4 = reinterpret_cast<const FAP_GLOBAL v4i *>(gptr(gd + (int64_t)r * ld + c0));
+  const FAP_GLOBAL v4i *p4 = reinterpret_cast<const FAP_GLOBAL v4i *>(gptr(gp + (int64_t)r * ld + c0));
+  v4i dv[4], pv[4];
 #pragma unroll
-  for (int c = 0; c < 16; ++c) {
-    D[r][c0 + c] = gd[(int64_t)r * ld + c0 + c];
-    P[r][c0 + c] = gp[(int64_t)r * ld + c0 + c];
+  for (int q = 0; q < 4; ++q) {
+    dv[q] = d4[q];
+    pv[q] = p4[q];
   }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      D[r][c0 + 4 * q + c] = dv[q][c];
+      P[r][c0 + 4 * q + c] = pv[q][c];
+    }
 }
 
 __device__ __forceinline__ void lds_to_tile(int (*D)[kTP], int (*P)[kTP], int *gd, int *gp,
                                             int64_t ld) {
   const int r = threadIdx.x >> 2, c0 = (threadIdx.x & 3) * 16;
+  FAP_GLOBAL v4i *d4 = reinterpret_cast<FAP_GLOBAL v4i *>(gptr(gd + (int64_t)r * ld + c0));
+  FAP_GLOBAL v4i *p4 = reinterpret_cast<FAP_GLOBAL v4i *>(gptr(gp + (int64_t)r * ld + c0));
 #pragma unroll
-  for (int c = 0; c < 16; ++c) {
-    gd[(int64_t)r * ld + c0 + c] = D[r][c0 + c];
-    gp[(int64_t)r * ld + c0 + c] = P[r][c0 + c];
+  for (int q = 0; q < 4; ++q) {
+    d4[q] = v4i{D[r][c0 + 4 * q], D[r][c0 + 4 * q + 1], D[r][c0 + 4 * q + 2], D[r][c0 + 4 * q + 3]};
+    p4[q] = v4i{P[r][c0 + 4 * q], P[r][c0 + 4 * q + 1], P[r][c0 + 4 * q + 2], P[r][c0 + 4 * q + 3]};
   }
 }
 
@@ -270,10 +287,15 @@ __global__ __launch_bounds__(256) void fwp_phase1_kernel(const PathPart *__restr
   __shared__ int D[kT][kTP];
   __shared__ int P[kT][kTP];
   tile_to_lds(D, P, tile(p.fw, p, r, r), tile(p.fwp, p, r, r), p.S);
-  int *rrow = rec_diag_row(p), *rrowp = rec_diag_rowp(p), *rcol = rec_diag_col(p);
+  FAP_GLOBAL int *rrow = gptr(rec_diag_row(p)), *rrowp = gptr(rec_diag_rowp(p)),
+                 *rcol = gptr(rec_diag_col(p));
   const int i = threadIdx.x >> 2, c0 = (threadIdx.x & 3) * 16;
+  __syncthreads();
   for (int k = 0; k < kT; ++k) {
-    __syncthreads();
+    // The records go to global memory and are read by later launches only, so the
+    // per-step barrier orders the LDS tile only (__syncthreads would also wait for
+    // the step's record stores).
+    if (k > 0) lds_barrier();
     if (i == k) {
 #pragma unroll
       for (int c = 0; c < 16; ++c) {
@@ -319,18 +341,19 @@ __global__ __launch_bounds__(256) void fwp_phase2_kernel(const PathPart *__restr
   int *gp = colp ? tile(p.fwp, p, o, r) : tile(p.fwp, p, r, o);
   tile_to_lds(D, P, gd, gp, p.S);
   {
-    const int *a = colp ? rec_diag_row(p) : rec_diag_col(p);
-    const int *b = rec_diag_rowp(p);
+    const FAP_GLOBAL int *a = gptr(colp ? rec_diag_row(p) : rec_diag_col(p));
+    const FAP_GLOBAL int *b = gptr(rec_diag_rowp(p));
     for (int e = threadIdx.x; e < kTT; e += 256) {
       R0[e / kT][e % kT] = a[e];
       if (colp) R1[e / kT][e % kT] = b[e];
     }
   }
-  int *rec = colp ? rec_colpanel(p, o) : rec_rowpanel(p, o);
-  int *recp = colp ? nullptr : rec_rowpanelp(p, o);
+  FAP_GLOBAL int *rec = gptr(colp ? rec_colpanel(p, o) : rec_rowpanel(p, o));
+  FAP_GLOBAL int *recp = gptr(colp ? nullptr : rec_rowpanelp(p, o));
   const int i = threadIdx.x >> 2, c0 = (threadIdx.x & 3) * 16;
+  __syncthreads();
   for (int k = 0; k < kT; ++k) {
-    __syncthreads();
+    if (k > 0) lds_barrier();  // LDS only: the records are read by later launches
     if (colp) {
       // cell (i, c) of tile (o, r): D[i][k] (this tile) + diag row k at step k.
       if (k >= c0 && k < c0 + 16) rec[k * kT + i] = D[i][k];  // k-major: phase 3 A operand
@@ -382,7 +405,8 @@ __global__ __launch_bounds__(256) void fwp_phase3_kernel(const PathPart *__restr
   __shared__ int Bs[kT][kT];  // Bs[k][j]: row panel record of tile (r, b)
   __shared__ int Bp[kT][kT];
   {
-    const int *ra = rec_colpanel(p, a), *rb = rec_rowpanel(p, b), *rbp = rec_rowpanelp(p, b);
+    const FAP_GLOBAL int *ra = gptr(rec_colpanel(p, a)), *rb = gptr(rec_rowpanel(p, b)),
+                         *rbp = gptr(rec_rowpanelp(p, b));
     for (int e = threadIdx.x; e < kTT; e += 256) {
       As[e / kT][e % kT] = ra[e];
       Bs[e / kT][e % kT] = rb[e];
@@ -390,7 +414,7 @@ __global__ __launch_bounds__(256) void fwp_phase3_kernel(const PathPart *__restr
     }
   }
   const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
-  int *gd = tile(p.fw, p, a, b), *gp = tile(p.fwp, p, a, b);
+  FAP_GLOBAL int *gd = gptr(tile(p.fw, p, a, b)), *gp = gptr(tile(p.fwp, p, a, b));
   int dv[4][4], pv[4][4];
 #pragma unroll
   for (int q = 0; q < 4; ++q)
@@ -431,11 +455,15 @@ __global__ __launch_bounds__(256) void fwp_phase3_kernel(const PathPart *__restr
 // Phase 2, keyed form: one wave per panel tile, each lane holding one line of 64 keys
 // in registers (no barriers: every line's update depends on that line and the diag
 // records only).  The 64 pivots are expanded at compile time (PanelSteps) so that key
-// k is a fixed register.
+// k is a fixed register.  Column panel: the diag records (64 x 64 keys) are staged in
+// LDS once and read as broadcasts (69 against 107 us per launch on MI355X: with scalar
+// loads each pivot waited for its own 256 bytes, SMEM returning out of order).  Row
+// panel (256 VGPRs, one wave per SIMD): scalar loads from rkc (83 against 135 us with
+// the LDS records, whose latency one wave cannot hide).
 //
 // Column panel tile (o, r), blockIdx.x = o' (o' < nb - 1, skipping r): lane i = row i;
-// pivot k uses the lane's own key k and the diag row record k (scalar loads); the
-// path of a cell whose tag is k + 1 is the diag row path record k.
+// pivot k uses the lane's own key k and the diag row record k; the path of a cell
+// whose tag is k + 1 is the diag row path record k.
 // Row panel tile (r, o): lane j = column j; pivot k uses the lane's own key k and the
 // diag column record k; a cell whose tag is k + 1 takes the path of cell (k, j) at
 // step k.  The tag of key k at step k is kept (8 bits each in tagw); after the pivots,
@@ -444,7 +472,7 @@ __global__ __launch_bounds__(256) void fwp_phase3_kernel(const PathPart *__restr
 template <int K, bool kRowPanel>
 struct PanelSteps {
   static __device__ __forceinline__ void run(uint32_t (&key)[kT], uint32_t (&tagw)[kT / 4],
-                                             const FAP_CONST uint32_t *rk,
+                                             const uint32_t *rk, const FAP_CONST uint32_t *rkc,
                                              FAP_GLOBAL uint32_t *recd, int lane) {
     if constexpr (K < kT) {
       const uint32_t kk = key[K];
@@ -455,12 +483,32 @@ struct PanelSteps {
       } else {
         recd[K * kT + lane] = dk;  // phase 3 A operand
       }
+      if constexpr (kRowPanel) {
 #pragma unroll
-      for (int c = 0; c < kT; ++c) key[c] = min(key[c], dk + rk[K * kT + c]);
-      PanelSteps<K + 1, kRowPanel>::run(key, tagw, rk, recd, lane);
+        for (int c = 0; c < kT; ++c) key[c] = min(key[c], dk + rkc[K * kT + c]);
+      } else {
+        const uint4 *row = reinterpret_cast<const uint4 *>(rk + K * kT);
+#pragma unroll
+        for (int c = 0; c < kT; c += 4) {
+          const uint4 v = row[c / 4];
+          key[c] = min(key[c], dk + v.x);
+          key[c + 1] = min(key[c + 1], dk + v.y);
+          key[c + 2] = min(key[c + 2], dk + v.z);
+          key[c + 3] = min(key[c + 3], dk + v.w);
+        }
+      }
+      PanelSteps<K + 1, kRowPanel>::run(key, tagw, rk, rkc, recd, lane);
     }
   }
 };
+
+// The 64 x 64 diag records into LDS (one wave: 16 B per lane per load).
+__device__ __forceinline__ void stage_records(uint32_t *rks, const int *rec, int lane) {
+  const FAP_GLOBAL v4i *src = reinterpret_cast<const FAP_GLOBAL v4i *>(gptr(rec));
+  v4i *dst = reinterpret_cast<v4i *>(rks);
+#pragma unroll
+  for (int q = 0; q < kTT / 4 / 64; ++q) dst[q * 64 + lane] = src[q * 64 + lane];
+}
 
 __global__ __launch_bounds__(64) void fwp_colpanel_key_kernel(const PathPart *__restrict__ parts,
                                                               const int *__restrict__ fwkey,
@@ -471,7 +519,11 @@ __global__ __launch_bounds__(64) void fwp_colpanel_key_kernel(const PathPart *__
   if (t >= p.nb - 1) return;
   const int o = t < r ? t : t + 1;
   const int lane = threadIdx.x;
-  __shared__ int tab[kT][kTP];  // diag row path records
+  // One LDS buffer: the diag row records during the pivots, then the diag row path
+  // records (tab).
+  __shared__ uint4 buf4[(kT * kTP + 3) / 4];
+  uint32_t *rks = reinterpret_cast<uint32_t *>(buf4);
+  int(*tab)[kTP] = reinterpret_cast<int(*)[kTP]>(buf4);
   FAP_GLOBAL int *gd = gptr(tile(p.fw, p, o, r) + (int64_t)lane * p.S);
   FAP_GLOBAL int *gp = gptr(tile(p.fwp, p, o, r) + (int64_t)lane * p.S);
   uint32_t key[kT];
@@ -483,13 +535,15 @@ __global__ __launch_bounds__(64) void fwp_colpanel_key_kernel(const PathPart *__
     key[c + 2] = fkey(v.z);
     key[c + 3] = fkey(v.w);
   }
+  stage_records(rks, rec_diag_row(p), lane);
+  __syncthreads();
+  uint32_t tagw[kT / 4];  // unused by the column panel
+  PanelSteps<0, false>::run(key, tagw, rks, nullptr,
+                            gptr(reinterpret_cast<uint32_t *>(rec_colpanel(p, o))), lane);
+  __syncthreads();
   const FAP_GLOBAL int *rp = gptr(rec_diag_rowp(p));
   for (int e = lane; e < kTT; e += 64) tab[e >> 6][e & 63] = rp[e];
   __syncthreads();
-  uint32_t tagw[kT / 4];  // unused by the column panel
-  PanelSteps<0, false>::run(key, tagw,
-                            uniform_cptr(reinterpret_cast<const uint32_t *>(rec_diag_row(p))),
-                            gptr(reinterpret_cast<uint32_t *>(rec_colpanel(p, o))), lane);
 #pragma unroll
   for (int c = 0; c < kT; c += 4) {
     const v4i v = {fkey_dist(key[c]), fkey_dist(key[c + 1]), fkey_dist(key[c + 2]),
@@ -514,7 +568,9 @@ __global__ __launch_bounds__(64) void fwp_rowpanel_key_kernel(const PathPart *__
   const unsigned lane = threadIdx.x;
   __shared__ int tab[kT][kTP];  // row paths of this tile, resolved as each step records them
   // The tile's rows are staged through tab (16 B per lane, 4 rows per load), the lane
-  // then reads its column: first the distances, then the paths.
+  // then reads its column: first the distances, then the paths.  The results go back
+  // the same way (on MI355X 83 us per launch, against 98 us writing each lane's column
+  // with 128 dword stores, although holding 64 paths besides the 64 keys spills).
   FAP_GLOBAL int *gd = gptr(tile(p.fw, p, r, o));
   FAP_GLOBAL int *gp = gptr(tile(p.fwp, p, r, o));
   const int sr = lane >> 4, sc = (lane & 15) * 4;  // staging: row sr + 4 q, columns sc..
@@ -547,7 +603,7 @@ __global__ __launch_bounds__(64) void fwp_rowpanel_key_kernel(const PathPart *__
   uint32_t tagw[kT / 4];
 #pragma unroll
   for (int q = 0; q < kT / 4; ++q) tagw[q] = 0;
-  PanelSteps<0, true>::run(key, tagw,
+  PanelSteps<0, true>::run(key, tagw, nullptr,
                            uniform_cptr(reinterpret_cast<const uint32_t *>(rec_diag_col(p))),
                            gptr(reinterpret_cast<uint32_t *>(rec_rowpanel(p, o))), lane);
   {
@@ -597,22 +653,37 @@ __global__ __launch_bounds__(256) void fwp_phase3_key_kernel(const PathPart *__r
   __shared__ uint4 Bs[kT][kT / 4];  // Bs[k][j / 4]: row panel keys of tile (r, b)
   __shared__ int Bp[kT][kTP];       // row panel path records
   {
-    const uint4 *ra = reinterpret_cast<const uint4 *>(rec_colpanel(p, a));
-    const uint4 *rb = reinterpret_cast<const uint4 *>(rec_rowpanel(p, b));
-    for (int e = threadIdx.x; e < kTT / 4; e += 256) {
-      As[e >> 4][e & 15] = ra[e];
-      Bs[e >> 4][e & 15] = rb[e];
+    const FAP_GLOBAL v4i *ra = reinterpret_cast<const FAP_GLOBAL v4i *>(gptr(rec_colpanel(p, a)));
+    const FAP_GLOBAL v4i *rb = reinterpret_cast<const FAP_GLOBAL v4i *>(gptr(rec_rowpanel(p, b)));
+    const FAP_GLOBAL int *rbp = gptr(rec_rowpanelp(p, b));
+    v4i x[kTT / 4 / 256], y[kTT / 4 / 256];
+    int z[kTT / 256];
+#pragma unroll
+    for (int u = 0; u < kTT / 4 / 256; ++u) {
+      x[u] = ra[threadIdx.x + 256 * u];
+      y[u] = rb[threadIdx.x + 256 * u];
     }
-    const int *rbp = rec_rowpanelp(p, b);
-    for (int e = threadIdx.x; e < kTT; e += 256) Bp[e >> 6][e & 63] = rbp[e];
+#pragma unroll
+    for (int u = 0; u < kTT / 256; ++u) z[u] = rbp[threadIdx.x + 256 * u];
+#pragma unroll
+    for (int u = 0; u < kTT / 4 / 256; ++u) {
+      const int e = threadIdx.x + 256 * u;
+      As[e >> 4][e & 15] = make_uint4(x[u].x, x[u].y, x[u].z, x[u].w);
+      Bs[e >> 4][e & 15] = make_uint4(y[u].x, y[u].y, y[u].z, y[u].w);
+    }
+#pragma unroll
+    for (int u = 0; u < kTT / 256; ++u) {
+      const int e = threadIdx.x + 256 * u;
+      Bp[e >> 6][e & 63] = z[u];
+    }
   }
   const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
-  int *gd = tile(p.fw, p, a, b) + (int64_t)(ty * 4) * p.S + tx * 4;
-  int *gp = tile(p.fwp, p, a, b) + (int64_t)(ty * 4) * p.S + tx * 4;
+  FAP_GLOBAL int *gd = gptr(tile(p.fw, p, a, b) + (int64_t)(ty * 4) * p.S + tx * 4);
+  FAP_GLOBAL int *gp = gptr(tile(p.fwp, p, a, b) + (int64_t)(ty * 4) * p.S + tx * 4);
   uint32_t key[4][4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const int4 v = *reinterpret_cast<const int4 *>(gd + (int64_t)q * p.S);
+    const v4i v = *reinterpret_cast<const FAP_GLOBAL v4i *>(gd + (int64_t)q * p.S);
     key[q][0] = fkey(v.x);
     key[q][1] = fkey(v.y);
     key[q][2] = fkey(v.z);
@@ -632,12 +703,9 @@ __global__ __launch_bounds__(256) void fwp_phase3_key_kernel(const PathPart *__r
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    int4 v;
-    v.x = fkey_dist(key[q][0]);
-    v.y = fkey_dist(key[q][1]);
-    v.z = fkey_dist(key[q][2]);
-    v.w = fkey_dist(key[q][3]);
-    *reinterpret_cast<int4 *>(gd + (int64_t)q * p.S) = v;
+    const v4i v = {fkey_dist(key[q][0]), fkey_dist(key[q][1]), fkey_dist(key[q][2]),
+                   fkey_dist(key[q][3])};
+    *reinterpret_cast<FAP_GLOBAL v4i *>(gd + (int64_t)q * p.S) = v;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       const uint32_t tg = key[q][c] & 0x7F;
