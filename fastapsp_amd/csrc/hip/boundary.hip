@@ -94,7 +94,7 @@ __global__ void bg_vert_kernel(const BgPart *__restrict__ parts, int np, int nb_
   bv.bs = p.bs;
   bv.bdy = p.bdy;
   bv.woff = p.cb + k * p.bdy;
-  for (int c = 0; c < p.bdy; ++c) wt[bv.woff + c] = p.fw[(int64_t)c * p.S + k];
+  for (int c = 0; c < p.bdy; ++c) wt[bv.woff + c] = gptr(p.fw)[(int64_t)c * p.S + k];
   int nc = 0;
   const int v = st2ed[p.offset + k];
   for (int e = row_off[v]; e < row_off[v + 1]; ++e) {
@@ -417,20 +417,22 @@ __device__ __forceinline__ ColInfo column_info(const BgPart *__restrict__ parts,
 template <int KP>
 __device__ __forceinline__ void load_fin(int (&fin)[KP], const ColInfo &ci, int c0) {
 #pragma unroll
-  for (int c = 0; c < KP; ++c) fin[c] = ci.fw[(int64_t)(c0 + c) * ci.S + ci.l];
+  for (int c = 0; c < KP; ++c) fin[c] = gptr(ci.fw)[(int64_t)(c0 + c) * ci.S + ci.l];
 #pragma unroll
   for (int c = 0; c < KP; ++c)
     if (c0 + c >= ci.bdy) fin[c] = kInf;
 }
 
 // The same from a symmetric in-part matrix: d_q(c, l) = d_q(l, c), so row l gives the
-// KP values contiguously (16-byte loads; KP <= 32 <= S columns are readable).
+// KP values contiguously (16-byte loads; KP <= 32 <= S columns are readable).  The
+// matrix pointer comes from the part table (a generic pointer): gptr makes the loads
+// global ones, which do not hold up the LDS counter as flat loads would.
 template <int KP>
 __device__ __forceinline__ void load_fin_sym(int (&fin)[KP], const ColInfo &ci) {
-  const int4 *row = reinterpret_cast<const int4 *>(ci.fw + (int64_t)ci.l * ci.S);
+  const FAP_GLOBAL v4i *row = reinterpret_cast<const FAP_GLOBAL v4i *>(gptr(ci.fw + (int64_t)ci.l * ci.S));
 #pragma unroll
   for (int c = 0; c < KP; c += 4) {
-    const int4 v = row[c / 4];
+    const v4i v = row[c / 4];
     fin[c] = v.x;
     fin[c + 1] = v.y;
     fin[c + 2] = v.z;
@@ -447,10 +449,11 @@ __device__ __forceinline__ void load_fin_sym(int (&fin)[KP], const ColInfo &ci) 
 // boundary stage).
 template <int KC>
 __device__ __forceinline__ void load_fin_sym_chunk(int (&fin)[KC], const ColInfo &ci, int c0) {
-  const int4 *row = reinterpret_cast<const int4 *>(ci.fw + (int64_t)ci.l * ci.S + c0);
+  const FAP_GLOBAL v4i *row =
+      reinterpret_cast<const FAP_GLOBAL v4i *>(gptr(ci.fw + (int64_t)ci.l * ci.S + c0));
 #pragma unroll
   for (int c = 0; c < KC; c += 4) {
-    const int4 v = row[c / 4];
+    const v4i v = row[c / 4];
     fin[c] = v.x;
     fin[c + 1] = v.y;
     fin[c + 2] = v.z;

@@ -30,9 +30,12 @@
 
 #include "fap_internal.h"
 #include "fw_desc.h"
+#include "gmem.h"
 
 namespace fap_hip {
 namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr int kDeg = 8;       // in-part neighbours kept in registers per vertex
 constexpr int kPlanes = 11;   // levels below 2048 > kInpartBfsMaxS
@@ -206,7 +209,8 @@ __global__ __launch_bounds__(kThreads) void inpart_bfs_kernel(const InpartDesc *
   for (int q = 0; q < Q; ++q) {
     const int v = tid + kThreads * q;
     if (v >= s) continue;
-    int4 *dst = reinterpret_cast<int4 *>(d.fw + (int64_t)v * d.S + src0);
+    // d.fw is a generic pointer from the descriptor: gptr gives global stores.
+    FAP_GLOBAL v4i *dst = reinterpret_cast<FAP_GLOBAL v4i *>(gptr(d.fw + (int64_t)v * d.S + src0));
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       uint32_t p32[kPlanes];
@@ -223,14 +227,14 @@ __global__ __launch_bounds__(kThreads) void inpart_bfs_kernel(const InpartDesc *
           for (int j = 0; j < kPlanes; ++j) x |= ((p32[j] >> b) & 1u) << j;
           out[e] = ((v32 >> b) & 1u) ? (int)x : kInf;
         }
-        dst[(32 * h + c) / 4] = make_int4(out[0], out[1], out[2], out[3]);
+        dst[(32 * h + c) / 4] = v4i{out[0], out[1], out[2], out[3]};
       }
     }
   }
   // Padding rows s .. S - 1: 16 lanes per row, 4 columns each.
   for (int r = s + (tid >> 4); r < d.S; r += kThreads / 16)
-    *reinterpret_cast<int4 *>(d.fw + (int64_t)r * d.S + src0 + (tid & 15) * 4) =
-        make_int4(kInf, kInf, kInf, kInf);
+    *reinterpret_cast<FAP_GLOBAL v4i *>(gptr(d.fw + (int64_t)r * d.S + src0 + (tid & 15) * 4)) =
+        v4i{kInf, kInf, kInf, kInf};
 }
 
 }  // namespace
