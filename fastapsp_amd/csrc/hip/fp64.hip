@@ -191,9 +191,11 @@ struct Fw64Desc {
   int nb;
 };
 
-__device__ __forceinline__ double *tile64(const Fw64Desc &d, int ti, int tj) {
+// Tiles are addressed in the global address space (the descriptor pointer is generic;
+// flat accesses would also hold up the LDS counter of the staging writes).
+__device__ __forceinline__ FAP_GLOBAL double *tile64(const Fw64Desc &d, int ti, int tj) {
   const int64_t S = (int64_t)d.nb * kT;
-  return d.mat + (int64_t)ti * kT * S + tj * kT;
+  return gptr(d.mat + (int64_t)ti * kT * S + tj * kT);
 }
 
 // Phase 1: the diagonal tile closed in LDS; 256 threads, 16 entries each.
@@ -202,7 +204,7 @@ __global__ __launch_bounds__(256) void fw64_phase1_kernel(const Fw64Desc *__rest
   if (r >= d.nb) return;
   const int64_t S = (int64_t)d.nb * kT;
   __shared__ double D[kT][kT];
-  double *g = tile64(d, r, r);
+  FAP_GLOBAL double *g = tile64(d, r, r);
   const int row = threadIdx.x >> 2, c0 = (threadIdx.x & 3) * 16;
 #pragma unroll
   for (int c = 0; c < 16; ++c) D[row][c0 + c] = g[row * S + c0 + c];
@@ -224,7 +226,8 @@ constexpr int kKc64 = 16;
 
 __device__ __forceinline__ int own_row64(int tr, int q) { return (q < 4 ? 0 : 32) + tr * 4 + (q & 3); }
 
-__device__ __forceinline__ void tile_product64(double (&acc)[8][8], const double *a, const double *b,
+__device__ __forceinline__ void tile_product64(double (&acc)[8][8], const FAP_GLOBAL double *a,
+                                               const FAP_GLOBAL double *b,
                                                int64_t S, double (*As)[kT], double (*Bs)[kT], int tr,
                                                int tc) {
   const int lane = threadIdx.x;
@@ -264,7 +267,7 @@ __global__ __launch_bounds__(64) void fw64_phase2_kernel(const Fw64Desc *__restr
   const int64_t S = (int64_t)d.nb * kT;
   __shared__ double As[kKc64][kT];
   __shared__ double Bs[kKc64][kT];
-  double *t = is_row ? tile64(d, r, o) : tile64(d, o, r);
+  FAP_GLOBAL double *t = is_row ? tile64(d, r, o) : tile64(d, o, r);
   const int tr = threadIdx.x >> 3, tc = threadIdx.x & 7;
   double acc[8][8];
 #pragma unroll
@@ -288,7 +291,7 @@ __global__ __launch_bounds__(64) void fw64_phase3_kernel(const Fw64Desc *__restr
   const int64_t S = (int64_t)d.nb * kT;
   __shared__ double As[kKc64][kT];
   __shared__ double Bs[kKc64][kT];
-  double *t = tile64(d, i, j);
+  FAP_GLOBAL double *t = tile64(d, i, j);
   const int tr = threadIdx.x >> 3, tc = threadIdx.x & 7;
   double acc[8][8];
 #pragma unroll
@@ -369,14 +372,14 @@ __global__ __launch_bounds__(256) void merge64_kernel(const Merge64Desc *__restr
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int kk = kq + q;
-        As[kk][i] = (row_ok && k0 + kk < d.K) ? d.A[(int64_t)(row0 + i) * d.lda + k0 + kk] : kInfD;
+        As[kk][i] = (row_ok && k0 + kk < d.K) ? gptr(d.A)[(int64_t)(row0 + i) * d.lda + k0 + kk] : kInfD;
       }
     }
     {  // Bs[kk][c] = B[k0 + kk][col0 + c]: 16 x 256, sixteen per thread
 #pragma unroll
       for (int q = 0; q < kMKc; ++q) {
         const int c = tid, j = col0 + c;
-        Bs[q][c] = (k0 + q < d.K && j < N) ? d.B[(int64_t)(k0 + q) * d.ldb + j] : kInfD;
+        Bs[q][c] = (k0 + q < d.K && j < N) ? gptr(d.B)[(int64_t)(k0 + q) * d.ldb + j] : kInfD;
       }
     }
     __syncthreads();
@@ -414,19 +417,20 @@ __global__ __launch_bounds__(256) void merge64_kernel(const Merge64Desc *__restr
     const int i = row0 + ty * 8 + r;
     if (i >= d.M) break;
     if (any_own) {
-      const double *frow = d.fw + (int64_t)(d.fw_row0 + i) * d.ldf;
+      const FAP_GLOBAL double *frow = gptr(d.fw + (int64_t)(d.fw_row0 + i) * d.ldf);
 #pragma unroll
       for (int c = 0; c < 8; ++c)
         if (local_col[c] >= 0) acc[r][c] = frow[local_col[c]];
     }
-    float *crow = d.C + (int64_t)i * d.ldc;
+    FAP_GLOBAL float *crow = gptr(d.C + (int64_t)i * d.ldc);
 #pragma unroll
     for (int g = 0; g < 2; ++g) {
       const int j = cols[4 * g];
       if (kVec && j + 3 < N) {
-        *reinterpret_cast<float4 *>(crow + j) =
-            make_float4(to_out(acc[r][4 * g]), to_out(acc[r][4 * g + 1]),
-                        to_out(acc[r][4 * g + 2]), to_out(acc[r][4 * g + 3]));
+        typedef float v4f __attribute__((ext_vector_type(4)));
+        *reinterpret_cast<FAP_GLOBAL v4f *>(crow + j) =
+            v4f{to_out(acc[r][4 * g]), to_out(acc[r][4 * g + 1]), to_out(acc[r][4 * g + 2]),
+                to_out(acc[r][4 * g + 3])};
       } else {
 #pragma unroll
         for (int e = 0; e < 4; ++e)

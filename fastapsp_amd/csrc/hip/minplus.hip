@@ -88,7 +88,9 @@ __global__ __launch_bounds__(256) void merge_kernel(const MergeDesc *__restrict_
       const int i = tid & 63;
       const int kq = (tid >> 6) * 8;
       const bool row_ok = row0 + i < d.M;
-      const int *arow = d.A + (int64_t)(row0 + i) * d.lda + k0;
+      // Descriptor pointers are generic: gptr keeps every access a global one (a flat
+      // access would also hold up the LDS counter of the staging writes).
+      const FAP_GLOBAL int *arow = gptr(d.A + (int64_t)(row0 + i) * d.lda + k0);
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const int kk = kq + q;
@@ -103,17 +105,17 @@ __global__ __launch_bounds__(256) void merge_kernel(const MergeDesc *__restrict_
       for (int q = 0; q < kMpKc / 4; ++q) {
         const int kk = (tid >> 6) + 4 * q;
         const bool k_ok = k0 + kk < d.K;
-        const int *brow = d.B + (int64_t)(k0 + kk) * d.ldb;
-        int4 v;
+        const FAP_GLOBAL int *brow = gptr(d.B + (int64_t)(k0 + kk) * d.ldb);
+        v4i v;
         if (kVec && k_ok && j + 3 < N) {
-          v = *reinterpret_cast<const int4 *>(brow + j);
+          v = *reinterpret_cast<const FAP_GLOBAL v4i *>(brow + j);
         } else {
           v.x = (k_ok && j + 0 < N) ? brow[j + 0] : kInf;
           v.y = (k_ok && j + 1 < N) ? brow[j + 1] : kInf;
           v.z = (k_ok && j + 2 < N) ? brow[j + 2] : kInf;
           v.w = (k_ok && j + 3 < N) ? brow[j + 3] : kInf;
         }
-        *reinterpret_cast<int4 *>(&Bs[kk][c4]) = v;
+        *reinterpret_cast<v4i *>(&Bs[kk][c4]) = v;
       }
     }
     __syncthreads();
@@ -153,18 +155,18 @@ __global__ __launch_bounds__(256) void merge_kernel(const MergeDesc *__restrict_
     const int i = row0 + ty * 8 + r;
     if (i >= d.M) break;
     if (any_own) {
-      const int *fwrow = d.fw + (int64_t)(d.fw_row0 + i) * d.ldf;
+      const FAP_GLOBAL int *fwrow = gptr(d.fw + (int64_t)(d.fw_row0 + i) * d.ldf);
 #pragma unroll
       for (int c = 0; c < 8; ++c)
         if (local_col[c] >= 0) acc[r][c] = min(acc[r][c], fwrow[local_col[c]]);
     }
-    int *crow = d.C + (int64_t)i * d.ldc;
+    FAP_GLOBAL int *crow = gptr(d.C + (int64_t)i * d.ldc);
 #pragma unroll
     for (int g = 0; g < 2; ++g) {
       const int j = cols[4 * g];
       if (kVec && j + 3 < N) {
         v4i v = {acc[r][4 * g + 0], acc[r][4 * g + 1], acc[r][4 * g + 2], acc[r][4 * g + 3]};
-        __builtin_nontemporal_store(v, reinterpret_cast<v4i *>(crow + j));
+        __builtin_nontemporal_store(v, reinterpret_cast<FAP_GLOBAL v4i *>(crow + j));
       } else {
 #pragma unroll
         for (int e = 0; e < 4; ++e)
