@@ -107,43 +107,67 @@ __device__ __forceinline__ int *rec_rowpanelp(const PathPart &p, int t) {
 constexpr int kMaxSlots = 64;
 constexpr int kSlotStride = 16;
 
-// Boundary row k = k0 + blockIdx.y (source src[k], row offset rows[k] in d_out and
-// d_path).
+// Boundary rows k0 .. k0 + nk - 1 (source src[k], row offset rows[k] in d_out and
+// d_path), kPredRows of them per thread: vertex v's in-arcs (up to kPredDeg of them)
+// are read once into registers and checked against every row (one row per thread
+// re-read the reverse CSR for each of the 1,297 rows of the 114,599-vertex graph).
+constexpr int kPredRows = 8;
+constexpr int kPredDeg = 8;
 __global__ __launch_bounds__(256) void pred_rows_kernel(
     const int *__restrict__ rro, const int *__restrict__ rcol, const int *__restrict__ rw, int n,
-    const int *__restrict__ src, const int64_t *__restrict__ rows, int k0,
+    const int *__restrict__ src, const int64_t *__restrict__ rows, int k0, int nk,
     const int *__restrict__ out, int *__restrict__ path, int *__restrict__ bmax_slots) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = k0 + blockIdx.y;
-  const int *d = out + rows[k];
-  const int dv = v < n ? d[v] : kInf;
-  int mx = dv < kInf ? dv : -1;
+  const int kb = k0 + blockIdx.y * kPredRows, ke = min(k0 + nk, kb + kPredRows);
+  int e0 = 0, deg = 0;
+  int us[kPredDeg], ws[kPredDeg];
+  if (v < n) {
+    e0 = rro[v];
+    deg = rro[v + 1] - e0;
+  }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off));
-  if ((threadIdx.x & 63) == 0 && mx >= 0) {
-    const int slot = (blockIdx.x * 4 + (threadIdx.x >> 6) + blockIdx.y) % kMaxSlots;
-    atomicMax(bmax_slots + slot * kSlotStride, mx);
-  }
-  if (v >= n) return;
-  const int s = src[k];
-  int p = -1;
-  if (v == s) {
-    p = s;
-  } else if (dv < kInf) {
-    int bd = kInf, bu = 0x7FFFFFFF;
-    for (int e = rro[v]; e < rro[v + 1]; ++e) {
-      const int u = rcol[e];
-      const int du = d[u];
-      if (du >= kInf || du + (rw ? rw[e] : 1) != dv) continue;
-      if (du > dv || (du == dv && u >= v)) continue;  // settled after v
-      if (du < bd || (du == bd && u < bu)) {
-        bd = du;
-        bu = u;
-      }
+  for (int a = 0; a < kPredDeg; ++a) {
+    us[a] = 0;
+    ws[a] = 1;
+    if (a < deg) {
+      us[a] = rcol[e0 + a];
+      if (rw) ws[a] = rw[e0 + a];
     }
-    p = bu == 0x7FFFFFFF ? -1 : bu;
   }
-  path[rows[k] + v] = p;
+  for (int k = kb; k < ke; ++k) {
+    const int *d = out + rows[k];
+    const int dv = v < n ? d[v] : kInf;
+    int mx = dv < kInf ? dv : -1;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off));
+    if ((threadIdx.x & 63) == 0 && mx >= 0) {
+      const int slot = (blockIdx.x * 4 + (threadIdx.x >> 6) + k) % kMaxSlots;
+      atomicMax(bmax_slots + slot * kSlotStride, mx);
+    }
+    if (v >= n) continue;
+    const int s = src[k];
+    int p = -1;
+    if (v == s) {
+      p = s;
+    } else if (dv < kInf) {
+      int bd = kInf, bu = 0x7FFFFFFF;
+      auto consider = [&](int u, int wt) {
+        const int du = d[u];
+        if (du >= kInf || du + wt != dv) return;
+        if (du > dv || (du == dv && u >= v)) return;  // settled after v
+        if (du < bd || (du == bd && u < bu)) {
+          bd = du;
+          bu = u;
+        }
+      };
+#pragma unroll
+      for (int a = 0; a < kPredDeg; ++a)
+        if (a < deg) consider(us[a], ws[a]);
+      for (int e = e0 + kPredDeg; e < e0 + deg; ++e) consider(rcol[e], rw ? rw[e] : 1);
+      p = bu == 0x7FFFFFFF ? -1 : bu;
+    }
+    path[rows[k] + v] = p;
+  }
 }
 
 // ---- build ----------------------------------------------------------------------
@@ -1148,10 +1172,11 @@ static int solve_parts_path(const fap_graph_dev *g, const int32_t *d_rro, const 
     if (s != FAP_OK)
       FAP_TRY(launch_sssp(g->d_row_offset, g->d_col, g->d_w, n, d_src, src_total, d_rows, d_out,
                           1, vec, st));
-    for (int k0 = 0; k0 < src_total; k0 += 65535) {
-      const int rows = std::min(65535, src_total - k0);
-      hipLaunchKernelGGL(pred_rows_kernel, dim3((n + 255) / 256, rows), dim3(256), 0, st, d_rro,
-                         d_rcol, d_rw, n, d_src, d_rows, k0, d_out, d_path, d_stat);
+    for (int k0 = 0; k0 < src_total; k0 += 65535 * kPredRows) {
+      const int nk = std::min(65535 * kPredRows, src_total - k0);
+      hipLaunchKernelGGL(pred_rows_kernel, dim3((n + 255) / 256, (nk + kPredRows - 1) / kPredRows),
+                         dim3(256), 0, st, d_rro, d_rcol, d_rw, n, d_src, d_rows, k0, nk, d_out,
+                         d_path, d_stat);
       FAP_CHECK_LAUNCH();
     }
     return FAP_OK;
