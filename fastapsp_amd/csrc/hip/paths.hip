@@ -217,8 +217,8 @@ __global__ __launch_bounds__(256) void build_path_kernel(
     rp[i] = ver;
   }
   __syncthreads();
-  int *dd = d.fw + (int64_t)i * d.S;
-  int *dp = d.fwp + (int64_t)i * d.S;
+  FAP_GLOBAL int *dd = gptr(d.fw + (int64_t)i * d.S);
+  FAP_GLOBAL int *dp = gptr(d.fwp + (int64_t)i * d.S);
   int mx = -1;
   for (int j = tid; j < d.S; j += 256) {
     dd[j] = rd[j];
@@ -799,7 +799,7 @@ __global__ __launch_bounds__(64) void merge_path_kernel(
     for (int q = 0; q < kMpRows; ++q) {
       const int i = r0 + q;
       if (i >= r1) continue;
-      const int *arow = p.fw + (int64_t)(p.bdy + i) * p.S + k0;  // A = FW[bdy + i][k]
+      const FAP_GLOBAL int *arow = gptr(p.fw + (int64_t)(p.bdy + i) * p.S + k0);  // A = FW[bdy + i][k]
 #pragma unroll
       for (int kk = 0; kk < kMpKc; ++kk) {
         const int av = k0 + kk < K ? arow[kk] : kInf;
@@ -832,8 +832,8 @@ __global__ __launch_bounds__(64) void merge_path_kernel(
       if (j >= n) continue;
       int dv = best[q][e], pv = bpath[q][e];
       if (own[e] >= 0) {
-        dv = p.fw[(int64_t)(p.bdy + i) * p.S + own[e]];
-        pv = p.fwp[(int64_t)(p.bdy + i) * p.S + own[e]];
+        dv = gptr(p.fw)[(int64_t)(p.bdy + i) * p.S + own[e]];
+        pv = gptr(p.fwp)[(int64_t)(p.bdy + i) * p.S + own[e]];
       }
       out[orow + j] = dv;
       pout[orow + j] = pv;
@@ -861,14 +861,14 @@ __global__ __launch_bounds__(256) void path_amax_kernel(const PathPart *__restri
   int mx = -1;
   for (int64_t e = threadIdx.x; e < (int64_t)inner * p.bdy; e += 256) {
     const int i = (int)(e / p.bdy), k = (int)(e - (int64_t)i * p.bdy);
-    const int v = p.fw[(int64_t)(p.bdy + i) * p.S + k];
+    const int v = gptr(p.fw)[(int64_t)(p.bdy + i) * p.S + k];
     if (v < kInf) mx = max(mx, v);
   }
   if (kp > 0) {
     for (int64_t e = threadIdx.x; e < (int64_t)inner * kp; e += 256) {
       const int i = (int)(e / kp), k = (int)(e - (int64_t)i * kp);
       uint32_t key = 0x80000000u;
-      if (k < p.bdy) key = min((uint32_t)p.fw[(int64_t)(p.bdy + i) * p.S + k], lim) << kb;
+      if (k < p.bdy) key = min((uint32_t)gptr(p.fw)[(int64_t)(p.bdy + i) * p.S + k], lim) << kb;
       akey[ko + e] = key;
     }
   }
@@ -1017,7 +1017,7 @@ __global__ __launch_bounds__(256) void decode_path_kernel(const PathPart *__rest
   const PathPart p = parts[lo];
   const int i = (int)(row - row_begin[lo]);
   const int64_t orow = (p.out_row + i) * ld_out;
-  const int *fr = p.fw + (int64_t)i * p.S, *fpr = p.fwp + (int64_t)i * p.S;
+  const FAP_GLOBAL int *fr = gptr(p.fw + (int64_t)i * p.S), *fpr = gptr(p.fwp + (int64_t)i * p.S);
   for (int c = threadIdx.x; c < p.s; c += 256) {
     const int v = st2ed[p.offset + c];
     out[orow + v] = fr[c];
