@@ -169,15 +169,15 @@ __device__ __forceinline__ int orow(int t, int q) { return (q < 4 ? 0 : 32) + t 
 // outside [0, kdim) or outside the valid rows/columns read as +inf.  One wave,
 // staged through its LDS slab As[kBKc][64], Bs[kBKc][64].
 template <typename T>
-__device__ __forceinline__ void wave_product(T (&acc)[8][8], const T *a, int64_t lda, int arows,
-                                             const T *b, int64_t ldb, int bcols, int kdim, T *As,
-                                             T *Bs, int lane) {
+__device__ __forceinline__ void wave_product(T (&acc)[8][8], const FAP_GLOBAL T *a, int64_t lda,
+                                             int arows, const FAP_GLOBAL T *b, int64_t ldb,
+                                             int bcols, int kdim, T *As, T *Bs, int lane) {
   const int tr = lane >> 3, tc = lane & 7;
   typedef T vec4 __attribute__((ext_vector_type(4)));
   for (int k0 = 0; k0 < kdim; k0 += kBKc) {
     if (lane < arows && k0 + kBKc <= kdim) {
       // A rows start 64-element aligned and k0 is a multiple of 16: vector loads.
-      const vec4 *src = reinterpret_cast<const vec4 *>(a + (int64_t)lane * lda + k0);
+      const FAP_GLOBAL vec4 *src = reinterpret_cast<const FAP_GLOBAL vec4 *>(a + (int64_t)lane * lda + k0);
 #pragma unroll
       for (int c = 0; c < kBKc / 4; ++c) {
         const vec4 v = src[c];
@@ -216,7 +216,8 @@ __device__ __forceinline__ void wave_product(T (&acc)[8][8], const T *a, int64_t
 }
 
 template <typename T>
-__device__ __forceinline__ void load_tile(T (&acc)[8][8], const T *t, int64_t ld, int lane) {
+__device__ __forceinline__ void load_tile(T (&acc)[8][8], const FAP_GLOBAL T *t, int64_t ld,
+                                          int lane) {
   const int tr = lane >> 3, tc = lane & 7;
 #pragma unroll
   for (int q = 0; q < 8; ++q)
@@ -225,7 +226,8 @@ __device__ __forceinline__ void load_tile(T (&acc)[8][8], const T *t, int64_t ld
 }
 
 template <typename T>
-__device__ __forceinline__ void store_tile(const T (&acc)[8][8], T *t, int64_t ld, int lane) {
+__device__ __forceinline__ void store_tile(const T (&acc)[8][8], FAP_GLOBAL T *t, int64_t ld,
+                                           int lane) {
   const int tr = lane >> 3, tc = lane & 7;
 #pragma unroll
   for (int q = 0; q < 8; ++q)
@@ -235,27 +237,33 @@ __device__ __forceinline__ void store_tile(const T (&acc)[8][8], T *t, int64_t l
 
 // ------------------------------------------------------------------ one part
 template <typename T>
-__device__ void process_part(const BatchWork &wk, T *M, char *lds) {
-  const T *rows = static_cast<const T *>(wk.rows);
+__device__ void process_part(const BatchWork &wk, T *Mgen, char *lds) {
+  // The work record and the scratch slot hold generic pointers: every access goes
+  // through the global address space (flat accesses also count against the LDS counter,
+  // so the LDS staging writes after them waited for memory).
+  FAP_GLOBAL T *M = gptr(Mgen);
+  const FAP_GLOBAL T *rows = gptr(static_cast<const T *>(wk.rows));
+  const FAP_GLOBAL int *st2ed = gptr(wk.st2ed), *row_off = gptr(wk.row_off), *col = gptr(wk.col);
+  const FAP_GLOBAL int *graph_id = gptr(wk.graph_id), *ed2st = gptr(wk.ed2st);
   const int64_t ldr = wk.ldr;
   typedef typename Dist<T>::Out Out;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int num = wk.num, bdy = wk.bdy, off = wk.offset, n = wk.n;
   const int S = (num + kBT - 1) / kBT * kBT, nb = S / kBT;
-  Out *out = static_cast<Out *>(wk.out);
+  FAP_GLOBAL Out *out = gptr(static_cast<Out *>(wk.out));
 
   // 2. build (the SSSP rows were written by batch_sssp_kernel)
   for (int i = 0; i < S; ++i)
     for (int j = tid; j < S; j += kBThreads)
-      M[(int64_t)i * S + j] = (i < bdy && j < num) ? rows[(int64_t)i * ldr + wk.st2ed[off + j]]
+      M[(int64_t)i * S + j] = (i < bdy && j < num) ? rows[(int64_t)i * ldr + st2ed[off + j]]
                                                    : Dist<T>::inf();
   __syncthreads();
   for (int i = bdy + tid; i < num; i += kBThreads) {
-    const int ver = wk.st2ed[off + i];
-    for (int e = wk.row_off[ver]; e < wk.row_off[ver + 1]; ++e) {
-      const int v = wk.col[e];
-      if (wk.graph_id[v] != wk.part_id) continue;
-      const int idx = wk.ed2st[v] - off;
+    const int ver = st2ed[off + i];
+    for (int e = row_off[ver]; e < row_off[ver + 1]; ++e) {
+      const int v = col[e];
+      if (graph_id[v] != wk.part_id) continue;
+      const int idx = ed2st[v] - off;
       if (idx >= 0 && idx < num) M[(int64_t)i * S + idx] = Dist<T>::weight(wk, e);
     }
   }
@@ -269,7 +277,7 @@ __device__ void process_part(const BatchWork &wk, T *M, char *lds) {
   for (int r = 0; r < nb; ++r) {
     {  // phase 1: pivot tile closed in LDS, kP1Per consecutive entries per thread
       T *D = reinterpret_cast<T *>(lds);
-      T *g = M + (int64_t)r * kBT * S + r * kBT;
+      FAP_GLOBAL T *g = M + (int64_t)r * kBT * S + r * kBT;
       const int row = tid / (kBT / kP1Per), c0 = (tid % (kBT / kP1Per)) * kP1Per;
 #pragma unroll
       for (int c = 0; c < kP1Per; ++c) D[row * kBT + c0 + c] = g[(int64_t)row * S + c0 + c];
@@ -286,12 +294,13 @@ __device__ void process_part(const BatchWork &wk, T *M, char *lds) {
     }
     __syncthreads();
     // phase 2: row tiles (r, o) and column tiles (o, r), o != r
-    const T *P = M + (int64_t)r * kBT * S + r * kBT;
+    const FAP_GLOBAL T *P = M + (int64_t)r * kBT * S + r * kBT;
     for (int t = wave; t < 2 * (nb - 1); t += kBWaves) {
       const bool is_row = t < nb - 1;
       int o = is_row ? t : t - (nb - 1);
       o += (o >= r);
-      T *tile = is_row ? M + (int64_t)r * kBT * S + o * kBT : M + (int64_t)o * kBT * S + r * kBT;
+      FAP_GLOBAL T *tile =
+          is_row ? M + (int64_t)r * kBT * S + o * kBT : M + (int64_t)o * kBT * S + r * kBT;
       T acc[8][8];
       load_tile(acc, tile, S, lane);
       if (is_row) wave_product(acc, P, S, kBT, tile, S, kBT, kBT, As, Bs, lane);
@@ -304,7 +313,7 @@ __device__ void process_part(const BatchWork &wk, T *M, char *lds) {
       int i = t / (nb - 1), j = t - i * (nb - 1);
       i += (i >= r);
       j += (j >= r);
-      T *tile = M + (int64_t)i * kBT * S + j * kBT;
+      FAP_GLOBAL T *tile = M + (int64_t)i * kBT * S + j * kBT;
       T acc[8][8];
       load_tile(acc, tile, S, lane);
       wave_product(acc, M + (int64_t)i * kBT * S + r * kBT, S, kBT,
@@ -334,7 +343,7 @@ __device__ void process_part(const BatchWork &wk, T *M, char *lds) {
       for (int c = 0; c < 8; ++c) {
         const int j = col0 + orow(tc, c);
         if (j >= n) continue;
-        const int own = wk.graph_id[j] == wk.part_id ? wk.ed2st[j] - off : -1;
+        const int own = graph_id[j] == wk.part_id ? ed2st[j] - off : -1;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
           const int i = row0 + orow(tr, q);
@@ -350,7 +359,7 @@ __device__ void process_part(const BatchWork &wk, T *M, char *lds) {
   // 5. boundary rows (after the merge, which reads them as its B operand)
   for (int i = 0; i < bdy; ++i)
     for (int j = tid; j < n; j += kBThreads) {
-      const T v = wk.graph_id[j] == wk.part_id ? M[(int64_t)i * S + wk.ed2st[j] - off]
+      const T v = graph_id[j] == wk.part_id ? M[(int64_t)i * S + ed2st[j] - off]
                                                : rows[(int64_t)i * ldr + j];
       out[(wk.out_row + i) * wk.ld_out + j] = Dist<T>::out(v);
     }
