@@ -210,9 +210,10 @@ typedef struct fap_stage_times {
  * Boundary rows: in the BATCHED form, when the parts cover every vertex and hold at
  * most 4096 boundary vertices in all (and FAP_MODE_FLAG_SSSP is absent), the boundary
  * rows come from the boundary graph instead of one SSSP per
- * boundary vertex: each part's Floyd-Warshall runs on its in-part arcs, the graph of
- * all boundary vertices (in-part boundary distances plus cut arcs) is closed by
- * Floyd-Warshall, and one min-plus pass writes every boundary row.  Unit-weight graphs
+ * boundary vertex: each part's in-part distances are computed on its in-part arcs, the
+ * graph of all boundary vertices (in-part boundary distances plus cut arcs) is closed
+ * by Bellman-Ford sweeps (for unit weights with work lists in LDS), and one min-plus
+ * pass writes every boundary row.  Unit-weight graphs
  * then always take the 16-bit Floyd-Warshall (no flag check, no wait), others int32.
  * The stage times report this stage as sssp_ms.  In this form, on unit-weight graphs
  * with component labels (every arc paired with its reverse) and parts of at most 2048
@@ -235,8 +236,13 @@ int fap_hip_last_stage_times(fap_stage_times *t);
  * min-plus run in a keyed form (distance and path source in one 32-bit key, one
  * unsigned minimum per candidate) when the part's values stay below the key limits,
  * in a compare-and-select form otherwise; the results are the same.  The call waits
- * for the stream twice to read the maxima that choose the forms.  Stage times as for
- * fap_hip_solve_parts, with parts_keyed / parts_fw_keyed counting the keyed parts. */
+ * for the stream twice to read the maxima that choose the forms.  Boundary-row
+ * distances: from the boundary graph (as in fap_hip_solve_parts, with the in-part BFS of
+ * the boundary sources) when the parts cover a unit-weight graph with component labels,
+ * hold at most 4096 boundary vertices and at most 2048 vertices each, else by SSSP; the
+ * predecessors follow from the distance rows either way.  Stage times as for
+ * fap_hip_solve_parts, with parts_keyed / parts_fw_keyed counting the keyed parts and
+ * boundary_graph set when the boundary graph gave the boundary rows. */
 int fap_hip_solve_parts_path(const fap_graph_dev *g, const int32_t *d_rrow_offset,
                              const int32_t *d_rcol, const int32_t *d_rw,
                              const fap_part_desc *parts, int32_t nparts, int32_t *d_out,
