@@ -68,7 +68,8 @@ class FapStageTimes(C.Structure):
     _fields_ = [("sssp_ms", C.c_float), ("build_ms", C.c_float), ("floyd_ms", C.c_float),
                 ("minplus_ms", C.c_float), ("decode_ms", C.c_float), ("total_ms", C.c_float),
                 ("parts_fw16", _i32), ("parts_redone", _i32), ("boundary_graph", _i32),
-                ("parts_keyed", _i32), ("parts_fw_keyed", _i32), ("parts_bfs", _i32)]
+                ("parts_keyed", _i32), ("parts_fw_keyed", _i32), ("parts_bfs", _i32),
+                ("merge_packed", _i32)]
 
 
 SIGNATURES = {

@@ -133,6 +133,10 @@ std::vector<Span> g_spans;
 hipEvent_t g_total[2] = {nullptr, nullptr};
 constexpr int kChunks = 8;
 int g_parts16 = 0, g_redone = 0, g_bgraph = 0, g_keyed = 0, g_fw_keyed = 0, g_bfs = 0;
+// Packed-merge report: the largest finite boundary-row entry, copied to pinned host
+// memory at the end of the pass, against the pass's limit (g_lim16 < 0: no report).
+int *g_maxb_host = nullptr;
+int g_lim16 = -1;
 
 // What the last pass left for the flag check of the 16-bit form.
 struct Pass16 {
@@ -157,6 +161,7 @@ int timing_start(hipStream_t st) {
   g_spans.clear();
   g_total[0] = g_total[1] = nullptr;
   g_parts16 = g_redone = g_bgraph = g_keyed = g_fw_keyed = g_bfs = 0;
+  g_lim16 = -1;
   FAP_TRY(g_events.take(&g_total[0]));
   FAP_CHECK_HIP(hipEventRecord(g_total[0], st));
   return FAP_OK;
@@ -652,6 +657,11 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
       }));
     }
   }
+  if (merge16 && bg_total > 0 && !append) {
+    if (!g_maxb_host) FAP_CHECK_HIP(hipHostMalloc((void **)&g_maxb_host, sizeof(int)));
+    FAP_CHECK_HIP(hipMemcpyAsync(g_maxb_host, d_maxb, sizeof(int), hipMemcpyDeviceToHost, st));
+    g_lim16 = lim16;
+  }
   FAP_TRY(event(st, &g_total[1]));
   if (use16) g_parts16 += P;
   if (bfs_parts) g_bfs += P;
@@ -677,6 +687,7 @@ static int solve_parts(const fap_graph_dev *g, const fap_part_desc *parts, int32
                             FAP_MODE_FLAG_FW)) == 0,
               "fap_hip_solve_parts: unknown mode %d", mode);
   g_parts16 = g_redone = g_bgraph = g_keyed = g_fw_keyed = g_bfs = 0;
+  g_lim16 = -1;
   FAP_TRY(solve_pass(g, parts, nparts, d_out, ld_out, mode, stream, false));
   if (g_pass16.parts.empty()) return FAP_OK;
   // Flag check of the 16-bit form: parts whose closure reached the cap inside one
@@ -732,6 +743,7 @@ extern "C" int fap_hip_last_stage_times(fap_stage_times *t) {
   acc.parts_keyed = g_keyed;
   acc.parts_fw_keyed = g_fw_keyed;
   acc.parts_bfs = g_bfs;
+  acc.merge_packed = g_lim16 < 0 || !g_total[1] ? -1 : (*g_maxb_host <= g_lim16 ? 1 : 0);
   *t = acc;
   return FAP_OK;
 }

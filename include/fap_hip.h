@@ -190,6 +190,8 @@ typedef struct fap_stage_times {
   int32_t parts_keyed;  /* path solves: parts whose min-plus ran in the keyed form */
   int32_t parts_fw_keyed; /* path solves: parts whose Floyd-Warshall ran in the keyed form */
   int32_t parts_bfs;    /* parts whose in-part distances came from the bit-parallel BFS */
+  int32_t merge_packed; /* boundary-graph form on unit weights: 1 when the merge ran on
+                           packed 16-bit rows, 0 when on int32 rows; -1 otherwise */
 } fap_stage_times;
 
 /* The whole per-part pipeline of fapGraph::run (fap/fap.cpp:195-503) for a list

@@ -179,3 +179,46 @@ def minplus_i32(a, b):
     lib().orc_minplus_i32(_p(a, C.c_int32), C.c_int64(max(K, 1)), _p(b, C.c_int32),
                           C.c_int64(N), _p(c, C.c_int32), C.c_int64(N), M, N, K)
     return c
+
+
+def compare_exact_rows(row_offset, col, weight, sources, got):
+    """Rows of `got` (one per source, any row stride) against exact BFS / integer
+    Dijkstra rows, computed and compared per thread without an n x n buffer.
+    Returns (number of differing rows, first differing row or -1)."""
+    ro, cv, src = i32(row_offset), i32(col), i32(sources)
+    n = len(ro) - 1
+    if got.dtype != np.int32 or got.shape[0] != len(src) or got.shape[1] < n or \
+            got.strides[1] != 4:
+        raise ValueError("got: int32 rows of at least n entries, one per source")
+    wa = None if weight is None else i32(weight)
+    first = C.c_int64()
+    f = lib().orc_compare_exact_rows
+    f.restype = C.c_int64
+    bad = f(C.c_int(n), _p(ro, C.c_int32), _p(cv, C.c_int32), _p(wa, C.c_int32),
+            _p(src, C.c_int32), C.c_int64(len(src)), got.ctypes.data_as(C.POINTER(C.c_int32)),
+            C.c_int64(got.strides[0] // 4), C.byref(first))
+    return int(bad), int(first.value)
+
+
+def compare_parts(first, last, csr, lay, got, row_base):
+    """Parts first..last (1-based) by the reference CPU path's restatement (orc_run_part,
+    OpenMP across parts) against `got`, whose row 0 is sorted row row_base.  Returns
+    (number of differing entries, first differing got row or -1)."""
+    ro, cv = i32(csr["row_offset"]), i32(csr["col"])
+    w = f32(csr["weight"])
+    n = len(ro) - 1
+    adj = i32(np.diff(ro))
+    lo = int(lay["offset"][first]) - row_base
+    hi = int(lay["offset"][last]) + int(lay["num"][last]) - row_base
+    if got.dtype != np.int32 or lo < 0 or got.shape[0] < hi or got.shape[1] < n or \
+            got.strides[1] != 4:
+        raise ValueError("got does not hold the rows of the parts")
+    args = [i32(lay[k]) for k in ("graph_id", "st2ed", "ed2st", "num", "offset", "bdy")]
+    fb = C.c_int64()
+    f = lib().orc_compare_parts
+    f.restype = C.c_int64
+    bad = f(C.c_int(first), C.c_int(last), C.c_int(n), _p(adj, C.c_int32), _p(ro, C.c_int32),
+            _p(cv, C.c_int32), _p(w, C.c_float), *[_p(a, C.c_int32) for a in args],
+            got.ctypes.data_as(C.POINTER(C.c_int32)), C.c_int64(got.strides[0] // 4),
+            C.c_int64(row_base), C.byref(fb))
+    return int(bad), int(fb.value)

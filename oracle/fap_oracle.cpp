@@ -218,42 +218,48 @@ void orc_to_i32(const float *in, int32_t *out, int64_t count) {
 
 // Independent exact distances: BFS (w == NULL) or integer Dijkstra, one row per
 // source, int32 with INT32_MAX / 2 for unreachable.  OpenMP over sources.
-void orc_exact_rows(int n, const int *row_offset, const int *col, const int *w, const int *src,
-                    int nsrc, int32_t *out) {
-#pragma omp parallel for schedule(dynamic)
-  for (int i = 0; i < nsrc; ++i) {
-    int32_t *d = out + (int64_t)i * n;
-    std::fill_n(d, n, kInf);
-    d[src[i]] = 0;
-    if (!w) {
-      std::vector<int> q(n);
-      int head = 0, tail = 0;
-      q[tail++] = src[i];
-      while (head < tail) {
-        const int u = q[head++];
-        for (int e = row_offset[u]; e < row_offset[u + 1]; ++e)
-          if (d[col[e]] == kInf) {
-            d[col[e]] = d[u] + 1;
-            q[tail++] = col[e];
-          }
-      }
-    } else {
-      typedef std::pair<int64_t, int> P;
-      std::priority_queue<P, std::vector<P>, std::greater<P>> q;
-      q.push({0, src[i]});
-      while (!q.empty()) {
-        const P x = q.top();
-        q.pop();
-        if (x.first > d[x.second]) continue;
-        for (int e = row_offset[x.second]; e < row_offset[x.second + 1]; ++e) {
-          const int64_t nd = x.first + w[e];
-          if (nd < d[col[e]]) {
-            d[col[e]] = (int32_t)nd;
-            q.push({nd, col[e]});
-          }
+static void exact_row(int n, const int *row_offset, const int *col, const int *w, int src,
+                      int32_t *d, std::vector<int> &q) {
+  std::fill_n(d, n, kInf);
+  d[src] = 0;
+  if (!w) {
+    q.resize(n);
+    int head = 0, tail = 0;
+    q[tail++] = src;
+    while (head < tail) {
+      const int u = q[head++];
+      for (int e = row_offset[u]; e < row_offset[u + 1]; ++e)
+        if (d[col[e]] == kInf) {
+          d[col[e]] = d[u] + 1;
+          q[tail++] = col[e];
+        }
+    }
+  } else {
+    typedef std::pair<int64_t, int> P;
+    std::priority_queue<P, std::vector<P>, std::greater<P>> pq;
+    pq.push({0, src});
+    while (!pq.empty()) {
+      const P x = pq.top();
+      pq.pop();
+      if (x.first > d[x.second]) continue;
+      for (int e = row_offset[x.second]; e < row_offset[x.second + 1]; ++e) {
+        const int64_t nd = x.first + w[e];
+        if (nd < d[col[e]]) {
+          d[col[e]] = (int32_t)nd;
+          pq.push({nd, col[e]});
         }
       }
     }
+  }
+}
+
+void orc_exact_rows(int n, const int *row_offset, const int *col, const int *w, const int *src,
+                    int nsrc, int32_t *out) {
+#pragma omp parallel
+  {
+    std::vector<int> q;
+#pragma omp for schedule(dynamic)
+    for (int i = 0; i < nsrc; ++i) exact_row(n, row_offset, col, w, src[i], out + (int64_t)i * n, q);
   }
 }
 
@@ -284,6 +290,82 @@ void orc_exact_rows_f64(int n, const int *row_offset, const int *col, const floa
       }
     }
   }
+}
+
+// Streaming checkers for full-size results (nothing n x n is materialised): each
+// computes its expected rows per thread and compares them with the rows a caller
+// copied out of the device, returning the number of differing rows / entries.
+//
+// Rows got[r * ld_got .. + n) against exact distances from src[r] (BFS when w is
+// NULL, else integer Dijkstra, as orc_exact_rows).  *first_bad = the first
+// differing row (or -1).
+int64_t orc_compare_exact_rows(int n, const int *row_offset, const int *col, const int *w,
+                               const int *src, int64_t nsrc, const int32_t *got, int64_t ld_got,
+                               int64_t *first_bad) {
+  int64_t bad = 0, first = -1;
+#pragma omp parallel
+  {
+    std::vector<int32_t> row(n);
+    std::vector<int> q;
+    int64_t my_bad = 0, my_first = -1;
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t i = 0; i < nsrc; ++i) {
+      exact_row(n, row_offset, col, w, src[i], row.data(), q);
+      if (std::memcmp(row.data(), got + i * ld_got, sizeof(int32_t) * n) != 0) {
+        ++my_bad;
+        if (my_first < 0 || i < my_first) my_first = i;
+      }
+    }
+#pragma omp critical
+    {
+      bad += my_bad;
+      if (my_first >= 0 && (first < 0 || my_first < first)) first = my_first;
+    }
+  }
+  if (first_bad) *first_bad = first;
+  return bad;
+}
+
+// Parts [first, last] (1-based) computed by orc_run_part (the reference CPU
+// path's arithmetic, in float with MAXVALUE) and compared, after the float ->
+// int32 mapping of orc_to_i32, with got: part p's row r is got row
+// blk_off[p] + r - row_base (stride ld_got).  Returns the number of differing
+// entries; *first_bad = the first differing got row (or -1).
+int64_t orc_compare_parts(int first, int last, int n, const int *adj_size,
+                          const int *row_offset, const int *col, const float *w,
+                          const int *graph_id, const int *st2ed, const int *ed2st,
+                          const int *blk_num, const int *blk_off, const int *bdy_num,
+                          const int32_t *got, int64_t ld_got, int64_t row_base,
+                          int64_t *first_bad) {
+  int64_t bad = 0, firstrow = -1;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int p = first; p <= last; ++p) {
+    const int s = blk_num[p];
+    std::vector<float> dist((size_t)s * n);
+    std::vector<int> path((size_t)s * n);
+    orc_run_part(p, n, adj_size, row_offset, col, w, graph_id, st2ed, ed2st, blk_num, blk_off,
+                 bdy_num, dist.data(), path.data());
+    int64_t my_bad = 0, my_first = -1;
+    for (int r = 0; r < s; ++r) {
+      const int64_t gr = (int64_t)blk_off[p] + r - row_base;
+      const int32_t *g = got + gr * ld_got;
+      const float *d = dist.data() + (size_t)r * n;
+      for (int j = 0; j < n; ++j) {
+        const int32_t want = d[j] >= kMaxValue ? kInf : (int32_t)d[j];
+        if (g[j] != want) {
+          ++my_bad;
+          if (my_first < 0) my_first = gr;
+        }
+      }
+    }
+#pragma omp critical
+    {
+      bad += my_bad;
+      if (my_first >= 0 && (firstrow < 0 || my_first < firstrow)) firstrow = my_first;
+    }
+  }
+  if (first_bad) *first_bad = firstrow;
+  return bad;
 }
 
 // Textbook Floyd-Warshall on int32 (k-i-j), for kernel-level checks.

@@ -231,3 +231,33 @@ def test_oracle_paths_are_valid_predecessors():
             else:
                 u = int(path[r, v])
                 assert (u, v) in arcs and dist[r, u] + 1 == dist[r, v]
+
+
+def test_streaming_comparators_agree_and_catch_planted_errors():
+    """compare_exact_rows / compare_parts (the full-size checkers) report zero on the
+    oracle's own result, and count a planted one-entry change in the checked rows --
+    in a boundary row and in an inner row, at a row stride wider than n."""
+    import fastapsp_amd as F
+
+    for g, w in [(F.FapGraph.road_grid(30, 30, 0.1, 0.02, 4, 5), None),
+                 (F.FapGraph.knn(700, 3, 1000, 5, 4), "w")]:
+        g.preCondition()
+        lay, csr = g.layout(), g.csr()
+        n = g.num_vertexs
+        wt = None if w is None else csr["weight"]
+        full = O.solve_parts(1, g.K, csr, lay)
+        wide = np.full((n, n + 13), -7, np.int32)
+        wide[:, :n] = full
+        # parts 2..K-1 only, in a block whose row 0 is part 2's first row
+        r0 = int(lay["offset"][2])
+        r1 = int(lay["offset"][g.K - 1] + lay["num"][g.K - 1])
+        blk = wide[r0:r1]
+        assert O.compare_exact_rows(csr["row_offset"], csr["col"], wt, lay["st2ed"][r0:r1],
+                                    blk) == (0, -1)
+        assert O.compare_parts(2, g.K - 1, csr, lay, blk, r0) == (0, -1)
+        for r in (int(lay["offset"][3]), int(lay["offset"][3] + lay["bdy"][3] + 1)):
+            bad = blk.copy()
+            bad[r - r0, int(lay["st2ed"][(r * 7) % n])] += 1
+            assert O.compare_exact_rows(csr["row_offset"], csr["col"], wt,
+                                        lay["st2ed"][r0:r1], bad) == (1, r - r0)
+            assert O.compare_parts(2, g.K - 1, csr, lay, bad, r0) == (1, r - r0)
