@@ -94,6 +94,7 @@ SIGNATURES = {
     "fap_hip_event_elapsed_ms": (C.c_int, [_vp, _vp, _pf32]),
     "fap_hip_sssp_batch": (C.c_int, [_vp, _vp, _vp, _i32, _i64, _vp, _i32, _vp, _i64, _vp]),
     "fap_hip_floyd": (C.c_int, [_vp, _i32, _i64, _vp]),
+    "fap_hip_floyd_noalloc": (C.c_int, [_vp, _i32, _i64, _vp]),
     "fap_hip_minplus": (C.c_int, [_vp, _i64, _vp, _i64, _vp, _i64, _i32, _i32, _i32, _vp]),
     "fap_hip_submat_build": (C.c_int, [_vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp,
                                        _i32, _i32, _i32, _i32, _vp]),

@@ -21,6 +21,11 @@ hipStream_t lib_stream();
 hipStream_t aux_stream(int i);
 // Compute units of the device (set by fap_hip_init).
 int device_cus();
+// Device the library runs on (set by fap_hip_init), -1 before.
+int lib_device();
+// Looks up the device address of the allocation-free Floyd-Warshall scratch on
+// `device` (current); called once per device by fap_hip_init.
+int floyd_scratch_setup(int device);
 inline hipStream_t resolve(void *stream) {
   return stream ? static_cast<hipStream_t>(stream) : lib_stream();
 }

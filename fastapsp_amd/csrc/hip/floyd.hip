@@ -254,9 +254,74 @@ __global__ void pad_out_kernel(int *__restrict__ dst, int64_t ld, int s, const i
   dst[(int64_t)i * ld + j] = src[(int64_t)i * S + j];
 }
 
+// ---- allocation-free variant (fap_hip_floyd_noalloc) -------------------------
+// One descriptor slot (256 bytes) followed by a padded matrix of at most
+// FAP_FLOYD_NOALLOC_MAX_S^2 entries.  In host code `scratch` is only a stand-in for
+// the device symbol: its device address on each device comes from
+// hipGetSymbolAddress (floyd_scratch_setup) and only that address is ever used.
+constexpr int kNoAllocMaxS = FAP_FLOYD_NOALLOC_MAX_S;
+constexpr int64_t kScratchDescInts = 64;
+static_assert(sizeof(FwDesc) <= 4 * kScratchDescInts, "descriptor slot too small");
+static_assert(kNoAllocMaxS % kT == 0, "limit must be a whole number of tiles");
+__device__ __align__(256) int scratch[kScratchDescInts + (int64_t)kNoAllocMaxS * kNoAllocMaxS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int floyd_scratch_setup(int device) {
+  FAP_REQUIRE(device >= 0 && device < kMaxDevices, "floyd scratch: device %d out of range",
+              device);
+  if (g_scratch_ptr[device]) return FAP_OK;
+  void *p = nullptr;
+  FAP_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  FAP_REQUIRE(p != nullptr, "floyd scratch: no device address");
+  g_scratch_ptr[device] = p;
+  return FAP_OK;
+}
+
+__global__ void set_fw_desc_kernel(FwDesc *__restrict__ dst, FwDesc h) { *dst = h; }
+
 }  // namespace fap_hip
 
 using namespace fap_hip;
+
+extern "C" int fap_hip_floyd_noalloc(int32_t *d_mat, int32_t s, int64_t ld, void *stream) {
+  if (!initialized()) {
+    set_error("fap_hip_init has not been called");
+    return FAP_ERR_STATE;
+  }
+  FAP_REQUIRE(s >= 0 && ld >= s, "fap_hip_floyd_noalloc: bad sizes s=%d ld=%lld", s,
+              (long long)ld);
+  FAP_REQUIRE(s <= kNoAllocMaxS,
+              "fap_hip_floyd_noalloc: s=%d exceeds the scratch limit of %d vertices", s,
+              kNoAllocMaxS);
+  if (s == 0) return FAP_OK;
+  FAP_REQUIRE(d_mat != nullptr, "fap_hip_floyd_noalloc: null matrix");
+  const int dev = lib_device();
+  FAP_REQUIRE(dev >= 0 && dev < kMaxDevices && g_scratch_ptr[dev],
+              "fap_hip_floyd_noalloc: scratch not set up on device %d", dev);
+  hipStream_t st = resolve(stream);
+  int *base = static_cast<int *>(g_scratch_ptr[dev]);
+  FwDesc *desc = reinterpret_cast<FwDesc *>(base);
+  int *pad = base + kScratchDescInts;
+  const int S = (s + kT - 1) / kT * kT;
+  const int64_t tot = (int64_t)S * S;
+  // Every entry of the padded S x S block is written, so nothing left by an earlier
+  // call (of any size) is read.
+  hipLaunchKernelGGL(pad_in_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, pad, S,
+                     d_mat, ld, s);
+  FAP_CHECK_LAUNCH();
+  hipLaunchKernelGGL(set_fw_desc_kernel, dim3(1), dim3(1), 0, st, desc, FwDesc{pad, S, S / kT});
+  FAP_CHECK_LAUNCH();
+  FAP_TRY(launch_fw(desc, 1, S / kT, st));
+  const int64_t out = (int64_t)s * s;
+  hipLaunchKernelGGL(pad_out_kernel, dim3((unsigned)((out + 255) / 256)), dim3(256), 0, st, d_mat,
+                     ld, s, pad, S);
+  FAP_CHECK_LAUNCH();
+  return FAP_OK;
+}
 
 extern "C" int fap_hip_floyd(int32_t *d_mat, int32_t s, int64_t ld, void *stream) {
   if (!initialized()) {

@@ -38,6 +38,7 @@ void clear_error() { g_last_error.clear(); }
 hipStream_t lib_stream() { return g_stream; }
 hipStream_t aux_stream(int i) { return g_aux[i & 1]; }
 int device_cus() { return g_cus; }
+int lib_device() { return g_device; }
 bool initialized() { return g_init; }
 
 int ws_fence_wait(hipStream_t st) {
@@ -176,6 +177,7 @@ int fap_hip_init(int device) {
   FAP_CHECK_HIP(hipStreamCreateWithFlags(&g_aux[1], hipStreamNonBlocking));
   FAP_CHECK_HIP(hipEventCreateWithFlags(&g_ws_fence, hipEventDisableTiming));
   FAP_CHECK_HIP(hipDeviceGetAttribute(&g_cus, hipDeviceAttributeMultiprocessorCount, device));
+  FAP_TRY(floyd_scratch_setup(device));
   g_device = device;
   g_init = true;
   return FAP_OK;

@@ -121,6 +121,20 @@ int fap_hip_sssp_batch(const int32_t *d_row_offset, const int32_t *d_col, const 
  * Replaces floyd_path_gpu / floyd_path (fap/kernel/floyd.h:150-170). */
 int fap_hip_floyd(int32_t *d_mat, int32_t s, int64_t ld, void *stream);
 
+/* Allocation-free variant of fap_hip_floyd.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and result as
+ * fap_hip_floyd; the padded matrix and the descriptor live in a scratch array that
+ * the library holds on every device it runs on until the process ends
+ * (4 * (64 + FAP_FLOYD_NOALLOC_MAX_S^2) bytes = 64 MiB), so the call allocates
+ * nothing, copies nothing from the host and does not wait for the stream: it only
+ * enqueues kernels on `stream`.  s > FAP_FLOYD_NOALLOC_MAX_S is refused with
+ * FAP_ERR_INVALID.  Every call on a device shares that device's scratch: calls that
+ * overlap on one device (on different streams without ordering between them, or
+ * from several host threads) race; order them on one stream. */
+#define FAP_FLOYD_NOALLOC_MAX_S 4096
+int fap_hip_floyd_noalloc(int32_t *d_mat, int32_t s, int64_t ld, void *stream);
+
 /* Min-plus product with overwrite semantics:
  *   C[i][j] = min(FAP_INF, min_k A[i][k] + B[k][j]),  0 <= i < M, 0 <= j < N.
  * K == 0 gives all FAP_INF.  Strides in elements.
