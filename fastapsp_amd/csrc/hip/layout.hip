@@ -232,3 +232,63 @@ extern "C" int fap_hip_submat_decode(const int32_t *d_sub, int64_t ld_sub, int32
   FAP_CHECK_HIP(hipStreamSynchronize(st));
   return FAP_OK;
 }
+
+// ---- float <-> int32 distance conversion ---------------------------------------
+// The reference's seam works on float distances with MAXVALUE = 1e8 for unreachable
+// (fap/utils/parameter.h:26); the int32 kernels use FAP_INF.  For integer weights
+// every finite reference distance is an integer below 2^24, so both maps are exact.
+namespace {
+
+__global__ void f32_to_i32_kernel(const float *__restrict__ in, int *__restrict__ out,
+                                  int64_t count) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float v = in[i];
+    out[i] = v >= 1e8f ? kInf : (int)v;
+  }
+}
+
+__global__ void i32_to_f32_kernel(const int *__restrict__ in, float *__restrict__ out,
+                                  int64_t count) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int v = in[i];
+    out[i] = v >= kInf ? 1e8f : (float)v;
+  }
+}
+
+unsigned conv_grid(int64_t count) {
+  return (unsigned)std::min<int64_t>((count + 255) / 256, 8LL * device_cus() * 8);
+}
+
+}  // namespace
+
+extern "C" int fap_hip_f32_to_i32(const float *d_src, int32_t *d_dst, int64_t count,
+                                  void *stream) {
+  if (!initialized()) {
+    set_error("fap_hip_init has not been called");
+    return FAP_ERR_STATE;
+  }
+  FAP_REQUIRE(count >= 0, "fap_hip_f32_to_i32: negative count");
+  if (count == 0) return FAP_OK;
+  FAP_REQUIRE(d_src && d_dst, "fap_hip_f32_to_i32: null pointer");
+  hipLaunchKernelGGL(f32_to_i32_kernel, dim3(conv_grid(count)), dim3(256), 0, resolve(stream),
+                     d_src, d_dst, count);
+  FAP_CHECK_LAUNCH();
+  return FAP_OK;
+}
+
+extern "C" int fap_hip_i32_to_f32(const int32_t *d_src, float *d_dst, int64_t count,
+                                  void *stream) {
+  if (!initialized()) {
+    set_error("fap_hip_init has not been called");
+    return FAP_ERR_STATE;
+  }
+  FAP_REQUIRE(count >= 0, "fap_hip_i32_to_f32: negative count");
+  if (count == 0) return FAP_OK;
+  FAP_REQUIRE(d_src && d_dst, "fap_hip_i32_to_f32: null pointer");
+  hipLaunchKernelGGL(i32_to_f32_kernel, dim3(conv_grid(count)), dim3(256), 0, resolve(stream),
+                     d_src, d_dst, count);
+  FAP_CHECK_LAUNCH();
+  return FAP_OK;
+}

@@ -115,8 +115,8 @@ constexpr int kPredRows = 8;
 constexpr int kPredDeg = 8;
 __global__ __launch_bounds__(256) void pred_rows_kernel(
     const int *__restrict__ rro, const int *__restrict__ rcol, const int *__restrict__ rw, int n,
-    const int *__restrict__ src, const int64_t *__restrict__ rows, int k0, int nk,
-    const int *__restrict__ out, int *__restrict__ path, int *__restrict__ bmax_slots) {
+    const int *__restrict__ src, const int64_t *__restrict__ rows, int64_t ld_rows, int k0,
+    int nk, const int *__restrict__ out, int *__restrict__ path, int *__restrict__ bmax_slots) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   const int kb = k0 + blockIdx.y * kPredRows, ke = min(k0 + nk, kb + kPredRows);
   int e0 = 0, deg = 0;
@@ -135,7 +135,8 @@ __global__ __launch_bounds__(256) void pred_rows_kernel(
     }
   }
   for (int k = kb; k < ke; ++k) {
-    const int *d = out + rows[k];
+    const int64_t row = rows ? rows[k] : (int64_t)k * ld_rows;  // element offset of row k
+    const int *d = out + row;
     const int dv = v < n ? d[v] : kInf;
     int mx = dv < kInf ? dv : -1;
 #pragma unroll
@@ -166,7 +167,7 @@ __global__ __launch_bounds__(256) void pred_rows_kernel(
       for (int e = e0 + kPredDeg; e < e0 + deg; ++e) consider(rcol[e], rw ? rw[e] : 1);
       p = bu == 0x7FFFFFFF ? -1 : bu;
     }
-    path[rows[k] + v] = p;
+    path[row + v] = p;
   }
 }
 
@@ -1175,8 +1176,8 @@ static int solve_parts_path(const fap_graph_dev *g, const int32_t *d_rro, const 
     for (int k0 = 0; k0 < src_total; k0 += 65535 * kPredRows) {
       const int nk = std::min(65535 * kPredRows, src_total - k0);
       hipLaunchKernelGGL(pred_rows_kernel, dim3((n + 255) / 256, (nk + kPredRows - 1) / kPredRows),
-                         dim3(256), 0, st, d_rro, d_rcol, d_rw, n, d_src, d_rows, k0, nk, d_out,
-                         d_path, d_stat);
+                         dim3(256), 0, st, d_rro, d_rcol, d_rw, n, d_src, d_rows, (int64_t)0, k0,
+                         nk, d_out, d_path, d_stat);
       FAP_CHECK_LAUNCH();
     }
     return FAP_OK;
@@ -1398,4 +1399,208 @@ extern "C" int fap_hip_solve_parts_path(const fap_graph_dev *g, const int32_t *d
                                  ld_out, st);
   const int f = ws_fence_record(st);
   return s != FAP_OK ? s : f;
+}
+
+// ---- kernel-level seam with predecessors ---------------------------------------
+// The reference's seam functions carry path matrices beside the distances:
+// floyd_path (fap/kernel/floyd.h:162-170), min_plus_path_advanced
+// (fap/kernel/minplus.h:93-105) and handle_boundry_path (fap/kernel/batch_sssp.h:
+// 280-303).  These entry points give the same (distance, path) results as their CPU
+// branches, using the kernels of the path pipeline above.
+namespace fap_hip {
+namespace {
+
+Workspace g_ws_seam_fw;    // padded distance + path matrices of fap_hip_floyd_path
+Workspace g_ws_seam_snap;  // its per-round records
+Workspace g_ws_seam_meta;  // part record + flag; predecessor statistics
+
+__global__ void pad_in_path_kernel(int *__restrict__ dd, int *__restrict__ dp, int S,
+                                   const int *__restrict__ sd, const int *__restrict__ sp,
+                                   int64_t ld, int s) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)S * S) return;
+  const int i = (int)(idx / S), j = (int)(idx - (int64_t)i * S);
+  const bool in = i < s && j < s;
+  dd[idx] = in ? sd[(int64_t)i * ld + j] : kInf;
+  dp[idx] = in ? sp[(int64_t)i * ld + j] : -1;
+}
+
+__global__ void pad_out_path_kernel(int *__restrict__ sd, int *__restrict__ sp, int64_t ld, int s,
+                                    const int *__restrict__ dd, const int *__restrict__ dp, int S) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)s * s) return;
+  const int i = (int)(idx / s), j = (int)(idx - (int64_t)i * s);
+  sd[(int64_t)i * ld + j] = dd[(int64_t)i * S + j];
+  sp[(int64_t)i * ld + j] = dp[(int64_t)i * S + j];
+}
+
+// C = A (x) B with paths: one 64 x 64 output tile per 256-thread workgroup, K in LDS
+// chunks of 16 taken in ascending order, so each thread's strict '<' keeps the
+// smallest k reaching the minimum (min_plus_path_CPU_advanced, minplus.h:51-64).
+// Thread (ty, tx) owns rows ty*4 .. ty*4+3 and columns tx, tx+16, tx+32, tx+48.
+constexpr int kSpT = 64, kSpKc = 16;
+__global__ __launch_bounds__(256) void minplus_path_seam_kernel(
+    const int *__restrict__ A, int64_t lda, const int *__restrict__ B, int64_t ldb,
+    const int *__restrict__ Bp, int64_t ldbp, int *__restrict__ C, int64_t ldc,
+    int *__restrict__ Cp, int64_t ldcp, int M, int N, int K) {
+  __shared__ int As[kSpKc][kSpT];
+  __shared__ int Bs[kSpKc][kSpT];
+  __shared__ int Bps[kSpKc][kSpT];
+  const int i0 = blockIdx.y * kSpT, j0 = blockIdx.x * kSpT;
+  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+  int best[4][4], bp[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      best[q][c] = kInf;
+      bp[q][c] = -1;
+    }
+  for (int k0 = 0; k0 < K; k0 += kSpKc) {
+    for (int e = tid; e < kSpKc * kSpT; e += 256) {
+      const int r = e / kSpKc, kk = e - r * kSpKc;  // A: row-major reads
+      As[kk][r] = (i0 + r < M && k0 + kk < K) ? A[(int64_t)(i0 + r) * lda + k0 + kk] : kInf;
+      const int kb = e / kSpT, c = e - kb * kSpT;  // B: coalesced rows
+      const bool ok = k0 + kb < K && j0 + c < N;
+      Bs[kb][c] = ok ? B[(int64_t)(k0 + kb) * ldb + j0 + c] : kInf;
+      Bps[kb][c] = ok ? Bp[(int64_t)(k0 + kb) * ldbp + j0 + c] : -1;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < kSpKc; ++kk) {
+      int a[4], b[4], pb[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a[q] = As[kk][ty * 4 + q];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        b[c] = Bs[kk][tx + 16 * c];
+        pb[c] = Bps[kk][tx + 16 * c];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int cand = a[q] + b[c];
+          if (cand < best[q][c]) {
+            best[q][c] = cand;
+            bp[q][c] = pb[c];
+          }
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int i = i0 + ty * 4 + q;
+    if (i >= M) continue;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int j = j0 + tx + 16 * c;
+      if (j >= N) continue;
+      C[(int64_t)i * ldc + j] = best[q][c];
+      Cp[(int64_t)i * ldcp + j] = bp[q][c];
+    }
+  }
+}
+
+}  // namespace
+}  // namespace fap_hip
+
+extern "C" int fap_hip_floyd_path(int32_t *d_mat, int32_t *d_path, int32_t s, int64_t ld,
+                                  void *stream) {
+  if (!initialized()) {
+    set_error("fap_hip_init has not been called");
+    return FAP_ERR_STATE;
+  }
+  FAP_REQUIRE(s >= 0 && ld >= s, "fap_hip_floyd_path: bad sizes s=%d ld=%lld", s, (long long)ld);
+  if (s == 0) return FAP_OK;
+  FAP_REQUIRE(d_mat && d_path, "fap_hip_floyd_path: null matrix");
+  FAP_REQUIRE(s <= kMaxS, "fap_hip_floyd_path: s=%d above the limit %d", s, kMaxS);
+  hipStream_t st = resolve(stream);
+  const int S = (int)align_up(s, kT), nb = S / kT;
+  const int64_t tot = (int64_t)S * S;
+  FAP_TRY(g_ws_seam_fw.ensure(tot * 8));
+  FAP_TRY(g_ws_seam_snap.ensure((int64_t)(3 + 3 * nb) * kTT * 4));
+  FAP_TRY(g_ws_seam_meta.ensure(512));
+  int *dd = static_cast<int *>(g_ws_seam_fw.ptr), *dp = dd + tot;
+  char *meta = static_cast<char *>(g_ws_seam_meta.ptr);
+  PathPart *d_pp = reinterpret_cast<PathPart *>(meta);
+  int *d_fwkey = reinterpret_cast<int *>(meta + 256);  // 0: compare-and-select form
+  const PathPart h{dd, dp, static_cast<int *>(g_ws_seam_snap.ptr), S, nb, s, 0, 0, 1, 0, 0};
+  FAP_CHECK_HIP(hipMemcpyAsync(d_pp, &h, sizeof(h), hipMemcpyHostToDevice, st));
+  FAP_TRY(fap_hip_fill_i32(d_fwkey, 0, 1, st));
+  hipLaunchKernelGGL(pad_in_path_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                     dd, dp, S, d_mat, d_path, ld, s);
+  FAP_CHECK_LAUNCH();
+  for (int r = 0; r < nb; ++r) {
+    hipLaunchKernelGGL(fwp_phase1_kernel, dim3(1), dim3(256), 0, st, d_pp, d_fwkey, r);
+    FAP_CHECK_LAUNCH();
+    if (nb == 1) continue;
+    hipLaunchKernelGGL(fwp_phase2_kernel, dim3(2 * (nb - 1), 1), dim3(256), 0, st, d_pp, d_fwkey,
+                       r);
+    FAP_CHECK_LAUNCH();
+    hipLaunchKernelGGL(fwp_phase3_kernel, dim3((nb - 1) * (nb - 1), 1), dim3(256), 0, st, d_pp,
+                       d_fwkey, r);
+    FAP_CHECK_LAUNCH();
+  }
+  const int64_t out = (int64_t)s * s;
+  hipLaunchKernelGGL(pad_out_path_kernel, dim3((unsigned)((out + 255) / 256)), dim3(256), 0, st,
+                     d_mat, d_path, ld, s, dd, dp, S);
+  FAP_CHECK_LAUNCH();
+  // The workspaces are shared between calls: finish before they can be reused.
+  FAP_CHECK_HIP(hipStreamSynchronize(st));
+  return FAP_OK;
+}
+
+extern "C" int fap_hip_minplus_path(const int32_t *d_A, int64_t lda, const int32_t *d_B,
+                                    int64_t ldb, const int32_t *d_Bpath, int64_t ldbp,
+                                    int32_t *d_C, int64_t ldc, int32_t *d_Cpath, int64_t ldcp,
+                                    int32_t M, int32_t N, int32_t K, void *stream) {
+  if (!initialized()) {
+    set_error("fap_hip_init has not been called");
+    return FAP_ERR_STATE;
+  }
+  FAP_REQUIRE(M >= 0 && N >= 0 && K >= 0, "fap_hip_minplus_path: negative size");
+  if (M == 0 || N == 0) return FAP_OK;
+  FAP_REQUIRE(d_C && d_Cpath && ldc >= N && ldcp >= N, "fap_hip_minplus_path: bad C");
+  FAP_REQUIRE(K == 0 || (d_A && d_B && d_Bpath && lda >= K && ldb >= N && ldbp >= N),
+              "fap_hip_minplus_path: bad A/B");
+  const int64_t gy = (M + kSpT - 1) / kSpT, gx = (N + kSpT - 1) / kSpT;
+  FAP_REQUIRE(gy <= 65535 && gx < (1LL << 31), "fap_hip_minplus_path: product too large");
+  hipStream_t st = resolve(stream);
+  hipLaunchKernelGGL(minplus_path_seam_kernel, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st,
+                     d_A, lda, d_B, ldb, d_Bpath, ldbp, d_C, ldc, d_Cpath, ldcp, M, N, K);
+  FAP_CHECK_LAUNCH();
+  return FAP_OK;
+}
+
+extern "C" int fap_hip_sssp_batch_path(const int32_t *d_row_offset, const int32_t *d_col,
+                                       const int32_t *d_w, const int32_t *d_rrow_offset,
+                                       const int32_t *d_rcol, const int32_t *d_rw, int32_t n,
+                                       int64_t m, const int32_t *d_src, int32_t nsrc,
+                                       int32_t *d_out, int32_t *d_path, int64_t ld_out,
+                                       void *stream) {
+  if (!initialized()) {
+    set_error("fap_hip_init has not been called");
+    return FAP_ERR_STATE;
+  }
+  FAP_REQUIRE(n >= 0 && m >= 0 && nsrc >= 0 && ld_out >= n,
+              "fap_hip_sssp_batch_path: bad sizes n=%d m=%lld nsrc=%d ld=%lld", n, (long long)m,
+              nsrc, (long long)ld_out);
+  if (nsrc == 0 || n == 0) return FAP_OK;
+  FAP_REQUIRE(d_path && d_rrow_offset && (m == 0 || d_rcol) && (d_rw != nullptr) == (d_w != nullptr),
+              "fap_hip_sssp_batch_path: null pointer (reverse CSR, or d_rw without d_w)");
+  hipStream_t st = resolve(stream);
+  FAP_TRY(fap_hip_sssp_batch(d_row_offset, d_col, d_w, n, m, d_src, nsrc, d_out, ld_out, stream));
+  FAP_TRY(g_ws_seam_meta.ensure(512 + 4LL * kMaxSlots * kSlotStride));
+  int *stat = reinterpret_cast<int *>(static_cast<char *>(g_ws_seam_meta.ptr) + 512);
+  for (int k0 = 0; k0 < nsrc; k0 += 65535 * kPredRows) {
+    const int nk = std::min(65535 * kPredRows, nsrc - k0);
+    hipLaunchKernelGGL(pred_rows_kernel, dim3((n + 255) / 256, (nk + kPredRows - 1) / kPredRows),
+                       dim3(256), 0, st, d_rrow_offset, d_rcol, d_rw, n, d_src,
+                       (const int64_t *)nullptr, ld_out, k0, nk, d_out, d_path, stat);
+    FAP_CHECK_LAUNCH();
+  }
+  FAP_CHECK_HIP(hipStreamSynchronize(st));
+  return FAP_OK;
 }

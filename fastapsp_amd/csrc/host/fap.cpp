@@ -8,6 +8,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstring>
 #include <sstream>
 
 #include "graph.h"
@@ -228,13 +230,21 @@ void fapGraph::setMode(int32_t m) {
   mode = m;
 }
 
-void fapGraph::add_stage_times() {
+void fapGraph::add_stage_times(int32_t parts) {
   fap_stage_times t;
   check(fap_hip_last_stage_times(&t), "stage times");
   time_sssp += t.sssp_ms * 1e-3;
   time_floyd += t.floyd_ms * 1e-3;
   time_min_plus += t.minplus_ms * 1e-3;
   time_data_transformation += (t.build_ms + t.decode_ms) * 1e-3;
+  // Workspaces only grow during a call, so the memory in use after it is what every
+  // stage of its parts had allocated.
+  int64_t in_use = 0, high = 0, budget = 0;
+  check(fap_hip_mem_stats(&in_use, &high, &budget), "memory statistics");
+  const double gb = (double)in_use / (1024.0 * 1024.0 * 1024.0);
+  mem_cons_sssp += parts * gb;
+  mem_cons_floyd += parts * gb;
+  mem_cons_min_plus += parts * gb;
 }
 
 // Reverse CSR (in-arcs of every vertex, with weights) for the predecessor rule of
@@ -264,6 +274,12 @@ void fapGraph::upload_reverse() {
 void fapGraph::run_path(const fap_part_desc *parts, int32_t nparts, int32_t *d_out,
                         int32_t *d_p) {
   require_int("predecessor matrices");
+  // A zero-weight arc lets the reference's Dijkstra settle equal distances in heap
+  // order (fap/kernel/batch_sssp.h:79-114), which the predecessor rule cannot follow.
+  for (float w : weight)
+    if (w == 0.0f)
+      throw FapError(FAP_ERR_INVALID,
+                     "predecessor matrices need positive weights (zero-weight arc present)");
   upload_reverse();
   fap_graph_dev g{num_vertexs, num_edges, d_row_offset, d_col, d_w,
                   d_graph_id,  d_st2ed,   d_ed2st,      d_comp, d_wf};
@@ -271,7 +287,7 @@ void fapGraph::run_path(const fap_part_desc *parts, int32_t nparts, int32_t *d_o
                                  nullptr),
         "solve with paths");
   check(fap_hip_stream_sync(nullptr), "solve with paths");
-  add_stage_times();
+  add_stage_times(nparts);
 }
 
 int32_t fapGraph::solve(bool path_needed) {
@@ -305,7 +321,7 @@ int32_t fapGraph::solve(bool path_needed) {
                               nullptr),
           "solve");
   check(fap_hip_stream_sync(nullptr), "solve");
-  add_stage_times();
+  add_stage_times((int32_t)parts.size());
   return 0;
 }
 
@@ -337,7 +353,7 @@ int32_t fapGraph::solveSubGraph(int32_t id, bool path_needed) {
   else
     check(fap_hip_solve_parts(&g, &part, 1, d_sub_dist, ld_dist, mode, nullptr), "solveSubGraph");
   check(fap_hip_stream_sync(nullptr), "solveSubGraph");
-  add_stage_times();
+  add_stage_times(1);
   current_subgraph_id = id;
   sub_rows = C_BlockVer_num[id];
   return 0;
@@ -418,6 +434,7 @@ std::vector<int32_t> fapGraph::getAllPath() {
 int32_t fapGraph::getCurrentSubGraphId() { return current_subgraph_id; }
 
 std::vector<int32_t> fapGraph::getSubGraphIndex(int32_t id) {
+  if (id == 0) return {};  // C_BlockVer_num[0] == 0 (fap/fap.cpp:560-566)
   if (id < 1 || id > K) throw FapError(FAP_ERR_INVALID, "sub graph id out of range");
   const int off = C_BlockVer_offset[id], sz = C_BlockVer_num[id];
   return std::vector<int32_t>(st2ed.begin() + off, st2ed.begin() + off + sz);
@@ -453,6 +470,8 @@ void fapGraph::getSubGraphDistanceRowsF32(int32_t id, int64_t row0, int64_t rows
 }
 
 std::vector<float> fapGraph::getSubGraphDistance(int32_t id) {
+  // Before any solveSubGraph the reference's current id is 0 and its block empty.
+  if (id == 0 && current_subgraph_id == 0) return {};
   if (float_weights) {
     if (id != current_subgraph_id || !d_sub_dist)
       throw FapError(FAP_ERR_STATE, "sub graph was not the last one solved");
@@ -473,6 +492,7 @@ void fapGraph::getSubGraphPathRowsI32(int32_t id, int64_t row0, int64_t rows, in
 }
 
 std::vector<int32_t> fapGraph::getSubGraphPath(int32_t id) {
+  if (id == 0 && current_subgraph_id == 0) return {};
   std::vector<int32_t> out((size_t)(std::max<int64_t>(sub_rows, 0) * num_vertexs));
   getSubGraphPathRowsI32(id, 0, sub_rows, out.data());
   return out;
@@ -546,6 +566,77 @@ int fapGraph::getBoundaryNum() {
   int s = 0;
   for (size_t i = 0; i < C_BlockBdy_num.size(); ++i) s += C_BlockBdy_num[i];
   return s;
+}
+
+double fapGraph::getMemConsSSSP() { return K > 0 ? mem_cons_sssp / K : 0.0; }
+double fapGraph::getMemConsFloyd() { return K > 0 ? mem_cons_floyd / K : 0.0; }
+double fapGraph::getMemConsMinPlus() { return K > 0 ? mem_cons_min_plus / K : 0.0; }
+
+float fapGraph::getDistanceP2P() {
+  throw FapError(FAP_ERR_STATE, "getDistanceP2P(): no vertex pair; use getDistanceP2P(u, v)");
+}
+
+std::vector<int32_t> fapGraph::getPathP2P() {
+  throw FapError(FAP_ERR_STATE, "getPathP2P(): no vertex pair; use getPathP2P(u, v)");
+}
+
+float fapGraph::getDistanceP2P(int32_t u, int32_t v) {
+  if (u < 0 || u >= num_vertexs || v < 0 || v >= num_vertexs)
+    throw FapError(FAP_ERR_INVALID, "vertex out of range");
+  float d = 0;
+  if (d_dist) {
+    check(fap_hip_memcpy_d2h(&d, d_dist + (int64_t)ed2st[u] * ld_dist + v, 4, nullptr),
+          "device to host copy");
+  } else if (d_sub_dist && current_subgraph_id > 0 && graph_id[u] == current_subgraph_id) {
+    const int64_t row = ed2st[u] - C_BlockVer_offset[current_subgraph_id];
+    check(fap_hip_memcpy_d2h(&d, d_sub_dist + row * ld_dist + v, 4, nullptr),
+          "device to host copy");
+  } else {
+    throw FapError(FAP_ERR_STATE, "no result holds the row of this vertex");
+  }
+  if (float_weights) return d;
+  int32_t i;
+  std::memcpy(&i, &d, 4);
+  return to_float(i);
+}
+
+std::vector<int32_t> fapGraph::getPathP2P(int32_t u, int32_t v) {
+  if (u < 0 || u >= num_vertexs || v < 0 || v >= num_vertexs)
+    throw FapError(FAP_ERR_INVALID, "vertex out of range");
+  std::vector<int32_t> pred(num_vertexs);
+  if (have_path) {
+    getPathRowsI32(ed2st[u], 1, pred.data());
+  } else if (have_sub_path && graph_id[u] == current_subgraph_id) {
+    getSubGraphPathRowsI32(current_subgraph_id, ed2st[u] - C_BlockVer_offset[current_subgraph_id],
+                           1, pred.data());
+  } else {
+    throw FapError(FAP_ERR_STATE, "no predecessor matrix holds the row of this vertex");
+  }
+  std::vector<int32_t> rev;
+  if (pred[v] < 0) return rev;
+  for (int32_t x = v; (int32_t)rev.size() <= num_vertexs; x = pred[x]) {
+    rev.push_back(x);
+    if (x == u) break;
+    if (pred[x] < 0) throw FapError(FAP_ERR_STATE, "broken predecessor chain");
+  }
+  if (rev.back() != u) throw FapError(FAP_ERR_STATE, "predecessor chain does not reach u");
+  return std::vector<int32_t>(rev.rbegin(), rev.rend());
+}
+
+void fapGraph::printResalts() {
+  if (!d_dist) throw FapError(FAP_ERR_STATE, "no solve() result");
+  const int64_t n = num_vertexs;
+  const int64_t chunk = std::max<int64_t>(1, (16LL << 20) / std::max<int64_t>(n, 1));
+  std::vector<float> buf;
+  for (int64_t r = 0; r < n; r += chunk) {
+    const int64_t rows = std::min(chunk, n - r);
+    buf.resize((size_t)(rows * n));
+    getDistanceRowsF32(r, rows, buf.data());
+    for (int64_t i = 0; i < rows; ++i)
+      for (int64_t j = 0; j < n; ++j)
+        std::printf("Distances from node %d:  to node %d:  to node %d: %.2f \n", (int)(r + i),
+                    (int)j, (int)j, buf[(size_t)(i * n + j)]);
+  }
 }
 
 int fapGraph::getInnerNum() {

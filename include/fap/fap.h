@@ -13,10 +13,21 @@
 //  * solveSubGraph recomputes on every call (no process-wide "already processed"
 //    skip, fap/fap.cpp:522-528);
 //  * getDistanceFromOnePoint uses the sorted position of the vertex (the
-//    reference indexes C_BlockVer_offset by vertex id, fap/fap.cpp:584-585).
+//    reference indexes C_BlockVer_offset by vertex id, fap/fap.cpp:584-585);
+//  * check_result is deterministic: it checks whole rows of evenly spaced sources
+//    against an FP64 Dijkstra, and the predecessors when a path matrix is given
+//    (the reference samples 20 sources x 10 columns with a time seed,
+//    fap/utils/checkResult.h:38-94);
+//  * getMemCons* report the library's device memory in use (GB) averaged over the
+//    parts solved, from its own allocation accounting (the reference sums
+//    cudaMemGetInfo readings per part, fap/fap.cpp:242-243,298-299,358);
+//  * getDistanceP2P / getPathP2P have (u, v) forms; the argument-free forms the
+//    reference declares but leaves unimplemented (fap/fap.cpp:669-675) throw.
 // Predecessor (path) matrices: solve(true) / solveSubGraph(id, true) produce them
 // for integer-weight graphs, equal to the reference CPU path's bit for bit (same
-// tie-breaking); float-weight graphs throw FapError(FAP_ERR_STATE) for them.
+// tie-breaking); float-weight graphs throw FapError(FAP_ERR_STATE) for them, graphs
+// with a zero-weight arc FapError(FAP_ERR_INVALID) (the reference's choice among
+// equal distances then follows its heap order).
 #pragma once
 
 #include <chrono>
@@ -57,6 +68,7 @@ class fapGraph {
   double time_min_plus = 0;
   double time_data_transformation = 0;
   double edge_cut = 0;
+  double mem_cons_sssp = 0, mem_cons_floyd = 0, mem_cons_min_plus = 0;  // GB x parts
 
   int32_t current_subgraph_id = 0;
 
@@ -91,7 +103,7 @@ class fapGraph {
   void init_from_csr(bool directed, bool weighted, int32_t K, const std::string &partitioner);
   void upload();
   void release();
-  void add_stage_times();
+  void add_stage_times(int32_t parts);
   void copy_rows(const int32_t *base, int64_t nrows, int64_t row0, int64_t rows, void *out) const;
   void require_int(const char *what) const;
   void upload_reverse();
@@ -170,6 +182,14 @@ class fapGraph {
   std::vector<fap_part_desc> partList() const;
   void *resultBuffer(int64_t *ld);
 
+  // fap/fap.h:136-138.  d(u, v) and the vertices of the chosen shortest path from u
+  // to v (u first, v last; empty when v is unreachable), from the solve() result
+  // (predecessors: solve(true)) or, when u lies in it, the last solved sub graph.
+  float getDistanceP2P(int32_t u, int32_t v);
+  std::vector<int32_t> getPathP2P(int32_t u, int32_t v);
+  // The reference's argument-free forms name no pair: they throw FapError(FAP_ERR_STATE).
+  float getDistanceP2P();
+  std::vector<int32_t> getPathP2P();
   double getTimeSSSP();
   double getTimeFloyd();
   double getTimeMinPlus();
@@ -177,7 +197,27 @@ class fapGraph {
   double getTimeData();
   int getBoundaryNum();
   int getInnerNum();
+  // fap/fap.h:143-145: device memory (GB) in use while the parts were solved, averaged
+  // over K as the reference does.
+  double getMemConsSSSP();
+  double getMemConsFloyd();
+  double getMemConsMinPlus();
   double getEdgeCut();
+
+  // fap/fap.h:151: every entry of the solve() result, one line each, in the
+  // reference's format (sorted row order, MAXVALUE for unreachable).
+  void printResalts();
+
+  // fap/fap.h:155-159: verifies a block of distance rows (row i = source source[i],
+  // `vertexs` entries each) and, when path is not NULL, its predecessors, against an
+  // FP64 Dijkstra over the given CSR.  Checks min(source_num, 20) evenly spaced rows
+  // in full.  Distances must agree within max(1e-5, 1e-6 |d|); both above MAXVALUE/10
+  // counts as unreachable.  A predecessor u of v must be an in-neighbour with
+  // d(u) + w(u, v) = d(v) (same tolerance), the source itself on the diagonal and -1
+  // where v is unreachable.  Prints the first mismatch; false when there is one or
+  // when there is no row to check.  graph_id (may be NULL) only labels the message.
+  bool check_result(float *dist, int *path, int *source, int source_num, int vertexs,
+                    int *adj_size, int *row_offset, int *col_val, float *weight, int *graph_id);
 
   // Layout accessors (sorted order): C_BlockVer_num / offset, C_BlockBdy_num.
   const std::vector<int> &blockNum() const { return C_BlockVer_num; }

@@ -143,6 +143,48 @@ int fap_hip_floyd_noalloc(int32_t *d_mat, int32_t s, int64_t ld, void *stream);
 int fap_hip_minplus(const int32_t *d_A, int64_t lda, const int32_t *d_B, int64_t ldb,
                     int32_t *d_C, int64_t ldc, int32_t M, int32_t N, int32_t K, void *stream);
 
+/* Seam entries with predecessor ("path") matrices, as the reference's seam carries
+ * them.  Distances are int32 (>= 0, FAP_INF for unreachable); the results equal the
+ * reference CPU branch's distances and paths bit for bit, tie-breaking included.
+ *
+ * Floyd-Warshall closure with paths, in place: where d[i][k] + d[k][j] < d[i][j]
+ * for pivots k in ascending order, d[i][j] takes the sum and path[i][j] = path[k][j]
+ * (the k-i-j loop with strict '>').  s <= 8192.  Replaces floyd_path
+ * (fap/kernel/floyd.h:162-170) -> floyd_CPU_path (:38-49). */
+int fap_hip_floyd_path(int32_t *d_mat, int32_t *d_path, int32_t s, int64_t ld, void *stream);
+
+/* Min-plus with paths, overwrite semantics: C[i][j] = min_k A[i][k] + B[k][j] over
+ * finite sums (FAP_INF if none) and Cpath[i][j] = Bpath[k][j] for the smallest k
+ * reaching it (-1 if none).  Replaces min_plus_path_advanced[_gpu]
+ * (fap/kernel/minplus.h:81-105) -> min_plus_path_CPU_advanced (:51-64). */
+int fap_hip_minplus_path(const int32_t *d_A, int64_t lda, const int32_t *d_B, int64_t ldb,
+                         const int32_t *d_Bpath, int64_t ldbp, int32_t *d_C, int64_t ldc,
+                         int32_t *d_Cpath, int64_t ldcp, int32_t M, int32_t N, int32_t K,
+                         void *stream);
+
+/* Boundary SSSP with predecessors: the rows of fap_hip_sssp_batch, and
+ * d_path[i * ld_out + v] = the vertex before v on the path Dijkstra with a
+ * (distance, vertex) min-heap and strict '>' relaxation picks (the first-settled
+ * in-neighbour u with d(u) + w(u, v) = d(v)), the source on its own entry, -1 when
+ * unreachable.  d_rrow_offset / d_rcol / d_rw: the reverse CSR (in-arcs of every
+ * vertex; d_rw NULL exactly when d_w is NULL).  The predecessors are the reference's
+ * for positive weights; with zero-weight arcs they are valid shortest-path
+ * predecessors but Dijkstra's choice among equal distances depends on its heap
+ * order (as for fap_hip_solve_parts_path).  Replaces handle_boundry_path
+ * (fap/kernel/batch_sssp.h:280-303) -> dijkstra_path (:79-114). */
+int fap_hip_sssp_batch_path(const int32_t *d_row_offset, const int32_t *d_col,
+                            const int32_t *d_w, const int32_t *d_rrow_offset,
+                            const int32_t *d_rcol, const int32_t *d_rw, int32_t n, int64_t m,
+                            const int32_t *d_src, int32_t nsrc, int32_t *d_out, int32_t *d_path,
+                            int64_t ld_out, void *stream);
+
+/* Distance conversion at the seam's edges: float (MAXVALUE = 1e8 for unreachable,
+ * fap/utils/parameter.h:26) -> int32 maps v >= 1e8 to FAP_INF and truncates others
+ * (exact for the integer-valued distances of integer weights); int32 -> float maps
+ * FAP_INF to 1e8. */
+int fap_hip_f32_to_i32(const float *d_src, int32_t *d_dst, int64_t count, void *stream);
+int fap_hip_i32_to_f32(const int32_t *d_src, float *d_dst, int64_t count, void *stream);
+
 /* Dense s x s sub-matrix of one part (row stride ld_sub >= s):
  *   rows i <  bdy : d_dist_rows[i * ld_dist + st2ed[start + j]]  (the SSSP rows)
  *   rows i >= bdy : FAP_INF, then each intra-part arc of st2ed[start+i] in CSR
@@ -248,7 +290,11 @@ int fap_hip_last_stage_times(fap_stage_times *t);
  * Floyd-Warshall, first k for the min-plus).  Integer weights only (g->d_wf NULL).
  * d_rrow_offset / d_rcol / d_rw: the reverse CSR (in-arcs of every vertex; d_rw NULL
  * exactly when g->d_w is NULL).  d_path has the layout of d_out.  Weights must be
- * non-negative.  One batched form.  Per part, the Floyd-Warshall panels and the
+ * positive: with a zero-weight arc the vertex Dijkstra settles first among equal
+ * distances depends on when it entered the heap, a tie-break no rule on the
+ * distance rows reproduces (the distances are still exact, the predecessors are
+ * then valid shortest-path predecessors, not necessarily the reference's).  One
+ * batched form.  Per part, the Floyd-Warshall panels and the
  * min-plus run in a keyed form (distance and path source in one 32-bit key, one
  * unsigned minimum per candidate) when the part's values stay below the key limits,
  * in a compare-and-select form otherwise; the results are the same.  The call waits

@@ -222,3 +222,37 @@ def compare_parts(first, last, csr, lay, got, row_base):
             got.ctypes.data_as(C.POINTER(C.c_int32)), C.c_int64(got.strides[0] // 4),
             C.c_int64(row_base), C.byref(fb))
     return int(bad), int(fb.value)
+
+
+def submat_build(dist_rows, path_rows, csr, lay, pid):
+    """MysubMatBuild_path for part pid from its boundary rows (float bdy x n with
+    MAXVALUE, int paths): the s x s sub-matrix and its paths, pre-filled MAXVALUE / -1."""
+    ro, cv = i32(csr["row_offset"]), i32(csr["col"])
+    w = f32(csr["weight"])
+    n = len(ro) - 1
+    adj = i32(np.diff(ro))
+    s, bdy, start = int(lay["num"][pid]), int(lay["bdy"][pid]), int(lay["offset"][pid])
+    sub = np.full((s, s), MAXVALUE, np.float32)
+    sp = np.full((s, s), -1, np.int32)
+    dr = f32(dist_rows).reshape(-1) if bdy else np.zeros(1, np.float32)
+    pr = i32(path_rows).reshape(-1) if bdy else np.zeros(1, np.int32)
+    lib().orc_submat_build(_p(sub, C.c_float), _p(sp, C.c_int32), _p(dr, C.c_float),
+                           _p(pr, C.c_int32), _p(i32(lay["graph_id"]), C.c_int32), start, s, bdy,
+                           n, _p(i32(lay["st2ed"]), C.c_int32), _p(i32(lay["ed2st"]), C.c_int32),
+                           _p(adj, C.c_int32), _p(ro, C.c_int32), _p(cv, C.c_int32),
+                           _p(w, C.c_float))
+    return sub, sp
+
+
+def submat_decode(sub, sub_path, st2ed, start, dist_block, path_block):
+    """MysubMatDecode_path: rows x s of (sub, sub_path) scattered into the columns
+    st2ed[start + j] of the rows x n blocks (modified in place, returned)."""
+    sub, sub_path = f32(sub), i32(sub_path)
+    rows, s = sub.shape
+    n = dist_block.shape[1]
+    assert dist_block.dtype == np.float32 and path_block.dtype == np.int32
+    assert dist_block.flags.c_contiguous and path_block.flags.c_contiguous
+    lib().orc_submat_decode(_p(sub, C.c_float), _p(sub_path, C.c_int32), _p(dist_block, C.c_float),
+                            _p(path_block, C.c_int32), start, rows, s, n,
+                            _p(i32(st2ed), C.c_int32))
+    return dist_block, path_block

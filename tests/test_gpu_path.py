@@ -158,6 +158,23 @@ def test_paths_refused_for_float_weights():
         g.solve(True)
 
 
+def test_paths_refused_for_zero_weights():
+    """A zero-weight arc makes the reference's predecessor choice follow its heap order:
+    path solves refuse such graphs; distance solves still run."""
+    base = F.FapGraph.road_grid(20, 20, 0.1, 0.02, 1, 3)
+    c = base.csr()
+    w = np.ones(len(c["col"]), np.float32)
+    w[5] = 0.0
+    g = F.FapGraph.from_csr(c["row_offset"], c["col"], w, True, 3)
+    g.preCondition()
+    with pytest.raises(L.FapError):
+        g.solve(True)
+    g.solve()
+    lay, csr = g.layout(), g.csr()
+    want = O.exact_rows(csr["row_offset"], csr["col"], csr["weight"], lay["st2ed"])
+    assert np.array_equal(g.getAllDistanceI32(), want)
+
+
 def _grid(W, H, w=None):
     """W x H 4-neighbour grid, vertex y * W + x, unit weights unless `w` (per arc)."""
     out = [[] for _ in range(W * H)]
