@@ -148,12 +148,49 @@ def minplus_bench(N, reps=3):
     ms = float(np.median(times))
     ops = 2.0 * N ** 3
     inst = 1.5 * N ** 3 / (ms * 1e-3) / 1e12
-    out = {"minplus_bench": {"N": N, "ms": ms, "ops": ops, "TOPs": ops / (ms * 1e-3) / 1e12,
-                             "valu_instr_T_per_s": inst, "valu_frac": inst / VALU_PEAK_TOPS,
-                             "sampled_entries_exact": ok}}
-    print(json.dumps(out), flush=True)
-    if not ok:
-        sys.exit(1)
+    return {"N": N, "ms": ms, "ops": ops, "TOPs": ops / (ms * 1e-3) / 1e12,
+            "valu_instr_T_per_s": inst, "valu_frac": inst / VALU_PEAK_TOPS,
+            "sampled_entries_exact": ok,
+            "workload": f"dense {N}x{N}x{N} int32 min-plus, values uniform in [0, 2^20]"}
+
+
+def knn_subgraph_bench(args):
+    """Directed integer-weighted kNN workload: 262,144 uniform points, arcs to the 3
+    nearest neighbours (786,432 arcs, weights uniform in [1, 1000]), K = 256;
+    solveSubGraph for every part in turn (each call: boundary SSSP, build,
+    Floyd-Warshall, min-plus, decode; the s_p x n block stays in HBM)."""
+    t0 = time.perf_counter()
+    g = F.FapGraph.knn(262144, 3, 1000, 2, 256)
+    g.preCondition()
+    lay = g.layout()
+    n = g.num_vertexs
+    num = lay["num"][1:].astype(np.int64)
+    bdy = lay["bdy"][1:].astype(np.int64)
+    log(f"knn graph n={n} m={g.num_edges} K={g.K} built in {time.perf_counter() - t0:.1f}s; "
+        f"boundary per part mean {bdy.mean():.1f} max {bdy.max()}, part size max {num.max()}")
+    g.solveSubGraph(1)  # warm-up (allocations)
+    L.check(L.load().fap_hip_device_sync(), "sync")
+    sums = {k: 0.0 for k in ("sssp_ms", "build_ms", "floyd_ms", "minplus_ms", "decode_ms",
+                             "total_ms")}
+    ts = time.perf_counter()
+    for p in range(1, g.K + 1):
+        g.solveSubGraph(p)
+        st = stage_times()
+        for k in sums:
+            sums[k] += st[k]
+    L.check(L.load().fap_hip_device_sync(), "sync")
+    wall = time.perf_counter() - ts
+    entries = float(num.sum()) * n
+    steps = float(((num - bdy) * bdy).sum()) * n + float((num.astype(np.float64) ** 3).sum())
+    log(f"knn solveSubGraph x {g.K}: {wall * 1e3:.1f} ms, {entries / wall:.3e} entries/s; "
+        f"stages (ms, all parts): {', '.join(f'{k} {v:.1f}' for k, v in sums.items())}")
+    g.close()
+    return {"workload": "kNN n=262144 (3 nearest neighbours, directed, int weights U[1,1000]), "
+                        "K=256, solveSubGraph over every part in turn",
+            "n": n, "K": 256, "parts": int(len(num)), "entries": entries, "wall_ms": wall * 1e3,
+            "entries_per_s": entries / wall, "ms_per_part": wall * 1e3 / len(num),
+            "boundary_mean": float(bdy.mean()), "boundary_max": int(bdy.max()),
+            "minplus_fw_steps": steps, "stages_ms_all_parts": sums}
 
 
 def batch_bench(args):
@@ -164,7 +201,11 @@ def batch_bench(args):
     replay of the captured batch (one persistent launch per distance type)."""
     from fastapsp_amd import autotune as AT
     t0 = time.perf_counter()
-    table = AT.tune(k_candidates=(2, 4, 8, 16, 32, 64), per_cu_candidates=(1, 2), max_copies=32)
+    if getattr(args, "tune_quick", False):
+        table = AT.tune(k_candidates=(8, 16), per_cu_candidates=(1, 2), max_copies=16)
+    else:
+        table = AT.tune(k_candidates=(2, 4, 8, 16, 32, 64), per_cu_candidates=(1, 2),
+                        max_copies=32)
     log(f"autotune {time.perf_counter() - t0:.1f}s: K per class {table['K']}, "
         f"per_cu {table['per_cu']}")
     rng = np.random.default_rng(2026)
@@ -230,7 +271,7 @@ def batch_bench(args):
            "autotune": {"K": table["K"], "per_cu": table["per_cu"],
                         "ms_per_graph": table["ms_per_graph"]},
            "plain_form": plain, "cpu_baseline": cpu}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def serve_bench(args):
@@ -298,11 +339,10 @@ def serve_bench(args):
             f"{st['batches']} batches, p50 {st['p50_ms']:.1f} ms, p99 {st['p99_ms']:.1f} ms, "
             f"{res['entries_per_s']:.3e} entries/s, peak {res['peak_reserved_GB']:.1f} GB")
     srv.close()
-    print(json.dumps({"serve": results, "pool": len(pool),
-                      "workload": "road-like and kNN graphs, n log-uniform in [1k, 64k], int "
-                                  "[1,1000] or float32 [0.5,10) weights, 50% full APSP / 50% "
-                                  "sub-graph requests, Poisson arrivals, 200 GB budget"}),
-          flush=True)
+    return {"serve": results, "pool": len(pool),
+            "workload": "road-like and kNN graphs, n log-uniform in [1k, 64k], int "
+                        "[1,1000] or float32 [0.5,10) weights, 50% full APSP / 50% "
+                        "sub-graph requests, Poisson arrivals, 200 GB budget"}
 
 
 def main():
@@ -318,6 +358,8 @@ def main():
     ap.add_argument("--paths", action="store_true",
                     help="also time solve(True): distances and predecessor matrices")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-legs", action="store_true",
+                    help="headline workload only (no kNN / min-plus / batch / serving legs)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--skip-forms", action="store_true",
                     help="skip the overlap and int32-FW comparison solves (for profiling)")
@@ -332,22 +374,24 @@ def main():
         probe(args.probe)
         return
     if args.minplus > 0:
-        minplus_bench(args.minplus)
-        return
-    if args.config == "serve":
-        serve_bench(args)
-        return
-    if args.config == "batch512":
-        if args.gpus != 1:
-            print("bench.py measures one GPU only", file=sys.stderr)
-            sys.exit(2)
-        batch_bench(args)
+        res = minplus_bench(args.minplus)
+        print(json.dumps({"minplus_bench": res}), flush=True)
+        if not res["sampled_entries_exact"]:
+            sys.exit(1)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU sharding is "
               "a design note, not a run", file=sys.stderr)
         sys.exit(2)
-
+    if args.config == "serve":
+        print(json.dumps(serve_bench(args)), flush=True)
+        return
+    if args.config == "batch512":
+        print(json.dumps(batch_bench(args)), flush=True)
+        return
+    if args.config == "knn262k":
+        print(json.dumps(knn_subgraph_bench(args)), flush=True)
+        return
     t0 = time.perf_counter()
     g, workload = make_graph(args.config)
     log(f"graph n={g.num_vertexs} m={g.num_edges} K={g.K} built+partitioned in "
@@ -497,6 +541,31 @@ def main():
             f"stages {pst}")
         g.solve()  # leave the distance-only result resident
 
+    # The same-kernel comparison: the batched form with boundary SSSP rows and the
+    # Floyd-Warshall in-part distances runs exactly the kernels the plain form launches
+    # part after part (SSSP, build, 16-bit Floyd-Warshall + unpack, merge, decode).
+    same = None
+    if not args.skip_plain:
+        g.setMode(F.FAP_MODE_BATCHED | F.FAP_MODE_FLAG_SSSP | F.FAP_MODE_FLAG_FW)
+        g.solve()
+        L.check(L.load().fap_hip_device_sync(), "sync")
+        ts = time.perf_counter()
+        for _ in range(3):
+            g.solve()
+        L.check(L.load().fap_hip_device_sync(), "sync")
+        same_ms = (time.perf_counter() - ts) * 1e3 / 3
+        sst = stage_times()
+        same = {"ms_per_step": same_ms, "stages_ms": sst,
+                "form": "batched, boundary SSSP rows, Floyd-Warshall in-part (the plain form's "
+                        "kernels)"}
+        if plain is not None:
+            plain["batched_vs_plain_same_kernels"] = plain["ms_per_step"] / same_ms
+            plain["batched_vs_plain_algorithmic"] = plain["ms_per_step"] / ms_per_step
+            log(f"batched with the plain form's kernels: {same_ms:.2f} ms/step "
+                f"(plain {plain['batched_vs_plain_same_kernels']:.2f}x slower)")
+        g.setMode(modes[args.mode])
+        g.solve()
+
     cpu = None
     if not args.skip_cpu:
         cpu = cpu_baseline(g, args.cpu_budget)
@@ -516,6 +585,26 @@ def main():
            "roofline": roofline, "cpu_baseline": cpu, "plain_form": plain}
     if paths is not None:
         out["paths"] = paths
+    if same is not None:
+        out["batched_same_kernels_form"] = same
+    if args.config == "road1" and not args.skip_legs:
+        # The other workloads, measured in the same run (each leg after the headline
+        # graph's memory is released; a failing leg reports its error, not the run's).
+        g.close()
+        legs = {}
+        for name, fn in [("knn262k_k256_subgraph", lambda: knn_subgraph_bench(args)),
+                         ("minplus_8192", lambda: minplus_bench(8192)),
+                         ("batch512", lambda: batch_bench(argparse.Namespace(
+                             graphs=512, steps=3, warmup=1, skip_plain=True, skip_cpu=True,
+                             cpu_budget=0.0, tune_quick=True))),
+                         ("serve", lambda: serve_bench(argparse.Namespace(pool=24)))]:
+            t0 = time.perf_counter()
+            try:
+                legs[name] = fn()
+            except Exception as e:  # noqa: BLE001 -- reported in the JSON line
+                legs[name] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"leg {name}: {time.perf_counter() - t0:.1f} s")
+        out["other_configs"] = legs
     print(json.dumps(out), flush=True)
 
 
