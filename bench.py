@@ -202,12 +202,13 @@ def batch_bench(args):
     from fastapsp_amd import autotune as AT
     t0 = time.perf_counter()
     if getattr(args, "tune_quick", False):
-        table = AT.tune(k_candidates=(8, 16), per_cu_candidates=(1, 2), max_copies=16)
+        table = AT.tune(k_candidates=(8, 16), per_cu_candidates=(0,), src_rows_candidates=(16,),
+                        merge_rows_candidates=(16,), max_copies=16)
     else:
-        table = AT.tune(k_candidates=(2, 4, 8, 16, 32, 64), per_cu_candidates=(1, 2),
-                        max_copies=32)
+        table = AT.tune(max_copies=32)
     log(f"autotune {time.perf_counter() - t0:.1f}s: K per class {table['K']}, "
-        f"per_cu {table['per_cu']}")
+        f"per_cu {table['per_cu']}, source rows {table['src_rows']}, "
+        f"merge rows {table['merge_rows']}")
     rng = np.random.default_rng(2026)
     sizes = np.exp(rng.uniform(np.log(256), np.log(8192), args.graphs)).astype(int)
     t0 = time.perf_counter()
@@ -221,8 +222,8 @@ def batch_bench(args):
     entries = float(sum(n * n for n in ns))
     log(f"{len(graphs)} graphs built in {time.perf_counter() - t0:.1f}s: n in [{min(ns)}, "
         f"{max(ns)}], sum n^2 = {entries:.3e} ({entries * 4 / 1e9:.1f} GB of results)")
-    per_cu = AT.per_cu_for(ns, table)
-    b = F.Batch(graphs, use_graph=True, per_cu=per_cu)
+    settings = AT.settings_for(ns, table)
+    b = F.Batch(graphs, use_graph=True, **settings)
     for _ in range(args.warmup):
         b.run()
     L.check(L.load().fap_hip_device_sync(), "sync")
@@ -267,9 +268,9 @@ def batch_bench(args):
            "config": {"workload": f"{len(graphs)} road-like graphs, n log-uniform in [256, 8192], "
                                   "half float32-weighted, one persistent launch per type",
                       "graphs": len(graphs), "sum_n2": entries, "n_min": min(ns),
-                      "n_max": max(ns), "per_cu": per_cu},
-           "autotune": {"K": table["K"], "per_cu": table["per_cu"],
-                        "ms_per_graph": table["ms_per_graph"]},
+                      "n_max": max(ns), "settings": settings},
+           "autotune": {k: table[k] for k in ("K", "per_cu", "src_rows", "merge_rows",
+                                               "ms_per_graph")},
            "plain_form": plain, "cpu_baseline": cpu}
     return out
 

@@ -4,12 +4,16 @@ For each size class of graphs (vertex counts [256, 512), [512, 1024), ..., [4096
 it times candidate settings on a batch that fills the GPU -- copies of a road-like
 graph of the class's middle size, half of them with float weights -- and keeps the
 fastest:
-  * K, the number of parts per graph: it sets the Floyd-Warshall size s_p (work
-    ~ K * s_p^3), the boundary count (SSSP sources and merge depth) and the number of
-    independent work items the persistent kernel can spread over the CUs;
-  * the persistent workgroups per CU (occupancy against scratch-slot memory).
-The result is a small table (JSON-serialisable) that `k_for(n)` and `per_cu_for(n)`
-read; `load_or_tune(path)` caches it on disk.
+  * K, the number of parts per graph: it sets the Floyd-Warshall block size s_p (work
+    ~ K * s_p^3 and (sum of b_p)^3 for the boundary graph), the boundary count (the
+    merge depth) and the number of independent work units;
+  * the boundary-source batch width: boundary-graph rows per unit of the boundary-row
+    product (8, 16 or 32 sources sharing one pass over a part's in-part block);
+  * the merge row tile: inner rows per wave of the int32 merge (8 or 16);
+  * the persistent merge workgroups per CU (0 = as many as stay resident).
+K is chosen first with the default tiles, then each tile knob in turn (coordinate
+search).  The result is a small table (JSON-serialisable) that `k_for(n)` and
+`settings_for(ns)` read; `load_or_tune(path)` caches it on disk.
 """
 from __future__ import annotations
 
@@ -25,9 +29,12 @@ from .batch import Batch
 from .graph import FapGraph
 
 SIZE_CLASSES = [(256, 512), (512, 1024), (1024, 2048), (2048, 4096), (4096, 8193)]
-K_CANDIDATES = (1, 2, 4, 8, 16, 32)
-PER_CU_CANDIDATES = (1, 2, 4)
-MAX_PART = 4096  # largest part of the batch form
+K_CANDIDATES = (2, 4, 8, 16, 32)
+PER_CU_CANDIDATES = (0, 2)
+SRC_ROWS_CANDIDATES = (8, 16, 32)
+MERGE_ROWS_CANDIDATES = (8, 16)
+MAX_PART = 8192  # largest part of the batch form
+KNOBS = ("per_cu", "src_rows", "merge_rows")
 
 
 def symmetric_float_weights(ro, col, seed: int, directed: bool = False) -> np.ndarray:
@@ -62,8 +69,8 @@ def size_class(n: int) -> int:
     return len(SIZE_CLASSES) - 1
 
 
-def _time_batch(graphs, per_cu: int, reps: int = 2) -> float:
-    b = Batch(graphs, use_graph=False, per_cu=per_cu)
+def _time_batch(graphs, cfg: dict, reps: int = 2) -> float:
+    b = Batch(graphs, use_graph=False, **cfg)
     b.run()
     best = math.inf
     for _ in range(reps):
@@ -76,38 +83,54 @@ def _time_batch(graphs, per_cu: int, reps: int = 2) -> float:
 
 def tune(copies_work: int = 512, max_copies: int = 48, seed: int = 4242,
          k_candidates=K_CANDIDATES, per_cu_candidates=PER_CU_CANDIDATES,
+         src_rows_candidates=SRC_ROWS_CANDIDATES, merge_rows_candidates=MERGE_ROWS_CANDIDATES,
          verbose: bool = False) -> dict:
-    """Time every (K, workgroups-per-CU) candidate per size class; returns the table."""
-    table = {"classes": SIZE_CLASSES, "K": [], "per_cu": [], "ms_per_graph": [], "trials": []}
+    """Per size class: K with the default tiles, then each knob in turn; the table
+    keeps the fastest setting seen (every trial is recorded)."""
+    table = {"classes": SIZE_CLASSES, "K": [], "per_cu": [], "src_rows": [], "merge_rows": [],
+             "ms_per_graph": [], "trials": []}
+    cands = {"per_cu": tuple(per_cu_candidates), "src_rows": tuple(src_rows_candidates),
+             "merge_rows": tuple(merge_rows_candidates)}
     for ci, (lo, hi) in enumerate(SIZE_CLASSES):
         n_mid = int(math.sqrt(lo * min(hi, 8192)))
-        best = None
         trials = []
+        best = None  # (ms, K, cfg, graphs)
+
+        def trial(K, cfg, graphs, copies):
+            nonlocal best
+            ms = _time_batch(graphs, cfg) * 1e3 / copies
+            trials.append(dict(K=K, ms_per_graph=ms, **cfg))
+            if verbose:
+                print(f"class {ci} n~{n_mid} K={K} {cfg}: {ms:.3f} ms/graph", flush=True)
+            if best is None or ms < best[0]:
+                best = (ms, K, dict(cfg), graphs, copies)
+
+        default = {k: cands[k][0] if k == "per_cu" else (16 if 16 in cands[k] else cands[k][0])
+                   for k in KNOBS}
         for K in k_candidates:
             base = road_graph(n_mid, seed + ci, K)
             if K > base.num_vertexs // 16:
                 continue
             base.preCondition()
-            lay_num = base.layout()["num"][1:]
-            if lay_num.max() > MAX_PART:
+            if base.layout()["num"][1:].max() > MAX_PART:
                 continue
             copies = int(min(max_copies, max(4, copies_work // K)))
             graphs = [road_graph(n_mid, seed + ci, K, float_weights=(c % 2 == 1))
                       for c in range(copies)]
-            for per_cu in per_cu_candidates:
-                dt = _time_batch(graphs, per_cu)
-                ms = dt * 1e3 / copies
-                trials.append({"K": K, "per_cu": per_cu, "ms_per_graph": ms})
-                if verbose:
-                    print(f"class {ci} n~{n_mid} K={K} per_cu={per_cu}: {ms:.3f} ms/graph",
-                          flush=True)
-                if best is None or ms < best[2]:
-                    best = (K, per_cu, ms)
-            del graphs
-        table["K"].append(best[0])
-        table["per_cu"].append(best[1])
-        table["ms_per_graph"].append(best[2])
+            trial(K, default, graphs, copies)
+        _, K, cfg, graphs, copies = best
+        for knob in KNOBS:
+            for c in cands[knob]:
+                if c == best[2][knob]:
+                    continue
+                trial(K, dict(best[2], **{knob: c}), graphs, copies)
+        ms, K, cfg = best[0], best[1], best[2]
+        table["K"].append(K)
+        for knob in KNOBS:
+            table[knob].append(cfg[knob])
+        table["ms_per_graph"].append(ms)
         table["trials"].append(trials)
+        best = None
     return table
 
 
@@ -115,13 +138,22 @@ def k_for(n: int, table: dict) -> int:
     return int(table["K"][size_class(n)])
 
 
-def per_cu_for(ns, table: dict) -> int:
-    """Workgroups per CU for a mixed batch: the choice of the class holding the most
-    work (sum of n^2)."""
+def _dominant_class(ns) -> int:
     work = np.zeros(len(SIZE_CLASSES))
     for n in ns:
         work[size_class(n)] += float(n) * n
-    return int(table["per_cu"][int(np.argmax(work))])
+    return int(np.argmax(work))
+
+
+def settings_for(ns, table: dict) -> dict:
+    """Batch settings (per_cu, src_rows, merge_rows) for a mixed batch: those of the
+    size class holding the most work (sum of n^2)."""
+    c = _dominant_class(ns)
+    return {k: int(table[k][c]) for k in KNOBS if k in table}
+
+
+def per_cu_for(ns, table: dict) -> int:
+    return int(table["per_cu"][_dominant_class(ns)])
 
 
 def load_or_tune(path: str | None = None, **kw) -> dict:

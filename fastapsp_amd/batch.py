@@ -1,9 +1,10 @@
 """Many independent graphs solved together (the batched form for many graphs).
 
-`Batch(graphs)` prepares one work table over every part of every graph (after each
+`Batch(graphs)` prepares the work tables over every part of every graph (after each
 graph's preCondition) and, with `use_graph=True`, captures the launch sequence as a
-hipGraph; `run()` replays it: one persistent launch per distance type in which
-workgroups pull parts from a device counter.  Results land in each graph's solve()
+hipGraph; `run()` replays it: per distance type, each stage (build, Floyd-Warshall,
+boundary graph, boundary rows, merge) is one launch over all graphs, the merge a
+persistent grid pulling units from a device counter.  Results land in each graph's solve()
 buffer, so the graphs' own getters return them.  `solve_batch(graphs, mode)` is the
 one-shot form; FAP_MODE_PLAIN there solves graph after graph with per-part launches
 (the reference's structure) for comparison.
@@ -23,8 +24,16 @@ def _handles(graphs) -> C.Array:
     return arr
 
 
+_ROWS_CODE = {0: 0, 8: 1, 16: 2, 32: 3}
+
+
 class Batch:
-    def __init__(self, graphs: Sequence, use_graph: bool = True, per_cu: int = 0):
+    """per_cu: persistent merge workgroups per CU (0: as many as stay resident);
+    src_rows: boundary sources per unit of the boundary-row product (8, 16, 32; 0 =
+    16); merge_rows: inner rows per wave of the merge (8 or 16; 0 = 16, FP64 always 8)."""
+
+    def __init__(self, graphs: Sequence, use_graph: bool = True, per_cu: int = 0,
+                 src_rows: int = 0, merge_rows: int = 0):
         self.graphs = list(graphs)
         for g in self.graphs:
             if not getattr(g, "_preconditioned", False):
@@ -34,7 +43,9 @@ class Batch:
         L.check(L.load().fap_graph_batch_create(C.byref(h), _handles(self.graphs),
                                                 len(self.graphs),
                                                 (L.FAP_BATCH_GRAPH if use_graph else 0)
-                                                | ((int(per_cu) & 0xFF) << 8)),
+                                                | ((int(per_cu) & 0xFF) << 8)
+                                                | (_ROWS_CODE[int(src_rows)] << 16)
+                                                | (_ROWS_CODE[int(merge_rows)] << 18)),
                 "batch create")
         self._h = h.value
 

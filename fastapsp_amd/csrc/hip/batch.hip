@@ -1,66 +1,97 @@
-// Many independent graphs in one persistent launch fed by a device work queue.
+// Many independent graphs, solved stage by stage: every stage is one launch over the
+// parts (or boundary graphs) of all graphs of a batch, and the whole sequence is
+// captured once as a hipGraph and replayed.
 //
-// The work table holds one entry per part of every graph (the unit fapGraph::run
-// processes, fap/fap.cpp:195-503).  Per distance type two launches:
-//   1 boundary SSSP   one wave per boundary source of every part of every graph,
-//                     label-correcting relaxation with atomicMin on the row,
-//                     frontier queues in LDS (batch_sssp.h:232-252).  SSSP is
-//                     latency-bound (levels x memory round trips), so it runs as
-//                     one wide launch where thousands of waves hide each other's
-//                     latency, not inside the per-part workgroups;
-// then a persistent grid of 512-thread workgroups pulls parts from an atomic
-// counter, largest first (the host sorts the table by a cost model), and runs the
-// rest of the per-part pipeline inside the workgroup (eight waves):
-//   2 build           the s x s matrix in the workgroup's scratch slot
-//                     (util_Myalgorithm.h:44-84, later arc wins, diagonal 0);
-//   3 Floyd-Warshall  blocked, 64 x 64 tiles: the pivot closed in LDS by the whole
-//                     workgroup, then the panel and remaining tiles shared by the
-//                     waves, each tile a register-tiled min-plus product staged
-//                     through the wave's own LDS slab (floyd.h:38-49);
-//   4 merge           inner rows x n columns = FW(inner, boundary) (x) SSSP rows,
-//                     64 x 64 output tiles per wave, the part's own columns taken
-//                     from the FW matrix in the epilogue (minplus.h:51-64 and
-//                     util_Myalgorithm.h:236-247);
-//   5 boundary rows   the SSSP rows with their own columns from the FW matrix.
-// Parts never depend on each other, so the only synchronisation is the workgroup
-// barrier between stages; a workgroup leaves when the counter passes the table end,
-// which every workgroup reaches.  Templated on the distance type: int32 (integer
-// weights, FAP_INF) and double (float weights, float32 output with MAXVALUE), the
-// arithmetic of fp64.hip.
+// Each graph follows fapGraph::run (fap/fap.cpp:195-503) for all its parts.  When an
+// item's parts cover its graph (a full APSP) the boundary rows come from the graph's
+// boundary graph instead of one SSSP per boundary vertex (SSSP is latency-bound: tens
+// of thousands of label-correcting waves took 60 % of the batch time):
+//   1 build           every part's s x s matrix from its in-part arcs (CSR order, a
+//                     later arc wins, util_Myalgorithm.h:44-84), diagonal 0;
+//   2 Floyd-Warshall  all matrices, batched blocked FW (floyd.hip / fp64.hip): the
+//                     in-part distances d_q (floyd.h:38-49);
+//   3 boundary graph  per graph, its boundary vertices (the first b_q sorted positions
+//                     of every part, util_preprocess.h:66-71) with the in-part blocks
+//                     d_q[B_q][B_q] and the cut arcs (both ends are boundary vertices,
+//                     util_preprocess.h:32-58) as arcs;
+//   4 Floyd-Warshall  all boundary-graph matrices: D_B = the graph's distances between
+//                     boundary vertices (every path between them is a path of the
+//                     boundary graph);
+//   5 boundary rows   d(x, v) = min_{c in B_q} D_B(x, c) + d_q(c, v) for v in part q
+//                     (the last entry into q): one min-plus product per (graph, part);
+//   6 merge           inner rows: d(i, v) = min(d_p(i, v) [v in p],
+//                     min_{k in B_p} d_p(i, k) + d(k, v)) (minplus.h:51-64 with the
+//                     decode of util_Myalgorithm.h:236-247 folded in), a persistent
+//                     grid pulling (part, row block, column block) units from a
+//                     device counter, largest parts first.
+// Items with a subset of their graph's parts (sub-graph requests) take the reference's
+// form instead: 0 boundary SSSP (one wave per boundary source, label-correcting,
+// batch_sssp.h:232-252), 1 build with those rows as the boundary rows, 2, 6.
+// Every quantity is an exact min/plus expression of integer weights (int32) or of
+// float32 weights summed in FP64 (float32 output with MAXVALUE), so the two forms agree
+// bit for bit with each other and with the per-graph pipeline.
 #include <algorithm>
 #include <cstring>
 #include <vector>
 
 #include "fap_internal.h"
+#include "fw_desc.h"
 #include "gmem.h"
+#include "mp_tile.h"
 
 namespace fap_hip {
+
+int launch_fw(const FwDesc *d_descs, int ndesc, int max_nb, hipStream_t stream);
+int launch_fw64(const Fw64Desc *d_descs, int nd, int max_nb, hipStream_t st);
+
 namespace {
 
-constexpr int kBT = 64;        // tile edge
-constexpr int kBKc = 16;       // k per LDS chunk of a tile product
-constexpr int kBQ = 1024;      // frontier queue entries per wave
-constexpr int kBWaves = 8;     // waves per workgroup of the persistent kernel
-constexpr int kBThreads = 64 * kBWaves;
-constexpr int kP1Per = kBT * kBT / kBThreads;  // pivot entries per thread in phase 1
-constexpr int kSWaves = 4;     // waves (sources) per workgroup of the SSSP launch
-constexpr int kBatchMaxS = 4096;  // largest part the batch form takes
+constexpr int kBT = 64;           // tile edge of the Floyd-Warshall matrices
+constexpr int kBatchMaxS = 8192;  // largest part
+constexpr int kBQ = 1024;         // SSSP frontier queue entries per wave
+constexpr int kSWaves = 4;        // waves (sources) per workgroup of the SSSP launch
+constexpr int kMCols = 256;       // merge: columns per unit (one per thread)
+constexpr int kKc = 64;           // k per LDS chunk of the products
 
-struct BatchWork {
+// One graph of the batch.
+struct BGraph {
   const int *row_off, *col, *w;  // CSR (w: int32 weights or NULL)
-  const float *wf;               // float weights (double path) or NULL
+  const float *wf;               // float32 weights (FP64 path) or NULL
   const int *graph_id, *st2ed, *ed2st;
-  void *out;                     // n x ld_out result of the graph (int32 or float32)
+  void *out;                     // n x ld_out result (int32 or float32)
   int64_t ld_out;
-  void *rows;                    // the part's SSSP rows (int32: its boundary rows of out)
+  void *R;                       // FP64 path: boundary rows (nb x ldr doubles)
   int64_t ldr;
-  int64_t out_row;               // the part's first row in out
-  int n, part_id, offset, num, bdy, pad;
+  void *MB;                      // boundary-graph matrix (SB x SB), full items only
+  int SB;
+  int n, nb;                     // vertices; boundary vertices (full items)
+  int bbase;                     // first entry of the graph in bpart / brow
+  int pid_base, max_pid;         // pid2idx[pid_base + part id] = part index, or -1
+  int gb_row_begin;              // first boundary-graph build row
+  int full;                      // 1: parts cover the graph (boundary-graph form)
 };
 
-// One boundary source: part `work`, its q-th boundary vertex.
-struct BatchSrc {
-  int work, q;
+// One part of one graph.
+struct BPart {
+  void *D;                       // S x S matrix (ld = S)
+  const void *Brows;             // the part's boundary rows (int32: rows of out)
+  int64_t ldb;
+  int64_t out_row;               // first result row of the part
+  int S, g, part_id, off, s, b;
+  int bs;                        // boundary-graph index of its first boundary vertex
+  int row_begin;                 // first build row (prefix of S)
+};
+
+struct BRowUnit {                // stage 5: rows r0 .. of graph parts[q].g, columns of q
+  int q, r0;
+};
+
+struct BMerge {                  // stage 6: one unit (part, row block, column block)
+  int p, rb, cb, pad;
+};
+
+struct BSrc {                    // stage 0: boundary source q of part p
+  int p, q;
 };
 
 template <typename T>
@@ -71,8 +102,9 @@ struct Dist<int> {
   typedef int Out;
   static __device__ __forceinline__ int inf() { return kInf; }
   static __device__ __forceinline__ int mn(int a, int b) { return min(a, b); }
-  static __device__ __forceinline__ int weight(const BatchWork &wk, int e) {
-    return wk.w ? wk.w[e] : 1;
+  static __device__ __forceinline__ int mn3(int a, int x, int y) { return min3_i32(a, x, y); }
+  static __device__ __forceinline__ int weight(const BGraph &g, int e) {
+    return g.w ? g.w[e] : 1;
   }
   static __device__ __forceinline__ int out(int v) { return v; }
   static __device__ __forceinline__ int load_agent(const int *p) {
@@ -86,8 +118,11 @@ struct Dist<double> {
   typedef float Out;
   static __device__ __forceinline__ double inf() { return __builtin_inf(); }
   static __device__ __forceinline__ double mn(double a, double b) { return fmin(a, b); }
-  static __device__ __forceinline__ double weight(const BatchWork &wk, int e) {
-    return wk.wf ? (double)wk.wf[e] : 1.0;
+  static __device__ __forceinline__ double mn3(double a, double x, double y) {
+    return fmin(a, fmin(x, y));
+  }
+  static __device__ __forceinline__ double weight(const BGraph &g, int e) {
+    return g.wf ? (double)g.wf[e] : (g.w ? (double)g.w[e] : 1.0);
   }
   static __device__ __forceinline__ float out(double v) { return v < 1e8 ? (float)v : 1e8f; }
   static __device__ __forceinline__ double load_agent(const double *p) {
@@ -109,16 +144,27 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// ------------------------------------------------------------------ SSSP
+// Largest i in [0, n) with key(i) <= x (key ascending, key(0) <= x).
+template <typename F>
+__device__ __forceinline__ int upper_index(int n, int x, F key) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (key(mid) <= x) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------- 0: boundary SSSP
 template <typename T>
-__device__ __forceinline__ void relax(const BatchWork &wk, T *row, int u, int *q_next,
-                                      int *q_count) {
+__device__ __forceinline__ void relax(const BGraph &g, T *row, int u, int *q_next, int *q_count) {
   const T du = Dist<T>::load_agent(row + u);
   if (!(du < Dist<T>::inf())) return;
-  const int e1 = wk.row_off[u + 1];
-  for (int e = wk.row_off[u]; e < e1; ++e) {
-    const int v = wk.col[e];
-    const T nd = du + Dist<T>::weight(wk, e);
+  const int e1 = g.row_off[u + 1];
+  for (int e = g.row_off[u]; e < e1; ++e) {
+    const int v = g.col[e];
+    const T nd = du + Dist<T>::weight(g, e);
     if (nd < row[v] && nd < Dist<T>::atomic_min(row + v, nd)) {
       const int pos = atomicAdd(q_count, 1);
       if (pos < kBQ) q_next[pos] = v;
@@ -126,11 +172,12 @@ __device__ __forceinline__ void relax(const BatchWork &wk, T *row, int u, int *q
   }
 }
 
-// One wave: the full row of source s (labels decrease only, so a stale plain read
-// of row[v] can only let an unnecessary atomic through).
+// One wave computes the full row of source s (labels only decrease, so a stale plain
+// read of row[v] can only let an unnecessary atomic through).  A frontier larger than
+// the LDS queue is replaced by a scan of the whole row.
 template <typename T>
-__device__ void wave_sssp(const BatchWork &wk, int s, T *row, int *queue, int *qcount, int lane) {
-  for (int v = lane; v < wk.n; v += 64) row[v] = (v == s) ? T(0) : Dist<T>::inf();
+__device__ void wave_sssp(const BGraph &g, int s, T *row, int *queue, int *qcount, int lane) {
+  for (int v = lane; v < g.n; v += 64) row[v] = (v == s) ? T(0) : Dist<T>::inf();
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   if (lane == 0) {
     queue[0] = s;
@@ -145,9 +192,9 @@ __device__ void wave_sssp(const BatchWork &wk, int s, T *row, int *queue, int *q
     int *q_next = queue + nxt * kBQ;
     int *q_cnt = qcount + nxt;
     if (!scan) {
-      for (int i = lane; i < qsize; i += 64) relax(wk, row, queue[cur * kBQ + i], q_next, q_cnt);
+      for (int i = lane; i < qsize; i += 64) relax(g, row, queue[cur * kBQ + i], q_next, q_cnt);
     } else {
-      for (int u = lane; u < wk.n; u += 64) relax(wk, row, u, q_next, q_cnt);
+      for (int u = lane; u < g.n; u += 64) relax(g, row, u, q_next, q_cnt);
     }
     wave_sync();
     const int cnt = __hip_atomic_load(q_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
@@ -161,242 +208,279 @@ __device__ void wave_sssp(const BatchWork &wk, int s, T *row, int *queue, int *q
   }
 }
 
-// ------------------------------------------------------------------ tile products
-__device__ __forceinline__ int orow(int t, int q) { return (q < 4 ? 0 : 32) + t * 4 + (q & 3); }
-
-// acc = min(acc, A (x) B) for a 64-row A block (rows at a, stride lda, columns
-// k in [0, kdim)) and a B block (rows k at b, stride ldb, 64 columns); entries
-// outside [0, kdim) or outside the valid rows/columns read as +inf.  One wave,
-// staged through its LDS slab As[kBKc][64], Bs[kBKc][64].
 template <typename T>
-__device__ __forceinline__ void wave_product(T (&acc)[8][8], const FAP_GLOBAL T *a, int64_t lda,
-                                             int arows, const FAP_GLOBAL T *b, int64_t ldb,
-                                             int bcols, int kdim, T *As, T *Bs, int lane) {
-  const int tr = lane >> 3, tc = lane & 7;
-  typedef T vec4 __attribute__((ext_vector_type(4)));
-  for (int k0 = 0; k0 < kdim; k0 += kBKc) {
-    if (lane < arows && k0 + kBKc <= kdim) {
-      // A rows start 64-element aligned and k0 is a multiple of 16: vector loads.
-      const FAP_GLOBAL vec4 *src = reinterpret_cast<const FAP_GLOBAL vec4 *>(a + (int64_t)lane * lda + k0);
-#pragma unroll
-      for (int c = 0; c < kBKc / 4; ++c) {
-        const vec4 v = src[c];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) As[(4 * c + e) * kBT + lane] = v[e];
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < kBKc; ++q) {
-        const int k = k0 + q;
-        As[q * kBT + lane] =
-            (lane < arows && k < kdim) ? a[(int64_t)lane * lda + k] : Dist<T>::inf();
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < kBKc; ++q) {
-      const int k = k0 + q;
-      Bs[q * kBT + lane] = (lane < bcols && k < kdim) ? b[(int64_t)k * ldb + lane] : Dist<T>::inf();
-    }
-    wave_sync();
-    const int kc = min(kBKc, kdim - k0);
-    for (int k = 0; k < kc; ++k) {
-      T av[8], bv[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        av[q] = As[k * kBT + orow(tr, q)];
-        bv[q] = Bs[k * kBT + orow(tc, q)];
-      }
-#pragma unroll
-      for (int q = 0; q < 8; ++q)
-#pragma unroll
-        for (int c = 0; c < 8; ++c) acc[q][c] = Dist<T>::mn(acc[q][c], av[q] + bv[c]);
-    }
-    wave_sync();  // the slab is refilled next chunk
-  }
-}
-
-template <typename T>
-__device__ __forceinline__ void load_tile(T (&acc)[8][8], const FAP_GLOBAL T *t, int64_t ld,
-                                          int lane) {
-  const int tr = lane >> 3, tc = lane & 7;
-#pragma unroll
-  for (int q = 0; q < 8; ++q)
-#pragma unroll
-    for (int c = 0; c < 8; ++c) acc[q][c] = t[(int64_t)orow(tr, q) * ld + orow(tc, c)];
-}
-
-template <typename T>
-__device__ __forceinline__ void store_tile(const T (&acc)[8][8], FAP_GLOBAL T *t, int64_t ld,
-                                           int lane) {
-  const int tr = lane >> 3, tc = lane & 7;
-#pragma unroll
-  for (int q = 0; q < 8; ++q)
-#pragma unroll
-    for (int c = 0; c < 8; ++c) t[(int64_t)orow(tr, q) * ld + orow(tc, c)] = acc[q][c];
-}
-
-// ------------------------------------------------------------------ one part
-template <typename T>
-__device__ void process_part(const BatchWork &wk, T *Mgen, char *lds) {
-  // The work record and the scratch slot hold generic pointers: every access goes
-  // through the global address space (flat accesses also count against the LDS counter,
-  // so the LDS staging writes after them waited for memory).
-  FAP_GLOBAL T *M = gptr(Mgen);
-  const FAP_GLOBAL T *rows = gptr(static_cast<const T *>(wk.rows));
-  const FAP_GLOBAL int *st2ed = gptr(wk.st2ed), *row_off = gptr(wk.row_off), *col = gptr(wk.col);
-  const FAP_GLOBAL int *graph_id = gptr(wk.graph_id), *ed2st = gptr(wk.ed2st);
-  const int64_t ldr = wk.ldr;
-  typedef typename Dist<T>::Out Out;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int num = wk.num, bdy = wk.bdy, off = wk.offset, n = wk.n;
-  const int S = (num + kBT - 1) / kBT * kBT, nb = S / kBT;
-  FAP_GLOBAL Out *out = gptr(static_cast<Out *>(wk.out));
-
-  // 2. build (the SSSP rows were written by batch_sssp_kernel)
-  for (int i = 0; i < S; ++i)
-    for (int j = tid; j < S; j += kBThreads)
-      M[(int64_t)i * S + j] = (i < bdy && j < num) ? rows[(int64_t)i * ldr + st2ed[off + j]]
-                                                   : Dist<T>::inf();
-  __syncthreads();
-  for (int i = bdy + tid; i < num; i += kBThreads) {
-    const int ver = st2ed[off + i];
-    for (int e = row_off[ver]; e < row_off[ver + 1]; ++e) {
-      const int v = col[e];
-      if (graph_id[v] != wk.part_id) continue;
-      const int idx = ed2st[v] - off;
-      if (idx >= 0 && idx < num) M[(int64_t)i * S + idx] = Dist<T>::weight(wk, e);
-    }
-  }
-  __syncthreads();
-  for (int i = tid; i < num; i += kBThreads) M[(int64_t)i * S + i] = T(0);
-  __syncthreads();
-
-  // 3. Floyd-Warshall
-  T *As = reinterpret_cast<T *>(lds) + wave * 2 * kBKc * kBT;
-  T *Bs = As + kBKc * kBT;
-  for (int r = 0; r < nb; ++r) {
-    {  // phase 1: pivot tile closed in LDS, kP1Per consecutive entries per thread
-      T *D = reinterpret_cast<T *>(lds);
-      FAP_GLOBAL T *g = M + (int64_t)r * kBT * S + r * kBT;
-      const int row = tid / (kBT / kP1Per), c0 = (tid % (kBT / kP1Per)) * kP1Per;
-#pragma unroll
-      for (int c = 0; c < kP1Per; ++c) D[row * kBT + c0 + c] = g[(int64_t)row * S + c0 + c];
-      for (int k = 0; k < kBT; ++k) {
-        __syncthreads();  // row k and column k do not change during step k
-        const T dik = D[row * kBT + k];
-#pragma unroll
-        for (int c = 0; c < kP1Per; ++c)
-          D[row * kBT + c0 + c] = Dist<T>::mn(D[row * kBT + c0 + c], dik + D[k * kBT + c0 + c]);
-      }
-      __syncthreads();
-#pragma unroll
-      for (int c = 0; c < kP1Per; ++c) g[(int64_t)row * S + c0 + c] = D[row * kBT + c0 + c];
-    }
-    __syncthreads();
-    // phase 2: row tiles (r, o) and column tiles (o, r), o != r
-    const FAP_GLOBAL T *P = M + (int64_t)r * kBT * S + r * kBT;
-    for (int t = wave; t < 2 * (nb - 1); t += kBWaves) {
-      const bool is_row = t < nb - 1;
-      int o = is_row ? t : t - (nb - 1);
-      o += (o >= r);
-      FAP_GLOBAL T *tile =
-          is_row ? M + (int64_t)r * kBT * S + o * kBT : M + (int64_t)o * kBT * S + r * kBT;
-      T acc[8][8];
-      load_tile(acc, tile, S, lane);
-      if (is_row) wave_product(acc, P, S, kBT, tile, S, kBT, kBT, As, Bs, lane);
-      else wave_product(acc, tile, S, kBT, P, S, kBT, kBT, As, Bs, lane);
-      store_tile(acc, tile, S, lane);
-    }
-    __syncthreads();
-    // phase 3: tiles outside the pivot row and column
-    for (int t = wave; t < (nb - 1) * (nb - 1); t += kBWaves) {
-      int i = t / (nb - 1), j = t - i * (nb - 1);
-      i += (i >= r);
-      j += (j >= r);
-      FAP_GLOBAL T *tile = M + (int64_t)i * kBT * S + j * kBT;
-      T acc[8][8];
-      load_tile(acc, tile, S, lane);
-      wave_product(acc, M + (int64_t)i * kBT * S + r * kBT, S, kBT,
-                   M + (int64_t)r * kBT * S + j * kBT, S, kBT, kBT, As, Bs, lane);
-      store_tile(acc, tile, S, lane);
-    }
-    __syncthreads();
-  }
-
-  // 4. merge: inner rows x all columns, own columns from the FW matrix
-  const int mrows = num - bdy;
-  if (mrows > 0) {
-    const int rbs = (mrows + kBT - 1) / kBT, cbs = (n + kBT - 1) / kBT;
-    const int tr = lane >> 3, tc = lane & 7;
-    for (int t = wave; t < rbs * cbs; t += kBWaves) {
-      const int rb = t / cbs, cb = t - rb * cbs;
-      const int row0 = rb * kBT, col0 = cb * kBT;
-      T acc[8][8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q)
-#pragma unroll
-        for (int c = 0; c < 8; ++c) acc[q][c] = Dist<T>::inf();
-      if (bdy > 0)
-        wave_product(acc, M + (int64_t)(bdy + row0) * S, S, min(kBT, mrows - row0), rows + col0,
-                     ldr, min(kBT, n - col0), bdy, As, Bs, lane);
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const int j = col0 + orow(tc, c);
-        if (j >= n) continue;
-        const int own = graph_id[j] == wk.part_id ? ed2st[j] - off : -1;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const int i = row0 + orow(tr, q);
-          if (i >= mrows) continue;
-          const T v = own >= 0 ? M[(int64_t)(bdy + i) * S + own] : acc[q][c];
-          out[(wk.out_row + bdy + i) * wk.ld_out + j] = Dist<T>::out(v);
-        }
-      }
-    }
-  }
-  __syncthreads();
-
-  // 5. boundary rows (after the merge, which reads them as its B operand)
-  for (int i = 0; i < bdy; ++i)
-    for (int j = tid; j < n; j += kBThreads) {
-      const T v = graph_id[j] == wk.part_id ? M[(int64_t)i * S + ed2st[j] - off]
-                                               : rows[(int64_t)i * ldr + j];
-      out[(wk.out_row + i) * wk.ld_out + j] = Dist<T>::out(v);
-    }
-  __syncthreads();
-}
-
-// Boundary SSSP of every part: one wave per source.
-template <typename T>
-__global__ __launch_bounds__(64 * kSWaves) void batch_sssp_kernel(const BatchWork *__restrict__ work,
-                                                                  const BatchSrc *__restrict__ srcs,
-                                                                  int nsrc) {
+__global__ __launch_bounds__(64 * kSWaves) void bsssp_kernel(const BGraph *__restrict__ graphs,
+                                                             const BPart *__restrict__ parts,
+                                                             const BSrc *__restrict__ srcs,
+                                                             int nsrc) {
   __shared__ int queues[kSWaves][2 * kBQ + 2];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = blockIdx.x * kSWaves + wave;
   if (i >= nsrc) return;  // whole wave leaves; no workgroup barrier is used
-  const BatchSrc sr = srcs[i];
-  const BatchWork wk = work[sr.work];
-  T *row = static_cast<T *>(wk.rows) + (int64_t)sr.q * wk.ldr;
-  wave_sssp<T>(wk, wk.st2ed[wk.offset + sr.q], row, queues[wave], queues[wave] + 2 * kBQ, lane);
+  const BSrc sr = srcs[i];
+  const BPart p = parts[sr.p];
+  const BGraph g = graphs[p.g];
+  T *row = static_cast<T *>(const_cast<void *>(p.Brows)) + (int64_t)sr.q * p.ldb;
+  wave_sssp<T>(g, g.st2ed[p.off + sr.q], row, queues[wave], queues[wave] + 2 * kBQ, lane);
 }
 
-// Persistent grid: each workgroup owns one scratch slot (an S_max x S_max matrix)
-// and pulls parts until the counter passes nwork.
+// ---------------------------------------------------------------- 1: build
+// One wave per padded row of every part.  Every entry is written once by the lane
+// owning its column: INF, the weight of the last in-part arc to it (CSR order), 0 on
+// the diagonal; for parts of partial items the boundary rows are the SSSP rows.
 template <typename T>
-__global__ __launch_bounds__(kBThreads) void batch_kernel(const BatchWork *__restrict__ work,
-                                                          int nwork, int *counter, char *scratch,
-                                                          int64_t slot_bytes) {
-  extern __shared__ __attribute__((aligned(16))) char lds[];
-  __shared__ int item_s;
-  T *M = reinterpret_cast<T *>(scratch + (int64_t)blockIdx.x * slot_bytes);
+__global__ __launch_bounds__(256) void bbuild_kernel(const BGraph *__restrict__ graphs,
+                                                     const BPart *__restrict__ parts, int np,
+                                                     int total_rows) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + wave;
+  if (row >= total_rows) return;
+  const int pi = upper_index(np, row, [&](int k) { return parts[k].row_begin; });
+  const BPart p = parts[pi];
+  const BGraph g = graphs[p.g];
+  const int i = row - p.row_begin;
+  FAP_GLOBAL T *D = gptr(static_cast<T *>(p.D)) + (int64_t)i * p.S;
+  if (i >= p.s) {
+    for (int t = lane; t < p.S; t += 64) D[t] = Dist<T>::inf();
+    return;
+  }
+  if (!g.full && i < p.b) {  // boundary row of a partial item: its SSSP row
+    const FAP_GLOBAL T *src = gptr(static_cast<const T *>(p.Brows)) + (int64_t)i * p.ldb;
+    for (int t = lane; t < p.S; t += 64)
+      D[t] = t < p.s ? src[g.st2ed[p.off + t]] : Dist<T>::inf();
+    return;
+  }
+  const int v = g.st2ed[p.off + i];
+  const int e0 = g.row_off[v], e1 = g.row_off[v + 1];
+  for (int t = lane; t < p.S; t += 64) {
+    T val = Dist<T>::inf();
+    for (int e = e0; e < e1; ++e) {
+      const int u = g.col[e];
+      if (g.graph_id[u] != p.part_id) continue;
+      if (g.ed2st[u] - p.off == t) val = Dist<T>::weight(g, e);
+    }
+    if (t == i) val = T(0);
+    D[t] = val;
+  }
+}
+
+// ---------------------------------------------------------------- 3: boundary graph
+// One wave per padded row of every boundary-graph matrix.  Row r = boundary vertex x
+// of part q (local c): the in-part block d_q[c][B_q], the cut arcs of x (minimum over
+// parallel arcs), 0 on the diagonal.
+template <typename T>
+__global__ __launch_bounds__(256) void bgb_kernel(const BGraph *__restrict__ graphs,
+                                                  const BPart *__restrict__ parts,
+                                                  const int *__restrict__ gbg, int ngb,
+                                                  const int *__restrict__ bpart,
+                                                  const int *__restrict__ pid2idx,
+                                                  int total_rows) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + wave;
+  if (row >= total_rows) return;
+  const int gi = gbg[upper_index(ngb, row, [&](int k) { return graphs[gbg[k]].gb_row_begin; })];
+  const BGraph g = graphs[gi];
+  const int r = row - g.gb_row_begin;
+  FAP_GLOBAL T *M = gptr(static_cast<T *>(g.MB)) + (int64_t)r * g.SB;
+  if (r >= g.nb) {
+    for (int t = lane; t < g.SB; t += 64) M[t] = Dist<T>::inf();
+    return;
+  }
+  const BPart q = parts[bpart[g.bbase + r]];
+  const int c = r - q.bs;
+  const FAP_GLOBAL T *Dq = gptr(static_cast<const T *>(q.D)) + (int64_t)c * q.S;
+  const int x = g.st2ed[q.off + c];
+  const int e0 = g.row_off[x], e1 = g.row_off[x + 1];
+  for (int t = lane; t < g.SB; t += 64) {
+    T val = (t >= q.bs && t < q.bs + q.b) ? Dq[t - q.bs] : Dist<T>::inf();
+    for (int e = e0; e < e1; ++e) {
+      const int y = g.col[e];
+      const int py = g.graph_id[y];
+      if (py == q.part_id || py < 1 || py > g.max_pid) continue;
+      const int qi = pid2idx[g.pid_base + py];
+      if (qi < 0) continue;
+      const BPart qy = parts[qi];
+      const int ly = g.ed2st[y] - qy.off;
+      if (ly < 0 || ly >= qy.b || qy.bs + ly != t) continue;
+      val = Dist<T>::mn(val, Dist<T>::weight(g, e));
+    }
+    if (t == r) val = T(0);
+    M[t] = val;
+  }
+}
+
+// ---------------------------------------------------------------- 5: boundary rows
+// Unit (q, r0): rows r0 .. r0 + RB - 1 of graph g's boundary graph against the columns
+// of part q: C[x][j] = min_{c < b_q} D_B[x][bs_q + c] + d_q[c][j], written to the result
+// at row brow[x], column st2ed[off_q + j] (and to the FP64 rows R).  D_B entries are
+// staged k-major in LDS (broadcast reads), d_q rows are read coalesced.
+template <typename T, int RB>
+__global__ __launch_bounds__(256) void brows_kernel(const BGraph *__restrict__ graphs,
+                                                    const BPart *__restrict__ parts,
+                                                    const BRowUnit *__restrict__ units,
+                                                    const int64_t *__restrict__ brow) {
+  typedef typename Dist<T>::Out Out;
+  __shared__ T As[kKc][RB];
+  const BRowUnit un = units[blockIdx.x];
+  const BPart q = parts[un.q];
+  const BGraph g = graphs[q.g];
+  const int tid = threadIdx.x, r0 = un.r0;
+  const FAP_GLOBAL T *MB = gptr(static_cast<const T *>(g.MB));
+  const FAP_GLOBAL T *Dq = gptr(static_cast<const T *>(q.D));
+  FAP_GLOBAL Out *out = gptr(static_cast<Out *>(g.out));
+  FAP_GLOBAL T *R = gptr(static_cast<T *>(g.R));
+  for (int j0 = 0; j0 < q.s; j0 += 256) {
+    const int j = j0 + tid;
+    const bool jok = j < q.s;
+    T acc[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) acc[r] = Dist<T>::inf();
+    for (int k0 = 0; k0 < q.b; k0 += kKc) {
+      const int kc = min(kKc, q.b - k0);
+      for (int e = tid; e < kKc * RB; e += 256) {
+        const int r = e / kKc, k = e - r * kKc;
+        As[k][r] = (k < kc && r0 + r < g.nb) ? MB[(int64_t)(r0 + r) * g.SB + q.bs + k0 + k]
+                                             : Dist<T>::inf();
+      }
+      __syncthreads();
+      for (int k = 0; k < kc; ++k) {
+        const T dv = jok ? Dq[(int64_t)(k0 + k) * q.S + j] : Dist<T>::inf();
+#pragma unroll
+        for (int r = 0; r < RB; ++r) acc[r] = Dist<T>::mn(acc[r], As[k][r] + dv);
+      }
+      __syncthreads();
+    }
+    if (jok) {
+      const int v = g.st2ed[q.off + j];
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        if (r0 + r >= g.nb) continue;
+        out[brow[g.bbase + r0 + r] * g.ld_out + v] = Dist<T>::out(acc[r]);
+        if (g.R) R[(int64_t)(r0 + r) * g.ldr + v] = acc[r];
+      }
+    }
+  }
+}
+
+// FP64 partial items: the SSSP rows (FP64) to the float32 result rows.
+__global__ __launch_bounds__(256) void bconv_kernel(const BGraph *__restrict__ graphs,
+                                                    const BPart *__restrict__ parts,
+                                                    const BSrc *__restrict__ srcs) {
+  const BSrc sr = srcs[blockIdx.x];
+  const BPart p = parts[sr.p];
+  const BGraph g = graphs[p.g];
+  const FAP_GLOBAL double *src = gptr(static_cast<const double *>(p.Brows)) + (int64_t)sr.q * p.ldb;
+  FAP_GLOBAL float *dst = gptr(static_cast<float *>(g.out)) + (p.out_row + sr.q) * g.ld_out;
+  for (int v = threadIdx.x; v < g.n; v += 256) dst[v] = Dist<double>::out(src[v]);
+}
+
+// ---------------------------------------------------------------- 6: merge
+// Persistent grid; a unit is (part, row block of 4 * RB inner rows, column block of 256
+// columns), the row blocks of one column block consecutive so its boundary-row panel
+// is reused from L2.  Wave w owns rows w*RB .. w*RB + RB - 1 of the block and lane l
+// the columns l + 64e (e < 4), so every load and store of a wave covers 256
+// contiguous bytes.  The 4*RB x 64 A block is staged k-major in LDS (broadcast reads);
+// the four waves read the same B rows (L1).  Own columns take min(d_p(i, v), product).
+constexpr int kMWaves = 4;
+template <typename T, int RB>
+__global__ __launch_bounds__(256) void bmerge_kernel(const BGraph *__restrict__ graphs,
+                                                     const BPart *__restrict__ parts,
+                                                     const BMerge *__restrict__ units,
+                                                     int total_units, int *counter) {
+  typedef typename Dist<T>::Out Out;
+  constexpr int kRows = kMWaves * RB;
+  __shared__ __attribute__((aligned(16))) T As[kKc][kRows];
+  __shared__ int unit_s;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   while (true) {
-    if (threadIdx.x == 0) item_s = atomicAdd(counter, 1);
+    if (tid == 0) unit_s = atomicAdd(counter, 1);
     __syncthreads();
-    const int item = item_s;
-    __syncthreads();
-    if (item >= nwork) break;  // uniform over the workgroup
-    process_part<T>(work[item], M, lds);
+    const int u = unit_s;
+    __syncthreads();  // every thread has read unit_s before thread 0 takes the next one
+    if (u >= total_units) break;  // uniform: every workgroup ends here
+    const BMerge m = units[u];
+    const BPart p = parts[m.p];
+    const BGraph g = graphs[p.g];
+    const int cb = m.cb, rb = m.rb;
+    const int inner = p.s - p.b, r0 = rb * kRows;
+    const FAP_GLOBAL T *D = gptr(static_cast<const T *>(p.D));
+    const FAP_GLOBAL T *B = gptr(static_cast<const T *>(p.Brows));
+    int col[4];
+    bool cok[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      col[e] = cb * kMCols + e * 64 + lane;
+      cok[e] = col[e] < g.n;
+    }
+    T acc[RB][4];
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc[r][e] = Dist<T>::inf();
+    for (int k0 = 0; k0 < p.b; k0 += kKc) {
+      const int kc = min(kKc, p.b - k0);
+      for (int x = tid; x < kKc * kRows; x += 256) {
+        const int r = x / kKc, k = x - r * kKc;
+        As[k][r] = (k < kc && r0 + r < inner) ? D[(int64_t)(p.b + r0 + r) * p.S + k0 + k]
+                                              : Dist<T>::inf();
+      }
+      __syncthreads();
+      const T *Aw = &As[0][wave * RB];
+      // B rows k and k + 1 of this chunk (INF past kc or past the last column); the
+      // next pair is loaded before the current pair is used.
+      auto load_pair = [&](int k, T (&x0)[4], T (&x1)[4]) {
+        const FAP_GLOBAL T *r0p = B + (int64_t)(k0 + k) * p.ldb;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          x0[e] = (cok[e] && k < kc) ? r0p[col[e]] : Dist<T>::inf();
+          x1[e] = (cok[e] && k + 1 < kc) ? r0p[p.ldb + col[e]] : Dist<T>::inf();
+        }
+      };
+      T b0[4], b1[4];
+      load_pair(0, b0, b1);
+      for (int k = 0; k < kc; k += 2) {
+        T n0[4], n1[4];
+        load_pair(k + 2, n0, n1);
+        T a0[RB], a1[RB];
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+          a0[r] = Aw[k * kRows + r];
+          a1[r] = Aw[(k + 1) * kRows + r];  // row kc of As is INF when kc is odd
+        }
+#pragma unroll
+        for (int r = 0; r < RB; ++r)
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            acc[r][e] = Dist<T>::mn3(acc[r][e], a0[r] + b0[e], a1[r] + b1[e]);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          b0[e] = n0[e];
+          b1[e] = n1[e];
+        }
+      }
+      __syncthreads();
+    }
+    int own[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      own[e] = cok[e] && g.graph_id[col[e]] == p.part_id ? g.ed2st[col[e]] - p.off : -1;
+      if (own[e] >= p.s) own[e] = -1;
+    }
+    FAP_GLOBAL Out *out = gptr(static_cast<Out *>(g.out));
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int i = r0 + wave * RB + r;
+      if (i >= inner) continue;
+      FAP_GLOBAL Out *orow = out + (p.out_row + p.b + i) * g.ld_out;
+      const FAP_GLOBAL T *drow = D + (int64_t)(p.b + i) * p.S;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (!cok[e]) continue;
+        T val = acc[r][e];
+        if (own[e] >= 0) val = Dist<T>::mn(val, drow[own[e]]);
+        orow[col[e]] = Dist<T>::out(val);
+      }
+    }
   }
 }
 
@@ -405,71 +489,367 @@ __global__ void reset_counter_kernel(int *c, int n) {
   if (i < n) c[i] = 0;
 }
 
-size_t lds_bytes(int elem) {
-  const size_t pivot = (size_t)kBT * kBT * elem;
-  const size_t slabs = (size_t)kBWaves * 2 * kBKc * kBT * elem;
-  return std::max(pivot, slabs);
-}
-
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+// Row-block sizes (template instances): stage 5 sources per unit, stage 6 rows per unit.
+int pick_rows(int code, int dflt) {
+  switch (code) {
+    case 1: return 8;
+    case 2: return 16;
+    case 3: return 32;
+    default: return dflt;
+  }
+}
 
 }  // namespace
 }  // namespace fap_hip
 
 using namespace fap_hip;
 
-// One kind of entries (int32 or double) of a batch.
+// One kind of entries (int32 or FP64) of a batch.
 struct fap_batch_kind {
-  int nwork = 0, grid = 0, nsrc = 0;
-  BatchWork *d_work = nullptr;
-  BatchSrc *d_src = nullptr;
-  char *d_scratch = nullptr;  // scratch slots, then (double path) the SSSP rows
-  int64_t slot_bytes = 0;
+  int ngraphs = 0, nparts = 0, build_rows = 0, max_nb = 0;
+  int ngb = 0, gb_rows = 0, gb_max_nb = 0, nunits5 = 0, nmerge = 0, merge_units = 0, nsrc = 0;
+  int rows6 = 16;  // merge rows per wave (FP64: 8, register budget)
+  int grid = 0;    // persistent merge workgroups
+  char *d_mem = nullptr;  // tables and scratch in one allocation
+  BGraph *graphs = nullptr;
+  BPart *parts = nullptr;
+  int *gbg = nullptr, *bpart = nullptr, *pid2idx = nullptr;
+  int64_t *brow = nullptr;
+  BRowUnit *units5 = nullptr;
+  BMerge *merge = nullptr;
+  BSrc *srcs = nullptr;
+  void *fw_descs = nullptr, *gb_descs = nullptr;
 };
 
 struct fap_batch {
-  fap_batch_kind kind[2];  // 0 int32, 1 double
+  fap_batch_kind kind[2];  // 0 int32, 1 FP64
   int *d_counter = nullptr;
+  int per_cu = 0, rows5 = 16, rows6 = 16;
   bool use_graph = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  hipStream_t captured_on = nullptr;
 };
 
-static int batch_launch(fap_batch *b, hipStream_t st) {
-  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, st, b->d_counter, 2);
+namespace {
+
+// Batched Floyd-Warshall over nd descriptors (the launchers put descriptors on the
+// grid's y dimension: at most 65,535 per launch).
+template <typename T>
+int launch_fw_chunks(void *descs, int nd, int max_nb, hipStream_t st) {
+  for (int d0 = 0; d0 < nd; d0 += 65535) {
+    const int cnt = std::min(65535, nd - d0);
+    if (sizeof(T) == 4) FAP_TRY(launch_fw(static_cast<FwDesc *>(descs) + d0, cnt, max_nb, st));
+    else FAP_TRY(launch_fw64(static_cast<Fw64Desc *>(descs) + d0, cnt, max_nb, st));
+  }
+  return FAP_OK;
+}
+
+template <typename T>
+int launch_kind(const fap_batch *b, const fap_batch_kind &q, int *counter, hipStream_t st) {
+  if (q.nparts == 0) return FAP_OK;
+  if (q.nsrc > 0) {
+    hipLaunchKernelGGL(bsssp_kernel<T>, dim3((q.nsrc + kSWaves - 1) / kSWaves),
+                       dim3(64 * kSWaves), 0, st, q.graphs, q.parts, q.srcs, q.nsrc);
+    FAP_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL(bbuild_kernel<T>, dim3((q.build_rows + 3) / 4), dim3(256), 0, st, q.graphs,
+                     q.parts, q.nparts, q.build_rows);
   FAP_CHECK_LAUNCH();
-  for (int k = 0; k < 2; ++k) {
-    fap_batch_kind &q = b->kind[k];
-    if (q.nwork == 0) continue;
-    const dim3 sgrid((q.nsrc + kSWaves - 1) / kSWaves);
-    if (k == 0) {
-      if (q.nsrc > 0)
-        hipLaunchKernelGGL(batch_sssp_kernel<int>, sgrid, dim3(64 * kSWaves), 0, st, q.d_work,
-                           q.d_src, q.nsrc);
-      hipLaunchKernelGGL(batch_kernel<int>, dim3(q.grid), dim3(kBThreads), lds_bytes(4), st,
-                         q.d_work, q.nwork, b->d_counter + k, q.d_scratch, q.slot_bytes);
-    } else {
-      if (q.nsrc > 0)
-        hipLaunchKernelGGL(batch_sssp_kernel<double>, sgrid, dim3(64 * kSWaves), 0, st, q.d_work,
-                           q.d_src, q.nsrc);
-      hipLaunchKernelGGL(batch_kernel<double>, dim3(q.grid), dim3(kBThreads), lds_bytes(8), st,
-                         q.d_work, q.nwork, b->d_counter + k, q.d_scratch, q.slot_bytes);
-    }
+  FAP_TRY(launch_fw_chunks<T>(q.fw_descs, q.nparts, q.max_nb, st));
+  if (q.ngb > 0) {
+    hipLaunchKernelGGL(bgb_kernel<T>, dim3((q.gb_rows + 3) / 4), dim3(256), 0, st, q.graphs,
+                       q.parts, q.gbg, q.ngb, q.bpart, q.pid2idx, q.gb_rows);
+    FAP_CHECK_LAUNCH();
+    FAP_TRY(launch_fw_chunks<T>(q.gb_descs, q.ngb, q.gb_max_nb, st));
+  }
+  if (q.nunits5 > 0) {
+    const dim3 g5(q.nunits5), t5(256);
+    if (b->rows5 == 8) hipLaunchKernelGGL((brows_kernel<T, 8>), g5, t5, 0, st, q.graphs, q.parts, q.units5, q.brow);
+    else if (b->rows5 == 32) hipLaunchKernelGGL((brows_kernel<T, 32>), g5, t5, 0, st, q.graphs, q.parts, q.units5, q.brow);
+    else hipLaunchKernelGGL((brows_kernel<T, 16>), g5, t5, 0, st, q.graphs, q.parts, q.units5, q.brow);
+    FAP_CHECK_LAUNCH();
+  }
+  if (sizeof(T) == 8 && q.nsrc > 0) {
+    hipLaunchKernelGGL(bconv_kernel, dim3(q.nsrc), dim3(256), 0, st, q.graphs, q.parts, q.srcs);
+    FAP_CHECK_LAUNCH();
+  }
+  if (q.merge_units > 0) {
+    const dim3 gm(std::min(q.grid, q.merge_units)), tm(256);
+    if (q.rows6 == 8)
+      hipLaunchKernelGGL((bmerge_kernel<T, 8>), gm, tm, 0, st, q.graphs, q.parts, q.merge, q.merge_units, counter);
+    else
+      hipLaunchKernelGGL((bmerge_kernel<T, 16>), gm, tm, 0, st, q.graphs, q.parts, q.merge, q.merge_units, counter);
     FAP_CHECK_LAUNCH();
   }
   return FAP_OK;
 }
 
+int batch_launch(fap_batch *b, hipStream_t st) {
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, st, b->d_counter, 2);
+  FAP_CHECK_LAUNCH();
+  FAP_TRY(launch_kind<int>(b, b->kind[0], b->d_counter, st));
+  FAP_TRY(launch_kind<double>(b, b->kind[1], b->d_counter + 1, st));
+  return FAP_OK;
+}
+
+// Host plan of one kind: the tables and the scratch layout, uploaded in one block.
+int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap_batch_item *> &items,
+              bool fp64) {
+  const int elem = fp64 ? 8 : 4;
+  q.rows6 = fp64 ? 8 : b->rows6;
+  // Persistent merge grid: as many workgroups as the device keeps resident (its
+  // register and LDS use), unless the caller fixed the workgroups per CU.
+  {
+    int per_cu = b->per_cu;
+    if (per_cu <= 0) {
+      const void *fn = fp64 ? (q.rows6 == 8 ? (const void *)bmerge_kernel<double, 8>
+                                            : (const void *)bmerge_kernel<double, 16>)
+                            : (q.rows6 == 8 ? (const void *)bmerge_kernel<int, 8>
+                                            : (const void *)bmerge_kernel<int, 16>);
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess ||
+          per_cu <= 0) {
+        (void)hipGetLastError();
+        per_cu = 2;
+      }
+    }
+    q.grid = per_cu * std::max(device_cus(), 1);
+  }
+  std::vector<BGraph> graphs;
+  std::vector<BPart> parts;
+  std::vector<int> gbg, bpart, pid2idx;
+  std::vector<int64_t> brow;
+  std::vector<BRowUnit> units5;
+  std::vector<BSrc> srcs;
+  std::vector<double> pcost;
+  // Scratch offsets (bytes) of every part matrix, boundary-graph matrix and FP64 row
+  // block; resolved to pointers once the allocation exists.
+  std::vector<int64_t> d_at, mb_at, r_at, brows_at;
+  int64_t scratch = 0;
+  auto take = [&](int64_t bytes) {
+    const int64_t o = scratch;
+    scratch = align_up(scratch + bytes, 256);
+    return o;
+  };
+  int build_rows = 0, max_nb = 0, gb_rows = 0, gb_max_nb = 0;
+  for (const fap_batch_item *it : items) {
+    const fap_graph_dev &d = it->g;
+    BGraph G{};
+    G.row_off = d.d_row_offset;
+    G.col = d.d_col;
+    G.w = d.d_w;
+    G.wf = d.d_wf;
+    G.graph_id = d.d_graph_id;
+    G.st2ed = d.d_st2ed;
+    G.ed2st = d.d_ed2st;
+    G.out = it->d_out;
+    G.ld_out = it->ld_out;
+    G.n = d.n;
+    G.ldr = align_up(d.n, 2);
+    G.bbase = (int)bpart.size();
+    G.pid_base = (int)pid2idx.size();
+    const int gi = (int)graphs.size();
+    std::vector<const fap_part_desc *> ps;
+    int max_pid = 0;
+    for (int k = 0; k < it->nparts; ++k) {
+      const fap_part_desc &pd = it->parts[k];
+      FAP_REQUIRE(pd.part_id >= 1 && pd.num >= 0 && pd.bdy >= 0 && pd.bdy <= pd.num &&
+                      pd.offset >= 0 && (int64_t)pd.offset + pd.num <= d.n && pd.out_row >= 0,
+                  "fap_hip_batch_create: bad part %d (id %d)", k, pd.part_id);
+      FAP_REQUIRE(pd.num <= kBatchMaxS, "fap_hip_batch_create: part with %d vertices (limit %d)",
+                  pd.num, kBatchMaxS);
+      max_pid = std::max(max_pid, pd.part_id);
+      if (pd.num > 0) ps.push_back(&pd);
+    }
+    // Full item: the parts tile [0, n) (then the boundary-graph form).
+    {
+      std::vector<std::pair<int, int>> span;
+      for (const fap_part_desc *pd : ps) span.push_back({pd->offset, pd->num});
+      std::sort(span.begin(), span.end());
+      int next = 0;
+      bool cover = true;
+      for (const auto &x : span) {
+        cover = cover && x.first == next;
+        next = x.first + x.second;
+      }
+      G.full = cover && next == d.n;
+    }
+    G.max_pid = max_pid;
+    pid2idx.resize(pid2idx.size() + max_pid + 1, -1);
+    int nb = 0;
+    for (const fap_part_desc *pd : ps) {
+      FAP_REQUIRE(pid2idx[G.pid_base + pd->part_id] < 0,
+                  "fap_hip_batch_create: part id %d repeated", pd->part_id);
+      pid2idx[G.pid_base + pd->part_id] = (int)parts.size();
+      BPart P{};
+      P.S = (int)align_up(pd->num, kBT);
+      P.g = gi;
+      P.part_id = pd->part_id;
+      P.off = pd->offset;
+      P.s = pd->num;
+      P.b = pd->bdy;
+      P.bs = nb;
+      P.row_begin = build_rows;
+      P.out_row = pd->out_row;
+      build_rows += P.S;
+      max_nb = std::max(max_nb, P.S / kBT);
+      d_at.push_back(take((int64_t)P.S * P.S * elem));
+      if (G.full) {
+        for (int c = 0; c < P.b; ++c) {
+          bpart.push_back((int)parts.size());
+          brow.push_back(P.out_row + c);
+        }
+        nb += P.b;
+        brows_at.push_back(-1);
+      } else {
+        // Partial item: SSSP rows (int32: the part's rows of the result).
+        brows_at.push_back(fp64 ? take((int64_t)P.b * G.ldr * 8) : -1);
+        for (int c = 0; c < P.b; ++c) srcs.push_back(BSrc{(int)parts.size(), c});
+      }
+      const double s = P.s, bb = P.b;
+      pcost.push_back((s - bb) * bb * d.n + (s - bb) * d.n);
+      parts.push_back(P);
+    }
+    G.nb = G.full ? nb : 0;
+    if (G.nb > 0) {
+      G.SB = (int)align_up(G.nb, kBT);
+      G.gb_row_begin = gb_rows;
+      gb_rows += G.SB;
+      gb_max_nb = std::max(gb_max_nb, G.SB / kBT);
+      gbg.push_back(gi);
+      mb_at.push_back(take((int64_t)G.SB * G.SB * elem));
+      r_at.push_back(fp64 ? take((int64_t)G.nb * G.ldr * 8) : -1);
+      for (int pi = (int)parts.size() - (int)ps.size(); pi < (int)parts.size(); ++pi)
+        for (int r0 = 0; r0 < G.nb; r0 += b->rows5) units5.push_back(BRowUnit{pi, r0});
+    } else {
+      mb_at.push_back(-1);
+      r_at.push_back(-1);
+    }
+    graphs.push_back(G);
+  }
+  // Merge entries: parts with inner rows, largest first.
+  std::vector<int> order;
+  for (int k = 0; k < (int)parts.size(); ++k)
+    if (parts[k].s > parts[k].b) order.push_back(k);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return pcost[x] > pcost[y]; });
+  // Units of a part: the row blocks of one column block consecutive (its boundary-row
+  // panel stays in L2 while they run).
+  std::vector<BMerge> merge;
+  for (int k : order) {
+    const BPart &P = parts[k];
+    const int rbs = (P.s - P.b + kMWaves * q.rows6 - 1) / (kMWaves * q.rows6);
+    const int cbs = (graphs[P.g].n + kMCols - 1) / kMCols;
+    FAP_REQUIRE((int64_t)merge.size() + (int64_t)rbs * cbs < (1LL << 30),
+                "fap_hip_batch_create: too many merge units");
+    for (int cb = 0; cb < cbs; ++cb)
+      for (int rb = 0; rb < rbs; ++rb) merge.push_back(BMerge{k, rb, cb, 0});
+  }
+  const int64_t units = (int64_t)merge.size();
+  FAP_REQUIRE(units5.size() < (1u << 31) && srcs.size() < (1u << 31),
+              "fap_hip_batch_create: batch too large");
+  // Tables after the scratch, one allocation.
+  auto table = [&](int64_t bytes) { return take(std::max<int64_t>(bytes, 16)); };
+  const int64_t o_graphs = table(sizeof(BGraph) * graphs.size());
+  const int64_t o_parts = table(sizeof(BPart) * parts.size());
+  const int64_t o_gbg = table(4 * gbg.size()), o_bpart = table(4 * bpart.size());
+  const int64_t o_pid = table(4 * pid2idx.size()), o_brow = table(8 * brow.size());
+  const int64_t o_u5 = table(sizeof(BRowUnit) * units5.size());
+  const int64_t o_merge = table(sizeof(BMerge) * merge.size());
+  const int64_t o_src = table(sizeof(BSrc) * srcs.size());
+  const int64_t o_fw = table((fp64 ? sizeof(Fw64Desc) : sizeof(FwDesc)) * parts.size());
+  const int64_t o_gbd = table((fp64 ? sizeof(Fw64Desc) : sizeof(FwDesc)) * gbg.size());
+  FAP_TRY(dev_alloc((void **)&q.d_mem, std::max<int64_t>(scratch, 256)));
+  char *base = q.d_mem;
+  for (size_t k = 0; k < parts.size(); ++k) {
+    BPart &P = parts[k];
+    P.D = base + d_at[k];
+    const BGraph &G = graphs[P.g];
+    if (brows_at[k] >= 0) {
+      P.Brows = base + brows_at[k];
+      P.ldb = G.ldr;
+    } else if (G.full && fp64) {
+      P.ldb = G.ldr;  // set below from the graph's R block
+    } else {
+      P.Brows = static_cast<int32_t *>(G.out) + P.out_row * G.ld_out;  // int32 result rows
+      P.ldb = G.ld_out;
+    }
+  }
+  for (size_t gi = 0; gi < graphs.size(); ++gi) {
+    BGraph &G = graphs[gi];
+    if (mb_at[gi] >= 0) G.MB = base + mb_at[gi];
+    if (r_at[gi] >= 0) G.R = base + r_at[gi];
+  }
+  for (BPart &P : parts) {
+    const BGraph &G = graphs[P.g];
+    if (G.full && fp64) P.Brows = static_cast<double *>(G.R) + (int64_t)P.bs * G.ldr;
+  }
+  std::vector<char> host(scratch - o_graphs);
+  auto put = [&](int64_t o, const void *src, size_t bytes) {
+    if (bytes) std::memcpy(host.data() + (o - o_graphs), src, bytes);
+  };
+  put(o_graphs, graphs.data(), sizeof(BGraph) * graphs.size());
+  put(o_parts, parts.data(), sizeof(BPart) * parts.size());
+  put(o_gbg, gbg.data(), 4 * gbg.size());
+  put(o_bpart, bpart.data(), 4 * bpart.size());
+  put(o_pid, pid2idx.data(), 4 * pid2idx.size());
+  put(o_brow, brow.data(), 8 * brow.size());
+  put(o_u5, units5.data(), sizeof(BRowUnit) * units5.size());
+  put(o_merge, merge.data(), sizeof(BMerge) * merge.size());
+  put(o_src, srcs.data(), sizeof(BSrc) * srcs.size());
+  if (fp64) {
+    std::vector<Fw64Desc> fd, gd;
+    for (const BPart &P : parts) fd.push_back(Fw64Desc{static_cast<double *>(P.D), P.S / kBT});
+    for (int gi : gbg)
+      gd.push_back(Fw64Desc{static_cast<double *>(graphs[gi].MB), graphs[gi].SB / kBT});
+    put(o_fw, fd.data(), sizeof(Fw64Desc) * fd.size());
+    put(o_gbd, gd.data(), sizeof(Fw64Desc) * gd.size());
+  } else {
+    std::vector<FwDesc> fd, gd;
+    for (const BPart &P : parts) fd.push_back(FwDesc{static_cast<int *>(P.D), P.S, P.S / kBT});
+    for (int gi : gbg)
+      gd.push_back(FwDesc{static_cast<int *>(graphs[gi].MB), graphs[gi].SB, graphs[gi].SB / kBT});
+    put(o_fw, fd.data(), sizeof(FwDesc) * fd.size());
+    put(o_gbd, gd.data(), sizeof(FwDesc) * gd.size());
+  }
+  if (hipMemcpy(base + o_graphs, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("fap_hip_batch_create: table upload failed");
+    return FAP_ERR_HIP;
+  }
+  q.ngraphs = (int)graphs.size();
+  q.nparts = (int)parts.size();
+  q.build_rows = build_rows;
+  q.max_nb = max_nb;
+  q.ngb = (int)gbg.size();
+  q.gb_rows = gb_rows;
+  q.gb_max_nb = gb_max_nb;
+  q.nunits5 = (int)units5.size();
+  q.nmerge = (int)merge.size();
+  q.merge_units = (int)units;
+  q.nsrc = (int)srcs.size();
+  q.graphs = reinterpret_cast<BGraph *>(base + o_graphs);
+  q.parts = reinterpret_cast<BPart *>(base + o_parts);
+  q.gbg = reinterpret_cast<int *>(base + o_gbg);
+  q.bpart = reinterpret_cast<int *>(base + o_bpart);
+  q.pid2idx = reinterpret_cast<int *>(base + o_pid);
+  q.brow = reinterpret_cast<int64_t *>(base + o_brow);
+  q.units5 = reinterpret_cast<BRowUnit *>(base + o_u5);
+  q.merge = reinterpret_cast<BMerge *>(base + o_merge);
+  q.srcs = reinterpret_cast<BSrc *>(base + o_src);
+  q.fw_descs = base + o_fw;
+  q.gb_descs = base + o_gbd;
+  return FAP_OK;
+}
+
+}  // namespace
+
 extern "C" int fap_hip_batch_destroy(fap_batch *b) {
   if (!b) return FAP_OK;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  for (auto &q : b->kind) {
-    if (q.d_work) dev_free(q.d_work);
-    if (q.d_src) dev_free(q.d_src);
-    if (q.d_scratch) dev_free(q.d_scratch);
-  }
+  for (auto &q : b->kind)
+    if (q.d_mem) dev_free(q.d_mem);
   if (b->d_counter) dev_free(b->d_counter);
   delete b;
   return FAP_OK;
@@ -482,13 +862,10 @@ extern "C" int fap_hip_batch_create(fap_batch **out, const fap_batch_item *items
     return FAP_ERR_STATE;
   }
   FAP_REQUIRE(out && nitems >= 0 && (nitems == 0 || items), "fap_hip_batch_create: null argument");
-  FAP_REQUIRE((flags & ~(FAP_BATCH_GRAPH | 0xFF00)) == 0, "fap_hip_batch_create: unknown flags %d",
-              flags);
-  const int per_cu = ((flags >> 8) & 0xFF) ? ((flags >> 8) & 0xFF) : 2;  // workgroups per CU
+  FAP_REQUIRE((flags & ~(FAP_BATCH_GRAPH | 0xFF00 | 0xF0000)) == 0,
+              "fap_hip_batch_create: unknown flags %d", flags);
   *out = nullptr;
-  std::vector<BatchWork> work[2];
-  std::vector<double> cost[2];
-  int smax[2] = {0, 0};
+  std::vector<const fap_batch_item *> kinds[2];
   for (int g = 0; g < nitems; ++g) {
     const fap_batch_item &it = items[g];
     const fap_graph_dev &d = it.g;
@@ -496,96 +873,16 @@ extern "C" int fap_hip_batch_create(fap_batch **out, const fap_batch_item *items
                     (d.m == 0 || d.d_col) && it.d_out && it.ld_out >= d.n &&
                     (it.nparts == 0 || it.parts),
                 "fap_hip_batch_create: bad item %d", g);
-    const int k = it.is_float ? 1 : 0;
-    for (int p = 0; p < it.nparts; ++p) {
-      const fap_part_desc &q = it.parts[p];
-      FAP_REQUIRE(q.part_id >= 1 && q.num >= 0 && q.bdy >= 0 && q.bdy <= q.num && q.offset >= 0 &&
-                      (int64_t)q.offset + q.num <= d.n && q.out_row >= 0,
-                  "fap_hip_batch_create: item %d part %d bad", g, p);
-      FAP_REQUIRE(q.num <= kBatchMaxS, "fap_hip_batch_create: part with %d vertices (limit %d)",
-                  q.num, kBatchMaxS);
-      if (q.num == 0) continue;
-      BatchWork w{d.d_row_offset, d.d_col, d.d_w, d.d_wf, d.d_graph_id, d.d_st2ed, d.d_ed2st,
-                  it.d_out, it.ld_out, nullptr, 0, q.out_row, d.n, q.part_id, q.offset, q.num,
-                  q.bdy, 0};
-      work[k].push_back(w);
-      const double s = q.num, b = q.bdy, inner = q.num - q.bdy;
-      cost[k].push_back(s * s * s + inner * b * d.n + b * (d.n + (double)d.m) * 4.0);
-      smax[k] = std::max(smax[k], (int)align_up(q.num, kBT));
-    }
+    kinds[it.is_float ? 1 : 0].push_back(&it);
   }
   fap_batch *b = new fap_batch;
   b->use_graph = (flags & FAP_BATCH_GRAPH) != 0;
+  b->per_cu = (flags >> 8) & 0xFF;  // merge workgroups per CU (0: as many as fit)
+  b->rows5 = pick_rows((flags >> 16) & 0x3, 16);
+  b->rows6 = pick_rows((flags >> 18) & 0x3, 16) == 8 ? 8 : 16;
   int s = dev_alloc((void **)&b->d_counter, 256);
-  static bool attr_set = false;
-  if (s == FAP_OK && !attr_set) {
-    if (hipFuncSetAttribute((const void *)batch_kernel<int>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes(4)) !=
-            hipSuccess ||
-        hipFuncSetAttribute((const void *)batch_kernel<double>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes(8)) !=
-            hipSuccess) {
-      set_error("fap_hip_batch_create: cannot raise the LDS limit");
-      s = FAP_ERR_HIP;
-    }
-    attr_set = s == FAP_OK;
-  }
-  int dev = 0, cus = 0;
-  if (s == FAP_OK) {
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (cus <= 0) cus = 256;
-  }
-  for (int k = 0; k < 2 && s == FAP_OK; ++k) {
-    const int nw = (int)work[k].size();
-    if (nw == 0) continue;
-    // Largest parts first (longest-processing-time order).
-    std::vector<int> order(nw);
-    for (int i = 0; i < nw; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int x, int y) { return cost[k][x] > cost[k][y]; });
-    std::vector<BatchWork> sorted(nw);
-    for (int i = 0; i < nw; ++i) sorted[i] = work[k][order[i]];
-    fap_batch_kind &q = b->kind[k];
-    const int elem = k ? 8 : 4;
-    q.nwork = nw;
-    q.grid = std::min(nw, per_cu * cus);
-    q.slot_bytes = align_up((int64_t)smax[k] * smax[k] * elem, 256);
-    // Scratch slots: at most 16 GiB in all (fewer resident workgroups for big parts).
-    q.grid = (int)std::max<int64_t>(1, std::min<int64_t>(q.grid, (16LL << 30) / q.slot_bytes));
-    // SSSP rows: the int32 path writes a part's boundary rows of the result in place;
-    // the double path gets b_p x roundup(n, 2) doubles per part after the slots.
-    const int64_t slots_bytes = q.slot_bytes * q.grid;
-    int64_t rows_bytes = 0;
-    std::vector<int64_t> rows_at(nw, 0);
-    std::vector<BatchSrc> srcs;
-    for (int i = 0; i < nw; ++i) {
-      BatchWork &w = sorted[i];
-      if (k) {
-        w.ldr = align_up(w.n, 2);
-        rows_at[i] = slots_bytes + rows_bytes;
-        rows_bytes += align_up((int64_t)w.bdy * w.ldr * 8, 256);
-      } else {
-        w.ldr = w.ld_out;
-        w.rows = static_cast<int32_t *>(w.out) + w.out_row * w.ld_out;
-      }
-      for (int j = 0; j < w.bdy; ++j) srcs.push_back(BatchSrc{i, j});
-    }
-    q.nsrc = (int)srcs.size();
-    s = dev_alloc((void **)&q.d_scratch, std::max<int64_t>(slots_bytes + rows_bytes, 256));
-    if (s == FAP_OK && k)
-      for (int i = 0; i < nw; ++i) sorted[i].rows = q.d_scratch + rows_at[i];
-    if (s == FAP_OK) s = dev_alloc((void **)&q.d_work, sizeof(BatchWork) * nw);
-    if (s == FAP_OK) s = dev_alloc((void **)&q.d_src, sizeof(BatchSrc) * std::max(q.nsrc, 1));
-    if (s == FAP_OK &&
-        (hipMemcpy(q.d_work, sorted.data(), sizeof(BatchWork) * nw, hipMemcpyHostToDevice) !=
-             hipSuccess ||
-         (q.nsrc > 0 && hipMemcpy(q.d_src, srcs.data(), sizeof(BatchSrc) * q.nsrc,
-                                  hipMemcpyHostToDevice) != hipSuccess))) {
-      set_error("fap_hip_batch_create: work table upload failed");
-      s = FAP_ERR_HIP;
-    }
-  }
+  for (int k = 0; k < 2 && s == FAP_OK; ++k)
+    if (!kinds[k].empty()) s = plan_kind(b, b->kind[k], kinds[k], k == 1);
   if (s == FAP_OK && b->use_graph) {
     hipStream_t st = nullptr;
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "fap_internal.h"
+#include "fw_desc.h"
 #include "gmem.h"
 
 namespace fap_hip {
@@ -186,11 +187,6 @@ __global__ __launch_bounds__(256) void build64_kernel(const Build64Desc *__restr
 }
 
 // ---------------------------------------------------------------- Floyd-Warshall
-struct Fw64Desc {
-  double *mat;  // S x S, ld = S = nb * 64
-  int nb;
-};
-
 // Tiles are addressed in the global address space (the descriptor pointer is generic;
 // flat accesses would also hold up the LDS counter of the staging writes).
 __device__ __forceinline__ FAP_GLOBAL double *tile64(const Fw64Desc &d, int ti, int tj) {
@@ -198,25 +194,43 @@ __device__ __forceinline__ FAP_GLOBAL double *tile64(const Fw64Desc &d, int ti, 
   return gptr(d.mat + (int64_t)ti * kT * S + tj * kT);
 }
 
-// Phase 1: the diagonal tile closed in LDS; 256 threads, 16 entries each.
+// Phase 1: the diagonal tile closed by 256 threads.  Wave w holds rows w, w + 4, ...,
+// w + 60 in registers, lane j their column j.  At step k, column k (the entries
+// D[i][k] of a wave's rows) comes from lane k by readlane and row k from LDS, where the
+// wave owning row k published it after step k - 1 (two buffers, one barrier per step).
+// Row k and column k do not change during step k (D[k][k] = 0, or the padded row and
+// column are +inf throughout), so this is the k-i-j loop.  An earlier form with 16
+// consecutive entries per thread and a 64-double LDS row stride hit every row of a wave
+// on the same banks.
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 __global__ __launch_bounds__(256) void fw64_phase1_kernel(const Fw64Desc *__restrict__ descs, int r) {
   const Fw64Desc d = descs[blockIdx.x];
   if (r >= d.nb) return;
   const int64_t S = (int64_t)d.nb * kT;
-  __shared__ double D[kT][kT];
+  __shared__ double Rk[2][kT];
   FAP_GLOBAL double *g = tile64(d, r, r);
-  const int row = threadIdx.x >> 2, c0 = (threadIdx.x & 3) * 16;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double v[16];
 #pragma unroll
-  for (int c = 0; c < 16; ++c) D[row][c0 + c] = g[row * S + c0 + c];
-  for (int k = 0; k < kT; ++k) {
-    __syncthreads();  // row k and column k do not change during step k
-    const double dik = D[row][k];
-#pragma unroll
-    for (int c = 0; c < 16; ++c) D[row][c0 + c] = fmin(D[row][c0 + c], dik + D[k][c0 + c]);
-  }
+  for (int q = 0; q < 16; ++q) v[q] = g[(wave + 4 * q) * S + lane];
+  if (wave == 0) Rk[0][lane] = v[0];
   __syncthreads();
 #pragma unroll
-  for (int c = 0; c < 16; ++c) g[row * S + c0 + c] = D[row][c0 + c];
+  for (int k = 0; k < kT; ++k) {
+    const double rk = Rk[k & 1][lane];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = fmin(v[q], readlane_f64(v[q], k) + rk);
+    if (k + 1 < kT && wave == ((k + 1) & 3)) Rk[(k + 1) & 1][lane] = v[(k + 1) >> 2];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) g[(wave + 4 * q) * S + lane] = v[q];
 }
 
 // Phases 2 and 3: one wave per 64 x 64 tile, 8 x 8 accumulators per lane (rows
@@ -305,6 +319,8 @@ __global__ __launch_bounds__(64) void fw64_phase3_kernel(const Fw64Desc *__restr
     for (int c = 0; c < 8; ++c) t[own_row64(tr, q) * S + own_row64(tc, c)] = acc[q][c];
 }
 
+}  // namespace
+
 // Phase 2 reads the row tile it writes only through its own workgroup and the
 // pivot tile, which phase 2 does not write; phase 3 reads panel tiles only.
 int launch_fw64(const Fw64Desc *d_descs, int nd, int max_nb, hipStream_t st) {
@@ -322,6 +338,8 @@ int launch_fw64(const Fw64Desc *d_descs, int nd, int max_nb, hipStream_t st) {
   }
   return FAP_OK;
 }
+
+namespace {
 
 // ---------------------------------------------------------------- merge
 // C[i][j] = min_k A[i][k] + B[k][j] (overwrite, +inf when K == 0) for the inner rows

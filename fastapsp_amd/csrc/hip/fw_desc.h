@@ -12,6 +12,12 @@ struct FwDesc {
   int nb;
 };
 
+// One FP64 matrix of the batched Floyd-Warshall (fp64.hip): S x S, ld = S = nb * 64.
+struct Fw64Desc {
+  double *mat;
+  int nb;
+};
+
 // Cap of the 16-bit Floyd-Warshall: an entry holds min(distance, kCap16).  Twice the
 // cap is 0x7C00, the bit pattern of +inf in IEEE half precision, so every stored
 // value and every sum of two stays a non-negative half whose bit pattern orders as

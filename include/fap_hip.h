@@ -332,18 +332,26 @@ typedef struct fap_batch_item {
   int64_t ld_out;
 } fap_batch_item;
 
-/* A prepared batch: the work table of every part of every graph (largest first),
- * one scratch slot per resident workgroup, and optionally the captured launch
- * sequence (FAP_BATCH_GRAPH: fap_hip_batch_run replays a hipGraph).  Running it is
- * one persistent launch per distance type (int32, FP64) in which workgroups pull
- * parts from a device counter and run a part's whole pipeline; parts of at most
- * 4096 vertices.  The item arrays are copied; device buffers must outlive the batch.
- * No counterpart in the reference (its solve() loops over parts of one graph,
+/* A prepared batch: the work tables of every part of every graph and optionally the
+ * captured launch sequence (FAP_BATCH_GRAPH: fap_hip_batch_run replays a hipGraph).
+ * Per distance type (int32, FP64) every stage is one launch over all graphs: build,
+ * batched Floyd-Warshall of the parts, and for items whose parts cover their graph
+ * the boundary graph (built, closed by Floyd-Warshall, then one min-plus pass writing
+ * the boundary rows); items with a subset of the parts take one SSSP per boundary
+ * vertex instead.  The merge of the inner rows is a persistent grid pulling (part, row
+ * block, column block) units from a device counter, largest parts first.  Parts of at
+ * most 8192 vertices.  The item arrays are copied; device buffers must outlive the
+ * batch.  No counterpart in the reference (its solve() loops over parts of one graph,
  * fap/fap.cpp:507-518). */
 typedef struct fap_batch fap_batch;
 #define FAP_BATCH_GRAPH 0x1
-/* flags bits 8..15: persistent workgroups per CU (0 = 2). */
+/* flags bits 8..15: persistent merge workgroups per CU (0 = as many as stay resident). */
 #define FAP_BATCH_PER_CU(x) (((x)&0xFF) << 8)
+/* flags bits 16..17: boundary sources per unit of the boundary-row product
+ * (0 = 16, 1 = 8, 2 = 16, 3 = 32); bits 18..19: inner rows per wave of the int32 merge
+ * (0 = 16, 1 = 8, 2 = 16). */
+#define FAP_BATCH_SRC_ROWS(code) (((code)&0x3) << 16)
+#define FAP_BATCH_MERGE_ROWS(code) (((code)&0x3) << 18)
 int fap_hip_batch_create(fap_batch **b, const fap_batch_item *items, int32_t nitems,
                          int32_t flags);
 int fap_hip_batch_run(fap_batch *b, void *stream);

@@ -121,11 +121,16 @@ def test_batch_without_graph_capture():
 def test_autotune_small_table():
     from fastapsp_amd import autotune as AT
     say("autotune (small candidate set)")
-    table = AT.tune(k_candidates=(2, 4), per_cu_candidates=(1, 2), max_copies=4)
+    table = AT.tune(k_candidates=(2, 4), per_cu_candidates=(0, 1), src_rows_candidates=(8, 16),
+                    merge_rows_candidates=(8, 16), max_copies=4)
     assert len(table["K"]) == len(AT.SIZE_CLASSES)
-    assert all(k in (2, 4) for k in table["K"]) and all(p in (1, 2) for p in table["per_cu"])
-    for trials, k, p, ms in zip(table["trials"], table["K"], table["per_cu"],
-                                table["ms_per_graph"]):
+    assert all(k in (2, 4) for k in table["K"]) and all(p in (0, 1) for p in table["per_cu"])
+    assert all(r in (8, 16) for r in table["src_rows"] + table["merge_rows"])
+    for c, (trials, ms) in enumerate(zip(table["trials"], table["ms_per_graph"])):
         assert ms == min(t["ms_per_graph"] for t in trials)
-        assert any(t["K"] == k and t["per_cu"] == p and t["ms_per_graph"] == ms for t in trials)
+        chosen = {k: table[k][c] for k in ("K",) + AT.KNOBS}
+        assert any(all(t[k] == v for k, v in chosen.items()) and t["ms_per_graph"] == ms
+                   for t in trials)
     assert AT.k_for(300, table) == table["K"][0] and AT.k_for(8000, table) == table["K"][-1]
+    s = AT.settings_for([8000, 8100, 300], table)
+    assert s == {k: table[k][-1] for k in AT.KNOBS}

@@ -79,7 +79,8 @@ def check_float_graph(g):
 @pytest.mark.timeout(900)
 def test_batch512_stated_workload():
     t0 = time.time()
-    table = AT.tune(k_candidates=(8, 16), per_cu_candidates=(1, 2), max_copies=16)
+    table = AT.tune(k_candidates=(8, 16), src_rows_candidates=(16,), merge_rows_candidates=(16,),
+                    max_copies=16)
     say(f"autotune {time.time() - t0:.0f}s: K per class {table['K']}")
     rng = np.random.default_rng(2026)
     sizes = np.exp(rng.uniform(np.log(256), np.log(8192), 512)).astype(int)
@@ -90,7 +91,7 @@ def test_batch512_stated_workload():
     ns = [g.num_vertexs for g in graphs]
     assert len(graphs) == 512 and min(ns) < 400 and max(ns) > 7000
     say(f"512 graphs, n in [{min(ns)}, {max(ns)}], sum n^2 = {sum(n * n for n in ns):.3e}")
-    b = F.Batch(graphs, use_graph=True, per_cu=AT.per_cu_for(ns, table))
+    b = F.Batch(graphs, use_graph=True, **AT.settings_for(ns, table))
     for rep in range(2):
         for g in graphs:
             p, ld = g.deviceDistance()
