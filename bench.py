@@ -201,14 +201,17 @@ def batch_bench(args):
     replay of the captured batch (one persistent launch per distance type)."""
     from fastapsp_amd import autotune as AT
     t0 = time.perf_counter()
-    if getattr(args, "tune_quick", False):
+    if getattr(args, "no_tune", False):  # fixed settings (profiling runs)
+        table = {"K": [8, 8, 8, 16, 16], "per_cu": [0] * 5, "src_rows": [32] * 5,
+                 "merge_rows": [8] * 5, "packed": [1] * 5, "ms_per_graph": [None] * 5}
+    elif getattr(args, "tune_quick", False):
         table = AT.tune(k_candidates=(8, 16), per_cu_candidates=(0,), src_rows_candidates=(16,),
-                        merge_rows_candidates=(16,), max_copies=16)
+                        merge_rows_candidates=(8, 16), max_copies=16)
     else:
         table = AT.tune(max_copies=32)
     log(f"autotune {time.perf_counter() - t0:.1f}s: K per class {table['K']}, "
         f"per_cu {table['per_cu']}, source rows {table['src_rows']}, "
-        f"merge rows {table['merge_rows']}")
+        f"merge rows {table['merge_rows']}, packed {table['packed']}")
     rng = np.random.default_rng(2026)
     sizes = np.exp(rng.uniform(np.log(256), np.log(8192), args.graphs)).astype(int)
     t0 = time.perf_counter()
@@ -270,7 +273,7 @@ def batch_bench(args):
                       "graphs": len(graphs), "sum_n2": entries, "n_min": min(ns),
                       "n_max": max(ns), "settings": settings},
            "autotune": {k: table[k] for k in ("K", "per_cu", "src_rows", "merge_rows",
-                                               "ms_per_graph")},
+                                               "packed", "ms_per_graph")},
            "plain_form": plain, "cpu_baseline": cpu}
     return out
 
@@ -359,6 +362,8 @@ def main():
     ap.add_argument("--paths", action="store_true",
                     help="also time solve(True): distances and predecessor matrices")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="batch512: fixed settings instead of the autotuner (profiling)")
     ap.add_argument("--skip-legs", action="store_true",
                     help="headline workload only (no kNN / min-plus / batch / serving legs)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)

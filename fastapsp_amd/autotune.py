@@ -10,7 +10,8 @@ fastest:
   * the boundary-source batch width: boundary-graph rows per unit of the boundary-row
     product (8, 16 or 32 sources sharing one pass over a part's in-part block);
   * the merge row tile: inner rows per wave of the int32 merge (8 or 16);
-  * the persistent merge workgroups per CU (0 = as many as stay resident).
+  * the persistent merge workgroups per CU (0 = as many as stay resident);
+  * the packed 16-bit merge of unit-weight graphs, on or off.
 K is chosen first with the default tiles, then each tile knob in turn (coordinate
 search).  The result is a small table (JSON-serialisable) that `k_for(n)` and
 `settings_for(ns)` read; `load_or_tune(path)` caches it on disk.
@@ -33,8 +34,9 @@ K_CANDIDATES = (2, 4, 8, 16, 32)
 PER_CU_CANDIDATES = (0, 2)
 SRC_ROWS_CANDIDATES = (8, 16, 32)
 MERGE_ROWS_CANDIDATES = (8, 16)
+PACKED_CANDIDATES = (1, 0)
 MAX_PART = 8192  # largest part of the batch form
-KNOBS = ("per_cu", "src_rows", "merge_rows")
+KNOBS = ("per_cu", "src_rows", "merge_rows", "packed")
 
 
 def symmetric_float_weights(ro, col, seed: int, directed: bool = False) -> np.ndarray:
@@ -84,13 +86,13 @@ def _time_batch(graphs, cfg: dict, reps: int = 2) -> float:
 def tune(copies_work: int = 512, max_copies: int = 48, seed: int = 4242,
          k_candidates=K_CANDIDATES, per_cu_candidates=PER_CU_CANDIDATES,
          src_rows_candidates=SRC_ROWS_CANDIDATES, merge_rows_candidates=MERGE_ROWS_CANDIDATES,
-         verbose: bool = False) -> dict:
+         packed_candidates=PACKED_CANDIDATES, verbose: bool = False) -> dict:
     """Per size class: K with the default tiles, then each knob in turn; the table
     keeps the fastest setting seen (every trial is recorded)."""
     table = {"classes": SIZE_CLASSES, "K": [], "per_cu": [], "src_rows": [], "merge_rows": [],
-             "ms_per_graph": [], "trials": []}
+             "packed": [], "ms_per_graph": [], "trials": []}
     cands = {"per_cu": tuple(per_cu_candidates), "src_rows": tuple(src_rows_candidates),
-             "merge_rows": tuple(merge_rows_candidates)}
+             "merge_rows": tuple(merge_rows_candidates), "packed": tuple(packed_candidates)}
     for ci, (lo, hi) in enumerate(SIZE_CLASSES):
         n_mid = int(math.sqrt(lo * min(hi, 8192)))
         trials = []
@@ -105,7 +107,7 @@ def tune(copies_work: int = 512, max_copies: int = 48, seed: int = 4242,
             if best is None or ms < best[0]:
                 best = (ms, K, dict(cfg), graphs, copies)
 
-        default = {k: cands[k][0] if k == "per_cu" else (16 if 16 in cands[k] else cands[k][0])
+        default = {k: 16 if k in ("src_rows", "merge_rows") and 16 in cands[k] else cands[k][0]
                    for k in KNOBS}
         for K in k_candidates:
             base = road_graph(n_mid, seed + ci, K)

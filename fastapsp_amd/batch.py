@@ -30,10 +30,12 @@ _ROWS_CODE = {0: 0, 8: 1, 16: 2, 32: 3}
 class Batch:
     """per_cu: persistent merge workgroups per CU (0: as many as stay resident);
     src_rows: boundary sources per unit of the boundary-row product (8, 16, 32; 0 =
-    16); merge_rows: inner rows per wave of the merge (8 or 16; 0 = 16, FP64 always 8)."""
+    16); merge_rows: inner rows per wave of the merge (8 or 16; 0 = 16, FP64 always 8);
+    packed: 1 = the int32 merge of unit-weight graphs (n <= 15872) on packed 16-bit
+    distances, 0 = on int32."""
 
     def __init__(self, graphs: Sequence, use_graph: bool = True, per_cu: int = 0,
-                 src_rows: int = 0, merge_rows: int = 0):
+                 src_rows: int = 0, merge_rows: int = 0, packed: int = 1):
         self.graphs = list(graphs)
         for g in self.graphs:
             if not getattr(g, "_preconditioned", False):
@@ -45,7 +47,8 @@ class Batch:
                                                 (L.FAP_BATCH_GRAPH if use_graph else 0)
                                                 | ((int(per_cu) & 0xFF) << 8)
                                                 | (_ROWS_CODE[int(src_rows)] << 16)
-                                                | (_ROWS_CODE[int(merge_rows)] << 18)),
+                                                | (_ROWS_CODE[int(merge_rows)] << 18)
+                                                | (0 if packed else L.FAP_BATCH_NO_PACKED)),
                 "batch create")
         self._h = h.value
 

@@ -38,6 +38,7 @@
 #include "fw_desc.h"
 #include "gmem.h"
 #include "mp_tile.h"
+#include "pk16.h"
 
 namespace fap_hip {
 
@@ -69,6 +70,8 @@ struct BGraph {
   int pid_base, max_pid;         // pid2idx[pid_base + part id] = part index, or -1
   int gb_row_begin;              // first boundary-graph build row
   int full;                      // 1: parts cover the graph (boundary-graph form)
+  int packed;                    // 1: merge on packed 16-bit distances (unit weights,
+                                 //    n <= kCap16: every finite distance below the cap)
 };
 
 // One part of one graph.
@@ -382,105 +385,153 @@ __global__ __launch_bounds__(256) void bconv_kernel(const BGraph *__restrict__ g
 // contiguous bytes.  The 4*RB x 64 A block is staged k-major in LDS (broadcast reads);
 // the four waves read the same B rows (L1).  Own columns take min(d_p(i, v), product).
 constexpr int kMWaves = 4;
-template <typename T, int RB>
-__global__ __launch_bounds__(256) void bmerge_kernel(const BGraph *__restrict__ graphs,
-                                                     const BPart *__restrict__ parts,
-                                                     const BMerge *__restrict__ units,
-                                                     int total_units, int *counter) {
+
+// One merge unit.  kPacked (int32, unit weights, n <= kCap16): A and B capped at kCap16,
+// two columns per register, a step on two entries one v_pk_add_u16 and half a
+// v_pk_minimum3_f16 (pk16.h); every finite distance is below the cap, so a result at
+// the cap is unreachable.
+template <typename T, int RB, bool kPacked>
+__device__ __forceinline__ void merge_unit(const BGraph &g, const BPart &p, int rb, int cb,
+                                           T (*As)[kMWaves * RB], int tid) {
   typedef typename Dist<T>::Out Out;
   constexpr int kRows = kMWaves * RB;
-  __shared__ __attribute__((aligned(16))) T As[kKc][kRows];
-  __shared__ int unit_s;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  while (true) {
-    if (tid == 0) unit_s = atomicAdd(counter, 1);
-    __syncthreads();
-    const int u = unit_s;
-    __syncthreads();  // every thread has read unit_s before thread 0 takes the next one
-    if (u >= total_units) break;  // uniform: every workgroup ends here
-    const BMerge m = units[u];
-    const BPart p = parts[m.p];
-    const BGraph g = graphs[p.g];
-    const int cb = m.cb, rb = m.rb;
-    const int inner = p.s - p.b, r0 = rb * kRows;
-    const FAP_GLOBAL T *D = gptr(static_cast<const T *>(p.D));
-    const FAP_GLOBAL T *B = gptr(static_cast<const T *>(p.Brows));
-    int col[4];
-    bool cok[4];
+  const int wave = tid >> 6, lane = tid & 63;
+  const int inner = p.s - p.b, r0 = rb * kRows;
+  const FAP_GLOBAL T *D = gptr(static_cast<const T *>(p.D));
+  const FAP_GLOBAL T *B = gptr(static_cast<const T *>(p.Brows));
+  int col[4];
+  bool cok[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      col[e] = cb * kMCols + e * 64 + lane;
-      cok[e] = col[e] < g.n;
-    }
-    T acc[RB][4];
+  for (int e = 0; e < 4; ++e) {
+    col[e] = cb * kMCols + e * 64 + lane;
+    cok[e] = col[e] < g.n;
+  }
+  constexpr int kAcc = kPacked ? 2 : 4;
+  T acc[RB][kAcc];
 #pragma unroll
-    for (int r = 0; r < RB; ++r)
+  for (int r = 0; r < RB; ++r)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) acc[r][e] = Dist<T>::inf();
-    for (int k0 = 0; k0 < p.b; k0 += kKc) {
-      const int kc = min(kKc, p.b - k0);
-      for (int x = tid; x < kKc * kRows; x += 256) {
-        const int r = x / kKc, k = x - r * kKc;
-        As[k][r] = (k < kc && r0 + r < inner) ? D[(int64_t)(p.b + r0 + r) * p.S + k0 + k]
-                                              : Dist<T>::inf();
+    for (int e = 0; e < kAcc; ++e) acc[r][e] = kPacked ? (T)kPk16Top : Dist<T>::inf();
+  for (int k0 = 0; k0 < p.b; k0 += kKc) {
+    const int kc = min(kKc, p.b - k0);
+    for (int x = tid; x < kKc * kRows; x += 256) {
+      const int r = x / kKc, k = x - r * kKc;
+      T a = (k < kc && r0 + r < inner) ? D[(int64_t)(p.b + r0 + r) * p.S + k0 + k]
+                                       : Dist<T>::inf();
+      if constexpr (kPacked) {
+        const unsigned u = (unsigned)a < (unsigned)kCap16 ? (unsigned)a : (unsigned)kCap16;
+        a = (T)(u * 0x10001u);
       }
-      __syncthreads();
-      const T *Aw = &As[0][wave * RB];
-      // B rows k and k + 1 of this chunk (INF past kc or past the last column); the
-      // next pair is loaded before the current pair is used.
-      auto load_pair = [&](int k, T (&x0)[4], T (&x1)[4]) {
-        const FAP_GLOBAL T *r0p = B + (int64_t)(k0 + k) * p.ldb;
+      As[k][r] = a;
+    }
+    __syncthreads();
+    const T *Aw = &As[0][wave * RB];
+    // B rows k and k + 1 of this chunk (INF past kc or past the last column).
+    auto load_pair = [&](int k, T (&x0)[4], T (&x1)[4]) {
+      const FAP_GLOBAL T *r0p = B + (int64_t)(k0 + k) * p.ldb;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          x0[e] = (cok[e] && k < kc) ? r0p[col[e]] : Dist<T>::inf();
-          x1[e] = (cok[e] && k + 1 < kc) ? r0p[p.ldb + col[e]] : Dist<T>::inf();
-        }
-      };
+      for (int e = 0; e < 4; ++e) {
+        x0[e] = (cok[e] && k < kc) ? r0p[col[e]] : Dist<T>::inf();
+        x1[e] = (cok[e] && k + 1 < kc) ? r0p[p.ldb + col[e]] : Dist<T>::inf();
+      }
+    };
+    for (int k = 0; k < kc; k += 2) {
       T b0[4], b1[4];
-      load_pair(0, b0, b1);
-      for (int k = 0; k < kc; k += 2) {
-        T n0[4], n1[4];
-        load_pair(k + 2, n0, n1);
-        T a0[RB], a1[RB];
+      load_pair(k, b0, b1);
+      T a0[RB], a1[RB];
 #pragma unroll
-        for (int r = 0; r < RB; ++r) {
-          a0[r] = Aw[k * kRows + r];
-          a1[r] = Aw[(k + 1) * kRows + r];  // row kc of As is INF when kc is odd
-        }
+      for (int r = 0; r < RB; ++r) {
+        a0[r] = Aw[k * kRows + r];
+        a1[r] = Aw[(k + 1) * kRows + r];  // row kc of As is INF when kc is odd
+      }
+      if constexpr (kPacked) {
+        auto cap = [](T v) -> unsigned {
+          const unsigned u = (unsigned)v;
+          return u < (unsigned)kCap16 ? u : (unsigned)kCap16;
+        };
+        const unsigned p0[2] = {cap(b0[0]) | (cap(b0[1]) << 16u), cap(b0[2]) | (cap(b0[3]) << 16u)};
+        const unsigned p1[2] = {cap(b1[0]) | (cap(b1[1]) << 16u), cap(b1[2]) | (cap(b1[3]) << 16u)};
+#pragma unroll
+        for (int r = 0; r < RB; ++r)
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const unsigned s0 = __builtin_bit_cast(
+                unsigned, (us2)(__builtin_bit_cast(us2, (unsigned)a0[r]) + __builtin_bit_cast(us2, p0[h])));
+            const unsigned s1 = __builtin_bit_cast(
+                unsigned, (us2)(__builtin_bit_cast(us2, (unsigned)a1[r]) + __builtin_bit_cast(us2, p1[h])));
+            acc[r][h] = (T)pk_min3((unsigned)acc[r][h], s0, s1);
+          }
+      } else {
 #pragma unroll
         for (int r = 0; r < RB; ++r)
 #pragma unroll
           for (int e = 0; e < 4; ++e)
             acc[r][e] = Dist<T>::mn3(acc[r][e], a0[r] + b0[e], a1[r] + b1[e]);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          b0[e] = n0[e];
-          b1[e] = n1[e];
-        }
       }
-      __syncthreads();
     }
-    int own[4];
+    __syncthreads();
+  }
+  int own[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    own[e] = cok[e] && g.graph_id[col[e]] == p.part_id ? g.ed2st[col[e]] - p.off : -1;
+    if (own[e] >= p.s) own[e] = -1;
+  }
+  FAP_GLOBAL Out *out = gptr(static_cast<Out *>(g.out));
+#pragma unroll
+  for (int r = 0; r < RB; ++r) {
+    const int i = r0 + wave * RB + r;
+    if (i >= inner) continue;
+    FAP_GLOBAL Out *orow = out + (p.out_row + p.b + i) * g.ld_out;
+    const FAP_GLOBAL T *drow = D + (int64_t)(p.b + i) * p.S;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      own[e] = cok[e] && g.graph_id[col[e]] == p.part_id ? g.ed2st[col[e]] - p.off : -1;
-      if (own[e] >= p.s) own[e] = -1;
+      if (!cok[e]) continue;
+      T val;
+      if constexpr (kPacked) {
+        const unsigned h = ((unsigned)acc[r][e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
+        val = h >= (unsigned)kCap16 ? Dist<T>::inf() : (T)h;
+      } else {
+        val = acc[r][e];
+      }
+      if (own[e] >= 0) val = Dist<T>::mn(val, drow[own[e]]);
+      orow[col[e]] = Dist<T>::out(val);
     }
-    FAP_GLOBAL Out *out = gptr(static_cast<Out *>(g.out));
-#pragma unroll
-    for (int r = 0; r < RB; ++r) {
-      const int i = r0 + wave * RB + r;
-      if (i >= inner) continue;
-      FAP_GLOBAL Out *orow = out + (p.out_row + p.b + i) * g.ld_out;
-      const FAP_GLOBAL T *drow = D + (int64_t)(p.b + i) * p.S;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        if (!cok[e]) continue;
-        T val = acc[r][e];
-        if (own[e] >= 0) val = Dist<T>::mn(val, drow[own[e]]);
-        orow[col[e]] = Dist<T>::out(val);
+  }
+}
+
+// Persistent grid; a unit is (part, row block of 4 * RB inner rows, column block of 256
+// columns), the row blocks of one column block consecutive so its boundary-row panel
+// is reused from L2.  Wave w owns rows w*RB .. w*RB + RB - 1 of the block and lane l
+// the columns l + 64e (e < 4), so every load and store of a wave covers 256
+// contiguous bytes.  The 4*RB x 64 A block is staged k-major in LDS (broadcast reads);
+// the four waves read the same B rows (L1).  Own columns take min(d_p(i, v), product).
+template <typename T, int RB>
+__global__ __launch_bounds__(256) void bmerge_kernel(const BGraph *__restrict__ graphs,
+                                                     const BPart *__restrict__ parts,
+                                                     const BMerge *__restrict__ units,
+                                                     int total_units, int *counter) {
+  __shared__ __attribute__((aligned(16))) T As[kKc][kMWaves * RB];
+  __shared__ int unit_s;
+  const int tid = threadIdx.x;
+  while (true) {
+    if (tid == 0) unit_s = atomicAdd(counter, 1);
+    __syncthreads();
+    // Uniform by construction; readfirstlane lets the compiler keep the unit's
+    // descriptors and pointers in scalar registers (scalar loads, scalar address math).
+    const int u = __builtin_amdgcn_readfirstlane(unit_s);
+    __syncthreads();  // every thread has read unit_s before thread 0 takes the next one
+    if (u >= total_units) break;  // uniform: every workgroup ends here
+    const BMerge m = units[u];
+    const BPart p = parts[m.p];
+    const BGraph g = graphs[p.g];
+    if constexpr (sizeof(T) == 4) {
+      if (g.packed) {
+        merge_unit<T, RB, true>(g, p, m.rb, m.cb, As, tid);
+        continue;
       }
     }
+    merge_unit<T, RB, false>(g, p, m.rb, m.cb, As, tid);
   }
 }
 
@@ -527,6 +578,7 @@ struct fap_batch {
   fap_batch_kind kind[2];  // 0 int32, 1 FP64
   int *d_counter = nullptr;
   int per_cu = 0, rows5 = 16, rows6 = 16;
+  bool packed = true;  // packed 16-bit merge where exact
   bool use_graph = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -647,6 +699,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
     G.ld_out = it->ld_out;
     G.n = d.n;
     G.ldr = align_up(d.n, 2);
+    G.packed = !fp64 && b->packed && d.d_w == nullptr && d.d_wf == nullptr && d.n <= kCap16;
     G.bbase = (int)bpart.size();
     G.pid_base = (int)pid2idx.size();
     const int gi = (int)graphs.size();
@@ -862,7 +915,7 @@ extern "C" int fap_hip_batch_create(fap_batch **out, const fap_batch_item *items
     return FAP_ERR_STATE;
   }
   FAP_REQUIRE(out && nitems >= 0 && (nitems == 0 || items), "fap_hip_batch_create: null argument");
-  FAP_REQUIRE((flags & ~(FAP_BATCH_GRAPH | 0xFF00 | 0xF0000)) == 0,
+  FAP_REQUIRE((flags & ~(FAP_BATCH_GRAPH | 0xFF00 | 0xF0000 | FAP_BATCH_NO_PACKED)) == 0,
               "fap_hip_batch_create: unknown flags %d", flags);
   *out = nullptr;
   std::vector<const fap_batch_item *> kinds[2];
@@ -880,6 +933,7 @@ extern "C" int fap_hip_batch_create(fap_batch **out, const fap_batch_item *items
   b->per_cu = (flags >> 8) & 0xFF;  // merge workgroups per CU (0: as many as fit)
   b->rows5 = pick_rows((flags >> 16) & 0x3, 16);
   b->rows6 = pick_rows((flags >> 18) & 0x3, 16) == 8 ? 8 : 16;
+  b->packed = (flags & FAP_BATCH_NO_PACKED) == 0;
   int s = dev_alloc((void **)&b->d_counter, 256);
   for (int k = 0; k < 2 && s == FAP_OK; ++k)
     if (!kinds[k].empty()) s = plan_kind(b, b->kind[k], kinds[k], k == 1);

@@ -233,44 +233,69 @@ __global__ __launch_bounds__(256) void fw64_phase1_kernel(const Fw64Desc *__rest
   for (int q = 0; q < 16; ++q) g[(wave + 4 * q) * S + lane] = v[q];
 }
 
-// Phases 2 and 3: one wave per 64 x 64 tile, 8 x 8 accumulators per lane (rows
-// tr*4 + {0..3}, 32 + tr*4 + {0..3}; columns likewise with tc), k in LDS chunks
-// of 16: As[k][i] (transposed A), Bs[k][j].
+// Phases 2 and 3: one 256-thread workgroup per 64 x 64 tile; thread t owns rows
+// (t / 16) * 4 + {0..3} and columns (t % 16) + 16 * {0..3} (4 x 4 accumulators, so
+// ~64 VGPRs and many waves per SIMD; a one-wave form with 8 x 8 doubles per lane used
+// 248-256 VGPRs and ran at one or two waves per SIMD).  k in LDS chunks of 16:
+// As[k][i] (transposed A, four consecutive rows read as two 16-byte broadcasts),
+// Bs[k][j] (16 consecutive lanes read 16 consecutive doubles: conflict-free).
 constexpr int kKc64 = 16;
 
-__device__ __forceinline__ int own_row64(int tr, int q) { return (q < 4 ? 0 : 32) + tr * 4 + (q & 3); }
-
-__device__ __forceinline__ void tile_product64(double (&acc)[8][8], const FAP_GLOBAL double *a,
-                                               const FAP_GLOBAL double *b,
-                                               int64_t S, double (*As)[kT], double (*Bs)[kT], int tr,
+__device__ __forceinline__ void tile_product64(double (&acc)[4][4], const FAP_GLOBAL double *a,
+                                               const FAP_GLOBAL double *b, int64_t S,
+                                               double (*As)[kT], double (*Bs)[kT], int tr,
                                                int tc) {
-  const int lane = threadIdx.x;
+  const int t = threadIdx.x;
   for (int k0 = 0; k0 < kT; k0 += kKc64) {
     if (k0 > 0) __syncthreads();
+    // 16 x 64 entries of each operand, four per thread: A row-major (i = e / 16),
+    // B row-major (k = e / 64).
 #pragma unroll
-    for (int q = 0; q < kKc64; ++q) As[q][lane] = a[lane * S + k0 + q];
-#pragma unroll
-    for (int q = 0; q < kKc64; ++q) Bs[q][lane] = b[(k0 + q) * S + lane];
+    for (int q = 0; q < 4; ++q) {
+      const int e = t + 256 * q;
+      const int ai = e >> 4, ak = e & 15;
+      As[ak][ai] = a[ai * S + k0 + ak];
+      const int bk = e >> 6, bj = e & 63;
+      Bs[bk][bj] = b[(k0 + bk) * S + bj];
+    }
     __syncthreads();
 #pragma unroll 4
     for (int k = 0; k < kKc64; ++k) {
-      double av[8], bv[8];
+      double av[4], bv[4];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        av[q] = As[k][own_row64(tr, q)];
-        bv[q] = Bs[k][own_row64(tc, q)];
+      for (int q = 0; q < 4; ++q) {
+        av[q] = As[k][tr * 4 + q];
+        bv[q] = Bs[k][tc + 16 * q];
       }
 #pragma unroll
-      for (int q = 0; q < 8; ++q)
+      for (int q = 0; q < 4; ++q)
 #pragma unroll
-        for (int c = 0; c < 8; ++c) acc[q][c] = fmin(acc[q][c], av[q] + bv[c]);
+        for (int c = 0; c < 4; ++c) acc[q][c] = fmin(acc[q][c], av[q] + bv[c]);
     }
   }
 }
 
 __device__ __forceinline__ int skip64(int o, int p) { return o < p ? o : o + 1; }
 
-__global__ __launch_bounds__(64) void fw64_phase2_kernel(const Fw64Desc *__restrict__ descs, int r) {
+__device__ __forceinline__ void load_tile64(double (&acc)[4][4], const FAP_GLOBAL double *t,
+                                            int64_t S, int tr, int tc) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[q][c] = t[(tr * 4 + q) * S + tc + 16 * c];
+}
+
+__device__ __forceinline__ void store_tile64(const double (&acc)[4][4], FAP_GLOBAL double *t,
+                                             int64_t S, int tr, int tc) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) t[(tr * 4 + q) * S + tc + 16 * c] = acc[q][c];
+}
+
+// The tile is an operand of its own product: every thread stores only after the
+// barrier that follows the last chunk's staging, so no read sees a new value.
+__global__ __launch_bounds__(256) void fw64_phase2_kernel(const Fw64Desc *__restrict__ descs, int r) {
   const Fw64Desc d = descs[blockIdx.y];
   const int others = d.nb - 1;
   int o = blockIdx.x;
@@ -282,21 +307,15 @@ __global__ __launch_bounds__(64) void fw64_phase2_kernel(const Fw64Desc *__restr
   __shared__ double As[kKc64][kT];
   __shared__ double Bs[kKc64][kT];
   FAP_GLOBAL double *t = is_row ? tile64(d, r, o) : tile64(d, o, r);
-  const int tr = threadIdx.x >> 3, tc = threadIdx.x & 7;
-  double acc[8][8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q)
-#pragma unroll
-    for (int c = 0; c < 8; ++c) acc[q][c] = t[own_row64(tr, q) * S + own_row64(tc, c)];
+  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+  double acc[4][4];
+  load_tile64(acc, t, S, tr, tc);
   if (is_row) tile_product64(acc, tile64(d, r, r), t, S, As, Bs, tr, tc);
   else tile_product64(acc, t, tile64(d, r, r), S, As, Bs, tr, tc);
-#pragma unroll
-  for (int q = 0; q < 8; ++q)
-#pragma unroll
-    for (int c = 0; c < 8; ++c) t[own_row64(tr, q) * S + own_row64(tc, c)] = acc[q][c];
+  store_tile64(acc, t, S, tr, tc);
 }
 
-__global__ __launch_bounds__(64) void fw64_phase3_kernel(const Fw64Desc *__restrict__ descs, int r) {
+__global__ __launch_bounds__(256) void fw64_phase3_kernel(const Fw64Desc *__restrict__ descs, int r) {
   const Fw64Desc d = descs[blockIdx.y];
   const int others = d.nb - 1;
   const int o = blockIdx.x;
@@ -306,17 +325,11 @@ __global__ __launch_bounds__(64) void fw64_phase3_kernel(const Fw64Desc *__restr
   __shared__ double As[kKc64][kT];
   __shared__ double Bs[kKc64][kT];
   FAP_GLOBAL double *t = tile64(d, i, j);
-  const int tr = threadIdx.x >> 3, tc = threadIdx.x & 7;
-  double acc[8][8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q)
-#pragma unroll
-    for (int c = 0; c < 8; ++c) acc[q][c] = t[own_row64(tr, q) * S + own_row64(tc, c)];
+  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+  double acc[4][4];
+  load_tile64(acc, t, S, tr, tc);
   tile_product64(acc, tile64(d, i, r), tile64(d, r, j), S, As, Bs, tr, tc);
-#pragma unroll
-  for (int q = 0; q < 8; ++q)
-#pragma unroll
-    for (int c = 0; c < 8; ++c) t[own_row64(tr, q) * S + own_row64(tc, c)] = acc[q][c];
+  store_tile64(acc, t, S, tr, tc);
 }
 
 }  // namespace
@@ -329,9 +342,9 @@ int launch_fw64(const Fw64Desc *d_descs, int nd, int max_nb, hipStream_t st) {
     FAP_CHECK_LAUNCH();
     const int others = max_nb - 1;
     if (others > 0) {
-      hipLaunchKernelGGL(fw64_phase2_kernel, dim3(2 * others, nd), dim3(64), 0, st, d_descs, r);
+      hipLaunchKernelGGL(fw64_phase2_kernel, dim3(2 * others, nd), dim3(256), 0, st, d_descs, r);
       FAP_CHECK_LAUNCH();
-      hipLaunchKernelGGL(fw64_phase3_kernel, dim3(others * others, nd), dim3(64), 0, st, d_descs,
+      hipLaunchKernelGGL(fw64_phase3_kernel, dim3(others * others, nd), dim3(256), 0, st, d_descs,
                          r);
       FAP_CHECK_LAUNCH();
     }

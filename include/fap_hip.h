@@ -352,6 +352,10 @@ typedef struct fap_batch fap_batch;
  * (0 = 16, 1 = 8, 2 = 16). */
 #define FAP_BATCH_SRC_ROWS(code) (((code)&0x3) << 16)
 #define FAP_BATCH_MERGE_ROWS(code) (((code)&0x3) << 18)
+/* The int32 merge of unit-weight graphs with n <= 15872 vertices runs on packed 16-bit
+ * distances (exact: every finite distance is below the 16-bit cap) unless this flag
+ * is set. */
+#define FAP_BATCH_NO_PACKED 0x100000
 int fap_hip_batch_create(fap_batch **b, const fap_batch_item *items, int32_t nitems,
                          int32_t flags);
 int fap_hip_batch_run(fap_batch *b, void *stream);

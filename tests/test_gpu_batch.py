@@ -122,7 +122,7 @@ def test_autotune_small_table():
     from fastapsp_amd import autotune as AT
     say("autotune (small candidate set)")
     table = AT.tune(k_candidates=(2, 4), per_cu_candidates=(0, 1), src_rows_candidates=(8, 16),
-                    merge_rows_candidates=(8, 16), max_copies=4)
+                    merge_rows_candidates=(8, 16), packed_candidates=(1, 0), max_copies=4)
     assert len(table["K"]) == len(AT.SIZE_CLASSES)
     assert all(k in (2, 4) for k in table["K"]) and all(p in (0, 1) for p in table["per_cu"])
     assert all(r in (8, 16) for r in table["src_rows"] + table["merge_rows"])
