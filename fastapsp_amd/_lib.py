@@ -102,6 +102,7 @@ SIGNATURES = {
     "fap_hip_sssp_batch_path": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _vp, _i32,
                                           _vp, _vp, _i64, _vp]),
     "fap_hip_f32_to_i32": (C.c_int, [_vp, _vp, _i64, _vp]),
+    "fap_hip_check_layout": (C.c_int, [_vp, _vp, _i32, C.POINTER(_i32)]),
     "fap_hip_i32_to_f32": (C.c_int, [_vp, _vp, _i64, _vp]),
     "fap_hip_minplus": (C.c_int, [_vp, _i64, _vp, _i64, _vp, _i64, _i32, _i32, _i32, _vp]),
     "fap_hip_submat_build": (C.c_int, [_vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp,

@@ -224,7 +224,9 @@ __global__ __launch_bounds__(64 * kSWaves) void bsssp_kernel(const BGraph *__res
   const BPart p = parts[sr.p];
   const BGraph g = graphs[p.g];
   T *row = static_cast<T *>(const_cast<void *>(p.Brows)) + (int64_t)sr.q * p.ldb;
-  wave_sssp<T>(g, g.st2ed[p.off + sr.q], row, queues[wave], queues[wave] + 2 * kBQ, lane);
+  const int s = g.st2ed[p.off + sr.q];
+  if (s < 0 || s >= g.n) return;  // a corrupt st2ed entry: no row
+  wave_sssp<T>(g, s, row, queues[wave], queues[wave] + 2 * kBQ, lane);
 }
 
 // ---------------------------------------------------------------- 1: build
@@ -353,8 +355,8 @@ __global__ __launch_bounds__(256) void brows_kernel(const BGraph *__restrict__ g
       }
       __syncthreads();
     }
-    if (jok) {
-      const int v = g.st2ed[q.off + j];
+    const int v = jok ? g.st2ed[q.off + j] : -1;
+    if (v >= 0 && v < g.n) {  // (a corrupt st2ed entry writes nothing)
 #pragma unroll
       for (int r = 0; r < RB; ++r) {
         if (r0 + r >= g.nb) continue;

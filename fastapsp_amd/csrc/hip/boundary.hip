@@ -64,13 +64,13 @@ __device__ __forceinline__ int find_part_by_bs(const BgPart *__restrict__ parts,
 
 // G_B target of cut arc e out of a vertex of part `pid`, or -1 for an in-part arc.
 __device__ __forceinline__ int cut_target(int e, int pid, const BgPart *__restrict__ parts,
-                                          const int *__restrict__ pidx_by_pid,
+                                          const int *__restrict__ pidx_by_pid, int npid,
                                           const int *__restrict__ col,
                                           const int *__restrict__ graph_id,
                                           const int *__restrict__ ed2st) {
   const int u = col[e];
   const int pu = graph_id[u];
-  if (pu == pid) return -1;
+  if (pu == pid || pu < 0 || pu >= npid) return -1;  // in-part arc, or a label outside the table
   const BgPart q = parts[pidx_by_pid[pu]];
   const int lk = ed2st[u] - q.offset;
   return (lk >= 0 && lk < q.bdy) ? q.bs + lk : -1;
@@ -80,7 +80,7 @@ __device__ __forceinline__ int cut_target(int e, int pid, const BgPart *__restri
 // (wt[woff + c] = d_q(c, k), column k of the part's boundary block) and the output
 // row that holds its distance row.
 __global__ void bg_vert_kernel(const BgPart *__restrict__ parts, int np, int nb_total,
-                               const int *__restrict__ pidx_by_pid, const int *__restrict__ row_off,
+                               const int *__restrict__ pidx_by_pid, int npid, const int *__restrict__ row_off,
                                const int *__restrict__ col, const int *__restrict__ w,
                                const int *__restrict__ graph_id, const int *__restrict__ st2ed,
                                const int *__restrict__ ed2st, BgVert *__restrict__ verts,
@@ -98,7 +98,7 @@ __global__ void bg_vert_kernel(const BgPart *__restrict__ parts, int np, int nb_
   int nc = 0;
   const int v = st2ed[p.offset + k];
   for (int e = row_off[v]; e < row_off[v + 1]; ++e) {
-    const int t = cut_target(e, p.part_id, parts, pidx_by_pid, col, graph_id, ed2st);
+    const int t = cut_target(e, p.part_id, parts, pidx_by_pid, npid, col, graph_id, ed2st);
     if (t < 0) continue;
     if (nc < kBgCutW) {
       bv.cut_t[nc] = t;
@@ -129,7 +129,7 @@ constexpr int kClPad = 8;  // slack entries after the clique weights and the LDS
 template <int VPT>
 __global__ __launch_bounds__(kClThreads) void bg_closure_kernel(
     const BgVert *__restrict__ verts, const int *__restrict__ wt, int nb_total, int Sb,
-    const BgPart *__restrict__ parts, int np, const int *__restrict__ pidx_by_pid,
+    const BgPart *__restrict__ parts, int np, const int *__restrict__ pidx_by_pid, int npid,
     const int *__restrict__ row_off, const int *__restrict__ col, const int *__restrict__ w,
     const int *__restrict__ graph_id, const int *__restrict__ st2ed,
     const int *__restrict__ ed2st, int *__restrict__ out) {
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(kClThreads) void bg_closure_kernel(
         const BgPart p = parts[find_part_by_bs(parts, np, u)];
         const int vtx = st2ed[p.offset + (u - p.bs)];
         for (int e = row_off[vtx]; e < row_off[vtx + 1]; ++e) {
-          const int t = cut_target(e, p.part_id, parts, pidx_by_pid, col, graph_id, ed2st);
+          const int t = cut_target(e, p.part_id, parts, pidx_by_pid, npid, col, graph_id, ed2st);
           if (t < 0) continue;
           const int nd = du + (w ? w[e] : 1);
           if (nd < dist[t] && nd < atomicMin(&dist[t], nd)) changed = true;
@@ -228,7 +228,7 @@ constexpr int kClwThreads = 512;
 template <int VPT, int NS>
 __global__ __launch_bounds__(kClwThreads) void bg_closure_lds_kernel(
     const BgVert *__restrict__ verts, const int *__restrict__ wt, int wlen, int nb_total, int Sb,
-    const BgPart *__restrict__ parts, int np, const int *__restrict__ pidx_by_pid,
+    const BgPart *__restrict__ parts, int np, const int *__restrict__ pidx_by_pid, int npid,
     const int *__restrict__ row_off, const int *__restrict__ col,
     const int *__restrict__ graph_id, const int *__restrict__ st2ed,
     const int *__restrict__ ed2st, int cb_total, int *__restrict__ out) {
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(kClwThreads) void bg_closure_lds_kernel(
           const BgPart p = parts[find_part_by_bs(parts, np, u)];
           const int vtx = st2ed[p.offset + (u - p.bs)];
           for (int e = row_off[vtx]; e < row_off[vtx + 1]; ++e) {
-            const int t = cut_target(e, p.part_id, parts, pidx_by_pid, col, graph_id, ed2st);
+            const int t = cut_target(e, p.part_id, parts, pidx_by_pid, npid, col, graph_id, ed2st);
             if (t < 0) continue;
             if (nd < dist[t] && nd < atomicMin(&dist[t], nd)) {
               changed = true;
@@ -402,12 +402,16 @@ struct ColInfo {  // one output column: its part's G_B offset and in-part column
 // A column past n gets part 0's matrix with no boundary (every fin entry FAP_INF),
 // so its loads stay in bounds.
 __device__ __forceinline__ ColInfo column_info(const BgPart *__restrict__ parts,
-                                               const int *__restrict__ pidx_by_pid,
+                                               const int *__restrict__ pidx_by_pid, int npid,
                                                const int *__restrict__ graph_id,
                                                const int *__restrict__ ed2st, int j, int n) {
-  if (j < n) {
-    const BgPart q = parts[pidx_by_pid[graph_id[j]]];
-    return ColInfo{q.bs, q.bdy, q.S, ed2st[j] - q.offset, q.fw};
+  // A label outside the table, or a sorted position outside its part's padded matrix
+  // (inconsistent layouts from a C-ABI caller), reads like a column past n: in bounds.
+  const int pid = j < n ? graph_id[j] : -1;
+  if (pid >= 0 && pid < npid) {
+    const BgPart q = parts[pidx_by_pid[pid]];
+    const int l = ed2st[j] - q.offset;
+    if (l >= 0 && l < q.S) return ColInfo{q.bs, q.bdy, q.S, l, q.fw};
   }
   return ColInfo{0, 0, parts[0].S, 0, parts[0].fw};
 }
@@ -517,7 +521,7 @@ __device__ __forceinline__ void bg_rows_pick(int wb, const int *lds, int ldl,
 template <int KP, bool kSym>
 __global__ __launch_bounds__(256) void bg_rows_kernel(
     const BgPart *__restrict__ parts, const int64_t *__restrict__ orow, int nb_total,
-    const int *__restrict__ mat, int Sb, int R, const int *__restrict__ pidx_by_pid,
+    const int *__restrict__ mat, int Sb, int R, const int *__restrict__ pidx_by_pid, int npid,
     const int *__restrict__ graph_id, const int *__restrict__ ed2st, int n,
     int *__restrict__ out, int64_t ld_out, int *__restrict__ max_b) {
   extern __shared__ int lds[];
@@ -530,7 +534,7 @@ __global__ __launch_bounds__(256) void bg_rows_kernel(
   int fin[4][KP];
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
-    const ColInfo ci = column_info(parts, pidx_by_pid, graph_id, ed2st, jb + e * 256, n);
+    const ColInfo ci = column_info(parts, pidx_by_pid, npid, graph_id, ed2st, jb + e * 256, n);
     bs[e] = ci.bs;
     if (kSym) load_fin_sym<KP>(fin[e], ci);
     else load_fin<KP>(fin[e], ci, 0);
@@ -563,7 +567,7 @@ __global__ __launch_bounds__(256) void bg_rows_kernel(
 template <int KC, bool kSym>
 __global__ __launch_bounds__(256) void bg_rows_wide_kernel(
     const BgPart *__restrict__ parts, const int64_t *__restrict__ orow, int nb_total,
-    const int *__restrict__ mat, int Sb, int R, int nchunks, const int *__restrict__ pidx_by_pid,
+    const int *__restrict__ mat, int Sb, int R, int nchunks, const int *__restrict__ pidx_by_pid, int npid,
     const int *__restrict__ graph_id, const int *__restrict__ ed2st, int n,
     int *__restrict__ out, int64_t ld_out, int *__restrict__ max_b) {
   extern __shared__ int lds[];
@@ -575,7 +579,7 @@ __global__ __launch_bounds__(256) void bg_rows_wide_kernel(
   ColInfo ci[4];
 #pragma unroll
   for (int e = 0; e < 4; ++e)
-    ci[e] = column_info(parts, pidx_by_pid, graph_id, ed2st, jb + e * 256, n);
+    ci[e] = column_info(parts, pidx_by_pid, npid, graph_id, ed2st, jb + e * 256, n);
   int mx = 0;  // largest finite distance written by this thread
   for (int g0 = gbeg; g0 < gend; g0 += R) {
     const int rows = min(R, gend - g0);
@@ -633,13 +637,13 @@ __global__ __launch_bounds__(256) void bg_rows_wide_kernel(
 
 }  // namespace
 
-int launch_bg_vert(const BgPart *d_parts, int np, int nb_total, const int *d_pidx_by_pid,
+int launch_bg_vert(const BgPart *d_parts, int np, int nb_total, const int *d_pidx_by_pid, int npid,
                    const int *row_off, const int *col, const int *w, const int *graph_id,
                    const int *st2ed, const int *ed2st, BgVert *d_verts, int *d_wt,
                    int64_t *d_orow, hipStream_t stream) {
   if (nb_total <= 0) return FAP_OK;
   hipLaunchKernelGGL(bg_vert_kernel, dim3((nb_total + 255) / 256), dim3(256), 0, stream, d_parts,
-                     np, nb_total, d_pidx_by_pid, row_off, col, w, graph_id, st2ed, ed2st,
+                     np, nb_total, d_pidx_by_pid, npid, row_off, col, w, graph_id, st2ed, ed2st,
                      d_verts, d_wt, d_orow);
   FAP_CHECK_LAUNCH();
   return FAP_OK;
@@ -647,7 +651,7 @@ int launch_bg_vert(const BgPart *d_parts, int np, int nb_total, const int *d_pid
 
 int launch_bg_closure(const BgVert *d_verts, const int *d_wt, int cb_total, bool unit,
                       int nb_total, int Sb, const BgPart *d_parts, int np,
-                      const int *d_pidx_by_pid, const int *row_off, const int *col, const int *w,
+                      const int *d_pidx_by_pid, int npid, const int *row_off, const int *col, const int *w,
                       const int *graph_id, const int *st2ed, const int *ed2st, int *d_mat,
                       hipStream_t stream) {
   if (nb_total <= 0) return FAP_OK;
@@ -671,7 +675,7 @@ int launch_bg_closure(const BgVert *d_verts, const int *d_wt, int cb_total, bool
     if (vptw <= V && ns == NS) {                                                              \
       hipLaunchKernelGGL((bg_closure_lds_kernel<V, NS>), dim3(grid), dim3(kClwThreads),        \
                          lds_of(NS), stream, d_verts, d_wt, wlen, nb_total, Sb, d_parts, np,  \
-                         d_pidx_by_pid, row_off, col, graph_id, st2ed, ed2st, cb_total,       \
+                         d_pidx_by_pid, npid, row_off, col, graph_id, st2ed, ed2st, cb_total,       \
                          d_mat);                                                              \
       FAP_CHECK_LAUNCH();                                                                     \
       return FAP_OK;                                                                          \
@@ -688,7 +692,7 @@ int launch_bg_closure(const BgVert *d_verts, const int *d_wt, int cb_total, bool
   if (vpt <= V) {                                                                             \
     hipLaunchKernelGGL(bg_closure_kernel<V>, dim3(nb_total), dim3(kClThreads),                 \
                        (size_t)(Sb + kClPad) * 4,                                              \
-                       stream, d_verts, d_wt, nb_total, Sb, d_parts, np, d_pidx_by_pid,        \
+                       stream, d_verts, d_wt, nb_total, Sb, d_parts, np, d_pidx_by_pid, npid,  \
                        row_off, col, w, graph_id, st2ed, ed2st, d_mat);                        \
     FAP_CHECK_LAUNCH();                                                                       \
     return FAP_OK;                                                                            \
@@ -708,7 +712,7 @@ int launch_bg_closure(const BgVert *d_verts, const int *d_wt, int cb_total, bool
 int bg_rows_chunk(int ldl) { return std::max(1, std::min(kRowsMax, 53248 / (ldl * 4))); }
 
 int launch_bg_rows(const BgPart *d_parts, const int64_t *d_orow, int nb_total, const int *d_mat,
-                   int Sb, int max_bdy, const int *d_pidx_by_pid, const int *graph_id,
+                   int Sb, int max_bdy, const int *d_pidx_by_pid, int npid, const int *graph_id,
                    const int *ed2st, int n, int *d_out, int64_t ld_out, int *d_max_b,
                    bool sym, hipStream_t stream) {
   if (nb_total <= 0 || n <= 0) return FAP_OK;
@@ -728,11 +732,11 @@ int launch_bg_rows(const BgPart *d_parts, const int64_t *d_orow, int nb_total, c
   case KPV:                                                                                   \
     if (sym)                                                                                  \
       hipLaunchKernelGGL((bg_rows_kernel<KPV, true>), grid, dim3(256), lds, stream, d_parts,   \
-                         d_orow, nb_total, d_mat, Sb, R, d_pidx_by_pid, graph_id, ed2st, n,    \
+                         d_orow, nb_total, d_mat, Sb, R, d_pidx_by_pid, npid, graph_id, ed2st, n,    \
                          d_out, ld_out, d_max_b);                                              \
     else                                                                                      \
       hipLaunchKernelGGL((bg_rows_kernel<KPV, false>), grid, dim3(256), lds, stream, d_parts,  \
-                         d_orow, nb_total, d_mat, Sb, R, d_pidx_by_pid, graph_id, ed2st, n,    \
+                         d_orow, nb_total, d_mat, Sb, R, d_pidx_by_pid, npid, graph_id, ed2st, n,    \
                          d_out, ld_out, d_max_b);                                              \
     break;
   if (narrow) {
@@ -749,11 +753,11 @@ int launch_bg_rows(const BgPart *d_parts, const int64_t *d_orow, int nb_total, c
     const int nchunks = (max_bdy + KC - 1) / KC;
     if (sym)
       hipLaunchKernelGGL((bg_rows_wide_kernel<16, true>), grid, dim3(256), lds, stream, d_parts,
-                         d_orow, nb_total, d_mat, Sb, R, nchunks, d_pidx_by_pid, graph_id, ed2st,
+                         d_orow, nb_total, d_mat, Sb, R, nchunks, d_pidx_by_pid, npid, graph_id, ed2st,
                          n, d_out, ld_out, d_max_b);
     else
       hipLaunchKernelGGL((bg_rows_wide_kernel<16, false>), grid, dim3(256), lds, stream, d_parts,
-                         d_orow, nb_total, d_mat, Sb, R, nchunks, d_pidx_by_pid, graph_id, ed2st,
+                         d_orow, nb_total, d_mat, Sb, R, nchunks, d_pidx_by_pid, npid, graph_id, ed2st,
                          n, d_out, ld_out, d_max_b);
   }
 #undef FAP_BG_CASE
@@ -842,12 +846,12 @@ int boundary_rows_unit(const fap_graph_dev *g, const fap_part_desc *act, int P, 
                             groups, s_max, g->d_row_offset, g->d_col, g->d_graph_id, g->d_st2ed,
                             g->d_ed2st, st));
   FAP_TRY(fap_hip_fill_i32(d_max, 0, 1, st));
-  FAP_TRY(launch_bg_vert(d_bgp, P, nb_total, d_pidx, g->d_row_offset, g->d_col, nullptr,
+  FAP_TRY(launch_bg_vert(d_bgp, P, nb_total, d_pidx, max_pid + 1, g->d_row_offset, g->d_col, nullptr,
                          g->d_graph_id, g->d_st2ed, g->d_ed2st, d_verts, d_wt, d_orow, st));
-  FAP_TRY(launch_bg_closure(d_verts, d_wt, cb, s_max < 0xFFFF, nb_total, Sb, d_bgp, P, d_pidx,
+  FAP_TRY(launch_bg_closure(d_verts, d_wt, cb, s_max < 0xFFFF, nb_total, Sb, d_bgp, P, d_pidx, max_pid + 1,
                             g->d_row_offset, g->d_col, nullptr, g->d_graph_id, g->d_st2ed,
                             g->d_ed2st, d_mat, st));
-  return launch_bg_rows(d_bgp, d_orow, nb_total, d_mat, Sb, max_bdy, d_pidx, g->d_graph_id,
+  return launch_bg_rows(d_bgp, d_orow, nb_total, d_mat, Sb, max_bdy, d_pidx, max_pid + 1, g->d_graph_id,
                         g->d_ed2st, g->n, d_out, ld_out, d_max, true, st);
 }
 

@@ -5,6 +5,9 @@
 // three-kernel GPU twin (buildFromCSR / buildFromSubGraph / setDiagonal, :87-191,
 // which builds the MAXVALUE init on the host and copies it over).  Here one
 // workgroup assembles one row in LDS and writes it once, coalesced.
+#include <algorithm>
+#include <vector>
+
 #include "fap_internal.h"
 #include "fw_desc.h"
 
@@ -290,5 +293,76 @@ extern "C" int fap_hip_i32_to_f32(const int32_t *d_src, float *d_dst, int64_t co
   hipLaunchKernelGGL(i32_to_f32_kernel, dim3(conv_grid(count)), dim3(256), 0, resolve(stream),
                      d_src, d_dst, count);
   FAP_CHECK_LAUNCH();
+  return FAP_OK;
+}
+
+// ---- layout validation -----------------------------------------------------------
+// Sorted position pos (one thread each): v = st2ed[pos] in range, ed2st[v] == pos, v's
+// label is the id of the part whose rows hold pos, and an inner vertex (local index >=
+// the part's boundary count) has no arc to another part.  Any violation sets *bad.
+namespace {
+
+__global__ void check_layout_kernel(const int *__restrict__ row_off, const int *__restrict__ col,
+                                    const int *__restrict__ graph_id, const int *__restrict__ st2ed,
+                                    const int *__restrict__ ed2st, int n,
+                                    const fap_part_desc *__restrict__ parts, int np,
+                                    int *__restrict__ bad) {
+  const int pos = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= n) return;
+  int lo = 0, hi = np - 1;  // parts sorted by offset, tiling [0, n)
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (parts[mid].offset <= pos) lo = mid;
+    else hi = mid - 1;
+  }
+  const fap_part_desc q = parts[lo];
+  const int v = st2ed[pos];
+  bool ok = v >= 0 && v < n && pos >= q.offset && pos < q.offset + q.num;
+  if (ok) ok = ed2st[v] == pos && graph_id[v] == q.part_id;
+  if (ok && pos - q.offset >= q.bdy)
+    for (int e = row_off[v]; e < row_off[v + 1] && ok; ++e) {
+      const int u = col[e];
+      ok = u >= 0 && u < n && graph_id[u] == q.part_id;
+    }
+  if (!ok) atomicOr(bad, 1);
+}
+
+}  // namespace
+
+extern "C" int fap_hip_check_layout(const fap_graph_dev *g, const fap_part_desc *parts,
+                                    int32_t nparts, int32_t *ok) {
+  if (!initialized()) {
+    set_error("fap_hip_init has not been called");
+    return FAP_ERR_STATE;
+  }
+  FAP_REQUIRE(g && ok && nparts >= 1 && parts && g->n > 0 && g->d_row_offset && g->d_graph_id &&
+                  g->d_st2ed && g->d_ed2st && (g->m == 0 || g->d_col),
+              "fap_hip_check_layout: null argument");
+  *ok = 0;
+  std::vector<fap_part_desc> ps(parts, parts + nparts);
+  std::sort(ps.begin(), ps.end(),
+            [](const fap_part_desc &a, const fap_part_desc &b) { return a.offset < b.offset; });
+  int next = 0;
+  for (const fap_part_desc &q : ps) {
+    if (q.offset != next || q.num < 0 || q.bdy < 0 || q.bdy > q.num) return FAP_OK;  // no tiling
+    next += q.num;
+  }
+  if (next != g->n) return FAP_OK;
+  hipStream_t st = lib_stream();
+  static Workspace ws;
+  FAP_TRY(ws.ensure(256 + sizeof(fap_part_desc) * ps.size()));
+  int *d_bad = static_cast<int *>(ws.ptr);
+  fap_part_desc *d_parts = reinterpret_cast<fap_part_desc *>(static_cast<char *>(ws.ptr) + 256);
+  FAP_TRY(fap_hip_fill_i32(d_bad, 0, 1, st));
+  FAP_CHECK_HIP(hipMemcpyAsync(d_parts, ps.data(), sizeof(fap_part_desc) * ps.size(),
+                               hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(check_layout_kernel, dim3((g->n + 255) / 256), dim3(256), 0, st,
+                     g->d_row_offset, g->d_col, g->d_graph_id, g->d_st2ed, g->d_ed2st, g->n,
+                     d_parts, (int)ps.size(), d_bad);
+  FAP_CHECK_LAUNCH();
+  int bad = 1;
+  FAP_CHECK_HIP(hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, st));
+  FAP_CHECK_HIP(hipStreamSynchronize(st));
+  *ok = bad == 0;
   return FAP_OK;
 }

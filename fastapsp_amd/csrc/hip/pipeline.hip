@@ -36,17 +36,18 @@ int launch_sssp(const int *d_row_off, const int *d_col, const int *d_w, int n, c
                 int nsrc, const int64_t *d_rows, int *d_out, int64_t ld, bool vec_fill,
                 hipStream_t stream);
 int launch_fw(const FwDesc *d_descs, int ndesc, int max_nb, hipStream_t stream);
-int launch_bg_vert(const BgPart *d_parts, int np, int nb_total, const int *d_pidx_by_pid,
+int launch_bg_vert(const BgPart *d_parts, int np, int nb_total, const int *d_pidx_by_pid, int npid,
                    const int *row_off, const int *col, const int *w, const int *graph_id,
                    const int *st2ed, const int *ed2st, BgVert *d_verts, int *d_wt,
                    int64_t *d_orow, hipStream_t stream);
 int launch_bg_closure(const BgVert *d_verts, const int *d_wt, int cb_total, bool unit,
                       int nb_total, int Sb, const BgPart *d_parts, int np,
-                      const int *d_pidx_by_pid, const int *row_off, const int *col, const int *w,
+                      const int *d_pidx_by_pid, int npid, const int *row_off, const int *col,
+                      const int *w,
                       const int *graph_id, const int *st2ed, const int *ed2st, int *d_mat,
                       hipStream_t stream);
 int launch_bg_rows(const BgPart *d_parts, const int64_t *d_orow, int nb_total, const int *d_mat,
-                   int Sb, int max_bdy, const int *d_pidx_by_pid, const int *graph_id,
+                   int Sb, int max_bdy, const int *d_pidx_by_pid, int npid, const int *graph_id,
                    const int *ed2st, int n, int *d_out, int64_t ld_out, int *d_max_b,
                    bool sym, hipStream_t stream);
 int launch_fw16(const Fw16Desc *d_descs, int ndesc, int max_nb, hipStream_t stream);
@@ -553,15 +554,17 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   auto run_boundary_graph = [&](hipStream_t on) -> int {
     if (bg_total == 0) return FAP_OK;
     FAP_TRY(fap_hip_fill_i32(d_maxb, 0, 1, on));
-    FAP_TRY(launch_bg_vert(d_bgp, P, bg_total, d_pidx, g->d_row_offset, g->d_col, g->d_w,
+    const int npid = (int)pidx_by_pid.size();
+    FAP_TRY(launch_bg_vert(d_bgp, P, bg_total, d_pidx, npid, g->d_row_offset, g->d_col, g->d_w,
                            g->d_graph_id, g->d_st2ed, g->d_ed2st, bg_verts, bg_wt, bg_orow, on));
     // Unit weights: in-part distances below s_max < 2^16, so the clique weights fit
     // 16 bits.
     FAP_TRY(launch_bg_closure(bg_verts, bg_wt, bg_cb, g->d_w == nullptr && s_max < 0xFFFF,
-                              bg_total, Sb, d_bgp, P, d_pidx, g->d_row_offset, g->d_col, g->d_w,
+                              bg_total, Sb, d_bgp, P, d_pidx, npid, g->d_row_offset, g->d_col,
+                              g->d_w,
                               g->d_graph_id, g->d_st2ed, g->d_ed2st, bg_mat, on));
     // Paired arcs (component labels) of unit weight: the in-part matrices are symmetric.
-    return launch_bg_rows(d_bgp, bg_orow, bg_total, bg_mat, Sb, bg_maxb, d_pidx, g->d_graph_id,
+    return launch_bg_rows(d_bgp, bg_orow, bg_total, bg_mat, Sb, bg_maxb, d_pidx, npid, g->d_graph_id,
                           g->d_ed2st, n, d_out, ld_out, d_maxb,
                           g->d_comp != nullptr && g->d_w == nullptr, on);
   };

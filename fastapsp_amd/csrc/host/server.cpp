@@ -116,16 +116,28 @@ void scheduler(fap_server *s) {
     for (size_t i = 0; i < s->inflight.size();) {
       Inflight &f = s->inflight[i];
       int32_t done = 0;
-      if (fap_hip_event_query(f.event, &done) != FAP_OK) done = 1;  // failed launch: reap
-      if (!done) {
+      // A failed query means the batch's work cannot be trusted: wait for its stream to
+      // drain (so nothing still writes the buffers), then fail its requests and free
+      // their results instead of reporting them done.
+      const bool failed = fap_hip_event_query(f.event, &done) != FAP_OK;
+      if (failed) (void)fap_hip_stream_sync(s->streams[f.slot]);
+      if (!done && !failed) {
         ++i;
         continue;
       }
       const double t = now_s();
       for (int64_t tk : f.tickets) {
         Request &r = s->reqs[tk];
-        r.state = kDone;
         r.t_done = t;
+        if (failed) {
+          r.state = kFailed;
+          ++s->failed;
+          if (r.d_out) fap_hip_free(r.d_out);
+          r.d_out = nullptr;
+          s->reserved -= r.bytes;
+          continue;
+        }
+        r.state = kDone;
         s->latency_ms.push_back((t - r.t_submit) * 1e3);
         s->entries += (double)r.rows * r.cols;
         ++s->completed;

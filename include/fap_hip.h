@@ -250,6 +250,16 @@ typedef struct fap_stage_times {
                            packed 16-bit rows, 0 when on int32 rows; -1 otherwise */
 } fap_stage_times;
 
+/* Optional check of a C-ABI caller's layout before fap_hip_solve_parts (whose
+ * boundary-graph form relies on it; fapGraph builds it correctly): the parts tile
+ * [0, n) in sorted positions, st2ed / ed2st are inverse permutations, every vertex's
+ * label is the id of the part holding its position, and no inner vertex (local index
+ * >= the part's boundary count) has an arc to another part (util_preprocess.h:32-106).
+ * *ok = 1 when all hold.  Synchronous; O(n + m).  With a layout that fails it, solves
+ * stay within their buffers but their results are undefined. */
+int fap_hip_check_layout(const fap_graph_dev *g, const fap_part_desc *parts, int32_t nparts,
+                         int32_t *ok);
+
 /* The whole per-part pipeline of fapGraph::run (fap/fap.cpp:195-503) for a list
  * of parts: boundary SSSP -> sub-matrix build -> Floyd-Warshall -> min-plus of
  * inner rows against the boundary rows, with the decode of in-part columns fused

@@ -8,6 +8,7 @@ reference's CPU branch, bit for bit, distances and predecessors:
 * fap_hip_f32_to_i32 / fap_hip_i32_to_f32: MAXVALUE <-> FAP_INF at the seam's edges.
 Inputs use small weights so that ties are everywhere: the predecessor chosen on a tie
 is what these tests pin."""
+import ctypes as C
 import sys
 
 import numpy as np
@@ -193,3 +194,42 @@ def test_distance_conversions():
     L.check(L.load().fap_hip_i32_to_f32(di.ptr, dback.ptr, f.size, None), "i32_to_f32")
     back = dback.download(f.shape, np.float32)
     assert np.array_equal(back, to_f(got))
+
+
+def _check_layout(csr, lay, K):
+    n = len(csr["row_offset"]) - 1
+    bufs = [dev(np.ascontiguousarray(x, np.int32)) for x in
+            (csr["row_offset"], csr["col"], lay["graph_id"], lay["st2ed"], lay["ed2st"])]
+    g = L.FapGraphDev(n, len(csr["col"]), bufs[0].ptr, bufs[1].ptr, None, bufs[2].ptr,
+                      bufs[3].ptr, bufs[4].ptr, None, None)
+    parts = (L.FapPartDesc * K)(*[L.FapPartDesc(p, int(lay["offset"][p]), int(lay["num"][p]),
+                                                int(lay["bdy"][p]), int(lay["offset"][p]))
+                                  for p in range(1, K + 1)])
+    ok = C.c_int32(-1)
+    L.check(L.load().fap_hip_check_layout(C.byref(g), parts, K, C.byref(ok)), "check_layout")
+    return ok.value
+
+
+def test_check_layout_accepts_valid_and_rejects_corrupt():
+    """fap_hip_check_layout: the layout fapGraph builds passes; a label moved to another
+    part, a boundary count one short (an inner vertex with a cut arc) and two swapped
+    st2ed entries each fail; a part list that does not tile [0, n) fails."""
+    say("check_layout")
+    g = F.FapGraph.road_grid(30, 30, 0.1, 0.02, 4, 5)
+    g.preCondition()
+    csr, lay = g.csr(), g.layout()
+    assert _check_layout(csr, lay, g.K) == 1
+    bad = {k: v.copy() for k, v in lay.items()}
+    v = int(bad["st2ed"][int(bad["offset"][2]) + int(bad["bdy"][2])])  # an inner vertex of part 2
+    bad["graph_id"][v] = 3
+    assert _check_layout(csr, bad, g.K) == 0
+    bad = {k: v.copy() for k, v in lay.items()}
+    p = int(np.argmax(bad["bdy"][1:])) + 1
+    bad["bdy"][p] -= 1
+    assert _check_layout(csr, bad, g.K) == 0
+    bad = {k: v.copy() for k, v in lay.items()}
+    bad["st2ed"][[0, 1]] = bad["st2ed"][[1, 0]]
+    assert _check_layout(csr, bad, g.K) == 0
+    bad = {k: v.copy() for k, v in lay.items()}
+    bad["num"][1] -= 1
+    assert _check_layout(csr, bad, g.K) == 0
