@@ -48,7 +48,7 @@ int launch_fw64(const Fw64Desc *d_descs, int nd, int max_nb, hipStream_t st);
 namespace {
 
 constexpr int kBT = 64;           // tile edge of the Floyd-Warshall matrices
-constexpr int kBatchMaxS = 8192;  // largest part
+constexpr int kBatchMaxS = kMaxPart;  // largest part
 constexpr int kBQ = 1024;         // SSSP frontier queue entries per wave
 constexpr int kSWaves = 4;        // waves (sources) per workgroup of the SSSP launch
 constexpr int kMCols = 256;       // merge: columns per unit (one per thread)

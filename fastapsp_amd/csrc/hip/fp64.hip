@@ -40,7 +40,7 @@ typedef unsigned long long u64;
 constexpr double kInfD = __builtin_inf();
 constexpr float kMaxValue = 1e8f;
 constexpr int kT = 64;
-constexpr int kMaxS64 = 8192;
+constexpr int kBuildChunk64 = 8192;  // LDS row buffer: 64 KB
 
 __device__ __forceinline__ u64 dbits(double x) { return (u64)__double_as_longlong(x); }
 __device__ __forceinline__ double bitsd(u64 x) { return __longlong_as_double((long long)x); }
@@ -160,30 +160,36 @@ __global__ __launch_bounds__(256) void build64_kernel(const Build64Desc *__restr
   if (grow >= total_rows) return;
   const Build64Desc d = descs[find_begin(descs, nd, grow, build_begin)];
   const int i = grow - d.row_begin;
-  __shared__ double rowbuf[kMaxS64];
+  __shared__ double rowbuf[kBuildChunk64];
   const int tid = threadIdx.x;
-  if (i < d.s && i < d.bdy) {
-    const double *src = d.rows + (int64_t)i * d.ld_rows;
-    for (int j = tid; j < d.S; j += 256) rowbuf[j] = (j < d.s) ? src[st2ed[d.offset + j]] : kInfD;
-  } else {
-    for (int j = tid; j < d.S; j += 256) rowbuf[j] = kInfD;
-  }
-  __syncthreads();
-  if (tid == 0 && i < d.s) {
-    if (i >= d.bdy) {
-      const int ver = st2ed[d.offset + i];
-      for (int e = row_off[ver]; e < row_off[ver + 1]; ++e) {
-        const int nb = col[e];
-        if (graph_id[nb] != d.part_id) continue;
-        const int idx = ed2st[nb] - d.offset;
-        if (idx >= 0 && idx < d.s) rowbuf[idx] = w ? (double)w[e] : 1.0;
-      }
+  const bool from_rows = i < d.s && i < d.bdy;
+  for (int c0 = 0; c0 < d.S; c0 += kBuildChunk64) {  // rows in chunks of LDS size
+    const int cw = min(kBuildChunk64, d.S - c0);
+    if (c0 > 0) __syncthreads();
+    if (from_rows) {
+      const double *src = d.rows + (int64_t)i * d.ld_rows;
+      for (int j = tid; j < cw; j += 256)
+        rowbuf[j] = (c0 + j < d.s) ? src[st2ed[d.offset + c0 + j]] : kInfD;
+    } else {
+      for (int j = tid; j < cw; j += 256) rowbuf[j] = kInfD;
     }
-    rowbuf[i] = 0.0;
+    __syncthreads();
+    if (tid == 0 && i < d.s) {
+      if (!from_rows) {
+        const int ver = st2ed[d.offset + i];
+        for (int e = row_off[ver]; e < row_off[ver + 1]; ++e) {
+          const int nb = col[e];
+          if (graph_id[nb] != d.part_id) continue;
+          const int idx = ed2st[nb] - d.offset - c0;
+          if (idx >= 0 && idx < cw && c0 + idx < d.s) rowbuf[idx] = w ? (double)w[e] : 1.0;
+        }
+      }
+      if (i >= c0 && i < c0 + cw) rowbuf[i - c0] = 0.0;
+    }
+    __syncthreads();
+    double *dst = d.fw + (int64_t)i * d.S + c0;
+    for (int j = tid; j < cw; j += 256) dst[j] = rowbuf[j];
   }
-  __syncthreads();
-  double *dst = d.fw + (int64_t)i * d.S;
-  for (int j = tid; j < d.S; j += 256) dst[j] = rowbuf[j];
 }
 
 // ---------------------------------------------------------------- Floyd-Warshall
@@ -532,8 +538,8 @@ static int solve_parts_f32(const fap_graph_dev *g, const fap_part_desc *parts, i
     FAP_REQUIRE(q.part_id >= 1 && q.num >= 0 && q.bdy >= 0 && q.bdy <= q.num && q.offset >= 0 &&
                     (int64_t)q.offset + q.num <= n && q.out_row >= 0,
                 "fap_hip_solve_parts_f32: bad part %d", p);
-    FAP_REQUIRE(q.num <= kMaxS64, "fap_hip_solve_parts_f32: part %d has %d vertices (limit %d)",
-                q.part_id, q.num, kMaxS64);
+    FAP_REQUIRE(q.num <= kMaxPart, "fap_hip_solve_parts_f32: part %d has %d vertices (limit %d)",
+                q.part_id, q.num, kMaxPart);
     if (q.num > 0) act.push_back(q);
   }
   const int P = (int)act.size();

@@ -5,6 +5,10 @@
 
 namespace fap_hip {
 
+// Largest part the pipelines take: an int32 Floyd-Warshall matrix of 32768^2 entries is
+// 4 GB, and every index of such a matrix fits 31 bits.
+constexpr int kMaxPart = 32768;
+
 // One padded S x S Floyd-Warshall matrix (S = ld = nb * 64, FAP_INF padding).
 struct FwDesc {
   int *mat;

@@ -13,7 +13,7 @@
 
 namespace fap_hip {
 
-constexpr int kBuildMaxS = 8192;  // LDS row buffer: 32 KB
+constexpr int kBuildChunk = 8192;  // LDS row buffer: 32 KB
 
 template <typename D>
 __device__ __forceinline__ int find_by_begin(const D *__restrict__ d, int nd, int key,
@@ -32,7 +32,8 @@ __device__ __forceinline__ int find_by_begin(const D *__restrict__ d, int nd, in
 //   other rows   : FAP_INF, then the part's own out-arcs of st2ed[offset + i] in
 //                  CSR order (a later arc overwrites an earlier one)
 //   diagonal     : 0;  padding (j >= s or i >= s): FAP_INF.
-// With d.fw16 set, the row is written as 16-bit entries min(value, kCap16).
+// With d.fw16 set, the row is written as 16-bit entries min(value, kCap16).  Rows
+// longer than the LDS buffer are assembled in chunks of kBuildChunk columns.
 template <bool kVec>
 __global__ __launch_bounds__(256) void build_kernel(const BuildDesc *__restrict__ descs, int nd,
                                                     int total_rows,
@@ -46,46 +47,51 @@ __global__ __launch_bounds__(256) void build_kernel(const BuildDesc *__restrict_
   if (grow >= total_rows) return;
   const BuildDesc d = descs[find_by_begin(descs, nd, grow, &BuildDesc::row_begin)];
   const int i = grow - d.row_begin;
-  __shared__ int rowbuf[kBuildMaxS];
+  __shared__ int rowbuf[kBuildChunk];
   const int tid = threadIdx.x;
-
-  if (i < d.s && i < d.bdy && d.dist_rows) {
-    const int *src = d.dist_rows + (int64_t)i * d.ld_dist;
-    for (int j = tid; j < d.S; j += 256) rowbuf[j] = (j < d.s) ? src[st2ed[d.offset + j]] : kInf;
-  } else {
-    for (int j = tid; j < d.S; j += 256) rowbuf[j] = kInf;
-  }
-  __syncthreads();
-  if (tid == 0 && i < d.s) {
-    if (i >= d.bdy || !d.dist_rows) {
-      const int ver = st2ed[d.offset + i];
-      const int e1 = row_off[ver + 1];
-      for (int e = row_off[ver]; e < e1; ++e) {
-        const int nb = col[e];
-        if (graph_id[nb] != d.part_id) continue;
-        const int idx = ed2st[nb] - d.offset;
-        if (idx >= 0 && idx < d.s) rowbuf[idx] = w ? w[e] : 1;
+  const bool from_rows = i < d.s && i < d.bdy && d.dist_rows;
+  for (int c0 = 0; c0 < d.S; c0 += kBuildChunk) {
+    const int cw = min(kBuildChunk, d.S - c0);  // a multiple of 64
+    if (c0 > 0) __syncthreads();  // the previous chunk has been written out
+    if (from_rows) {
+      const int *src = d.dist_rows + (int64_t)i * d.ld_dist;
+      for (int j = tid; j < cw; j += 256)
+        rowbuf[j] = (c0 + j < d.s) ? src[st2ed[d.offset + c0 + j]] : kInf;
+    } else {
+      for (int j = tid; j < cw; j += 256) rowbuf[j] = kInf;
+    }
+    __syncthreads();
+    if (tid == 0 && i < d.s) {
+      if (!from_rows) {
+        const int ver = st2ed[d.offset + i];
+        const int e1 = row_off[ver + 1];
+        for (int e = row_off[ver]; e < e1; ++e) {
+          const int nb = col[e];
+          if (graph_id[nb] != d.part_id) continue;
+          const int idx = ed2st[nb] - d.offset - c0;
+          if (idx >= 0 && idx < cw && c0 + idx < d.s) rowbuf[idx] = w ? w[e] : 1;
+        }
       }
+      if (i >= c0 && i < c0 + cw) rowbuf[i - c0] = 0;
     }
-    rowbuf[i] = 0;
-  }
-  __syncthreads();
-  if (d.fw16) {
-    // Capped 16-bit row, two entries per 32-bit word (S is a multiple of 64).
-    unsigned *dst16 = reinterpret_cast<unsigned *>(d.fw16 + (int64_t)i * d.S);
-    for (int w = tid; w < d.S / 2; w += 256) {
-      const unsigned lo = (unsigned)min(rowbuf[2 * w], kCap16);
-      const unsigned hi = (unsigned)min(rowbuf[2 * w + 1], kCap16);
-      dst16[w] = lo | (hi << 16);
+    __syncthreads();
+    if (d.fw16) {
+      // Capped 16-bit row, two entries per 32-bit word (chunks are multiples of 64).
+      unsigned *dst16 = reinterpret_cast<unsigned *>(d.fw16 + (int64_t)i * d.S + c0);
+      for (int x = tid; x < cw / 2; x += 256) {
+        const unsigned lo = (unsigned)min(rowbuf[2 * x], kCap16);
+        const unsigned hi = (unsigned)min(rowbuf[2 * x + 1], kCap16);
+        dst16[x] = lo | (hi << 16);
+      }
+      continue;
     }
-    return;
-  }
-  int *dst = d.fw + (int64_t)i * d.ldf;
-  if (kVec) {
-    for (int j = tid * 4; j < d.S; j += 1024)
-      *reinterpret_cast<int4 *>(dst + j) = *reinterpret_cast<const int4 *>(&rowbuf[j]);
-  } else {
-    for (int j = tid; j < d.S; j += 256) dst[j] = rowbuf[j];
+    int *dst = d.fw + (int64_t)i * d.ldf + c0;
+    if (kVec) {
+      for (int j = tid * 4; j < cw; j += 1024)
+        *reinterpret_cast<int4 *>(dst + j) = *reinterpret_cast<const int4 *>(&rowbuf[j]);
+    } else {
+      for (int j = tid; j < cw; j += 256) dst[j] = rowbuf[j];
+    }
   }
 }
 
@@ -177,7 +183,7 @@ extern "C" int fap_hip_submat_build(int32_t *d_sub, int64_t ld_sub, const int32_
   FAP_REQUIRE(s >= 0 && bdy >= 0 && bdy <= s && start >= 0 && start + s <= n && ld_sub >= s,
               "fap_hip_submat_build: bad sizes");
   if (s == 0) return FAP_OK;
-  FAP_REQUIRE(s <= kBuildMaxS, "fap_hip_submat_build: s=%d above %d", s, kBuildMaxS);
+  FAP_REQUIRE(s <= kMaxPart, "fap_hip_submat_build: s=%d above %d", s, kMaxPart);
   FAP_REQUIRE(d_sub && d_row_offset && d_graph_id && d_st2ed && d_ed2st &&
                   (bdy == 0 || (d_dist_rows && ld_dist >= n)),
               "fap_hip_submat_build: null pointer");

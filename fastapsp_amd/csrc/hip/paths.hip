@@ -60,7 +60,7 @@ constexpr int kT = 64;          // Floyd-Warshall tile edge
 constexpr int kTT = kT * kT;    // entries per tile
 constexpr int kTP = kT + 1;     // LDS row stride of a tile: a column of a wave's rows
                                 // then spreads over all banks
-constexpr int kMaxS = 8192;
+constexpr int kBuildChunkP = 8192;  // LDS row buffers of build_path_kernel: 64 KB
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
@@ -184,47 +184,54 @@ __global__ __launch_bounds__(256) void build_path_kernel(
   const int pi = find_part(parts, np, grow, &PathPart::row_begin);
   const PathPart d = parts[pi];
   const int i = grow - d.row_begin;
-  __shared__ int rd[kMaxS];
-  __shared__ int rp[kMaxS];
+  __shared__ int rd[kBuildChunkP];
+  __shared__ int rp[kBuildChunkP];
   const int tid = threadIdx.x;
-  if (i < d.s && i < d.bdy) {
-    const int64_t base = (d.out_row + i) * ld_out;
-    for (int j = tid; j < d.S; j += 256) {
-      const int v = j < d.s ? st2ed[d.offset + j] : 0;
-      rd[j] = j < d.s ? out[base + v] : kInf;
-      rp[j] = j < d.s ? pout[base + v] : -1;
-    }
-  } else {
-    for (int j = tid; j < d.S; j += 256) {
-      rd[j] = kInf;
-      rp[j] = -1;
-    }
-  }
-  __syncthreads();
-  if (tid == 0 && i < d.s) {
-    const int ver = st2ed[d.offset + i];
-    if (i >= d.bdy) {  // boundary rows: the SSSP row replaces the arcs
-      for (int e = row_off[ver]; e < row_off[ver + 1]; ++e) {
-        const int nb = col[e];
-        if (graph_id[nb] != d.part_id) continue;
-        const int idx = ed2st[nb] - d.offset;
-        if (idx >= 0 && idx < d.s) {
-          rd[idx] = w ? w[e] : 1;
-          rp[idx] = ver;
-        }
+  const int ver = i < d.s ? st2ed[d.offset + i] : 0;
+  int mx = -1;
+  for (int c0 = 0; c0 < d.S; c0 += kBuildChunkP) {  // rows in chunks of LDS size
+    const int cw = min(kBuildChunkP, d.S - c0);
+    if (c0 > 0) __syncthreads();
+    if (i < d.s && i < d.bdy) {
+      const int64_t base = (d.out_row + i) * ld_out;
+      for (int j = tid; j < cw; j += 256) {
+        const bool in = c0 + j < d.s;
+        const int v = in ? st2ed[d.offset + c0 + j] : 0;
+        rd[j] = in ? out[base + v] : kInf;
+        rp[j] = in ? pout[base + v] : -1;
+      }
+    } else {
+      for (int j = tid; j < cw; j += 256) {
+        rd[j] = kInf;
+        rp[j] = -1;
       }
     }
-    rd[i] = 0;
-    rp[i] = ver;
-  }
-  __syncthreads();
-  FAP_GLOBAL int *dd = gptr(d.fw + (int64_t)i * d.S);
-  FAP_GLOBAL int *dp = gptr(d.fwp + (int64_t)i * d.S);
-  int mx = -1;
-  for (int j = tid; j < d.S; j += 256) {
-    dd[j] = rd[j];
-    dp[j] = rp[j];
-    if (rd[j] < kInf) mx = max(mx, rd[j]);
+    __syncthreads();
+    if (tid == 0 && i < d.s) {
+      if (i >= d.bdy) {  // boundary rows: the SSSP row replaces the arcs
+        for (int e = row_off[ver]; e < row_off[ver + 1]; ++e) {
+          const int nb = col[e];
+          if (graph_id[nb] != d.part_id) continue;
+          const int idx = ed2st[nb] - d.offset - c0;
+          if (idx >= 0 && idx < cw && c0 + idx < d.s) {
+            rd[idx] = w ? w[e] : 1;
+            rp[idx] = ver;
+          }
+        }
+      }
+      if (i >= c0 && i < c0 + cw) {
+        rd[i - c0] = 0;
+        rp[i - c0] = ver;
+      }
+    }
+    __syncthreads();
+    FAP_GLOBAL int *dd = gptr(d.fw + (int64_t)i * d.S + c0);
+    FAP_GLOBAL int *dp = gptr(d.fwp + (int64_t)i * d.S + c0);
+    for (int j = tid; j < cw; j += 256) {
+      dd[j] = rd[j];
+      dp[j] = rp[j];
+      if (rd[j] < kInf) mx = max(mx, rd[j]);
+    }
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off));
@@ -1050,7 +1057,7 @@ static int solve_parts_path(const fap_graph_dev *g, const int32_t *d_rro, const 
   for (int k = 0; k < nparts; ++k) {
     const fap_part_desc &q = parts[k];
     FAP_REQUIRE(q.part_id >= 1 && q.num >= 0 && q.bdy >= 0 && q.bdy <= q.num && q.offset >= 0 &&
-                    (int64_t)q.offset + q.num <= n && q.out_row >= 0 && q.num <= kMaxS,
+                    (int64_t)q.offset + q.num <= n && q.out_row >= 0 && q.num <= kMaxPart,
                 "fap_hip_solve_parts_path: bad part %d", k);
     if (q.num > 0) act.push_back(q);
   }
@@ -1515,7 +1522,7 @@ extern "C" int fap_hip_floyd_path(int32_t *d_mat, int32_t *d_path, int32_t s, in
   FAP_REQUIRE(s >= 0 && ld >= s, "fap_hip_floyd_path: bad sizes s=%d ld=%lld", s, (long long)ld);
   if (s == 0) return FAP_OK;
   FAP_REQUIRE(d_mat && d_path, "fap_hip_floyd_path: null matrix");
-  FAP_REQUIRE(s <= kMaxS, "fap_hip_floyd_path: s=%d above the limit %d", s, kMaxS);
+  FAP_REQUIRE(s <= kMaxPart, "fap_hip_floyd_path: s=%d above the limit %d", s, kMaxPart);
   hipStream_t st = resolve(stream);
   const int S = (int)align_up(s, kT), nb = S / kT;
   const int64_t tot = (int64_t)S * S;

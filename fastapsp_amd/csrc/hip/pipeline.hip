@@ -81,7 +81,6 @@ int launch_gather_sources(const int *offsets, const int *src_begin, const int64_
                           int nparts, int total, const int *st2ed, int64_t ld, int *src,
                           int64_t *rows, hipStream_t stream);
 
-constexpr int kBuildMaxS = 8192;
 constexpr int kTile = 64;
 constexpr int kStreamMaxK = 32;  // larger boundary counts use the LDS-tiled product
 // Boundary-graph form of the boundary rows (boundary.hip): at most this many boundary
@@ -205,9 +204,9 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
                     (int64_t)q.offset + q.num <= n && q.out_row >= 0,
                 "fap_hip_solve_parts: bad part %d (id %d off %d num %d bdy %d)", p, q.part_id,
                 q.offset, q.num, q.bdy);
-    FAP_REQUIRE(q.num <= kBuildMaxS,
+    FAP_REQUIRE(q.num <= kMaxPart,
                 "fap_hip_solve_parts: part %d has %d vertices (limit %d); use a larger K",
-                q.part_id, q.num, kBuildMaxS);
+                q.part_id, q.num, kMaxPart);
     if (q.num > 0) act.push_back(q);
   }
   const int P = (int)act.size();
@@ -242,8 +241,9 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
     hier = cover && next == n && bg_total <= kBgMaxTotal;
   }
   // 16-bit Floyd-Warshall: on in-part matrices of unit-weight graphs every finite
-  // entry is at most s - 1 < 8192 < the cap, so the cap means "unreachable inside
-  // the part" exactly; on SSSP-form matrices of graphs with component labels the
+  // entry is at most s - 1, below the cap for parts of at most kCap16 vertices, so the
+  // cap means "unreachable inside the part" exactly; on SSSP-form matrices of graphs
+  // with component labels the
   // cap is checked against the labels (and the part redone in int32 if needed).
   // Bit-parallel in-part BFS (inpart.hip) instead of build + Floyd-Warshall: the
   // boundary-graph form needs in-part distances only, which on unit weights with
@@ -253,8 +253,10 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   const bool bfs_parts = hier && g->d_w == nullptr && g->d_comp != nullptr &&
                          !(mode & (FAP_MODE_FLAG_INT32_FW | FAP_MODE_FLAG_FW)) &&
                          s_max_all <= kInpartBfsMaxS;
+  // In the boundary-graph form the 16-bit closure has no cap check: it needs every
+  // in-part distance (at most s - 1 for unit weights) below the cap.
   const bool use16 = !(mode & FAP_MODE_FLAG_INT32_FW) && !bfs_parts &&
-                     (hier ? g->d_w == nullptr : g->d_comp != nullptr);
+                     (hier ? g->d_w == nullptr && s_max_all <= kCap16 : g->d_comp != nullptr);
   const bool flags16 = use16 && !hier;
   const int nchunks = overlap ? std::min(kChunks, P) : 1;
   std::vector<int> chunk_of(P);
