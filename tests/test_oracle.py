@@ -261,3 +261,39 @@ def test_streaming_comparators_agree_and_catch_planted_errors():
             assert O.compare_exact_rows(csr["row_offset"], csr["col"], wt,
                                         lay["st2ed"][r0:r1], bad) == (1, r - r0)
             assert O.compare_parts(2, g.K - 1, csr, lay, bad, r0) == (1, r - r0)
+
+
+def test_int32_kernels_pinned_to_reference_float_kernels():
+    """orc_floyd_i32 / orc_minplus_i32 (the int32 checkers of the GPU kernel tests) equal
+    the reference's floyd_CPU_path / min_plus_path_CPU_advanced on integer inputs below
+    2^24 (exact in float; MAXVALUE <-> INT32_MAX / 2): through the golden vectors written
+    from the compiled reference, and live against it when it is built."""
+    g = gold("floyd.npz")
+    m = O.to_i32(g["mat"])
+    assert np.array_equal(O.floyd_i32(m), O.to_i32(g["mat_out"]))
+    g = gold("minplus.npz")
+    assert np.array_equal(O.minplus_i32(O.to_i32(g["A"]), O.to_i32(g["B"])), O.to_i32(g["C"]))
+    ref = O.ref()
+    if ref is None:
+        pytest.skip("compiled reference kernels not built (oracle/_ref)")
+    P = lambda a, t: a.ctypes.data_as(C.POINTER(t))  # noqa: E731
+    rng = np.random.default_rng(21)
+    for trial in range(6):
+        s = int(rng.integers(2, 90))
+        mat = rng.integers(0, 1 << 12, (s, s)).astype(np.float32)
+        mat[rng.random((s, s)) < 0.6] = O.MAXVALUE
+        np.fill_diagonal(mat, 0)
+        m0, q0 = mat.copy(), np.zeros((s, s), np.int32)
+        ref.ref_floyd_cpu_path(s, P(m0, C.c_float), P(q0, C.c_int32))
+        assert np.array_equal(O.floyd_i32(O.to_i32(mat)), O.to_i32(m0))
+        M, N, K = (int(x) for x in rng.integers(1, 70, 3))
+        a = rng.integers(0, 1 << 20, (M, K)).astype(np.float32)
+        b = rng.integers(0, 1 << 20, (K, N)).astype(np.float32)
+        a[rng.random((M, K)) < 0.3] = O.MAXVALUE
+        b[rng.random((K, N)) < 0.3] = O.MAXVALUE
+        c = np.empty((M, N), np.float32)
+        cp = np.empty((M, N), np.int32)
+        bp = np.zeros((K, N), np.int32)
+        ref.ref_min_plus_cpu_advanced(P(a, C.c_float), P(b, C.c_float), P(bp, C.c_int32),
+                                      P(c, C.c_float), P(cp, C.c_int32), M, N, K)
+        assert np.array_equal(O.minplus_i32(O.to_i32(a), O.to_i32(b)), O.to_i32(c))
