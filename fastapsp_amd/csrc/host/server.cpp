@@ -95,17 +95,33 @@ struct fap_server {
 
 namespace {
 
-// Scratch a request brings to a batch: its parts' share of the slot matrices (one per
-// resident workgroup, at most one per part) and, for float weights, its SSSP rows.
+// Scratch a request brings to a batch, an upper bound of what fap_hip_batch_create
+// allocates for it: each part's padded in-part matrix; when the parts tile the graph
+// (a full APSP) the padded boundary-graph matrix and, for float weights, the FP64
+// boundary rows of the whole graph, otherwise (a sub-graph) the part's FP64 boundary
+// rows for float weights; and the work tables (merge units of at least 32 rows x 256
+// columns at 16 bytes each, boundary-row units of at least 8 rows, part and graph
+// records).
 int64_t scratch_estimate(const Request &r, bool is_float, int n) {
-  int64_t bytes = 0;
   const int elem = is_float ? 8 : 4;
+  const int64_t ldr = align_up(n, 2);
+  int64_t bytes = 0, tables = 4096, nb = 0, covered = 0;
   for (const fap_part_desc &p : r.parts) {
     const int64_t S = align_up(p.num, 64);
     bytes += align_up(S * S * elem, 256);
-    if (is_float) bytes += align_up((int64_t)p.bdy * align_up(n, 2) * 8, 256);
+    nb += p.bdy;
+    covered += p.num;
+    tables += ((int64_t)(p.num - p.bdy + 31) / 32) * ((n + 255) / 256) * 16 + 4 * p.part_id + 256;
   }
-  return bytes + (1 << 16);
+  if (covered == n && nb > 0) {
+    const int64_t SB = align_up(nb, 64);
+    bytes += align_up(SB * SB * elem, 256);
+    if (is_float) bytes += align_up(nb * ldr * 8, 256);
+    tables += (int64_t)r.parts.size() * ((nb + 7) / 8) * 8 + 12 * nb;
+  } else if (is_float) {
+    for (const fap_part_desc &p : r.parts) bytes += align_up((int64_t)p.bdy * ldr * 8, 256);
+  }
+  return bytes + align_up(tables, 256) * 2 + (1 << 16);
 }
 
 void scheduler(fap_server *s) {
