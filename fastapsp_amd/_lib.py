@@ -111,6 +111,10 @@ SIGNATURES = {
     "fap_hip_solve_parts": (C.c_int, [C.POINTER(FapGraphDev), C.POINTER(FapPartDesc), _i32,
                                       _vp, _i64, _i32, _vp]),
     "fap_hip_last_stage_times": (C.c_int, [C.POINTER(FapStageTimes)]),
+    "fap_hip_set_merge_shape": (C.c_int, [C.c_int32, C.c_int32, C.c_int32]),
+    "fap_hip_get_merge_shape": (C.c_int, [C.POINTER(C.c_int32)] * 3),
+    "fap_hip_write_probe": (C.c_int, [_vp, _i64, _i32, _i32, _i32, _vp, C.POINTER(C.c_float)]),
+    "fap_hip_malloc_placed": (C.c_int, [C.POINTER(_vp), _i64, _i64, _i32, C.POINTER(C.c_float)]),
     "fap_hip_solve_parts_path": (C.c_int, [C.POINTER(FapGraphDev), _vp, _vp, _vp,
                                            C.POINTER(FapPartDesc), _i32, _vp, _vp, _i64, _vp]),
     "fap_hip_solve_parts_f32": (C.c_int, [C.POINTER(FapGraphDev), C.POINTER(FapPartDesc), _i32,
@@ -219,6 +223,14 @@ def last_stage_times() -> dict:
     t = FapStageTimes()
     check(load().fap_hip_last_stage_times(C.byref(t)), "stage times")
     return {k: getattr(t, k) for k, _ in FapStageTimes._fields_}
+
+
+def mem_stats() -> dict:
+    """fap_hip_mem_stats: device bytes in use through the library, the high-water mark
+    and the budget (0: none)."""
+    u, h, b = C.c_int64(), C.c_int64(), C.c_int64()
+    check(load().fap_hip_mem_stats(C.byref(u), C.byref(h), C.byref(b)), "mem_stats")
+    return {"in_use": u.value, "high_water": h.value, "budget": b.value}
 
 
 def ptr_i32(a: np.ndarray):

@@ -11,6 +11,13 @@
 
 namespace fap_hip {
 
+// Work-unit shape of the streaming merge: rows per unit, unit order (1: the column
+// blocks of one row block back to back) and resident waves per CU (0: no cap).
+struct StreamShape {
+  int rows, col_major, waves_per_cu;
+};
+StreamShape stream_shape();
+
 // Thread-local message of the last failing C-ABI call.
 void set_error(const char *fmt, ...);
 void clear_error();

@@ -22,6 +22,9 @@
 //    (MergeDesc.tile_begin) and tiles that share a B panel are mapped to the same
 //    XCD back to back, so each 32 KB B panel comes from HBM once per XCD L2;
 //  * output rows are written once with non-temporal 16-byte stores.
+#include <algorithm>
+#include <atomic>
+#include <vector>
 #include "fap_internal.h"
 #include "fw_desc.h"
 #include "gmem.h"
@@ -190,16 +193,20 @@ __global__ __launch_bounds__(256) void merge_kernel(const MergeDesc *__restrict_
 // distances in [0, FAP_INF] and meet B = FAP_INF, so they never win a minimum.
 constexpr int kStCols = 256;
 
-// Work-unit shape of the streaming merge: rows per unit and unit order inside a
-// product (0: the row blocks of one column block back to back, 1: the column blocks
-// of one row block back to back).  Measured in one process on MI355X (the 52.5 GB
-// result of the 114,599-vertex graph, K-groups on three streams): 128 rows in
-// column-block order 10.7 ms; 64 rows 10.9, 256 rows 10.8; 128 rows in row-block
-// order 11.1.
-struct StreamShape {
-  int rows, col_major;
-};
-StreamShape stream_shape() { return StreamShape{128, 1}; }
+// Work-unit shape of the streaming merge (StreamShape, fap_internal.h): rows per
+// unit, unit order inside a product (0: the row blocks of one column block back to
+// back, 1: the column blocks of one row block back to back) and a cap on the resident
+// waves per CU (0: none; enforced through the LDS each one-wave workgroup reserves).
+// Default measured in one process on MI355X (the 52.5 GB result of the 114,599-vertex
+// graph, K-groups on three streams): 128 rows in column-block order 10.7 ms; 64 rows
+// 10.9, 256 rows 10.8; 128 rows in row-block order 11.1.  Set with
+// fap_hip_set_merge_shape; a solve reads it once, when it plans its merge units.
+std::atomic<int64_t> g_stream_shape{128 | (1 << 16)};
+
+StreamShape stream_shape() {
+  const int64_t v = g_stream_shape.load(std::memory_order_relaxed);
+  return StreamShape{(int)(v & 0xFFFF), (int)((v >> 16) & 0xFF), (int)((v >> 24) & 0xFF)};
+}
 
 // One row per loop iteration, so a wave keeps few row stores in flight: on MI355X
 // the merge stage of a 52.5 GB result took 11.8 ms with 8 rows in flight per wave
@@ -387,15 +394,54 @@ __global__ __launch_bounds__(64) void merge_stream_kernel(const MergeDesc *__res
   }
 }
 
-int merge_stream_units(int M, int N) {
+// Write probe with the streaming merge's store pattern: units of `rows` rows x 256
+// columns, the column blocks of one row block back to back, the merge's XCD remap, one
+// wave per unit writing one 1 KB row segment (four 256-byte runs of non-temporal dword
+// stores) per row.  Only every `stride`-th row block is written.  Its duration on a
+// buffer tells how fast the merge will write that buffer: on MI355X the same merge
+// measured 9.0 ms in some 52.5 GB ranges of device memory and 10.6 ms in others.
+__global__ __launch_bounds__(64) void write_probe_kernel(int *__restrict__ C, int64_t ldc,
+                                                         int M, int N, int rows, int stride,
+                                                         int total_units, int value) {
+  const int t = xcd_logical_tile();
+  if (t >= total_units) return;
+  const int ncb = (N + kStCols - 1) / kStCols;
+  const int rbi = t / ncb;
+  const int cb = t - rbi * ncb;
+  const int64_t row0 = (int64_t)rbi * stride * rows;
+  const int row1 = (int)min<int64_t>(row0 + rows, M);
+  const int j = cb * kStCols + threadIdx.x;
+  FAP_GLOBAL int *crow = gptr(C + row0 * ldc);
+  for (int i = (int)row0; i < row1; ++i, crow += ldc)
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      if (j + 64 * e < N) __builtin_nontemporal_store(value + i, crow + j + 64 * e);
+}
+
+// Row blocks the placement probe of fap_hip_malloc_placed writes: one in 8 (on MI355X
+// about 1 ms for a 52.5 GB buffer, and the same ordering of buffers as the full merge:
+// 9.0 against 10.6 ms when its probe read 0.93-0.97 against 1.17-1.20 ms).
+constexpr int kPlacementSample = 8;
+
+int launch_write_probe(int *d, int64_t ld, int M, int N, int rows, int stride, int value,
+                       hipStream_t stream) {
+  if (M <= 0 || N <= 0) return FAP_OK;
+  const int64_t rbs = ((int64_t)M + rows - 1) / rows;
+  const int64_t units = (rbs + stride - 1) / stride * ((N + kStCols - 1) / kStCols);
+  FAP_REQUIRE(units < (1LL << 30), "write probe: too many units");
+  const int grid = (int)((units + 7) / 8 * 8);
+  hipLaunchKernelGGL(write_probe_kernel, dim3(grid), dim3(64), 0, stream, d, ld, M, N, rows,
+                     stride, (int)units, value);
+  FAP_CHECK_LAUNCH();
+  return FAP_OK;
+}
+
+int merge_stream_units(int M, int N, const StreamShape &sh) {
   if (M <= 0 || N <= 0) return 0;
-  const int r = stream_shape().rows;
+  const int r = sh.rows;
   return ((M + r - 1) / r) * ((N + kStCols - 1) / kStCols);
 }
-int merge_stream_row_blocks(int M) {
-  const int r = stream_shape().rows;
-  return (M + r - 1) / r;
-}
+int merge_stream_row_blocks(int M, const StreamShape &sh) { return (M + sh.rows - 1) / sh.rows; }
 // K rounded up to a multiple of 2 (the pairs of v_min3 / v_pk_minimum3): against a
 // multiple of 4, the merge stage of the 52.5 GB result measured 10.39 ms against 10.61
 // in one process and buffer (K of the headline parts: 2..23, mean 10).
@@ -404,13 +450,15 @@ int merge_stream_kpad(int K) { return K <= 0 ? 2 : (K + 1) / 2 * 2; }
 // kp: padded K, a multiple of 2 in [2, 32].
 int launch_merge_stream(const MergeDesc *d_descs, int nd, int total_units, int N, int kp,
                         const int *d_graph_id, const int *d_ed2st, const int *d_max_b, int lim16,
-                        hipStream_t stream) {
+                        const StreamShape &sh, hipStream_t stream) {
   if (nd <= 0 || total_units <= 0 || N <= 0) return FAP_OK;
   const int grid = (total_units + 7) / 8 * 8;
-  const StreamShape sh = stream_shape();
+  // Resident-wave cap: each one-wave workgroup reserves 1/waves_per_cu of the CU's
+  // 160 KB LDS (the kernel itself uses none).
+  const size_t lds = sh.waves_per_cu > 0 ? (size_t)(163840 / sh.waves_per_cu) / 1024 * 1024 : 0;
 #define FAP_STREAM_CASE(KP)                                                                   \
   case KP:                                                                                    \
-    hipLaunchKernelGGL((merge_stream_kernel<KP>), dim3(grid), dim3(64), 0, stream, d_descs,    \
+    hipLaunchKernelGGL((merge_stream_kernel<KP>), dim3(grid), dim3(64), lds, stream, d_descs,  \
                        nd, total_units, N, d_graph_id, d_ed2st, d_max_b, lim16, sh.rows,      \
                        sh.col_major);                                                          \
     break;
@@ -463,6 +511,124 @@ int merge_tiles(int M, int N) {
 }  // namespace fap_hip
 
 using namespace fap_hip;
+
+extern "C" int fap_hip_set_merge_shape(int32_t rows, int32_t unit_order, int32_t waves_per_cu) {
+  if (rows < 8 || rows > 1024 || (rows & 7) || unit_order < 0 || unit_order > 1 ||
+      waves_per_cu < 0 || (waves_per_cu > 0 && waves_per_cu < 4) || waves_per_cu > 32) {
+    set_error("fap_hip_set_merge_shape: rows %d (8..1024, multiple of 8), order %d (0/1), "
+              "waves per CU %d (0 or 4..32)", rows, unit_order, waves_per_cu);
+    return FAP_ERR_INVALID;
+  }
+  g_stream_shape.store(rows | ((int64_t)unit_order << 16) | ((int64_t)waves_per_cu << 24),
+                       std::memory_order_relaxed);
+  return FAP_OK;
+}
+
+extern "C" int fap_hip_get_merge_shape(int32_t *rows, int32_t *unit_order, int32_t *waves_per_cu) {
+  const StreamShape sh = stream_shape();
+  if (rows) *rows = sh.rows;
+  if (unit_order) *unit_order = sh.col_major;
+  if (waves_per_cu) *waves_per_cu = sh.waves_per_cu;
+  return FAP_OK;
+}
+
+extern "C" int fap_hip_write_probe(int32_t *d, int64_t ld, int32_t rows, int32_t cols,
+                                   int32_t sample_stride, void *stream, float *ms) {
+  if (!initialized()) {
+    set_error("fap_hip_init has not been called");
+    return FAP_ERR_STATE;
+  }
+  FAP_REQUIRE(d && rows >= 0 && cols >= 0 && ld >= cols && sample_stride >= 1 && ms,
+              "fap_hip_write_probe: invalid arguments");
+  const hipStream_t st = resolve(stream);
+  hipEvent_t e0, e1;
+  FAP_CHECK_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    set_error("fap_hip_write_probe: event creation failed");
+    return FAP_ERR_HIP;
+  }
+  int s = hipEventRecord(e0, st) == hipSuccess ? FAP_OK : FAP_ERR_HIP;
+  if (s == FAP_OK) s = launch_write_probe(d, ld, rows, cols, stream_shape().rows, sample_stride, 0, st);
+  if (s == FAP_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                      hipEventElapsedTime(ms, e0, e1) != hipSuccess)) {
+    set_error("fap_hip_write_probe: timing failed");
+    s = FAP_ERR_HIP;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return s;
+}
+
+extern "C" int fap_hip_malloc_placed(void **d_ptr, int64_t rows, int64_t ld, int32_t candidates,
+                                     float *probe_ms) {
+  if (!initialized()) {
+    set_error("fap_hip_init has not been called");
+    return FAP_ERR_STATE;
+  }
+  FAP_REQUIRE(d_ptr && rows > 0 && ld > 0 && rows < (1LL << 31) && ld < (1LL << 31) &&
+                  candidates >= 1 && candidates <= 8,
+              "fap_hip_malloc_placed: invalid arguments");
+  *d_ptr = nullptr;
+  const int64_t bytes = rows * ld * 4;
+  const hipStream_t st = resolve(nullptr);
+  std::vector<std::pair<void *, float>> got;
+  auto release = [&](void *keep) {
+    for (auto &x : got)
+      if (x.first != keep) (void)dev_free(x.first);
+  };
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (candidates > 1) {
+    FAP_CHECK_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+      (void)hipEventDestroy(e0);
+      set_error("fap_hip_malloc_placed: event creation failed");
+      return FAP_ERR_HIP;
+    }
+  }
+  int status = FAP_OK;
+  for (int c = 0; c < candidates && status == FAP_OK; ++c) {
+    void *p = nullptr;
+    const int s = dev_alloc(&p, bytes);
+    if (s != FAP_OK) {
+      if (c == 0) status = s;  // not even one buffer: the caller's error
+      else clear_error();      // no room for another candidate: choose among those held
+      break;
+    }
+    float best = 0.0f;
+    if (candidates > 1) {
+      best = 1e30f;
+      for (int rep = 0; rep < 2 && status == FAP_OK; ++rep) {
+        float ms = 0.0f;
+        if (hipEventRecord(e0, st) != hipSuccess) status = FAP_ERR_HIP;
+        if (status == FAP_OK)
+          status = launch_write_probe(static_cast<int *>(p), ld, (int)rows, (int)ld,
+                                      stream_shape().rows, kPlacementSample, 0, st);
+        if (status == FAP_OK &&
+            (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+             hipEventElapsedTime(&ms, e0, e1) != hipSuccess)) {
+          set_error("fap_hip_malloc_placed: probe timing failed");
+          status = FAP_ERR_HIP;
+        }
+        best = std::min(best, ms);
+      }
+    }
+    got.push_back({p, best});
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (status != FAP_OK) {
+    release(nullptr);
+    return status;
+  }
+  size_t pick = 0;
+  for (size_t k = 1; k < got.size(); ++k)
+    if (got[k].second < got[pick].second) pick = k;
+  release(got[pick].first);
+  *d_ptr = got[pick].first;
+  if (probe_ms) *probe_ms = got[pick].second;
+  return FAP_OK;
+}
 
 extern "C" int fap_hip_minplus(const int32_t *d_A, int64_t lda, const int32_t *d_B, int64_t ldb,
                                int32_t *d_C, int64_t ldc, int32_t M, int32_t N, int32_t K,

@@ -63,12 +63,12 @@ int launch_bfs_lds(const int4 *d_ell, const int *d_row_off, const int *d_col, in
 int launch_merge(const MergeDesc *d_descs, int nd, int total_tiles, int N, bool vec,
                  const int *d_graph_id, const int *d_ed2st, hipStream_t stream);
 int merge_tiles(int M, int N);
-int merge_stream_units(int M, int N);
-int merge_stream_row_blocks(int M);
+int merge_stream_units(int M, int N, const StreamShape &sh);
+int merge_stream_row_blocks(int M, const StreamShape &sh);
 int merge_stream_kpad(int K);
 int launch_merge_stream(const MergeDesc *d_descs, int nd, int total_units, int N, int kp,
                         const int *d_graph_id, const int *d_ed2st, const int *d_max_b, int lim16,
-                        hipStream_t stream);
+                        const StreamShape &sh, hipStream_t stream);
 int launch_inpart_bfs(const InpartDesc *d_descs, int nd, int total_groups, int s_max,
                       const int *row_off, const int *col, const int *graph_id, const int *st2ed,
                       const int *ed2st, hipStream_t stream);
@@ -354,6 +354,7 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   std::vector<MergeDesc> mdg;
   std::vector<MergeGroup> groups;
   std::vector<int> part_merge(P, -1);  // plain form: the group of part p
+  const StreamShape shape = stream_shape();  // one snapshot for the plan and the launches
   {
     std::vector<int> order;
     for (int p = 0; p < P; ++p)
@@ -366,8 +367,8 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
       MergeDesc m = md[p];
       const int kind = merge_kind(m.K);
       const bool stream_kernel = kind > 0;
-      m.row_blocks = stream_kernel ? merge_stream_row_blocks(m.M) : (m.M + 63) / 64;
-      const int64_t units = stream_kernel ? merge_stream_units(m.M, n) : merge_tiles(m.M, n);
+      m.row_blocks = stream_kernel ? merge_stream_row_blocks(m.M, shape) : (m.M + 63) / 64;
+      const int64_t units = stream_kernel ? merge_stream_units(m.M, n, shape) : merge_tiles(m.M, n);
       const bool new_group = !batched || groups.empty() || groups.back().kind != kind ||
                              groups.back().chunk != chunk_of[p];
       if (new_group) groups.push_back(MergeGroup{kind, (int)mdg.size(), 0, 0, chunk_of[p]});
@@ -548,7 +549,7 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
                           on);
     return launch_merge_stream(d_md + gr.first, gr.count, gr.units, n, gr.kind, g->d_graph_id,
                                g->d_ed2st,
-                               merge16 && bg_total > 0 ? d_maxb : nullptr, lim16, on);
+                               merge16 && bg_total > 0 ? d_maxb : nullptr, lim16, shape, on);
   };
 
   // Boundary graph: vertices, closure (its distance matrix), then every part's

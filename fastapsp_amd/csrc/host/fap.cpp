@@ -20,6 +20,11 @@ const float MAXVALUE = 1e8f;
 
 namespace {
 
+// Result buffers from this size on are placed by probe or by solving (the merge's
+// speed depends on where the buffer lies in device memory).
+constexpr int64_t kPlacedBytes = int64_t(4) << 30;
+constexpr int kPlaceCandidates = 3;
+
 void check(int status, const char *what) {
   if (status != FAP_OK) {
     std::ostringstream os;
@@ -290,28 +295,84 @@ void fapGraph::run_path(const fap_part_desc *parts, int32_t nparts, int32_t *d_o
   add_stage_times(nparts);
 }
 
+// The n x n result buffers: from 4 GiB on, placed by write probe among four
+// candidate allocations (the merge writes some ranges of device memory ~15 % faster
+// than others; the probe costs about 2 ms per candidate for a 52.5 GB buffer).
+void fapGraph::alloc_result(int32_t **d, const char *what) {
+  const int64_t bytes = (int64_t)num_vertexs * ld_dist * 4;
+  if (bytes >= kPlacedBytes)
+    check(fap_hip_malloc_placed((void **)d, num_vertexs, ld_dist, 4, nullptr), what);
+  else
+    check(fap_hip_malloc((void **)d, bytes), what);
+}
+
+// The first distance-only solve of a graph whose result reaches kPlacedBytes places
+// the result by solving: the merge writes the result buffer and reads the boundary
+// rows back from it, and how fast depends on where the buffer lies in device memory
+// (the 114,599-vertex headline merge measured 8.9, 9.3 and 10.2-10.6 ms in different
+// allocations of the same size; a pure write probe separates the last group only).
+// The solve runs into up to three buffers held side by side (fewer if memory or the
+// budget runs out) and the one whose merge stage was fastest is kept with its result;
+// later solves reuse it.
+void fapGraph::solve_placed(const fap_graph_dev &g, const std::vector<fap_part_desc> &parts) {
+  const int64_t bytes = (int64_t)num_vertexs * ld_dist * 4;
+  std::vector<std::pair<int32_t *, float>> cand;
+  for (int c = 0; c < kPlaceCandidates; ++c) {
+    int32_t *p = nullptr;
+    const int s = fap_hip_malloc((void **)&p, bytes);
+    if (s != FAP_OK) {
+      if (c == 0) check(s, "n x n distance allocation");
+      break;  // no room for another candidate: choose among those held
+    }
+    cand.push_back({p, 0.0f});
+    const int r = float_weights
+                      ? fap_hip_solve_parts_f32(&g, parts.data(), (int32_t)parts.size(),
+                                                reinterpret_cast<float *>(p), ld_dist, mode,
+                                                nullptr)
+                      : fap_hip_solve_parts(&g, parts.data(), (int32_t)parts.size(), p, ld_dist,
+                                            mode, nullptr);
+    fap_stage_times t{};
+    int st = r == FAP_OK ? fap_hip_stream_sync(nullptr) : r;
+    if (st == FAP_OK) st = fap_hip_last_stage_times(&t);
+    if (st != FAP_OK) {
+      for (auto &x : cand) fap_hip_free(x.first);
+      check(st, "solve");
+    }
+    cand.back().second = t.minplus_ms;
+  }
+  size_t best = 0;
+  for (size_t k = 1; k < cand.size(); ++k)
+    if (cand[k].second < cand[best].second) best = k;
+  for (size_t k = 0; k < cand.size(); ++k)
+    if (k != best) fap_hip_free(cand[k].first);
+  d_dist = cand[best].first;
+}
+
 int32_t fapGraph::solve(bool path_needed) {
   if (!preconditioned) throw FapError(FAP_ERR_STATE, "solve() before preCondition()");
   if (path_needed) require_int("predecessor matrices");
   if (!d_row_offset) upload();
-  if (!d_dist)
-    check(fap_hip_malloc((void **)&d_dist, (int64_t)num_vertexs * ld_dist * 4),
-          "n x n distance allocation");
+  const bool place = !d_dist && !path_needed &&
+                     (int64_t)num_vertexs * ld_dist * 4 >= kPlacedBytes;
+  if (!d_dist && !place) alloc_result(&d_dist, "n x n distance allocation");
   std::vector<fap_part_desc> parts;
   for (int p = 1; p <= K; ++p)
     parts.push_back(fap_part_desc{p, C_BlockVer_offset[p], C_BlockVer_num[p], C_BlockBdy_num[p],
                                   (int64_t)C_BlockVer_offset[p]});
   have_path = false;
   if (path_needed) {
-    if (!d_path)
-      check(fap_hip_malloc((void **)&d_path, (int64_t)num_vertexs * ld_dist * 4),
-            "n x n path allocation");
+    if (!d_path) alloc_result(&d_path, "n x n path allocation");
     run_path(parts.data(), (int32_t)parts.size(), d_dist, d_path);
     have_path = true;
     return 0;
   }
   fap_graph_dev g{num_vertexs, num_edges, d_row_offset, d_col, d_w,
                   d_graph_id,  d_st2ed,   d_ed2st,      d_comp, d_wf};
+  if (!d_dist) {
+    solve_placed(g, parts);
+    add_stage_times((int32_t)parts.size());
+    return 0;
+  }
   if (float_weights)
     check(fap_hip_solve_parts_f32(&g, parts.data(), (int32_t)parts.size(),
                                   reinterpret_cast<float *>(d_dist), ld_dist, mode, nullptr),
@@ -543,9 +604,7 @@ std::vector<fap_part_desc> fapGraph::partList() const {
 
 void *fapGraph::resultBuffer(int64_t *ld) {
   if (!d_row_offset) upload();
-  if (!d_dist)
-    check(fap_hip_malloc((void **)&d_dist, (int64_t)num_vertexs * ld_dist * 4),
-          "n x n distance allocation");
+  if (!d_dist) alloc_result(&d_dist, "n x n distance allocation");
   if (ld) *ld = ld_dist;
   return d_dist;
 }

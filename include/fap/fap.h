@@ -104,6 +104,8 @@ class fapGraph {
   void upload();
   void release();
   void add_stage_times(int32_t parts);
+  void alloc_result(int32_t **d, const char *what);
+  void solve_placed(const fap_graph_dev &g, const std::vector<fap_part_desc> &parts);
   void copy_rows(const int32_t *base, int64_t nrows, int64_t row0, int64_t rows, void *out) const;
   void require_int(const char *what) const;
   void upload_reverse();

@@ -292,6 +292,39 @@ int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *parts, int3
                         int32_t *d_out, int64_t ld_out, int32_t mode, void *stream);
 int fap_hip_last_stage_times(fap_stage_times *t);
 
+/* Work-unit shape of the streaming min-plus merge of fap_hip_solve_parts (the
+ * MatrixMulCUDA6_path launches of fap/kernel/cuda/minplus.cu, for K <= 32): rows per
+ * unit (8..1024, a multiple of 8; one wave keeps a K x 256-column panel of boundary rows
+ * in registers and writes `rows` rows of 256 columns), unit order (1: the column blocks
+ * of one row block back to back, 0: the row blocks of one column block) and a cap on
+ * the resident waves per CU (0: none, else 4..32).  Results do not depend on the shape.  Default
+ * 128 / 1 / 0.  A solve reads the shape once, when it plans its merge; set it between
+ * solves.  No counterpart in the reference (its tile is fixed at compile time). */
+int fap_hip_set_merge_shape(int32_t rows, int32_t unit_order, int32_t waves_per_cu);
+int fap_hip_get_merge_shape(int32_t *rows, int32_t *unit_order, int32_t *waves_per_cu);
+
+/* Write probe: stores into `rows` x `cols` int32 entries (row stride ld) with the
+ * streaming merge's store pattern (units of the merge shape's rows x 256 columns, one
+ * 1 KB row segment per wave and row), only every sample_stride-th row block, and
+ * returns the device time in *ms (the call waits for the stream).  Overwrites the
+ * buffer.  How fast the merge writes a result buffer depends on where the buffer
+ * lies in device memory; the probe measures it.  No counterpart in the reference. */
+int fap_hip_write_probe(int32_t *d, int64_t ld, int32_t rows, int32_t cols,
+                        int32_t sample_stride, void *stream, float *ms);
+
+/* Allocates a rows x ld int32 result buffer (as fap_hip_malloc; free it with
+ * fap_hip_free) placed where the merge writes fastest: up to `candidates` buffers of
+ * that size are allocated side by side (as many as memory and the budget allow, at
+ * least one), each is timed with the write probe on one row block in 8, the fastest
+ * is kept and the others are freed.  On MI355X a linear fill of 52.5 GB measured 9.5 ms
+ * in some ranges of device memory and 7.9-8.1 ms in others, and the headline merge
+ * 10.6 against 9.0 ms; which range a new allocation lands in varies.  *probe_ms (may
+ * be NULL) receives the kept buffer's probe time (0 when candidates == 1).  The buffer
+ * content is undefined.  The high-water mark counts the candidates held at once.  No
+ * counterpart in the reference. */
+int fap_hip_malloc_placed(void **d_ptr, int64_t rows, int64_t ld, int32_t candidates,
+                          float *probe_ms);
+
 /* The same pipeline with predecessor matrices, as fapGraph::run with is_path_needed
  * (fap/fap.cpp:195-503) computes them on its CPU path: d_path[row][v] = the vertex
  * before v on the chosen shortest path from the row's source, -1 if unreachable, the

@@ -141,6 +141,11 @@ int fap_hip_free(void *p) {
   return FAP_OK;
 }
 
+int fap_hip_malloc_placed(void **p, int64_t rows, int64_t ld, int32_t, float *probe_ms) {
+  if (probe_ms) *probe_ms = 0.0f;
+  return fap_hip_malloc(p, rows * ld * 4);
+}
+
 int fap_hip_mem_stats(int64_t *in_use, int64_t *high_water, int64_t *budget) {
   std::lock_guard<std::mutex> lk(g_mem_mu);
   if (in_use) *in_use = g_in_use;

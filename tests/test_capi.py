@@ -65,3 +65,24 @@ def test_status_codes_without_device():
     if not L._initialised:
         assert lib.fap_hip_minplus(None, 1, None, 1, None, 1, 1, 1, 1, None) == L.FAP_ERR_STATE
         assert "fap_hip_init" in L.last_error()
+
+
+def test_merge_shape_setter_validates_and_round_trips():
+    """fap_hip_set_merge_shape is host state only: bad shapes are refused with
+    FAP_ERR_INVALID and leave the shape unchanged; good ones read back."""
+    lib = L.load()
+    get = lambda: tuple(v.value for v in _get_shape(lib))  # noqa: E731
+    before = get()
+    for bad in [(0, 1, 0), (12, 1, 0), (2048, 1, 0), (128, 2, 0), (128, 1, 2), (128, 1, 33)]:
+        assert lib.fap_hip_set_merge_shape(*bad) == L.FAP_ERR_INVALID, bad
+        assert get() == before
+    try:
+        assert lib.fap_hip_set_merge_shape(32, 0, 8) == 0 and get() == (32, 0, 8)
+    finally:
+        assert lib.fap_hip_set_merge_shape(*before) == 0
+
+
+def _get_shape(lib):
+    vals = [C.c_int32(), C.c_int32(), C.c_int32()]
+    assert lib.fap_hip_get_merge_shape(*[C.byref(v) for v in vals]) == 0
+    return vals
