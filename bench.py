@@ -203,7 +203,7 @@ def batch_bench(args):
     t0 = time.perf_counter()
     if getattr(args, "no_tune", False):  # fixed settings (profiling runs)
         table = {"K": [8, 8, 8, 16, 16], "per_cu": [0] * 5, "src_rows": [32] * 5,
-                 "merge_rows": [8] * 5, "packed": [1] * 5, "ms_per_graph": [None] * 5}
+                 "merge_rows": [16] * 5, "packed": [1] * 5, "ms_per_graph": [None] * 5}
     elif getattr(args, "tune_quick", False):
         table = AT.tune(k_candidates=(8, 16), per_cu_candidates=(0,), src_rows_candidates=(16,),
                         merge_rows_candidates=(8, 16), max_copies=16)

@@ -32,10 +32,11 @@ class Batch:
     src_rows: boundary sources per unit of the boundary-row product (8, 16, 32; 0 =
     16); merge_rows: inner rows per wave of the merge (8 or 16; 0 = 16, FP64 always 8);
     packed: 1 = the int32 merge of unit-weight graphs (n <= 15872) on packed 16-bit
-    distances, 0 = on int32."""
+    distances, 0 = on int32; bfs: 1 = in-part distances of unit-weight symmetric graphs
+    by bit-parallel BFS, 0 = by Floyd-Warshall."""
 
     def __init__(self, graphs: Sequence, use_graph: bool = True, per_cu: int = 0,
-                 src_rows: int = 0, merge_rows: int = 0, packed: int = 1):
+                 src_rows: int = 0, merge_rows: int = 0, packed: int = 1, bfs: int = 1):
         self.graphs = list(graphs)
         for g in self.graphs:
             if not getattr(g, "_preconditioned", False):
@@ -48,7 +49,8 @@ class Batch:
                                                 | ((int(per_cu) & 0xFF) << 8)
                                                 | (_ROWS_CODE[int(src_rows)] << 16)
                                                 | (_ROWS_CODE[int(merge_rows)] << 18)
-                                                | (0 if packed else L.FAP_BATCH_NO_PACKED)),
+                                                | (0 if packed else L.FAP_BATCH_NO_PACKED)
+                                                | (0 if bfs else L.FAP_BATCH_NO_BFS)),
                 "batch create")
         self._h = h.value
 

@@ -44,6 +44,8 @@ namespace fap_hip {
 
 int launch_fw(const FwDesc *d_descs, int ndesc, int max_nb, hipStream_t stream);
 int launch_fw64(const Fw64Desc *d_descs, int nd, int max_nb, hipStream_t st);
+int launch_inpart_bfs(const InpartDesc *d_descs, int nd, int total_groups, int s_max,
+                      hipStream_t stream);
 
 namespace {
 
@@ -83,6 +85,7 @@ struct BPart {
   int S, g, part_id, off, s, b;
   int bs;                        // boundary-graph index of its first boundary vertex
   int row_begin;                 // first build row (prefix of S)
+  int bfs;                       // in-part distances by bit-parallel BFS (no build, no FW)
 };
 
 struct BRowUnit {                // stage 5: rows r0 .. of graph parts[q].g, columns of q
@@ -242,6 +245,7 @@ __global__ __launch_bounds__(256) void bbuild_kernel(const BGraph *__restrict__ 
   if (row >= total_rows) return;
   const int pi = upper_index(np, row, [&](int k) { return parts[k].row_begin; });
   const BPart p = parts[pi];
+  if (p.bfs) return;  // the BFS writes the whole matrix
   const BGraph g = graphs[p.g];
   const int i = row - p.row_begin;
   FAP_GLOBAL T *D = gptr(static_cast<T *>(p.D)) + (int64_t)i * p.S;
@@ -387,6 +391,11 @@ __global__ __launch_bounds__(256) void bconv_kernel(const BGraph *__restrict__ g
 // contiguous bytes.  The 4*RB x 64 A block is staged k-major in LDS (broadcast reads);
 // the four waves read the same B rows (L1).  Own columns take min(d_p(i, v), product).
 constexpr int kMWaves = 4;
+// Row padding of the staged A block: 16 bytes, so the staging writes (64 lanes on 64
+// consecutive k of one row) spread over 16 banks instead of one, and the broadcast
+// reads of a row's entries stay 16-byte aligned.
+template <typename T>
+constexpr int kAPad = 16 / (int)sizeof(T);
 
 // One merge unit.  kPacked (int32, unit weights, n <= kCap16): A and B capped at kCap16,
 // two columns per register, a step on two entries one v_pk_add_u16 and half a
@@ -394,19 +403,21 @@ constexpr int kMWaves = 4;
 // the cap is unreachable.
 template <typename T, int RB, bool kPacked>
 __device__ __forceinline__ void merge_unit(const BGraph &g, const BPart &p, int rb, int cb,
-                                           T (*As)[kMWaves * RB], int tid) {
+                                           T (*As)[kMWaves * RB + kAPad<T>], int tid) {
   typedef typename Dist<T>::Out Out;
   constexpr int kRows = kMWaves * RB;
+  constexpr int kLdA = kRows + kAPad<T>;
   const int wave = tid >> 6, lane = tid & 63;
   const int inner = p.s - p.b, r0 = rb * kRows;
   const FAP_GLOBAL T *D = gptr(static_cast<const T *>(p.D));
   const FAP_GLOBAL T *B = gptr(static_cast<const T *>(p.Brows));
-  int col[4];
+  int col[4], colc[4];  // colc: clamped to the last column, so every load is in bounds
   bool cok[4];
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     col[e] = cb * kMCols + e * 64 + lane;
     cok[e] = col[e] < g.n;
+    colc[e] = min(col[e], g.n - 1);
   }
   constexpr int kAcc = kPacked ? 2 : 4;
   T acc[RB][kAcc];
@@ -428,47 +439,77 @@ __device__ __forceinline__ void merge_unit(const BGraph &g, const BPart &p, int 
     }
     __syncthreads();
     const T *Aw = &As[0][wave * RB];
-    // B rows k and k + 1 of this chunk (INF past kc or past the last column).
-    auto load_pair = [&](int k, T (&x0)[4], T (&x1)[4]) {
-      const FAP_GLOBAL T *r0p = B + (int64_t)(k0 + k) * p.ldb;
+    // B row k of this chunk: loads always in bounds (clamped column), INF selected
+    // afterwards for columns past the last, so no load sits behind a branch.
+    auto load_row = [&](int k, T (&x)[4]) {
+      const FAP_GLOBAL T *rp = B + (int64_t)(k0 + k) * p.ldb;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        x0[e] = (cok[e] && k < kc) ? r0p[col[e]] : Dist<T>::inf();
-        x1[e] = (cok[e] && k + 1 < kc) ? r0p[p.ldb + col[e]] : Dist<T>::inf();
+        const T v = rp[colc[e]];
+        x[e] = cok[e] ? v : Dist<T>::inf();
       }
     };
-    for (int k = 0; k < kc; k += 2) {
-      T b0[4], b1[4];
-      load_pair(k, b0, b1);
-      T a0[RB], a1[RB];
+    if constexpr (sizeof(T) == 8) {
+      // FP64 has no three-operand minimum: one k per step (fewer live registers).
+      for (int k = 0; k < kc; ++k) {
+        T bv[4];
+        load_row(k, bv);
 #pragma unroll
-      for (int r = 0; r < RB; ++r) {
-        a0[r] = Aw[k * kRows + r];
-        a1[r] = Aw[(k + 1) * kRows + r];  // row kc of As is INF when kc is odd
+        for (int r = 0; r < RB; ++r) {
+          const T a = Aw[k * kLdA + r];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) acc[r][e] = Dist<T>::mn(acc[r][e], a + bv[e]);
+        }
       }
-      if constexpr (kPacked) {
-        auto cap = [](T v) -> unsigned {
-          const unsigned u = (unsigned)v;
-          return u < (unsigned)kCap16 ? u : (unsigned)kCap16;
-        };
-        const unsigned p0[2] = {cap(b0[0]) | (cap(b0[1]) << 16u), cap(b0[2]) | (cap(b0[3]) << 16u)};
-        const unsigned p1[2] = {cap(b1[0]) | (cap(b1[1]) << 16u), cap(b1[2]) | (cap(b1[3]) << 16u)};
+    } else {
+      // B rows k and k + 1 of this chunk (INF past kc or past the last column).  Masked
+      // loads here: the clamped form holds more registers (136 against 108 VGPRs for 16
+      // rows per wave, one wave less per SIMD).
+      auto load_pair = [&](int k, T (&x0)[4], T (&x1)[4]) {
+        const FAP_GLOBAL T *r0p = B + (int64_t)(k0 + k) * p.ldb;
 #pragma unroll
-        for (int r = 0; r < RB; ++r)
+        for (int e = 0; e < 4; ++e) {
+          x0[e] = (cok[e] && k < kc) ? r0p[col[e]] : Dist<T>::inf();
+          x1[e] = (cok[e] && k + 1 < kc) ? r0p[p.ldb + col[e]] : Dist<T>::inf();
+        }
+      };
+      for (int k = 0; k < kc; k += 2) {
+        T b0[4], b1[4];
+        load_pair(k, b0, b1);
+        T a0[RB], a1[RB];
 #pragma unroll
-          for (int h = 0; h < 2; ++h) {
-            const unsigned s0 = __builtin_bit_cast(
-                unsigned, (us2)(__builtin_bit_cast(us2, (unsigned)a0[r]) + __builtin_bit_cast(us2, p0[h])));
-            const unsigned s1 = __builtin_bit_cast(
-                unsigned, (us2)(__builtin_bit_cast(us2, (unsigned)a1[r]) + __builtin_bit_cast(us2, p1[h])));
-            acc[r][h] = (T)pk_min3((unsigned)acc[r][h], s0, s1);
-          }
-      } else {
+        for (int r = 0; r < RB; ++r) {
+          a0[r] = Aw[k * kLdA + r];
+          a1[r] = Aw[(k + 1) * kLdA + r];  // row kc of As is INF when kc is odd
+        }
+        if constexpr (kPacked) {
+          auto cap = [](T v) -> unsigned {
+            const unsigned u = (unsigned)v;
+            return u < (unsigned)kCap16 ? u : (unsigned)kCap16;
+          };
+          const unsigned p0[2] = {cap(b0[0]) | (cap(b0[1]) << 16u),
+                                  cap(b0[2]) | (cap(b0[3]) << 16u)};
+          const unsigned p1[2] = {cap(b1[0]) | (cap(b1[1]) << 16u),
+                                  cap(b1[2]) | (cap(b1[3]) << 16u)};
 #pragma unroll
-        for (int r = 0; r < RB; ++r)
+          for (int r = 0; r < RB; ++r)
 #pragma unroll
-          for (int e = 0; e < 4; ++e)
-            acc[r][e] = Dist<T>::mn3(acc[r][e], a0[r] + b0[e], a1[r] + b1[e]);
+            for (int h = 0; h < 2; ++h) {
+              const unsigned s0 = __builtin_bit_cast(
+                  unsigned,
+                  (us2)(__builtin_bit_cast(us2, (unsigned)a0[r]) + __builtin_bit_cast(us2, p0[h])));
+              const unsigned s1 = __builtin_bit_cast(
+                  unsigned,
+                  (us2)(__builtin_bit_cast(us2, (unsigned)a1[r]) + __builtin_bit_cast(us2, p1[h])));
+              acc[r][h] = (T)pk_min3((unsigned)acc[r][h], s0, s1);
+            }
+        } else {
+#pragma unroll
+          for (int r = 0; r < RB; ++r)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              acc[r][e] = Dist<T>::mn3(acc[r][e], a0[r] + b0[e], a1[r] + b1[e]);
+        }
       }
     }
     __syncthreads();
@@ -513,7 +554,7 @@ __global__ __launch_bounds__(256) void bmerge_kernel(const BGraph *__restrict__ 
                                                      const BPart *__restrict__ parts,
                                                      const BMerge *__restrict__ units,
                                                      int total_units, int *counter) {
-  __shared__ __attribute__((aligned(16))) T As[kKc][kMWaves * RB];
+  __shared__ __attribute__((aligned(16))) T As[kKc][kMWaves * RB + kAPad<T>];
   __shared__ int unit_s;
   const int tid = threadIdx.x;
   while (true) {
@@ -565,6 +606,11 @@ struct fap_batch_kind {
   int ngb = 0, gb_rows = 0, gb_max_nb = 0, nunits5 = 0, nmerge = 0, merge_units = 0, nsrc = 0;
   int rows6 = 16;  // merge rows per wave (FP64: 8, register budget)
   int grid = 0;    // persistent merge workgroups
+  int nfw = 0;     // parts whose in-part matrix comes from Floyd-Warshall
+  // In-part BFS descriptors: parts of at most 1024 vertices first (n2 of them, g2
+  // source groups), then those up to kInpartBfsMaxS (n4, g4).
+  int nip2 = 0, gip2 = 0, nip4 = 0, gip4 = 0;
+  InpartDesc *ip = nullptr;
   char *d_mem = nullptr;  // tables and scratch in one allocation
   BGraph *graphs = nullptr;
   BPart *parts = nullptr;
@@ -581,6 +627,7 @@ struct fap_batch {
   int *d_counter = nullptr;
   int per_cu = 0, rows5 = 16, rows6 = 16;
   bool packed = true;  // packed 16-bit merge where exact
+  bool bfs = true;     // in-part BFS where exact
   bool use_graph = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -608,10 +655,14 @@ int launch_kind(const fap_batch *b, const fap_batch_kind &q, int *counter, hipSt
                        dim3(64 * kSWaves), 0, st, q.graphs, q.parts, q.srcs, q.nsrc);
     FAP_CHECK_LAUNCH();
   }
-  hipLaunchKernelGGL(bbuild_kernel<T>, dim3((q.build_rows + 3) / 4), dim3(256), 0, st, q.graphs,
-                     q.parts, q.nparts, q.build_rows);
-  FAP_CHECK_LAUNCH();
-  FAP_TRY(launch_fw_chunks<T>(q.fw_descs, q.nparts, q.max_nb, st));
+  if (q.nfw > 0) {
+    hipLaunchKernelGGL(bbuild_kernel<T>, dim3((q.build_rows + 3) / 4), dim3(256), 0, st,
+                       q.graphs, q.parts, q.nparts, q.build_rows);
+    FAP_CHECK_LAUNCH();
+    FAP_TRY(launch_fw_chunks<T>(q.fw_descs, q.nfw, q.max_nb, st));
+  }
+  if (q.nip2 > 0) FAP_TRY(launch_inpart_bfs(q.ip, q.nip2, q.gip2, 1024, st));
+  if (q.nip4 > 0) FAP_TRY(launch_inpart_bfs(q.ip + q.nip2, q.nip4, q.gip4, kInpartBfsMaxS, st));
   if (q.ngb > 0) {
     hipLaunchKernelGGL(bgb_kernel<T>, dim3((q.gb_rows + 3) / 4), dim3(256), 0, st, q.graphs,
                        q.parts, q.gbg, q.ngb, q.bpart, q.pid2idx, q.gb_rows);
@@ -747,6 +798,10 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
       P.bs = nb;
       P.row_begin = build_rows;
       P.out_row = pd->out_row;
+      // Unit weights on paired arcs (component labels present): the in-part distances
+      // of a full item are BFS distances, symmetric.
+      P.bfs = !fp64 && b->bfs && G.full && d.d_w == nullptr && d.d_wf == nullptr &&
+              d.d_comp != nullptr && P.s <= kInpartBfsMaxS;
       build_rows += P.S;
       max_nb = std::max(max_nb, P.S / kBT);
       d_at.push_back(take((int64_t)P.S * P.S * elem));
@@ -813,6 +868,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   const int64_t o_merge = table(sizeof(BMerge) * merge.size());
   const int64_t o_src = table(sizeof(BSrc) * srcs.size());
   const int64_t o_fw = table((fp64 ? sizeof(Fw64Desc) : sizeof(FwDesc)) * parts.size());
+  const int64_t o_ip = table(sizeof(InpartDesc) * parts.size());
   const int64_t o_gbd = table((fp64 ? sizeof(Fw64Desc) : sizeof(FwDesc)) * gbg.size());
   FAP_TRY(dev_alloc((void **)&q.d_mem, std::max<int64_t>(scratch, 256)));
   char *base = q.d_mem;
@@ -859,13 +915,31 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
       gd.push_back(Fw64Desc{static_cast<double *>(graphs[gi].MB), graphs[gi].SB / kBT});
     put(o_fw, fd.data(), sizeof(Fw64Desc) * fd.size());
     put(o_gbd, gd.data(), sizeof(Fw64Desc) * gd.size());
+    q.nfw = (int)fd.size();
   } else {
     std::vector<FwDesc> fd, gd;
-    for (const BPart &P : parts) fd.push_back(FwDesc{static_cast<int *>(P.D), P.S, P.S / kBT});
+    std::vector<InpartDesc> ip2, ip4;
+    for (const BPart &P : parts) {
+      if (!P.bfs) {
+        fd.push_back(FwDesc{static_cast<int *>(P.D), P.S, P.S / kBT});
+        continue;
+      }
+      const BGraph &G = graphs[P.g];
+      std::vector<InpartDesc> &l = P.s <= 1024 ? ip2 : ip4;
+      int &groups = P.s <= 1024 ? q.gip2 : q.gip4;
+      l.push_back(InpartDesc{static_cast<int *>(P.D), P.S, P.s, P.off, P.part_id, groups, 0,
+                             G.row_off, G.col, G.graph_id, G.st2ed, G.ed2st});
+      groups += (P.s + 63) / 64;
+    }
     for (int gi : gbg)
       gd.push_back(FwDesc{static_cast<int *>(graphs[gi].MB), graphs[gi].SB, graphs[gi].SB / kBT});
     put(o_fw, fd.data(), sizeof(FwDesc) * fd.size());
     put(o_gbd, gd.data(), sizeof(FwDesc) * gd.size());
+    q.nfw = (int)fd.size();
+    q.nip2 = (int)ip2.size();
+    q.nip4 = (int)ip4.size();
+    ip2.insert(ip2.end(), ip4.begin(), ip4.end());
+    put(o_ip, ip2.data(), sizeof(InpartDesc) * ip2.size());
   }
   if (hipMemcpy(base + o_graphs, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) {
     (void)hipGetLastError();
@@ -893,6 +967,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   q.merge = reinterpret_cast<BMerge *>(base + o_merge);
   q.srcs = reinterpret_cast<BSrc *>(base + o_src);
   q.fw_descs = base + o_fw;
+  q.ip = reinterpret_cast<InpartDesc *>(base + o_ip);
   q.gb_descs = base + o_gbd;
   return FAP_OK;
 }
@@ -917,7 +992,8 @@ extern "C" int fap_hip_batch_create(fap_batch **out, const fap_batch_item *items
     return FAP_ERR_STATE;
   }
   FAP_REQUIRE(out && nitems >= 0 && (nitems == 0 || items), "fap_hip_batch_create: null argument");
-  FAP_REQUIRE((flags & ~(FAP_BATCH_GRAPH | 0xFF00 | 0xF0000 | FAP_BATCH_NO_PACKED)) == 0,
+  FAP_REQUIRE((flags & ~(FAP_BATCH_GRAPH | 0xFF00 | 0xF0000 | FAP_BATCH_NO_PACKED |
+                         FAP_BATCH_NO_BFS)) == 0,
               "fap_hip_batch_create: unknown flags %d", flags);
   *out = nullptr;
   std::vector<const fap_batch_item *> kinds[2];
@@ -936,6 +1012,7 @@ extern "C" int fap_hip_batch_create(fap_batch **out, const fap_batch_item *items
   b->rows5 = pick_rows((flags >> 16) & 0x3, 16);
   b->rows6 = pick_rows((flags >> 18) & 0x3, 16) == 8 ? 8 : 16;
   b->packed = (flags & FAP_BATCH_NO_PACKED) == 0;
+  b->bfs = (flags & FAP_BATCH_NO_BFS) == 0;
   int s = dev_alloc((void **)&b->d_counter, 256);
   for (int k = 0; k < 2 && s == FAP_OK; ++k)
     if (!kinds[k].empty()) s = plan_kind(b, b->kind[k], kinds[k], k == 1);

@@ -766,8 +766,7 @@ int launch_bg_rows(const BgPart *d_parts, const int64_t *d_orow, int nb_total, c
 }
 
 int launch_inpart_bfs(const InpartDesc *d_descs, int nd, int total_groups, int s_max,
-                      const int *row_off, const int *col, const int *graph_id, const int *st2ed,
-                      const int *ed2st, hipStream_t stream);
+                      hipStream_t stream);
 
 namespace {
 Workspace g_ub_meta;   // part tables, G_B vertices, output rows, clique weights, maximum
@@ -812,7 +811,9 @@ int boundary_rows_unit(const fap_graph_dev *g, const fap_part_desc *act, int P, 
   for (int p = 0; p < P; ++p) {
     const fap_part_desc &q = act[p];
     const int ga = std::min(S[p] / 64, (q.bdy + 63) / 64);
-    if (ga > 0) ipa.push_back(InpartDesc{fw[p], S[p], q.num, q.offset, q.part_id, groups, 0});
+    if (ga > 0)
+      ipa.push_back(InpartDesc{fw[p], S[p], q.num, q.offset, q.part_id, groups, 0,
+                               g->d_row_offset, g->d_col, g->d_graph_id, g->d_st2ed, g->d_ed2st});
     groups += ga;
     bgp[p] = BgPart{fw[p], S[p], q.bdy, q.offset, q.part_id, bs, cb, q.out_row};
     pidx[q.part_id] = p;
@@ -843,8 +844,7 @@ int boundary_rows_unit(const fap_graph_dev *g, const fap_part_desc *act, int P, 
   int *d_max = reinterpret_cast<int *>(meta + o_max);
   int *d_mat = static_cast<int *>(g_ub_mat.ptr);
   FAP_TRY(launch_inpart_bfs(reinterpret_cast<const InpartDesc *>(meta + o_ip), (int)ipa.size(),
-                            groups, s_max, g->d_row_offset, g->d_col, g->d_graph_id, g->d_st2ed,
-                            g->d_ed2st, st));
+                            groups, s_max, st));
   FAP_TRY(fap_hip_fill_i32(d_max, 0, 1, st));
   FAP_TRY(launch_bg_vert(d_bgp, P, nb_total, d_pidx, max_pid + 1, g->d_row_offset, g->d_col, nullptr,
                          g->d_graph_id, g->d_st2ed, g->d_ed2st, d_verts, d_wt, d_orow, st));

@@ -111,14 +111,17 @@ __device__ __forceinline__ void load_a_chunk(int (*As)[kT], const gint *g, int64
   }
 }
 
-// B tile rows k0 .. k0+31, columns 0..63 -> Bs[k][j].
+// B tile rows k0 .. k0+31, columns 0..63 -> Bs[k][j]: 16 lanes per row, four rows per
+// instruction, so the 16-byte LDS writes of each 8 lanes cover 128 contiguous bytes
+// (all banks once) and the global reads 256 contiguous bytes per row.
 __device__ __forceinline__ void load_b_chunk(int (*Bs)[kT], const gint *g, int64_t ld, int k0) {
-  const int k = threadIdx.x >> 1;
-  const int half = threadIdx.x & 1;
-  const gv4i *src = reinterpret_cast<const gv4i *>(g + (int64_t)(k0 + k) * ld + half * 32);
-  v4i *dst = reinterpret_cast<v4i *>(&Bs[k][half * 32]);
+  const int c4 = (threadIdx.x & 15) * 4;
 #pragma unroll
-  for (int c = 0; c < 8; ++c) dst[c] = src[c];
+  for (int q = 0; q < kKc / 4; ++q) {
+    const int k = 4 * q + (threadIdx.x >> 4);
+    *reinterpret_cast<v4i *>(&Bs[k][c4]) =
+        *reinterpret_cast<const gv4i *>(g + (int64_t)(k0 + k) * ld + c4);
+  }
 }
 
 __device__ __forceinline__ int own_row(int tr, int q) { return (q < 4 ? 0 : 32) + tr * 4 + (q & 3); }

@@ -246,10 +246,13 @@ __global__ __launch_bounds__(256) void fw64_phase1_kernel(const Fw64Desc *__rest
 // As[k][i] (transposed A, four consecutive rows read as two 16-byte broadcasts),
 // Bs[k][j] (16 consecutive lanes read 16 consecutive doubles: conflict-free).
 constexpr int kKc64 = 16;
+// As rows padded by 16 bytes: the staging writes (16 lanes on 16 consecutive k of one
+// row) land in 16 distinct bank pairs instead of one.
+constexpr int kLdA64 = kT + 2;
 
 __device__ __forceinline__ void tile_product64(double (&acc)[4][4], const FAP_GLOBAL double *a,
                                                const FAP_GLOBAL double *b, int64_t S,
-                                               double (*As)[kT], double (*Bs)[kT], int tr,
+                                               double (*As)[kLdA64], double (*Bs)[kT], int tr,
                                                int tc) {
   const int t = threadIdx.x;
   for (int k0 = 0; k0 < kT; k0 += kKc64) {
@@ -310,8 +313,8 @@ __global__ __launch_bounds__(256) void fw64_phase2_kernel(const Fw64Desc *__rest
   if (!is_row) o -= others;
   o = skip64(o, r);
   const int64_t S = (int64_t)d.nb * kT;
-  __shared__ double As[kKc64][kT];
-  __shared__ double Bs[kKc64][kT];
+  __shared__ __attribute__((aligned(16))) double As[kKc64][kLdA64];
+  __shared__ __attribute__((aligned(16))) double Bs[kKc64][kT];
   FAP_GLOBAL double *t = is_row ? tile64(d, r, o) : tile64(d, o, r);
   const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
   double acc[4][4];
@@ -328,8 +331,8 @@ __global__ __launch_bounds__(256) void fw64_phase3_kernel(const Fw64Desc *__rest
   if (r >= d.nb || o >= others * others) return;
   const int i = skip64(o / others, r), j = skip64(o % others, r);
   const int64_t S = (int64_t)d.nb * kT;
-  __shared__ double As[kKc64][kT];
-  __shared__ double Bs[kKc64][kT];
+  __shared__ __attribute__((aligned(16))) double As[kKc64][kLdA64];
+  __shared__ __attribute__((aligned(16))) double Bs[kKc64][kT];
   FAP_GLOBAL double *t = tile64(d, i, j);
   const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
   double acc[4][4];

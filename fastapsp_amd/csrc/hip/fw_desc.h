@@ -48,6 +48,9 @@ struct InpartDesc {
   int *fw;
   int S, s, offset, part_id;
   int group_begin, g0;
+  // The part's graph (CSR, part labels, sorted order), so one launch can serve parts
+  // of many graphs.
+  const int *row_off, *col, *graph_id, *st2ed, *ed2st;
 };
 constexpr int kInpartBfsMaxS = 2048;  // levels fit 11 bit-planes; 8 LDS bytes per vertex
 

@@ -72,12 +72,7 @@ __device__ __forceinline__ void bfs_gather(unsigned long long (&acc)[Q],
 
 template <int Q>
 __global__ __launch_bounds__(kThreads) void inpart_bfs_kernel(const InpartDesc *__restrict__ descs,
-                                                         int nd, int total_groups,
-                                                         const int *__restrict__ row_off,
-                                                         const int *__restrict__ col,
-                                                         const int *__restrict__ graph_id,
-                                                         const int *__restrict__ st2ed,
-                                                         const int *__restrict__ ed2st) {
+                                                         int nd, int total_groups) {
   // Frontier words, two buffers of bfs_stride<Q>() words (level L reads buffer
   // (L - 1) & 1 and writes the other: one barrier per level), then the three
   // level flags.
@@ -94,6 +89,11 @@ __global__ __launch_bounds__(kThreads) void inpart_bfs_kernel(const InpartDesc *
     else hi = mid - 1;
   }
   const InpartDesc d = descs[lo];
+  const int *__restrict__ row_off = d.row_off;
+  const int *__restrict__ col = d.col;
+  const int *__restrict__ graph_id = d.graph_id;
+  const int *__restrict__ st2ed = d.st2ed;
+  const int *__restrict__ ed2st = d.ed2st;
   const int src0 = (d.g0 + blk - d.group_begin) * 64;
   const int s = d.s, tid = threadIdx.x;
 
@@ -241,18 +241,15 @@ __global__ __launch_bounds__(kThreads) void inpart_bfs_kernel(const InpartDesc *
 
 
 int launch_inpart_bfs(const InpartDesc *d_descs, int nd, int total_groups, int s_max,
-                      const int *row_off, const int *col, const int *graph_id, const int *st2ed,
-                      const int *ed2st, hipStream_t stream) {
+                      hipStream_t stream) {
   if (nd <= 0 || total_groups <= 0) return FAP_OK;
   FAP_REQUIRE(s_max <= kInpartBfsMaxS, "launch_inpart_bfs: part of %d vertices", s_max);
   if (s_max <= 2 * kThreads)
-    hipLaunchKernelGGL(inpart_bfs_kernel<2>, dim3(total_groups), dim3(kThreads),
-                       0, stream, d_descs, nd, total_groups, row_off,
-                       col, graph_id, st2ed, ed2st);
+    hipLaunchKernelGGL(inpart_bfs_kernel<2>, dim3(total_groups), dim3(kThreads), 0, stream,
+                       d_descs, nd, total_groups);
   else
-    hipLaunchKernelGGL(inpart_bfs_kernel<4>, dim3(total_groups), dim3(kThreads),
-                       0, stream, d_descs, nd, total_groups, row_off,
-                       col, graph_id, st2ed, ed2st);
+    hipLaunchKernelGGL(inpart_bfs_kernel<4>, dim3(total_groups), dim3(kThreads), 0, stream,
+                       d_descs, nd, total_groups);
   FAP_CHECK_LAUNCH();
   return FAP_OK;
 }

@@ -197,11 +197,14 @@ constexpr int kStCols = 256;
 // unit, unit order inside a product (0: the row blocks of one column block back to
 // back, 1: the column blocks of one row block back to back) and a cap on the resident
 // waves per CU (0: none; enforced through the LDS each one-wave workgroup reserves).
-// Default measured in one process on MI355X (the 52.5 GB result of the 114,599-vertex
-// graph, K-groups on three streams): 128 rows in column-block order 10.7 ms; 64 rows
-// 10.9, 256 rows 10.8; 128 rows in row-block order 11.1.  Set with
-// fap_hip_set_merge_shape; a solve reads it once, when it plans its merge units.
-std::atomic<int64_t> g_stream_shape{128 | (1 << 16)};
+// Default: 128 rows in row-block order (a unit's B panel is reused from L2 by the next
+// row blocks of its column block).  Measured on MI355X (the 52.5 GB result of the
+// 114,599-vertex graph, K-groups on three streams, one result buffer per process):
+// 9.11 ms against 9.25 for column-block order; 64 rows 9.45, 256 rows 9.41, 512 rows
+// 9.88; a cap of 24 resident waves per CU 9.11.  In a result buffer in a slowly
+// written range of device memory the order was reversed (10.73 against 10.59 ms).
+// Set with fap_hip_set_merge_shape; a solve reads it once, when it plans its merge.
+std::atomic<int64_t> g_stream_shape{128};
 
 StreamShape stream_shape() {
   const int64_t v = g_stream_shape.load(std::memory_order_relaxed);

@@ -70,8 +70,7 @@ int launch_merge_stream(const MergeDesc *d_descs, int nd, int total_units, int N
                         const int *d_graph_id, const int *d_ed2st, const int *d_max_b, int lim16,
                         const StreamShape &sh, hipStream_t stream);
 int launch_inpart_bfs(const InpartDesc *d_descs, int nd, int total_groups, int s_max,
-                      const int *row_off, const int *col, const int *graph_id, const int *st2ed,
-                      const int *ed2st, hipStream_t stream);
+                      hipStream_t stream);
 int launch_build(const BuildDesc *d_descs, int nd, int total_rows, bool vec, const int *row_off,
                  const int *col, const int *w, const int *graph_id, const int *st2ed,
                  const int *ed2st, hipStream_t stream);
@@ -331,9 +330,14 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
     ud[p] = UnpackDesc{fw16, fw, S[p], q.num, q.offset, p, batched ? unpack_total : 0};
     {  // nb[p] = ceil(s / 64) groups of 64 sources
       const int ga = std::min(nb[p], (q.bdy + 63) / 64);
-      if (ga > 0) ipa.push_back(InpartDesc{fw, S[p], q.num, q.offset, q.part_id, ipa_groups, 0});
+      if (ga > 0)
+        ipa.push_back(InpartDesc{fw, S[p], q.num, q.offset, q.part_id, ipa_groups, 0,
+                                 g->d_row_offset, g->d_col, g->d_graph_id, g->d_st2ed,
+                                 g->d_ed2st});
       if (nb[p] > ga)
-        ipb.push_back(InpartDesc{fw, S[p], q.num, q.offset, q.part_id, ipb_groups, ga});
+        ipb.push_back(InpartDesc{fw, S[p], q.num, q.offset, q.part_id, ipb_groups, ga,
+                                 g->d_row_offset, g->d_col, g->d_graph_id, g->d_st2ed,
+                                 g->d_ed2st});
       ipa_groups += ga;
       ipb_groups += nb[p] - ga;
     }
@@ -514,13 +518,10 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
         hipEvent_t fork;
         FAP_TRY(event(on, &fork));
         FAP_CHECK_HIP(hipStreamWaitEvent(side, fork, 0));
-        FAP_TRY(launch_inpart_bfs(d_ipb, (int)ipb.size(), ipb_groups, s_max_all,
-                                  g->d_row_offset, g->d_col, g->d_graph_id, g->d_st2ed,
-                                  g->d_ed2st, side));
+        FAP_TRY(launch_inpart_bfs(d_ipb, (int)ipb.size(), ipb_groups, s_max_all, side));
         FAP_TRY(event(side, &ipb_done));
       }
-      return launch_inpart_bfs(d_ipa, (int)ipa.size(), ipa_groups, s_max_all, g->d_row_offset,
-                               g->d_col, g->d_graph_id, g->d_st2ed, g->d_ed2st, on);
+      return launch_inpart_bfs(d_ipa, (int)ipa.size(), ipa_groups, s_max_all, on);
     }
     if (!use16) return launch_fw(d_fd + p0, p1 - p0, mnb, on);
     FAP_TRY(launch_fw16(d_fd16 + p0, p1 - p0, mnb, on));

@@ -298,7 +298,7 @@ int fap_hip_last_stage_times(fap_stage_times *t);
  * in registers and writes `rows` rows of 256 columns), unit order (1: the column blocks
  * of one row block back to back, 0: the row blocks of one column block) and a cap on
  * the resident waves per CU (0: none, else 4..32).  Results do not depend on the shape.  Default
- * 128 / 1 / 0.  A solve reads the shape once, when it plans its merge; set it between
+ * 128 / 0 / 0.  A solve reads the shape once, when it plans its merge; set it between
  * solves.  No counterpart in the reference (its tile is fixed at compile time). */
 int fap_hip_set_merge_shape(int32_t rows, int32_t unit_order, int32_t waves_per_cu);
 int fap_hip_get_merge_shape(int32_t *rows, int32_t *unit_order, int32_t *waves_per_cu);
@@ -383,7 +383,7 @@ typedef struct fap_batch_item {
  * the boundary rows); items with a subset of the parts take one SSSP per boundary
  * vertex instead.  The merge of the inner rows is a persistent grid pulling (part, row
  * block, column block) units from a device counter, largest parts first.  Parts of at
- * most 8192 vertices.  The item arrays are copied; device buffers must outlive the
+ * most 32768 vertices.  The item arrays are copied; device buffers must outlive the
  * batch.  No counterpart in the reference (its solve() loops over parts of one graph,
  * fap/fap.cpp:507-518). */
 typedef struct fap_batch fap_batch;
@@ -399,6 +399,11 @@ typedef struct fap_batch fap_batch;
  * distances (exact: every finite distance is below the 16-bit cap) unless this flag
  * is set. */
 #define FAP_BATCH_NO_PACKED 0x100000
+/* In-part distances of the parts of full items on unit-weight graphs with component
+ * labels (every arc paired with its reverse) come from the bit-parallel BFS of
+ * fap_hip_solve_parts (parts of at most 2048 vertices) instead of build + Floyd-Warshall,
+ * unless this flag is set.  The results are the same. */
+#define FAP_BATCH_NO_BFS 0x200000
 int fap_hip_batch_create(fap_batch **b, const fap_batch_item *items, int32_t nitems,
                          int32_t flags);
 int fap_hip_batch_run(fap_batch *b, void *stream);

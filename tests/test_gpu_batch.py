@@ -134,3 +134,27 @@ def test_autotune_small_table():
     assert AT.k_for(300, table) == table["K"][0] and AT.k_for(8000, table) == table["K"][-1]
     s = AT.settings_for([8000, 8100, 300], table)
     assert s == {k: table[k][-1] for k in AT.KNOBS}
+
+
+def test_batch_inpart_bfs_and_floyd_warshall_agree():
+    """Unit-weight symmetric graphs take the in-part BFS for parts of at most 2048
+    vertices (two kernel widths: up to 1024 and up to 2048) and build + Floyd-Warshall
+    above; FAP_BATCH_NO_BFS forces Floyd-Warshall everywhere.  Both equal exact BFS."""
+    graphs = [F.FapGraph.road_like(1800, 0.55, 1e-3, 21, 3),   # parts ~600
+              F.FapGraph.road_like(3000, 0.55, 1e-3, 22, 2),   # parts ~1500
+              F.FapGraph.road_like(5000, 0.55, 1e-3, 23, 2),   # parts ~2500: Floyd-Warshall
+              F.FapGraph.road_grid(30, 30, 0.1, 0.02, 24, 5)]
+    for g in graphs:
+        g.preCondition()
+    sizes = [int(x) for g in graphs for x in g.layout()["num"][1:]]
+    assert min(sizes) < 1024 and any(1024 < x <= 2048 for x in sizes) and max(sizes) > 2048
+    for bfs in (1, 0):
+        say(f"batch with in-part BFS = {bfs}")
+        b = F.Batch(graphs, use_graph=True, bfs=bfs)
+        for rep in range(2):
+            for g in graphs:
+                clobber(g, 4242 + rep)
+            b.run()
+            for g in graphs:
+                check_graph(g)
+        b.close()

@@ -22,9 +22,13 @@ def road():
     return g
 
 
-@pytest.mark.parametrize("shape", [(128, 1, 0), (8, 1, 4), (32, 0, 8), (256, 1, 16), (24, 0, 0)])
+@pytest.mark.parametrize("shape", [(128, 0, 0), (128, 1, 0), (8, 1, 4), (32, 0, 8), (256, 1, 16),
+                                   (24, 0, 0)])
 def test_merge_shape_exact(road, shape):
     lib = L.load()
+    vals = [C.c_int32(), C.c_int32(), C.c_int32()]
+    L.check(lib.fap_hip_get_merge_shape(*[C.byref(v) for v in vals]), "get shape")
+    default = tuple(v.value for v in vals)
     L.check(lib.fap_hip_set_merge_shape(*shape), "set shape")
     try:
         L.check(lib.fap_hip_fill_i32(road.deviceDistance()[0], 777,
@@ -34,7 +38,7 @@ def test_merge_shape_exact(road, shape):
         assert st["boundary_graph"] == 1
         got = road.getDistanceRowsI32(0, road.num_vertexs)
     finally:
-        L.check(lib.fap_hip_set_merge_shape(128, 1, 0), "reset shape")
+        L.check(lib.fap_hip_set_merge_shape(*default), "reset shape")
     csr, lay = road.csr(), road.layout()
     bad, first = O.compare_exact_rows(csr["row_offset"], csr["col"], None, lay["st2ed"], got)
     assert bad == 0, f"shape {shape}: {bad} rows differ (first {first})"
