@@ -63,7 +63,7 @@ struct BGraph {
   const int *graph_id, *st2ed, *ed2st;
   void *out;                     // n x ld_out result (int32 or float32)
   int64_t ld_out;
-  void *R;                       // FP64 path: boundary rows (nb x ldr doubles)
+  void *R;                       // boundary rows of a full item (nb x ldr, +inf padded)
   int64_t ldr;
   void *MB;                      // boundary-graph matrix (SB x SB), full items only
   int SB;
@@ -371,16 +371,19 @@ __global__ __launch_bounds__(256) void brows_kernel(const BGraph *__restrict__ g
   }
 }
 
-// FP64 partial items: the SSSP rows (FP64) to the float32 result rows.
+// Partial items: the SSSP rows (scratch) to the result rows (int32, or float32 from
+// FP64).
+template <typename T>
 __global__ __launch_bounds__(256) void bconv_kernel(const BGraph *__restrict__ graphs,
                                                     const BPart *__restrict__ parts,
                                                     const BSrc *__restrict__ srcs) {
+  typedef typename Dist<T>::Out Out;
   const BSrc sr = srcs[blockIdx.x];
   const BPart p = parts[sr.p];
   const BGraph g = graphs[p.g];
-  const FAP_GLOBAL double *src = gptr(static_cast<const double *>(p.Brows)) + (int64_t)sr.q * p.ldb;
-  FAP_GLOBAL float *dst = gptr(static_cast<float *>(g.out)) + (p.out_row + sr.q) * g.ld_out;
-  for (int v = threadIdx.x; v < g.n; v += 256) dst[v] = Dist<double>::out(src[v]);
+  const FAP_GLOBAL T *src = gptr(static_cast<const T *>(p.Brows)) + (int64_t)sr.q * p.ldb;
+  FAP_GLOBAL Out *dst = gptr(static_cast<Out *>(g.out)) + (p.out_row + sr.q) * g.ld_out;
+  for (int v = threadIdx.x; v < g.n; v += 256) dst[v] = Dist<T>::out(src[v]);
 }
 
 // ---------------------------------------------------------------- 6: merge
@@ -411,13 +414,12 @@ __device__ __forceinline__ void merge_unit(const BGraph &g, const BPart &p, int 
   const int inner = p.s - p.b, r0 = rb * kRows;
   const FAP_GLOBAL T *D = gptr(static_cast<const T *>(p.D));
   const FAP_GLOBAL T *B = gptr(static_cast<const T *>(p.Brows));
-  int col[4], colc[4];  // colc: clamped to the last column, so every load is in bounds
+  int col[4];
   bool cok[4];
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     col[e] = cb * kMCols + e * 64 + lane;
     cok[e] = col[e] < g.n;
-    colc[e] = min(col[e], g.n - 1);
   }
   constexpr int kAcc = kPacked ? 2 : 4;
   T acc[RB][kAcc];
@@ -439,43 +441,57 @@ __device__ __forceinline__ void merge_unit(const BGraph &g, const BPart &p, int 
     }
     __syncthreads();
     const T *Aw = &As[0][wave * RB];
-    // B row k of this chunk: loads always in bounds (clamped column), INF selected
-    // afterwards for columns past the last, so no load sits behind a branch.
-    auto load_row = [&](int k, T (&x)[4]) {
-      const FAP_GLOBAL T *rp = B + (int64_t)(k0 + k) * p.ldb;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const T v = rp[colc[e]];
-        x[e] = cok[e] ? v : Dist<T>::inf();
-      }
-    };
     if constexpr (sizeof(T) == 8) {
-      // FP64 has no three-operand minimum: one k per step (fewer live registers).
-      for (int k = 0; k < kc; ++k) {
-        T bv[4];
-        load_row(k, bv);
+      // FP64 has no three-operand minimum: one k per step (fewer live registers).  The
+      // FP64 boundary rows are padded with +inf to a multiple of 256 columns, so every
+      // load is in bounds without a mask, and row k + 1 is in flight while row k is
+      // used (two register buffers).
+      auto load_b = [&](int k, T (&x)[4]) {
+        const FAP_GLOBAL T *rp = B + (int64_t)(k0 + k) * p.ldb;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) x[e] = rp[col[e]];
+      };
+      auto step = [&](int k, const T (&bv)[4]) {
 #pragma unroll
         for (int r = 0; r < RB; ++r) {
           const T a = Aw[k * kLdA + r];
 #pragma unroll
           for (int e = 0; e < 4; ++e) acc[r][e] = Dist<T>::mn(acc[r][e], a + bv[e]);
         }
-      }
-    } else {
-      // B rows k and k + 1 of this chunk (INF past kc or past the last column).  Masked
-      // loads here: the clamped form holds more registers (136 against 108 VGPRs for 16
-      // rows per wave, one wave less per SIMD).
-      auto load_pair = [&](int k, T (&x0)[4], T (&x1)[4]) {
-        const FAP_GLOBAL T *r0p = B + (int64_t)(k0 + k) * p.ldb;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          x0[e] = (cok[e] && k < kc) ? r0p[col[e]] : Dist<T>::inf();
-          x1[e] = (cok[e] && k + 1 < kc) ? r0p[p.ldb + col[e]] : Dist<T>::inf();
-        }
       };
+      T b0[4], b1[4];
+      load_b(0, b0);
+      int k = 0;
+      for (; k + 1 < kc; k += 2) {
+        load_b(k + 1, b1);
+        step(k, b0);
+        // Unconditional (the last row again at the end): a load behind a branch makes
+        // the wait before step k + 1 cover it too.
+        load_b(min(k + 2, kc - 1), b0);
+        step(k + 1, b1);
+      }
+      if (k < kc) step(k, b0);
+    } else {
+      // Rows k, k + 1 of the chunk as one pair (the three-operand minimum), the next
+      // pair in flight while this one is used; rows past kc read as +inf (row kc of As
+      // is +inf when kc is odd, and a pair past the end loads row kc - 1 again).
+      auto load_b = [&](int k, T (&x)[4]) {
+        const FAP_GLOBAL T *rp = B + (int64_t)(k0 + min(k, kc - 1)) * p.ldb;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) x[e] = rp[col[e]];
+      };
+      T n0[4], n1[4];
+      load_b(0, n0);
+      load_b(1, n1);
       for (int k = 0; k < kc; k += 2) {
         T b0[4], b1[4];
-        load_pair(k, b0, b1);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          b0[e] = n0[e];
+          b1[e] = n1[e];
+        }
+        load_b(k + 2, n0);
+        load_b(k + 3, n1);
         T a0[RB], a1[RB];
 #pragma unroll
         for (int r = 0; r < RB; ++r) {
@@ -550,7 +566,7 @@ __device__ __forceinline__ void merge_unit(const BGraph &g, const BPart &p, int 
 // contiguous bytes.  The 4*RB x 64 A block is staged k-major in LDS (broadcast reads);
 // the four waves read the same B rows (L1).  Own columns take min(d_p(i, v), product).
 template <typename T, int RB>
-__global__ __launch_bounds__(256) void bmerge_kernel(const BGraph *__restrict__ graphs,
+__global__ __launch_bounds__(256, 4) void bmerge_kernel(const BGraph *__restrict__ graphs,
                                                      const BPart *__restrict__ parts,
                                                      const BMerge *__restrict__ units,
                                                      int total_units, int *counter) {
@@ -576,6 +592,12 @@ __global__ __launch_bounds__(256) void bmerge_kernel(const BGraph *__restrict__ 
     }
     merge_unit<T, RB, false>(g, p, m.rb, m.cb, As, tid);
   }
+}
+
+__global__ void fill_f64_kernel(double *__restrict__ p, int64_t n, double v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
 }
 
 __global__ void reset_counter_kernel(int *c, int n) {
@@ -676,8 +698,8 @@ int launch_kind(const fap_batch *b, const fap_batch_kind &q, int *counter, hipSt
     else hipLaunchKernelGGL((brows_kernel<T, 16>), g5, t5, 0, st, q.graphs, q.parts, q.units5, q.brow);
     FAP_CHECK_LAUNCH();
   }
-  if (sizeof(T) == 8 && q.nsrc > 0) {
-    hipLaunchKernelGGL(bconv_kernel, dim3(q.nsrc), dim3(256), 0, st, q.graphs, q.parts, q.srcs);
+  if (q.nsrc > 0) {
+    hipLaunchKernelGGL(bconv_kernel<T>, dim3(q.nsrc), dim3(256), 0, st, q.graphs, q.parts, q.srcs);
     FAP_CHECK_LAUNCH();
   }
   if (q.merge_units > 0) {
@@ -751,7 +773,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
     G.out = it->d_out;
     G.ld_out = it->ld_out;
     G.n = d.n;
-    G.ldr = align_up(d.n, 2);
+    G.ldr = align_up(d.n, kMCols);  // boundary rows: +inf padding to whole column blocks
     G.packed = !fp64 && b->packed && d.d_w == nullptr && d.d_wf == nullptr && d.n <= kCap16;
     G.bbase = (int)bpart.size();
     G.pid_base = (int)pid2idx.size();
@@ -814,7 +836,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
         brows_at.push_back(-1);
       } else {
         // Partial item: SSSP rows (int32: the part's rows of the result).
-        brows_at.push_back(fp64 ? take((int64_t)P.b * G.ldr * 8) : -1);
+        brows_at.push_back(take((int64_t)P.b * G.ldr * elem));
         for (int c = 0; c < P.b; ++c) srcs.push_back(BSrc{(int)parts.size(), c});
       }
       const double s = P.s, bb = P.b;
@@ -829,7 +851,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
       gb_max_nb = std::max(gb_max_nb, G.SB / kBT);
       gbg.push_back(gi);
       mb_at.push_back(take((int64_t)G.SB * G.SB * elem));
-      r_at.push_back(fp64 ? take((int64_t)G.nb * G.ldr * 8) : -1);
+      r_at.push_back(take((int64_t)G.nb * G.ldr * elem));
       for (int pi = (int)parts.size() - (int)ps.size(); pi < (int)parts.size(); ++pi)
         for (int r0 = 0; r0 < G.nb; r0 += b->rows5) units5.push_back(BRowUnit{pi, r0});
     } else {
@@ -872,19 +894,24 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   const int64_t o_gbd = table((fp64 ? sizeof(Fw64Desc) : sizeof(FwDesc)) * gbg.size());
   FAP_TRY(dev_alloc((void **)&q.d_mem, std::max<int64_t>(scratch, 256)));
   char *base = q.d_mem;
+  if (o_graphs > 0) {
+    // Scratch to +inf (FAP_INF for int32) once: the padding columns of the boundary
+    // rows are never written by the kernels and must read as unreachable; everything
+    // else is rewritten on every run.
+    if (fp64)
+      hipLaunchKernelGGL(fill_f64_kernel, dim3(1024), dim3(256), 0, nullptr,
+                         reinterpret_cast<double *>(base), o_graphs / 8, __builtin_huge_val());
+    else
+      FAP_TRY(fap_hip_fill_i32(reinterpret_cast<int32_t *>(base), kInf, o_graphs / 4, nullptr));
+    FAP_CHECK_LAUNCH();
+    FAP_CHECK_HIP(hipDeviceSynchronize());
+  }
   for (size_t k = 0; k < parts.size(); ++k) {
     BPart &P = parts[k];
     P.D = base + d_at[k];
     const BGraph &G = graphs[P.g];
-    if (brows_at[k] >= 0) {
-      P.Brows = base + brows_at[k];
-      P.ldb = G.ldr;
-    } else if (G.full && fp64) {
-      P.ldb = G.ldr;  // set below from the graph's R block
-    } else {
-      P.Brows = static_cast<int32_t *>(G.out) + P.out_row * G.ld_out;  // int32 result rows
-      P.ldb = G.ld_out;
-    }
+    P.ldb = G.ldr;
+    if (brows_at[k] >= 0) P.Brows = base + brows_at[k];  // full items: set below from R
   }
   for (size_t gi = 0; gi < graphs.size(); ++gi) {
     BGraph &G = graphs[gi];
@@ -893,7 +920,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   }
   for (BPart &P : parts) {
     const BGraph &G = graphs[P.g];
-    if (G.full && fp64) P.Brows = static_cast<double *>(G.R) + (int64_t)P.bs * G.ldr;
+    if (G.full && G.R) P.Brows = static_cast<char *>(G.R) + (int64_t)P.bs * G.ldr * elem;
   }
   std::vector<char> host(scratch - o_graphs);
   auto put = [&](int64_t o, const void *src, size_t bytes) {
