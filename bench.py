@@ -203,7 +203,8 @@ def batch_bench(args):
     t0 = time.perf_counter()
     if getattr(args, "no_tune", False):  # fixed settings (profiling runs)
         table = {"K": [8, 8, 8, 16, 16], "per_cu": [0] * 5, "src_rows": [32] * 5,
-                 "merge_rows": [16] * 5, "packed": [1] * 5, "ms_per_graph": [None] * 5}
+                 "merge_rows": [16] * 5, "packed": [1] * 5, "unit_grid": [1] * 5,
+                 "ms_per_graph": [None] * 5}
     elif getattr(args, "tune_quick", False):
         table = AT.tune(k_candidates=(8, 16), per_cu_candidates=(0,), src_rows_candidates=(16,),
                         merge_rows_candidates=(8, 16), max_copies=16)
@@ -211,7 +212,8 @@ def batch_bench(args):
         table = AT.tune(max_copies=32)
     log(f"autotune {time.perf_counter() - t0:.1f}s: K per class {table['K']}, "
         f"per_cu {table['per_cu']}, source rows {table['src_rows']}, "
-        f"merge rows {table['merge_rows']}, packed {table['packed']}")
+        f"merge rows {table['merge_rows']}, packed {table['packed']}, "
+        f"unit grid {table['unit_grid']}")
     rng = np.random.default_rng(2026)
     sizes = np.exp(rng.uniform(np.log(256), np.log(8192), args.graphs)).astype(int)
     t0 = time.perf_counter()
@@ -226,6 +228,8 @@ def batch_bench(args):
     log(f"{len(graphs)} graphs built in {time.perf_counter() - t0:.1f}s: n in [{min(ns)}, "
         f"{max(ns)}], sum n^2 = {entries:.3e} ({entries * 4 / 1e9:.1f} GB of results)")
     settings = AT.settings_for(ns, table)
+    if getattr(args, "unit_grid", None) is not None:
+        settings["unit_grid"] = int(args.unit_grid)
     b = F.Batch(graphs, use_graph=True, **settings)
     for _ in range(args.warmup):
         b.run()
@@ -273,7 +277,7 @@ def batch_bench(args):
                       "graphs": len(graphs), "sum_n2": entries, "n_min": min(ns),
                       "n_max": max(ns), "settings": settings},
            "autotune": {k: table[k] for k in ("K", "per_cu", "src_rows", "merge_rows",
-                                               "packed", "ms_per_graph")},
+                                               "packed", "unit_grid", "ms_per_graph")},
            "plain_form": plain, "cpu_baseline": cpu}
     return out
 
@@ -357,6 +361,9 @@ def main():
     ap.add_argument("--config", default="road1",
                     help="road1 (default), road0, batch512 (many-graph batch) or serve")
     ap.add_argument("--graphs", type=int, default=512, help="graphs of the batch512 config")
+    ap.add_argument("--unit-grid", type=int, default=None,
+                    help="batch512: 1 = merge as a plain grid, 0 = persistent queue "
+                         "(default: the autotuner's choice)")
     ap.add_argument("--pool", type=int, default=24, help="resident graphs of the serve config")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--paths", action="store_true",

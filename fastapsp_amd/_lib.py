@@ -34,6 +34,7 @@ FAP_MODE_FLAG_FW = 0x400
 FAP_BATCH_GRAPH = 0x1
 FAP_BATCH_NO_PACKED = 0x100000
 FAP_BATCH_NO_BFS = 0x200000
+FAP_BATCH_UNIT_GRID = 0x400000
 
 # Every exported symbol of include/fap_hip.h: name -> (restype, argtypes).
 _i32, _i64, _u64, _f64 = C.c_int32, C.c_int64, C.c_uint64, C.c_double

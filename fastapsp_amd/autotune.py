@@ -11,7 +11,9 @@ fastest:
     product (8, 16 or 32 sources sharing one pass over a part's in-part block);
   * the merge row tile: inner rows per wave of the int32 merge (8 or 16);
   * the persistent merge workgroups per CU (0 = as many as stay resident);
-  * the packed 16-bit merge of unit-weight graphs, on or off.
+  * the packed 16-bit merge of unit-weight graphs, on or off;
+  * the merge as the persistent work queue or as a plain grid of one workgroup per unit
+    (with both distance types in a batch the two launch sequences run side by side).
 K is chosen first with the default tiles, then each tile knob in turn (coordinate
 search).  The result is a small table (JSON-serialisable) that `k_for(n)` and
 `settings_for(ns)` read; `load_or_tune(path)` caches it on disk.
@@ -35,8 +37,9 @@ PER_CU_CANDIDATES = (0, 2)
 SRC_ROWS_CANDIDATES = (8, 16, 32)
 MERGE_ROWS_CANDIDATES = (8, 16)
 PACKED_CANDIDATES = (1, 0)
-MAX_PART = 8192  # largest part of the batch form
-KNOBS = ("per_cu", "src_rows", "merge_rows", "packed")
+UNIT_GRID_CANDIDATES = (0, 1)
+MAX_PART = 32768  # largest part of the batch form
+KNOBS = ("per_cu", "src_rows", "merge_rows", "packed", "unit_grid")
 
 
 def symmetric_float_weights(ro, col, seed: int, directed: bool = False) -> np.ndarray:
@@ -86,13 +89,15 @@ def _time_batch(graphs, cfg: dict, reps: int = 2) -> float:
 def tune(copies_work: int = 512, max_copies: int = 48, seed: int = 4242,
          k_candidates=K_CANDIDATES, per_cu_candidates=PER_CU_CANDIDATES,
          src_rows_candidates=SRC_ROWS_CANDIDATES, merge_rows_candidates=MERGE_ROWS_CANDIDATES,
-         packed_candidates=PACKED_CANDIDATES, verbose: bool = False) -> dict:
+         packed_candidates=PACKED_CANDIDATES, unit_grid_candidates=UNIT_GRID_CANDIDATES,
+         verbose: bool = False) -> dict:
     """Per size class: K with the default tiles, then each knob in turn; the table
     keeps the fastest setting seen (every trial is recorded)."""
     table = {"classes": SIZE_CLASSES, "K": [], "per_cu": [], "src_rows": [], "merge_rows": [],
-             "packed": [], "ms_per_graph": [], "trials": []}
+             "packed": [], "unit_grid": [], "ms_per_graph": [], "trials": []}
     cands = {"per_cu": tuple(per_cu_candidates), "src_rows": tuple(src_rows_candidates),
-             "merge_rows": tuple(merge_rows_candidates), "packed": tuple(packed_candidates)}
+             "merge_rows": tuple(merge_rows_candidates), "packed": tuple(packed_candidates),
+             "unit_grid": tuple(unit_grid_candidates)}
     for ci, (lo, hi) in enumerate(SIZE_CLASSES):
         n_mid = int(math.sqrt(lo * min(hi, 8192)))
         trials = []

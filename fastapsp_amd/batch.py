@@ -33,10 +33,12 @@ class Batch:
     16); merge_rows: inner rows per wave of the merge (8 or 16; 0 = 16, FP64 always 8);
     packed: 1 = the int32 merge of unit-weight graphs (n <= 15872) on packed 16-bit
     distances, 0 = on int32; bfs: 1 = in-part distances of unit-weight symmetric graphs
-    by bit-parallel BFS, 0 = by Floyd-Warshall."""
+    by bit-parallel BFS, 0 = by Floyd-Warshall; unit_grid: 1 = the merge as a plain grid
+    of one workgroup per unit, 0 = the persistent work queue."""
 
     def __init__(self, graphs: Sequence, use_graph: bool = True, per_cu: int = 0,
-                 src_rows: int = 0, merge_rows: int = 0, packed: int = 1, bfs: int = 1):
+                 src_rows: int = 0, merge_rows: int = 0, packed: int = 1, bfs: int = 1,
+                 unit_grid: int = 0):
         self.graphs = list(graphs)
         for g in self.graphs:
             if not getattr(g, "_preconditioned", False):
@@ -50,7 +52,8 @@ class Batch:
                                                 | (_ROWS_CODE[int(src_rows)] << 16)
                                                 | (_ROWS_CODE[int(merge_rows)] << 18)
                                                 | (0 if packed else L.FAP_BATCH_NO_PACKED)
-                                                | (0 if bfs else L.FAP_BATCH_NO_BFS)),
+                                                | (0 if bfs else L.FAP_BATCH_NO_BFS)
+                                                | (L.FAP_BATCH_UNIT_GRID if unit_grid else 0)),
                 "batch create")
         self._h = h.value
 

@@ -573,6 +573,22 @@ __global__ __launch_bounds__(256, 4) void bmerge_kernel(const BGraph *__restrict
   __shared__ __attribute__((aligned(16))) T As[kKc][kMWaves * RB + kAPad<T>];
   __shared__ int unit_s;
   const int tid = threadIdx.x;
+  auto run = [&](int u) {
+    const BMerge m = units[u];
+    const BPart p = parts[m.p];
+    const BGraph g = graphs[p.g];
+    if constexpr (sizeof(T) == 4) {
+      if (g.packed) {
+        merge_unit<T, RB, true>(g, p, m.rb, m.cb, As, tid);
+        return;
+      }
+    }
+    merge_unit<T, RB, false>(g, p, m.rb, m.cb, As, tid);
+  };
+  if (!counter) {  // plain grid: one unit per workgroup, in unit order
+    if ((int)blockIdx.x < total_units) run(blockIdx.x);
+    return;
+  }
   while (true) {
     if (tid == 0) unit_s = atomicAdd(counter, 1);
     __syncthreads();
@@ -581,16 +597,7 @@ __global__ __launch_bounds__(256, 4) void bmerge_kernel(const BGraph *__restrict
     const int u = __builtin_amdgcn_readfirstlane(unit_s);
     __syncthreads();  // every thread has read unit_s before thread 0 takes the next one
     if (u >= total_units) break;  // uniform: every workgroup ends here
-    const BMerge m = units[u];
-    const BPart p = parts[m.p];
-    const BGraph g = graphs[p.g];
-    if constexpr (sizeof(T) == 4) {
-      if (g.packed) {
-        merge_unit<T, RB, true>(g, p, m.rb, m.cb, As, tid);
-        continue;
-      }
-    }
-    merge_unit<T, RB, false>(g, p, m.rb, m.cb, As, tid);
+    run(u);
   }
 }
 
@@ -650,9 +657,14 @@ struct fap_batch {
   int per_cu = 0, rows5 = 16, rows6 = 16;
   bool packed = true;  // packed 16-bit merge where exact
   bool bfs = true;     // in-part BFS where exact
+  bool unit_grid = false;  // merge as a plain grid instead of the persistent queue
   bool use_graph = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // Both kinds present: the FP64 kind runs on `side`, forked from and joined back into
+  // the caller's stream (two branches of the captured graph).
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace {
@@ -703,21 +715,35 @@ int launch_kind(const fap_batch *b, const fap_batch_kind &q, int *counter, hipSt
     FAP_CHECK_LAUNCH();
   }
   if (q.merge_units > 0) {
-    const dim3 gm(std::min(q.grid, q.merge_units)), tm(256);
+    // Persistent work queue, or (unit_grid) a plain grid of one workgroup per unit whose
+    // slots free up as units end, so the other kind's launches interleave.
+    const dim3 gm(b->unit_grid ? q.merge_units : std::min(q.grid, q.merge_units)), tm(256);
+    int *c = b->unit_grid ? nullptr : counter;
     if (q.rows6 == 8)
-      hipLaunchKernelGGL((bmerge_kernel<T, 8>), gm, tm, 0, st, q.graphs, q.parts, q.merge, q.merge_units, counter);
+      hipLaunchKernelGGL((bmerge_kernel<T, 8>), gm, tm, 0, st, q.graphs, q.parts, q.merge, q.merge_units, c);
     else
-      hipLaunchKernelGGL((bmerge_kernel<T, 16>), gm, tm, 0, st, q.graphs, q.parts, q.merge, q.merge_units, counter);
+      hipLaunchKernelGGL((bmerge_kernel<T, 16>), gm, tm, 0, st, q.graphs, q.parts, q.merge, q.merge_units, c);
     FAP_CHECK_LAUNCH();
   }
   return FAP_OK;
 }
 
+// The two kinds share nothing but the counter array (one counter each), so they run
+// side by side: the Floyd-Warshall rounds of one (short, latency-bound launches) fill
+// in around the other's long launches.
 int batch_launch(fap_batch *b, hipStream_t st) {
   hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, st, b->d_counter, 2);
   FAP_CHECK_LAUNCH();
+  if (!b->side) {
+    FAP_TRY(launch_kind<int>(b, b->kind[0], b->d_counter, st));
+    return launch_kind<double>(b, b->kind[1], b->d_counter + 1, st);
+  }
+  FAP_CHECK_HIP(hipEventRecord(b->ev_fork, st));
+  FAP_CHECK_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
+  FAP_TRY(launch_kind<double>(b, b->kind[1], b->d_counter + 1, b->side));
   FAP_TRY(launch_kind<int>(b, b->kind[0], b->d_counter, st));
-  FAP_TRY(launch_kind<double>(b, b->kind[1], b->d_counter + 1, st));
+  FAP_CHECK_HIP(hipEventRecord(b->ev_join, b->side));
+  FAP_CHECK_HIP(hipStreamWaitEvent(st, b->ev_join, 0));
   return FAP_OK;
 }
 
@@ -1005,6 +1031,12 @@ extern "C" int fap_hip_batch_destroy(fap_batch *b) {
   if (!b) return FAP_OK;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->side) {
+    (void)hipStreamSynchronize(b->side);
+    (void)hipStreamDestroy(b->side);
+  }
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  if (b->ev_join) (void)hipEventDestroy(b->ev_join);
   for (auto &q : b->kind)
     if (q.d_mem) dev_free(q.d_mem);
   if (b->d_counter) dev_free(b->d_counter);
@@ -1020,7 +1052,7 @@ extern "C" int fap_hip_batch_create(fap_batch **out, const fap_batch_item *items
   }
   FAP_REQUIRE(out && nitems >= 0 && (nitems == 0 || items), "fap_hip_batch_create: null argument");
   FAP_REQUIRE((flags & ~(FAP_BATCH_GRAPH | 0xFF00 | 0xF0000 | FAP_BATCH_NO_PACKED |
-                         FAP_BATCH_NO_BFS)) == 0,
+                         FAP_BATCH_NO_BFS | FAP_BATCH_UNIT_GRID)) == 0,
               "fap_hip_batch_create: unknown flags %d", flags);
   *out = nullptr;
   std::vector<const fap_batch_item *> kinds[2];
@@ -1040,9 +1072,19 @@ extern "C" int fap_hip_batch_create(fap_batch **out, const fap_batch_item *items
   b->rows6 = pick_rows((flags >> 18) & 0x3, 16) == 8 ? 8 : 16;
   b->packed = (flags & FAP_BATCH_NO_PACKED) == 0;
   b->bfs = (flags & FAP_BATCH_NO_BFS) == 0;
+  b->unit_grid = (flags & FAP_BATCH_UNIT_GRID) != 0;
   int s = dev_alloc((void **)&b->d_counter, 256);
   for (int k = 0; k < 2 && s == FAP_OK; ++k)
     if (!kinds[k].empty()) s = plan_kind(b, b->kind[k], kinds[k], k == 1);
+  if (s == FAP_OK && b->kind[0].nparts > 0 && b->kind[1].nparts > 0) {
+    if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("fap_hip_batch_create: stream or event creation failed");
+      s = FAP_ERR_HIP;
+    }
+  }
   if (s == FAP_OK && b->use_graph) {
     hipStream_t st = nullptr;
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {

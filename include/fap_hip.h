@@ -404,6 +404,11 @@ typedef struct fap_batch fap_batch;
  * fap_hip_solve_parts (parts of at most 2048 vertices) instead of build + Floyd-Warshall,
  * unless this flag is set.  The results are the same. */
 #define FAP_BATCH_NO_BFS 0x200000
+/* The inner-row merge as a plain grid (one workgroup per unit, in unit order) instead
+ * of the persistent work queue; with both distance types in a batch their launch
+ * sequences run side by side, and a plain grid lets the other type's launches take
+ * workgroup slots as merge units end. */
+#define FAP_BATCH_UNIT_GRID 0x400000
 int fap_hip_batch_create(fap_batch **b, const fap_batch_item *items, int32_t nitems,
                          int32_t flags);
 int fap_hip_batch_run(fap_batch *b, void *stream);

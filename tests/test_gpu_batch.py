@@ -90,9 +90,12 @@ def test_batch_one_shot_and_plain():
             check_graph(g)
 
 
-def test_batch_graph_replays():
+@pytest.mark.parametrize("unit_grid", [0, 1])
+def test_batch_graph_replays(unit_grid):
+    """int32 and FP64 items together: their launch sequences run on two streams (two
+    branches of the captured graph); the merge as the persistent queue or a plain grid."""
     graphs = make_graphs()
-    b = F.Batch(graphs, use_graph=True)
+    b = F.Batch(graphs, use_graph=True, unit_grid=unit_grid)
     other = F.FapGraph.road_grid(30, 30, 0.1, 0.02, 99, 4)
     other.preCondition()
     for rep in range(3):
@@ -148,9 +151,9 @@ def test_batch_inpart_bfs_and_floyd_warshall_agree():
         g.preCondition()
     sizes = [int(x) for g in graphs for x in g.layout()["num"][1:]]
     assert min(sizes) < 1024 and any(1024 < x <= 2048 for x in sizes) and max(sizes) > 2048
-    for bfs in (1, 0):
-        say(f"batch with in-part BFS = {bfs}")
-        b = F.Batch(graphs, use_graph=True, bfs=bfs)
+    for bfs, unit_grid in ((1, 0), (0, 1)):
+        say(f"batch with in-part BFS = {bfs}, plain-grid merge = {unit_grid}")
+        b = F.Batch(graphs, use_graph=True, bfs=bfs, unit_grid=unit_grid)
         for rep in range(2):
             for g in graphs:
                 clobber(g, 4242 + rep)
