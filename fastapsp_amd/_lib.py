@@ -58,7 +58,7 @@ class FapPartDesc(C.Structure):
 
 class FapBatchItem(C.Structure):
     _fields_ = [("g", FapGraphDev), ("parts", C.POINTER(FapPartDesc)), ("nparts", _i32),
-                ("is_float", _i32), ("d_out", _vp), ("ld_out", _i64)]
+                ("is_float", _i32), ("d_out", _vp), ("ld_out", _i64), ("flags", _i32)]
 
 
 class FapServerStats(C.Structure):
@@ -164,6 +164,7 @@ SIGNATURES = {
     "fap_graph_get_subgraph_dist": (C.c_int, [_vp, _i32, _pi32]),
     "fap_graph_get_subgraph_rows": (C.c_int, [_vp, _i32, _i64, _i64, _pi32]),
     "fap_graph_is_float": (C.c_int, [_vp, _pi32]),
+    "fap_graph_boundary_by_sssp": (C.c_int, [_vp, _pi32]),
     "fap_graph_get_weights_f32": (C.c_int, [_vp, _pf32]),
     "fap_graph_get_dist_rows_f32": (C.c_int, [_vp, _i64, _i64, _pf32]),
     "fap_graph_get_subgraph_rows_f32": (C.c_int, [_vp, _i32, _i64, _i64, _pf32]),

@@ -827,7 +827,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
         cover = cover && x.first == next;
         next = x.first + x.second;
       }
-      G.full = cover && next == d.n;
+      G.full = cover && next == d.n && !(it->flags & FAP_ITEM_BOUNDARY_SSSP);
     }
     G.max_pid = max_pid;
     pid2idx.resize(pid2idx.size() + max_pid + 1, -1);

@@ -266,6 +266,11 @@ int fap_graph_is_float(fap_graph *h, int32_t *is_float) {
   return guarded([&] { *is_float = h->g->isFloat() ? 1 : 0; });
 }
 
+int fap_graph_boundary_by_sssp(fap_graph *h, int32_t *by_sssp) {
+  FAP_HANDLE(h);
+  return guarded([&] { *by_sssp = h->g->boundaryBySssp() ? 1 : 0; });
+}
+
 int fap_graph_get_dist_rows_f32(fap_graph *h, int64_t row0, int64_t rows, float *h_out) {
   FAP_HANDLE(h);
   return guarded([&] { h->g->getDistanceRowsF32(row0, rows, h_out); });
@@ -321,6 +326,7 @@ void batch_items(fap_graph **graphs, int32_t count, std::vector<fap_batch_item> 
     it.nparts = (int32_t)parts[i].size();
     it.is_float = g.isFloat() ? 1 : 0;
     it.d_out = g.resultBuffer(&it.ld_out);
+    it.flags = g.boundaryBySssp() ? FAP_ITEM_BOUNDARY_SSSP : 0;
   }
 }
 }  // namespace

@@ -150,6 +150,35 @@ void fapGraph::setPartition(const std::vector<int> &ids) {
   release();
 }
 
+// True when some pair u -> v inside one part has parallel arcs and the one that comes
+// last in CSR order is not the lightest.  The reference's sub-matrix build keeps the
+// later arc (util_Myalgorithm.h:44-84) while its boundary rows come from Dijkstra,
+// which takes the lightest; the boundary-graph form derives the boundary rows from the
+// built in-part matrices, so for such graphs only the SSSP form reproduces the
+// reference.
+static bool later_arc_heavier(int32_t n, const std::vector<int32_t> &ro,
+                              const std::vector<int32_t> &col, const std::vector<float> &w,
+                              const std::vector<int32_t> &gid) {
+  std::vector<std::pair<int32_t, int32_t>> arcs;  // (head, CSR index)
+  for (int32_t u = 0; u < n; ++u) {
+    arcs.clear();
+    for (int32_t e = ro[u]; e < ro[u + 1]; ++e)
+      if (gid[col[e]] == gid[u]) arcs.push_back({col[e], e});
+    std::sort(arcs.begin(), arcs.end());
+    for (size_t a = 0; a < arcs.size();) {
+      size_t b = a;
+      float lightest = w[arcs[a].second];
+      while (b + 1 < arcs.size() && arcs[b + 1].first == arcs[a].first) {
+        ++b;
+        lightest = std::min(lightest, w[arcs[b].second]);
+      }
+      if (w[arcs[b].second] > lightest) return true;  // arcs[b]: the last in CSR order
+      a = b + 1;
+    }
+  }
+  return false;
+}
+
 void fapGraph::preCondition() {
   Csr g;
   g.n = num_vertexs;
@@ -163,6 +192,7 @@ void fapGraph::preCondition() {
   max_subgraph_vertexs = 0;
   for (int p = 1; p <= K; ++p) max_subgraph_vertexs = std::max(max_subgraph_vertexs, C_BlockVer_num[p]);
   is_split = num_vertexs > max_vertexs_num_limit;
+  boundary_by_sssp = later_arc_heavier(num_vertexs, row_offset, col_val, weight, graph_id);
   release();  // device copies of an earlier layout are stale
   preconditioned = true;
 }
@@ -330,7 +360,7 @@ void fapGraph::solve_placed(const fap_graph_dev &g, const std::vector<fap_part_d
                                                 reinterpret_cast<float *>(p), ld_dist, mode,
                                                 nullptr)
                       : fap_hip_solve_parts(&g, parts.data(), (int32_t)parts.size(), p, ld_dist,
-                                            mode, nullptr);
+                                            solve_mode(), nullptr);
     fap_stage_times t{};
     int st = r == FAP_OK ? fap_hip_stream_sync(nullptr) : r;
     if (st == FAP_OK) st = fap_hip_last_stage_times(&t);
@@ -378,8 +408,8 @@ int32_t fapGraph::solve(bool path_needed) {
                                   reinterpret_cast<float *>(d_dist), ld_dist, mode, nullptr),
           "solve");
   else
-    check(fap_hip_solve_parts(&g, parts.data(), (int32_t)parts.size(), d_dist, ld_dist, mode,
-                              nullptr),
+    check(fap_hip_solve_parts(&g, parts.data(), (int32_t)parts.size(), d_dist, ld_dist,
+                              solve_mode(), nullptr),
           "solve");
   check(fap_hip_stream_sync(nullptr), "solve");
   add_stage_times((int32_t)parts.size());

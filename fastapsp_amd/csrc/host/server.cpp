@@ -224,6 +224,7 @@ void scheduler(fap_server *s) {
           it.is_float = ge.is_float ? 1 : 0;
           it.d_out = r.d_out;
           it.ld_out = r.ld;
+          it.flags = ge.g->boundaryBySssp() ? FAP_ITEM_BOUNDARY_SSSP : 0;
           items.push_back(it);
         }
         Inflight f;
@@ -297,13 +298,15 @@ void pipeline_lane(fap_server *s) {
     const GraphEntry &ge = s->graphs[r.graph];
     const fap_graph_dev dev = ge.dev;
     const bool is_float = ge.is_float;
+    const bool by_sssp = ge.g->boundaryBySssp();
     std::vector<fap_part_desc> parts = r.parts;
     const int64_t bytes = r.bytes, ld = r.ld;
     lk.unlock();
     void *d_out = nullptr;
     bool ok = fap_hip_malloc(&d_out, std::max<int64_t>(bytes, 4)) == FAP_OK;
     // int32 FW keeps the call asynchronous (the 16-bit form reads flags back).
-    const int32_t mode = FAP_MODE_BATCHED | FAP_MODE_FLAG_INT32_FW;
+    const int32_t mode = FAP_MODE_BATCHED | FAP_MODE_FLAG_INT32_FW |
+                         (by_sssp ? FAP_MODE_FLAG_SSSP : 0);
     if (ok)
       ok = (is_float ? fap_hip_solve_parts_f32(&dev, parts.data(), (int32_t)parts.size(),
                                                static_cast<float *>(d_out), ld, mode,

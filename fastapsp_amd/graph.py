@@ -183,6 +183,14 @@ class FapGraph:
     def getAllDistanceI32(self) -> np.ndarray:
         return self.getDistanceRowsI32(0, self.num_vertexs)
 
+    def boundaryBySssp(self) -> bool:
+        """True when preCondition found parallel arcs inside a part whose later arc (CSR
+        order) is heavier; solve() then computes the boundary rows by SSSP, as the
+        reference does (its sub-matrix build keeps the later arc)."""
+        v = C.c_int32()
+        L.check(L.load().fap_graph_boundary_by_sssp(self._h, C.byref(v)), "boundaryBySssp")
+        return bool(v.value)
+
     def isFloat(self) -> bool:
         """True when a weight is not an integer below 2^24: solves run the float path
         (FP64 arithmetic, float32 results) and the int32 getters raise."""

@@ -105,6 +105,10 @@ class fapGraph {
   void release();
   void add_stage_times(int32_t parts);
   void alloc_result(int32_t **d, const char *what);
+  // Parallel in-part arcs whose later arc is heavier: boundary rows by SSSP (the
+  // reference's semantics for such graphs); set by preCondition.
+  bool boundary_by_sssp = false;
+  int32_t solve_mode() const { return mode | (boundary_by_sssp ? FAP_MODE_FLAG_SSSP : 0); }
   void solve_placed(const fap_graph_dev &g, const std::vector<fap_part_desc> &parts);
   void copy_rows(const int32_t *base, int64_t nrows, int64_t row0, int64_t rows, void *out) const;
   void require_int(const char *what) const;
@@ -139,6 +143,9 @@ class fapGraph {
   void setPartition(const std::vector<int> &graph_id);
   void preCondition();
   bool isSplit();
+  // Parallel arcs inside a part whose later arc is heavier (set by preCondition): the
+  // boundary rows then come from SSSP, as in the reference.
+  bool boundaryBySssp() const { return boundary_by_sssp; }
   std::vector<int> getGraphId();
 
   // Execution form of solve(): FAP_MODE_BATCHED (default), FAP_MODE_OVERLAP or

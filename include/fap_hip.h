@@ -373,7 +373,13 @@ typedef struct fap_batch_item {
   int32_t is_float;
   void *d_out;
   int64_t ld_out;
+  int32_t flags; /* FAP_ITEM_* */
 } fap_batch_item;
+/* Boundary rows by SSSP even when the parts tile the graph: for graphs with parallel
+ * arcs inside a part whose later arc (CSR order) is heavier, where the reference's
+ * sub-matrix build keeps the later arc and its boundary rows the lightest
+ * (fapGraph sets it from preCondition). */
+#define FAP_ITEM_BOUNDARY_SSSP 0x1
 
 /* A prepared batch: the work tables of every part of every graph and optionally the
  * captured launch sequence (FAP_BATCH_GRAPH: fap_hip_batch_run replays a hipGraph).
@@ -485,6 +491,11 @@ int fap_graph_get_subgraph_rows(fap_graph *g, int32_t id, int64_t row0, int64_t 
 /* *is_float = 1 when some weight is not an integer below 2^24: solves then run
  * fap_hip_solve_parts_f32 and the int32 getters return FAP_ERR_STATE. */
 int fap_graph_is_float(fap_graph *g, int32_t *is_float);
+/* 1 when preCondition found parallel arcs inside a part whose later arc is heavier:
+ * solve() then takes the boundary rows from SSSP (FAP_MODE_FLAG_SSSP), the reference's
+ * semantics (its sub-matrix build keeps the later arc, util_Myalgorithm.h:44-84, its
+ * Dijkstra the lightest). */
+int fap_graph_boundary_by_sssp(fap_graph *g, int32_t *by_sssp);
 /* Float32 rows (MAXVALUE = 1e8 for unreachable) of the solve() result / of the last
  * solveSubGraph block, for any graph. */
 int fap_graph_get_dist_rows_f32(fap_graph *g, int64_t row0, int64_t rows, float *h_out);

@@ -243,3 +243,37 @@ def test_boundary_graph_parts_without_inner_vertices():
     assert np.array_equal(a, want)
     b, _ = _solve(g, F.FAP_MODE_BATCHED | SSSP)
     assert np.array_equal(a, b)
+
+
+def test_parallel_arcs_follow_the_reference():
+    """Parallel in-part arcs whose later arc is heavier: the reference keeps the later
+    arc in the build and the lightest in its Dijkstra boundary rows; solve() then takes
+    the SSSP form (boundary_graph == 0) and equals the oracle's restatement of the
+    reference path, in the single-graph pipeline and in the batch.  With the later arc
+    lighter the boundary-graph form applies and equals it too (and exact APSP)."""
+    from test_host import _with_parallel_arcs
+    base = F.FapGraph.road_grid(24, 24, 0.1, 0.02, 3, 5)
+    base.preCondition()
+    csr, lay = base.csr(), base.layout()
+    for heavier in (True, False):
+        ro, col, w = _with_parallel_arcs(csr, lay, heavier_later=heavier, count=40)
+        g = F.FapGraph.from_csr(ro, col, w, True, 5)
+        g.setPartition(lay["graph_id"])
+        g.preCondition()
+        assert g.boundaryBySssp() == heavier
+        c, l2 = g.csr(), g.layout()
+        want = O.solve_parts(1, g.K, c, l2)
+        g.solve()
+        st = L.last_stage_times()
+        assert st["boundary_graph"] == (0 if heavier else 1)
+        assert np.array_equal(g.getAllDistanceI32(), want)
+        if not heavier:
+            exact = O.exact_rows(c["row_offset"], c["col"], c["weight"].astype(np.int32),
+                                 l2["st2ed"])
+            assert np.array_equal(want, exact)
+        p, ld = g.deviceDistance()
+        L.check(L.load().fap_hip_fill_i32(p, 99, g.num_vertexs * ld, None), "fill")
+        b = F.Batch([g], use_graph=True)
+        b.run()
+        assert np.array_equal(g.getAllDistanceI32(), want)
+        b.close()

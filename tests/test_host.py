@@ -183,3 +183,51 @@ def test_api_errors():
     h = C.c_void_p()
     assert lib.fap_graph_create_from_file(C.byref(h), b"x", 0, 0, 2, b"KaHIP") in (
         L.FAP_ERR_IO, L.FAP_ERR_INVALID)
+
+
+def _with_parallel_arcs(csr, lay, heavier_later=True, count=12, seed=5):
+    """The graph's CSR with a second arc appended after `count` in-part arcs u -> v
+    (same row, later in CSR order), 3 heavier (or 1 lighter) than the first."""
+    ro, col, w = csr["row_offset"], csr["col"], csr["weight"].astype(np.float32)
+    gid = lay["graph_id"]
+    rng = np.random.default_rng(seed)
+    n = len(ro) - 1
+    cand = [e for u in range(n) for e in range(ro[u], ro[u + 1]) if gid[col[e]] == gid[u]]
+    pick = set(rng.choice(len(cand), size=count, replace=False).tolist())
+    extra = {cand[i] for i in pick}
+    nro, ncol, nw = [0], [], []
+    for u in range(n):
+        for e in range(ro[u], ro[u + 1]):
+            ncol.append(col[e])
+            nw.append(w[e])
+        for e in range(ro[u], ro[u + 1]):
+            if e in extra:
+                ncol.append(col[e])
+                nw.append(w[e] + 3 if heavier_later else max(w[e] - 1, 1))
+        nro.append(len(ncol))
+    return (np.array(nro, np.int32), np.array(ncol, np.int32), np.array(nw, np.float32))
+
+
+def test_parallel_arcs_later_heavier_detected_and_reference_semantics():
+    """preCondition flags a graph whose parallel in-part arcs have a heavier later arc
+    (not one whose later arc is lighter, nor the plain graph); on such a graph the
+    reference CPU path (the oracle's restatement, later arc kept in the build) differs
+    from exact APSP, which is why solve() must not take the boundary-graph form there."""
+    base = F.FapGraph.road_grid(16, 16, 0.1, 0.02, 3, 4)
+    base.preCondition()
+    assert not base.boundaryBySssp()
+    csr, lay = base.csr(), base.layout()
+    ro, col, w = _with_parallel_arcs(csr, lay, heavier_later=True)
+    g = F.FapGraph.from_csr(ro, col, w, True, 4)
+    g.setPartition(lay["graph_id"])
+    g.preCondition()
+    assert g.boundaryBySssp()
+    ro2, col2, w2 = _with_parallel_arcs(csr, lay, heavier_later=False)
+    g2 = F.FapGraph.from_csr(ro2, col2, w2, True, 4)
+    g2.setPartition(lay["graph_id"])
+    g2.preCondition()
+    assert not g2.boundaryBySssp()
+    c, l2 = g.csr(), g.layout()
+    ref, _ = O.solve_parts_path(1, g.K, c, l2)
+    exact = O.exact_rows(c["row_offset"], c["col"], c["weight"].astype(np.int32), l2["st2ed"])
+    assert not np.array_equal(ref, exact)
