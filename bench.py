@@ -376,6 +376,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--skip-forms", action="store_true",
                     help="skip the overlap and int32-FW comparison solves (for profiling)")
+    ap.add_argument("--in-part", default="auto", choices=["auto", "fw"],
+                    help="in-part distances of the timed solves: auto (bit-parallel BFS where "
+                         "it applies) or fw (Floyd-Warshall; for profiling its kernels)")
     ap.add_argument("--mode", default="batched", choices=["batched", "overlap", "plain"],
                     help="execution form of the timed solves (default: batched)")
     ap.add_argument("--minplus", type=int, default=0,
@@ -419,6 +422,8 @@ def main():
         f"part size mean {num.mean():.1f} max {num.max()}")
 
     modes = {"batched": F.FAP_MODE_BATCHED, "overlap": F.FAP_MODE_OVERLAP, "plain": F.FAP_MODE_PLAIN}
+    if args.in_part == "fw":
+        modes = {k: v | F.FAP_MODE_FLAG_FW for k, v in modes.items()}
     g.setMode(modes[args.mode])
     for i in range(args.warmup):
         g.solve()

@@ -347,14 +347,9 @@ void fapGraph::alloc_result(int32_t **d, const char *what) {
 void fapGraph::solve_placed(const fap_graph_dev &g, const std::vector<fap_part_desc> &parts) {
   const int64_t bytes = (int64_t)num_vertexs * ld_dist * 4;
   std::vector<std::pair<int32_t *, float>> cand;
-  for (int c = 0; c < kPlaceCandidates; ++c) {
-    int32_t *p = nullptr;
-    const int s = fap_hip_malloc((void **)&p, bytes);
-    if (s != FAP_OK) {
-      if (c == 0) check(s, "n x n distance allocation");
-      break;  // no room for another candidate: choose among those held
-    }
-    cand.push_back({p, 0.0f});
+  // One solve into buffer p; returns its merge-stage time (frees every candidate and
+  // throws on failure).
+  auto solve_into = [&](int32_t *p) -> float {
     const int r = float_weights
                       ? fap_hip_solve_parts_f32(&g, parts.data(), (int32_t)parts.size(),
                                                 reinterpret_cast<float *>(p), ld_dist, mode,
@@ -368,8 +363,21 @@ void fapGraph::solve_placed(const fap_graph_dev &g, const std::vector<fap_part_d
       for (auto &x : cand) fap_hip_free(x.first);
       check(st, "solve");
     }
-    cand.back().second = t.minplus_ms;
+    return t.minplus_ms;
+  };
+  for (int c = 0; c < kPlaceCandidates; ++c) {
+    int32_t *p = nullptr;
+    const int s = fap_hip_malloc((void **)&p, bytes);
+    if (s != FAP_OK) {
+      if (c == 0) check(s, "n x n distance allocation");
+      break;  // no room for another candidate: choose among those held
+    }
+    cand.push_back({p, 0.0f});
+    cand.back().second = solve_into(p);
   }
+  // The first solve of a graph also pays one-time costs (workspace allocation, first
+  // launches): time the first candidate again, warm, when there is a choice to make.
+  if (cand.size() > 1) cand[0].second = solve_into(cand[0].first);
   size_t best = 0;
   for (size_t k = 1; k < cand.size(); ++k)
     if (cand[k].second < cand[best].second) best = k;
