@@ -53,7 +53,7 @@ constexpr int kBT = 64;           // tile edge of the Floyd-Warshall matrices
 constexpr int kBatchMaxS = kMaxPart;  // largest part
 constexpr int kBQ = 1024;         // SSSP frontier queue entries per wave
 constexpr int kSWaves = 4;        // waves (sources) per workgroup of the SSSP launch
-constexpr int kMCols = 256;       // merge: columns per unit (one per thread)
+constexpr int kMCols = 256;       // merge: columns per unit (four per lane)
 constexpr int kKc = 64;           // k per LDS chunk of the products
 
 // One graph of the batch.
@@ -88,8 +88,8 @@ struct BPart {
   int bfs;                       // in-part distances by bit-parallel BFS (no build, no FW)
 };
 
-struct BRowUnit {                // stage 5: rows r0 .. of graph parts[q].g, columns of q
-  int q, r0;
+struct BRowUnit {                // stage 5: boundary-graph rows r0 .. of graph parts[q].g,
+  int q, r0, cb, pad;            // columns cb * 256 .. of part q
 };
 
 struct BMerge {                  // stage 6: one unit (part, row block, column block)
@@ -318,59 +318,6 @@ __global__ __launch_bounds__(256) void bgb_kernel(const BGraph *__restrict__ gra
   }
 }
 
-// ---------------------------------------------------------------- 5: boundary rows
-// Unit (q, r0): rows r0 .. r0 + RB - 1 of graph g's boundary graph against the columns
-// of part q: C[x][j] = min_{c < b_q} D_B[x][bs_q + c] + d_q[c][j], written to the result
-// at row brow[x], column st2ed[off_q + j] (and to the FP64 rows R).  D_B entries are
-// staged k-major in LDS (broadcast reads), d_q rows are read coalesced.
-template <typename T, int RB>
-__global__ __launch_bounds__(256) void brows_kernel(const BGraph *__restrict__ graphs,
-                                                    const BPart *__restrict__ parts,
-                                                    const BRowUnit *__restrict__ units,
-                                                    const int64_t *__restrict__ brow) {
-  typedef typename Dist<T>::Out Out;
-  __shared__ T As[kKc][RB];
-  const BRowUnit un = units[blockIdx.x];
-  const BPart q = parts[un.q];
-  const BGraph g = graphs[q.g];
-  const int tid = threadIdx.x, r0 = un.r0;
-  const FAP_GLOBAL T *MB = gptr(static_cast<const T *>(g.MB));
-  const FAP_GLOBAL T *Dq = gptr(static_cast<const T *>(q.D));
-  FAP_GLOBAL Out *out = gptr(static_cast<Out *>(g.out));
-  FAP_GLOBAL T *R = gptr(static_cast<T *>(g.R));
-  for (int j0 = 0; j0 < q.s; j0 += 256) {
-    const int j = j0 + tid;
-    const bool jok = j < q.s;
-    T acc[RB];
-#pragma unroll
-    for (int r = 0; r < RB; ++r) acc[r] = Dist<T>::inf();
-    for (int k0 = 0; k0 < q.b; k0 += kKc) {
-      const int kc = min(kKc, q.b - k0);
-      for (int e = tid; e < kKc * RB; e += 256) {
-        const int r = e / kKc, k = e - r * kKc;
-        As[k][r] = (k < kc && r0 + r < g.nb) ? MB[(int64_t)(r0 + r) * g.SB + q.bs + k0 + k]
-                                             : Dist<T>::inf();
-      }
-      __syncthreads();
-      for (int k = 0; k < kc; ++k) {
-        const T dv = jok ? Dq[(int64_t)(k0 + k) * q.S + j] : Dist<T>::inf();
-#pragma unroll
-        for (int r = 0; r < RB; ++r) acc[r] = Dist<T>::mn(acc[r], As[k][r] + dv);
-      }
-      __syncthreads();
-    }
-    const int v = jok ? g.st2ed[q.off + j] : -1;
-    if (v >= 0 && v < g.n) {  // (a corrupt st2ed entry writes nothing)
-#pragma unroll
-      for (int r = 0; r < RB; ++r) {
-        if (r0 + r >= g.nb) continue;
-        out[brow[g.bbase + r0 + r] * g.ld_out + v] = Dist<T>::out(acc[r]);
-        if (g.R) R[(int64_t)(r0 + r) * g.ldr + v] = acc[r];
-      }
-    }
-  }
-}
-
 // Partial items: the SSSP rows (scratch) to the result rows (int32, or float32 from
 // FP64).
 template <typename T>
@@ -400,181 +347,308 @@ constexpr int kMWaves = 4;
 template <typename T>
 constexpr int kAPad = 16 / (int)sizeof(T);
 
-// One merge unit.  kPacked (int32, unit weights, n <= kCap16): A and B capped at kCap16,
-// two columns per register, a step on two entries one v_pk_add_u16 and half a
-// v_pk_minimum3_f16 (pk16.h); every finite distance is below the cap, so a result at
-// the cap is unreachable.
+// Operands of one min-plus unit: rows 0 .. 4*RB - 1 of A (the first `rows` valid, the
+// rest +inf) against columns col0 + l + 64e (e < 4) of B, over k < kdim.  B columns
+// above colmax are read at colmax (their results are never stored), so every B load
+// is in bounds without a mask.
+template <typename T>
+struct MpArgs {
+  const FAP_GLOBAL T *A;
+  int64_t lda;
+  int rows, kdim;
+  const FAP_GLOBAL T *B;
+  int64_t ldb;
+  int col0, colmax;
+};
+
+// Stages A[r][k0 + k] (r < 4*RB, k < kKc; +inf outside rows < m.rows, k < kc) k-major
+// in LDS; packed: capped at kCap16 and broadcast to both halves.
+template <typename T, int RB, bool kPacked>
+__device__ __forceinline__ void mp_stage(const MpArgs<T> &m, T (*As)[kMWaves * RB + kAPad<T>],
+                                         int k0, int kc, int tid) {
+  constexpr int kRows = kMWaves * RB;
+  for (int x = tid; x < kKc * kRows; x += 256) {
+    const int r = x / kKc, k = x - r * kKc;
+    T a = (k < kc && r < m.rows) ? m.A[(int64_t)r * m.lda + k0 + k] : Dist<T>::inf();
+    if constexpr (kPacked) {
+      const unsigned u = (unsigned)a < (unsigned)kCap16 ? (unsigned)a : (unsigned)kCap16;
+      a = (T)(u * 0x10001u);
+    }
+    As[k][r] = a;
+  }
+}
+
+template <typename T, int RB, bool kPacked>
+__device__ __forceinline__ void mp_init(T (&acc)[RB][kPacked ? 2 : 4]) {
+#pragma unroll
+  for (int r = 0; r < RB; ++r)
+#pragma unroll
+    for (int e = 0; e < (kPacked ? 2 : 4); ++e) acc[r][e] = kPacked ? (T)kPk16Top : Dist<T>::inf();
+}
+
+// acc[r][.] = min(acc, min_{k < kc} As[k][wave*RB + r] + B[k0 + k][col]) for the wave's
+// RB rows and the lane's four columns col0 + l + 64e.  kPacked (int32, unit weights,
+// n <= kCap16): two columns per register, a step on two entries one v_pk_add_u16 and
+// half a v_pk_minimum3_f16 (pk16.h); every finite distance is below the cap, so a
+// result at the cap is unreachable.  The four waves read the same B rows (L1).
+template <typename T, int RB, bool kPacked>
+__device__ __forceinline__ void mp_chunk(const MpArgs<T> &m, const T (*As)[kMWaves * RB + kAPad<T>],
+                                         int k0, int kc, int col0, T (&acc)[RB][kPacked ? 2 : 4],
+                                         int tid) {
+  constexpr int kLdA = kMWaves * RB + kAPad<T>;
+  const int wave = tid >> 6, lane = tid & 63;
+  const FAP_GLOBAL T *B = m.B;
+  int col[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) col[e] = min(col0 + e * 64 + lane, m.colmax);
+  const T *Aw = &As[0][wave * RB];
+  if constexpr (sizeof(T) == 8) {
+    // FP64 has no three-operand minimum: one k per step (fewer live registers), row
+    // k + 1 in flight while row k is used (two register buffers).
+    auto load_b = [&](int k, T (&x)[4]) {
+      const FAP_GLOBAL T *rp = B + (int64_t)(k0 + k) * m.ldb;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) x[e] = rp[col[e]];
+    };
+    auto step = [&](int k, const T (&bv)[4]) {
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const T a = Aw[k * kLdA + r];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[r][e] = Dist<T>::mn(acc[r][e], a + bv[e]);
+      }
+    };
+    T b0[4], b1[4];
+    load_b(0, b0);
+    int k = 0;
+    for (; k + 1 < kc; k += 2) {
+      load_b(k + 1, b1);
+      step(k, b0);
+      // Unconditional (the last row again at the end): a load behind a branch makes
+      // the wait before step k + 1 cover it too.
+      load_b(min(k + 2, kc - 1), b0);
+      step(k + 1, b1);
+    }
+    if (k < kc) step(k, b0);
+  } else {
+    // Rows k, k + 1 of the chunk as one pair (the three-operand minimum), the next
+    // pair in flight while this one is used; rows past kc read as +inf (row kc of As
+    // is +inf when kc is odd, and a pair past the end loads row kc - 1 again).
+    auto load_b = [&](int k, T (&x)[4]) {
+      const FAP_GLOBAL T *rp = B + (int64_t)(k0 + min(k, kc - 1)) * m.ldb;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) x[e] = rp[col[e]];
+    };
+    T n0[4], n1[4];
+    load_b(0, n0);
+    load_b(1, n1);
+    for (int k = 0; k < kc; k += 2) {
+      T b0[4], b1[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        b0[e] = n0[e];
+        b1[e] = n1[e];
+      }
+      load_b(k + 2, n0);
+      load_b(k + 3, n1);
+      T a0[RB], a1[RB];
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        a0[r] = Aw[k * kLdA + r];
+        a1[r] = Aw[(k + 1) * kLdA + r];  // row kc of As is INF when kc is odd
+      }
+      if constexpr (kPacked) {
+        auto cap = [](T v) -> unsigned {
+          const unsigned u = (unsigned)v;
+          return u < (unsigned)kCap16 ? u : (unsigned)kCap16;
+        };
+        const unsigned p0[2] = {cap(b0[0]) | (cap(b0[1]) << 16u),
+                                cap(b0[2]) | (cap(b0[3]) << 16u)};
+        const unsigned p1[2] = {cap(b1[0]) | (cap(b1[1]) << 16u),
+                                cap(b1[2]) | (cap(b1[3]) << 16u)};
+#pragma unroll
+        for (int r = 0; r < RB; ++r)
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const unsigned s0 = __builtin_bit_cast(
+                unsigned,
+                (us2)(__builtin_bit_cast(us2, (unsigned)a0[r]) + __builtin_bit_cast(us2, p0[h])));
+            const unsigned s1 = __builtin_bit_cast(
+                unsigned,
+                (us2)(__builtin_bit_cast(us2, (unsigned)a1[r]) + __builtin_bit_cast(us2, p1[h])));
+            acc[r][h] = (T)pk_min3((unsigned)acc[r][h], s0, s1);
+          }
+      } else {
+#pragma unroll
+        for (int r = 0; r < RB; ++r)
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            acc[r][e] = Dist<T>::mn3(acc[r][e], a0[r] + b0[e], a1[r] + b1[e]);
+      }
+    }
+  }
+}
+
+// One unit: acc over every k chunk, then epi(acc) (after the workgroup's last LDS read).
+template <typename T, int RB, bool kPacked, typename Epi>
+__device__ __forceinline__ void mp_unit(const MpArgs<T> &m, T (*As)[kMWaves * RB + kAPad<T>],
+                                        int tid, Epi &&epi) {
+  T acc[RB][kPacked ? 2 : 4];
+  mp_init<T, RB, kPacked>(acc);
+  for (int k0 = 0; k0 < m.kdim; k0 += kKc) {
+    const int kc = min(kKc, m.kdim - k0);
+    mp_stage<T, RB, kPacked>(m, As, k0, kc, tid);
+    __syncthreads();
+    mp_chunk<T, RB, kPacked>(m, As, k0, kc, m.col0, acc, tid);
+    __syncthreads();
+  }
+  epi(acc);
+}
+
+// Entry e (column col0 + l + 64e) of accumulator row r, unpacked.
+template <typename T, int RB, bool kPacked>
+__device__ __forceinline__ T mp_value(const T (&acc)[RB][kPacked ? 2 : 4], int r, int e) {
+  if constexpr (kPacked) {
+    const unsigned h = ((unsigned)acc[r][e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
+    return h >= (unsigned)kCap16 ? Dist<T>::inf() : (T)h;
+  } else {
+    return acc[r][e];
+  }
+}
+
+// One merge unit: inner rows r0 .. r0 + 4*RB - 1 of part p against the 256 columns of
+// column block cb.  Own columns take min(d_p(i, v), product) (the decode).
 template <typename T, int RB, bool kPacked>
 __device__ __forceinline__ void merge_unit(const BGraph &g, const BPart &p, int rb, int cb,
                                            T (*As)[kMWaves * RB + kAPad<T>], int tid) {
   typedef typename Dist<T>::Out Out;
   constexpr int kRows = kMWaves * RB;
-  constexpr int kLdA = kRows + kAPad<T>;
   const int wave = tid >> 6, lane = tid & 63;
   const int inner = p.s - p.b, r0 = rb * kRows;
   const FAP_GLOBAL T *D = gptr(static_cast<const T *>(p.D));
-  const FAP_GLOBAL T *B = gptr(static_cast<const T *>(p.Brows));
-  int col[4];
-  bool cok[4];
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    col[e] = cb * kMCols + e * 64 + lane;
-    cok[e] = col[e] < g.n;
-  }
-  constexpr int kAcc = kPacked ? 2 : 4;
-  T acc[RB][kAcc];
-#pragma unroll
-  for (int r = 0; r < RB; ++r)
-#pragma unroll
-    for (int e = 0; e < kAcc; ++e) acc[r][e] = kPacked ? (T)kPk16Top : Dist<T>::inf();
-  for (int k0 = 0; k0 < p.b; k0 += kKc) {
-    const int kc = min(kKc, p.b - k0);
-    for (int x = tid; x < kKc * kRows; x += 256) {
-      const int r = x / kKc, k = x - r * kKc;
-      T a = (k < kc && r0 + r < inner) ? D[(int64_t)(p.b + r0 + r) * p.S + k0 + k]
-                                       : Dist<T>::inf();
-      if constexpr (kPacked) {
-        const unsigned u = (unsigned)a < (unsigned)kCap16 ? (unsigned)a : (unsigned)kCap16;
-        a = (T)(u * 0x10001u);
-      }
-      As[k][r] = a;
-    }
-    __syncthreads();
-    const T *Aw = &As[0][wave * RB];
-    if constexpr (sizeof(T) == 8) {
-      // FP64 has no three-operand minimum: one k per step (fewer live registers).  The
-      // FP64 boundary rows are padded with +inf to a multiple of 256 columns, so every
-      // load is in bounds without a mask, and row k + 1 is in flight while row k is
-      // used (two register buffers).
-      auto load_b = [&](int k, T (&x)[4]) {
-        const FAP_GLOBAL T *rp = B + (int64_t)(k0 + k) * p.ldb;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) x[e] = rp[col[e]];
-      };
-      auto step = [&](int k, const T (&bv)[4]) {
-#pragma unroll
-        for (int r = 0; r < RB; ++r) {
-          const T a = Aw[k * kLdA + r];
-#pragma unroll
-          for (int e = 0; e < 4; ++e) acc[r][e] = Dist<T>::mn(acc[r][e], a + bv[e]);
-        }
-      };
-      T b0[4], b1[4];
-      load_b(0, b0);
-      int k = 0;
-      for (; k + 1 < kc; k += 2) {
-        load_b(k + 1, b1);
-        step(k, b0);
-        // Unconditional (the last row again at the end): a load behind a branch makes
-        // the wait before step k + 1 cover it too.
-        load_b(min(k + 2, kc - 1), b0);
-        step(k + 1, b1);
-      }
-      if (k < kc) step(k, b0);
-    } else {
-      // Rows k, k + 1 of the chunk as one pair (the three-operand minimum), the next
-      // pair in flight while this one is used; rows past kc read as +inf (row kc of As
-      // is +inf when kc is odd, and a pair past the end loads row kc - 1 again).
-      auto load_b = [&](int k, T (&x)[4]) {
-        const FAP_GLOBAL T *rp = B + (int64_t)(k0 + min(k, kc - 1)) * p.ldb;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) x[e] = rp[col[e]];
-      };
-      T n0[4], n1[4];
-      load_b(0, n0);
-      load_b(1, n1);
-      for (int k = 0; k < kc; k += 2) {
-        T b0[4], b1[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          b0[e] = n0[e];
-          b1[e] = n1[e];
-        }
-        load_b(k + 2, n0);
-        load_b(k + 3, n1);
-        T a0[RB], a1[RB];
-#pragma unroll
-        for (int r = 0; r < RB; ++r) {
-          a0[r] = Aw[k * kLdA + r];
-          a1[r] = Aw[(k + 1) * kLdA + r];  // row kc of As is INF when kc is odd
-        }
-        if constexpr (kPacked) {
-          auto cap = [](T v) -> unsigned {
-            const unsigned u = (unsigned)v;
-            return u < (unsigned)kCap16 ? u : (unsigned)kCap16;
-          };
-          const unsigned p0[2] = {cap(b0[0]) | (cap(b0[1]) << 16u),
-                                  cap(b0[2]) | (cap(b0[3]) << 16u)};
-          const unsigned p1[2] = {cap(b1[0]) | (cap(b1[1]) << 16u),
-                                  cap(b1[2]) | (cap(b1[3]) << 16u)};
-#pragma unroll
-          for (int r = 0; r < RB; ++r)
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-              const unsigned s0 = __builtin_bit_cast(
-                  unsigned,
-                  (us2)(__builtin_bit_cast(us2, (unsigned)a0[r]) + __builtin_bit_cast(us2, p0[h])));
-              const unsigned s1 = __builtin_bit_cast(
-                  unsigned,
-                  (us2)(__builtin_bit_cast(us2, (unsigned)a1[r]) + __builtin_bit_cast(us2, p1[h])));
-              acc[r][h] = (T)pk_min3((unsigned)acc[r][h], s0, s1);
-            }
-        } else {
-#pragma unroll
-          for (int r = 0; r < RB; ++r)
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-              acc[r][e] = Dist<T>::mn3(acc[r][e], a0[r] + b0[e], a1[r] + b1[e]);
-        }
-      }
-    }
-    __syncthreads();
-  }
-  int own[4];
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    own[e] = cok[e] && g.graph_id[col[e]] == p.part_id ? g.ed2st[col[e]] - p.off : -1;
-    if (own[e] >= p.s) own[e] = -1;
-  }
+  MpArgs<T> m;
+  m.A = D + (int64_t)(p.b + r0) * p.S;
+  m.lda = p.S;
+  m.rows = inner - r0;
+  m.kdim = p.b;
+  m.B = gptr(static_cast<const T *>(p.Brows));
+  m.ldb = p.ldb;
+  m.col0 = cb * kMCols;
+  m.colmax = INT32_MAX;  // boundary rows are padded with +inf to whole column blocks
   FAP_GLOBAL Out *out = gptr(static_cast<Out *>(g.out));
-#pragma unroll
-  for (int r = 0; r < RB; ++r) {
-    const int i = r0 + wave * RB + r;
-    if (i >= inner) continue;
-    FAP_GLOBAL Out *orow = out + (p.out_row + p.b + i) * g.ld_out;
-    const FAP_GLOBAL T *drow = D + (int64_t)(p.b + i) * p.S;
+  mp_unit<T, RB, kPacked>(m, As, tid, [&](const T (&acc)[RB][kPacked ? 2 : 4]) {
+    int col[4], own[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      if (!cok[e]) continue;
-      T val;
-      if constexpr (kPacked) {
-        const unsigned h = ((unsigned)acc[r][e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
-        val = h >= (unsigned)kCap16 ? Dist<T>::inf() : (T)h;
-      } else {
-        val = acc[r][e];
-      }
-      if (own[e] >= 0) val = Dist<T>::mn(val, drow[own[e]]);
-      orow[col[e]] = Dist<T>::out(val);
+      col[e] = cb * kMCols + e * 64 + lane;
+      own[e] = col[e] < g.n && g.graph_id[col[e]] == p.part_id ? g.ed2st[col[e]] - p.off : -1;
+      if (own[e] >= p.s) own[e] = -1;
     }
-  }
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int i = r0 + wave * RB + r;
+      if (i >= inner) continue;
+      FAP_GLOBAL Out *orow = out + (p.out_row + p.b + i) * g.ld_out;
+      const FAP_GLOBAL T *drow = D + (int64_t)(p.b + i) * p.S;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (col[e] >= g.n) continue;
+        T val = mp_value<T, RB, kPacked>(acc, r, e);
+        if (own[e] >= 0) val = Dist<T>::mn(val, drow[own[e]]);
+        orow[col[e]] = Dist<T>::out(val);
+      }
+    }
+  });
 }
 
-// Persistent grid; a unit is (part, row block of 4 * RB inner rows, column block of 256
-// columns), the row blocks of one column block consecutive so its boundary-row panel
-// is reused from L2.  Wave w owns rows w*RB .. w*RB + RB - 1 of the block and lane l
-// the columns l + 64e (e < 4), so every load and store of a wave covers 256
-// contiguous bytes.  The 4*RB x 64 A block is staged k-major in LDS (broadcast reads);
-// the four waves read the same B rows (L1).  Own columns take min(d_p(i, v), product).
+// ---------------------------------------------------------------- 5: boundary rows
+// Unit (q, r0, cb): boundary-graph rows x = r0 .. r0 + 4*RB - 1 of graph g against
+// columns j = cb*256 .. of part q: C[x][j] = min_{c < b_q} D_B[x][bs_q + c] + d_q[c][j]
+// (the same min-plus unit as the merge: D_B rows staged k-major in LDS, d_q rows read
+// four columns per lane, three-operand or packed 16-bit minima), written to the result
+// at row brow[x], column st2ed[off_q + j], and to the padded boundary rows R.  The
+// first form (one column per thread, two VALU instructions per step, 32 broadcast LDS
+// reads per k) spent as many LDS cycles as VALU cycles.
+template <typename T, int RB, bool kPacked>
+__device__ __forceinline__ void brows_unit(const BGraph &g, const BPart &q, int r0, int cb,
+                                           const int64_t *__restrict__ brow,
+                                           T (*As)[kMWaves * RB + kAPad<T>], int tid) {
+  typedef typename Dist<T>::Out Out;
+  const int wave = tid >> 6, lane = tid & 63;
+  MpArgs<T> m;
+  m.A = gptr(static_cast<const T *>(g.MB)) + (int64_t)r0 * g.SB + q.bs;
+  m.lda = g.SB;
+  m.rows = g.nb - r0;
+  m.kdim = q.b;
+  m.B = gptr(static_cast<const T *>(q.D));
+  m.ldb = q.S;
+  m.col0 = cb * kMCols;
+  m.colmax = q.S - 1;  // padded columns read +inf; clamped ones are never stored
+  FAP_GLOBAL Out *out = gptr(static_cast<Out *>(g.out));
+  FAP_GLOBAL T *R = gptr(static_cast<T *>(g.R));
+  mp_unit<T, RB, kPacked>(m, As, tid, [&](const T (&acc)[RB][kPacked ? 2 : 4]) {
+    int v[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int j = cb * kMCols + e * 64 + lane;
+      v[e] = j < q.s ? g.st2ed[q.off + j] : -1;
+      if (v[e] >= g.n) v[e] = -1;  // (a corrupt st2ed entry writes nothing)
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int x = r0 + wave * RB + r;
+      if (x >= g.nb) continue;
+      FAP_GLOBAL Out *orow = out + brow[g.bbase + x] * g.ld_out;
+      FAP_GLOBAL T *rrow = R + (int64_t)x * g.ldr;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (v[e] < 0) continue;
+        const T val = mp_value<T, RB, kPacked>(acc, r, e);
+        orow[v[e]] = Dist<T>::out(val);
+        if (g.R) rrow[v[e]] = val;
+      }
+    }
+  });
+}
+
+template <typename T, int RB>
+__global__ __launch_bounds__(256) void brows_kernel(const BGraph *__restrict__ graphs,
+                                                    const BPart *__restrict__ parts,
+                                                    const BRowUnit *__restrict__ units,
+                                                    const int64_t *__restrict__ brow) {
+  __shared__ __attribute__((aligned(16))) T As[kKc][kMWaves * RB + kAPad<T>];
+  const BRowUnit un = units[blockIdx.x];
+  if (un.q < 0) return;  // past the end of its XCD's list
+  const BPart q = parts[un.q];
+  const BGraph g = graphs[q.g];
+  if constexpr (sizeof(T) == 4) {
+    if (g.packed) {
+      brows_unit<T, RB, true>(g, q, un.r0, un.cb, brow, As, threadIdx.x);
+      return;
+    }
+  }
+  brows_unit<T, RB, false>(g, q, un.r0, un.cb, brow, As, threadIdx.x);
+}
+
+// A unit is (part, row block of 4 * RB inner rows, column block of 256 columns).  The
+// units arrive in launch slots: slot 8 * i + x holds the i-th unit of XCD x (or none),
+// where all row blocks of one (part, column block) are on one XCD, so its boundary-row
+// panel is read from HBM once and then from that XCD's L2 (with units pulled in global
+// order every XCD fetched every panel: 7x the algorithmic reads).  Persistent grid
+// (`counters`, one per XCD): workgroup b starts on XCD b % 8, takes the units of its own
+// list and then helps the others; plain grid (no counters): workgroup L runs slot L.
 template <typename T, int RB>
 __global__ __launch_bounds__(256, 4) void bmerge_kernel(const BGraph *__restrict__ graphs,
                                                      const BPart *__restrict__ parts,
-                                                     const BMerge *__restrict__ units,
-                                                     int total_units, int *counter) {
+                                                     const BMerge *__restrict__ slots,
+                                                     int nslots, int *counters,
+                                                     const int *__restrict__ lens) {
   __shared__ __attribute__((aligned(16))) T As[kKc][kMWaves * RB + kAPad<T>];
   __shared__ int unit_s;
   const int tid = threadIdx.x;
   auto run = [&](int u) {
-    const BMerge m = units[u];
+    const BMerge m = slots[u];
+    if (m.p < 0) return;  // past the end of its XCD's list
     const BPart p = parts[m.p];
     const BGraph g = graphs[p.g];
     if constexpr (sizeof(T) == 4) {
@@ -585,18 +659,29 @@ __global__ __launch_bounds__(256, 4) void bmerge_kernel(const BGraph *__restrict
     }
     merge_unit<T, RB, false>(g, p, m.rb, m.cb, As, tid);
   };
-  if (!counter) {  // plain grid: one unit per workgroup, in unit order
-    if ((int)blockIdx.x < total_units) run(blockIdx.x);
+  if (!counters) {  // plain grid: one slot per workgroup
+    if ((int)blockIdx.x < nslots) run(blockIdx.x);
     return;
   }
+  const int home = blockIdx.x & 7;
   while (true) {
-    if (tid == 0) unit_s = atomicAdd(counter, 1);
+    if (tid == 0) {
+      int u = -1;
+      for (int t = 0; t < 8 && u < 0; ++t) {  // own list first, then the others
+        const int x = (home + t) & 7;
+        if (__hip_atomic_load(counters + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= lens[x])
+          continue;
+        const int i = atomicAdd(counters + x, 1);
+        if (i < lens[x]) u = 8 * i + x;
+      }
+      unit_s = u;
+    }
     __syncthreads();
     // Uniform by construction; readfirstlane lets the compiler keep the unit's
     // descriptors and pointers in scalar registers (scalar loads, scalar address math).
     const int u = __builtin_amdgcn_readfirstlane(unit_s);
     __syncthreads();  // every thread has read unit_s before thread 0 takes the next one
-    if (u >= total_units) break;  // uniform: every workgroup ends here
+    if (u < 0) break;  // uniform: every list is drained
     run(u);
   }
 }
@@ -613,6 +698,36 @@ __global__ void reset_counter_kernel(int *c, int n) {
 }
 
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+// Launch slots for units grouped by the data they share: group g is units
+// [gb[g], gb[g + 1]); groups are dealt to the 8 XCDs, largest cost first, each to the
+// XCD with the least cost so far, and slot 8 * i + x holds the i-th unit of XCD x
+// (`none` past the end of its list).  A grid launched over the slots puts workgroup L on
+// XCD L % 8, so the units of a group share one L2.
+template <typename U>
+std::vector<U> xcd_slots(const std::vector<U> &units, const std::vector<int> &gb,
+                         const std::vector<double> &gcost, U none, int (&len)[8]) {
+  const int ng = (int)gcost.size();
+  std::vector<int> order(ng);
+  for (int g = 0; g < ng; ++g) order[g] = g;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return gcost[a] > gcost[b]; });
+  std::vector<int> lists[8];
+  double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int g : order) {
+    const int x = (int)(std::min_element(load, load + 8) - load);
+    for (int u = gb[g]; u < gb[g + 1]; ++u) lists[x].push_back(u);
+    load[x] += gcost[g];
+  }
+  size_t maxlen = 0;
+  for (int x = 0; x < 8; ++x) {
+    len[x] = (int)lists[x].size();
+    maxlen = std::max(maxlen, lists[x].size());
+  }
+  std::vector<U> slots(8 * maxlen, none);
+  for (int x = 0; x < 8; ++x)
+    for (size_t i = 0; i < lists[x].size(); ++i) slots[8 * i + x] = units[lists[x][i]];
+  return slots;
+}
 
 // Row-block sizes (template instances): stage 5 sources per unit, stage 6 rows per unit.
 int pick_rows(int code, int dflt) {
@@ -634,6 +749,9 @@ struct fap_batch_kind {
   int ngraphs = 0, nparts = 0, build_rows = 0, max_nb = 0;
   int ngb = 0, gb_rows = 0, gb_max_nb = 0, nunits5 = 0, nmerge = 0, merge_units = 0, nsrc = 0;
   int rows6 = 16;  // merge rows per wave (FP64: 8, register budget)
+  int rb5 = 8;     // boundary-row unit rows per wave (4 * rb5 boundary-graph rows per unit)
+  int mlen[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // merge units per XCD list
+  int *d_mlen = nullptr;
   int grid = 0;    // persistent merge workgroups
   int nfw = 0;     // parts whose in-part matrix comes from Floyd-Warshall
   // In-part BFS descriptors: parts of at most 1024 vertices first (n2 of them, g2
@@ -705,9 +823,9 @@ int launch_kind(const fap_batch *b, const fap_batch_kind &q, int *counter, hipSt
   }
   if (q.nunits5 > 0) {
     const dim3 g5(q.nunits5), t5(256);
-    if (b->rows5 == 8) hipLaunchKernelGGL((brows_kernel<T, 8>), g5, t5, 0, st, q.graphs, q.parts, q.units5, q.brow);
-    else if (b->rows5 == 32) hipLaunchKernelGGL((brows_kernel<T, 32>), g5, t5, 0, st, q.graphs, q.parts, q.units5, q.brow);
-    else hipLaunchKernelGGL((brows_kernel<T, 16>), g5, t5, 0, st, q.graphs, q.parts, q.units5, q.brow);
+    if (q.rb5 == 4) hipLaunchKernelGGL((brows_kernel<T, 4>), g5, t5, 0, st, q.graphs, q.parts, q.units5, q.brow);
+    else if (q.rb5 == 16) hipLaunchKernelGGL((brows_kernel<T, 16>), g5, t5, 0, st, q.graphs, q.parts, q.units5, q.brow);
+    else hipLaunchKernelGGL((brows_kernel<T, 8>), g5, t5, 0, st, q.graphs, q.parts, q.units5, q.brow);
     FAP_CHECK_LAUNCH();
   }
   if (q.nsrc > 0) {
@@ -720,27 +838,29 @@ int launch_kind(const fap_batch *b, const fap_batch_kind &q, int *counter, hipSt
     const dim3 gm(b->unit_grid ? q.merge_units : std::min(q.grid, q.merge_units)), tm(256);
     int *c = b->unit_grid ? nullptr : counter;
     if (q.rows6 == 8)
-      hipLaunchKernelGGL((bmerge_kernel<T, 8>), gm, tm, 0, st, q.graphs, q.parts, q.merge, q.merge_units, c);
+      hipLaunchKernelGGL((bmerge_kernel<T, 8>), gm, tm, 0, st, q.graphs, q.parts, q.merge,
+                         q.merge_units, c, q.d_mlen);
     else
-      hipLaunchKernelGGL((bmerge_kernel<T, 16>), gm, tm, 0, st, q.graphs, q.parts, q.merge, q.merge_units, c);
+      hipLaunchKernelGGL((bmerge_kernel<T, 16>), gm, tm, 0, st, q.graphs, q.parts, q.merge,
+                         q.merge_units, c, q.d_mlen);
     FAP_CHECK_LAUNCH();
   }
   return FAP_OK;
 }
 
-// The two kinds share nothing but the counter array (one counter each), so they run
+// The two kinds share nothing but the counter array (eight counters each), so they run
 // side by side: the Floyd-Warshall rounds of one (short, latency-bound launches) fill
 // in around the other's long launches.
 int batch_launch(fap_batch *b, hipStream_t st) {
-  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, st, b->d_counter, 2);
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, st, b->d_counter, 16);
   FAP_CHECK_LAUNCH();
   if (!b->side) {
     FAP_TRY(launch_kind<int>(b, b->kind[0], b->d_counter, st));
-    return launch_kind<double>(b, b->kind[1], b->d_counter + 1, st);
+    return launch_kind<double>(b, b->kind[1], b->d_counter + 8, st);
   }
   FAP_CHECK_HIP(hipEventRecord(b->ev_fork, st));
   FAP_CHECK_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
-  FAP_TRY(launch_kind<double>(b, b->kind[1], b->d_counter + 1, b->side));
+  FAP_TRY(launch_kind<double>(b, b->kind[1], b->d_counter + 8, b->side));
   FAP_TRY(launch_kind<int>(b, b->kind[0], b->d_counter, st));
   FAP_CHECK_HIP(hipEventRecord(b->ev_join, b->side));
   FAP_CHECK_HIP(hipStreamWaitEvent(st, b->ev_join, 0));
@@ -752,6 +872,9 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
               bool fp64) {
   const int elem = fp64 ? 8 : 4;
   q.rows6 = fp64 ? 8 : b->rows6;
+  // Boundary-source batch width (8, 16 or 32 sources) -> rows per wave of a unit of
+  // 4 waves (4, 8 or 16; FP64 at most 8, register budget).
+  q.rb5 = std::min(b->rows5 / 2, fp64 ? 8 : 16);
   // Persistent merge grid: as many workgroups as the device keeps resident (its
   // register and LDS use), unless the caller fixed the workgroups per CU.
   {
@@ -879,7 +1002,8 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
       mb_at.push_back(take((int64_t)G.SB * G.SB * elem));
       r_at.push_back(take((int64_t)G.nb * G.ldr * elem));
       for (int pi = (int)parts.size() - (int)ps.size(); pi < (int)parts.size(); ++pi)
-        for (int r0 = 0; r0 < G.nb; r0 += b->rows5) units5.push_back(BRowUnit{pi, r0});
+        for (int r0 = 0; r0 < G.nb; r0 += kMWaves * q.rb5)
+          for (int cb = 0; cb * kMCols < parts[pi].s; ++cb) units5.push_back(BRowUnit{pi, r0, cb, 0});
     } else {
       mb_at.push_back(-1);
       r_at.push_back(-1);
@@ -893,16 +1017,22 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return pcost[x] > pcost[y]; });
   // Units of a part: the row blocks of one column block consecutive (its boundary-row
   // panel stays in L2 while they run).
-  std::vector<BMerge> merge;
+  std::vector<BMerge> munits;
+  std::vector<int> mgb{0};
+  std::vector<double> mcost;
   for (int k : order) {
     const BPart &P = parts[k];
     const int rbs = (P.s - P.b + kMWaves * q.rows6 - 1) / (kMWaves * q.rows6);
     const int cbs = (graphs[P.g].n + kMCols - 1) / kMCols;
-    FAP_REQUIRE((int64_t)merge.size() + (int64_t)rbs * cbs < (1LL << 30),
+    FAP_REQUIRE((int64_t)munits.size() + (int64_t)rbs * cbs < (1LL << 27),
                 "fap_hip_batch_create: too many merge units");
-    for (int cb = 0; cb < cbs; ++cb)
-      for (int rb = 0; rb < rbs; ++rb) merge.push_back(BMerge{k, rb, cb, 0});
+    for (int cb = 0; cb < cbs; ++cb) {
+      for (int rb = 0; rb < rbs; ++rb) munits.push_back(BMerge{k, rb, cb, 0});
+      mgb.push_back((int)munits.size());
+      mcost.push_back((double)rbs * kMWaves * q.rows6 * (P.b + 8));
+    }
   }
+  const std::vector<BMerge> merge = xcd_slots(munits, mgb, mcost, BMerge{-1, 0, 0, 0}, q.mlen);
   const int64_t units = (int64_t)merge.size();
   FAP_REQUIRE(units5.size() < (1u << 31) && srcs.size() < (1u << 31),
               "fap_hip_batch_create: batch too large");
@@ -914,6 +1044,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   const int64_t o_pid = table(4 * pid2idx.size()), o_brow = table(8 * brow.size());
   const int64_t o_u5 = table(sizeof(BRowUnit) * units5.size());
   const int64_t o_merge = table(sizeof(BMerge) * merge.size());
+  const int64_t o_mlen = table(sizeof(q.mlen));
   const int64_t o_src = table(sizeof(BSrc) * srcs.size());
   const int64_t o_fw = table((fp64 ? sizeof(Fw64Desc) : sizeof(FwDesc)) * parts.size());
   const int64_t o_ip = table(sizeof(InpartDesc) * parts.size());
@@ -960,6 +1091,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   put(o_brow, brow.data(), 8 * brow.size());
   put(o_u5, units5.data(), sizeof(BRowUnit) * units5.size());
   put(o_merge, merge.data(), sizeof(BMerge) * merge.size());
+  put(o_mlen, q.mlen, sizeof(q.mlen));
   put(o_src, srcs.data(), sizeof(BSrc) * srcs.size());
   if (fp64) {
     std::vector<Fw64Desc> fd, gd;
@@ -1018,6 +1150,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   q.brow = reinterpret_cast<int64_t *>(base + o_brow);
   q.units5 = reinterpret_cast<BRowUnit *>(base + o_u5);
   q.merge = reinterpret_cast<BMerge *>(base + o_merge);
+  q.d_mlen = reinterpret_cast<int *>(base + o_mlen);
   q.srcs = reinterpret_cast<BSrc *>(base + o_src);
   q.fw_descs = base + o_fw;
   q.ip = reinterpret_cast<InpartDesc *>(base + o_ip);

@@ -28,6 +28,14 @@ hipStream_t lib_stream();
 hipStream_t aux_stream(int i);
 // Compute units of the device (set by fap_hip_init).
 int device_cus();
+
+// Workgroup L of a grid of `total` is dispatched to XCD L % 8; returns the index that
+// gives every XCD one contiguous range [x * total / 8, ...) (consecutive tiles of a part
+// then share that XCD's L2).
+__device__ __forceinline__ unsigned xcd_contiguous(unsigned L, unsigned total) {
+  const unsigned per = total >> 3, rem = total & 7, x = L & 7, slot = L >> 3;
+  return x < rem ? x * (per + 1) + slot : rem * (per + 1) + (x - rem) * per + slot;
+}
 // Device the library runs on (set by fap_hip_init), -1 before.
 int lib_device();
 // Looks up the device address of the allocation-free Floyd-Warshall scratch on

@@ -239,12 +239,12 @@ __global__ __launch_bounds__(256) void fw64_phase1_kernel(const Fw64Desc *__rest
   for (int q = 0; q < 16; ++q) g[(wave + 4 * q) * S + lane] = v[q];
 }
 
-// Phases 2 and 3: one 256-thread workgroup per 64 x 64 tile; thread t owns rows
-// (t / 16) * 4 + {0..3} and columns (t % 16) + 16 * {0..3} (4 x 4 accumulators, so
-// ~64 VGPRs and many waves per SIMD; a one-wave form with 8 x 8 doubles per lane used
-// 248-256 VGPRs and ran at one or two waves per SIMD).  k in LDS chunks of 16:
-// As[k][i] (transposed A, four consecutive rows read as two 16-byte broadcasts),
-// Bs[k][j] (16 consecutive lanes read 16 consecutive doubles: conflict-free).
+// Phase 3 keeps the LDS-tiled form: one 256-thread workgroup per 64 x 64 tile, thread t
+// owns rows (t / 16) * 4 + {0..3} and columns (t % 16) + 16 * {0..3} (4 x 4
+// accumulators), k in LDS chunks of 16: As[k][i] (transposed A, four consecutive rows
+// read as two 16-byte broadcasts), Bs[k][j] (16 consecutive lanes read 16 consecutive
+// doubles: conflict-free).  Its A tile differs per tile row, so scalar loads of A (the
+// phase-2 form below) miss the scalar cache: measured 2x slower here.
 constexpr int kKc64 = 16;
 // As rows padded by 16 bytes: the staging writes (16 lanes on 16 consecutive k of one
 // row) land in 16 distinct bank pairs instead of one.
@@ -284,8 +284,6 @@ __device__ __forceinline__ void tile_product64(double (&acc)[4][4], const FAP_GL
   }
 }
 
-__device__ __forceinline__ int skip64(int o, int p) { return o < p ? o : o + 1; }
-
 __device__ __forceinline__ void load_tile64(double (&acc)[4][4], const FAP_GLOBAL double *t,
                                             int64_t S, int tr, int tc) {
 #pragma unroll
@@ -302,32 +300,115 @@ __device__ __forceinline__ void store_tile64(const double (&acc)[4][4], FAP_GLOB
     for (int c = 0; c < 4; ++c) t[(tr * 4 + q) * S + tc + 16 * c] = acc[q][c];
 }
 
-// The tile is an operand of its own product: every thread stores only after the
-// barrier that follows the last chunk's staging, so no read sees a new value.
-__global__ __launch_bounds__(256) void fw64_phase2_kernel(const Fw64Desc *__restrict__ descs, int r) {
-  const Fw64Desc d = descs[blockIdx.y];
-  const int others = d.nb - 1;
-  int o = blockIdx.x;
-  if (r >= d.nb || o >= 2 * others) return;
-  const bool is_row = o < others;
-  if (!is_row) o -= others;
-  o = skip64(o, r);
-  const int64_t S = (int64_t)d.nb * kT;
-  __shared__ __attribute__((aligned(16))) double As[kKc64][kLdA64];
-  __shared__ __attribute__((aligned(16))) double Bs[kKc64][kT];
-  FAP_GLOBAL double *t = is_row ? tile64(d, r, o) : tile64(d, o, r);
-  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
-  double acc[4][4];
-  load_tile64(acc, t, S, tr, tc);
-  if (is_row) tile_product64(acc, tile64(d, r, r), t, S, As, Bs, tr, tc);
-  else tile_product64(acc, t, tile64(d, r, r), S, As, Bs, tr, tc);
-  store_tile64(acc, t, S, tr, tc);
+// Phase 2: C = min(C, A (x) B) on one 64 x 64 tile per 512-thread workgroup.
+// Wave w owns rows 8w .. 8w + 7 of C, lane l column l (8 accumulators).  B (64 x 64)
+// is staged in LDS and read as Bs[k][lane] (one conflict-free 8-byte read per k for
+// 16 VALU instructions); the A entries of a wave's rows are wave-uniform and come
+// from scalar loads (SGPR operands of v_add_f64), two k ahead of their use.  The
+// earlier form (4 x 4 accumulators, A and B both from LDS) spent as many LDS cycles
+// as VALU cycles per k.  Tiles of one part run back to back on one XCD (its L2 keeps
+// the panels).
+constexpr int kFwRows = 8;    // rows of C per wave
+constexpr int kFwWaves = 8;   // waves per workgroup (kFwRows * kFwWaves = 64)
+constexpr int kFwLdB = kT;    // Bs row stride (doubles)
+
+// acc[q] = min(acc[q], min_k A[q][k] + Bs[k][lane]) over the 64 k of the tile, where
+// `a` points at row 0 of the wave's A rows (stride S, wave-uniform).
+__device__ __forceinline__ void tile_rows64(double (&acc)[kFwRows], const FAP_CONST double *a,
+                                           int64_t S, const double (*Bs)[kFwLdB], int lane) {
+  double an[kFwRows][2];
+#pragma unroll
+  for (int q = 0; q < kFwRows; ++q) {
+    an[q][0] = a[q * S + 0];
+    an[q][1] = a[q * S + 1];
+  }
+#pragma unroll 2
+  for (int k = 0; k < kT; k += 2) {
+    double ac[kFwRows][2];
+#pragma unroll
+    for (int q = 0; q < kFwRows; ++q) {
+      ac[q][0] = an[q][0];
+      ac[q][1] = an[q][1];
+    }
+    // Next pair (the last pair again at the end: an in-bounds, unused load).
+    const int kn = k + 2 < kT ? k + 2 : k;
+#pragma unroll
+    for (int q = 0; q < kFwRows; ++q) {
+      an[q][0] = a[q * S + kn];
+      an[q][1] = a[q * S + kn + 1];
+    }
+    const double b0 = Bs[k][lane], b1 = Bs[k + 1][lane];
+#pragma unroll
+    for (int q = 0; q < kFwRows; ++q) acc[q] = fmin(acc[q], ac[q][0] + b0);
+#pragma unroll
+    for (int q = 0; q < kFwRows; ++q) acc[q] = fmin(acc[q], ac[q][1] + b1);
+  }
 }
 
-__global__ __launch_bounds__(256) void fw64_phase3_kernel(const Fw64Desc *__restrict__ descs, int r) {
-  const Fw64Desc d = descs[blockIdx.y];
+// Stages the 64 x 64 tile t (row stride S) into Bs: 512 threads, 8 doubles each.
+__device__ __forceinline__ void stage_tile64(double (*Bs)[kFwLdB], const FAP_GLOBAL double *t,
+                                             int64_t S) {
+  const int e = threadIdx.x, row = e >> 3, c0 = (e & 7) * 8;
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const FAP_GLOBAL d2 *src = reinterpret_cast<const FAP_GLOBAL d2 *>(t + row * S + c0);
+  d2 *dst = reinterpret_cast<d2 *>(&Bs[row][c0]);
+  d2 v[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) v[q] = src[q];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) dst[q] = v[q];
+}
+
+__device__ __forceinline__ int skip64(int o, int p) { return o < p ? o : o + 1; }
+
+// One launch of phase 2 or 3 over every descriptor: grid (tiles per descriptor,
+// descriptors), remapped so each XCD runs a contiguous range of (descriptor, tile).
+//   phase 2, row panel T(r, j):    A = pivot tile (r, r), B = T(r, j) itself;
+//   phase 2, column panel T(i, r): A = T(i, r) itself, B = pivot tile (r, r);
+//   phase 3, T(i, j):              A = T(i, r), B = T(r, j).
+// A tile that is its own operand is safe: as B it is read from the LDS copy, and as A
+// each wave reads only the rows it writes itself, before it writes them.
+template <int kPhase>
+__global__ __launch_bounds__(64 * kFwWaves) void fw64_tile_kernel(const Fw64Desc *__restrict__ descs,
+                                                                  int r) {
+  const unsigned L = xcd_contiguous(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+  const int o = (int)(L % gridDim.x), di = (int)(L / gridDim.x);
+  const Fw64Desc d = descs[di];
   const int others = d.nb - 1;
-  const int o = blockIdx.x;
+  if (r >= d.nb || o >= (kPhase == 2 ? 2 * others : others * others)) return;
+  const int64_t S = (int64_t)d.nb * kT;
+  int ci, cj, ai, aj, bi, bj;
+  if (kPhase == 2) {
+    if (o < others) {  // row panel
+      ci = r, cj = skip64(o, r), ai = r, aj = r, bi = ci, bj = cj;
+    } else {           // column panel
+      ci = skip64(o - others, r), cj = r, ai = ci, aj = cj, bi = r, bj = r;
+    }
+  } else {
+    ci = skip64(o / others, r), cj = skip64(o % others, r), ai = ci, aj = r, bi = r, bj = cj;
+  }
+  __shared__ __attribute__((aligned(16))) double Bs[kT][kFwLdB];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  FAP_GLOBAL double *c = tile64(d, ci, cj) + (int64_t)(wave * kFwRows) * S + lane;
+  stage_tile64(Bs, tile64(d, bi, bj), S);
+  double acc[kFwRows];
+#pragma unroll
+  for (int q = 0; q < kFwRows; ++q) acc[q] = c[q * S];
+  __syncthreads();
+  const FAP_CONST double *a =
+      uniform_cptr(d.mat + ((int64_t)ai * kT + wave * kFwRows) * S + (int64_t)aj * kT);
+  tile_rows64(acc, a, S, Bs, lane);
+#pragma unroll
+  for (int q = 0; q < kFwRows; ++q) c[q * S] = acc[q];
+}
+
+
+// Phase 3 with the LDS-tiled product; tiles of one part back to back on one XCD.
+__global__ __launch_bounds__(256) void fw64_phase3_kernel(const Fw64Desc *__restrict__ descs, int r) {
+  const unsigned L = xcd_contiguous(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+  const int o = (int)(L % gridDim.x), di = (int)(L / gridDim.x);
+  const Fw64Desc d = descs[di];
+  const int others = d.nb - 1;
   if (r >= d.nb || o >= others * others) return;
   const int i = skip64(o / others, r), j = skip64(o % others, r);
   const int64_t S = (int64_t)d.nb * kT;
@@ -351,7 +432,8 @@ int launch_fw64(const Fw64Desc *d_descs, int nd, int max_nb, hipStream_t st) {
     FAP_CHECK_LAUNCH();
     const int others = max_nb - 1;
     if (others > 0) {
-      hipLaunchKernelGGL(fw64_phase2_kernel, dim3(2 * others, nd), dim3(256), 0, st, d_descs, r);
+      hipLaunchKernelGGL(fw64_tile_kernel<2>, dim3(2 * others, nd), dim3(64 * kFwWaves), 0, st,
+                         d_descs, r);
       FAP_CHECK_LAUNCH();
       hipLaunchKernelGGL(fw64_phase3_kernel, dim3(others * others, nd), dim3(256), 0, st, d_descs,
                          r);
