@@ -1261,6 +1261,44 @@ extern "C" int fap_hip_batch_run(fap_batch *b, void *stream) {
   return batch_launch(b, st);
 }
 
+namespace fap_hip {
+
+// A full solve of one graph through the batch stages: in-part distances, the boundary
+// graph closed by Floyd-Warshall (no limit on its size), boundary rows by one min-plus
+// pass, merge.  The single-graph pipeline takes this route for full solves its own
+// boundary-graph form does not cover (float weights; more than its boundary limit):
+// the alternative there is one label-correcting SSSP per boundary vertex, which on a
+// weighted road graph of 62k vertices took 1.8-3 s.  Timed as one span (stage `stage`).
+int solve_one_batched(const fap_graph_dev *g, const fap_part_desc *parts, int32_t nparts,
+                      void *d_out, int64_t ld_out, bool is_float, int stage, hipStream_t st) {
+  fap_batch_item it{};
+  it.g = *g;
+  it.parts = parts;
+  it.nparts = nparts;
+  it.is_float = is_float ? 1 : 0;
+  it.d_out = d_out;
+  it.ld_out = ld_out;
+  fap_batch *b = nullptr;
+  FAP_TRY(fap_hip_batch_create(&b, &it, 1, 0));
+  hipEvent_t e0, e1;
+  int s = timing_record(st, &e0);
+  if (s == FAP_OK) s = fap_hip_batch_run(b, st);
+  if (s == FAP_OK) s = timing_record(st, &e1);
+  if (s == FAP_OK) {
+    timing_add_span(stage, e0, e1);
+    timing_note_bgraph();
+  }
+  // The batch's scratch is freed by destroy: wait for its work first.
+  if (hipStreamSynchronize(st) != hipSuccess && s == FAP_OK) {
+    set_error("solve_one_batched: stream synchronize failed");
+    s = FAP_ERR_HIP;
+  }
+  fap_hip_batch_destroy(b);
+  return s;
+}
+
+}  // namespace fap_hip
+
 extern "C" int fap_hip_solve_batch(const fap_batch_item *items, int32_t nitems, int32_t mode,
                                    void *stream) {
   if (!initialized()) {

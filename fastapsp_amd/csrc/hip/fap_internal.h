@@ -76,6 +76,11 @@ int timing_finish(hipStream_t st);                  // record the end of the cal
 void timing_note_keyed(int merge_parts, int fw_parts);  // path solves: keyed forms taken
 void timing_note_bgraph();  // boundary rows came from the boundary graph
 
+// Full solve of one graph through the batch stages (batch.hip: boundary graph closed
+// by Floyd-Warshall, any boundary count), timed as one span of stage `stage`.
+int solve_one_batched(const fap_graph_dev *g, const fap_part_desc *parts, int32_t nparts,
+                      void *d_out, int64_t ld_out, bool is_float, int stage, hipStream_t st);
+
 // Boundary rows of parts covering a unit-weight graph with paired arcs from the
 // boundary graph (boundary.hip); FAP_ERR_INVALID, nothing launched, when the parts do
 // not qualify.

@@ -596,6 +596,18 @@ int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 using namespace fap_hip;
 
+// The parts tile [0, n) (every vertex in exactly one of them).
+static bool covers_graph(std::vector<fap_part_desc> act, int n) {
+  std::sort(act.begin(), act.end(),
+            [](const fap_part_desc &a, const fap_part_desc &b) { return a.offset < b.offset; });
+  int next = 0;
+  for (const fap_part_desc &q : act) {
+    if (q.offset != next) return false;
+    next = q.offset + q.num;
+  }
+  return next == n;
+}
+
 static int solve_parts_f32(const fap_graph_dev *g, const fap_part_desc *parts, int32_t nparts,
                            float *d_out, int64_t ld_out, int32_t mode, void *stream) {
   if (!initialized()) {
@@ -630,6 +642,12 @@ static int solve_parts_f32(const fap_graph_dev *g, const fap_part_desc *parts, i
   const int P = (int)act.size();
   FAP_TRY(timing_start(st));
   if (P == 0) return timing_finish(st);
+  // Parts that tile the graph, batched form: the boundary rows come from the boundary
+  // graph (the batch stages), not from one SSSP per boundary vertex.
+  if (batched && !(mode & FAP_MODE_FLAG_SSSP) && covers_graph(act, n)) {
+    FAP_TRY(solve_one_batched(g, act.data(), P, d_out, ld_out, true, 3, st));
+    return timing_finish(st);
+  }
 
   const int64_t ldr = align_up(n, 2);  // SSSP row stride (doubles)
   std::vector<int> S(P), nb(P), offsets(P), src_begin(P);

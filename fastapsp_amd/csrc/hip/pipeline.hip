@@ -238,6 +238,15 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
       next = x.first + x.second;
     }
     hier = cover && next == n && bg_total <= kBgMaxTotal;
+    // More boundary vertices than the single-graph boundary-graph form holds: the
+    // batch stages (boundary graph closed by Floyd-Warshall) instead of boundary SSSP.
+    if (cover && next == n && !hier && !append) {
+      g_events.used = 0;
+      g_spans.clear();
+      FAP_TRY(timing_record(st, &g_total[0]));
+      FAP_TRY(solve_one_batched(g, act.data(), (int)act.size(), d_out, ld_out, false, 3, st));
+      return timing_record(st, &g_total[1]);
+    }
   }
   // 16-bit Floyd-Warshall: on in-part matrices of unit-weight graphs every finite
   // entry is at most s - 1, below the cap for parts of at most kCap16 vertices, so the

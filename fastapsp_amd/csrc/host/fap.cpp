@@ -352,8 +352,8 @@ void fapGraph::solve_placed(const fap_graph_dev &g, const std::vector<fap_part_d
   auto solve_into = [&](int32_t *p) -> float {
     const int r = float_weights
                       ? fap_hip_solve_parts_f32(&g, parts.data(), (int32_t)parts.size(),
-                                                reinterpret_cast<float *>(p), ld_dist, mode,
-                                                nullptr)
+                                                reinterpret_cast<float *>(p), ld_dist,
+                                                solve_mode(), nullptr)
                       : fap_hip_solve_parts(&g, parts.data(), (int32_t)parts.size(), p, ld_dist,
                                             solve_mode(), nullptr);
     fap_stage_times t{};
@@ -413,7 +413,7 @@ int32_t fapGraph::solve(bool path_needed) {
   }
   if (float_weights)
     check(fap_hip_solve_parts_f32(&g, parts.data(), (int32_t)parts.size(),
-                                  reinterpret_cast<float *>(d_dist), ld_dist, mode, nullptr),
+                                  reinterpret_cast<float *>(d_dist), ld_dist, solve_mode(), nullptr),
           "solve");
   else
     check(fap_hip_solve_parts(&g, parts.data(), (int32_t)parts.size(), d_dist, ld_dist,

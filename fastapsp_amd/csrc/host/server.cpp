@@ -96,30 +96,32 @@ struct fap_server {
 namespace {
 
 // Scratch a request brings to a batch, an upper bound of what fap_hip_batch_create
-// allocates for it: each part's padded in-part matrix; when the parts tile the graph
-// (a full APSP) the padded boundary-graph matrix and, for float weights, the FP64
-// boundary rows of the whole graph, otherwise (a sub-graph) the part's FP64 boundary
-// rows for float weights; and the work tables (merge units of at least 32 rows x 256
-// columns at 16 bytes each, boundary-row units of at least 8 rows, part and graph
-// records).
+// allocates for it (rows of n entries padded to 256 columns): each part's padded
+// in-part matrix; when the parts tile the graph (a full APSP) the padded
+// boundary-graph matrix and the boundary rows of the whole graph, otherwise (a
+// sub-graph) each part's SSSP boundary rows; and the work tables (merge units of at
+// least 32 rows x 256 columns at 16 bytes each plus the per-XCD slot padding,
+// boundary-row units of at least 16 rows x 256 columns, part and graph records).
 int64_t scratch_estimate(const Request &r, bool is_float, int n) {
   const int elem = is_float ? 8 : 4;
-  const int64_t ldr = align_up(n, 2);
-  int64_t bytes = 0, tables = 4096, nb = 0, covered = 0;
+  const int64_t ldr = align_up(n, 256);
+  int64_t bytes = 0, tables = 4096, nb = 0, covered = 0, brow_units = 0;
   for (const fap_part_desc &p : r.parts) {
     const int64_t S = align_up(p.num, 64);
     bytes += align_up(S * S * elem, 256);
     nb += p.bdy;
     covered += p.num;
-    tables += ((int64_t)(p.num - p.bdy + 31) / 32) * ((n + 255) / 256) * 16 + 4 * p.part_id + 256;
+    const int64_t rbs = (p.num - p.bdy + 31) / 32;
+    tables += (rbs * ((n + 255) / 256) + 8 * rbs) * 16 + 4 * p.part_id + 512;
+    brow_units += (p.num + 255) / 256;
   }
   if (covered == n && nb > 0) {
     const int64_t SB = align_up(nb, 64);
-    bytes += align_up(SB * SB * elem, 256);
-    if (is_float) bytes += align_up(nb * ldr * 8, 256);
-    tables += (int64_t)r.parts.size() * ((nb + 7) / 8) * 8 + 12 * nb;
-  } else if (is_float) {
-    for (const fap_part_desc &p : r.parts) bytes += align_up((int64_t)p.bdy * ldr * 8, 256);
+    bytes += align_up(SB * SB * elem, 256) + align_up(nb * ldr * elem, 256);
+    tables += brow_units * ((nb + 15) / 16) * 16 + 12 * nb;
+  } else {
+    for (const fap_part_desc &p : r.parts) bytes += align_up((int64_t)p.bdy * ldr * elem, 256);
+    tables += 8 * nb;
   }
   return bytes + align_up(tables, 256) * 2 + (1 << 16);
 }
