@@ -54,6 +54,7 @@ constexpr int kBatchMaxS = kMaxPart;  // largest part
 constexpr int kBQ = 1024;         // SSSP frontier queue entries per wave
 constexpr int kSWaves = 4;        // waves (sources) per workgroup of the SSSP launch
 constexpr int kMCols = 256;       // merge: columns per unit (four per lane)
+constexpr int kWideN = 16384;     // FP64 merge: 8 x 8 register tiles from this graph size
 constexpr int kKc = 64;           // k per LDS chunk of the products
 
 // One graph of the batch.
@@ -64,6 +65,8 @@ struct BGraph {
   void *out;                     // n x ld_out result (int32 or float32)
   int64_t ld_out;
   void *R;                       // boundary rows of a full item (nb x ldr, +inf padded)
+  void *Rs;                      // the same rows with columns in sorted-position order
+  int perm;                      // 1: boundary rows via Rs and bperm_kernel (scattered parts)
   int64_t ldr;
   void *MB;                      // boundary-graph matrix (SB x SB), full items only
   int SB;
@@ -584,31 +587,74 @@ __device__ __forceinline__ void brows_unit(const BGraph &g, const BPart &q, int 
   m.ldb = q.S;
   m.col0 = cb * kMCols;
   m.colmax = q.S - 1;  // padded columns read +inf; clamped ones are never stored
+  FAP_GLOBAL T *Rs = gptr(static_cast<T *>(g.Rs));
   FAP_GLOBAL Out *out = gptr(static_cast<Out *>(g.out));
   FAP_GLOBAL T *R = gptr(static_cast<T *>(g.R));
   mp_unit<T, RB, kPacked>(m, As, tid, [&](const T (&acc)[RB][kPacked ? 2 : 4]) {
-    int v[4];
+    if (!g.perm) {  // straight to the vertex columns
+      int v[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int j = cb * kMCols + e * 64 + lane;
-      v[e] = j < q.s ? g.st2ed[q.off + j] : -1;
-      if (v[e] >= g.n) v[e] = -1;  // (a corrupt st2ed entry writes nothing)
+      for (int e = 0; e < 4; ++e) {
+        const int j = cb * kMCols + e * 64 + lane;
+        v[e] = j < q.s ? g.st2ed[q.off + j] : -1;
+        if (v[e] >= g.n) v[e] = -1;  // (a corrupt st2ed entry writes nothing)
+      }
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const int x = r0 + wave * RB + r;
+        if (x >= g.nb) continue;
+        FAP_GLOBAL Out *orow = out + brow[g.bbase + x] * g.ld_out;
+        FAP_GLOBAL T *rrow = R + (int64_t)x * g.ldr;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          if (v[e] < 0) continue;
+          const T val = mp_value<T, RB, kPacked>(acc, r, e);
+          orow[v[e]] = Dist<T>::out(val);
+          rrow[v[e]] = val;
+        }
+      }
+      return;
     }
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
       const int x = r0 + wave * RB + r;
       if (x >= g.nb) continue;
-      FAP_GLOBAL Out *orow = out + brow[g.bbase + x] * g.ld_out;
-      FAP_GLOBAL T *rrow = R + (int64_t)x * g.ldr;
+      FAP_GLOBAL T *rrow = Rs + (int64_t)x * g.ldr + q.off;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        if (v[e] < 0) continue;
-        const T val = mp_value<T, RB, kPacked>(acc, r, e);
-        orow[v[e]] = Dist<T>::out(val);
-        if (g.R) rrow[v[e]] = val;
+        const int j = cb * kMCols + e * 64 + lane;
+        if (j < q.s) rrow[j] = mp_value<T, RB, kPacked>(acc, r, e);
       }
     }
   });
+}
+
+// Boundary row x of graph g from sorted-position order to vertex order: the result row
+// brow[x] and the padded row R[x] take Rs[x][ed2st[v]] at column v (gathered reads of
+// one row, coalesced writes).  For graphs whose parts hold scattered vertex ids (kNN
+// point sets): writing the boundary-row product straight to vertex columns scattered
+// every store (6 of 33 ms of a 60k-vertex kNN graph's solve); on grid-numbered road
+// graphs the direct writes are cheaper (the extra pass cost 1.2 ms of the 512-graph
+// batch).
+template <typename T>
+__global__ __launch_bounds__(256) void bperm_kernel(const BGraph *__restrict__ graphs,
+                                                    const int *__restrict__ gbg, int ngb,
+                                                    const int64_t *__restrict__ brow) {
+  typedef typename Dist<T>::Out Out;
+  const int row = blockIdx.x;
+  const int gi = gbg[upper_index(ngb, row, [&](int k) { return graphs[gbg[k]].bbase; })];
+  const BGraph g = graphs[gi];
+  const int x = row - g.bbase;
+  if (!g.perm || x < 0 || x >= g.nb) return;
+  const FAP_GLOBAL T *src = gptr(static_cast<const T *>(g.Rs)) + (int64_t)x * g.ldr;
+  FAP_GLOBAL T *rrow = gptr(static_cast<T *>(g.R)) + (int64_t)x * g.ldr;
+  FAP_GLOBAL Out *orow = gptr(static_cast<Out *>(g.out)) + brow[g.bbase + x] * g.ld_out;
+  for (int v = threadIdx.x; v < g.n; v += 256) {
+    const int pos = g.ed2st[v];
+    const T val = (pos >= 0 && pos < g.n) ? src[pos] : Dist<T>::inf();
+    rrow[v] = val;
+    orow[v] = Dist<T>::out(val);
+  }
 }
 
 template <typename T, int RB>
@@ -630,37 +676,19 @@ __global__ __launch_bounds__(256) void brows_kernel(const BGraph *__restrict__ g
   brows_unit<T, RB, false>(g, q, un.r0, un.cb, brow, As, threadIdx.x);
 }
 
-// A unit is (part, row block of 4 * RB inner rows, column block of 256 columns).  The
-// units arrive in launch slots: slot 8 * i + x holds the i-th unit of XCD x (or none),
+// Units arrive in launch slots: slot 8 * i + x holds the i-th unit of XCD x (or none),
 // where all row blocks of one (part, column block) are on one XCD, so its boundary-row
 // panel is read from HBM once and then from that XCD's L2 (with units pulled in global
 // order every XCD fetched every panel: 7x the algorithmic reads).  Persistent grid
 // (`counters`, one per XCD): workgroup b starts on XCD b % 8, takes the units of its own
 // list and then helps the others; plain grid (no counters): workgroup L runs slot L.
-template <typename T, int RB>
-__global__ __launch_bounds__(256, 4) void bmerge_kernel(const BGraph *__restrict__ graphs,
-                                                     const BPart *__restrict__ parts,
-                                                     const BMerge *__restrict__ slots,
-                                                     int nslots, int *counters,
-                                                     const int *__restrict__ lens) {
-  __shared__ __attribute__((aligned(16))) T As[kKc][kMWaves * RB + kAPad<T>];
-  __shared__ int unit_s;
+template <typename Run>
+__device__ __forceinline__ void for_each_unit(const BMerge *__restrict__ slots, int nslots,
+                                              int *counters, const int *__restrict__ lens,
+                                              int *unit_s, Run &&run) {
   const int tid = threadIdx.x;
-  auto run = [&](int u) {
-    const BMerge m = slots[u];
-    if (m.p < 0) return;  // past the end of its XCD's list
-    const BPart p = parts[m.p];
-    const BGraph g = graphs[p.g];
-    if constexpr (sizeof(T) == 4) {
-      if (g.packed) {
-        merge_unit<T, RB, true>(g, p, m.rb, m.cb, As, tid);
-        return;
-      }
-    }
-    merge_unit<T, RB, false>(g, p, m.rb, m.cb, As, tid);
-  };
   if (!counters) {  // plain grid: one slot per workgroup
-    if ((int)blockIdx.x < nslots) run(blockIdx.x);
+    if ((int)blockIdx.x < nslots && slots[blockIdx.x].p >= 0) run(slots[blockIdx.x]);
     return;
   }
   const int home = blockIdx.x & 7;
@@ -674,16 +702,130 @@ __global__ __launch_bounds__(256, 4) void bmerge_kernel(const BGraph *__restrict
         const int i = atomicAdd(counters + x, 1);
         if (i < lens[x]) u = 8 * i + x;
       }
-      unit_s = u;
+      *unit_s = u;
     }
     __syncthreads();
     // Uniform by construction; readfirstlane lets the compiler keep the unit's
     // descriptors and pointers in scalar registers (scalar loads, scalar address math).
-    const int u = __builtin_amdgcn_readfirstlane(unit_s);
+    const int u = __builtin_amdgcn_readfirstlane(*unit_s);
     __syncthreads();  // every thread has read unit_s before thread 0 takes the next one
     if (u < 0) break;  // uniform: every list is drained
-    run(u);
+    if (slots[u].p >= 0) run(slots[u]);
   }
+}
+
+// int32 merge: a unit is (part, row block of 4 * RB inner rows, column block of 256
+// columns), on the min-plus unit above.
+template <typename T, int RB>
+__global__ __launch_bounds__(256, 4) void bmerge_kernel(const BGraph *__restrict__ graphs,
+                                                     const BPart *__restrict__ parts,
+                                                     const BMerge *__restrict__ slots,
+                                                     int nslots, int *counters,
+                                                     const int *__restrict__ lens) {
+  __shared__ __attribute__((aligned(16))) T As[kKc][kMWaves * RB + kAPad<T>];
+  __shared__ int unit_s;
+  const int tid = threadIdx.x;
+  for_each_unit(slots, nslots, counters, lens, &unit_s, [&](const BMerge &m) {
+    const BPart p = parts[m.p];
+    const BGraph g = graphs[p.g];
+    if (g.packed) {
+      merge_unit<T, RB, true>(g, p, m.rb, m.cb, As, tid);
+      return;
+    }
+    merge_unit<T, RB, false>(g, p, m.rb, m.cb, As, tid);
+  });
+}
+
+// FP64 merge: a unit is (part, row block of 64 inner rows, column block of 256
+// columns); thread t holds an 8 x 8 register tile, rows (t / 32) * 8 + {0..7} and
+// columns 4 (t % 32) + {0..3}, 128 + 4 (t % 32) + {0..3}; A (64 x 16) and B (16 x 256)
+// staged in LDS per k chunk of 16.  Per k: 16 LDS values for 64 steps.  The first
+// form (8 rows x 4 columns per lane, B read from L1 per wave) ran at half the rate.
+constexpr int kM64Rows = 64, kM64Kc = 16;
+
+__device__ __forceinline__ void merge_unit64(const BGraph &g, const BPart &p, int rb, int cb,
+                                             double (*As)[kM64Rows], double (*Bs)[kMCols],
+                                             int tid) {
+  const int inner = p.s - p.b, row0 = rb * kM64Rows, col0 = cb * kMCols;
+  const FAP_GLOBAL double *D = gptr(static_cast<const double *>(p.D));
+  const FAP_GLOBAL double *B = gptr(static_cast<const double *>(p.Brows));
+  const int tx = tid & 31, ty = tid >> 5;
+  double acc[8][8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[r][c] = __builtin_inf();
+  for (int k0 = 0; k0 < p.b; k0 += kM64Kc) {
+    const int kc = min(kM64Kc, p.b - k0);
+    {  // As[kk][i] = A[row0 + i][k0 + kk], four per thread
+      const int i = tid & 63, kq = (tid >> 6) * 4;
+      const bool row_ok = row0 + i < inner;
+      const FAP_GLOBAL double *arow = D + (int64_t)(p.b + row0 + i) * p.S + k0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        As[kq + q][i] = (row_ok && kq + q < kc) ? arow[kq + q] : __builtin_inf();
+    }
+    // Bs[kk][c] = B[k0 + kk][col0 + c]: the boundary rows are +inf-padded to whole
+    // column blocks; rows past kc read as +inf.
+#pragma unroll
+    for (int q = 0; q < kM64Kc; ++q)
+      Bs[q][tid] = q < kc ? B[(int64_t)(k0 + q) * p.ldb + col0 + tid] : __builtin_inf();
+    __syncthreads();
+    for (int k = 0; k < kc; ++k) {
+      double av[8], bv[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) av[r] = As[k][ty * 8 + r];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) bv[4 * h + e] = Bs[k][h * 128 + tx * 4 + e];
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[r][c] = fmin(acc[r][c], av[r] + bv[c]);
+    }
+    __syncthreads();
+  }
+  int col[8], own[8];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int c = 4 * h + e;
+      col[c] = col0 + h * 128 + tx * 4 + e;
+      own[c] = col[c] < g.n && g.graph_id[col[c]] == p.part_id ? g.ed2st[col[c]] - p.off : -1;
+      if (own[c] >= p.s) own[c] = -1;
+    }
+  FAP_GLOBAL float *out = gptr(static_cast<float *>(g.out));
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int i = row0 + ty * 8 + r;
+    if (i >= inner) break;
+    const FAP_GLOBAL double *drow = D + (int64_t)(p.b + i) * p.S;
+    FAP_GLOBAL float *orow = out + (p.out_row + p.b + i) * g.ld_out;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      if (col[c] >= g.n) continue;
+      double v = acc[r][c];
+      if (own[c] >= 0) v = fmin(v, drow[own[c]]);
+      orow[col[c]] = Dist<double>::out(v);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void bmerge64_kernel(const BGraph *__restrict__ graphs,
+                                                       const BPart *__restrict__ parts,
+                                                       const BMerge *__restrict__ slots,
+                                                       int nslots, int *counters,
+                                                       const int *__restrict__ lens) {
+  __shared__ __attribute__((aligned(16))) double As[kM64Kc][kM64Rows];
+  __shared__ __attribute__((aligned(16))) double Bs[kM64Kc][kMCols];
+  __shared__ int unit_s;
+  const int tid = threadIdx.x;
+  for_each_unit(slots, nslots, counters, lens, &unit_s, [&](const BMerge &m) {
+    const BPart p = parts[m.p];
+    merge_unit64(graphs[p.g], p, m.rb, m.cb, As, Bs, tid);
+  });
 }
 
 __global__ void fill_f64_kernel(double *__restrict__ p, int64_t n, double v) {
@@ -750,9 +892,16 @@ struct fap_batch_kind {
   int ngb = 0, gb_rows = 0, gb_max_nb = 0, nunits5 = 0, nmerge = 0, merge_units = 0, nsrc = 0;
   int rows6 = 16;  // merge rows per wave (FP64: 8, register budget)
   int rb5 = 8;     // boundary-row unit rows per wave (4 * rb5 boundary-graph rows per unit)
+  int nbrows = 0;  // boundary rows of full items (one workgroup each in bperm_kernel)
   int mlen[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // merge units per XCD list
   int *d_mlen = nullptr;
+  // FP64 parts of graphs with at least kWideN vertices: the 8 x 8 register-tile merge.
+  BMerge *merge_w = nullptr;
+  int merge_units_w = 0;
+  int mlen_w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int *d_mlen_w = nullptr;
   int grid = 0;    // persistent merge workgroups
+  int grid_w = 0;  // persistent workgroups of the 8 x 8 FP64 merge
   int nfw = 0;     // parts whose in-part matrix comes from Floyd-Warshall
   // In-part BFS descriptors: parts of at most 1024 vertices first (n2 of them, g2
   // source groups), then those up to kInpartBfsMaxS (n4, g4).
@@ -827,6 +976,9 @@ int launch_kind(const fap_batch *b, const fap_batch_kind &q, int *counter, hipSt
     else if (q.rb5 == 16) hipLaunchKernelGGL((brows_kernel<T, 16>), g5, t5, 0, st, q.graphs, q.parts, q.units5, q.brow);
     else hipLaunchKernelGGL((brows_kernel<T, 8>), g5, t5, 0, st, q.graphs, q.parts, q.units5, q.brow);
     FAP_CHECK_LAUNCH();
+    hipLaunchKernelGGL(bperm_kernel<T>, dim3(q.nbrows), dim3(256), 0, st, q.graphs, q.gbg, q.ngb,
+                       q.brow);
+    FAP_CHECK_LAUNCH();
   }
   if (q.nsrc > 0) {
     hipLaunchKernelGGL(bconv_kernel<T>, dim3(q.nsrc), dim3(256), 0, st, q.graphs, q.parts, q.srcs);
@@ -845,14 +997,21 @@ int launch_kind(const fap_batch *b, const fap_batch_kind &q, int *counter, hipSt
                          q.merge_units, c, q.d_mlen);
     FAP_CHECK_LAUNCH();
   }
+  if (q.merge_units_w > 0) {
+    const dim3 gm(b->unit_grid ? q.merge_units_w : std::min(q.grid_w, q.merge_units_w)), tm(256);
+    hipLaunchKernelGGL(bmerge64_kernel, gm, tm, 0, st, q.graphs, q.parts, q.merge_w,
+                       q.merge_units_w, b->unit_grid ? nullptr : counter + 16, q.d_mlen_w);
+    FAP_CHECK_LAUNCH();
+  }
   return FAP_OK;
 }
 
-// The two kinds share nothing but the counter array (eight counters each), so they run
+// The two kinds share nothing but the counter array (eight counters per merge launch:
+// int32 at 0, FP64 at 8 and 24), so they run
 // side by side: the Floyd-Warshall rounds of one (short, latency-bound launches) fill
 // in around the other's long launches.
 int batch_launch(fap_batch *b, hipStream_t st) {
-  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, st, b->d_counter, 16);
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, st, b->d_counter, 32);
   FAP_CHECK_LAUNCH();
   if (!b->side) {
     FAP_TRY(launch_kind<int>(b, b->kind[0], b->d_counter, st));
@@ -871,27 +1030,26 @@ int batch_launch(fap_batch *b, hipStream_t st) {
 int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap_batch_item *> &items,
               bool fp64) {
   const int elem = fp64 ? 8 : 4;
-  q.rows6 = fp64 ? 8 : b->rows6;
+  q.rows6 = fp64 ? 8 : b->rows6;  // rows per unit: 4 * rows6 (FP64: register budget)
   // Boundary-source batch width (8, 16 or 32 sources) -> rows per wave of a unit of
   // 4 waves (4, 8 or 16; FP64 at most 8, register budget).
   q.rb5 = std::min(b->rows5 / 2, fp64 ? 8 : 16);
   // Persistent merge grid: as many workgroups as the device keeps resident (its
   // register and LDS use), unless the caller fixed the workgroups per CU.
-  {
+  auto grid_of = [&](const void *fn) {
     int per_cu = b->per_cu;
-    if (per_cu <= 0) {
-      const void *fn = fp64 ? (q.rows6 == 8 ? (const void *)bmerge_kernel<double, 8>
-                                            : (const void *)bmerge_kernel<double, 16>)
-                            : (q.rows6 == 8 ? (const void *)bmerge_kernel<int, 8>
-                                            : (const void *)bmerge_kernel<int, 16>);
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess ||
-          per_cu <= 0) {
-        (void)hipGetLastError();
-        per_cu = 2;
-      }
+    if (per_cu <= 0 &&
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess ||
+         per_cu <= 0)) {
+      (void)hipGetLastError();
+      per_cu = 2;
     }
-    q.grid = per_cu * std::max(device_cus(), 1);
-  }
+    return per_cu * std::max(device_cus(), 1);
+  };
+  q.grid = grid_of(fp64 ? (const void *)bmerge_kernel<double, 8>
+                        : (q.rows6 == 8 ? (const void *)bmerge_kernel<int, 8>
+                                        : (const void *)bmerge_kernel<int, 16>));
+  if (fp64) q.grid_w = grid_of((const void *)bmerge64_kernel);
   std::vector<BGraph> graphs;
   std::vector<BPart> parts;
   std::vector<int> gbg, bpart, pid2idx;
@@ -901,7 +1059,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   std::vector<double> pcost;
   // Scratch offsets (bytes) of every part matrix, boundary-graph matrix and FP64 row
   // block; resolved to pointers once the allocation exists.
-  std::vector<int64_t> d_at, mb_at, r_at, brows_at;
+  std::vector<int64_t> d_at, mb_at, r_at, rs_at, brows_at;
   int64_t scratch = 0;
   auto take = [&](int64_t bytes) {
     const int64_t o = scratch;
@@ -993,6 +1151,21 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
       parts.push_back(P);
     }
     G.nb = G.full ? nb : 0;
+    // Vertex-id locality of the parts: 64-byte lines (16 ids) a part's vertices touch,
+    // over all parts, per vertex; scattered numbering takes the sorted-order rows.
+    if (G.nb > 0) {
+      std::vector<int> st(d.n);
+      FAP_CHECK_HIP(hipMemcpy(st.data(), d.d_st2ed, 4LL * d.n, hipMemcpyDeviceToHost));
+      int64_t lines = 0;
+      std::vector<int> blk;
+      for (const fap_part_desc *pd : ps) {
+        blk.clear();
+        for (int j = 0; j < pd->num; ++j) blk.push_back(st[pd->offset + j] >> 4);
+        std::sort(blk.begin(), blk.end());
+        lines += std::unique(blk.begin(), blk.end()) - blk.begin();
+      }
+      G.perm = lines * 4 > d.n;
+    }
     if (G.nb > 0) {
       G.SB = (int)align_up(G.nb, kBT);
       G.gb_row_begin = gb_rows;
@@ -1001,12 +1174,14 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
       gbg.push_back(gi);
       mb_at.push_back(take((int64_t)G.SB * G.SB * elem));
       r_at.push_back(take((int64_t)G.nb * G.ldr * elem));
+      rs_at.push_back(G.perm ? take((int64_t)G.nb * G.ldr * elem) : -1);
       for (int pi = (int)parts.size() - (int)ps.size(); pi < (int)parts.size(); ++pi)
         for (int r0 = 0; r0 < G.nb; r0 += kMWaves * q.rb5)
           for (int cb = 0; cb * kMCols < parts[pi].s; ++cb) units5.push_back(BRowUnit{pi, r0, cb, 0});
     } else {
       mb_at.push_back(-1);
       r_at.push_back(-1);
+      rs_at.push_back(-1);
     }
     graphs.push_back(G);
   }
@@ -1016,23 +1191,36 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
     if (parts[k].s > parts[k].b) order.push_back(k);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return pcost[x] > pcost[y]; });
   // Units of a part: the row blocks of one column block consecutive (its boundary-row
-  // panel stays in L2 while they run).
-  std::vector<BMerge> munits;
-  std::vector<int> mgb{0};
-  std::vector<double> mcost;
-  for (int k : order) {
-    const BPart &P = parts[k];
-    const int rbs = (P.s - P.b + kMWaves * q.rows6 - 1) / (kMWaves * q.rows6);
-    const int cbs = (graphs[P.g].n + kMCols - 1) / kMCols;
-    FAP_REQUIRE((int64_t)munits.size() + (int64_t)rbs * cbs < (1LL << 27),
-                "fap_hip_batch_create: too many merge units");
-    for (int cb = 0; cb < cbs; ++cb) {
-      for (int rb = 0; rb < rbs; ++rb) munits.push_back(BMerge{k, rb, cb, 0});
-      mgb.push_back((int)munits.size());
-      mcost.push_back((double)rbs * kMWaves * q.rows6 * (P.b + 8));
+  // panel stays in L2 while they run).  FP64 parts of graphs with at least kWideN
+  // vertices take the 8 x 8 register-tile merge (64-row units), the others the 8 x 4
+  // form (32-row units): measured on MI355X, the 512-graph batch (n <= 8463) took 26.2
+  // against 31.2 ms with the 8 x 4 form, a 60k-vertex kNN graph 12 against 29 ms with
+  // the 8 x 8 form.
+  auto build_units = [&](bool wide, int (&len)[8]) -> std::vector<BMerge> {
+    const int rows = wide ? kM64Rows : kMWaves * q.rows6;
+    std::vector<BMerge> munits;
+    std::vector<int> mgb{0};
+    std::vector<double> mcost;
+    for (int k : order) {
+      const BPart &P = parts[k];
+      if (fp64 && (graphs[P.g].n >= kWideN) != wide) continue;
+      const int rbs = (P.s - P.b + rows - 1) / rows;
+      const int cbs = (graphs[P.g].n + kMCols - 1) / kMCols;
+      for (int cb = 0; cb < cbs; ++cb) {
+        for (int rb = 0; rb < rbs; ++rb) munits.push_back(BMerge{k, rb, cb, 0});
+        mgb.push_back((int)munits.size());
+        mcost.push_back((double)rbs * rows * (P.b + 8));
+      }
     }
-  }
-  const std::vector<BMerge> merge = xcd_slots(munits, mgb, mcost, BMerge{-1, 0, 0, 0}, q.mlen);
+    return xcd_slots(munits, mgb, mcost, BMerge{-1, 0, 0, 0}, len);
+  };
+  int64_t nunits_all = 0;
+  for (int k : order)
+    nunits_all += (int64_t)((parts[k].s - parts[k].b + 31) / 32) *
+                  ((graphs[parts[k].g].n + kMCols - 1) / kMCols);
+  FAP_REQUIRE(nunits_all < (1LL << 27), "fap_hip_batch_create: too many merge units");
+  const std::vector<BMerge> merge = build_units(false, q.mlen);
+  const std::vector<BMerge> merge_w = fp64 ? build_units(true, q.mlen_w) : std::vector<BMerge>();
   const int64_t units = (int64_t)merge.size();
   FAP_REQUIRE(units5.size() < (1u << 31) && srcs.size() < (1u << 31),
               "fap_hip_batch_create: batch too large");
@@ -1044,7 +1232,8 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   const int64_t o_pid = table(4 * pid2idx.size()), o_brow = table(8 * brow.size());
   const int64_t o_u5 = table(sizeof(BRowUnit) * units5.size());
   const int64_t o_merge = table(sizeof(BMerge) * merge.size());
-  const int64_t o_mlen = table(sizeof(q.mlen));
+  const int64_t o_merge_w = table(sizeof(BMerge) * merge_w.size());
+  const int64_t o_mlen = table(sizeof(q.mlen)), o_mlen_w = table(sizeof(q.mlen_w));
   const int64_t o_src = table(sizeof(BSrc) * srcs.size());
   const int64_t o_fw = table((fp64 ? sizeof(Fw64Desc) : sizeof(FwDesc)) * parts.size());
   const int64_t o_ip = table(sizeof(InpartDesc) * parts.size());
@@ -1074,6 +1263,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
     BGraph &G = graphs[gi];
     if (mb_at[gi] >= 0) G.MB = base + mb_at[gi];
     if (r_at[gi] >= 0) G.R = base + r_at[gi];
+    if (rs_at[gi] >= 0) G.Rs = base + rs_at[gi];
   }
   for (BPart &P : parts) {
     const BGraph &G = graphs[P.g];
@@ -1091,7 +1281,9 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   put(o_brow, brow.data(), 8 * brow.size());
   put(o_u5, units5.data(), sizeof(BRowUnit) * units5.size());
   put(o_merge, merge.data(), sizeof(BMerge) * merge.size());
+  put(o_merge_w, merge_w.data(), sizeof(BMerge) * merge_w.size());
   put(o_mlen, q.mlen, sizeof(q.mlen));
+  put(o_mlen_w, q.mlen_w, sizeof(q.mlen_w));
   put(o_src, srcs.data(), sizeof(BSrc) * srcs.size());
   if (fp64) {
     std::vector<Fw64Desc> fd, gd;
@@ -1139,6 +1331,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   q.gb_rows = gb_rows;
   q.gb_max_nb = gb_max_nb;
   q.nunits5 = (int)units5.size();
+  q.nbrows = (int)bpart.size();
   q.nmerge = (int)merge.size();
   q.merge_units = (int)units;
   q.nsrc = (int)srcs.size();
@@ -1151,6 +1344,9 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   q.units5 = reinterpret_cast<BRowUnit *>(base + o_u5);
   q.merge = reinterpret_cast<BMerge *>(base + o_merge);
   q.d_mlen = reinterpret_cast<int *>(base + o_mlen);
+  q.merge_w = reinterpret_cast<BMerge *>(base + o_merge_w);
+  q.merge_units_w = (int)merge_w.size();
+  q.d_mlen_w = reinterpret_cast<int *>(base + o_mlen_w);
   q.srcs = reinterpret_cast<BSrc *>(base + o_src);
   q.fw_descs = base + o_fw;
   q.ip = reinterpret_cast<InpartDesc *>(base + o_ip);

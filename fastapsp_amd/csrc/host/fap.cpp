@@ -156,6 +156,38 @@ void fapGraph::setPartition(const std::vector<int> &ids) {
 // which takes the lightest; the boundary-graph form derives the boundary rows from the
 // built in-part matrices, so for such graphs only the SSSP form reproduces the
 // reference.
+// BFS from the first boundary vertex of up to four parts: true when they reach on
+// average fewer than n / 8 vertices.  Measured on MI355X: a directed 3-NN graph of 60k
+// vertices (reach 7-665) solved in 21 ms with one SSSP per boundary vertex and 29-31
+// ms with the boundary graph closed by Floyd-Warshall; an undirected weighted road
+// graph of 62k vertices (reach n, 250-450 levels) in 1.8 s against 67 ms.
+static bool small_boundary_reach(int32_t n, const std::vector<int32_t> &ro,
+                                 const std::vector<int32_t> &col, const std::vector<int32_t> &st2ed,
+                                 const std::vector<int32_t> &off, const std::vector<int32_t> &bdy,
+                                 int32_t K) {
+  std::vector<int32_t> seen(n, -1), queue;
+  int64_t reached = 0;
+  int samples = 0;
+  for (int32_t t = 0; t < 4 && K > 0; ++t) {
+    const int32_t p = 1 + (int32_t)((int64_t)t * K / 4);
+    if (p > K || bdy[p] == 0) continue;
+    const int32_t s = st2ed[off[p]];
+    queue.assign(1, s);
+    seen[s] = t;
+    for (size_t h = 0; h < queue.size(); ++h) {
+      const int32_t u = queue[h];
+      for (int32_t e = ro[u]; e < ro[u + 1]; ++e)
+        if (seen[col[e]] != t) {
+          seen[col[e]] = t;
+          queue.push_back(col[e]);
+        }
+    }
+    reached += (int64_t)queue.size();
+    ++samples;
+  }
+  return samples > 0 && reached * 8 < (int64_t)samples * n;
+}
+
 static bool later_arc_heavier(int32_t n, const std::vector<int32_t> &ro,
                               const std::vector<int32_t> &col, const std::vector<float> &w,
                               const std::vector<int32_t> &gid) {
@@ -193,6 +225,8 @@ void fapGraph::preCondition() {
   for (int p = 1; p <= K; ++p) max_subgraph_vertexs = std::max(max_subgraph_vertexs, C_BlockVer_num[p]);
   is_split = num_vertexs > max_vertexs_num_limit;
   boundary_by_sssp = later_arc_heavier(num_vertexs, row_offset, col_val, weight, graph_id);
+  sssp_cheap = small_boundary_reach(num_vertexs, row_offset, col_val, st2ed, C_BlockVer_offset,
+                                    C_BlockBdy_num, K);
   release();  // device copies of an earlier layout are stale
   preconditioned = true;
 }

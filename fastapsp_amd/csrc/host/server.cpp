@@ -98,7 +98,8 @@ namespace {
 // Scratch a request brings to a batch, an upper bound of what fap_hip_batch_create
 // allocates for it (rows of n entries padded to 256 columns): each part's padded
 // in-part matrix; when the parts tile the graph (a full APSP) the padded
-// boundary-graph matrix and the boundary rows of the whole graph, otherwise (a
+// boundary-graph matrix and the boundary rows of the whole graph (in vertex and in
+// sorted-position order), otherwise (a
 // sub-graph) each part's SSSP boundary rows; and the work tables (merge units of at
 // least 32 rows x 256 columns at 16 bytes each plus the per-XCD slot padding,
 // boundary-row units of at least 16 rows x 256 columns, part and graph records).
@@ -117,7 +118,7 @@ int64_t scratch_estimate(const Request &r, bool is_float, int n) {
   }
   if (covered == n && nb > 0) {
     const int64_t SB = align_up(nb, 64);
-    bytes += align_up(SB * SB * elem, 256) + align_up(nb * ldr * elem, 256);
+    bytes += align_up(SB * SB * elem, 256) + 2 * align_up(nb * ldr * elem, 256);
     tables += brow_units * ((nb + 15) / 16) * 16 + 12 * nb;
   } else {
     for (const fap_part_desc &p : r.parts) bytes += align_up((int64_t)p.bdy * ldr * elem, 256);
@@ -300,7 +301,7 @@ void pipeline_lane(fap_server *s) {
     const GraphEntry &ge = s->graphs[r.graph];
     const fap_graph_dev dev = ge.dev;
     const bool is_float = ge.is_float;
-    const bool by_sssp = ge.g->boundaryBySssp();
+    const bool by_sssp = (ge.g->solveModeFlags() & FAP_MODE_FLAG_SSSP) != 0;
     std::vector<fap_part_desc> parts = r.parts;
     const int64_t bytes = r.bytes, ld = r.ld;
     lk.unlock();

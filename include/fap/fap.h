@@ -108,7 +108,13 @@ class fapGraph {
   // Parallel in-part arcs whose later arc is heavier: boundary rows by SSSP (the
   // reference's semantics for such graphs); set by preCondition.
   bool boundary_by_sssp = false;
-  int32_t solve_mode() const { return mode | (boundary_by_sssp ? FAP_MODE_FLAG_SSSP : 0); }
+  // Float weights and boundary sources that reach few vertices (sampled by BFS in
+  // preCondition; e.g. directed k-NN graphs): one SSSP per boundary vertex is then
+  // cheaper than closing the boundary graph, which full float solves do otherwise.
+  bool sssp_cheap = false;
+  int32_t solve_mode() const {
+    return mode | (boundary_by_sssp || (float_weights && sssp_cheap) ? FAP_MODE_FLAG_SSSP : 0);
+  }
   void solve_placed(const fap_graph_dev &g, const std::vector<fap_part_desc> &parts);
   void copy_rows(const int32_t *base, int64_t nrows, int64_t row0, int64_t rows, void *out) const;
   void require_int(const char *what) const;
@@ -146,6 +152,9 @@ class fapGraph {
   // Parallel arcs inside a part whose later arc is heavier (set by preCondition): the
   // boundary rows then come from SSSP, as in the reference.
   bool boundaryBySssp() const { return boundary_by_sssp; }
+  // Mode flags of this graph's full solves (FAP_MODE_FLAG_SSSP when the boundary rows
+  // come from SSSP: boundaryBySssp(), or float weights with a small sampled reach).
+  int32_t solveModeFlags() const { return solve_mode() & ~0xFF; }
   std::vector<int> getGraphId();
 
   // Execution form of solve(): FAP_MODE_BATCHED (default), FAP_MODE_OVERLAP or
