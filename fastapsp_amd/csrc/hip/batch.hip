@@ -67,6 +67,8 @@ struct BGraph {
   void *R;                       // boundary rows of a full item (nb x ldr, +inf padded)
   void *Rs;                      // the same rows with columns in sorted-position order
   int perm;                      // 1: boundary rows via Rs and bperm_kernel (scattered parts)
+  int rlo, rcnt;                 // boundary-graph rows produced (all, or one part's)
+  int pbase;                     // first of them in the row numbering of bperm_kernel
   int64_t ldr;
   void *MB;                      // boundary-graph matrix (SB x SB), full items only
   int SB;
@@ -581,7 +583,8 @@ __device__ __forceinline__ void brows_unit(const BGraph &g, const BPart &q, int 
   MpArgs<T> m;
   m.A = gptr(static_cast<const T *>(g.MB)) + (int64_t)r0 * g.SB + q.bs;
   m.lda = g.SB;
-  m.rows = g.nb - r0;
+  const int rend = g.rlo + g.rcnt;  // rows r0 .. rend - 1 are produced
+  m.rows = rend - r0;
   m.kdim = q.b;
   m.B = gptr(static_cast<const T *>(q.D));
   m.ldb = q.S;
@@ -602,9 +605,9 @@ __device__ __forceinline__ void brows_unit(const BGraph &g, const BPart &q, int 
 #pragma unroll
       for (int r = 0; r < RB; ++r) {
         const int x = r0 + wave * RB + r;
-        if (x >= g.nb) continue;
+        if (x >= rend) continue;
         FAP_GLOBAL Out *orow = out + brow[g.bbase + x] * g.ld_out;
-        FAP_GLOBAL T *rrow = R + (int64_t)x * g.ldr;
+        FAP_GLOBAL T *rrow = R + (int64_t)(x - g.rlo) * g.ldr;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           if (v[e] < 0) continue;
@@ -618,8 +621,8 @@ __device__ __forceinline__ void brows_unit(const BGraph &g, const BPart &q, int 
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
       const int x = r0 + wave * RB + r;
-      if (x >= g.nb) continue;
-      FAP_GLOBAL T *rrow = Rs + (int64_t)x * g.ldr + q.off;
+      if (x >= rend) continue;
+      FAP_GLOBAL T *rrow = Rs + (int64_t)(x - g.rlo) * g.ldr + q.off;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int j = cb * kMCols + e * 64 + lane;
@@ -642,12 +645,13 @@ __global__ __launch_bounds__(256) void bperm_kernel(const BGraph *__restrict__ g
                                                     const int64_t *__restrict__ brow) {
   typedef typename Dist<T>::Out Out;
   const int row = blockIdx.x;
-  const int gi = gbg[upper_index(ngb, row, [&](int k) { return graphs[gbg[k]].bbase; })];
+  const int gi = gbg[upper_index(ngb, row, [&](int k) { return graphs[gbg[k]].pbase; })];
   const BGraph g = graphs[gi];
-  const int x = row - g.bbase;
-  if (!g.perm || x < 0 || x >= g.nb) return;
-  const FAP_GLOBAL T *src = gptr(static_cast<const T *>(g.Rs)) + (int64_t)x * g.ldr;
-  FAP_GLOBAL T *rrow = gptr(static_cast<T *>(g.R)) + (int64_t)x * g.ldr;
+  const int xr = row - g.pbase;  // row of R / Rs
+  if (!g.perm || xr < 0 || xr >= g.rcnt) return;
+  const int x = g.rlo + xr;
+  const FAP_GLOBAL T *src = gptr(static_cast<const T *>(g.Rs)) + (int64_t)xr * g.ldr;
+  FAP_GLOBAL T *rrow = gptr(static_cast<T *>(g.R)) + (int64_t)xr * g.ldr;
   FAP_GLOBAL Out *orow = gptr(static_cast<Out *>(g.out)) + brow[g.bbase + x] * g.ld_out;
   for (int v = threadIdx.x; v < g.n; v += 256) {
     const int pos = g.ed2st[v];
@@ -1060,6 +1064,8 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   // Scratch offsets (bytes) of every part matrix, boundary-graph matrix and FP64 row
   // block; resolved to pointers once the allocation exists.
   std::vector<int64_t> d_at, mb_at, r_at, rs_at, brows_at;
+  std::vector<char> emitted;  // per part: its rows are produced (merge units)
+  int nbrows = 0;             // boundary-graph rows produced over all graphs
   int64_t scratch = 0;
   auto take = [&](int64_t bytes) {
     const int64_t o = scratch;
@@ -1151,6 +1157,21 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
       parts.push_back(P);
     }
     G.nb = G.full ? nb : 0;
+    // Boundary-graph rows produced: all of them, or those of the one part asked for.
+    G.rlo = 0;
+    G.rcnt = G.nb;
+    const int pfirst = (int)parts.size() - (int)ps.size();
+    emitted.resize(parts.size(), 1);
+    if (it->only_part > 0) {
+      FAP_REQUIRE(G.full, "fap_hip_batch_create: only_part needs parts that tile the graph");
+      FAP_REQUIRE(it->only_part <= max_pid && pid2idx[G.pid_base + it->only_part] >= 0,
+                  "fap_hip_batch_create: only_part %d is not a part of the item", it->only_part);
+      const BPart &O = parts[pid2idx[G.pid_base + it->only_part]];
+      G.rlo = O.bs;
+      G.rcnt = O.b;
+      for (int pi = pfirst; pi < (int)parts.size(); ++pi)
+        emitted[pi] = parts[pi].part_id == it->only_part;
+    }
     // Vertex-id locality of the parts: 64-byte lines (16 ids) a part's vertices touch,
     // over all parts, per vertex; scattered numbering takes the sorted-order rows.
     if (G.nb > 0) {
@@ -1173,10 +1194,12 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
       gb_max_nb = std::max(gb_max_nb, G.SB / kBT);
       gbg.push_back(gi);
       mb_at.push_back(take((int64_t)G.SB * G.SB * elem));
-      r_at.push_back(take((int64_t)G.nb * G.ldr * elem));
-      rs_at.push_back(G.perm ? take((int64_t)G.nb * G.ldr * elem) : -1);
-      for (int pi = (int)parts.size() - (int)ps.size(); pi < (int)parts.size(); ++pi)
-        for (int r0 = 0; r0 < G.nb; r0 += kMWaves * q.rb5)
+      r_at.push_back(take((int64_t)std::max(G.rcnt, 1) * G.ldr * elem));
+      rs_at.push_back(G.perm ? take((int64_t)std::max(G.rcnt, 1) * G.ldr * elem) : -1);
+      G.pbase = nbrows;
+      nbrows += G.rcnt;
+      for (int pi = pfirst; pi < (int)parts.size(); ++pi)
+        for (int r0 = G.rlo; r0 < G.rlo + G.rcnt; r0 += kMWaves * q.rb5)
           for (int cb = 0; cb * kMCols < parts[pi].s; ++cb) units5.push_back(BRowUnit{pi, r0, cb, 0});
     } else {
       mb_at.push_back(-1);
@@ -1188,7 +1211,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   // Merge entries: parts with inner rows, largest first.
   std::vector<int> order;
   for (int k = 0; k < (int)parts.size(); ++k)
-    if (parts[k].s > parts[k].b) order.push_back(k);
+    if (parts[k].s > parts[k].b && emitted[k]) order.push_back(k);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return pcost[x] > pcost[y]; });
   // Units of a part: the row blocks of one column block consecutive (its boundary-row
   // panel stays in L2 while they run).  FP64 parts of graphs with at least kWideN
@@ -1267,7 +1290,9 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   }
   for (BPart &P : parts) {
     const BGraph &G = graphs[P.g];
-    if (G.full && G.R) P.Brows = static_cast<char *>(G.R) + (int64_t)P.bs * G.ldr * elem;
+    // (only the emitted part's rows exist when the item asks for one part)
+    if (G.full && G.R && P.bs >= G.rlo && P.bs + P.b <= G.rlo + G.rcnt)
+      P.Brows = static_cast<char *>(G.R) + (int64_t)(P.bs - G.rlo) * G.ldr * elem;
   }
   std::vector<char> host(scratch - o_graphs);
   auto put = [&](int64_t o, const void *src, size_t bytes) {
@@ -1331,7 +1356,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   q.gb_rows = gb_rows;
   q.gb_max_nb = gb_max_nb;
   q.nunits5 = (int)units5.size();
-  q.nbrows = (int)bpart.size();
+  q.nbrows = nbrows;
   q.nmerge = (int)merge.size();
   q.merge_units = (int)units;
   q.nsrc = (int)srcs.size();
@@ -1521,7 +1546,18 @@ extern "C" int fap_hip_solve_batch(const fap_batch_item *items, int32_t nitems, 
   }
   fap_batch *b = nullptr;
   FAP_TRY(fap_hip_batch_create(&b, items, nitems, 0));
-  int s = fap_hip_batch_run(b, st);
+  // Stage times of the call (fap_hip_last_stage_times): the whole batch as the merge span.
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int s = timing_start(st);
+  if (s == FAP_OK) s = timing_record(st, &e0);
+  if (s == FAP_OK) s = fap_hip_batch_run(b, st);
+  if (s == FAP_OK) s = timing_record(st, &e1);
+  if (s == FAP_OK) {
+    timing_add_span(3, e0, e1);
+    for (int i = 0; i < nitems; ++i)
+      if (items[i].only_part > 0) timing_note_bgraph();
+    s = timing_finish(st);
+  }
   if (s == FAP_OK && hipStreamSynchronize(st) != hipSuccess) {
     set_error("fap_hip_solve_batch: stream synchronize failed");
     s = FAP_ERR_HIP;

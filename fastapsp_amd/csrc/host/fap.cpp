@@ -479,12 +479,27 @@ int32_t fapGraph::solveSubGraph(int32_t id, bool path_needed) {
   }
   fap_graph_dev g{num_vertexs, num_edges, d_row_offset, d_col, d_w,
                   d_graph_id,  d_st2ed,   d_ed2st,      d_comp, d_wf};
-  if (float_weights)
+  if (subgraph_by_boundary_graph()) {
+    // Weighted graph whose boundary sources reach far: every part's in-part distances
+    // and the closed boundary graph, then this part's rows (a one-part batch item).
+    std::vector<fap_part_desc> all = partList();
+    all[id - 1].out_row = 0;  // the block's first row (the other parts write nothing)
+    fap_batch_item it{};
+    it.g = g;
+    it.parts = all.data();
+    it.nparts = (int32_t)all.size();
+    it.is_float = float_weights ? 1 : 0;
+    it.d_out = d_sub_dist;
+    it.ld_out = ld_dist;
+    it.only_part = id;
+    check(fap_hip_solve_batch(&it, 1, FAP_MODE_BATCHED, nullptr), "solveSubGraph");
+  } else if (float_weights) {
     check(fap_hip_solve_parts_f32(&g, &part, 1, reinterpret_cast<float *>(d_sub_dist), ld_dist,
                                   mode, nullptr),
           "solveSubGraph");
-  else
+  } else {
     check(fap_hip_solve_parts(&g, &part, 1, d_sub_dist, ld_dist, mode, nullptr), "solveSubGraph");
+  }
   check(fap_hip_stream_sync(nullptr), "solveSubGraph");
   add_stage_times(1);
   current_subgraph_id = id;

@@ -51,6 +51,7 @@ enum State { kPending = 0, kRunning = 1, kDone = 2, kFailed = 3 };
 
 struct Request {
   int graph = 0, sub_id = 0;
+  int only_part = 0;  // > 0: parts tile the graph, only this part's rows are produced
   int state = kPending;
   double t_submit = 0, t_done = 0;
   void *d_out = nullptr;
@@ -118,7 +119,8 @@ int64_t scratch_estimate(const Request &r, bool is_float, int n) {
   }
   if (covered == n && nb > 0) {
     const int64_t SB = align_up(nb, 64);
-    bytes += align_up(SB * SB * elem, 256) + 2 * align_up(nb * ldr * elem, 256);
+    const int64_t rows = r.only_part > 0 ? std::max(r.parts[r.only_part - 1].bdy, 1) : nb;
+    bytes += align_up(SB * SB * elem, 256) + 2 * align_up(rows * ldr * elem, 256);
     tables += brow_units * ((nb + 15) / 16) * 16 + 12 * nb;
   } else {
     for (const fap_part_desc &p : r.parts) bytes += align_up((int64_t)p.bdy * ldr * elem, 256);
@@ -228,6 +230,7 @@ void scheduler(fap_server *s) {
           it.d_out = r.d_out;
           it.ld_out = r.ld;
           it.flags = ge.g->boundaryBySssp() ? FAP_ITEM_BOUNDARY_SSSP : 0;
+          it.only_part = r.only_part;
           items.push_back(it);
         }
         Inflight f;
@@ -304,19 +307,31 @@ void pipeline_lane(fap_server *s) {
     const bool by_sssp = (ge.g->solveModeFlags() & FAP_MODE_FLAG_SSSP) != 0;
     std::vector<fap_part_desc> parts = r.parts;
     const int64_t bytes = r.bytes, ld = r.ld;
+    const int only_part = r.only_part;
     lk.unlock();
     void *d_out = nullptr;
     bool ok = fap_hip_malloc(&d_out, std::max<int64_t>(bytes, 4)) == FAP_OK;
     // int32 FW keeps the call asynchronous (the 16-bit form reads flags back).
     const int32_t mode = FAP_MODE_BATCHED | FAP_MODE_FLAG_INT32_FW |
                          (by_sssp ? FAP_MODE_FLAG_SSSP : 0);
-    if (ok)
+    if (ok && only_part > 0) {  // one part's rows from the boundary graph
+      fap_batch_item it{};
+      it.g = dev;
+      it.parts = parts.data();
+      it.nparts = (int32_t)parts.size();
+      it.is_float = is_float ? 1 : 0;
+      it.d_out = d_out;
+      it.ld_out = ld;
+      it.only_part = only_part;
+      ok = fap_hip_solve_batch(&it, 1, FAP_MODE_BATCHED, s->pipe_stream) == FAP_OK;
+    } else if (ok) {
       ok = (is_float ? fap_hip_solve_parts_f32(&dev, parts.data(), (int32_t)parts.size(),
                                                static_cast<float *>(d_out), ld, mode,
                                                s->pipe_stream)
                      : fap_hip_solve_parts(&dev, parts.data(), (int32_t)parts.size(),
                                            static_cast<int32_t *>(d_out), ld, mode,
                                            s->pipe_stream)) == FAP_OK;
+    }
     if (ok) ok = fap_hip_stream_sync(s->pipe_stream) == FAP_OK;
     const double t = now_s();
     lk.lock();
@@ -437,6 +452,12 @@ int fap_server_submit(fap_server *s, int32_t graph, int32_t sub_id, int64_t *tic
   if (sub_id == 0) {
     r.parts = ge.parts;  // out_row = offset: the n x n layout of solve()
     r.rows = ge.dev.n;
+  } else if (ge.g->subGraphByBoundaryGraph()) {
+    // Every part (in-part distances, boundary graph), rows of part sub_id only.
+    r.parts = ge.parts;
+    r.parts[sub_id - 1].out_row = 0;
+    r.only_part = sub_id;
+    r.rows = r.parts[sub_id - 1].num;
   } else {
     fap_part_desc p = ge.parts[sub_id - 1];
     p.out_row = 0;

@@ -374,6 +374,11 @@ typedef struct fap_batch_item {
   void *d_out;
   int64_t ld_out;
   int32_t flags; /* FAP_ITEM_* */
+  /* > 0: the parts tile the graph but only the rows of the part with this id are
+   * produced (a sub-graph block, solveSubGraph(id)): boundary rows from the boundary
+   * graph (every part's in-part distances and the closed boundary graph) instead of
+   * one SSSP per boundary vertex; the other parts' out_row are ignored. */
+  int32_t only_part;
 } fap_batch_item;
 /* Boundary rows by SSSP even when the parts tile the graph: for graphs with parallel
  * arcs inside a part whose later arc (CSR order) is heavier, where the reference's

@@ -234,12 +234,22 @@ int fap_hip_batch_create(fap_batch **b, const fap_batch_item *items, int32_t nit
   return FAP_OK;
 }
 
+// One item: every part, or only the part it.only_part names (its out_row is the block's
+// first row; the other parts write nothing).
+static int host_item(const fap_batch_item &it, const fap_part_desc *parts, int nparts) {
+  if (it.only_part > 0) {
+    for (int k = 0; k < nparts; ++k)
+      if (parts[k].part_id == it.only_part)
+        return host_solve(&it.g, parts + k, 1, it.d_out, it.is_float != 0, nullptr, it.ld_out);
+    return FAP_ERR_INVALID;
+  }
+  return host_solve(&it.g, parts, nparts, it.d_out, it.is_float != 0, nullptr, it.ld_out);
+}
+
 int fap_hip_batch_run(fap_batch *b, void *) {
   HostBatch *h = reinterpret_cast<HostBatch *>(b);
   for (size_t i = 0; i < h->items.size(); ++i) {
-    const fap_batch_item &it = h->items[i];
-    const int s = host_solve(&it.g, h->parts[i].data(), (int)h->parts[i].size(), it.d_out,
-                             it.is_float != 0, nullptr, it.ld_out);
+    const int s = host_item(h->items[i], h->parts[i].data(), (int)h->parts[i].size());
     if (s != FAP_OK) return s;
   }
   return FAP_OK;
@@ -252,8 +262,7 @@ int fap_hip_batch_destroy(fap_batch *b) {
 
 int fap_hip_solve_batch(const fap_batch_item *items, int32_t nitems, int32_t, void *) {
   for (int i = 0; i < nitems; ++i) {
-    const int s = host_solve(&items[i].g, items[i].parts, items[i].nparts, items[i].d_out,
-                             items[i].is_float != 0, nullptr, items[i].ld_out);
+    const int s = host_item(items[i], items[i].parts, items[i].nparts);
     if (s != FAP_OK) return s;
   }
   return FAP_OK;

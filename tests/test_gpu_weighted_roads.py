@@ -5,7 +5,9 @@ boundary vertices than the single-graph boundary-graph form holds, and float wei
   label-correcting SSSP per boundary vertex, the reference's structure), bit for bit;
 * sampled rows equal an independent integer Dijkstra (int weights) or lie within
   relative 1e-6 of an FP64 Dijkstra and equal its float32 rounding (float weights);
-* solveSubGraph blocks equal the rows of solve()."""
+* solveSubGraph blocks (one part of a batch item whose parts tile the graph: every
+  part's in-part distances and the closed boundary graph, then that part's rows) equal
+  the rows of solve() and the boundary-SSSP form's block."""
 import sys
 
 import numpy as np
@@ -64,6 +66,19 @@ def check_against_sssp_form(g, fl, block=2048):
     g.setMode(F.FAP_MODE_BATCHED)
 
 
+def check_subgraph_against_sssp_form(g, p, fl):
+    """solveSubGraph(p) from the boundary graph against the boundary-SSSP form."""
+    num = int(g.layout()["num"][p])
+    g.solveSubGraph(p)
+    a = g.getSubGraphDistanceRowsF32(p, 0, num) if fl else g.getSubGraphDistanceI32(p)
+    g.setMode(SSSP)
+    g.solveSubGraph(p)
+    assert L.last_stage_times()["boundary_graph"] == 0
+    b = g.getSubGraphDistanceRowsF32(p, 0, num) if fl else g.getSubGraphDistanceI32(p)
+    g.setMode(F.FAP_MODE_BATCHED)
+    assert np.array_equal(a, b), p
+
+
 @pytest.mark.timeout(900)
 def test_int_weights_beyond_the_single_graph_boundary_limit():
     say("road-like ~31k, int weights 1..1000, K=80 (4571 boundary vertices)")
@@ -84,7 +99,9 @@ def test_int_weights_beyond_the_single_graph_boundary_limit():
         off, num = int(lay["offset"][p]), int(lay["num"][p])
         full = g.getDistanceRowsI32(off, num)
         g.solveSubGraph(p)
+        assert L.last_stage_times()["boundary_graph"] == 1  # one-part batch item
         assert np.array_equal(g.getSubGraphDistanceI32(p), full), p
+    check_subgraph_against_sssp_form(g, len(lay["num"]) // 3, False)
     check_against_sssp_form(g, False)
     say("every entry equals the boundary-SSSP form")
 
@@ -112,6 +129,8 @@ def test_float_weights_boundary_graph_form():
     off, num = int(lay["offset"][p]), int(lay["num"][p])
     full = g.getDistanceRowsF32(off, num)
     g.solveSubGraph(p)
+    assert L.last_stage_times()["boundary_graph"] == 1  # one-part batch item
     assert np.array_equal(g.getSubGraphDistanceRowsF32(p, 0, num), full)
+    check_subgraph_against_sssp_form(g, 1, True)
     check_against_sssp_form(g, True)
     say("every entry equals the boundary-SSSP form")
