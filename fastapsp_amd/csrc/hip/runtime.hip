@@ -1,8 +1,10 @@
 // Runtime part of the C-ABI: device selection, library stream, budgeted device
 // memory, copies, fills and events.
 #include <cstdarg>
+#include <map>
 #include <mutex>
 #include <unordered_map>
+#include <vector>
 
 #include "fap_internal.h"
 
@@ -22,7 +24,27 @@ bool g_ws_fence_set = false;
 int64_t g_budget = 0;  // 0 = unlimited
 int64_t g_in_use = 0;
 int64_t g_high = 0;
-std::unordered_map<void *, int64_t> g_sizes;
+struct Block {
+  int64_t bytes;   // size of the device allocation
+  int64_t booked;  // bytes the caller asked for (what in_use and the budget count)
+};
+std::unordered_map<void *, Block> g_sizes;
+// Freed blocks kept for reuse: hipMalloc / hipFree of multi-GB buffers while kernels
+// run on other streams took 0.6-2.3 s each in the request server (0.4 ms on an idle
+// device), so blocks up to kCacheBlockMax go back to this cache (at most kCacheMax in
+// all) and a later request takes the smallest cached block of at least its size and at
+// most twice it.  Larger blocks (a placed n x n result) are freed at once.
+constexpr int64_t kCacheBlockMax = 32LL << 30;
+constexpr int64_t kCacheMax = 64LL << 30;
+std::multimap<int64_t, void *> g_cache;
+int64_t g_cached = 0;
+
+// Frees every cached block (with g_mu held by the caller).
+void drop_cache_locked(std::vector<void *> &out) {
+  for (auto &kv : g_cache) out.push_back(kv.second);
+  g_cache.clear();
+  g_cached = 0;
+}
 }  // namespace
 
 void set_error(const char *fmt, ...) {
@@ -70,8 +92,28 @@ int dev_alloc(void **p, int64_t bytes) {
                 (long long)g_in_use, (long long)bytes, (long long)g_budget);
       return FAP_ERR_BUDGET;
     }
+    auto it = g_cache.lower_bound(bytes);
+    if (it != g_cache.end() && it->first <= 2 * bytes) {
+      *p = it->second;
+      g_sizes[*p] = Block{it->first, bytes};
+      g_cached -= it->first;
+      g_cache.erase(it);
+      g_in_use += bytes;
+      if (g_in_use > g_high) g_high = g_in_use;
+      return FAP_OK;
+    }
   }
   hipError_t e = hipMalloc(p, (size_t)bytes);
+  if (e != hipSuccess) {  // out of memory: give the cached blocks back and try again
+    (void)hipGetLastError();
+    std::vector<void *> drop;
+    {
+      std::lock_guard<std::mutex> lk(g_mu);
+      drop_cache_locked(drop);
+    }
+    for (void *q : drop) (void)hipFree(q);
+    e = drop.empty() ? e : hipMalloc(p, (size_t)bytes);
+  }
   if (e != hipSuccess) {
     (void)hipGetLastError();
     *p = nullptr;
@@ -79,7 +121,7 @@ int dev_alloc(void **p, int64_t bytes) {
     return FAP_ERR_NOMEM;
   }
   std::lock_guard<std::mutex> lk(g_mu);
-  g_sizes[*p] = bytes;
+  g_sizes[*p] = Block{bytes, bytes};
   g_in_use += bytes;
   if (g_in_use > g_high) g_high = g_in_use;
   return FAP_OK;
@@ -87,6 +129,7 @@ int dev_alloc(void **p, int64_t bytes) {
 
 int dev_free(void *p) {
   if (!p) return FAP_OK;
+  std::vector<void *> drop;
   {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_sizes.find(p);
@@ -94,10 +137,23 @@ int dev_free(void *p) {
       set_error("fap_hip_free: pointer %p was not allocated by fap_hip_malloc", p);
       return FAP_ERR_INVALID;
     }
-    g_in_use -= it->second;
+    const Block b = it->second;
+    g_in_use -= b.booked;
     g_sizes.erase(it);
+    if (b.bytes > kCacheBlockMax) {
+      drop.push_back(p);
+    } else {
+      g_cache.insert({b.bytes, p});
+      g_cached += b.bytes;
+      while (g_cached > kCacheMax) {  // the largest cached blocks go first
+        auto last = std::prev(g_cache.end());
+        g_cached -= last->first;
+        drop.push_back(last->second);
+        g_cache.erase(last);
+      }
+    }
   }
-  FAP_CHECK_HIP(hipFree(p));
+  for (void *q : drop) FAP_CHECK_HIP(hipFree(q));
   return FAP_OK;
 }
 
