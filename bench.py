@@ -204,7 +204,7 @@ def batch_bench(args):
     if getattr(args, "no_tune", False):  # fixed settings (profiling runs)
         table = {"K": [8, 8, 8, 16, 16], "per_cu": [0] * 5, "src_rows": [32] * 5,
                  "merge_rows": [16] * 5, "packed": [1] * 5, "unit_grid": [1] * 5,
-                 "ms_per_graph": [None] * 5}
+                 "fw_tile": [64] * 5, "ms_per_graph": [None] * 5}
     elif getattr(args, "tune_quick", False):
         table = AT.tune(k_candidates=(8, 16), per_cu_candidates=(0,), src_rows_candidates=(16,),
                         merge_rows_candidates=(8, 16), max_copies=16)
@@ -213,7 +213,7 @@ def batch_bench(args):
     log(f"autotune {time.perf_counter() - t0:.1f}s: K per class {table['K']}, "
         f"per_cu {table['per_cu']}, source rows {table['src_rows']}, "
         f"merge rows {table['merge_rows']}, packed {table['packed']}, "
-        f"unit grid {table['unit_grid']}")
+        f"unit grid {table['unit_grid']}, FW tile {table['fw_tile']}")
     rng = np.random.default_rng(2026)
     sizes = np.exp(rng.uniform(np.log(256), np.log(8192), args.graphs)).astype(int)
     t0 = time.perf_counter()
@@ -231,6 +231,9 @@ def batch_bench(args):
     if getattr(args, "unit_grid", None) is not None:
         settings["unit_grid"] = int(args.unit_grid)
     b = F.Batch(graphs, use_graph=True, **settings)
+    tiles = settings.get("fw_tile", [64] * len(graphs))
+    settings = dict(settings, fw_tile={"64": sum(1 for t in tiles if t == 64),
+                                       "32": sum(1 for t in tiles if t == 32)})
     for _ in range(args.warmup):
         b.run()
     L.check(L.load().fap_hip_device_sync(), "sync")
@@ -277,7 +280,8 @@ def batch_bench(args):
                       "graphs": len(graphs), "sum_n2": entries, "n_min": min(ns),
                       "n_max": max(ns), "settings": settings},
            "autotune": {k: table[k] for k in ("K", "per_cu", "src_rows", "merge_rows",
-                                               "packed", "unit_grid", "ms_per_graph")},
+                                               "packed", "unit_grid", "fw_tile",
+                                               "ms_per_graph")},
            "plain_form": plain, "cpu_baseline": cpu}
     return out
 

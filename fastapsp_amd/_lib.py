@@ -35,6 +35,8 @@ FAP_BATCH_GRAPH = 0x1
 FAP_BATCH_NO_PACKED = 0x100000
 FAP_BATCH_NO_BFS = 0x200000
 FAP_BATCH_UNIT_GRID = 0x400000
+FAP_BATCH_FW_TILE32 = 0x800000
+FAP_ITEM_FW_TILE32 = 0x2
 
 # Every exported symbol of include/fap_hip.h: name -> (restype, argtypes).
 _i32, _i64, _u64, _f64 = C.c_int32, C.c_int64, C.c_uint64, C.c_double
@@ -137,6 +139,8 @@ SIGNATURES = {
     "fap_hip_stream_create": (C.c_int, [C.POINTER(_vp)]),
     "fap_hip_stream_destroy": (C.c_int, [_vp]),
     "fap_graph_batch_create": (C.c_int, [C.POINTER(_vp), C.POINTER(_vp), _i32, _i32]),
+    "fap_graph_batch_create_ex": (C.c_int, [C.POINTER(_vp), C.POINTER(_vp), _i32, _i32,
+                                            C.POINTER(_i32)]),
     "fap_graph_solve_batch": (C.c_int, [C.POINTER(_vp), _i32, _i32]),
     "fap_graph_create_from_file": (C.c_int, [C.POINTER(_vp), _cp, C.c_int, C.c_int, _i32, _cp]),
     "fap_mtx_read": (C.c_int, [_cp, C.c_int, C.c_int, _pi32, _pi64, _pi32, _pi32, _pf32]),

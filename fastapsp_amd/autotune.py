@@ -13,7 +13,10 @@ fastest:
   * the persistent merge workgroups per CU (0 = as many as stay resident);
   * the packed 16-bit merge of unit-weight graphs, on or off;
   * the merge as the persistent work queue or as a plain grid of one workgroup per unit
-    (with both distance types in a batch the two launch sequences run side by side).
+    (with both distance types in a batch the two launch sequences run side by side);
+  * the Floyd-Warshall block size of the float-weight parts: 64 x 64 or 32 x 32 tiles
+    (matrices padded to multiples of the tile: 32 pads small parts less, 64 reuses each
+    staged panel for more steps).
 K is chosen first with the default tiles, then each tile knob in turn (coordinate
 search).  The result is a small table (JSON-serialisable) that `k_for(n)` and
 `settings_for(ns)` read; `load_or_tune(path)` caches it on disk.
@@ -38,8 +41,9 @@ SRC_ROWS_CANDIDATES = (8, 16, 32)
 MERGE_ROWS_CANDIDATES = (8, 16)
 PACKED_CANDIDATES = (1, 0)
 UNIT_GRID_CANDIDATES = (0, 1)
+FW_TILE_CANDIDATES = (64, 32)
 MAX_PART = 32768  # largest part of the batch form
-KNOBS = ("per_cu", "src_rows", "merge_rows", "packed", "unit_grid")
+KNOBS = ("per_cu", "src_rows", "merge_rows", "packed", "unit_grid", "fw_tile")
 
 
 def symmetric_float_weights(ro, col, seed: int, directed: bool = False) -> np.ndarray:
@@ -90,14 +94,14 @@ def tune(copies_work: int = 512, max_copies: int = 48, seed: int = 4242,
          k_candidates=K_CANDIDATES, per_cu_candidates=PER_CU_CANDIDATES,
          src_rows_candidates=SRC_ROWS_CANDIDATES, merge_rows_candidates=MERGE_ROWS_CANDIDATES,
          packed_candidates=PACKED_CANDIDATES, unit_grid_candidates=UNIT_GRID_CANDIDATES,
-         verbose: bool = False) -> dict:
+         fw_tile_candidates=FW_TILE_CANDIDATES, verbose: bool = False) -> dict:
     """Per size class: K with the default tiles, then each knob in turn; the table
     keeps the fastest setting seen (every trial is recorded)."""
     table = {"classes": SIZE_CLASSES, "K": [], "per_cu": [], "src_rows": [], "merge_rows": [],
-             "packed": [], "unit_grid": [], "ms_per_graph": [], "trials": []}
+             "packed": [], "unit_grid": [], "fw_tile": [], "ms_per_graph": [], "trials": []}
     cands = {"per_cu": tuple(per_cu_candidates), "src_rows": tuple(src_rows_candidates),
              "merge_rows": tuple(merge_rows_candidates), "packed": tuple(packed_candidates),
-             "unit_grid": tuple(unit_grid_candidates)}
+             "unit_grid": tuple(unit_grid_candidates), "fw_tile": tuple(fw_tile_candidates)}
     for ci, (lo, hi) in enumerate(SIZE_CLASSES):
         n_mid = int(math.sqrt(lo * min(hi, 8192)))
         trials = []
@@ -153,10 +157,14 @@ def _dominant_class(ns) -> int:
 
 
 def settings_for(ns, table: dict) -> dict:
-    """Batch settings (per_cu, src_rows, merge_rows) for a mixed batch: those of the
-    size class holding the most work (sum of n^2)."""
+    """Batch settings for a mixed batch: the launch-wide knobs (per_cu, src_rows,
+    merge_rows, packed, unit_grid) of the size class holding the most work (sum of n^2),
+    and the Floyd-Warshall tile of each graph's own size class (a list, one per graph)."""
     c = _dominant_class(ns)
-    return {k: int(table[k][c]) for k in KNOBS if k in table}
+    out = {k: int(table[k][c]) for k in KNOBS if k in table and k != "fw_tile"}
+    if "fw_tile" in table:
+        out["fw_tile"] = [int(table["fw_tile"][size_class(n)]) for n in ns]
+    return out
 
 
 def per_cu_for(ns, table: dict) -> int:

@@ -34,18 +34,25 @@ class Batch:
     packed: 1 = the int32 merge of unit-weight graphs (n <= 15872) on packed 16-bit
     distances, 0 = on int32; bfs: 1 = in-part distances of unit-weight symmetric graphs
     by bit-parallel BFS, 0 = by Floyd-Warshall; unit_grid: 1 = the merge as a plain grid
-    of one workgroup per unit, 0 = the persistent work queue."""
+    of one workgroup per unit, 0 = the persistent work queue; fw_tile: Floyd-Warshall tile
+    edge of the float-weight (FP64) parts and boundary graphs, 64 or 32, for every graph
+    or a list with one entry per graph (the autotuner's choice per size class)."""
 
     def __init__(self, graphs: Sequence, use_graph: bool = True, per_cu: int = 0,
                  src_rows: int = 0, merge_rows: int = 0, packed: int = 1, bfs: int = 1,
-                 unit_grid: int = 0):
+                 unit_grid: int = 0, fw_tile=64):
         self.graphs = list(graphs)
         for g in self.graphs:
             if not getattr(g, "_preconditioned", False):
                 g.preCondition()
         self._h = None
         h = C.c_void_p()
-        L.check(L.load().fap_graph_batch_create(C.byref(h), _handles(self.graphs),
+        tiles = list(fw_tile) if isinstance(fw_tile, (list, tuple)) else [fw_tile] * len(self.graphs)
+        if len(tiles) != len(self.graphs) or any(int(t) not in (32, 64) for t in tiles):
+            raise ValueError("fw_tile: 64 or 32, for every graph or one per graph")
+        item_flags = (C.c_int32 * max(len(tiles), 1))(
+            *[L.FAP_ITEM_FW_TILE32 if int(t) == 32 else 0 for t in tiles])
+        L.check(L.load().fap_graph_batch_create_ex(C.byref(h), _handles(self.graphs),
                                                 len(self.graphs),
                                                 (L.FAP_BATCH_GRAPH if use_graph else 0)
                                                 | ((int(per_cu) & 0xFF) << 8)
@@ -53,7 +60,8 @@ class Batch:
                                                 | (_ROWS_CODE[int(merge_rows)] << 18)
                                                 | (0 if packed else L.FAP_BATCH_NO_PACKED)
                                                 | (0 if bfs else L.FAP_BATCH_NO_BFS)
-                                                | (L.FAP_BATCH_UNIT_GRID if unit_grid else 0)),
+                                                | (L.FAP_BATCH_UNIT_GRID if unit_grid else 0),
+                                                item_flags),
                 "batch create")
         self._h = h.value
 

@@ -44,6 +44,7 @@ namespace fap_hip {
 
 int launch_fw(const FwDesc *d_descs, int ndesc, int max_nb, hipStream_t stream);
 int launch_fw64(const Fw64Desc *d_descs, int nd, int max_nb, hipStream_t st);
+int launch_fw64_t32(const Fw64Desc *d_descs, int nd, int max_nb, hipStream_t st);
 int launch_inpart_bfs(const InpartDesc *d_descs, int nd, int total_groups, int s_max,
                       hipStream_t stream);
 
@@ -68,6 +69,7 @@ struct BGraph {
   void *Rs;                      // the same rows with columns in sorted-position order
   int perm;                      // 1: boundary rows via Rs and bperm_kernel (scattered parts)
   int rlo, rcnt;                 // boundary-graph rows produced (all, or one part's)
+  int tile;                      // Floyd-Warshall tile edge of its parts and boundary graph
   int pbase;                     // first of them in the row numbering of bperm_kernel
   int64_t ldr;
   void *MB;                      // boundary-graph matrix (SB x SB), full items only
@@ -897,6 +899,9 @@ struct fap_batch_kind {
   int rows6 = 16;  // merge rows per wave (FP64: 8, register budget)
   int rb5 = 8;     // boundary-row unit rows per wave (4 * rb5 boundary-graph rows per unit)
   int nbrows = 0;  // boundary rows of full items (one workgroup each in bperm_kernel)
+  // Floyd-Warshall descriptors: the 64-tile ones first, then (FP64) the 32-tile ones,
+  // each group with its own rounds.
+  int nfw64 = 0, ngb64 = 0, nfw32 = 0, ngb32 = 0, max_nb32 = 0, gb_max_nb32 = 0;
   int mlen[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // merge units per XCD list
   int *d_mlen = nullptr;
   // FP64 parts of graphs with at least kWideN vertices: the 8 x 8 register-tile merge.
@@ -929,6 +934,7 @@ struct fap_batch {
   bool packed = true;  // packed 16-bit merge where exact
   bool bfs = true;     // in-part BFS where exact
   bool unit_grid = false;  // merge as a plain grid instead of the persistent queue
+  bool fw32 = false;       // FP64 Floyd-Warshall on 32 x 32 tiles
   bool use_graph = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -943,10 +949,12 @@ namespace {
 // Batched Floyd-Warshall over nd descriptors (the launchers put descriptors on the
 // grid's y dimension: at most 65,535 per launch).
 template <typename T>
-int launch_fw_chunks(void *descs, int nd, int max_nb, hipStream_t st) {
+int launch_fw_chunks(void *descs, int nd, int max_nb, int tile, hipStream_t st) {
   for (int d0 = 0; d0 < nd; d0 += 65535) {
     const int cnt = std::min(65535, nd - d0);
     if (sizeof(T) == 4) FAP_TRY(launch_fw(static_cast<FwDesc *>(descs) + d0, cnt, max_nb, st));
+    else if (tile == 32)
+      FAP_TRY(launch_fw64_t32(static_cast<Fw64Desc *>(descs) + d0, cnt, max_nb, st));
     else FAP_TRY(launch_fw64(static_cast<Fw64Desc *>(descs) + d0, cnt, max_nb, st));
   }
   return FAP_OK;
@@ -964,7 +972,10 @@ int launch_kind(const fap_batch *b, const fap_batch_kind &q, int *counter, hipSt
     hipLaunchKernelGGL(bbuild_kernel<T>, dim3((q.build_rows + 3) / 4), dim3(256), 0, st,
                        q.graphs, q.parts, q.nparts, q.build_rows);
     FAP_CHECK_LAUNCH();
-    FAP_TRY(launch_fw_chunks<T>(q.fw_descs, q.nfw, q.max_nb, st));
+    FAP_TRY(launch_fw_chunks<T>(q.fw_descs, q.nfw64, q.max_nb, 64, st));
+    if (q.nfw32 > 0)
+      FAP_TRY(launch_fw_chunks<T>(static_cast<Fw64Desc *>(q.fw_descs) + q.nfw64, q.nfw32,
+                                  q.max_nb32, 32, st));
   }
   if (q.nip2 > 0) FAP_TRY(launch_inpart_bfs(q.ip, q.nip2, q.gip2, 1024, st));
   if (q.nip4 > 0) FAP_TRY(launch_inpart_bfs(q.ip + q.nip2, q.nip4, q.gip4, kInpartBfsMaxS, st));
@@ -972,7 +983,10 @@ int launch_kind(const fap_batch *b, const fap_batch_kind &q, int *counter, hipSt
     hipLaunchKernelGGL(bgb_kernel<T>, dim3((q.gb_rows + 3) / 4), dim3(256), 0, st, q.graphs,
                        q.parts, q.gbg, q.ngb, q.bpart, q.pid2idx, q.gb_rows);
     FAP_CHECK_LAUNCH();
-    FAP_TRY(launch_fw_chunks<T>(q.gb_descs, q.ngb, q.gb_max_nb, st));
+    FAP_TRY(launch_fw_chunks<T>(q.gb_descs, q.ngb64, q.gb_max_nb, 64, st));
+    if (q.ngb32 > 0)
+      FAP_TRY(launch_fw_chunks<T>(static_cast<Fw64Desc *>(q.gb_descs) + q.ngb64, q.ngb32,
+                                  q.gb_max_nb32, 32, st));
   }
   if (q.nunits5 > 0) {
     const dim3 g5(q.nunits5), t5(256);
@@ -1088,6 +1102,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
     G.n = d.n;
     G.ldr = align_up(d.n, kMCols);  // boundary rows: +inf padding to whole column blocks
     G.packed = !fp64 && b->packed && d.d_w == nullptr && d.d_wf == nullptr && d.n <= kCap16;
+    G.tile = fp64 && (b->fw32 || (it->flags & FAP_ITEM_FW_TILE32)) ? 32 : kBT;
     G.bbase = (int)bpart.size();
     G.pid_base = (int)pid2idx.size();
     const int gi = (int)graphs.size();
@@ -1124,7 +1139,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
                   "fap_hip_batch_create: part id %d repeated", pd->part_id);
       pid2idx[G.pid_base + pd->part_id] = (int)parts.size();
       BPart P{};
-      P.S = (int)align_up(pd->num, kBT);
+      P.S = (int)align_up(pd->num, G.tile);
       P.g = gi;
       P.part_id = pd->part_id;
       P.off = pd->offset;
@@ -1138,7 +1153,8 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
       P.bfs = !fp64 && b->bfs && G.full && d.d_w == nullptr && d.d_wf == nullptr &&
               d.d_comp != nullptr && P.s <= kInpartBfsMaxS;
       build_rows += P.S;
-      max_nb = std::max(max_nb, P.S / kBT);
+      if (G.tile == kBT) max_nb = std::max(max_nb, P.S / kBT);
+      else q.max_nb32 = std::max(q.max_nb32, P.S / G.tile);
       d_at.push_back(take((int64_t)P.S * P.S * elem));
       if (G.full) {
         for (int c = 0; c < P.b; ++c) {
@@ -1188,10 +1204,11 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
       G.perm = lines * 4 > d.n;
     }
     if (G.nb > 0) {
-      G.SB = (int)align_up(G.nb, kBT);
+      G.SB = (int)align_up(G.nb, G.tile);
       G.gb_row_begin = gb_rows;
       gb_rows += G.SB;
-      gb_max_nb = std::max(gb_max_nb, G.SB / kBT);
+      if (G.tile == kBT) gb_max_nb = std::max(gb_max_nb, G.SB / kBT);
+      else q.gb_max_nb32 = std::max(q.gb_max_nb32, G.SB / G.tile);
       gbg.push_back(gi);
       mb_at.push_back(take((int64_t)G.SB * G.SB * elem));
       r_at.push_back(take((int64_t)std::max(G.rcnt, 1) * G.ldr * elem));
@@ -1312,9 +1329,21 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   put(o_src, srcs.data(), sizeof(BSrc) * srcs.size());
   if (fp64) {
     std::vector<Fw64Desc> fd, gd;
-    for (const BPart &P : parts) fd.push_back(Fw64Desc{static_cast<double *>(P.D), P.S / kBT});
-    for (int gi : gbg)
-      gd.push_back(Fw64Desc{static_cast<double *>(graphs[gi].MB), graphs[gi].SB / kBT});
+    // 64-tile descriptors first, then the 32-tile ones (their own launch sequence).
+    for (int t32 = 0; t32 < 2; ++t32) {
+      for (const BPart &P : parts)
+        if ((graphs[P.g].tile == 32) == (t32 == 1))
+          fd.push_back(Fw64Desc{static_cast<double *>(P.D), P.S / graphs[P.g].tile});
+      for (int gi : gbg)
+        if ((graphs[gi].tile == 32) == (t32 == 1))
+          gd.push_back(Fw64Desc{static_cast<double *>(graphs[gi].MB), graphs[gi].SB / graphs[gi].tile});
+      if (t32 == 0) {
+        q.nfw64 = (int)fd.size();
+        q.ngb64 = (int)gd.size();
+      }
+    }
+    q.nfw32 = (int)fd.size() - q.nfw64;
+    q.ngb32 = (int)gd.size() - q.ngb64;
     put(o_fw, fd.data(), sizeof(Fw64Desc) * fd.size());
     put(o_gbd, gd.data(), sizeof(Fw64Desc) * gd.size());
     q.nfw = (int)fd.size();
@@ -1337,7 +1366,8 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
       gd.push_back(FwDesc{static_cast<int *>(graphs[gi].MB), graphs[gi].SB, graphs[gi].SB / kBT});
     put(o_fw, fd.data(), sizeof(FwDesc) * fd.size());
     put(o_gbd, gd.data(), sizeof(FwDesc) * gd.size());
-    q.nfw = (int)fd.size();
+    q.nfw = q.nfw64 = (int)fd.size();
+    q.ngb64 = (int)gd.size();
     q.nip2 = (int)ip2.size();
     q.nip4 = (int)ip4.size();
     ip2.insert(ip2.end(), ip4.begin(), ip4.end());
@@ -1406,7 +1436,7 @@ extern "C" int fap_hip_batch_create(fap_batch **out, const fap_batch_item *items
   }
   FAP_REQUIRE(out && nitems >= 0 && (nitems == 0 || items), "fap_hip_batch_create: null argument");
   FAP_REQUIRE((flags & ~(FAP_BATCH_GRAPH | 0xFF00 | 0xF0000 | FAP_BATCH_NO_PACKED |
-                         FAP_BATCH_NO_BFS | FAP_BATCH_UNIT_GRID)) == 0,
+                         FAP_BATCH_NO_BFS | FAP_BATCH_UNIT_GRID | FAP_BATCH_FW_TILE32)) == 0,
               "fap_hip_batch_create: unknown flags %d", flags);
   *out = nullptr;
   std::vector<const fap_batch_item *> kinds[2];
@@ -1427,6 +1457,7 @@ extern "C" int fap_hip_batch_create(fap_batch **out, const fap_batch_item *items
   b->packed = (flags & FAP_BATCH_NO_PACKED) == 0;
   b->bfs = (flags & FAP_BATCH_NO_BFS) == 0;
   b->unit_grid = (flags & FAP_BATCH_UNIT_GRID) != 0;
+  b->fw32 = (flags & FAP_BATCH_FW_TILE32) != 0;
   int s = dev_alloc((void **)&b->d_counter, 256);
   for (int k = 0; k < 2 && s == FAP_OK; ++k)
     if (!kinds[k].empty()) s = plan_kind(b, b->kind[k], kinds[k], k == 1);

@@ -424,6 +424,138 @@ __global__ __launch_bounds__(256) void fw64_phase3_kernel(const Fw64Desc *__rest
 
 }  // namespace
 
+namespace {
+
+// ---- 32 x 32 tiles: the Floyd-Warshall tile of the batch's FP64 parts when the
+// autotuner picks it (parts padded to multiples of 32 instead of 64: a part of 33-64
+// vertices does 1/8 of the padded work).  Same three phases on smaller tiles.
+constexpr int kT32 = 32;
+
+__device__ __forceinline__ FAP_GLOBAL double *tile32(const Fw64Desc &d, int ti, int tj) {
+  const int64_t S = (int64_t)d.nb * kT32;
+  return gptr(d.mat + (int64_t)ti * kT32 * S + tj * kT32);
+}
+
+// Phase 1: 128 threads; wave w holds rows w, w + 2, ..., w + 30 in registers, lane j
+// column j & 31 (lanes 32..63 mirror lanes 0..31 and store nothing).  Column k comes
+// from lane k by readlane, row k from LDS (published by its wave after step k - 1).
+__global__ __launch_bounds__(128) void fw32_phase1_kernel(const Fw64Desc *__restrict__ descs,
+                                                          int r) {
+  const Fw64Desc d = descs[blockIdx.x];
+  if (r >= d.nb) return;
+  const int64_t S = (int64_t)d.nb * kT32;
+  __shared__ double Rk[2][kT32];
+  FAP_GLOBAL double *g = tile32(d, r, r);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 31;
+  double v[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) v[q] = g[(wave + 2 * q) * S + col];
+  if (wave == 0 && lane < kT32) Rk[0][lane] = v[0];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kT32; ++k) {
+    const double rk = Rk[k & 1][col];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = fmin(v[q], readlane_f64(v[q], k) + rk);
+    if (k + 1 < kT32 && wave == ((k + 1) & 1) && lane < kT32) Rk[(k + 1) & 1][lane] = v[(k + 1) >> 1];
+    __syncthreads();
+  }
+  if (lane < kT32) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) g[(wave + 2 * q) * S + col] = v[q];
+  }
+}
+
+// C = min(C, A (x) B) on 32 x 32 tiles, one wave: thread t owns rows (t / 8) * 4 +
+// {0..3} and columns (t % 8) + 8 * {0..3}; k in LDS chunks of 16.  A tile that is its own
+// operand is read (staged) before any thread stores.
+__device__ __forceinline__ void tile_product32(double (&acc)[4][4], const FAP_GLOBAL double *a,
+                                               const FAP_GLOBAL double *b, int64_t S,
+                                               double (*As)[kT32 + 2], double (*Bs)[kT32],
+                                               int tr, int tc) {
+  const int t = threadIdx.x;
+  for (int k0 = 0; k0 < kT32; k0 += 16) {
+    if (k0 > 0) __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = t + 64 * q;
+      const int ai = e >> 4, ak = e & 15;
+      As[ak][ai] = a[ai * S + k0 + ak];
+      const int bk = e >> 5, bj = e & 31;
+      Bs[bk][bj] = b[(k0 + bk) * S + bj];
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < 16; ++k) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        av[q] = As[k][tr * 4 + q];
+        bv[q] = Bs[k][tc + 8 * q];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[q][c] = fmin(acc[q][c], av[q] + bv[c]);
+    }
+  }
+}
+
+// Phase 2 (kPhase 2, grid x = 2 * (nb - 1)) and phase 3 (kPhase 3, grid x = (nb - 1)^2);
+// the tiles of one part back to back on one XCD.
+template <int kPhase>
+__global__ __launch_bounds__(64) void fw32_tile_kernel(const Fw64Desc *__restrict__ descs, int r) {
+  const unsigned L = xcd_contiguous(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+  const int o = (int)(L % gridDim.x), di = (int)(L / gridDim.x);
+  const Fw64Desc d = descs[di];
+  const int others = d.nb - 1;
+  if (r >= d.nb || o >= (kPhase == 2 ? 2 * others : others * others)) return;
+  const int64_t S = (int64_t)d.nb * kT32;
+  int ci, cj, ai, aj, bi, bj;
+  if (kPhase == 2) {
+    if (o < others) {  // row panel: P* (x) T(r, j)
+      ci = r, cj = skip64(o, r), ai = r, aj = r, bi = ci, bj = cj;
+    } else {           // column panel: T(i, r) (x) P*
+      ci = skip64(o - others, r), cj = r, ai = ci, aj = cj, bi = r, bj = r;
+    }
+  } else {
+    ci = skip64(o / others, r), cj = skip64(o % others, r), ai = ci, aj = r, bi = r, bj = cj;
+  }
+  __shared__ __attribute__((aligned(16))) double As[16][kT32 + 2];
+  __shared__ __attribute__((aligned(16))) double Bs[16][kT32];
+  FAP_GLOBAL double *c = tile32(d, ci, cj);
+  const int tr = threadIdx.x >> 3, tc = threadIdx.x & 7;
+  double acc[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[q][e] = c[(tr * 4 + q) * S + tc + 8 * e];
+  tile_product32(acc, tile32(d, ai, aj), tile32(d, bi, bj), S, As, Bs, tr, tc);
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) c[(tr * 4 + q) * S + tc + 8 * e] = acc[q][e];
+}
+
+}  // namespace
+
+// The same rounds on 32 x 32 tiles (descriptors' nb counts 32-tiles).
+int launch_fw64_t32(const Fw64Desc *d_descs, int nd, int max_nb, hipStream_t st) {
+  for (int r = 0; r < max_nb; ++r) {
+    hipLaunchKernelGGL(fw32_phase1_kernel, dim3(nd), dim3(128), 0, st, d_descs, r);
+    FAP_CHECK_LAUNCH();
+    const int others = max_nb - 1;
+    if (others > 0) {
+      hipLaunchKernelGGL(fw32_tile_kernel<2>, dim3(2 * others, nd), dim3(64), 0, st, d_descs, r);
+      FAP_CHECK_LAUNCH();
+      hipLaunchKernelGGL(fw32_tile_kernel<3>, dim3(others * others, nd), dim3(64), 0, st, d_descs,
+                         r);
+      FAP_CHECK_LAUNCH();
+    }
+  }
+  return FAP_OK;
+}
+
 // Phase 2 reads the row tile it writes only through its own workgroup and the
 // pivot tile, which phase 2 does not write; phase 3 reads panel tiles only.
 int launch_fw64(const Fw64Desc *d_descs, int nd, int max_nb, hipStream_t st) {

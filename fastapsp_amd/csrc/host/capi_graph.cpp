@@ -331,15 +331,22 @@ void batch_items(fap_graph **graphs, int32_t count, std::vector<fap_batch_item> 
 }
 }  // namespace
 
-int fap_graph_batch_create(fap_batch **b, fap_graph **graphs, int32_t count, int32_t flags) {
+int fap_graph_batch_create_ex(fap_batch **b, fap_graph **graphs, int32_t count, int32_t flags,
+                              const int32_t *item_flags) {
   if (!b) return FAP_ERR_INVALID;
   return guarded([&] {
     std::vector<fap_batch_item> items;
     std::vector<std::vector<fap_part_desc>> parts;
     batch_items(graphs, count, items, parts);
+    if (item_flags)
+      for (int32_t i = 0; i < count; ++i) items[i].flags |= item_flags[i];
     const int s = fap_hip_batch_create(b, items.data(), count, flags);
     if (s != FAP_OK) throw fap::FapError(s, fap_hip_last_error());
   });
+}
+
+int fap_graph_batch_create(fap_batch **b, fap_graph **graphs, int32_t count, int32_t flags) {
+  return fap_graph_batch_create_ex(b, graphs, count, flags, nullptr);
 }
 
 int fap_graph_solve_batch(fap_graph **graphs, int32_t count, int32_t mode) {

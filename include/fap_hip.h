@@ -385,6 +385,9 @@ typedef struct fap_batch_item {
  * sub-matrix build keeps the later arc and its boundary rows the lightest
  * (fapGraph sets it from preCondition). */
 #define FAP_ITEM_BOUNDARY_SSSP 0x1
+/* Floyd-Warshall of this (float-weight) item's parts and boundary graph on 32 x 32 tiles
+ * (the per-graph form of FAP_BATCH_FW_TILE32; an autotuner knob per size class). */
+#define FAP_ITEM_FW_TILE32 0x2
 
 /* A prepared batch: the work tables of every part of every graph and optionally the
  * captured launch sequence (FAP_BATCH_GRAPH: fap_hip_batch_run replays a hipGraph).
@@ -420,6 +423,10 @@ typedef struct fap_batch fap_batch;
  * sequences run side by side, and a plain grid lets the other type's launches take
  * workgroup slots as merge units end. */
 #define FAP_BATCH_UNIT_GRID 0x400000
+/* Floyd-Warshall of the FP64 (float-weight) parts and boundary graphs on 32 x 32 tiles
+ * (matrices padded to multiples of 32) instead of 64 x 64: less padding for small parts,
+ * more rounds; an autotuner knob per size class.  Results are the same. */
+#define FAP_BATCH_FW_TILE32 0x800000
 int fap_hip_batch_create(fap_batch **b, const fap_batch_item *items, int32_t nitems,
                          int32_t flags);
 int fap_hip_batch_run(fap_batch *b, void *stream);
@@ -516,6 +523,10 @@ int fap_graph_edge_cut(fap_graph *g, double *edge_cut);
  * over the graphs' device views and their solve() result buffers, so the graphs'
  * getters return the batch results. */
 int fap_graph_batch_create(fap_batch **b, fap_graph **graphs, int32_t count, int32_t flags);
+/* The same with per-graph item flags (FAP_ITEM_*, OR-ed into each graph's item; NULL =
+ * none), e.g. the Floyd-Warshall tile the autotuner picked for each graph's size class. */
+int fap_graph_batch_create_ex(fap_batch **b, fap_graph **graphs, int32_t count, int32_t flags,
+                              const int32_t *item_flags);
 int fap_graph_solve_batch(fap_graph **graphs, int32_t count, int32_t mode);
 
 /* ---------------------------------------------------------------- serving */

@@ -136,7 +136,38 @@ def test_autotune_small_table():
                    for t in trials)
     assert AT.k_for(300, table) == table["K"][0] and AT.k_for(8000, table) == table["K"][-1]
     s = AT.settings_for([8000, 8100, 300], table)
-    assert s == {k: table[k][-1] for k in AT.KNOBS}
+    assert s == dict({k: table[k][-1] for k in AT.KNOBS if k != "fw_tile"},
+                     fw_tile=[table["fw_tile"][-1]] * 2 + [table["fw_tile"][0]])
+    assert all(t in (64, 32) for t in table["fw_tile"])
+
+
+def test_batch_fw_tile_32_equals_64():
+    """The FP64 Floyd-Warshall on 32 x 32 tiles (per graph, FAP_ITEM_FW_TILE32) gives the
+    same float32 results as on 64 x 64 tiles, for parts of 20-300 vertices (1-10 tiles),
+    mixed in one batch; both equal the float32 rounding of an FP64 Dijkstra."""
+    graphs = []
+    for i, (side, K) in enumerate(((12, 4), (20, 3), (30, 5), (40, 6))):
+        g = F.FapGraph.road_grid(side, side, 0.1, 0.02, 60 + i, K)
+        c = g.csr()
+        w = float_weights(c["row_offset"], c["col"], False, 60 + i)
+        g = F.FapGraph.from_csr(c["row_offset"], c["col"], w, False, K)
+        g.preCondition()
+        graphs.append(g)
+    tiles = [32, 64, 32, 64]
+    b = F.Batch(graphs, use_graph=False, fw_tile=tiles)
+    b.run()
+    first = [g.getAllDistance().copy() for g in graphs]
+    b.close()
+    b = F.Batch(graphs, use_graph=False, fw_tile=[64 if t == 32 else 32 for t in tiles])
+    b.run()
+    for g, a in zip(graphs, first):
+        assert np.array_equal(g.getAllDistance(), a)
+        csr, lay = g.csr(), g.layout()
+        want = O.exact_rows_f64(csr["row_offset"], csr["col"], csr["weight"], lay["st2ed"])
+        unreach = np.isinf(want)
+        assert np.array_equal(a == F.MAXVALUE, unreach)
+        assert np.array_equal(a[~unreach], want[~unreach].astype(np.float32))
+    b.close()
 
 
 def test_batch_inpart_bfs_and_floyd_warshall_agree():
