@@ -52,44 +52,37 @@ __device__ __forceinline__ int panel_tiles(const FwDesc &d, int r) {
   return min(kPanel, d.nb - kPanel * r);
 }
 
-// Phase 1: the (W x W) diagonal block, W = 64 * panel tiles, closed in LDS.  Each
-// thread owns 16 consecutive entries of one row.
+// Phase 1: the 64 x 64 diagonal tile closed by 256 threads.  Wave w holds rows w, w + 4,
+// ..., w + 60 in registers, lane j their column j.  At step k, column k (the entries
+// D[i][k] of a wave's rows) comes from lane k by readlane and row k from LDS, where the
+// wave owning row k published it after step k - 1 (two buffers, one barrier per step).
+// Row k and column k do not change during step k (D[k][k] = 0, or the padded row and
+// column are FAP_INF throughout), so this is the k-i-j loop.  The earlier form (16
+// consecutive entries per thread, the tile in LDS, three dependent LDS accesses per step)
+// took ~41 us per tile on MI355X, the bound of a single part's Floyd-Warshall (one tile
+// per round).
+static_assert(kPanel == 1, "phase 1 closes one 64-tile");
 __global__ __launch_bounds__(kP1Threads) void fw_phase1_kernel(const FwDesc *__restrict__ descs, int r) {
   const FwDesc d = descs[blockIdx.y];
-  if (kPanel * r >= d.nb) return;
-  const int W = kT * panel_tiles(d, r);
-  __shared__ int D[kPanel * kT][kPanel * kT];
-  gint *g = tile_ptr(d, kPanel * r, kPanel * r);
-  constexpr int kRowThreads = kPanel * kT / 16;
-  const int row = threadIdx.x / kRowThreads, col0 = (threadIdx.x % kRowThreads) * 16;
-  const bool mine = row < W && col0 < W;
-  if (mine) {
-    const gv4i *src = reinterpret_cast<const gv4i *>(g + (int64_t)row * d.ld + col0);
+  if (r >= d.nb) return;
+  __shared__ int Rk[2][kT];
+  gint *g = tile_ptr(d, r, r);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int v[16];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) *reinterpret_cast<v4i *>(&D[row][col0 + 4 * c]) = src[c];
-  }
-  for (int k = 0; k < W; ++k) {
-    __syncthreads();
-    // Row k and column k do not change during step k (D[k][k] = 0), so one
-    // barrier per step orders the in-place update.
-    if (mine) {
-      const int dik = D[row][k];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const v4i rk = *reinterpret_cast<const v4i *>(&D[k][col0 + 4 * c]);
-        v4i v = *reinterpret_cast<v4i *>(&D[row][col0 + 4 * c]);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) v[e] = min(v[e], dik + rk[e]);
-        *reinterpret_cast<v4i *>(&D[row][col0 + 4 * c]) = v;
-      }
-    }
-  }
+  for (int q = 0; q < 16; ++q) v[q] = g[(int64_t)(wave + 4 * q) * d.ld + lane];
+  if (wave == 0) Rk[0][lane] = v[0];
   __syncthreads();
-  if (mine) {
-    gv4i *dst = reinterpret_cast<gv4i *>(g + (int64_t)row * d.ld + col0);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) dst[c] = *reinterpret_cast<const v4i *>(&D[row][col0 + 4 * c]);
+  for (int k = 0; k < kT; ++k) {
+    const int rk = Rk[k & 1][lane];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = min(v[q], __builtin_amdgcn_readlane(v[q], k) + rk);
+    if (k + 1 < kT && wave == ((k + 1) & 3)) Rk[(k + 1) & 1][lane] = v[(k + 1) >> 2];
+    __syncthreads();
   }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) g[(int64_t)(wave + 4 * q) * d.ld + lane] = v[q];
 }
 
 // Phases 2 and 3: one wave64 per 64x64 tile product, 8 x 8 accumulators per lane
