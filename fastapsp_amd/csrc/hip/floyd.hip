@@ -206,6 +206,84 @@ __global__ __launch_bounds__(64) void fw_phase3_kernel(const FwDesc *__restrict_
   store_acc8(acc, c, d.ld, tr, tc);
 }
 
+// Phases 2 and 3 with four waves per tile (thread t: rows (t / 16) * 4 + {0..3},
+// columns (t % 16) * 4 + {0..3}), for launches of few tiles: a single part's
+// Floyd-Warshall has (nb - 1)^2 tiles per round, too few to fill the device one wave per
+// tile, so each round's latency is that of one wave's 64^3-step product.
+constexpr int kWideThreads = 256;
+constexpr int64_t kWideTiles = 1024;  // phase-3 tiles per round below which the wide form runs
+
+__device__ __forceinline__ void load_a_chunk_w(int (*As)[kT], const gint *g, int64_t ld, int k0) {
+  const int i = threadIdx.x >> 2, kq = (threadIdx.x & 3) * 8;
+  const gv4i *src = reinterpret_cast<const gv4i *>(g + (int64_t)i * ld + k0 + kq);
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const v4i t = src[c];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) As[kq + 4 * c + e][i] = t[e];
+  }
+}
+
+__device__ __forceinline__ void load_b_chunk_w(int (*Bs)[kT], const gint *g, int64_t ld, int k0) {
+  const int k = threadIdx.x >> 3, c8 = (threadIdx.x & 7) * 8;
+  const gv4i *src = reinterpret_cast<const gv4i *>(g + (int64_t)(k0 + k) * ld + c8);
+  *reinterpret_cast<v4i *>(&Bs[k][c8]) = src[0];
+  *reinterpret_cast<v4i *>(&Bs[k][c8 + 4]) = src[1];
+}
+
+__device__ __forceinline__ void tile_product_w(int (&acc)[4][4], const gint *a, const gint *b,
+                                               int64_t ld, int (*As)[kT], int (*Bs)[kT], int tr,
+                                               int tc) {
+  for (int k0 = 0; k0 < kT; k0 += kKc) {
+    if (k0 > 0) __syncthreads();
+    load_a_chunk_w(As, a, ld, k0);
+    load_b_chunk_w(Bs, b, ld, k0);
+    __syncthreads();
+    mp_accumulate<4, 4, kKc>(acc, &As[0][0], kT, &Bs[0][0], kT, tr * 4, 0, tc * 4, 0);
+  }
+}
+
+// kPhase 2: 2 (nb - 1) panel tiles (row panel first); 3: (nb - 1)^2 tiles.  A tile that
+// is its own operand (phase 2) is fully staged before any thread stores.
+template <int kPhase>
+__global__ __launch_bounds__(kWideThreads) void fw_wide_kernel(const FwDesc *__restrict__ descs,
+                                                               int r) {
+  const FwDesc d = descs[blockIdx.y];
+  const int p = r, others = d.nb - 1;
+  const int t = blockIdx.x;
+  if (r >= d.nb || t >= (kPhase == 2 ? 2 * others : others * others)) return;
+  gint *c;
+  const gint *a, *b;
+  if (kPhase == 2) {
+    const bool is_row = t < others;
+    const int o = skip_panel(is_row ? t : t - others, p, 1);
+    c = is_row ? tile_ptr(d, p, o) : tile_ptr(d, o, p);
+    a = is_row ? tile_ptr(d, p, p) : c;
+    b = is_row ? c : tile_ptr(d, p, p);
+  } else {
+    const int i = skip_panel(t / others, p, 1), j = skip_panel(t % others, p, 1);
+    c = tile_ptr(d, i, j);
+    a = tile_ptr(d, i, p);
+    b = tile_ptr(d, p, j);
+  }
+  __shared__ int As[kKc][kT];
+  __shared__ int Bs[kKc][kT];
+  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+  int acc[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const v4i x = *reinterpret_cast<const gv4i *>(c + (int64_t)(tr * 4 + q) * d.ld + tc * 4);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[q][e] = x[e];
+  }
+  tile_product_w(acc, a, b, d.ld, As, Bs, tr, tc);
+  __syncthreads();  // (phase 2) every read of the tile precedes the first store
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    *reinterpret_cast<gv4i *>(c + (int64_t)(tr * 4 + q) * d.ld + tc * 4) =
+        v4i{acc[q][0], acc[q][1], acc[q][2], acc[q][3]};
+}
+
 // max_nb: the largest tile count of the descriptors; rounds cover kPanel tiles.
 int launch_fw_round(const FwDesc *d_descs, int ndesc, int max_nb, int r, hipStream_t stream) {
   if (ndesc <= 0 || kPanel * r >= max_nb) return FAP_OK;
@@ -217,7 +295,14 @@ int launch_fw_round(const FwDesc *d_descs, int ndesc, int max_nb, int r, hipStre
   // A part with fewer tiles may have a full panel where the largest has a short one.
   if (wt < kPanel) others = std::max(others, max_nb - 1 - kPanel);
   const int p2 = 2 * kPanel * others;
-  if (others > 0) {  // a single-panel matrix has no tile outside the pivot panel
+  if (others > 0 && (int64_t)ndesc * others * others < kWideTiles) {  // few tiles
+    hipLaunchKernelGGL(fw_wide_kernel<2>, dim3(p2, ndesc), dim3(kWideThreads), 0, stream,
+                       d_descs, r);
+    FAP_CHECK_LAUNCH();
+    hipLaunchKernelGGL(fw_wide_kernel<3>, dim3(others * others, ndesc), dim3(kWideThreads), 0,
+                       stream, d_descs, r);
+    FAP_CHECK_LAUNCH();
+  } else if (others > 0) {  // a single-panel matrix has no tile outside the pivot panel
     hipLaunchKernelGGL(fw_phase2_kernel, dim3(p2, ndesc), dim3(64), 0, stream, d_descs, r);
     FAP_CHECK_LAUNCH();
     hipLaunchKernelGGL(fw_phase3_kernel, dim3(others * others, ndesc), dim3(64), 0, stream,
