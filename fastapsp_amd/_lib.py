@@ -36,6 +36,7 @@ FAP_BATCH_NO_PACKED = 0x100000
 FAP_BATCH_NO_BFS = 0x200000
 FAP_BATCH_UNIT_GRID = 0x400000
 FAP_BATCH_FW_TILE32 = 0x800000
+FAP_BATCH_NO_NESTED = 0x1000000
 FAP_ITEM_FW_TILE32 = 0x2
 
 # Every exported symbol of include/fap_hip.h: name -> (restype, argtypes).
@@ -125,6 +126,7 @@ SIGNATURES = {
                                           _vp, _i64, _i32, _vp]),
     "fap_hip_batch_create": (C.c_int, [C.POINTER(_vp), C.POINTER(FapBatchItem), _i32, _i32]),
     "fap_hip_batch_run": (C.c_int, [_vp, _vp]),
+    "fap_hip_batch_nested_parts": (C.c_int, [_vp]),
     "fap_hip_batch_destroy": (C.c_int, [_vp]),
     "fap_hip_solve_batch": (C.c_int, [C.POINTER(FapBatchItem), _i32, _i32, _vp]),
     "fap_hip_event_query": (C.c_int, [_vp, _pi32]),

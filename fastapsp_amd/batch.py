@@ -36,11 +36,14 @@ class Batch:
     by bit-parallel BFS, 0 = by Floyd-Warshall; unit_grid: 1 = the merge as a plain grid
     of one workgroup per unit, 0 = the persistent work queue; fw_tile: Floyd-Warshall tile
     edge of the float-weight (FP64) parts and boundary graphs, 64 or 32, for every graph
-    or a list with one entry per graph (the autotuner's choice per size class)."""
+    or a list with one entry per graph (the autotuner's choice per size class); nested:
+    1 = in-part distances of weighted parts of 160+ vertices by the nested closure
+    (sub-parts of at most 64 vertices, their crossing graph, two min-plus passes), 0 = by
+    Floyd-Warshall on the whole part (the same matrices)."""
 
     def __init__(self, graphs: Sequence, use_graph: bool = True, per_cu: int = 0,
                  src_rows: int = 0, merge_rows: int = 0, packed: int = 1, bfs: int = 1,
-                 unit_grid: int = 0, fw_tile=64):
+                 unit_grid: int = 0, fw_tile=64, nested: int = 1):
         self.graphs = list(graphs)
         for g in self.graphs:
             if not getattr(g, "_preconditioned", False):
@@ -60,7 +63,8 @@ class Batch:
                                                 | (_ROWS_CODE[int(merge_rows)] << 18)
                                                 | (0 if packed else L.FAP_BATCH_NO_PACKED)
                                                 | (0 if bfs else L.FAP_BATCH_NO_BFS)
-                                                | (L.FAP_BATCH_UNIT_GRID if unit_grid else 0),
+                                                | (L.FAP_BATCH_UNIT_GRID if unit_grid else 0)
+                                                | (0 if nested else L.FAP_BATCH_NO_NESTED),
                                                 item_flags),
                 "batch create")
         self._h = h.value
@@ -70,6 +74,13 @@ class Batch:
         L.check(L.load().fap_hip_batch_run(self._h, None), "batch run")
         if sync:
             L.check(L.load().fap_hip_stream_sync(None), "batch sync")
+
+    def nested_parts(self) -> int:
+        """Parts whose in-part distances come from the nested closure."""
+        r = L.load().fap_hip_batch_nested_parts(self._h)
+        if r < 0:
+            L.check(r, "batch nested parts")
+        return r
 
     def close(self) -> None:
         if self._h is not None:

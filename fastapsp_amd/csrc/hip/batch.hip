@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "fap_internal.h"
+#include "graph.h"
 #include "fw_desc.h"
 #include "gmem.h"
 #include "mp_tile.h"
@@ -93,6 +94,7 @@ struct BPart {
   int bs;                        // boundary-graph index of its first boundary vertex
   int row_begin;                 // first build row (prefix of S)
   int bfs;                       // in-part distances by bit-parallel BFS (no build, no FW)
+  int nested;                    // in-part distances by the nested closure (no build, no FW)
 };
 
 struct BRowUnit {                // stage 5: boundary-graph rows r0 .. of graph parts[q].g,
@@ -261,7 +263,7 @@ __global__ __launch_bounds__(256) void bbuild_kernel(const BGraph *__restrict__ 
   if (row >= total_rows) return;
   const int pi = upper_index(np, row, [&](int k) { return parts[k].row_begin; });
   const BPart p = parts[pi];
-  if (p.bfs) return;  // the BFS writes the whole matrix
+  if (p.bfs || p.nested) return;  // the BFS or the nested closure writes the matrix
   const BGraph g = graphs[p.g];
   const int i = row - p.row_begin;
   FAP_GLOBAL T *D = gptr(static_cast<T *>(p.D)) + (int64_t)i * p.S;
@@ -450,47 +452,48 @@ __device__ __forceinline__ void mp_stage(const MpArgs<T> &m, T (*As)[kMWaves * R
   }
 }
 
-template <typename T, int RB, bool kPacked>
-__device__ __forceinline__ void mp_init(T (&acc)[RB][kPacked ? 2 : 4]) {
+template <typename T, int RB, bool kPacked, int kE = 4>
+__device__ __forceinline__ void mp_init(T (&acc)[RB][kPacked ? 2 : kE]) {
 #pragma unroll
   for (int r = 0; r < RB; ++r)
 #pragma unroll
-    for (int e = 0; e < (kPacked ? 2 : 4); ++e) acc[r][e] = kPacked ? (T)kPk16Top : Dist<T>::inf();
+    for (int e = 0; e < (kPacked ? 2 : kE); ++e) acc[r][e] = kPacked ? (T)kPk16Top : Dist<T>::inf();
 }
 
 // acc[r][.] = min(acc, min_{k < kc} As[k][wave*RB + r] + B[k0 + k][col]) for the wave's
-// RB rows and the lane's four columns col0 + l + 64e.  kPacked (int32, unit weights,
+// RB rows and the lane's kE (four unless noted) columns col0 + l + 64e.  kPacked (int32, unit weights,
 // n <= kCap16): two columns per register, a step on two entries one v_pk_add_u16 and
 // half a v_pk_minimum3_f16 (pk16.h); every finite distance is below the cap, so a
 // result at the cap is unreachable.  The four waves read the same B rows (L1).
-template <typename T, int RB, bool kPacked>
+template <typename T, int RB, bool kPacked, int kE = 4>
 __device__ __forceinline__ void mp_chunk(const MpArgs<T> &m, const T (*As)[kMWaves * RB + kAPad<T>],
-                                         int k0, int kc, int col0, T (&acc)[RB][kPacked ? 2 : 4],
+                                         int k0, int kc, int col0, T (&acc)[RB][kPacked ? 2 : kE],
                                          int tid) {
+  static_assert(!kPacked || kE == 4, "packed units hold four columns per lane");
   constexpr int kLdA = kMWaves * RB + kAPad<T>;
   const int wave = tid >> 6, lane = tid & 63;
   const FAP_GLOBAL T *B = m.B;
-  int col[4];
+  int col[kE];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) col[e] = min(col0 + e * 64 + lane, m.colmax);
+  for (int e = 0; e < kE; ++e) col[e] = min(col0 + e * 64 + lane, m.colmax);
   const T *Aw = &As[0][wave * RB];
   if constexpr (sizeof(T) == 8) {
     // FP64 has no three-operand minimum: one k per step (fewer live registers), row
     // k + 1 in flight while row k is used (two register buffers).
-    auto load_b = [&](int k, T (&x)[4]) {
+    auto load_b = [&](int k, T (&x)[kE]) {
       const FAP_GLOBAL T *rp = B + (int64_t)(k0 + k) * m.ldb;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) x[e] = rp[col[e]];
+      for (int e = 0; e < kE; ++e) x[e] = rp[col[e]];
     };
-    auto step = [&](int k, const T (&bv)[4]) {
+    auto step = [&](int k, const T (&bv)[kE]) {
 #pragma unroll
       for (int r = 0; r < RB; ++r) {
         const T a = Aw[k * kLdA + r];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) acc[r][e] = Dist<T>::mn(acc[r][e], a + bv[e]);
+        for (int e = 0; e < kE; ++e) acc[r][e] = Dist<T>::mn(acc[r][e], a + bv[e]);
       }
     };
-    T b0[4], b1[4];
+    T b0[kE], b1[kE];
     load_b(0, b0);
     int k = 0;
     for (; k + 1 < kc; k += 2) {
@@ -506,18 +509,18 @@ __device__ __forceinline__ void mp_chunk(const MpArgs<T> &m, const T (*As)[kMWav
     // Rows k, k + 1 of the chunk as one pair (the three-operand minimum), the next
     // pair in flight while this one is used; rows past kc read as +inf (row kc of As
     // is +inf when kc is odd, and a pair past the end loads row kc - 1 again).
-    auto load_b = [&](int k, T (&x)[4]) {
+    auto load_b = [&](int k, T (&x)[kE]) {
       const FAP_GLOBAL T *rp = B + (int64_t)(k0 + min(k, kc - 1)) * m.ldb;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) x[e] = rp[col[e]];
+      for (int e = 0; e < kE; ++e) x[e] = rp[col[e]];
     };
-    T n0[4], n1[4];
+    T n0[kE], n1[kE];
     load_b(0, n0);
     load_b(1, n1);
     for (int k = 0; k < kc; k += 2) {
-      T b0[4], b1[4];
+      T b0[kE], b1[kE];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
+      for (int e = 0; e < kE; ++e) {
         b0[e] = n0[e];
         b1[e] = n1[e];
       }
@@ -554,7 +557,7 @@ __device__ __forceinline__ void mp_chunk(const MpArgs<T> &m, const T (*As)[kMWav
 #pragma unroll
         for (int r = 0; r < RB; ++r)
 #pragma unroll
-          for (int e = 0; e < 4; ++e)
+          for (int e = 0; e < kE; ++e)
             acc[r][e] = Dist<T>::mn3(acc[r][e], a0[r] + b0[e], a1[r] + b1[e]);
       }
     }
@@ -562,16 +565,16 @@ __device__ __forceinline__ void mp_chunk(const MpArgs<T> &m, const T (*As)[kMWav
 }
 
 // One unit: acc over every k chunk, then epi(acc) (after the workgroup's last LDS read).
-template <typename T, int RB, bool kPacked, typename Epi>
+template <typename T, int RB, bool kPacked, int kE = 4, typename Epi>
 __device__ __forceinline__ void mp_unit(const MpArgs<T> &m, T (*As)[kMWaves * RB + kAPad<T>],
                                         int tid, Epi &&epi) {
-  T acc[RB][kPacked ? 2 : 4];
-  mp_init<T, RB, kPacked>(acc);
+  T acc[RB][kPacked ? 2 : kE];
+  mp_init<T, RB, kPacked, kE>(acc);
   for (int k0 = 0; k0 < m.kdim; k0 += kKc) {
     const int kc = min(kKc, m.kdim - k0);
     mp_stage<T, RB, kPacked>(m, As, k0, kc, tid);
     __syncthreads();
-    mp_chunk<T, RB, kPacked>(m, As, k0, kc, m.col0, acc, tid);
+    mp_chunk<T, RB, kPacked, kE>(m, As, k0, kc, m.col0, acc, tid);
     __syncthreads();
   }
   epi(acc);
@@ -745,6 +748,182 @@ __global__ __launch_bounds__(256) void brows_kernel(const BGraph *__restrict__ g
     }
   }
   brows_unit<T, RB, false>(g, q, un.r0, un.cb, brow, As, threadIdx.x);
+}
+
+// ---------------------------------------------------------------- 2': nested closure
+// In-part distances of a weighted part of a full item (distances from its own arcs
+// alone) with at least kNestMinS vertices, without Floyd-Warshall on its s x s matrix
+// (s^3 steps).  The host cuts the part into sub-parts of at most kNestT vertices
+// (multilevel partitioner) and orders each sub-part's vertices crossing-first, where a
+// crossing vertex has an in-part arc to or from another sub-part.  Then:
+//   a  every sub-part's kNestT x kNestT matrix from its arcs (diagonal 0), closed by one
+//      Floyd-Warshall tile: w_q, distances inside sub-part q;
+//   b  the crossing graph X of the part: its crossing vertices, the closed blocks
+//      w_q[crossing][crossing] and the arcs between sub-parts; closed by the batched
+//      Floyd-Warshall: X* = in-part distances between crossing vertices;
+//   c  crossing rows: d(x, v) = min_{y crossing in q(v)} X*(x, y) + w_q(v)(y, v)
+//      (the last entry into v's sub-part; y = x covers paths that never leave it);
+//   d  every row: d(i, v) = min(w_q(i)(i, v) [v in q(i)],
+//                               min_{x crossing in q(i)} w_q(i)(i, x) + d(x, v))
+//      (the first exit from i's sub-part), stored at (sigma(i), sigma(v)) of the part's
+//      matrix, sigma = sub-part position -> the part's own vertex order.
+// Each is an exact min over the same in-part paths as Floyd-Warshall's closure of the
+// same arcs (parallel arcs: the later one, as in the build), so the matrices are equal
+// entry for entry.  Work for a 530-vertex road part: about 3e7 steps against 1.9e8.
+constexpr int kNestT = 64;      // largest sub-part: one Floyd-Warshall tile
+template <typename T>
+constexpr int kNestRB = sizeof(T) == 8 ? 8 : 16;  // rows per wave of the part-row units
+constexpr int kNestMinS = 160;  // smallest part taking the nested closure
+
+struct NPart {
+  void *D;         // the part's matrix (S x S), as for Floyd-Warshall
+  void *X;         // crossing graph (SX x SX)
+  void *R;         // crossing rows (nx x ldr, columns in sub-part positions)
+  int64_t ldr;
+  int S, s, nx, SX;
+  int pos0, pad;   // first entry of the part in sigma / aoff
+};
+
+struct NSub {
+  void *W;         // closed sub-part matrix (kNestT x kNestT)
+  int part;        // index into NPart
+  int st, n, b;    // first position, vertices, crossing vertices (positions st .. st+b-1)
+  int x0;          // crossing-graph index of its first crossing vertex
+  int row0, xrow0; // first row in the build / crossing-graph row numbering
+};
+
+struct NUnit {
+  int sub, rb, cb, pad;
+};
+
+// a: row `a` of sub-part k's matrix.  acode >= 0: arc to sub-part column acode; < 0: to
+// crossing index -1 - acode (another sub-part).  Arcs are unique per (tail, head).
+template <typename T>
+__global__ __launch_bounds__(256) void nbuild_kernel(const NPart *__restrict__ parts,
+                                                     const NSub *__restrict__ subs, int nsubs,
+                                                     const int *__restrict__ aoff,
+                                                     const int *__restrict__ acode,
+                                                     const T *__restrict__ aw, int total_rows) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + wave;
+  if (row >= total_rows) return;
+  const NSub q = subs[upper_index(nsubs, row, [&](int k) { return subs[k].row0; })];
+  const int a = row - q.row0;
+  const int gp = parts[q.part].pos0 + q.st + a;
+  T val = Dist<T>::inf();
+  const int e1 = aoff[gp + 1];
+  for (int e = aoff[gp]; e < e1; ++e)  // uniform
+    if (acode[e] == lane) val = aw[e];
+  if (lane == a) val = T(0);
+  if (lane < q.n) gptr(static_cast<T *>(q.W))[(int64_t)a * kNestT + lane] = val;
+}
+
+// b: row of crossing vertex c of sub-part k in its part's crossing graph.
+template <typename T>
+__global__ __launch_bounds__(256) void nxgraph_kernel(const NPart *__restrict__ parts,
+                                                      const NSub *__restrict__ subs, int nsubs,
+                                                      const int *__restrict__ aoff,
+                                                      const int *__restrict__ acode,
+                                                      const T *__restrict__ aw, int total_rows) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + wave;
+  if (row >= total_rows) return;
+  const NSub q = subs[upper_index(nsubs, row, [&](int k) { return subs[k].xrow0; })];
+  const NPart P = parts[q.part];
+  const int c = row - q.xrow0, g = q.x0 + c;
+  const int gp = P.pos0 + q.st + c;
+  const int e0 = aoff[gp], e1 = aoff[gp + 1];
+  const FAP_GLOBAL T *wr = gptr(static_cast<const T *>(q.W)) + (int64_t)c * kNestT;
+  FAP_GLOBAL T *xr = gptr(static_cast<T *>(P.X)) + (int64_t)g * P.SX;
+  for (int j = lane; j < P.nx; j += 64) {
+    T val = (j >= q.x0 && j < q.x0 + q.b) ? wr[j - q.x0] : Dist<T>::inf();
+    for (int e = e0; e < e1; ++e)  // uniform
+      if (acode[e] == -1 - j) val = aw[e];
+    if (j == g) val = T(0);
+    xr[j] = val;
+  }
+}
+
+// c: crossing rows r0 .. r0 + 4*RB - 1 of one part against the columns of one sub-part
+// (one column per lane: sub-parts hold at most 64 vertices).
+template <typename T, int RB>
+__global__ __launch_bounds__(256) void nxrows_kernel(const NPart *__restrict__ parts,
+                                                     const NSub *__restrict__ subs,
+                                                     const NUnit *__restrict__ units) {
+  constexpr int kRows = kMWaves * RB;
+  __shared__ __attribute__((aligned(16))) T As[kKc][kRows + kAPad<T>];
+  const NUnit un = units[blockIdx.x];
+  const NSub q = subs[un.sub];
+  const NPart P = parts[q.part];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int r0 = un.rb * kRows;
+  MpArgs<T> m;
+  m.A = gptr(static_cast<const T *>(P.X)) + (int64_t)r0 * P.SX + q.x0;
+  m.lda = P.SX;
+  m.rows = P.nx - r0;
+  m.kdim = q.b;
+  m.B = gptr(static_cast<const T *>(q.W));
+  m.ldb = kNestT;
+  m.col0 = 0;
+  m.colmax = kNestT - 1;
+  FAP_GLOBAL T *R = gptr(static_cast<T *>(P.R));
+  mp_unit<T, RB, false, 1>(m, As, tid, [&](const T (&acc)[RB][1]) {
+    if (lane >= q.n) return;
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int g = r0 + wave * RB + r;
+      if (g < P.nx) R[(int64_t)g * P.ldr + q.st + lane] = acc[r][0];
+    }
+  });
+}
+
+// d: rows r0 .. of sub-part q against positions cb * 256 .. of its part.
+template <typename T, int RB>
+__global__ __launch_bounds__(256) void nrows_kernel(const NPart *__restrict__ parts,
+                                                    const NSub *__restrict__ subs,
+                                                    const NUnit *__restrict__ units,
+                                                    const int *__restrict__ sigma) {
+  constexpr int kRows = kMWaves * RB;
+  __shared__ __attribute__((aligned(16))) T As[kKc][kRows + kAPad<T>];
+  const NUnit un = units[blockIdx.x];
+  const NSub q = subs[un.sub];
+  const NPart P = parts[q.part];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int r0 = un.rb * kRows;
+  const FAP_GLOBAL T *W = gptr(static_cast<const T *>(q.W));
+  MpArgs<T> m;
+  m.A = W + (int64_t)r0 * kNestT;
+  m.lda = kNestT;
+  m.rows = q.n - r0;
+  m.kdim = q.b;
+  m.B = gptr(static_cast<const T *>(P.R)) + (int64_t)q.x0 * P.ldr;
+  m.ldb = P.ldr;
+  m.col0 = un.cb * kMCols;
+  m.colmax = INT32_MAX;  // crossing rows are padded with +inf to whole column blocks
+  FAP_GLOBAL T *D = gptr(static_cast<T *>(P.D));
+  mp_unit<T, RB, false>(m, As, tid, [&](const T (&acc)[RB][4]) {
+    int dcol[4], own[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int pos = un.cb * kMCols + e * 64 + lane;
+      dcol[e] = pos < P.s ? sigma[P.pos0 + pos] : -1;
+      own[e] = pos >= q.st && pos < q.st + q.n ? pos - q.st : -1;
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int a = r0 + wave * RB + r;
+      if (a >= q.n) continue;
+      FAP_GLOBAL T *drow = D + (int64_t)sigma[P.pos0 + q.st + a] * P.S;
+      const FAP_GLOBAL T *wrow = W + (int64_t)a * kNestT;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (dcol[e] < 0) continue;
+        T val = acc[r][e];
+        if (own[e] >= 0) val = Dist<T>::mn(val, wrow[own[e]]);
+        drow[dcol[e]] = val;
+      }
+    }
+  });
 }
 
 // Units arrive in launch slots: slot 8 * i + x holds the i-th unit of XCD x (or none),
@@ -990,6 +1169,17 @@ struct fap_batch_kind {
   BMerge *merge = nullptr;
   BSrc *srcs = nullptr;
   void *fw_descs = nullptr, *gb_descs = nullptr;
+  // Nested closure: parts, sub-parts, build rows (sum of sub-part sizes), crossing-graph
+  // rows, product units (crossing rows, part rows), crossing-graph FW descriptors (FP64:
+  // 64-tile ones, then 32-tile ones).
+  int nnest = 0, nnsub = 0, nest_rows = 0, nx_rows = 0, nu3 = 0, nu4 = 0;
+  int nx64 = 0, nx32 = 0, nx_max_nb = 0, nx_max_nb32 = 0;
+  NPart *npart = nullptr;
+  NSub *nsub = nullptr;
+  int *naoff = nullptr, *nacode = nullptr, *nsigma = nullptr;
+  void *naw = nullptr;
+  NUnit *nu3d = nullptr, *nu4d = nullptr;
+  void *nw_descs = nullptr, *nx_descs = nullptr;
 };
 
 struct fap_batch {
@@ -1000,6 +1190,7 @@ struct fap_batch {
   bool bfs = true;     // in-part BFS where exact
   bool unit_grid = false;  // merge as a plain grid instead of the persistent queue
   bool fw32 = false;       // FP64 Floyd-Warshall on 32 x 32 tiles
+  bool nested = true;      // nested closure of large weighted parts
   bool use_graph = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1025,6 +1216,33 @@ int launch_fw_chunks(void *descs, int nd, int max_nb, int tile, hipStream_t st) 
   return FAP_OK;
 }
 
+// Nested closure of the kind's large weighted parts (stages a-d above).
+template <typename T>
+int launch_nested(const fap_batch_kind &q, hipStream_t st) {
+  hipLaunchKernelGGL(nbuild_kernel<T>, dim3((q.nest_rows + 3) / 4), dim3(256), 0, st, q.npart,
+                     q.nsub, q.nnsub, q.naoff, q.nacode, static_cast<const T *>(q.naw),
+                     q.nest_rows);
+  FAP_CHECK_LAUNCH();
+  FAP_TRY(launch_fw_chunks<T>(q.nw_descs, q.nnsub, 1, kBT, st));
+  if (q.nx_rows > 0) {
+    hipLaunchKernelGGL(nxgraph_kernel<T>, dim3((q.nx_rows + 3) / 4), dim3(256), 0, st, q.npart,
+                       q.nsub, q.nnsub, q.naoff, q.nacode, static_cast<const T *>(q.naw),
+                       q.nx_rows);
+    FAP_CHECK_LAUNCH();
+    FAP_TRY(launch_fw_chunks<T>(q.nx_descs, q.nx64, q.nx_max_nb, kBT, st));
+    if (q.nx32 > 0)
+      FAP_TRY(launch_fw_chunks<T>(static_cast<Fw64Desc *>(q.nx_descs) + q.nx64, q.nx32,
+                                  q.nx_max_nb32, 32, st));
+    hipLaunchKernelGGL((nxrows_kernel<T, 16>), dim3(q.nu3), dim3(256), 0, st, q.npart, q.nsub,
+                       q.nu3d);
+    FAP_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL((nrows_kernel<T, kNestRB<T>>), dim3(q.nu4), dim3(256), 0, st, q.npart, q.nsub,
+                     q.nu4d, q.nsigma);
+  FAP_CHECK_LAUNCH();
+  return FAP_OK;
+}
+
 template <typename T>
 int launch_kind(const fap_batch *b, const fap_batch_kind &q, int *counter, hipStream_t st) {
   if (q.nparts == 0) return FAP_OK;
@@ -1042,6 +1260,7 @@ int launch_kind(const fap_batch *b, const fap_batch_kind &q, int *counter, hipSt
       FAP_TRY(launch_fw_chunks<T>(static_cast<Fw64Desc *>(q.fw_descs) + q.nfw64, q.nfw32,
                                   q.max_nb32, 32, st));
   }
+  if (q.nnest > 0) FAP_TRY(launch_nested<T>(q, st));
   if (q.nip2 > 0) FAP_TRY(launch_inpart_bfs(q.ip, q.nip2, q.gip2, 1024, st));
   if (q.nip4 > 0) FAP_TRY(launch_inpart_bfs(q.ip + q.nip2, q.nip4, q.gip4, kInpartBfsMaxS, st));
   if (q.ngb > 0) {
@@ -1109,6 +1328,120 @@ int batch_launch(fap_batch *b, hipStream_t st) {
   return FAP_OK;
 }
 
+// Host copy of one graph's CSR and layout (the nested closure's planning input).
+struct HostCsr {
+  std::vector<int> row_off, col, graph_id, st2ed, ed2st, w;
+  std::vector<float> wf;
+};
+
+int fetch_csr(const fap_graph_dev &d, HostCsr &h) {
+  auto get = [](auto &v, const void *src, int64_t count) {
+    v.resize(count);
+    return count == 0 || hipMemcpy(v.data(), src, count * sizeof(v[0]), hipMemcpyDeviceToHost) ==
+                             hipSuccess;
+  };
+  const bool ok = get(h.row_off, d.d_row_offset, (int64_t)d.n + 1) && get(h.col, d.d_col, d.m) &&
+                  get(h.graph_id, d.d_graph_id, d.n) && get(h.st2ed, d.d_st2ed, d.n) &&
+                  get(h.ed2st, d.d_ed2st, d.n) && (!d.d_w || get(h.w, d.d_w, d.m)) &&
+                  (!d.d_wf || get(h.wf, d.d_wf, d.m));
+  if (!ok) {
+    (void)hipGetLastError();
+    set_error("fap_hip_batch_create: graph copy to the host failed");
+    return FAP_ERR_HIP;
+  }
+  return FAP_OK;
+}
+
+// One part's nested-closure plan: sub-parts of at most kNestT vertices, positions
+// (sub-part by sub-part, crossing vertices first), and every in-part arc coded from its
+// tail's position (see nbuild_kernel).
+struct NestLocal {
+  std::vector<int> sigma;              // position -> the part's vertex (0 .. s-1)
+  std::vector<int> st, n, b, x0;       // per sub-part
+  std::vector<int> aoff{0}, acode;     // arcs per position
+  std::vector<double> aw;
+  int nx = 0;                          // crossing vertices
+};
+
+bool nest_local(const HostCsr &h, int off, int s, int part_id, uint64_t seed, NestLocal &L) {
+  // In-part arcs in CSR order; a later arc between the same ends replaces the earlier
+  // one (the build's rule), loops are dropped (the diagonal is 0).
+  std::vector<std::vector<std::pair<int, double>>> out(s);
+  for (int i = 0; i < s; ++i) {
+    const int v = h.st2ed[off + i];
+    for (int e = h.row_off[v]; e < h.row_off[v + 1]; ++e) {
+      const int u = h.col[e];
+      if (h.graph_id[u] != part_id) continue;
+      const int j = h.ed2st[u] - off;
+      if (j == i || j < 0 || j >= s) continue;
+      const double w = !h.wf.empty() ? (double)h.wf[e] : !h.w.empty() ? (double)h.w[e] : 1.0;
+      auto &l = out[i];
+      auto it = std::find_if(l.begin(), l.end(), [&](const std::pair<int, double> &x) {
+        return x.first == j;
+      });
+      if (it != l.end()) it->second = w;
+      else l.push_back({j, w});
+    }
+  }
+  fap::Csr c;
+  c.n = s;
+  c.row_offset.assign(s + 1, 0);
+  for (int i = 0; i < s; ++i) {
+    for (const auto &x : out[i]) c.col_val.push_back(x.first);
+    c.row_offset[i + 1] = (int)c.col_val.size();
+  }
+  std::vector<int> sp;
+  for (int K = (s + 57) / 58, attempt = 0; attempt < 4 && sp.empty(); ++K, ++attempt) {
+    try {
+      sp = fap::partition_kway(c, K, seed, nullptr);
+    } catch (...) {
+      return false;
+    }
+    std::vector<int> cnt(K + 1, 0);
+    for (int x : sp) ++cnt[x];
+    if (*std::max_element(cnt.begin(), cnt.end()) > kNestT) sp.clear();
+  }
+  if (sp.empty()) return false;
+  std::vector<char> cross(s, 0);
+  for (int i = 0; i < s; ++i)
+    for (const auto &x : out[i])
+      if (sp[i] != sp[x.first]) cross[i] = cross[x.first] = 1;
+  L.sigma.resize(s);
+  for (int i = 0; i < s; ++i) L.sigma[i] = i;
+  std::sort(L.sigma.begin(), L.sigma.end(), [&](int x, int y) {
+    if (sp[x] != sp[y]) return sp[x] < sp[y];
+    if (cross[x] != cross[y]) return cross[x] > cross[y];
+    return x < y;
+  });
+  std::vector<int> inv(s), subof(s);
+  for (int p = 0; p < s; ++p) {
+    const int i = L.sigma[p];
+    inv[i] = p;
+    if (p == 0 || sp[i] != sp[L.sigma[p - 1]]) {
+      L.st.push_back(p);
+      L.n.push_back(0);
+      L.b.push_back(0);
+      L.x0.push_back(L.nx);
+    }
+    subof[p] = (int)L.st.size() - 1;
+    ++L.n.back();
+    if (cross[i]) {
+      ++L.b.back();
+      ++L.nx;
+    }
+  }
+  for (int p = 0; p < s; ++p) {
+    const int q = subof[p];
+    for (const auto &x : out[L.sigma[p]]) {
+      const int pj = inv[x.first], qj = subof[pj];
+      L.acode.push_back(qj == q ? pj - L.st[q] : -1 - (L.x0[qj] + pj - L.st[qj]));
+      L.aw.push_back(x.second);
+    }
+    L.aoff.push_back((int)L.acode.size());
+  }
+  return true;
+}
+
 // Host plan of one kind: the tables and the scratch layout, uploaded in one block.
 int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap_batch_item *> &items,
               bool fp64) {
@@ -1152,6 +1485,14 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
     return o;
   };
   int build_rows = 0, max_nb = 0, gb_rows = 0, gb_max_nb = 0;
+  // Nested closure tables (host), with the scratch offsets of their matrices.
+  std::vector<NPart> nps;
+  std::vector<NSub> nss;
+  std::vector<int> np_src, np_tile, nsig, naoff{0}, nacode;
+  std::vector<double> naw;
+  std::vector<NUnit> u3, u4;
+  std::vector<int64_t> w_at, x_at, rr_at;
+  int nest_rows = 0, nx_rows = 0;
   for (const fap_batch_item *it : items) {
     const fap_graph_dev &d = it->g;
     BGraph G{};
@@ -1288,6 +1629,67 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
       r_at.push_back(-1);
       rs_at.push_back(-1);
     }
+    // Nested closure of the item's large weighted parts (full items: a partial item
+    // closes its matrices around SSSP rows).
+    if (b->nested && G.full) {
+      HostCsr hc;
+      bool fetched = false;
+      const int xt = fp64 ? G.tile : kBT;
+      for (int pi = pfirst; pi < (int)parts.size(); ++pi) {
+        BPart &P = parts[pi];
+        if (P.bfs || P.s < kNestMinS) continue;
+        if (!fetched) {
+          FAP_TRY(fetch_csr(d, hc));
+          fetched = true;
+        }
+        NestLocal L;
+        if (!nest_local(hc, P.off, P.s, P.part_id, 0x5eedULL + (uint64_t)pi, L)) continue;
+        P.nested = 1;
+        NPart N{};
+        N.S = P.S;
+        N.s = P.s;
+        N.nx = L.nx;
+        N.SX = L.nx > 0 ? (int)align_up(L.nx, xt) : 0;
+        N.ldr = align_up(P.s, kMCols);
+        N.pos0 = (int)nsig.size();
+        const int npi = (int)nps.size();
+        np_src.push_back(pi);
+        np_tile.push_back(xt);
+        x_at.push_back(L.nx > 0 ? take((int64_t)N.SX * N.SX * elem) : -1);
+        rr_at.push_back(L.nx > 0 ? take((int64_t)L.nx * N.ldr * elem) : -1);
+        if (L.nx > 0) {
+          if (xt == kBT) q.nx_max_nb = std::max(q.nx_max_nb, N.SX / kBT);
+          else q.nx_max_nb32 = std::max(q.nx_max_nb32, N.SX / xt);
+        }
+        const int rows4 = kMWaves * (fp64 ? kNestRB<double> : kNestRB<int>);
+        for (size_t k = 0; k < L.st.size(); ++k) {
+          NSub Q{};
+          Q.part = npi;
+          Q.st = L.st[k];
+          Q.n = L.n[k];
+          Q.b = L.b[k];
+          Q.x0 = L.x0[k];
+          Q.row0 = nest_rows;
+          nest_rows += Q.n;
+          Q.xrow0 = nx_rows;
+          nx_rows += Q.b;
+          w_at.push_back(take((int64_t)kNestT * kNestT * elem));
+          const int qi = (int)nss.size();
+          for (int rb = 0; rb * kMWaves * 16 < L.nx; ++rb) u3.push_back(NUnit{qi, rb, 0, 0});
+          for (int rb = 0; rb * rows4 < Q.n; ++rb)
+            for (int cb = 0; cb * kMCols < P.s; ++cb) u4.push_back(NUnit{qi, rb, cb, 0});
+          nss.push_back(Q);
+        }
+        const int abase = (int)nacode.size();
+        for (int pp = 0; pp < P.s; ++pp) {
+          nsig.push_back(L.sigma[pp]);
+          naoff.push_back(abase + L.aoff[pp + 1]);
+        }
+        nacode.insert(nacode.end(), L.acode.begin(), L.acode.end());
+        naw.insert(naw.end(), L.aw.begin(), L.aw.end());
+        nps.push_back(N);
+      }
+    }
     graphs.push_back(G);
   }
   // Merge entries: parts with inner rows, largest first.
@@ -1343,6 +1745,12 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   const int64_t o_fw = table((fp64 ? sizeof(Fw64Desc) : sizeof(FwDesc)) * parts.size());
   const int64_t o_ip = table(sizeof(InpartDesc) * parts.size());
   const int64_t o_gbd = table((fp64 ? sizeof(Fw64Desc) : sizeof(FwDesc)) * gbg.size());
+  const size_t fwd = fp64 ? sizeof(Fw64Desc) : sizeof(FwDesc);
+  const int64_t o_np = table(sizeof(NPart) * nps.size()), o_ns = table(sizeof(NSub) * nss.size());
+  const int64_t o_sig = table(4 * nsig.size()), o_aoff = table(4 * naoff.size());
+  const int64_t o_acode = table(4 * nacode.size()), o_aw = table((int64_t)elem * naw.size());
+  const int64_t o_u3 = table(sizeof(NUnit) * u3.size()), o_u4 = table(sizeof(NUnit) * u4.size());
+  const int64_t o_nwd = table(fwd * nss.size()), o_nxd = table(fwd * nps.size());
   FAP_TRY(dev_alloc((void **)&q.d_mem, std::max<int64_t>(scratch, 256)));
   char *base = q.d_mem;
   if (o_graphs > 0) {
@@ -1376,6 +1784,13 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
     if (G.full && G.R && P.bs >= G.rlo && P.bs + P.b <= G.rlo + G.rcnt)
       P.Brows = static_cast<char *>(G.R) + (int64_t)(P.bs - G.rlo) * G.ldr * elem;
   }
+  for (size_t k = 0; k < nps.size(); ++k) {
+    NPart &N = nps[k];
+    N.D = base + d_at[np_src[k]];
+    if (x_at[k] >= 0) N.X = base + x_at[k];
+    if (rr_at[k] >= 0) N.R = base + rr_at[k];
+  }
+  for (size_t k = 0; k < nss.size(); ++k) nss[k].W = base + w_at[k];
   std::vector<char> host(scratch - o_graphs);
   auto put = [&](int64_t o, const void *src, size_t bytes) {
     if (bytes) std::memcpy(host.data() + (o - o_graphs), src, bytes);
@@ -1392,12 +1807,44 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   put(o_mlen, q.mlen, sizeof(q.mlen));
   put(o_mlen_w, q.mlen_w, sizeof(q.mlen_w));
   put(o_src, srcs.data(), sizeof(BSrc) * srcs.size());
+  put(o_np, nps.data(), sizeof(NPart) * nps.size());
+  put(o_ns, nss.data(), sizeof(NSub) * nss.size());
+  put(o_sig, nsig.data(), 4 * nsig.size());
+  put(o_aoff, naoff.data(), 4 * naoff.size());
+  put(o_acode, nacode.data(), 4 * nacode.size());
+  put(o_u3, u3.data(), sizeof(NUnit) * u3.size());
+  put(o_u4, u4.data(), sizeof(NUnit) * u4.size());
+  if (fp64) {
+    put(o_aw, naw.data(), 8 * naw.size());
+    std::vector<Fw64Desc> wd, xd;
+    for (const NSub &Q : nss) wd.push_back(Fw64Desc{static_cast<double *>(Q.W), 1});
+    for (int t32 = 0; t32 < 2; ++t32)
+      for (size_t k = 0; k < nps.size(); ++k)
+        if (nps[k].nx > 0 && (np_tile[k] == 32) == (t32 == 1))
+          xd.push_back(Fw64Desc{static_cast<double *>(nps[k].X), nps[k].SX / np_tile[k]});
+    q.nx32 = 0;
+    for (size_t k = 0; k < nps.size(); ++k) q.nx32 += nps[k].nx > 0 && np_tile[k] == 32;
+    q.nx64 = (int)xd.size() - q.nx32;
+    put(o_nwd, wd.data(), sizeof(Fw64Desc) * wd.size());
+    put(o_nxd, xd.data(), sizeof(Fw64Desc) * xd.size());
+  } else {
+    std::vector<int> awi(naw.size());
+    for (size_t e = 0; e < naw.size(); ++e) awi[e] = (int)naw[e];
+    put(o_aw, awi.data(), 4 * awi.size());
+    std::vector<FwDesc> wd, xd;
+    for (const NSub &Q : nss) wd.push_back(FwDesc{static_cast<int *>(Q.W), kNestT, 1});
+    for (const NPart &N : nps)
+      if (N.nx > 0) xd.push_back(FwDesc{static_cast<int *>(N.X), N.SX, N.SX / kBT});
+    q.nx64 = (int)xd.size();
+    put(o_nwd, wd.data(), sizeof(FwDesc) * wd.size());
+    put(o_nxd, xd.data(), sizeof(FwDesc) * xd.size());
+  }
   if (fp64) {
     std::vector<Fw64Desc> fd, gd;
     // 64-tile descriptors first, then the 32-tile ones (their own launch sequence).
     for (int t32 = 0; t32 < 2; ++t32) {
       for (const BPart &P : parts)
-        if ((graphs[P.g].tile == 32) == (t32 == 1))
+        if (!P.nested && (graphs[P.g].tile == 32) == (t32 == 1))
           fd.push_back(Fw64Desc{static_cast<double *>(P.D), P.S / graphs[P.g].tile});
       for (int gi : gbg)
         if ((graphs[gi].tile == 32) == (t32 == 1))
@@ -1416,6 +1863,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
     std::vector<FwDesc> fd, gd;
     std::vector<InpartDesc> ip2, ip4;
     for (const BPart &P : parts) {
+      if (P.nested) continue;
       if (!P.bfs) {
         fd.push_back(FwDesc{static_cast<int *>(P.D), P.S, P.S / kBT});
         continue;
@@ -1471,6 +1919,22 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   q.fw_descs = base + o_fw;
   q.ip = reinterpret_cast<InpartDesc *>(base + o_ip);
   q.gb_descs = base + o_gbd;
+  q.nnest = (int)nps.size();
+  q.nnsub = (int)nss.size();
+  q.nest_rows = nest_rows;
+  q.nx_rows = nx_rows;
+  q.nu3 = (int)u3.size();
+  q.nu4 = (int)u4.size();
+  q.npart = reinterpret_cast<NPart *>(base + o_np);
+  q.nsub = reinterpret_cast<NSub *>(base + o_ns);
+  q.nsigma = reinterpret_cast<int *>(base + o_sig);
+  q.naoff = reinterpret_cast<int *>(base + o_aoff);
+  q.nacode = reinterpret_cast<int *>(base + o_acode);
+  q.naw = base + o_aw;
+  q.nu3d = reinterpret_cast<NUnit *>(base + o_u3);
+  q.nu4d = reinterpret_cast<NUnit *>(base + o_u4);
+  q.nw_descs = base + o_nwd;
+  q.nx_descs = base + o_nxd;
   return FAP_OK;
 }
 
@@ -1501,7 +1965,8 @@ extern "C" int fap_hip_batch_create(fap_batch **out, const fap_batch_item *items
   }
   FAP_REQUIRE(out && nitems >= 0 && (nitems == 0 || items), "fap_hip_batch_create: null argument");
   FAP_REQUIRE((flags & ~(FAP_BATCH_GRAPH | 0xFF00 | 0xF0000 | FAP_BATCH_NO_PACKED |
-                         FAP_BATCH_NO_BFS | FAP_BATCH_UNIT_GRID | FAP_BATCH_FW_TILE32)) == 0,
+                         FAP_BATCH_NO_BFS | FAP_BATCH_UNIT_GRID | FAP_BATCH_FW_TILE32 |
+                         FAP_BATCH_NO_NESTED)) == 0,
               "fap_hip_batch_create: unknown flags %d", flags);
   *out = nullptr;
   std::vector<const fap_batch_item *> kinds[2];
@@ -1523,6 +1988,7 @@ extern "C" int fap_hip_batch_create(fap_batch **out, const fap_batch_item *items
   b->bfs = (flags & FAP_BATCH_NO_BFS) == 0;
   b->unit_grid = (flags & FAP_BATCH_UNIT_GRID) != 0;
   b->fw32 = (flags & FAP_BATCH_FW_TILE32) != 0;
+  b->nested = (flags & FAP_BATCH_NO_NESTED) == 0;
   int s = dev_alloc((void **)&b->d_counter, 256);
   for (int k = 0; k < 2 && s == FAP_OK; ++k)
     if (!kinds[k].empty()) s = plan_kind(b, b->kind[k], kinds[k], k == 1);
@@ -1566,6 +2032,11 @@ extern "C" int fap_hip_batch_create(fap_batch **out, const fap_batch_item *items
   }
   *out = b;
   return FAP_OK;
+}
+
+extern "C" int fap_hip_batch_nested_parts(const fap_batch *b) {
+  FAP_REQUIRE(b, "fap_hip_batch_nested_parts: null batch");
+  return b->kind[0].nnest + b->kind[1].nnest;
 }
 
 extern "C" int fap_hip_batch_run(fap_batch *b, void *stream) {

@@ -427,9 +427,17 @@ typedef struct fap_batch fap_batch;
  * (matrices padded to multiples of 32) instead of 64 x 64: less padding for small parts,
  * more rounds; an autotuner knob per size class.  Results are the same. */
 #define FAP_BATCH_FW_TILE32 0x800000
+/* In-part distances of weighted parts (of items whose parts tile the graph) by
+ * Floyd-Warshall on the whole part even above 160 vertices, instead of the nested
+ * closure (sub-parts of at most 64 vertices closed one tile each, then their crossing
+ * graph, then two min-plus passes): the same matrices, for comparisons. */
+#define FAP_BATCH_NO_NESTED 0x1000000
 int fap_hip_batch_create(fap_batch **b, const fap_batch_item *items, int32_t nitems,
                          int32_t flags);
 int fap_hip_batch_run(fap_batch *b, void *stream);
+/* Parts of the batch whose in-part distances come from the nested closure (see
+ * FAP_BATCH_NO_NESTED), or a negative status.  No counterpart in the reference. */
+int fap_hip_batch_nested_parts(const fap_batch *b);
 int fap_hip_batch_destroy(fap_batch *b);
 /* Solve every item: FAP_MODE_PLAIN = graph after graph, each the per-part launch
  * sequence (int32 FW); FAP_MODE_BATCHED / OVERLAP = one persistent launch (create,
