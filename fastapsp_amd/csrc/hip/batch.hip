@@ -35,7 +35,6 @@
 #include <vector>
 
 #include "fap_internal.h"
-#include "graph.h"
 #include "fw_desc.h"
 #include "gmem.h"
 #include "mp_tile.h"
@@ -754,7 +753,7 @@ __global__ __launch_bounds__(256) void brows_kernel(const BGraph *__restrict__ g
 // In-part distances of a weighted part of a full item (distances from its own arcs
 // alone) with at least kNestMinS vertices, without Floyd-Warshall on its s x s matrix
 // (s^3 steps).  The host cuts the part into sub-parts of at most kNestT vertices
-// (multilevel partitioner) and orders each sub-part's vertices crossing-first, where a
+// (recursive BFS bisection) and orders each sub-part's vertices crossing-first, where a
 // crossing vertex has an in-part arc to or from another sub-part.  Then:
 //   a  every sub-part's kNestT x kNestT matrix from its arcs (diagonal 0), closed by one
 //      Floyd-Warshall tile: w_q, distances inside sub-part q;
@@ -1363,7 +1362,57 @@ struct NestLocal {
   int nx = 0;                          // crossing vertices
 };
 
-bool nest_local(const HostCsr &h, int off, int s, int part_id, uint64_t seed, NestLocal &L) {
+// Sub-parts of at most kNestT vertices (ids 0 ..) by recursive bisection along BFS
+// orders: a set is ordered by hop distance from a pseudo-peripheral vertex (the last one
+// reached from its first vertex; further components follow in the same way) and cut
+// where the first floor(k/2) of its k = ceil(size / kNestT) sub-parts end.  Linear work
+// per level: a multilevel partitioner took 1-3 ms per 400-600-vertex part.
+std::vector<int> bfs_sub_parts(const std::vector<std::vector<int>> &adj) {
+  const int s = (int)adj.size();
+  std::vector<int> sp(s, 0), in_set(s, -1), seen(s, -1), order, queue;
+  std::vector<std::vector<int>> stack{std::vector<int>(s)};
+  for (int i = 0; i < s; ++i) stack[0][i] = i;
+  int stamp = 0, next_id = 0;
+  auto bfs = [&](int root, int tag, std::vector<int> &out) {  // within the current set
+    seen[root] = tag;
+    queue.assign(1, root);
+    for (size_t h = 0; h < queue.size(); ++h)
+      for (int u : adj[queue[h]])
+        if (in_set[u] == stamp && seen[u] != tag) {
+          seen[u] = tag;
+          queue.push_back(u);
+        }
+    out.insert(out.end(), queue.begin(), queue.end());
+    return queue.back();
+  };
+  int tag = 0;
+  while (!stack.empty()) {
+    std::vector<int> V = std::move(stack.back());
+    stack.pop_back();
+    if ((int)V.size() <= kNestT) {
+      for (int v : V) sp[v] = next_id;
+      ++next_id;
+      continue;
+    }
+    ++stamp;
+    for (int v : V) in_set[v] = stamp;
+    order.clear();
+    const int probe_tag = ++tag, order_tag = ++tag;
+    for (int v : V) {
+      if (seen[v] == order_tag) continue;
+      std::vector<int> scratch;
+      const int far = bfs(v, probe_tag, scratch);  // component of v, farthest vertex
+      bfs(far, order_tag, order);
+    }
+    const int k = ((int)V.size() + kNestT - 1) / kNestT;
+    const int left = (int)((int64_t)V.size() * (k / 2) / k);
+    stack.emplace_back(order.begin() + left, order.end());
+    stack.emplace_back(order.begin(), order.begin() + left);
+  }
+  return sp;
+}
+
+bool nest_local(const HostCsr &h, int off, int s, int part_id, NestLocal &L) {
   // In-part arcs in CSR order; a later arc between the same ends replaces the earlier
   // one (the build's rule), loops are dropped (the diagonal is 0).
   std::vector<std::vector<std::pair<int, double>>> out(s);
@@ -1383,25 +1432,13 @@ bool nest_local(const HostCsr &h, int off, int s, int part_id, uint64_t seed, Ne
       else l.push_back({j, w});
     }
   }
-  fap::Csr c;
-  c.n = s;
-  c.row_offset.assign(s + 1, 0);
-  for (int i = 0; i < s; ++i) {
-    for (const auto &x : out[i]) c.col_val.push_back(x.first);
-    c.row_offset[i + 1] = (int)c.col_val.size();
-  }
-  std::vector<int> sp;
-  for (int K = (s + 57) / 58, attempt = 0; attempt < 4 && sp.empty(); ++K, ++attempt) {
-    try {
-      sp = fap::partition_kway(c, K, seed, nullptr);
-    } catch (...) {
-      return false;
+  std::vector<std::vector<int>> und(s);
+  for (int i = 0; i < s; ++i)
+    for (const auto &x : out[i]) {
+      und[i].push_back(x.first);
+      und[x.first].push_back(i);
     }
-    std::vector<int> cnt(K + 1, 0);
-    for (int x : sp) ++cnt[x];
-    if (*std::max_element(cnt.begin(), cnt.end()) > kNestT) sp.clear();
-  }
-  if (sp.empty()) return false;
+  const std::vector<int> sp = bfs_sub_parts(und);
   std::vector<char> cross(s, 0);
   for (int i = 0; i < s; ++i)
     for (const auto &x : out[i])
@@ -1643,7 +1680,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
           fetched = true;
         }
         NestLocal L;
-        if (!nest_local(hc, P.off, P.s, P.part_id, 0x5eedULL + (uint64_t)pi, L)) continue;
+        if (!nest_local(hc, P.off, P.s, P.part_id, L)) continue;
         P.nested = 1;
         NPart N{};
         N.S = P.S;
