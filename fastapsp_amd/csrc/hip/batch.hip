@@ -2104,7 +2104,8 @@ int solve_one_batched(const fap_graph_dev *g, const fap_part_desc *parts, int32_
   it.d_out = d_out;
   it.ld_out = ld_out;
   fap_batch *b = nullptr;
-  FAP_TRY(fap_hip_batch_create(&b, &it, 1, 0));
+  // One run: the nested closure's host planning would cost more than it saves.
+  FAP_TRY(fap_hip_batch_create(&b, &it, 1, FAP_BATCH_NO_NESTED));
   hipEvent_t e0, e1;
   int s = timing_record(st, &e0);
   if (s == FAP_OK) s = fap_hip_batch_run(b, st);
@@ -2149,7 +2150,7 @@ extern "C" int fap_hip_solve_batch(const fap_batch_item *items, int32_t nitems, 
     return FAP_OK;
   }
   fap_batch *b = nullptr;
-  FAP_TRY(fap_hip_batch_create(&b, items, nitems, 0));
+  FAP_TRY(fap_hip_batch_create(&b, items, nitems, FAP_BATCH_NO_NESTED));  // one run
   // Stage times of the call (fap_hip_last_stage_times): the whole batch as the merge span.
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int s = timing_start(st);

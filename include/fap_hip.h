@@ -430,7 +430,11 @@ typedef struct fap_batch fap_batch;
 /* In-part distances of weighted parts (of items whose parts tile the graph) by
  * Floyd-Warshall on the whole part even above 160 vertices, instead of the nested
  * closure (sub-parts of at most 64 vertices closed one tile each, then their crossing
- * graph, then two min-plus passes): the same matrices, for comparisons. */
+ * graph, then two min-plus passes): the same matrices.  The nested closure is planned on
+ * the host at creation (a copy of each graph's CSR, a BFS bisection of each part, about
+ * 0.1 ms per part), so it pays for batches that are replayed; the one-run forms
+ * (fap_hip_solve_batch, the single-graph routes through the batch stages, the request
+ * server) set this flag. */
 #define FAP_BATCH_NO_NESTED 0x1000000
 int fap_hip_batch_create(fap_batch **b, const fap_batch_item *items, int32_t nitems,
                          int32_t flags);
