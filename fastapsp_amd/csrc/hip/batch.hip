@@ -45,6 +45,9 @@ namespace fap_hip {
 int launch_fw(const FwDesc *d_descs, int ndesc, int max_nb, hipStream_t stream);
 int launch_fw64(const Fw64Desc *d_descs, int nd, int max_nb, hipStream_t st);
 int launch_fw64_t32(const Fw64Desc *d_descs, int nd, int max_nb, hipStream_t st);
+int launch_fw16(const Fw16Desc *d_descs, int ndesc, int max_nb, hipStream_t stream);
+int launch_fw16_unpack(const UnpackDesc *d_descs, int nd, int64_t total_items,
+                       const int *d_comp_sorted, int *d_flags, hipStream_t stream);
 int launch_inpart_bfs(const InpartDesc *d_descs, int nd, int total_groups, int s_max,
                       hipStream_t stream);
 
@@ -73,6 +76,8 @@ struct BGraph {
   int pbase;                     // first of them in the row numbering of bperm_kernel
   int64_t ldr;
   void *MB;                      // boundary-graph matrix (SB x SB), full items only
+  void *MB16;                    // int32 packed graphs: the same matrix capped at kCap16
+                                 //   as uint16, closed by the 16-bit Floyd-Warshall
   int SB;
   int n, nb;                     // vertices; boundary vertices (full items)
   int bbase;                     // first entry of the graph in bpart / brow
@@ -335,8 +340,21 @@ __global__ __launch_bounds__(256) void bgb_kernel(const BGraph *__restrict__ gra
   const BGraph g = graphs[gi];
   const int r = row - g.gb_row_begin;
   FAP_GLOBAL T *M = gptr(static_cast<T *>(g.MB)) + (int64_t)r * g.SB;
+  // Packed graphs (every finite distance below kCap16): the row goes to the 16-bit
+  // matrix, capped (the cap reads as unreachable).
+  FAP_GLOBAL uint16_t *M16 =
+      g.MB16 ? gptr(static_cast<uint16_t *>(g.MB16)) + (int64_t)r * g.SB : nullptr;
+  auto put = [&](int t, T v) {
+    if constexpr (sizeof(T) == 4) {
+      if (M16) {
+        M16[t] = (uint16_t)min((unsigned)v, (unsigned)kCap16);
+        return;
+      }
+    }
+    M[t] = v;
+  };
   if (r >= g.nb) {
-    for (int t = lane; t < g.SB; t += 64) M[t] = Dist<T>::inf();
+    for (int t = lane; t < g.SB; t += 64) put(t, Dist<T>::inf());
     return;
   }
   const BPart q = parts[bpart[g.bbase + r]];
@@ -386,7 +404,7 @@ __global__ __launch_bounds__(256) void bgb_kernel(const BGraph *__restrict__ gra
 #pragma unroll
     for (int qq = 0; qq < kCols; ++qq) {
       const int t = t0 + qq * 64 + lane;
-      if (t < g.SB) M[t] = t == r ? T(0) : val[qq];
+      if (t < g.SB) put(t, t == r ? T(0) : val[qq]);
     }
   }
 }
@@ -1168,6 +1186,12 @@ struct fap_batch_kind {
   BMerge *merge = nullptr;
   BSrc *srcs = nullptr;
   void *fw_descs = nullptr, *gb_descs = nullptr;
+  // int32 boundary graphs of packed graphs: 16-bit Floyd-Warshall, then expansion into
+  // the int32 matrix (items of 8 entries).
+  int ngb16 = 0, gb16_max_nb = 0;
+  int64_t gb16_items = 0;
+  Fw16Desc *gb16_descs = nullptr;
+  UnpackDesc *gb16_unpack = nullptr;
   // Nested closure: parts, sub-parts, build rows (sum of sub-part sizes), crossing-graph
   // rows, product units (crossing rows, part rows), crossing-graph FW descriptors (FP64:
   // 64-tile ones, then 32-tile ones).
@@ -1266,7 +1290,11 @@ int launch_kind(const fap_batch *b, const fap_batch_kind &q, int *counter, hipSt
     hipLaunchKernelGGL(bgb_kernel<T>, dim3((q.gb_rows + 3) / 4), dim3(256), 0, st, q.graphs,
                        q.parts, q.gbg, q.ngb, q.bpart, q.pid2idx, q.gb_rows);
     FAP_CHECK_LAUNCH();
-    FAP_TRY(launch_fw_chunks<T>(q.gb_descs, q.ngb64, q.gb_max_nb, 64, st));
+    if (q.ngb64 > 0) FAP_TRY(launch_fw_chunks<T>(q.gb_descs, q.ngb64, q.gb_max_nb, 64, st));
+    if (q.ngb16 > 0) {
+      FAP_TRY(launch_fw16(q.gb16_descs, q.ngb16, q.gb16_max_nb, st));
+      FAP_TRY(launch_fw16_unpack(q.gb16_unpack, q.ngb16, q.gb16_items, nullptr, nullptr, st));
+    }
     if (q.ngb32 > 0)
       FAP_TRY(launch_fw_chunks<T>(static_cast<Fw64Desc *>(q.gb_descs) + q.ngb64, q.ngb32,
                                   q.gb_max_nb32, 32, st));
@@ -1512,7 +1540,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   std::vector<double> pcost;
   // Scratch offsets (bytes) of every part matrix, boundary-graph matrix and FP64 row
   // block; resolved to pointers once the allocation exists.
-  std::vector<int64_t> d_at, mb_at, r_at, rs_at, brows_at;
+  std::vector<int64_t> d_at, mb_at, mb16_at, r_at, rs_at, brows_at;
   std::vector<char> emitted;  // per part: its rows are produced (merge units)
   int nbrows = 0;             // boundary-graph rows produced over all graphs
   int64_t scratch = 0;
@@ -1654,6 +1682,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
       else q.gb_max_nb32 = std::max(q.gb_max_nb32, G.SB / G.tile);
       gbg.push_back(gi);
       mb_at.push_back(take((int64_t)G.SB * G.SB * elem));
+      mb16_at.push_back(!fp64 && G.packed ? take((int64_t)G.SB * G.SB * 2) : -1);
       r_at.push_back(take((int64_t)std::max(G.rcnt, 1) * G.ldr * elem));
       rs_at.push_back(G.perm ? take((int64_t)std::max(G.rcnt, 1) * G.ldr * elem) : -1);
       G.pbase = nbrows;
@@ -1663,6 +1692,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
           for (int cb = 0; cb * kMCols < parts[pi].s; ++cb) units5.push_back(BRowUnit{pi, r0, cb, 0});
     } else {
       mb_at.push_back(-1);
+      mb16_at.push_back(-1);
       r_at.push_back(-1);
       rs_at.push_back(-1);
     }
@@ -1782,6 +1812,8 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   const int64_t o_fw = table((fp64 ? sizeof(Fw64Desc) : sizeof(FwDesc)) * parts.size());
   const int64_t o_ip = table(sizeof(InpartDesc) * parts.size());
   const int64_t o_gbd = table((fp64 ? sizeof(Fw64Desc) : sizeof(FwDesc)) * gbg.size());
+  const int64_t o_gb16 = table(sizeof(Fw16Desc) * gbg.size());
+  const int64_t o_ud16 = table(sizeof(UnpackDesc) * gbg.size());
   const size_t fwd = fp64 ? sizeof(Fw64Desc) : sizeof(FwDesc);
   const int64_t o_np = table(sizeof(NPart) * nps.size()), o_ns = table(sizeof(NSub) * nss.size());
   const int64_t o_sig = table(4 * nsig.size()), o_aoff = table(4 * naoff.size());
@@ -1812,6 +1844,7 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   for (size_t gi = 0; gi < graphs.size(); ++gi) {
     BGraph &G = graphs[gi];
     if (mb_at[gi] >= 0) G.MB = base + mb_at[gi];
+    if (mb16_at[gi] >= 0) G.MB16 = base + mb16_at[gi];
     if (r_at[gi] >= 0) G.R = base + r_at[gi];
     if (rs_at[gi] >= 0) G.Rs = base + rs_at[gi];
   }
@@ -1912,10 +1945,28 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
                              G.row_off, G.col, G.graph_id, G.st2ed, G.ed2st});
       groups += (P.s + 63) / 64;
     }
-    for (int gi : gbg)
-      gd.push_back(FwDesc{static_cast<int *>(graphs[gi].MB), graphs[gi].SB, graphs[gi].SB / kBT});
+    // Boundary graphs of packed graphs on 16 bits (half the VALU steps of int32).
+    std::vector<Fw16Desc> g16;
+    std::vector<UnpackDesc> u16;
+    gb_max_nb = 0;
+    for (int gi : gbg) {
+      const BGraph &G = graphs[gi];
+      if (G.MB16) {
+        g16.push_back(Fw16Desc{static_cast<uint16_t *>(G.MB16), G.SB, G.SB / kBT});
+        u16.push_back(UnpackDesc{static_cast<const uint16_t *>(G.MB16), static_cast<int *>(G.MB),
+                                 G.SB, G.nb, 0, 0, q.gb16_items});
+        q.gb16_items += (int64_t)G.SB * G.SB / 8;
+        q.gb16_max_nb = std::max(q.gb16_max_nb, G.SB / kBT);
+        continue;
+      }
+      gd.push_back(FwDesc{static_cast<int *>(G.MB), G.SB, G.SB / kBT});
+      gb_max_nb = std::max(gb_max_nb, G.SB / kBT);
+    }
     put(o_fw, fd.data(), sizeof(FwDesc) * fd.size());
     put(o_gbd, gd.data(), sizeof(FwDesc) * gd.size());
+    put(o_gb16, g16.data(), sizeof(Fw16Desc) * g16.size());
+    put(o_ud16, u16.data(), sizeof(UnpackDesc) * u16.size());
+    q.ngb16 = (int)g16.size();
     q.nfw = q.nfw64 = (int)fd.size();
     q.ngb64 = (int)gd.size();
     q.nip2 = (int)ip2.size();
@@ -1956,6 +2007,8 @@ int plan_kind(const fap_batch *b, fap_batch_kind &q, const std::vector<const fap
   q.fw_descs = base + o_fw;
   q.ip = reinterpret_cast<InpartDesc *>(base + o_ip);
   q.gb_descs = base + o_gbd;
+  q.gb16_descs = reinterpret_cast<Fw16Desc *>(base + o_gb16);
+  q.gb16_unpack = reinterpret_cast<UnpackDesc *>(base + o_ud16);
   q.nnest = (int)nps.size();
   q.nnsub = (int)nss.size();
   q.nest_rows = nest_rows;
