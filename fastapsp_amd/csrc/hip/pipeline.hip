@@ -106,6 +106,9 @@ Workspace g_ws_meta;  // descriptors, sources, row indices
 Workspace g_ws_ell;   // ELL adjacency of the unit-weight graph
 Workspace g_ws_bg;    // closed boundary-graph matrix
 Workspace g_ws_bgv;   // boundary-graph vertices and clique weights
+// Set only inside fap_hip_prepare_parts: the Floyd-Warshall matrices go to this
+// caller-owned buffer instead of g_ws_fw and the pass stops after Floyd-Warshall.
+int *g_prep_fw = nullptr;
 
 struct EventPool {
   std::vector<hipEvent_t> ev;
@@ -284,8 +287,11 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
     fw_total += sz;
     fw_max = std::max(fw_max, sz);
   }
-  FAP_TRY(g_ws_fw.ensure((batched ? fw_total : fw_max) * 4));
-  int *fw_base = static_cast<int *>(g_ws_fw.ptr);
+  int *fw_base = g_prep_fw;
+  if (!fw_base) {
+    FAP_TRY(g_ws_fw.ensure((batched ? fw_total : fw_max) * 4));
+    fw_base = static_cast<int *>(g_ws_fw.ptr);
+  }
   uint16_t *fw16_base = nullptr;
   if (use16) {
     FAP_TRY(g_ws_fw16.ensure((batched ? fw_total : fw_max) * 2));
@@ -611,6 +617,7 @@ static int solve_pass(const fap_graph_dev *g, const fap_part_desc *parts, int32_
       cdec = overlap ? chunk_dec[c] : dec_total;
       const int64_t cunpack = overlap ? chunk_unpack[c] : unpack_total;
       FAP_TRY(span(2, on, [&](hipStream_t s2) { return run_fw(p0, p1, cmax_nb, cunpack, s2); }));
+      if (g_prep_fw) continue;  // prepared parts: no merge, no decode
       if (hier) FAP_TRY(span(0, on, run_boundary_graph));  // hier: one chunk
       if (ipb_done) FAP_CHECK_HIP(hipStreamWaitEvent(on, ipb_done, 0));
       FAP_TRY(span(3, on, [&](hipStream_t s2) -> int {
@@ -737,6 +744,43 @@ extern "C" int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *
   FAP_TRY(ws_fence_wait(st));
   const int s = solve_parts(g, parts, nparts, d_out, ld_out, mode, stream);
   const int f = ws_fence_record(st);  // also after a failed call: its launches used the workspaces
+  return s != FAP_OK ? s : f;
+}
+
+extern "C" int fap_hip_prepare_parts(const fap_graph_dev *g, const fap_part_desc *parts,
+                                     int32_t nparts, int32_t *d_rows, int64_t ld_rows,
+                                     int32_t *d_fw, int64_t *fw_off, int64_t *fw_entries,
+                                     void *stream) {
+  if (!initialized()) {
+    set_error("fap_hip_init has not been called");
+    return FAP_ERR_STATE;
+  }
+  FAP_REQUIRE(g && nparts >= 0 && (nparts == 0 || parts) && fw_off && fw_entries,
+              "fap_hip_prepare_parts: null argument");
+  // Same offsets as the batched pass: non-empty parts in list order, S x S each.
+  int64_t total = 0;
+  for (int p = 0; p < nparts; ++p) {
+    if (parts[p].num <= 0) {
+      fw_off[p] = -1;
+      continue;
+    }
+    const int64_t S = align_up(parts[p].num, kTile);
+    fw_off[p] = total;
+    total += S * S;
+  }
+  *fw_entries = total;
+  if (!d_fw) return FAP_OK;  // size query
+  FAP_REQUIRE(d_rows && ld_rows >= g->n, "fap_hip_prepare_parts: bad boundary-row buffer");
+  hipStream_t st = resolve(stream);
+  FAP_TRY(ws_fence_wait(st));
+  g_parts16 = g_redone = g_bgraph = g_keyed = g_fw_keyed = g_bfs = 0;
+  g_lim16 = -1;
+  g_prep_fw = d_fw;
+  const int s = solve_pass(g, parts, nparts, d_rows, ld_rows,
+                           FAP_MODE_BATCHED | FAP_MODE_FLAG_SSSP | FAP_MODE_FLAG_INT32_FW, stream,
+                           false);
+  g_prep_fw = nullptr;
+  const int f = ws_fence_record(st);
   return s != FAP_OK ? s : f;
 }
 

@@ -169,6 +169,14 @@ int fap_graph_set_mode(fap_graph *h, int32_t mode) {
   return guarded([&] { h->g->setMode(mode); });
 }
 
+int fap_graph_set_subgraph_sweep(fap_graph *h, int32_t after_calls, int32_t *ready) {
+  FAP_HANDLE(h);
+  return guarded([&] {
+    if (after_calls >= 0) h->g->setSubGraphSweep(after_calls);
+    if (ready) *ready = h->g->subGraphSweepState();
+  });
+}
+
 int fap_graph_solve(fap_graph *h) {
   FAP_HANDLE(h);
   return guarded([&] { h->g->solve(); });

@@ -121,6 +121,7 @@ fapGraph::fapGraph(int32_t n, int64_t m, std::vector<int32_t> ro, std::vector<in
 fapGraph::~fapGraph() { release(); }
 
 void fapGraph::release() {
+  free_sweep();
   int32_t *ptrs[] = {d_row_offset, d_col,  d_w,    d_graph_id, d_st2ed, d_ed2st, d_comp,
                      d_dist,       d_sub_dist, d_path, d_sub_path, d_rro,  d_rcol,  d_rw};
   for (int32_t *p : ptrs)
@@ -468,6 +469,27 @@ int32_t fapGraph::solveSubGraph(int32_t id, bool path_needed) {
     check(fap_hip_malloc((void **)&d_sub_dist, rows * ld_dist * 4), "sub-graph allocation");
   fap_part_desc part{id, C_BlockVer_offset[id], C_BlockVer_num[id], C_BlockBdy_num[id], 0};
   have_sub_path = false;
+  fap_graph_dev g{num_vertexs, num_edges, d_row_offset, d_col, d_w,
+                  d_graph_id,  d_st2ed,   d_ed2st,      d_comp, d_wf};
+  // A sweep (MoreTaskMpiExample.cpp:91-96 walks every task of its rank): from the
+  // sweep_after-th distance-only call on, the boundary rows and Floyd-Warshall matrices
+  // of all parts come from one batched pass and each call writes only its block.
+  // Weighted graphs whose boundary sources reach far keep the boundary-graph route
+  // (one SSSP per boundary vertex is the slow part there).
+  if (!path_needed && !float_weights && !subgraph_by_boundary_graph() && sweep_after > 0 &&
+      !sweep_failed && (sweep_ready || ++sweep_calls >= sweep_after) &&
+      (sweep_ready || prepare_sweep(g))) {
+    const int64_t off = sweep_fw_off[id - 1];
+    check(fap_hip_merge_prepared(&g, &part, off < 0 ? nullptr : d_sweep_fw + off,
+                                 d_sweep_rows + sweep_row0[id - 1] * ld_dist, ld_dist,
+                                 sweep_sparse ? 1 : 0, d_sub_dist, ld_dist, nullptr),
+          "solveSubGraph");
+    check(fap_hip_stream_sync(nullptr), "solveSubGraph");
+    add_stage_times(1);
+    current_subgraph_id = id;
+    sub_rows = C_BlockVer_num[id];
+    return 0;
+  }
   if (path_needed) {
     if (!d_sub_path)
       check(fap_hip_malloc((void **)&d_sub_path, rows * ld_dist * 4), "sub-graph path allocation");
@@ -477,8 +499,6 @@ int32_t fapGraph::solveSubGraph(int32_t id, bool path_needed) {
     sub_rows = C_BlockVer_num[id];
     return 0;
   }
-  fap_graph_dev g{num_vertexs, num_edges, d_row_offset, d_col, d_w,
-                  d_graph_id,  d_st2ed,   d_ed2st,      d_comp, d_wf};
   if (subgraph_by_boundary_graph()) {
     // Weighted graph whose boundary sources reach far: every part's in-part distances
     // and the closed boundary graph, then this part's rows (a one-part batch item).
@@ -505,6 +525,56 @@ int32_t fapGraph::solveSubGraph(int32_t id, bool path_needed) {
   current_subgraph_id = id;
   sub_rows = C_BlockVer_num[id];
   return 0;
+}
+
+// Largest device footprint of a solveSubGraph sweep (boundary rows + matrices).
+static constexpr int64_t kSweepMaxBytes = 32LL << 30;
+
+bool fapGraph::prepare_sweep(const fap_graph_dev &g) {
+  std::vector<fap_part_desc> all = partList();
+  sweep_row0.assign(all.size(), 0);
+  int64_t rows = 0;
+  for (size_t i = 0; i < all.size(); ++i) {
+    sweep_row0[i] = rows;
+    all[i].out_row = rows;
+    rows += all[i].bdy;
+  }
+  sweep_fw_off.assign(all.size(), -1);
+  int64_t fw_entries = 0;
+  check(fap_hip_prepare_parts(&g, all.data(), (int32_t)all.size(), nullptr, ld_dist, nullptr,
+                              sweep_fw_off.data(), &fw_entries, nullptr),
+        "solveSubGraph sweep");
+  const int64_t bytes = 4 * (fw_entries + rows * ld_dist);
+  // Over the cap, or the budget refuses the buffers: the calls keep the per-part form.
+  if (bytes > kSweepMaxBytes ||
+      fap_hip_malloc((void **)&d_sweep_fw, 4 * std::max<int64_t>(fw_entries, 1)) != FAP_OK ||
+      fap_hip_malloc((void **)&d_sweep_rows, 4 * std::max<int64_t>(rows, 1) * ld_dist) !=
+          FAP_OK) {
+    free_sweep();
+    sweep_failed = true;
+    return false;
+  }
+  check(fap_hip_prepare_parts(&g, all.data(), (int32_t)all.size(), d_sweep_rows, ld_dist,
+                              d_sweep_fw, sweep_fw_off.data(), &fw_entries, nullptr),
+        "solveSubGraph sweep");
+  sweep_sparse = sssp_cheap;
+  sweep_ready = true;
+  return true;
+}
+
+void fapGraph::free_sweep() {
+  if (d_sweep_fw) fap_hip_free(d_sweep_fw);
+  if (d_sweep_rows) fap_hip_free(d_sweep_rows);
+  d_sweep_fw = d_sweep_rows = nullptr;
+  sweep_fw_off.clear();
+  sweep_row0.clear();
+  sweep_ready = sweep_failed = false;
+  sweep_calls = 0;
+}
+
+void fapGraph::setSubGraphSweep(int32_t after_calls) {
+  sweep_after = after_calls;
+  if (after_calls <= 0) free_sweep();
 }
 
 std::vector<int32_t> fapGraph::getMapping() { return st2ed; }

@@ -110,6 +110,24 @@ class FapGraph:
     def setMode(self, mode: int) -> None:
         L.check(L.load().fap_graph_set_mode(self._h, int(mode)), "setMode")
 
+    def setSubGraphSweep(self, after_calls: int) -> None:
+        """Distance-only solveSubGraph calls on integer-weight graphs: from the
+        after_calls-th call on (default 3, 0 = never) every part's boundary rows and
+        Floyd-Warshall matrix come from one batched pass and each call writes only its
+        block (identical results)."""
+        L.check(L.load().fap_graph_set_subgraph_sweep(self._h, int(after_calls), None),
+                "setSubGraphSweep")
+
+    def subGraphSweepState(self) -> int:
+        """0 = not prepared, 1 = prepared (dense product), 2 = prepared (sparse)."""
+        v = C.c_int32()
+        L.check(L.load().fap_graph_set_subgraph_sweep(self._h, -1, C.byref(v)),
+                "subGraphSweepState")
+        return int(v.value)
+
+    def subGraphSweepReady(self) -> bool:
+        return self.subGraphSweepState() != 0
+
     def solve(self, is_path_needed: bool = False) -> int:
         """fap/fap.h:127.  With is_path_needed the predecessor matrix is produced too
         (integer-weight graphs; the reference CPU path's tie-breaking)."""

@@ -123,6 +123,17 @@ class fapGraph {
     return (d_w || float_weights) && !boundary_by_sssp && !sssp_cheap &&
            (mode & 0xFF) == FAP_MODE_BATCHED && !(mode & FAP_MODE_FLAG_SSSP);
   }
+  // solveSubGraph sweep: from the sweep_after-th distance-only call on, every part's
+  // boundary rows and Floyd-Warshall matrix come from one batched pass (kept until the
+  // layout changes) and each call only writes its block.  sweep_sparse: the blocks
+  // take the sparse product (boundary rows that reach few vertices).
+  int32_t sweep_after = 3;
+  int32_t sweep_calls = 0;
+  bool sweep_ready = false, sweep_failed = false, sweep_sparse = false;
+  int32_t *d_sweep_fw = nullptr, *d_sweep_rows = nullptr;
+  std::vector<int64_t> sweep_fw_off, sweep_row0;
+  bool prepare_sweep(const fap_graph_dev &g);
+  void free_sweep();
   void solve_placed(const fap_graph_dev &g, const std::vector<fap_part_desc> &parts);
   void copy_rows(const int32_t *base, int64_t nrows, int64_t row0, int64_t rows, void *out) const;
   void require_int(const char *what) const;
@@ -174,6 +185,14 @@ class fapGraph {
 
   int32_t solve(bool is_path_needed = false);
   int32_t solveSubGraph(int32_t sub_graph_id, bool is_path_needed = false);
+  // Distance-only solveSubGraph calls on integer-weight graphs: from the after_calls-th
+  // call on (default 3; 0 = never), one batched pass prepares every part's boundary rows
+  // and Floyd-Warshall matrix, held in device memory until the layout changes (about
+  // 4 x (sum of boundary counts x n + sum of padded s_p^2) bytes, at most 32 GiB), and
+  // each call then only writes its block.  Results are identical either way.
+  void setSubGraphSweep(int32_t after_calls);
+  // 0 = not prepared, 1 = prepared (dense product), 2 = prepared (sparse product).
+  int32_t subGraphSweepState() const { return sweep_ready ? (sweep_sparse ? 2 : 1) : 0; }
 
   std::vector<int32_t> getMapping();
   std::vector<float> getAllDistance();

@@ -290,6 +290,34 @@ int fap_hip_check_layout(const fap_graph_dev *g, const fap_part_desc *parts, int
  * is set; the stage times report it as floyd_ms. */
 int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *parts, int32_t nparts,
                         int32_t *d_out, int64_t ld_out, int32_t mode, void *stream);
+
+/* A sweep of solveSubGraph calls (the reference solves every part of its rank in turn,
+ * app/MoreTaskMpiExample.cpp:91-96, each call one run() of fap/fap.cpp:195-503), split
+ * into a once-per-graph half and a per-part half.  No single reference function: this
+ * pair replaces the repeated handle_boundry_path + MysubMatBuild_path + floyd_path of
+ * fap.cpp:405-428 with one batched pass.
+ * fap_hip_prepare_parts: boundary SSSP, sub-matrix build and int32 Floyd-Warshall of
+ * every listed part in one batched pass.  Part p's boundary rows go to rows out_row ..
+ * out_row + bdy - 1 of d_rows (stride ld_rows >= n); its padded S x S Floyd-Warshall
+ * matrix (S = s_p rounded up to 64) to d_fw + fw_off[p] (fw_off[p] = -1 for an empty
+ * part).  With d_fw == NULL only fw_off and *fw_entries (int32 entries of d_fw) are
+ * computed.  Asynchronous on `stream`. */
+int fap_hip_prepare_parts(const fap_graph_dev *g, const fap_part_desc *parts, int32_t nparts,
+                          int32_t *d_rows, int64_t ld_rows, int32_t *d_fw, int64_t *fw_off,
+                          int64_t *fw_entries, void *stream);
+
+/* One part's s_p x n block (rows out_row .. of d_out, stride ld_out >= n) from its
+ * prepared Floyd-Warshall matrix d_fw (S x S) and boundary rows d_rows (bdy rows, stride
+ * ld_rows): the boundary rows copied with their own columns from FW (MysubMatDecode_path,
+ * fap/fap.cpp:492-496), the inner rows by min-plus against the boundary rows
+ * (min_plus_path_advanced, fap.cpp:437-485) with own columns from FW.  sparse != 0:
+ * inner rows filled with FAP_INF, then the product over the columns some boundary row
+ * reaches only (for graphs whose boundary rows are mostly unreachable); the block is
+ * identical in both forms.  Stage times: the row copy as sssp_ms, the product as
+ * minplus_ms, own columns as decode_ms.  Asynchronous on `stream`. */
+int fap_hip_merge_prepared(const fap_graph_dev *g, const fap_part_desc *part,
+                           const int32_t *d_fw, const int32_t *d_rows, int64_t ld_rows,
+                           int32_t sparse, int32_t *d_out, int64_t ld_out, void *stream);
 int fap_hip_last_stage_times(fap_stage_times *t);
 
 /* Work-unit shape of the streaming min-plus merge of fap_hip_solve_parts (the
@@ -489,6 +517,12 @@ int fap_graph_is_split(fap_graph *g, int *is_split);
 int fap_graph_set_mode(fap_graph *g, int32_t mode);
 int fap_graph_solve(fap_graph *g);
 int fap_graph_solve_subgraph(fap_graph *g, int32_t id);
+/* fapGraph::setSubGraphSweep (after_calls >= 0; a negative value changes nothing) and
+ * *ready = 0 before the sweep's batched pass has run, 1 after it (dense product), 2 after
+ * it (sparse product); ready may be NULL.  No reference
+ * counterpart: the reference recomputes every part's SSSP and Floyd-Warshall on each
+ * solveSubGraph call (fap/fap.cpp:405-428). */
+int fap_graph_set_subgraph_sweep(fap_graph *g, int32_t after_calls, int32_t *ready);
 /* solve(is_path_needed) / solveSubGraph(id, is_path_needed) (fap/fap.h:127-128). */
 int fap_graph_solve_ex(fap_graph *g, int32_t is_path_needed);
 int fap_graph_solve_subgraph_ex(fap_graph *g, int32_t id, int32_t is_path_needed);

@@ -211,6 +211,32 @@ int fap_hip_solve_parts(const fap_graph_dev *g, const fap_part_desc *parts, int3
   return host_solve(g, parts, nparts, d_out, false, nullptr, ld_out);
 }
 
+// The sweep pair: offsets as the device library computes them; the per-part block is
+// solved from the graph directly (the prepared buffers are not read).
+int fap_hip_prepare_parts(const fap_graph_dev *, const fap_part_desc *parts, int32_t nparts,
+                          int32_t *, int64_t, int32_t *, int64_t *fw_off, int64_t *fw_entries,
+                          void *) {
+  if (!fw_off || !fw_entries || nparts < 0 || (nparts > 0 && !parts)) return FAP_ERR_INVALID;
+  int64_t total = 0;
+  for (int p = 0; p < nparts; ++p) {
+    if (parts[p].num <= 0) {
+      fw_off[p] = -1;
+      continue;
+    }
+    const int64_t S = (parts[p].num + 63) / 64 * 64;
+    fw_off[p] = total;
+    total += S * S;
+  }
+  *fw_entries = total;
+  return FAP_OK;
+}
+
+int fap_hip_merge_prepared(const fap_graph_dev *g, const fap_part_desc *part, const int32_t *,
+                           const int32_t *, int64_t, int32_t, int32_t *d_out, int64_t ld_out,
+                           void *) {
+  return host_solve(g, part, 1, d_out, false, nullptr, ld_out);
+}
+
 int fap_hip_solve_parts_f32(const fap_graph_dev *g, const fap_part_desc *parts, int32_t nparts,
                             float *d_out, int64_t ld_out, int32_t, void *) {
   return host_solve(g, parts, nparts, d_out, true, nullptr, ld_out);
