@@ -160,7 +160,8 @@ SIGNATURES = {
     "fap_graph_solve": (C.c_int, [_vp]),
     "fap_graph_set_subgraph_sweep": (C.c_int, [_vp, _i32, _pi32]),
     "fap_hip_prepare_parts": (C.c_int, [_vp, _vp, _i32, _vp, _i64, _vp, _pi64, _pi64, _vp]),
-    "fap_hip_merge_prepared": (C.c_int, [_vp, _vp, _vp, _vp, _i64, _i32, _vp, _i64, _vp]),
+    "fap_hip_merge_prepared": (C.c_int, [_vp, _vp, _vp, _vp, _i64, _i64, _vp, _i64, _vp]),
+    "fap_hip_count_reached": (C.c_int, [_vp, _vp, _i32, _vp, _i64, _pi32, _vp]),
     "fap_graph_solve_subgraph": (C.c_int, [_vp, _i32]),
     "fap_graph_solve_ex": (C.c_int, [_vp, _i32]),
     "fap_graph_solve_subgraph_ex": (C.c_int, [_vp, _i32, _i32]),

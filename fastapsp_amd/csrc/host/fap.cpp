@@ -482,7 +482,7 @@ int32_t fapGraph::solveSubGraph(int32_t id, bool path_needed) {
     const int64_t off = sweep_fw_off[id - 1];
     check(fap_hip_merge_prepared(&g, &part, off < 0 ? nullptr : d_sweep_fw + off,
                                  d_sweep_rows + sweep_row0[id - 1] * ld_dist, ld_dist,
-                                 sweep_sparse ? 1 : 0, d_sub_dist, ld_dist, nullptr),
+                                 sweep_reached[id - 1], d_sub_dist, ld_dist, nullptr),
           "solveSubGraph");
     check(fap_hip_stream_sync(nullptr), "solveSubGraph");
     add_stage_times(1);
@@ -557,7 +557,19 @@ bool fapGraph::prepare_sweep(const fap_graph_dev &g) {
   check(fap_hip_prepare_parts(&g, all.data(), (int32_t)all.size(), d_sweep_rows, ld_dist,
                               d_sweep_fw, sweep_fw_off.data(), &fw_entries, nullptr),
         "solveSubGraph sweep");
-  sweep_sparse = sssp_cheap;
+  // Parts whose boundary rows reach at most n / 8 columns take the sparse product (its
+  // compact matrix is then at most an eighth of the block).
+  std::vector<int32_t> counts(all.size(), 0);
+  check(fap_hip_count_reached(&g, all.data(), (int32_t)all.size(), d_sweep_rows, ld_dist,
+                              counts.data(), nullptr),
+        "solveSubGraph sweep");
+  sweep_reached.assign(all.size(), -1);
+  sweep_sparse = false;
+  for (size_t i = 0; i < all.size(); ++i)
+    if ((int64_t)counts[i] * 8 <= num_vertexs) {
+      sweep_reached[i] = counts[i];
+      sweep_sparse = true;
+    }
   sweep_ready = true;
   return true;
 }
@@ -568,6 +580,7 @@ void fapGraph::free_sweep() {
   d_sweep_fw = d_sweep_rows = nullptr;
   sweep_fw_off.clear();
   sweep_row0.clear();
+  sweep_reached.clear();
   sweep_ready = sweep_failed = false;
   sweep_calls = 0;
 }

@@ -125,13 +125,14 @@ class fapGraph {
   }
   // solveSubGraph sweep: from the sweep_after-th distance-only call on, every part's
   // boundary rows and Floyd-Warshall matrix come from one batched pass (kept until the
-  // layout changes) and each call only writes its block.  sweep_sparse: the blocks
-  // take the sparse product (boundary rows that reach few vertices).
+  // layout changes) and each call only writes its block.  sweep_reached[p]: the columns
+  // part p's boundary rows reach when its block takes the sparse product, -1 for the
+  // dense product; sweep_sparse: some part takes the sparse product.
   int32_t sweep_after = 3;
   int32_t sweep_calls = 0;
   bool sweep_ready = false, sweep_failed = false, sweep_sparse = false;
   int32_t *d_sweep_fw = nullptr, *d_sweep_rows = nullptr;
-  std::vector<int64_t> sweep_fw_off, sweep_row0;
+  std::vector<int64_t> sweep_fw_off, sweep_row0, sweep_reached;
   bool prepare_sweep(const fap_graph_dev &g);
   void free_sweep();
   void solve_placed(const fap_graph_dev &g, const std::vector<fap_part_desc> &parts);

@@ -306,18 +306,26 @@ int fap_hip_prepare_parts(const fap_graph_dev *g, const fap_part_desc *parts, in
                           int32_t *d_rows, int64_t ld_rows, int32_t *d_fw, int64_t *fw_off,
                           int64_t *fw_entries, void *stream);
 
+/* Per part of `parts` (boundary rows out_row .. out_row + bdy - 1 of d_rows, as written by
+ * fap_hip_prepare_parts): counts[p] = the columns outside the part that one of its
+ * boundary rows reaches (an entry below FAP_INF).  Waits for the stream. */
+int fap_hip_count_reached(const fap_graph_dev *g, const fap_part_desc *parts, int32_t nparts,
+                          const int32_t *d_rows, int64_t ld_rows, int32_t *counts, void *stream);
+
 /* One part's s_p x n block (rows out_row .. of d_out, stride ld_out >= n) from its
  * prepared Floyd-Warshall matrix d_fw (S x S) and boundary rows d_rows (bdy rows, stride
  * ld_rows): the boundary rows copied with their own columns from FW (MysubMatDecode_path,
  * fap/fap.cpp:492-496), the inner rows by min-plus against the boundary rows
- * (min_plus_path_advanced, fap.cpp:437-485) with own columns from FW.  sparse != 0:
- * inner rows filled with FAP_INF, then the product over the columns some boundary row
- * reaches only (for graphs whose boundary rows are mostly unreachable); the block is
- * identical in both forms.  Stage times: the row copy as sssp_ms, the product as
- * minplus_ms, own columns as decode_ms.  Asynchronous on `stream`. */
+ * (min_plus_path_advanced, fap.cpp:437-485) with own columns from FW.  reached < 0: the
+ * dense product over every column (stage times: row copy as sssp_ms, product as
+ * minplus_ms, boundary rows' own columns as decode_ms).  reached >= 0 (the part's
+ * fap_hip_count_reached count): the product over the reached columns only, into a
+ * compact matrix, then one pass writes every row (slot map as sssp_ms, product as
+ * minplus_ms, the row pass as decode_ms).  The block is identical in both forms.
+ * Asynchronous on `stream`. */
 int fap_hip_merge_prepared(const fap_graph_dev *g, const fap_part_desc *part,
                            const int32_t *d_fw, const int32_t *d_rows, int64_t ld_rows,
-                           int32_t sparse, int32_t *d_out, int64_t ld_out, void *stream);
+                           int64_t reached, int32_t *d_out, int64_t ld_out, void *stream);
 int fap_hip_last_stage_times(fap_stage_times *t);
 
 /* Work-unit shape of the streaming min-plus merge of fap_hip_solve_parts (the

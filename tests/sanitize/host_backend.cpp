@@ -232,9 +232,15 @@ int fap_hip_prepare_parts(const fap_graph_dev *, const fap_part_desc *parts, int
 }
 
 int fap_hip_merge_prepared(const fap_graph_dev *g, const fap_part_desc *part, const int32_t *,
-                           const int32_t *, int64_t, int32_t, int32_t *d_out, int64_t ld_out,
+                           const int32_t *, int64_t, int64_t, int32_t *d_out, int64_t ld_out,
                            void *) {
   return host_solve(g, part, 1, d_out, false, nullptr, ld_out);
+}
+
+int fap_hip_count_reached(const fap_graph_dev *, const fap_part_desc *, int32_t nparts,
+                          const int32_t *, int64_t, int32_t *counts, void *) {
+  for (int p = 0; p < nparts; ++p) counts[p] = 0;
+  return FAP_OK;
 }
 
 int fap_hip_solve_parts_f32(const fap_graph_dev *g, const fap_part_desc *parts, int32_t nparts,
