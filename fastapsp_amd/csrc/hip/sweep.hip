@@ -183,8 +183,8 @@ __global__ __launch_bounds__(256) void own_cols_kernel(const int *__restrict__ f
     if (c0 + r < num && i0 + tx < num) Q[(int64_t)(c0 + r) * ldq + i0 + tx] = t[tx][r];
 }
 
-constexpr int kGrRows = 8;    // rows of the block per workgroup of the row pass
-constexpr int kGrChunks = 2;  // 1024-column chunks per workgroup (4 columns per thread each)
+constexpr int kGrRows = 16;   // rows of the block per workgroup of the row pass
+constexpr int kGrChunks = 1;  // 1024-column chunks per workgroup (4 columns per thread each)
 
 // Rows blockIdx.y * kGrRows .. (+ kGrRows) of the block over kGrChunks chunks of 1024
 // columns.  Column v with slot sv: an own column (sv < num) or, in inner rows, a reached
@@ -381,11 +381,13 @@ extern "C" int fap_hip_merge_prepared(const fap_graph_dev *g, const fap_part_des
   const int64_t ldq = (q.num + kGrRows - 1) / kGrRows * kGrRows;
   FAP_TRY(g_ws_sweep.ensure(o_q + (sparse ? 4LL * (q.num + nq) * ldq : 0)));
   char *meta = static_cast<char *>(g_ws_sweep.ptr);
-  char host[512] = {};
-  std::memcpy(host + o_md, &m, sizeof m);
-  std::memcpy(host + o_dd, &dd, sizeof dd);
-  FAP_CHECK_HIP(hipMemcpyAsync(meta, host, sizeof host, hipMemcpyHostToDevice, st));
-  FAP_CHECK_HIP(hipStreamSynchronize(st));  // the descriptors come from the stack
+  if (!sparse) {  // the dense kernels read descriptors; the sparse ones take arguments
+    char host[512] = {};
+    std::memcpy(host + o_md, &m, sizeof m);
+    std::memcpy(host + o_dd, &dd, sizeof dd);
+    FAP_CHECK_HIP(hipMemcpyAsync(meta, host, sizeof host, hipMemcpyHostToDevice, st));
+    FAP_CHECK_HIP(hipStreamSynchronize(st));  // the descriptors come from the stack
+  }
   const MergeDesc *d_md = reinterpret_cast<const MergeDesc *>(meta + o_md);
   const DecodeDesc *d_dd = reinterpret_cast<const DecodeDesc *>(meta + o_dd);
   int *d_cnt = reinterpret_cast<int *>(meta + o_cnt);
