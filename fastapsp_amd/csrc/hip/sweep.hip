@@ -288,7 +288,6 @@ extern "C" int fap_hip_count_reached(const fap_graph_dev *g, const fap_part_desc
   }
   FAP_REQUIRE(g && g->n > 0 && g->d_graph_id && nparts >= 0 && (nparts == 0 || (parts && counts)),
               "fap_hip_count_reached: null argument");
-  FAP_REQUIRE(nparts <= 65535, "fap_hip_count_reached: at most 65535 parts");
   if (nparts == 0) return FAP_OK;
   int64_t rows = 0;
   std::vector<int4> tab(nparts);
@@ -308,10 +307,13 @@ extern "C" int fap_hip_count_reached(const fap_graph_dev *g, const fap_part_desc
   FAP_CHECK_HIP(hipMemcpyAsync(meta, tab.data(), o_cnt, hipMemcpyHostToDevice, st));
   int *d_counts = reinterpret_cast<int *>(meta + o_cnt);
   FAP_TRY(fap_hip_fill_i32(d_counts, 0, nparts, st));
-  hipLaunchKernelGGL(count_reached_kernel, dim3((g->n + 255) / 256, nparts), dim3(256), 0, st,
-                     d_rows, ld_rows, g->n, g->d_graph_id, reinterpret_cast<const int4 *>(meta),
-                     d_counts);
-  FAP_CHECK_LAUNCH();
+  for (int p0 = 0; p0 < nparts; p0 += 65535) {  // parts on the grid's y dimension
+    const int cnt = std::min(65535, nparts - p0);
+    hipLaunchKernelGGL(count_reached_kernel, dim3((g->n + 255) / 256, cnt), dim3(256), 0, st,
+                       d_rows, ld_rows, g->n, g->d_graph_id,
+                       reinterpret_cast<const int4 *>(meta) + p0, d_counts + p0);
+    FAP_CHECK_LAUNCH();
+  }
   FAP_CHECK_HIP(hipMemcpyAsync(counts, d_counts, 4LL * nparts, hipMemcpyDeviceToHost, st));
   FAP_CHECK_HIP(hipStreamSynchronize(st));
   return ws_fence_record(st);
