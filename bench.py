@@ -212,6 +212,50 @@ def knn_subgraph_bench(args):
                               "stages_ms_all_parts": per_sums}}
 
 
+def wroad_subgraph_bench(args):
+    """Weighted road-like graph (~62k vertices, integer weights in [1, 1000] equal both
+    ways, K = 64): solveSubGraph over every part in turn.  Its blocks come from the boundary
+    graph (every call closes every part and the boundary graph); with the sweep (library
+    default, from the third call) one full solve() runs and each block is a copy."""
+    from fastapsp_amd.autotune import road_graph, symmetric_float_weights
+    t0 = time.perf_counter()
+    g0 = road_graph(62000, 77, 64)
+    c = g0.csr()
+    g0.close()
+    w = symmetric_float_weights(c["row_offset"], c["col"], 77)
+    w = (np.floor((w - 0.5) / 9.5 * 1000) + 1).astype(np.float32)
+    g = F.FapGraph.from_csr(c["row_offset"], c["col"], w, False, 64)
+    g.preCondition()
+    n = g.num_vertexs
+    num = g.layout()["num"][1:].astype(np.int64)
+    entries = float(num.sum()) * n
+    log(f"weighted road n={n} m={g.num_edges} K={g.K} built in {time.perf_counter() - t0:.1f}s")
+
+    def sweep(after_calls):
+        g.setSubGraphSweep(0)
+        g.setSubGraphSweep(after_calls)
+        L.check(L.load().fap_hip_device_sync(), "sync")
+        ts = time.perf_counter()
+        for p in range(1, g.K + 1):
+            g.solveSubGraph(p)
+        L.check(L.load().fap_hip_device_sync(), "sync")
+        return time.perf_counter() - ts, g.subGraphSweepState()
+
+    g.setSubGraphSweep(0)
+    g.solveSubGraph(1)  # warm-up (allocations)
+    per_wall, _ = sweep(0)
+    wall, state = sweep(3)
+    log(f"weighted road solveSubGraph x {g.K}: per-part form {per_wall * 1e3:.1f} ms, sweep "
+        f"{wall * 1e3:.1f} ms ({entries / wall:.3e} entries/s, state {state})")
+    g.close()
+    return {"workload": f"road-like n={n}, integer weights U[1,1000] both ways, K=64, "
+                        "solveSubGraph over every part in turn",
+            "n": n, "K": 64, "entries": entries, "wall_ms": wall * 1e3,
+            "entries_per_s": entries / wall,
+            "sweep": {0: "off", 1: "dense", 2: "sparse", 3: "full solve"}[state],
+            "per_part_form": {"wall_ms": per_wall * 1e3, "entries_per_s": entries / per_wall}}
+
+
 def batch_bench(args):
     """Many-graph batch: 512 road-like graphs (64x64-grid construction scaled: 10% of edges
     dropped, 2% diagonals, largest component) with log-uniform sizes in [256, 8192], half
@@ -384,7 +428,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="road1",
-                    help="road1 (default), road0, batch512 (many-graph batch) or serve")
+                    help="road1 (default), road0, batch512 (many-graph batch), serve, knn262k or "
+                         "wroad62k (solveSubGraph sweeps)")
     ap.add_argument("--graphs", type=int, default=512, help="graphs of the batch512 config")
     ap.add_argument("--unit-grid", type=int, default=None,
                     help="batch512: 1 = merge as a plain grid, 0 = persistent queue "
@@ -432,6 +477,9 @@ def main():
         return
     if args.config == "knn262k":
         print(json.dumps(knn_subgraph_bench(args)), flush=True)
+        return
+    if args.config == "wroad62k":
+        print(json.dumps(wroad_subgraph_bench(args)), flush=True)
         return
     t0 = time.perf_counter()
     g, workload = make_graph(args.config)
@@ -636,6 +684,7 @@ def main():
         g.close()
         legs = {}
         for name, fn in [("knn262k_k256_subgraph", lambda: knn_subgraph_bench(args)),
+                         ("wroad62k_k64_subgraph", lambda: wroad_subgraph_bench(args)),
                          ("minplus_8192", lambda: minplus_bench(8192)),
                          ("batch512", lambda: batch_bench(argparse.Namespace(
                              graphs=512, steps=3, warmup=1, skip_plain=True, skip_cpu=True,

@@ -459,6 +459,10 @@ int32_t fapGraph::solve(bool path_needed) {
   return 0;
 }
 
+// Largest device footprint of a solveSubGraph sweep (boundary rows + matrices, or the
+// n x n result of the full-solve form).
+static constexpr int64_t kSweepMaxBytes = 32LL << 30;
+
 int32_t fapGraph::solveSubGraph(int32_t id, bool path_needed) {
   if (!preconditioned) throw FapError(FAP_ERR_STATE, "solveSubGraph() before preCondition()");
   if (path_needed) require_int("predecessor matrices");
@@ -499,6 +503,33 @@ int32_t fapGraph::solveSubGraph(int32_t id, bool path_needed) {
     sub_rows = C_BlockVer_num[id];
     return 0;
   }
+  // Weighted graphs whose blocks come from the boundary graph close every part and the
+  // boundary graph again on each call.  In a sweep (from the sweep_after-th call on),
+  // when the n x n result fits in kSweepMaxBytes, one full solve() runs instead and each
+  // block is a copy of the part's rows of it (solve() writes part p at row offset_p).
+  if (!path_needed && subgraph_by_boundary_graph() && sweep_after > 0 && !sweep_failed &&
+      (sweep_full || ++sweep_calls >= sweep_after)) {
+    if (!sweep_full) {
+      sweep_failed = (int64_t)num_vertexs * ld_dist * 4 > kSweepMaxBytes;
+      if (!sweep_failed) {
+        try {
+          solve(false);
+          sweep_full = true;
+        } catch (const FapError &) {  // e.g. the HBM budget refuses the result
+          sweep_failed = true;
+        }
+      }
+    }
+    if (sweep_full) {
+      check(fap_hip_memcpy_d2d(d_sub_dist, d_dist + (int64_t)C_BlockVer_offset[id] * ld_dist,
+                               (int64_t)C_BlockVer_num[id] * ld_dist * 4, nullptr),
+            "solveSubGraph");
+      check(fap_hip_stream_sync(nullptr), "solveSubGraph");
+      current_subgraph_id = id;
+      sub_rows = C_BlockVer_num[id];
+      return 0;
+    }
+  }
   if (subgraph_by_boundary_graph()) {
     // Weighted graph whose boundary sources reach far: every part's in-part distances
     // and the closed boundary graph, then this part's rows (a one-part batch item).
@@ -527,8 +558,6 @@ int32_t fapGraph::solveSubGraph(int32_t id, bool path_needed) {
   return 0;
 }
 
-// Largest device footprint of a solveSubGraph sweep (boundary rows + matrices).
-static constexpr int64_t kSweepMaxBytes = 32LL << 30;
 
 bool fapGraph::prepare_sweep(const fap_graph_dev &g) {
   std::vector<fap_part_desc> all = partList();
@@ -581,7 +610,7 @@ void fapGraph::free_sweep() {
   sweep_fw_off.clear();
   sweep_row0.clear();
   sweep_reached.clear();
-  sweep_ready = sweep_failed = false;
+  sweep_ready = sweep_failed = sweep_full = false;
   sweep_calls = 0;
 }
 

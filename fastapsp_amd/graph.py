@@ -119,7 +119,8 @@ class FapGraph:
                 "setSubGraphSweep")
 
     def subGraphSweepState(self) -> int:
-        """0 = not prepared, 1 = prepared (dense product), 2 = prepared (sparse)."""
+        """0 = not prepared, 1 = prepared (dense product), 2 = prepared (sparse), 3 = blocks
+        from a full solve() (weighted graphs whose blocks come from the boundary graph)."""
         v = C.c_int32()
         L.check(L.load().fap_graph_set_subgraph_sweep(self._h, -1, C.byref(v)),
                 "subGraphSweepState")

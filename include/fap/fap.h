@@ -131,6 +131,9 @@ class fapGraph {
   int32_t sweep_after = 3;
   int32_t sweep_calls = 0;
   bool sweep_ready = false, sweep_failed = false, sweep_sparse = false;
+  // Full-solve form of the sweep (weighted graphs whose blocks come from the boundary
+  // graph): the blocks are rows of the resident solve() result.
+  bool sweep_full = false;
   int32_t *d_sweep_fw = nullptr, *d_sweep_rows = nullptr;
   std::vector<int64_t> sweep_fw_off, sweep_row0, sweep_reached;
   bool prepare_sweep(const fap_graph_dev &g);
@@ -190,10 +193,17 @@ class fapGraph {
   // call on (default 3; 0 = never), one batched pass prepares every part's boundary rows
   // and Floyd-Warshall matrix, held in device memory until the layout changes (about
   // 4 x (sum of boundary counts x n + sum of padded s_p^2) bytes, at most 32 GiB), and
-  // each call then only writes its block.  Results are identical either way.
+  // each call then only writes its block.  Weighted graphs whose blocks come from the
+  // boundary graph (subGraphByBoundaryGraph()) instead run one full solve() when its
+  // n x n result fits in 32 GiB, and each block is a copy of its rows.  Results are
+  // identical either way.
   void setSubGraphSweep(int32_t after_calls);
-  // 0 = not prepared, 1 = prepared (dense product), 2 = prepared (sparse product).
-  int32_t subGraphSweepState() const { return sweep_ready ? (sweep_sparse ? 2 : 1) : 0; }
+  // 0 = not prepared, 1 = prepared (dense product), 2 = prepared (sparse product),
+  // 3 = blocks taken from a full solve() (weighted graphs whose blocks come from the
+  // boundary graph, when the n x n result fits in 32 GiB).
+  int32_t subGraphSweepState() const {
+    return sweep_full ? 3 : sweep_ready ? (sweep_sparse ? 2 : 1) : 0;
+  }
 
   std::vector<int32_t> getMapping();
   std::vector<float> getAllDistance();

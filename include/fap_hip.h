@@ -527,7 +527,7 @@ int fap_graph_solve(fap_graph *g);
 int fap_graph_solve_subgraph(fap_graph *g, int32_t id);
 /* fapGraph::setSubGraphSweep (after_calls >= 0; a negative value changes nothing) and
  * *ready = 0 before the sweep's batched pass has run, 1 after it (dense product), 2 after
- * it (sparse product); ready may be NULL.  No reference
+ * it (sparse product), 3 when the blocks come from a full solve(); ready may be NULL.  No reference
  * counterpart: the reference recomputes every part's SSSP and Floyd-Warshall on each
  * solveSubGraph call (fap/fap.cpp:405-428). */
 int fap_graph_set_subgraph_sweep(fap_graph *g, int32_t after_calls, int32_t *ready);

@@ -164,6 +164,11 @@ int fap_hip_memcpy_d2h(void *dst, const void *src, int64_t bytes, void *) {
   return FAP_OK;
 }
 
+int fap_hip_memcpy_d2d(void *dst, const void *src, int64_t bytes, void *) {
+  if (bytes > 0) std::memcpy(dst, src, bytes);
+  return FAP_OK;
+}
+
 int fap_hip_memcpy2d_d2h_i32(int32_t *dst, int64_t h_ld, const int32_t *src, int64_t d_ld,
                              int64_t rows, int64_t cols, void *) {
   if (rows < 0 || cols < 0 || h_ld < cols || d_ld < cols) return FAP_ERR_INVALID;

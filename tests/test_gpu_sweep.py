@@ -11,6 +11,7 @@ import pytest
 
 import fastapsp_amd as F
 import oracle as O
+from fastapsp_amd.autotune import road_graph, symmetric_float_weights
 
 pytestmark = pytest.mark.gpu
 
@@ -82,3 +83,36 @@ def test_sweep_threshold_and_layout_change():
     off, num = int(lay["offset"][2]), int(lay["num"][2])
     assert np.array_equal(g.getSubGraphDistanceI32(2),
                           O.solve_parts(2, 2, csr, lay, row_base=off, rows=num))
+
+
+@pytest.mark.parametrize("integer", [True, False])
+def test_sweep_weighted_road_full_solve_form(integer):
+    """Weighted road graphs take their blocks from the boundary graph (every call closes
+    every part and the boundary graph); in a sweep one full solve() runs and each block is
+    a copy of its rows.  Every block against the per-call form, bit for bit."""
+    say(f"sweep: weighted road, {'int' if integer else 'float'} weights, full-solve form")
+    g0 = road_graph(9000, 61, 12)
+    c = g0.csr()
+    w = symmetric_float_weights(c["row_offset"], c["col"], 61)
+    if integer:
+        w = (np.floor((w - 0.5) / 9.5 * 1000) + 1).astype(np.float32)
+    g = F.FapGraph.from_csr(c["row_offset"], c["col"], w, False, 12)
+    g.preCondition()
+    lay = g.layout()
+
+    def block(p):
+        num = int(lay["num"][p])
+        return (g.getSubGraphDistanceI32(p) if integer
+                else g.getSubGraphDistanceRowsF32(p, 0, num))
+
+    g.setSubGraphSweep(0)
+    per_call = {}
+    for p in range(1, g.K + 1):
+        g.solveSubGraph(p)
+        per_call[p] = block(p)
+    assert g.subGraphSweepState() == 0
+    g.setSubGraphSweep(2)
+    for p in [4, 1, 12, 7, 2, 3, 5, 6, 8, 9, 10, 11, 4]:
+        g.solveSubGraph(p)
+        assert np.array_equal(block(p), per_call[p]), f"part {p}"
+    assert g.subGraphSweepState() == 3
